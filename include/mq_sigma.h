@@ -1,0 +1,271 @@
+/*
+ * mq_sigma.h -- C-ABI of the MI355X (gfx950) sigma-vector / boundary-propagation engine.
+ *
+ * The entry points replace the execution layer of QCMaquis' contraction engine
+ * (sekelle/maquis-dmrg).  Every function returns an int status (MQ_OK == 0); on failure
+ * mq_last_error() returns a message.  No entry point ever calls exit().
+ * Reference interfaces replaced (paths relative to dmrg/framework/dmrg/ of the reference):
+ *
+ *   create_contraction_schedule     mp_tensors/contractions/create_schedule/site_hamil_schedule.hpp:40-93
+ *   super_hamil_mv                  solver/super_hamil_mv.hpp:100-155   (ietl::mult, solver/ietl_interface_dv.h:112-118)
+ *   overlap_mpo_left_step           mp_tensors/contractions/boundary_ops/move_boundary.hpp:147-279
+ *   overlap_mpo_right_step          mp_tensors/contractions/boundary_ops/move_boundary.hpp:281-404
+ *   solve<double> (Jacobi-Davidson) solver/solver.cpp:14-34, ietl/jacobi.h:386-474
+ *   DavidsonVector ops              solver/davidson_vector.cpp:197-254, solver/ietl_interface_dv.h:33-69
+ *
+ * Data conventions (identical to the reference):
+ *   - all matrices are column-major doubles;
+ *   - an MPS site tensor is "right-paired": one block per left-index sector lb (left_i order),
+ *     ls(lb) rows, columns = concatenation over physical sectors s (phys_i order) of
+ *     d_s * rs(lc+s) columns (sub-index major), i.e. ProductBasis(phys_i, right_i, fuse(-a,b));
+ *   - "left-paired": one block per right-index sector rb (right_i order), rows = concatenation
+ *     over s of d_s * ls(rc-s) rows, rs(rb) columns, i.e. ProductBasis(phys_i, left_i);
+ *   - a Davidson vector is the concatenation of the right-paired blocks, each padded to a
+ *     multiple of MQ_BUFFER_ALIGNMENT elements (solver/constants.h:6, davidson_vector.cpp:43-55);
+ *   - a boundary is a set of cohorts (bra sector, ket sector); cohort ci stores n_blocks(ci)
+ *     column-major ls x rs blocks back to back; offsets[ci][b] is the element offset of MPO
+ *     bond index b, -1 when absent (mp_tensors/boundary.h:76-234).
+ */
+#ifndef MQ_SIGMA_H
+#define MQ_SIGMA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MQ_BUFFER_ALIGNMENT 128u
+#define MQ_NONE 0xffffffffu
+
+enum mq_status {
+    MQ_OK = 0,
+    MQ_ERR_INVALID = 1,   /* bad argument, shape or descriptor */
+    MQ_ERR_HIP = 2,       /* a HIP runtime call failed */
+    MQ_ERR_NOMEM = 3,     /* device or host allocation failed */
+    MQ_ERR_CAPACITY = 4,  /* a size exceeds a fixed capacity */
+    MQ_ERR_INTERNAL = 5
+};
+
+enum mq_symmetry {
+    MQ_SYM_U1 = 1,     /* charge q[0]                 (block_matrix/symmetry/u1.h)    */
+    MQ_SYM_TWOU1 = 2,  /* charge (q[0], q[1])         (block_matrix/symmetry/2u1.h)   */
+    MQ_SYM_SU2U1 = 3   /* charge (N, 2S)              (block_matrix/symmetry/su2u1.h) */
+};
+
+enum mq_precision {
+    MQ_PREC_FP64 = 0   /* plain path: every stage in FP64 */
+};
+
+typedef struct { int32_t q[2]; } mq_charge;
+
+/* Index<SymmGroup>: (charge, size) pairs, in the caller's order (block_matrix/indexing_stable.hpp:97-258). */
+typedef struct {
+    uint32_t n;
+    const mq_charge* charges;
+    const uint32_t* sizes;
+} mq_index_desc;
+
+/* One site operator with its SparseOperator entries (block_matrix/sparse_operator.h:35-233).
+ * Block k maps physical sector in_charge[k] (W.basis().left_charge) to out_charge[k]
+ * (W.basis().right_charge); entry e: sigma(out, col[e]) += coeff[e] * psi(in, row[e]). */
+typedef struct {
+    int32_t spin;                 /* 2S of the operator (SU2U1), 0 for abelian groups */
+    uint32_t n_blocks;
+    const mq_charge* in_charge;
+    const mq_charge* out_charge;
+    const uint32_t* entry_begin;  /* n_blocks + 1 */
+    const uint32_t* row;
+    const uint32_t* col;
+    const uint8_t* row_spin;      /* SU2 site spins of the entry (0, 1 or 2); NULL => 0 */
+    const uint8_t* col_spin;
+    const double* coeff;
+} mq_op_desc;
+
+/* MPOTensor (mp_tensors/mpotensor.h:45-135): terms (b1, b2, op, scale) sorted by (b1, b2);
+ * several ops per (b1, b2) are allowed (term_descriptor).  Bond properties: 2S per bond index
+ * and the Hermitian-conjugate pairing (mp_tensors/mpotensor_detail.h:141-187). */
+typedef struct {
+    uint32_t row_dim, col_dim;
+    const int32_t* left_spin;     /* row_dim entries, may be NULL (all 0) */
+    const int32_t* right_spin;    /* col_dim entries, may be NULL */
+    const uint32_t* left_herm;    /* row_dim, conjugate bond index or MQ_NONE; may be NULL */
+    const int32_t* left_phase;
+    const uint32_t* right_herm;   /* col_dim */
+    const int32_t* right_phase;
+    uint32_t n_ops;
+    const mq_op_desc* ops;
+    uint32_t n_terms;
+    const uint32_t* term_b1;
+    const uint32_t* term_b2;
+    const uint32_t* term_op;
+    const double* term_scale;
+} mq_mpo_desc;
+
+/* Site tensor indices (MPSTensor row_dim / site_dim / col_dim). */
+typedef struct {
+    int32_t sym;
+    mq_index_desc left_i, phys_i, right_i;
+} mq_site_desc;
+
+/* BoundaryIndex<double, SymmGroup> (mp_tensors/boundary.h:76-234 and boundary_index_rt.hpp:37-86). */
+typedef struct {
+    mq_index_desc bra, ket;
+    uint32_t n_cohorts;
+    uint32_t aux_dim;
+    const uint32_t* bra_pos;        /* n_cohorts: lbrb_ci row  */
+    const uint32_t* ket_pos;        /* n_cohorts: lbrb_ci column */
+    const uint8_t* tr;              /* n_cohorts */
+    const uint32_t* n_blocks;       /* n_cohorts (BoundaryIndexRT::n_blocks) */
+    const int64_t* offsets;         /* n_cohorts * aux_dim, -1 == absent */
+    const double* conj_scales;      /* n_cohorts * aux_dim */
+    const uint8_t* transposes;      /* n_cohorts * aux_dim, may be NULL */
+} mq_bindex_desc;
+
+/* Flattened ScheduleNew<double> (solver/tasks.h:50-348): the reference's own plan format.
+ * MPSBlock m owns T-tuples [t_begin[m], t_begin[m+1]) and cohorts [coh_begin[m], coh_begin[m+1]).
+ * Within a cohort, units are (phys-out sector, sub-index) pairs in sfold order; a unit owns
+ * lines (b1 row id, number of tasks) and tasks (ti, col, alpha) in line order. */
+typedef struct {
+    uint32_t n_blocks;
+    const uint64_t* t_begin;
+    const uint32_t* t_mps_offset;
+    const uint32_t* t_ci;
+    const uint32_t* t_ci_eff;
+    const uint32_t* t_lb_ket;
+    const uint64_t* coh_begin;
+    const uint32_t* coh_lb;
+    const uint32_t* coh_rb;
+    const uint32_t* coh_ls;
+    const uint32_t* coh_rs;
+    const uint32_t* coh_ci;
+    const uint32_t* coh_ci_eff;
+    const uint32_t* coh_nSrows;
+    const uint32_t* coh_stripe;
+    const uint64_t* coh_unit_begin;
+    const uint32_t* unit_offset;
+    const uint32_t* unit_ms;
+    const uint64_t* unit_line_begin;
+    const uint32_t* line_b1;
+    const uint32_t* line_count;
+    const uint64_t* unit_task_begin;
+    const uint32_t* task_ti;
+    const uint32_t* task_col;
+    const double* task_alpha;
+    /* cohort -> new-boundary block offsets (boundary steps only, else NULL):
+       coh_mpo_offset_begin[c] .. +aux_dim, -1 == absent (Cohort::get_offsets) */
+    uint32_t aux_dim;
+    const int64_t* coh_mpo_offsets;
+} mq_schedule_desc;
+
+typedef struct mq_ctx mq_ctx;
+typedef struct mq_boundary mq_boundary;
+typedef struct mq_plan mq_plan;
+typedef struct mq_schedule mq_schedule;
+
+/* ---------------------------------------------------------------- context */
+const char* mq_last_error(void);
+int mq_ctx_create(int device, mq_ctx** out);
+int mq_ctx_destroy(mq_ctx* ctx);
+int mq_ctx_synchronize(mq_ctx* ctx);
+/* Stream the context launches on (a hipStream_t); lets callers time with events on it. */
+void* mq_ctx_stream(mq_ctx* ctx);
+
+/* ---------------------------------------------------------------- layouts */
+/* Number of elements and per-block offsets of the padded Davidson-vector layout of the
+ * right-paired tensor (n_blocks == left_i.n).  block_offsets may be NULL. */
+int mq_site_vector_layout(const mq_site_desc* site, uint64_t* n_elements, uint64_t* block_offsets);
+/* Same for the left-paired layout (one block per right_i sector). */
+int mq_site_left_layout(const mq_site_desc* site, uint64_t* n_elements, uint64_t* block_offsets);
+/* Reshape between the right-paired and left-paired layouts on the device
+ * (mp_tensors/reshapes.h:197-242 and its inverse).  src/dst are device pointers. */
+int mq_reshape_right_to_left(mq_ctx* ctx, const mq_site_desc* site, const double* src, double* dst);
+int mq_reshape_left_to_right(mq_ctx* ctx, const mq_site_desc* site, const double* src, double* dst);
+
+/* ---------------------------------------------------------------- boundaries */
+/* Boundary(ud, ld, ad) constructor (mp_tensors/boundary.h:258-283): one cohort per sector of
+ * idx, aux_dim blocks each, every element 1.0. */
+int mq_boundary_create_trivial(mq_ctx* ctx, const mq_index_desc* idx, uint32_t aux_dim, mq_boundary** out);
+/* Upload a boundary: host_cohorts[ci] points at n_blocks(ci)*ls*rs doubles (NULL for tr cohorts). */
+int mq_boundary_create(mq_ctx* ctx, const mq_bindex_desc* idx, const double* const* host_cohorts,
+                       mq_boundary** out);
+int mq_boundary_destroy(mq_boundary* b);
+/* Export the index.  Arrays returned stay valid until the boundary is destroyed. */
+int mq_boundary_index(const mq_boundary* b, mq_bindex_desc* out);
+/* Copy cohort ci to the host (n_blocks*ls*rs doubles). */
+int mq_boundary_download(const mq_boundary* b, uint32_t ci, double* host);
+/* Device pointer of cohort ci. */
+int mq_boundary_cohort_ptr(const mq_boundary* b, uint32_t ci, const double** dev);
+/* Boundary::traces() (mp_tensors/boundary.h:339-350): per bond index b, the sum of all elements
+ * of every block b; out has aux_dim entries. */
+int mq_boundary_traces(const mq_boundary* b, double* out);
+
+/* ---------------------------------------------------------------- schedules */
+/* Schedule builders run on the host only (no device needed); they take the boundary indices.
+ * create_contraction_schedule (site_hamil_schedule.hpp:40-93): right_tasks + site_hamil_tasks. */
+int mq_schedule_build_sigma(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* left,
+                            const mq_bindex_desc* right, mq_schedule** out);
+/* Schedule of overlap_mpo_left_step (move_boundary.hpp:147-213; left_tasks.hpp:38-172). */
+int mq_schedule_build_left(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* left,
+                           int symmetric, mq_schedule** out);
+/* Schedule of overlap_mpo_right_step (move_boundary.hpp:281-337; right_tasks.hpp:39-167). */
+int mq_schedule_build_right(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* right,
+                            int symmetric, mq_schedule** out);
+/* Index of the boundary a left/right-step schedule produces (BoundaryIndex after add_cohort and
+ * complement_transpose, move_boundary.hpp:202-209, 331-337). */
+int mq_schedule_out_index(const mq_schedule* s, mq_bindex_desc* out);
+int mq_schedule_export(const mq_schedule* s, mq_schedule_desc* out);
+/* Flop count of the schedule with the reference's model (tasks.cpp:388-398, 777-796). */
+int mq_schedule_flops(const mq_schedule* s, uint64_t* flops);
+int mq_schedule_destroy(mq_schedule* s);
+
+/* ---------------------------------------------------------------- sigma */
+/* super_hamil_mv replacement: plan for H_eff * psi at one site.  The plan keeps pointers to the
+ * device data of left/right; they must outlive the plan. */
+int mq_sigma_plan_create(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
+                         const mq_boundary* left, const mq_boundary* right, mq_plan** out);
+/* Plan from a caller-built ScheduleNew (the drop-in seam: the caller keeps its own
+ * create_contraction_schedule).  ket_rows[lb] = ls of psi block lb; ket_cols[lb] = its columns. */
+int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sched,
+                                       uint32_t n_ket_blocks, const uint32_t* ket_rows,
+                                       const uint32_t* ket_cols, const mq_boundary* left,
+                                       const mq_boundary* right, mq_plan** out);
+/* sigma = H_eff * psi; psi and sigma are device pointers in the padded Davidson-vector layout. */
+int mq_sigma_apply(mq_plan* plan, const double* psi, double* sigma);
+int mq_plan_vector_size(const mq_plan* plan, uint64_t* n_elements);
+int mq_plan_flops(const mq_plan* plan, uint64_t* flops);
+/* Bytes of device workspace held by the plan (T and S buffers and tables). */
+int mq_plan_workspace_bytes(const mq_plan* plan, uint64_t* bytes);
+int mq_plan_destroy(mq_plan* plan);
+
+/* ---------------------------------------------------------------- boundary steps */
+/* overlap_mpo_left_step: bra_lp is the bra in the left-paired layout, ket_rp the ket in the
+ * right-paired layout (both device pointers of the same site tensor shape). */
+int mq_boundary_left_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
+                          const double* bra_lp, const double* ket_rp, const mq_boundary* left,
+                          int symmetric, mq_boundary** out);
+/* overlap_mpo_right_step: bra_rp and ket_rp both right-paired. */
+int mq_boundary_right_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
+                           const double* bra_rp, const double* ket_rp, const mq_boundary* right,
+                           int symmetric, mq_boundary** out);
+
+/* ---------------------------------------------------------------- Davidson */
+/* Device-resident vector kernels (DavidsonVector ops, solver/davidson_vector.cpp:197-254). */
+int mq_vec_dot(mq_ctx* ctx, uint64_t n, const double* x, const double* y, double* result);
+int mq_vec_nrm2(mq_ctx* ctx, uint64_t n, const double* x, double* result);
+int mq_vec_axpy(mq_ctx* ctx, uint64_t n, double a, const double* x, double* y);
+int mq_vec_scal(mq_ctx* ctx, uint64_t n, double a, double* x);
+int mq_vec_copy(mq_ctx* ctx, uint64_t n, const double* x, double* y);
+
+/* Jacobi-Davidson smallest eigenpair of H_eff (ietl/jacobi.h:386-516, solver/ietl_jacobi_davidson.h:35-76).
+ * init and out are device vectors in the plan's layout (may alias).  gmres_iter == 0 uses the
+ * plain correction t = -r + (r.u)/(u.u) u (ietl/jacobi.h:150-156). */
+int mq_jd_solve(mq_plan* plan, const double* init, double* out, double tol, int max_iter,
+                int gmres_iter, double* energy, int* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MQ_SIGMA_H */

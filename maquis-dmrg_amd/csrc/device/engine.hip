@@ -1,0 +1,1145 @@
+// C-ABI implementation: contexts, device-resident boundaries, sigma plans, boundary steps,
+// reshapes and the device-resident Jacobi-Davidson driver.
+//
+// A schedule (the reference's ScheduleNew tables) is lowered into three device task tables:
+//   T stage  : grouped GEMM, T_ti = psi_slab * R_cohort (or L_b * ket slab for the left step)
+//   S stage  : gather/scale regions, S = sum alpha * T slab
+//   final    : grouped K-segmented GEMM, sigma[lb_out] = sum_cohorts L_cohort * S (owner-computes,
+//              no atomics), or the prop_l / prop_r products of the boundary steps.
+// All operand addresses are checked against their buffers on the host before any launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../host/host_types.hpp"
+#include "kernels.hpp"
+
+using namespace mq;
+
+namespace {
+thread_local std::string g_err;
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            ::mq::fail(MQ_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return MQ_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return MQ_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return MQ_ERR_INTERNAL;
+    }
+}
+
+// Device allocation owned by RAII.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            fail(MQ_ERR_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
+        }
+        bytes = n;
+    }
+    void ensure(size_t n) {
+        if (n > bytes) alloc(n);
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    b.alloc(v.size() * sizeof(T));
+    if (!v.empty()) {
+        HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+}
+
+inline uint64_t align32(uint64_t x) { return round_up(x, 32); }
+
+}  // namespace
+
+// ============================================================================ context
+struct mq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf red_partial, red_result;
+    DevBuf step_ws;   // grows; reused by boundary steps
+};
+
+struct mq_boundary {
+    mq_ctx* ctx = nullptr;
+    BoundaryIndex index;
+    std::vector<uint64_t> cohort_off;
+    uint64_t total = 0;
+    DevBuf data;
+    // storage for exported descriptors
+    mutable std::vector<mq_charge> bra_c, ket_c;
+    double* ptr() const { return data.as<double>(); }
+    void layout() {
+        cohort_off.resize(index.n_cohorts());
+        total = 0;
+        for (uint32_t ci = 0; ci < index.n_cohorts(); ++ci) {
+            cohort_off[ci] = total;
+            total += align32(index.cohort_size(ci));
+        }
+        if (total == 0) total = 32;
+    }
+};
+
+struct mq_schedule {
+    Schedule s;
+    uint64_t flops = 0;
+    mutable std::vector<mq_charge> bra_c, ket_c;
+};
+
+// Lowered stage tables (device) + the host copies used for validation.
+struct StagePlan {
+    std::vector<GemmTask> t_tasks, f_tasks;
+    std::vector<GemmSeg> t_segs, f_segs;
+    std::vector<GatherRegion> regions;
+    std::vector<uint64_t> src_off;
+    std::vector<double> alpha;
+    uint32_t t_tiles = 0, f_tiles = 0, g_chunks = 0;
+    uint64_t ws_elems = 0;
+    uint64_t s_begin = 0, s_elems = 0;   // S range in the workspace (zero-filled each apply)
+    DevBuf d_t_tasks, d_t_segs, d_f_tasks, d_f_segs, d_regions, d_src_off, d_alpha;
+    uint64_t sizes[N_BASES] = {0, 0, 0, 0, 0, 0};   // element counts of each base for validation
+
+    void finish_tables() {
+        uint32_t tile = 0;
+        for (auto& t : t_tasks) {
+            t.tiles_n = (t.N + GEMM_BN - 1) / GEMM_BN;
+            t.tile_begin = tile;
+            tile += ((t.M + GEMM_BM - 1) / GEMM_BM) * t.tiles_n;
+        }
+        t_tiles = tile;
+        tile = 0;
+        for (auto& t : f_tasks) {
+            t.tiles_n = (t.N + GEMM_BN - 1) / GEMM_BN;
+            t.tile_begin = tile;
+            tile += ((t.M + GEMM_BM - 1) / GEMM_BM) * t.tiles_n;
+        }
+        f_tiles = tile;
+        uint32_t ch = 0;
+        for (auto& r : regions) {
+            r.chunk_begin = ch;
+            uint64_t n = uint64_t(r.R) * r.C;
+            uint64_t c = (n + GATHER_CHUNK - 1) / GATHER_CHUNK;
+            if (c == 0) c = 1;
+            MQ_REQUIRE(uint64_t(ch) + c < (1ull << 31), "too many gather chunks");
+            ch += uint32_t(c);
+        }
+        g_chunks = ch;
+    }
+
+    // Every element touched by a GEMM segment / gather region must lie inside its buffer.
+    static uint64_t extent(uint32_t rows_contig, uint32_t ncols, uint32_t ld) {
+        if (rows_contig == 0 || ncols == 0) return 0;
+        return uint64_t(ncols - 1) * ld + rows_contig;
+    }
+    void check_in(uint8_t base, uint64_t off, uint64_t ext, const char* what) const {
+        MQ_REQUIRE(base < N_BASES, "bad base");
+        if (ext == 0) return;
+        if (off + ext > sizes[base])
+            fail(MQ_ERR_INTERNAL, std::string("operand out of bounds: ") + what + " base " + std::to_string(base) +
+                                      " off " + std::to_string(off) + " ext " + std::to_string(ext) + " size " +
+                                      std::to_string(sizes[base]));
+    }
+    void validate() const {
+        auto gemm = [&](const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs) {
+            for (const auto& t : tasks) {
+                MQ_REQUIRE(t.ldc >= t.M || t.N <= 1, "ldc < M");
+                check_in(t.c_base, t.c_off, extent(t.M, t.N, t.ldc), "C");
+                MQ_REQUIRE(t.seg0 <= t.seg1 && t.seg1 <= segs.size(), "segment range");
+                for (uint32_t s = t.seg0; s < t.seg1; ++s) {
+                    const GemmSeg& g = segs[s];
+                    if (!g.trans_a) check_in(g.a_base, g.a_off, extent(t.M, g.K, g.lda), "A");
+                    else check_in(g.a_base, g.a_off, extent(g.K, t.M, g.lda), "A^T");
+                    if (!g.trans_b) check_in(g.b_base, g.b_off, extent(g.K, t.N, g.ldb), "B");
+                    else check_in(g.b_base, g.b_off, extent(t.N, g.K, g.ldb), "B^T");
+                }
+            }
+        };
+        gemm(t_tasks, t_segs);
+        gemm(f_tasks, f_segs);
+        for (const auto& r : regions) {
+            check_in(BASE_WS, r.dst_off, extent(r.R, r.C, r.ld_dst), "S region");
+            MQ_REQUIRE(r.t0 <= r.t1 && r.t1 <= src_off.size(), "region task range");
+            for (uint32_t t = r.t0; t < r.t1; ++t) check_in(BASE_WS, src_off[t], uint64_t(r.R) * r.C, "T slab");
+        }
+    }
+    void upload_all(hipStream_t st) {
+        upload(d_t_tasks, t_tasks, st);
+        upload(d_t_segs, t_segs, st);
+        upload(d_f_tasks, f_tasks, st);
+        upload(d_f_segs, f_segs, st);
+        upload(d_regions, regions, st);
+        upload(d_src_off, src_off, st);
+        upload(d_alpha, alpha, st);
+    }
+    void run(hipStream_t st, Bases b) const {
+        if (s_elems) HIPCHK(launch_fill(st, b.p[BASE_WS] + s_begin, s_elems, 0.0));
+        HIPCHK(launch_gemm(st, d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()), d_t_segs.as<GemmSeg>(), t_tiles, b));
+        HIPCHK(launch_gather(st, d_regions.as<GatherRegion>(), uint32_t(regions.size()), d_src_off.as<uint64_t>(),
+                             d_alpha.as<double>(), g_chunks, b.p[BASE_WS], b.p[BASE_WS]));
+        HIPCHK(launch_gemm(st, d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()), d_f_segs.as<GemmSeg>(), f_tiles, b));
+    }
+};
+
+struct mq_plan {
+    mq_ctx* ctx = nullptr;
+    StagePlan st;
+    DevBuf ws;
+    const mq_boundary* left = nullptr;
+    const mq_boundary* right = nullptr;
+    uint64_t vec_size = 0;
+    uint64_t flops = 0;
+    std::vector<uint64_t> out_offsets;
+    // Jacobi-Davidson vectors (allocated on first use)
+    DevBuf jd_buf;
+    int jd_cap = 0;
+};
+
+// ============================================================================ lowering
+namespace {
+
+void add_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, uint8_t c_base, uint64_t c_off, uint32_t M,
+              uint32_t N, uint32_t ldc, const std::vector<GemmSeg>& s, uint8_t beta = 0) {
+    if (M == 0 || N == 0) return;
+    GemmTask t{};
+    t.c_off = c_off;
+    t.M = M;
+    t.N = N;
+    t.ldc = ldc;
+    t.seg0 = uint32_t(segs.size());
+    segs.insert(segs.end(), s.begin(), s.end());
+    t.seg1 = uint32_t(segs.size());
+    t.c_base = c_base;
+    t.beta = beta;
+    tasks.push_back(t);
+}
+
+GemmSeg seg(uint8_t ab, uint64_t ao, uint32_t lda, bool ta, uint8_t bb, uint64_t bo, uint32_t ldb, bool tb, uint32_t K) {
+    GemmSeg g{};
+    g.a_off = ao;
+    g.b_off = bo;
+    g.K = K;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.a_base = ab;
+    g.b_base = bb;
+    g.trans_a = ta;
+    g.trans_b = tb;
+    return g;
+}
+
+// T stage of sigma and of the right step: T_ti = ket_slab (ls x rs_ket) * [R_b] (rs_ket x nb*rs_bra)
+// (tasks.cpp:650-752).  Returns the workspace offset of every T.
+std::vector<uint64_t> lower_T_right(StagePlan& P, const Schedule& S, const BoundaryIndex& R,
+                                    const std::vector<uint64_t>& r_off, const std::vector<uint32_t>& ket_rows,
+                                    const std::vector<uint64_t>& ket_off, uint64_t& ws) {
+    std::vector<uint64_t> t_off(S.t_ci.size());
+    for (size_t ti = 0; ti < S.t_ci.size(); ++ti) {
+        uint32_t ci = S.t_ci[ti], cie = S.t_ci_eff[ti], lbk = S.t_lb_ket[ti];
+        MQ_REQUIRE(ci < R.n_cohorts() && cie < R.n_cohorts() && lbk < ket_rows.size(), "T tuple out of range");
+        uint32_t M = ket_rows[lbk];
+        uint32_t K = uint32_t(R.left_size(ci)), brs = uint32_t(R.right_size(ci));
+        uint32_t nb = R.n_blocks[cie];
+        MQ_REQUIRE(R.left_size(cie) * R.right_size(cie) == uint64_t(K) * brs, "cohort shape mismatch");
+        t_off[ti] = ws;
+        ws += align32(uint64_t(M) * nb * brs);
+        uint64_t a_off = ket_off[lbk] + uint64_t(S.t_mps_offset[ti]) * M;
+        if (ci == cie) {
+            add_gemm(P.t_tasks, P.t_segs, BASE_WS, t_off[ti], M, nb * brs, M,
+                     {seg(BASE_IN, a_off, M, false, BASE_RIGHT, r_off[cie], K, false, K)});
+        } else {
+            for (uint32_t b = 0; b < nb; ++b)
+                add_gemm(P.t_tasks, P.t_segs, BASE_WS, t_off[ti] + uint64_t(b) * M * brs, M, brs, M,
+                         {seg(BASE_IN, a_off, M, false, BASE_RIGHT, r_off[cie] + uint64_t(b) * K * brs, brs, true, K)});
+        }
+    }
+    return t_off;
+}
+
+// S stage of sigma / right step (create_s_r, tasks.cpp:466-494): S is (nSrows*ls) x stripe.
+std::vector<uint64_t> lower_S_right(StagePlan& P, const Schedule& S, const std::vector<uint64_t>& t_off, uint64_t& ws) {
+    std::vector<uint64_t> s_off(S.n_cohorts());
+    P.s_begin = ws;
+    for (uint32_t m = 0; m < S.n_blocks; ++m)
+        for (uint64_t c = S.coh_begin[m]; c < S.coh_begin[m + 1]; ++c) {
+            uint32_t ls = S.coh_ls[c], nS = S.coh_nSrows[c], stripe = S.coh_stripe[c];
+            uint64_t lda = uint64_t(ls) * nS;
+            MQ_REQUIRE(lda < (1ull << 32), "S leading dimension overflow");
+            s_off[c] = ws;
+            ws += align32(lda * stripe);
+            for (uint64_t u = S.coh_unit_begin[c]; u < S.coh_unit_begin[c + 1]; ++u) {
+                uint32_t ms = S.unit_ms[u], off = S.unit_offset[u];
+                uint64_t task = S.unit_task_begin[u];
+                for (uint64_t l = S.unit_line_begin[u]; l < S.unit_line_begin[u + 1]; ++l) {
+                    uint32_t b = S.line_b1[l], cnt = S.line_count[l];
+                    MQ_REQUIRE(b < nS && uint64_t(off) + ms <= stripe, "S region outside S");
+                    GatherRegion r{};
+                    r.dst_off = s_off[c] + uint64_t(off) * lda + uint64_t(b) * ls;
+                    r.R = ls;
+                    r.C = ms;
+                    r.ld_dst = uint32_t(lda);
+                    r.t0 = uint32_t(P.src_off.size());
+                    for (uint32_t k = 0; k < cnt; ++k, ++task) {
+                        uint64_t ti = S.t_begin[m] + S.task_ti[task];
+                        MQ_REQUIRE(ti < S.t_begin[m + 1], "task ti out of range");
+                        P.src_off.push_back(t_off[ti] + uint64_t(S.task_col[task]) * ls);
+                        P.alpha.push_back(S.task_alpha[task]);
+                    }
+                    r.t1 = uint32_t(P.src_off.size());
+                    if (uint64_t(r.R) * r.C) P.regions.push_back(r);
+                }
+                MQ_REQUIRE(task == S.unit_task_begin[u + 1], "line counts do not match the task list");
+            }
+        }
+    P.s_elems = ws - P.s_begin;
+    return s_off;
+}
+
+}  // namespace
+
+// ============================================================================ C-ABI
+extern "C" {
+
+const char* mq_last_error(void) { return g_err.c_str(); }
+
+int mq_ctx_create(int device, mq_ctx** out) {
+    return guarded([&] {
+        MQ_REQUIRE(out, "null output");
+        auto c = std::make_unique<mq_ctx>();
+        c->device = device;
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->red_partial.alloc(RED_BLOCKS * sizeof(double));
+        c->red_result.alloc(64 * sizeof(double));
+        *out = c.release();
+    });
+}
+
+int mq_ctx_destroy(mq_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int mq_ctx_synchronize(mq_ctx* ctx) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx, "null ctx");
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void* mq_ctx_stream(mq_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// ---------------------------------------------------------------------------- layouts
+int mq_site_vector_layout(const mq_site_desc* site, uint64_t* n, uint64_t* offs) {
+    return guarded([&] {
+        MQ_REQUIRE(site && n, "null argument");
+        SiteLayout L(*site);
+        *n = L.rp_total;
+        if (offs) std::copy(L.rp_offset.begin(), L.rp_offset.end(), offs);
+    });
+}
+
+int mq_site_left_layout(const mq_site_desc* site, uint64_t* n, uint64_t* offs) {
+    return guarded([&] {
+        MQ_REQUIRE(site && n, "null argument");
+        SiteLayout L(*site);
+        *n = L.lp_total;
+        if (offs) std::copy(L.lp_offset.begin(), L.lp_offset.end(), offs);
+    });
+}
+
+}  // extern "C"
+
+// Reshape right-paired <-> left-paired.  Per (left sector lb, physical sector s) slab the
+// right-paired elements are contiguous; the left-paired destination is a strided 3-d copy.
+// The device kernel is generated from the slab list through the gather machinery: a slab is
+// d_s*rs regions of ls contiguous elements with alpha = 1.
+namespace {
+int reshape(mq_ctx* ctx, const mq_site_desc* site, const double* src, double* dst, bool to_left) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && site && src && dst, "null argument");
+        SiteLayout L(*site);
+        ProductBasis rpb(L.phys_i, L.right_i, -1), lpb(L.phys_i, L.left_i, +1);
+        // One "region" per (lb, s, ss): R = ls rows, C = rs columns.
+        // to_left : dst = left-paired (column stride lp_rows), src = right-paired contiguous slab
+        // to_right: dst = right-paired (column stride ls),     src = left-paired strided -> use R=ls,
+        //           C=1 regions per column (contiguous in both layouts).
+        std::vector<GatherRegion> regs;
+        std::vector<uint64_t> so;
+        std::vector<double> al;
+        for (uint32_t lb = 0; lb < L.left_i.size(); ++lb)
+            for (uint32_t s = 0; s < L.phys_i.size(); ++s) {
+                uint32_t rb = L.right_i.position(L.left_i.charges[lb] + L.phys_i.charges[s]);
+                if (rb == L.right_i.size()) continue;
+                uint32_t ls = L.left_i.sizes[lb], rs = L.right_i.sizes[rb], d = L.phys_i.sizes[s];
+                uint64_t col0 = rpb(s, rb), row0 = lpb(s, lb);
+                for (uint32_t ss = 0; ss < d; ++ss) {
+                    uint64_t rp = L.rp_offset[lb] + (col0 + uint64_t(ss) * rs) * ls;   // ls x rs, ld ls
+                    uint64_t lp = L.lp_offset[rb] + row0 + uint64_t(ss) * ls;          // ls x rs, ld lp_rows
+                    if (to_left) {
+                        GatherRegion r{};
+                        r.dst_off = lp;
+                        r.R = ls;
+                        r.C = rs;
+                        r.ld_dst = L.lp_rows[rb];
+                        r.t0 = uint32_t(so.size());
+                        so.push_back(rp);
+                        al.push_back(1.0);
+                        r.t1 = uint32_t(so.size());
+                        if (ls && rs) regs.push_back(r);
+                    } else {
+                        for (uint32_t c = 0; c < rs; ++c) {
+                            GatherRegion r{};
+                            r.dst_off = rp + uint64_t(c) * ls;
+                            r.R = ls;
+                            r.C = 1;
+                            r.ld_dst = ls;
+                            r.t0 = uint32_t(so.size());
+                            so.push_back(lp + uint64_t(c) * L.lp_rows[rb]);
+                            al.push_back(1.0);
+                            r.t1 = uint32_t(so.size());
+                            if (ls) regs.push_back(r);
+                        }
+                    }
+                }
+            }
+        StagePlan P;
+        P.regions = std::move(regs);
+        P.src_off = std::move(so);
+        P.alpha = std::move(al);
+        P.finish_tables();
+        // validation: gather reads from src, writes to dst
+        uint64_t n_src = to_left ? L.rp_total : L.lp_total, n_dst = to_left ? L.lp_total : L.rp_total;
+        for (const auto& r : P.regions) {
+            MQ_REQUIRE(StagePlan::extent(r.R, r.C, r.ld_dst) + r.dst_off <= n_dst, "reshape dst out of range");
+            for (uint32_t t = r.t0; t < r.t1; ++t)
+                MQ_REQUIRE(P.src_off[t] + uint64_t(r.R) * r.C <= n_src, "reshape src out of range");
+        }
+        P.upload_all(ctx->stream);
+        HIPCHK(launch_fill(ctx->stream, dst, n_dst, 0.0));
+        HIPCHK(launch_gather(ctx->stream, P.d_regions.as<GatherRegion>(), uint32_t(P.regions.size()),
+                             P.d_src_off.as<uint64_t>(), P.d_alpha.as<double>(), P.g_chunks, const_cast<double*>(src),
+                             dst));
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // tables are freed on return
+    });
+}
+}  // namespace
+
+extern "C" {
+
+int mq_reshape_right_to_left(mq_ctx* ctx, const mq_site_desc* site, const double* src, double* dst) {
+    return reshape(ctx, site, src, dst, true);
+}
+int mq_reshape_left_to_right(mq_ctx* ctx, const mq_site_desc* site, const double* src, double* dst) {
+    return reshape(ctx, site, src, dst, false);
+}
+
+// ---------------------------------------------------------------------------- boundaries
+int mq_boundary_create_trivial(mq_ctx* ctx, const mq_index_desc* idx, uint32_t aux_dim, mq_boundary** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && idx && out && aux_dim > 0, "bad argument");
+        auto b = std::make_unique<mq_boundary>();
+        b->ctx = ctx;
+        Index I(*idx);
+        b->index = BoundaryIndex(I, I, aux_dim);
+        // Boundary(ud, ld, ad): one diagonal cohort per sector, blocks b*block_size (boundary.h:258-283)
+        for (uint32_t i = 0; i < I.size(); ++i) {
+            std::vector<int64_t> off(aux_dim);
+            uint64_t bs = uint64_t(I.sizes[i]) * I.sizes[i];
+            for (uint32_t k = 0; k < aux_dim; ++k) off[k] = int64_t(k * bs);
+            b->index.add_cohort(i, i, off);
+        }
+        b->layout();
+        b->data.alloc(b->total * sizeof(double));
+        HIPCHK(launch_fill(ctx->stream, b->ptr(), b->total, 1.0));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = b.release();
+    });
+}
+
+int mq_boundary_create(mq_ctx* ctx, const mq_bindex_desc* idx, const double* const* host, mq_boundary** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && idx && out, "bad argument");
+        auto b = std::make_unique<mq_boundary>();
+        b->ctx = ctx;
+        b->index = BoundaryIndex(*idx);
+        for (uint32_t ci = 0; ci < b->index.n_cohorts(); ++ci)
+            for (uint32_t k = 0; k < b->index.aux_dim; ++k) {
+                int64_t o = b->index.offset(ci, k);
+                uint32_t owner = b->index.tr[ci] ? b->index.cohort_index(b->index.ket.charges[b->index.ket_pos[ci]],
+                                                                         b->index.bra.charges[b->index.bra_pos[ci]])
+                                                 : ci;
+                if (o > -1 && owner < b->index.n_cohorts())
+                    MQ_REQUIRE(uint64_t(o) + b->index.block_size(ci) <= b->index.cohort_size(owner) ||
+                                   b->index.transposes[size_t(ci) * b->index.aux_dim + k],
+                               "boundary offset beyond cohort data");
+            }
+        b->layout();
+        b->data.alloc(b->total * sizeof(double));
+        HIPCHK(launch_fill(ctx->stream, b->ptr(), b->total, 0.0));
+        if (host)
+            for (uint32_t ci = 0; ci < b->index.n_cohorts(); ++ci) {
+                uint64_t n = b->index.cohort_size(ci);
+                if (n && host[ci])
+                    HIPCHK(hipMemcpyAsync(b->ptr() + b->cohort_off[ci], host[ci], n * sizeof(double),
+                                          hipMemcpyHostToDevice, ctx->stream));
+            }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = b.release();
+    });
+}
+
+int mq_boundary_destroy(mq_boundary* b) {
+    return guarded([&] { delete b; });
+}
+
+int mq_boundary_index(const mq_boundary* b, mq_bindex_desc* out) {
+    return guarded([&] {
+        MQ_REQUIRE(b && out, "bad argument");
+        b->index.fill_desc(out, b->bra_c, b->ket_c);
+    });
+}
+
+int mq_boundary_download(const mq_boundary* b, uint32_t ci, double* host) {
+    return guarded([&] {
+        MQ_REQUIRE(b && host && ci < b->index.n_cohorts(), "bad argument");
+        uint64_t n = b->index.cohort_size(ci);
+        if (n)
+            HIPCHK(hipMemcpyAsync(host, b->ptr() + b->cohort_off[ci], n * sizeof(double), hipMemcpyDeviceToHost,
+                                  b->ctx->stream));
+        HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    });
+}
+
+int mq_boundary_cohort_ptr(const mq_boundary* b, uint32_t ci, const double** dev) {
+    return guarded([&] {
+        MQ_REQUIRE(b && dev && ci < b->index.n_cohorts(), "bad argument");
+        *dev = b->ptr() + b->cohort_off[ci];
+    });
+}
+
+int mq_boundary_traces(const mq_boundary* b, double* out) {
+    return guarded([&] {
+        MQ_REQUIRE(b && out, "bad argument");
+        const BoundaryIndex& I = b->index;
+        std::vector<uint64_t> off;
+        std::vector<uint32_t> len, bb;
+        for (uint32_t ci = 0; ci < I.n_cohorts(); ++ci) {
+            if (I.tr[ci]) continue;
+            for (uint32_t k = 0; k < I.aux_dim; ++k) {
+                int64_t o = I.offset(ci, k);
+                if (o < 0 || I.transposes[size_t(ci) * I.aux_dim + k]) continue;
+                off.push_back(b->cohort_off[ci] + uint64_t(o));
+                len.push_back(uint32_t(I.block_size(ci)));
+                bb.push_back(k);
+            }
+        }
+        std::fill(out, out + I.aux_dim, 0.0);
+        if (off.empty()) return;
+        DevBuf d_off, d_len, d_out;
+        upload(d_off, off, b->ctx->stream);
+        upload(d_len, len, b->ctx->stream);
+        d_out.alloc(off.size() * sizeof(double));
+        HIPCHK(launch_block_sums(b->ctx->stream, d_off.as<uint64_t>(), d_len.as<uint32_t>(), nullptr,
+                                 uint32_t(off.size()), b->ptr(), d_out.as<double>()));
+        std::vector<double> part(off.size());
+        HIPCHK(hipMemcpyAsync(part.data(), d_out.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              b->ctx->stream));
+        HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        for (size_t i = 0; i < part.size(); ++i) out[bb[i]] += part[i];
+    });
+}
+
+// ---------------------------------------------------------------------------- schedules
+int mq_schedule_build_sigma(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* left,
+                            const mq_bindex_desc* right, mq_schedule** out) {
+    return guarded([&] {
+        MQ_REQUIRE(site && mpo && left && right && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        BoundaryIndex LI(*left), RI(*right);
+        auto s = std::make_unique<mq_schedule>();
+        s->s = build_sigma_schedule(L, W, LI, RI);
+        s->flops = schedule_flops(s->s, RI, L.left_i.sizes);
+        *out = s.release();
+    });
+}
+
+int mq_schedule_build_left(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* left,
+                           int symmetric, mq_schedule** out) {
+    return guarded([&] {
+        MQ_REQUIRE(site && mpo && left && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        BoundaryIndex LI(*left);
+        auto s = std::make_unique<mq_schedule>();
+        s->s = build_left_schedule(L, W, LI, symmetric != 0);
+        s->flops = schedule_flops(s->s, LI, L.right_i.sizes);
+        *out = s.release();
+    });
+}
+
+int mq_schedule_build_right(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* right,
+                            int symmetric, mq_schedule** out) {
+    return guarded([&] {
+        MQ_REQUIRE(site && mpo && right && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        BoundaryIndex RI(*right);
+        auto s = std::make_unique<mq_schedule>();
+        s->s = build_right_schedule(L, W, RI, symmetric != 0);
+        s->flops = schedule_flops(s->s, RI, L.left_i.sizes);
+        *out = s.release();
+    });
+}
+
+int mq_schedule_out_index(const mq_schedule* s, mq_bindex_desc* out) {
+    return guarded([&] {
+        MQ_REQUIRE(s && out, "bad argument");
+        MQ_REQUIRE(s->s.kind != Schedule::SIGMA, "sigma schedules produce no boundary");
+        s->s.out_index.fill_desc(out, s->bra_c, s->ket_c);
+    });
+}
+
+int mq_schedule_export(const mq_schedule* s, mq_schedule_desc* out) {
+    return guarded([&] {
+        MQ_REQUIRE(s && out, "bad argument");
+        s->s.export_desc(out);
+    });
+}
+
+int mq_schedule_flops(const mq_schedule* s, uint64_t* flops) {
+    return guarded([&] {
+        MQ_REQUIRE(s && flops, "bad argument");
+        *flops = s->flops;
+    });
+}
+
+int mq_schedule_destroy(mq_schedule* s) {
+    return guarded([&] { delete s; });
+}
+
+}  // extern "C"
+
+// ============================================================================ sigma plans
+namespace {
+
+// Lower a sigma schedule: T stage, S stage, then per output block one K-segmented GEMM over
+// every cohort that writes it (Cohort::contract, tasks.cpp:286-344).
+void lower_sigma(mq_plan* p, const Schedule& S, const std::vector<uint32_t>& ket_rows,
+                 const std::vector<uint32_t>& ket_cols, const std::vector<uint64_t>& ket_off) {
+    const BoundaryIndex& LI = p->left->index;
+    const BoundaryIndex& RI = p->right->index;
+    StagePlan& P = p->st;
+    uint64_t ws = 0;
+    std::vector<uint64_t> t_off = lower_T_right(P, S, RI, p->right->cohort_off, ket_rows, ket_off, ws);
+    std::vector<uint64_t> s_off = lower_S_right(P, S, t_off, ws);
+    // group cohorts by output block
+    std::vector<std::vector<uint32_t>> by_out(ket_rows.size());
+    for (uint32_t c = 0; c < S.n_cohorts(); ++c) {
+        MQ_REQUIRE(S.coh_rb[c] < ket_rows.size(), "cohort output block out of range");
+        by_out[S.coh_rb[c]].push_back(c);
+    }
+    for (uint32_t lbo = 0; lbo < ket_rows.size(); ++lbo) {
+        if (by_out[lbo].empty()) continue;
+        std::vector<GemmSeg> segs;
+        uint32_t M = ket_rows[lbo], N = ket_cols[lbo];
+        for (uint32_t c : by_out[lbo]) {
+            uint32_t ci = S.coh_ci[c], cie = S.coh_ci_eff[c];
+            uint32_t ls_in = S.coh_ls[c], ls_out = S.coh_rs[c], nS = S.coh_nSrows[c];
+            MQ_REQUIRE(ls_out == M && S.coh_stripe[c] == N, "cohort shape does not match the output block");
+            MQ_REQUIRE(cie < LI.n_cohorts() && uint64_t(nS) * ls_in * ls_out <= LI.cohort_size(cie),
+                       "left cohort smaller than the schedule assumes");
+            uint64_t ldS = uint64_t(nS) * ls_in;
+            if (ci == cie) {
+                segs.push_back(seg(BASE_LEFT, p->left->cohort_off[cie], ls_out, false, BASE_WS, s_off[c],
+                                   uint32_t(ldS), false, uint32_t(ldS)));
+            } else {
+                for (uint32_t b = 0; b < nS; ++b)
+                    segs.push_back(seg(BASE_LEFT, p->left->cohort_off[cie] + uint64_t(b) * ls_in * ls_out, ls_in, true,
+                                       BASE_WS, s_off[c] + uint64_t(b) * ls_in, uint32_t(ldS), false, ls_in));
+            }
+        }
+        add_gemm(P.f_tasks, P.f_segs, BASE_OUT, ket_off[lbo], M, N, M, segs);
+    }
+    P.ws_elems = ws;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sched, uint32_t n_ket_blocks,
+                                       const uint32_t* ket_rows, const uint32_t* ket_cols, const mq_boundary* left,
+                                       const mq_boundary* right, mq_plan** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && sched && ket_rows && ket_cols && left && right && out, "bad argument");
+        MQ_REQUIRE(sched->n_blocks == n_ket_blocks, "schedule block count != psi block count");
+        // sizes of the flattened arrays
+        uint32_t nc = uint32_t(sched->coh_begin[sched->n_blocks]);
+        uint32_t nu = uint32_t(sched->coh_unit_begin[nc]);
+        uint64_t nl = sched->unit_line_begin[nu], nt = sched->unit_task_begin[nu];
+        Schedule S;
+        S.import_desc(*sched, nc, nu, nl, nt);
+        auto p = std::make_unique<mq_plan>();
+        p->ctx = ctx;
+        p->left = left;
+        p->right = right;
+        std::vector<uint32_t> rows(ket_rows, ket_rows + n_ket_blocks), cols(ket_cols, ket_cols + n_ket_blocks);
+        std::vector<uint64_t> off(n_ket_blocks);
+        uint64_t tot = 0;
+        for (uint32_t b = 0; b < n_ket_blocks; ++b) off[b] = tot, tot += round_up(uint64_t(rows[b]) * cols[b], MQ_BUFFER_ALIGNMENT);
+        p->vec_size = tot;
+        p->out_offsets = off;
+        lower_sigma(p.get(), S, rows, cols, off);
+        p->flops = schedule_flops(S, right->index, rows);
+        StagePlan& P = p->st;
+        P.finish_tables();
+        P.sizes[BASE_WS] = P.ws_elems;
+        P.sizes[BASE_IN] = P.sizes[BASE_OUT] = tot;
+        P.sizes[BASE_LEFT] = left->total;
+        P.sizes[BASE_RIGHT] = right->total;
+        P.validate();
+        p->ws.alloc(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
+        P.upload_all(ctx->stream);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = p.release();
+    });
+}
+
+int mq_sigma_plan_create(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_boundary* left,
+                         const mq_boundary* right, mq_plan** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && site && mpo && left && right && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        Schedule S = build_sigma_schedule(L, W, left->index, right->index);
+        mq_schedule_desc d{};
+        S.export_desc(&d);
+        int rc = mq_sigma_plan_create_from_schedule(ctx, &d, L.left_i.size(), L.left_i.sizes.data(), L.rp_cols.data(),
+                                                    left, right, out);
+        if (rc != MQ_OK) fail(rc, g_err);
+    });
+}
+
+int mq_sigma_apply(mq_plan* p, const double* psi, double* sigma) {
+    return guarded([&] {
+        MQ_REQUIRE(p && psi && sigma, "bad argument");
+        MQ_REQUIRE(psi != sigma, "psi and sigma must not alias");
+        Bases b{};
+        b.p[BASE_WS] = p->ws.as<double>();
+        b.p[BASE_IN] = const_cast<double*>(psi);
+        b.p[BASE_OUT] = sigma;
+        b.p[BASE_LEFT] = p->left->ptr();
+        b.p[BASE_RIGHT] = p->right->ptr();
+        b.p[BASE_BRA] = nullptr;
+        HIPCHK(launch_fill(p->ctx->stream, sigma, p->vec_size, 0.0));
+        p->st.run(p->ctx->stream, b);
+    });
+}
+
+int mq_plan_vector_size(const mq_plan* p, uint64_t* n) {
+    return guarded([&] {
+        MQ_REQUIRE(p && n, "bad argument");
+        *n = p->vec_size;
+    });
+}
+
+int mq_plan_flops(const mq_plan* p, uint64_t* f) {
+    return guarded([&] {
+        MQ_REQUIRE(p && f, "bad argument");
+        *f = p->flops;
+    });
+}
+
+int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
+    return guarded([&] {
+        MQ_REQUIRE(p && bytes, "bad argument");
+        const StagePlan& P = p->st;
+        *bytes = p->ws.bytes + P.d_t_tasks.bytes + P.d_t_segs.bytes + P.d_f_tasks.bytes + P.d_f_segs.bytes +
+                 P.d_regions.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
+    });
+}
+
+int mq_plan_destroy(mq_plan* p) {
+    return guarded([&] {
+        if (p && p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+        delete p;
+    });
+}
+
+}  // extern "C"
+
+// ============================================================================ boundary steps
+namespace {
+
+// New boundary storage for the cohorts of a step: schedule cohort c is new cohort c
+// (add_cohort in schedule order, move_boundary.hpp:202-205, 331-336).
+std::unique_ptr<mq_boundary> new_boundary(mq_ctx* ctx, const Schedule& S) {
+    auto b = std::make_unique<mq_boundary>();
+    b->ctx = ctx;
+    b->index = S.out_index;
+    b->layout();
+    b->data.alloc(b->total * sizeof(double));
+    HIPCHK(launch_fill(ctx->stream, b->ptr(), b->total, 0.0));
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+// overlap_mpo_left_step (move_boundary.hpp:147-279): T_ti = L_b * ket slab (create_T_left,
+// tasks.cpp:545-648), S_left gather (create_s, tasks.cpp:425-464), L'(ci) = bra_lp^T * S (prop_l,
+// tasks.cpp:198-241).
+int mq_boundary_left_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_lp,
+                          const double* ket_rp, const mq_boundary* left, int symmetric, mq_boundary** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && site && mpo && bra_lp && ket_rp && left && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        Schedule S = build_left_schedule(L, W, left->index, symmetric != 0);
+        const BoundaryIndex& LI = left->index;
+        auto nb = new_boundary(ctx, S);
+        StagePlan P;
+        uint64_t ws = 0;
+        std::vector<uint64_t> t_off(S.t_ci.size());
+        for (uint32_t m = 0; m < S.n_blocks; ++m) {   // m = rb_ket
+            uint32_t N = L.right_i.sizes[m];
+            for (uint64_t ti = S.t_begin[m]; ti < S.t_begin[m + 1]; ++ti) {
+                uint32_t ci = S.t_ci[ti], cie = S.t_ci_eff[ti], lbk = S.t_lb_ket[ti];
+                uint32_t M = uint32_t(LI.left_size(ci)), K = uint32_t(LI.right_size(ci)), nbk = LI.n_blocks[cie];
+                MQ_REQUIRE(K == L.left_i.sizes[lbk], "left cohort ket size != ket block rows");
+                t_off[ti] = ws;
+                ws += align32(uint64_t(nbk) * M * N);
+                uint64_t b_off = L.rp_offset[lbk] + uint64_t(S.t_mps_offset[ti]) * K;
+                for (uint32_t b = 0; b < nbk; ++b) {
+                    uint64_t a_off = left->cohort_off[cie] + uint64_t(b) * M * K;
+                    bool tr = (ci != cie);
+                    add_gemm(P.t_tasks, P.t_segs, BASE_WS, t_off[ti] + uint64_t(b) * M * N, M, N, M,
+                             {seg(BASE_LEFT, a_off, tr ? K : M, tr, BASE_IN, b_off, K, false, K)});
+                }
+            }
+        }
+        // S (stripe x nSrows*rs), column stride stripe (create_s, tasks.cpp:425-454)
+        std::vector<uint64_t> s_off(S.n_cohorts());
+        P.s_begin = ws;
+        for (uint32_t m = 0; m < S.n_blocks; ++m)
+            for (uint64_t c = S.coh_begin[m]; c < S.coh_begin[m + 1]; ++c) {
+                uint32_t rs = S.coh_rs[c], nS = S.coh_nSrows[c], stripe = S.coh_stripe[c];
+                s_off[c] = ws;
+                ws += align32(uint64_t(stripe) * nS * rs);
+                for (uint64_t u = S.coh_unit_begin[c]; u < S.coh_unit_begin[c + 1]; ++u) {
+                    uint32_t ms = S.unit_ms[u], off = S.unit_offset[u];
+                    uint64_t task = S.unit_task_begin[u];
+                    for (uint64_t l = S.unit_line_begin[u]; l < S.unit_line_begin[u + 1]; ++l) {
+                        uint32_t b = S.line_b1[l], cnt = S.line_count[l];
+                        MQ_REQUIRE(b < nS && uint64_t(off) + ms <= stripe, "S region outside S");
+                        GatherRegion r{};
+                        r.dst_off = s_off[c] + uint64_t(b) * rs * stripe + off;
+                        r.R = ms;
+                        r.C = rs;
+                        r.ld_dst = stripe;
+                        r.t0 = uint32_t(P.src_off.size());
+                        for (uint32_t k = 0; k < cnt; ++k, ++task) {
+                            uint64_t ti = S.t_begin[m] + S.task_ti[task];
+                            MQ_REQUIRE(ti < S.t_begin[m + 1], "task ti out of range");
+                            P.src_off.push_back(t_off[ti] + uint64_t(S.task_col[task]) * rs);
+                            P.alpha.push_back(S.task_alpha[task]);
+                        }
+                        r.t1 = uint32_t(P.src_off.size());
+                        if (uint64_t(r.R) * r.C) P.regions.push_back(r);
+                    }
+                }
+            }
+        P.s_elems = ws - P.s_begin;
+        // prop_l: new cohort c (rs_bra x nSrows*rs_ket) = bra_lp(rb_bra)^T * S
+        for (uint32_t c = 0; c < S.n_cohorts(); ++c) {
+            uint32_t rbb = S.coh_lb[c], ls = S.coh_ls[c], rs = S.coh_rs[c], nS = S.coh_nSrows[c], stripe = S.coh_stripe[c];
+            MQ_REQUIRE(stripe == L.lp_rows[rbb] && ls == L.right_i.sizes[rbb], "prop_l shape mismatch");
+            MQ_REQUIRE(nb->index.cohort_size(c) == uint64_t(nS) * ls * rs, "new cohort size mismatch");
+            add_gemm(P.f_tasks, P.f_segs, BASE_OUT, nb->cohort_off[c], ls, nS * rs, ls,
+                     {seg(BASE_BRA, L.lp_offset[rbb], stripe, true, BASE_WS, s_off[c], stripe, false, stripe)});
+        }
+        P.ws_elems = ws;
+        P.finish_tables();
+        P.sizes[BASE_WS] = ws;
+        P.sizes[BASE_IN] = L.rp_total;
+        P.sizes[BASE_BRA] = L.lp_total;
+        P.sizes[BASE_OUT] = nb->total;
+        P.sizes[BASE_LEFT] = left->total;
+        P.validate();
+        P.upload_all(ctx->stream);
+        ctx->step_ws.ensure(std::max<uint64_t>(ws, 32) * sizeof(double));
+        Bases bs{};
+        bs.p[BASE_WS] = ctx->step_ws.as<double>();
+        bs.p[BASE_IN] = const_cast<double*>(ket_rp);
+        bs.p[BASE_OUT] = nb->ptr();
+        bs.p[BASE_LEFT] = left->ptr();
+        bs.p[BASE_BRA] = const_cast<double*>(bra_lp);
+        P.run(ctx->stream, bs);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = nb.release();
+    });
+}
+
+// overlap_mpo_right_step (move_boundary.hpp:281-404): T as in sigma, S = create_s_r, then
+// R'_b = S_b * bra_rp^T for every b-row (prop_r, tasks.cpp:243-284).
+int mq_boundary_right_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_rp,
+                           const double* ket_rp, const mq_boundary* right, int symmetric, mq_boundary** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && site && mpo && bra_rp && ket_rp && right && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        Schedule S = build_right_schedule(L, W, right->index, symmetric != 0);
+        auto nb = new_boundary(ctx, S);
+        StagePlan P;
+        uint64_t ws = 0;
+        std::vector<uint64_t> t_off = lower_T_right(P, S, right->index, right->cohort_off, L.left_i.sizes, L.rp_offset, ws);
+        std::vector<uint64_t> s_off = lower_S_right(P, S, t_off, ws);
+        for (uint32_t c = 0; c < S.n_cohorts(); ++c) {
+            uint32_t lbb = S.coh_rb[c], ls = S.coh_ls[c], rs = S.coh_rs[c], nS = S.coh_nSrows[c], stripe = S.coh_stripe[c];
+            MQ_REQUIRE(stripe == L.rp_cols[lbb] && rs == L.left_i.sizes[lbb], "prop_r shape mismatch");
+            MQ_REQUIRE(nb->index.cohort_size(c) == uint64_t(nS) * ls * rs, "new cohort size mismatch");
+            uint32_t ldS = nS * ls;
+            for (uint32_t b = 0; b < nS; ++b)
+                add_gemm(P.f_tasks, P.f_segs, BASE_OUT, nb->cohort_off[c] + uint64_t(b) * ls * rs, ls, rs, ls,
+                         {seg(BASE_WS, s_off[c] + uint64_t(b) * ls, ldS, false, BASE_BRA, L.rp_offset[lbb], rs, true, stripe)});
+        }
+        P.ws_elems = ws;
+        P.finish_tables();
+        P.sizes[BASE_WS] = ws;
+        P.sizes[BASE_IN] = L.rp_total;
+        P.sizes[BASE_BRA] = L.rp_total;
+        P.sizes[BASE_OUT] = nb->total;
+        P.sizes[BASE_RIGHT] = right->total;
+        P.validate();
+        P.upload_all(ctx->stream);
+        ctx->step_ws.ensure(std::max<uint64_t>(ws, 32) * sizeof(double));
+        Bases bs{};
+        bs.p[BASE_WS] = ctx->step_ws.as<double>();
+        bs.p[BASE_IN] = const_cast<double*>(ket_rp);
+        bs.p[BASE_OUT] = nb->ptr();
+        bs.p[BASE_RIGHT] = right->ptr();
+        bs.p[BASE_BRA] = const_cast<double*>(bra_rp);
+        P.run(ctx->stream, bs);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = nb.release();
+    });
+}
+
+// ---------------------------------------------------------------------------- vectors
+int mq_vec_dot(mq_ctx* ctx, uint64_t n, const double* x, const double* y, double* result) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && x && y && result, "bad argument");
+        HIPCHK(launch_dot(ctx->stream, n, x, y, ctx->red_partial.as<double>(), ctx->red_result.as<double>()));
+        HIPCHK(hipMemcpyAsync(result, ctx->red_result.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mq_vec_nrm2(mq_ctx* ctx, uint64_t n, const double* x, double* result) {
+    return guarded([&] {
+        double d = 0;
+        int rc = mq_vec_dot(ctx, n, x, x, &d);
+        if (rc) fail(rc, g_err);
+        *result = std::sqrt(d);
+    });
+}
+
+int mq_vec_axpy(mq_ctx* ctx, uint64_t n, double a, const double* x, double* y) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && x && y, "bad argument");
+        HIPCHK(launch_axpy(ctx->stream, n, a, x, y));
+    });
+}
+
+int mq_vec_scal(mq_ctx* ctx, uint64_t n, double a, double* x) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && x, "bad argument");
+        HIPCHK(launch_scal(ctx->stream, n, a, x));
+    });
+}
+
+int mq_vec_copy(mq_ctx* ctx, uint64_t n, const double* x, double* y) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && x && y, "bad argument");
+        if (n) HIPCHK(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+}  // extern "C"
+
+// ============================================================================ Jacobi-Davidson
+namespace {
+
+// Smallest eigenpair of the symmetric dim x dim matrix M (column-major, upper triangle used) by
+// cyclic Jacobi rotations; replaces LAPACK dsyevx with range 'I', il = iu = 1 (ietl/jacobi.h:476-516).
+double smallest_eigenpair(const std::vector<double>& Mfull, int ld, int dim, std::vector<double>& s) {
+    std::vector<double> A(dim * dim), V(dim * dim, 0.0);
+    for (int j = 0; j < dim; ++j)
+        for (int i = 0; i < dim; ++i) A[i + j * dim] = (i <= j) ? Mfull[i + j * ld] : Mfull[j + i * ld];
+    for (int i = 0; i < dim; ++i) V[i + i * dim] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0;
+        for (int j = 0; j < dim; ++j)
+            for (int i = 0; i < j; ++i) off += A[i + j * dim] * A[i + j * dim];
+        if (off < 1e-30) break;
+        for (int p = 0; p < dim; ++p)
+            for (int q = p + 1; q < dim; ++q) {
+                double apq = A[p + q * dim];
+                if (std::fabs(apq) < 1e-300) continue;
+                double app = A[p + p * dim], aqq = A[q + q * dim];
+                double theta = (aqq - app) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < dim; ++k) {
+                    double akp = A[k + p * dim], akq = A[k + q * dim];
+                    A[k + p * dim] = c * akp - sn * akq;
+                    A[k + q * dim] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < dim; ++k) {
+                    double apk = A[p + k * dim], aqk = A[q + k * dim];
+                    A[p + k * dim] = c * apk - sn * aqk;
+                    A[q + k * dim] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < dim; ++k) {
+                    double vkp = V[k + p * dim], vkq = V[k + q * dim];
+                    V[k + p * dim] = c * vkp - sn * vkq;
+                    V[k + q * dim] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    int imin = 0;
+    for (int i = 1; i < dim; ++i)
+        if (A[i + i * dim] < A[imin + imin * dim]) imin = i;
+    s.assign(dim, 0.0);
+    for (int i = 0; i < dim; ++i) s[i] = V[i + imin * dim];
+    return A[imin + imin * dim];
+}
+
+}  // namespace
+
+extern "C" {
+
+// ietl::jacobi_davidson::calculate_eigenvalue (ietl/jacobi.h:386-474) with
+// basic_iteration(max_iter, tol, tol) (ietl/iteration.h:40-78, solver/ietl_jacobi_davidson.h:58-64).
+int mq_jd_solve(mq_plan* p, const double* init, double* out, double tol, int max_iter, int gmres_iter,
+                double* energy, int* iterations) {
+    return guarded([&] {
+        MQ_REQUIRE(p && init && out && energy && max_iter > 0, "bad argument");
+        MQ_REQUIRE(gmres_iter == 0, "GMRES correction is not implemented; use gmres_iter = 0");
+        mq_ctx* ctx = p->ctx;
+        const uint64_t n = p->vec_size;
+        const uint64_t stride = align32(n);
+        // V[0..max_iter], VA[0..max_iter-1], u, uA
+        int cap = 2 * max_iter + 3;
+        if (p->jd_cap < cap) {
+            p->jd_buf.alloc(uint64_t(cap) * stride * sizeof(double));
+            p->jd_cap = cap;
+        }
+        double* base = p->jd_buf.as<double>();
+        auto V = [&](int i) { return base + uint64_t(i) * stride; };
+        auto VA = [&](int i) { return base + uint64_t(max_iter + 1 + i) * stride; };
+        double* u = base + uint64_t(2 * max_iter + 1) * stride;
+        double* uA = base + uint64_t(2 * max_iter + 2) * stride;
+        auto dot = [&](const double* x, const double* y) {
+            double r;
+            int rc = mq_vec_dot(ctx, n, x, y, &r);
+            if (rc) fail(rc, g_err);
+            return r;
+        };
+        auto axpy = [&](double a, const double* x, double* y) { HIPCHK(launch_axpy(ctx->stream, n, a, x, y)); };
+        auto scal = [&](double a, double* x) { HIPCHK(launch_scal(ctx->stream, n, a, x)); };
+        auto copy = [&](const double* x, double* y) {
+            HIPCHK(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        };
+
+        std::vector<double> M(size_t(max_iter) * max_iter, 0.0), s;
+        const double kappa = 0.25;
+        copy(init, V(0));
+        int it = 0;
+        double theta = 0;
+        while (true) {
+            double* t = V(it);
+            double tau = std::sqrt(dot(t, t));
+            for (int i = 0; i < it; ++i) axpy(-dot(V(i), t), V(i), t);
+            if (std::sqrt(dot(t, t)) < kappa * tau)
+                for (int i = 0; i < it; ++i) axpy(-dot(V(i), t), V(i), t);
+            double nt = std::sqrt(dot(t, t));
+            MQ_REQUIRE(nt > 0, "Jacobi-Davidson: zero search vector");
+            scal(1.0 / nt, t);
+            int rc = mq_sigma_apply(p, t, VA(it));
+            if (rc) fail(rc, g_err);
+            for (int i = 0; i <= it; ++i) M[i + size_t(it) * max_iter] = dot(V(i), VA(it));
+            theta = smallest_eigenpair(M, max_iter, it + 1, s);
+            copy(V(0), u);
+            scal(s[0], u);
+            copy(VA(0), uA);
+            scal(s[0], uA);
+            for (int j = 1; j <= it; ++j) {
+                axpy(s[j], V(j), u);
+                axpy(s[j], VA(j), uA);
+            }
+            axpy(-theta, u, uA);   // r = uA - theta u
+            ++it;
+            double rn = std::sqrt(dot(uA, uA));
+            bool conv = (rn <= tol * std::fabs(theta) || rn < tol);
+            if (conv || it >= max_iter) break;
+            // t = -r + (r.u)/(u.u) u   (jcd_gmres_solver with max_iter 0, ietl/jacobi.h:141-157)
+            double dru = dot(uA, u), duu = dot(u, u);
+            double* tn = V(it);
+            copy(uA, tn);
+            scal(-1.0, tn);
+            axpy(dru / duu, u, tn);
+        }
+        copy(u, out);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *energy = theta;
+        if (iterations) *iterations = it;
+    });
+}
+
+}  // extern "C"

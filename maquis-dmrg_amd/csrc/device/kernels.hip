@@ -1,0 +1,290 @@
+// gfx950 kernels of the sigma / boundary engine.
+//
+//  gemm_f64      grouped, K-segmented FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64); one launch
+//                covers every block product of a stage (the reference issues one cuBLAS call per
+//                product, solver/tasks.cpp:226-229, 277-280, 335-336, 630-639, 738-748).
+//  gather        the MPO-term gather/scale of T slabs into S (reference kernels
+//                compute_s_stackedv / compute_s_left_stackedv, solver/numeric/gpu_dgemm_ddot.cu:215-359):
+//                every region is owned by one workgroup chunk and written exactly once, so no
+//                atomics and no memset of S are needed for covered regions.
+//  dot/axpy/...  Davidson vector kernels with a fixed-shape (deterministic) reduction tree.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace mq {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------------- GEMM
+// Workgroup = 4 wavefronts = one 64x64 tile of C; wave w owns the 32x32 quadrant
+// (w&1, w>>1) as 2x2 MFMA 16x16 sub-tiles.  Operands are staged through LDS in
+// k-major order: As[k][m], Bs[k][n].
+//
+// v_mfma_f64_16x16x4f64 operand layout on gfx950 (lane l = 0..63, measured with one-hot operands):
+//   A: row l%16, k l/16      B: col l%16, k l/16      D[r]: row 4*r + l/16, col l%16
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* __restrict__ tasks,
+                                                                uint32_t n_tasks,
+                                                                const GemmSeg* __restrict__ segs,
+                                                                Bases bases) {
+    __shared__ double As[GEMM_BK][GEMM_BM + 1];
+    __shared__ double Bs[GEMM_BK][GEMM_BN + 1];
+
+    const uint32_t tile = blockIdx.x;
+    uint32_t lo = 0, hi = n_tasks;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (tasks[mid].tile_begin <= tile) lo = mid; else hi = mid;
+    }
+    const GemmTask tk = tasks[lo];
+    const uint32_t local = tile - tk.tile_begin;
+    const uint32_t m0 = (local / tk.tiles_n) * GEMM_BM;
+    const uint32_t n0 = (local % tk.tiles_n) * GEMM_BN;
+    const uint32_t M = tk.M, N = tk.N;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int l16 = lane & 15, lk = lane >> 4;
+
+    d4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+
+    for (uint32_t sgi = tk.seg0; sgi < tk.seg1; ++sgi) {
+        const GemmSeg sg = segs[sgi];
+        const double* __restrict__ A = bases.p[sg.a_base] + sg.a_off;
+        const double* __restrict__ B = bases.p[sg.b_base] + sg.b_off;
+        const uint32_t K = sg.K;
+        for (uint32_t k0 = 0; k0 < K; k0 += GEMM_BK) {
+            if (!sg.trans_a) {
+                const int m = tid & 63;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int k = (tid >> 6) + 4 * i;
+                    const uint32_t gm = m0 + m, gk = k0 + k;
+                    As[k][m] = (gm < M && gk < K) ? A[uint64_t(gk) * sg.lda + gm] : 0.0;
+                }
+            } else {
+                const int k = tid & 15;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int m = (tid >> 4) + 16 * i;
+                    const uint32_t gm = m0 + m, gk = k0 + k;
+                    As[k][m] = (gm < M && gk < K) ? A[uint64_t(gm) * sg.lda + gk] : 0.0;
+                }
+            }
+            if (!sg.trans_b) {
+                const int k = tid & 15;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int n = (tid >> 4) + 16 * i;
+                    const uint32_t gn = n0 + n, gk = k0 + k;
+                    Bs[k][n] = (gn < N && gk < K) ? B[uint64_t(gn) * sg.ldb + gk] : 0.0;
+                }
+            } else {
+                const int n = tid & 63;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int k = (tid >> 6) + 4 * i;
+                    const uint32_t gn = n0 + n, gk = k0 + k;
+                    Bs[k][n] = (gn < N && gk < K) ? B[uint64_t(gk) * sg.ldb + gn] : 0.0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < GEMM_BK; kk += 4) {
+                const int kr = kk + lk;
+                const double a0 = As[kr][wm + l16], a1 = As[kr][wm + 16 + l16];
+                const double b0 = Bs[kr][wn + l16], b1 = Bs[kr][wn + 16 + l16];
+                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+                acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+            }
+            __syncthreads();
+        }
+    }
+
+    double* __restrict__ C = bases.p[tk.c_base] + tk.c_off;
+    const uint64_t ldc = tk.ldc;
+    const d4* accs[4] = {&acc00, &acc01, &acc10, &acc11};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = q >> 1, j = q & 1;
+        const uint32_t gn = n0 + wn + j * 16 + l16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t gm = m0 + wm + i * 16 + 4 * r + lk;
+            if (gm < M && gn < N) {
+                double v = (*accs[q])[r];
+                double* p = C + uint64_t(gn) * ldc + gm;
+                if (tk.beta) v += *p;
+                *p = v;
+            }
+        }
+    }
+}
+
+hipError_t launch_gemm(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
+                       uint32_t n_tiles, Bases bases) {
+    if (n_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(gemm_f64_kernel, dim3(n_tiles), dim3(GEMM_THREADS), 0, st, tasks, n_tasks, segs, bases);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- gather
+// One workgroup = GATHER_CHUNK consecutive elements of one region.  The (src, alpha) list of
+// the region is uniform across the workgroup (scalar loads); every T slab read is a
+// contiguous, coalesced stream.
+__global__ __launch_bounds__(GATHER_THREADS) void gather_kernel(const GatherRegion* __restrict__ regions,
+                                                                uint32_t n_regions,
+                                                                const uint64_t* __restrict__ src_off,
+                                                                const double* __restrict__ alpha,
+                                                                const double* __restrict__ ws,
+                                                                double* __restrict__ dst_base) {
+    const uint32_t chunk = blockIdx.x;
+    uint32_t lo = 0, hi = n_regions;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (regions[mid].chunk_begin <= chunk) lo = mid; else hi = mid;
+    }
+    const GatherRegion rg = regions[lo];
+    const uint64_t total = uint64_t(rg.R) * rg.C;
+    const uint64_t e0 = uint64_t(chunk - rg.chunk_begin) * GATHER_CHUNK + threadIdx.x;
+    constexpr int PER = GATHER_CHUNK / GATHER_THREADS;
+    double acc[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) acc[j] = 0.0;
+    for (uint32_t t = rg.t0; t < rg.t1; ++t) {
+        const double a = alpha[t];
+        const double* __restrict__ src = ws + src_off[t];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint64_t e = e0 + uint64_t(j) * GATHER_THREADS;
+            if (e < total) acc[j] += a * src[e];
+        }
+    }
+    double* __restrict__ dst = dst_base + rg.dst_off;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const uint64_t e = e0 + uint64_t(j) * GATHER_THREADS;
+        if (e < total) {
+            const uint64_t c = e / rg.R, r = e - c * rg.R;
+            dst[c * rg.ld_dst + r] = acc[j];
+        }
+    }
+}
+
+hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n_regions, const uint64_t* src_off,
+                         const double* alpha, uint32_t n_chunks, double* ws, double* dst_base) {
+    if (n_chunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kernel, dim3(n_chunks), dim3(GATHER_THREADS), 0, st, regions, n_regions, src_off,
+                       alpha, ws, dst_base);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- elementwise
+__global__ void fill_kernel(double* __restrict__ x, uint64_t n, double v) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        x[i] = v;
+}
+
+static inline unsigned grid_for(uint64_t n, unsigned threads) {
+    uint64_t g = (n + threads - 1) / threads;
+    if (g > 8192) g = 8192;
+    if (g == 0) g = 1;
+    return unsigned(g);
+}
+
+hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, x, n, v);
+    return hipGetLastError();
+}
+
+__global__ void axpy_kernel(uint64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        y[i] += a * x[i];
+}
+__global__ void scal_kernel(uint64_t n, double a, double* __restrict__ x) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        x[i] *= a;
+}
+hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, n, a, x, y);
+    return hipGetLastError();
+}
+hipError_t launch_scal(hipStream_t st, uint64_t n, double a, double* x) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(scal_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, n, a, x);
+    return hipGetLastError();
+}
+
+__global__ void permute_kernel(uint64_t n, const uint64_t* __restrict__ idx, const double* __restrict__ src,
+                               double* __restrict__ dst) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+hipError_t launch_permute(hipStream_t st, uint64_t n, const uint64_t* idx, const double* src, double* dst) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(permute_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, n, idx, src, dst);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------- reductions
+// Wavefront reduction (64 lanes) followed by an LDS reduction over the 4 wavefronts.
+__device__ inline double block_sum_256(double v) {
+    __shared__ double red[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(uint64_t n, const double* __restrict__ x,
+                                                          const double* __restrict__ y, double* __restrict__ partial) {
+    double s = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) s += x[i] * y[i];
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ partial, int n,
+                                                           double* __restrict__ out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial, double* result) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(RED_BLOCKS), dim3(256), 0, st, n, x, y, partial);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(256), 0, st, partial, RED_BLOCKS, result);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void block_sums_kernel(const uint64_t* __restrict__ blk_off,
+                                                         const uint32_t* __restrict__ blk_len,
+                                                         const double* __restrict__ data, double* __restrict__ out) {
+    const uint64_t off = blk_off[blockIdx.x];
+    const uint32_t len = blk_len[blockIdx.x];
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < len; i += 256) s += data[off + i];
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+hipError_t launch_block_sums(hipStream_t st, const uint64_t* blk_off, const uint32_t* blk_len, const uint32_t*,
+                             uint32_t n_blk, const double* data, double* out_partial) {
+    if (n_blk == 0) return hipSuccess;
+    hipLaunchKernelGGL(block_sums_kernel, dim3(n_blk), dim3(256), 0, st, blk_off, blk_len, data, out_partial);
+    return hipGetLastError();
+}
+
+}  // namespace mq

@@ -1,0 +1,73 @@
+// Device task tables and kernel launchers of the sigma / boundary engine (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mq {
+
+// Operand bases: every task addresses memory as (base selector, element offset), so one
+// task table serves any psi / sigma pair handed to an apply call.
+enum Base : uint8_t {
+    BASE_WS = 0,      // plan / step workspace (T and S buffers)
+    BASE_IN = 1,      // psi (sigma) or ket (boundary steps)
+    BASE_OUT = 2,     // sigma (sigma) or new boundary data (steps)
+    BASE_LEFT = 3,    // left boundary data
+    BASE_RIGHT = 4,   // right boundary data
+    BASE_BRA = 5,     // bra data (boundary steps)
+    N_BASES = 6
+};
+
+struct Bases {
+    double* p[N_BASES];
+};
+
+// C (M x N, ldc) = beta*C + sum_seg op(A_seg) (M x K_seg) * op(B_seg) (K_seg x N)
+struct GemmTask {
+    uint64_t c_off;
+    uint32_t M, N, ldc;
+    uint32_t seg0, seg1;
+    uint32_t tiles_n;
+    uint32_t tile_begin;
+    uint8_t c_base, beta, pad0, pad1;
+};
+
+struct GemmSeg {
+    uint64_t a_off, b_off;
+    uint32_t K, lda, ldb;
+    uint8_t a_base, b_base, trans_a, trans_b;
+};
+
+// S-stage region: dst (R x C, column stride ld_dst) = sum_{t in [t0,t1)} alpha_t * src_t,
+// where src_t is R*C contiguous elements of the workspace starting at src_off[t].
+struct GatherRegion {
+    uint64_t dst_off;
+    uint32_t R, C, ld_dst;
+    uint32_t t0, t1;
+    uint32_t chunk_begin;
+};
+
+constexpr int GEMM_BM = 64;
+constexpr int GEMM_BN = 64;
+constexpr int GEMM_BK = 16;
+constexpr int GEMM_THREADS = 256;
+constexpr int GATHER_THREADS = 256;
+constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
+constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions
+
+hipError_t launch_gemm(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
+                       uint32_t n_tiles, Bases bases);
+hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n_regions,
+                         const uint64_t* src_off, const double* alpha, uint32_t n_chunks, double* ws,
+                         double* dst_base);
+hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v);
+hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial,
+                      double* result);
+hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y);
+hipError_t launch_scal(hipStream_t st, uint64_t n, double a, double* x);
+hipError_t launch_permute(hipStream_t st, uint64_t n, const uint64_t* idx, const double* src, double* dst);
+// per-b sums of boundary blocks: out[b] = sum of block b over all cohorts
+hipError_t launch_block_sums(hipStream_t st, const uint64_t* blk_off, const uint32_t* blk_len,
+                             const uint32_t* blk_b, uint32_t n_blk, const double* data, double* out_partial);
+
+}  // namespace mq

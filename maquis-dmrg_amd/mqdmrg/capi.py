@@ -1,0 +1,318 @@
+"""ctypes binding of include/mq_sigma.h (the engine's C-ABI).
+
+Structures mirror the C declarations field by field.  Numpy arrays backing a descriptor are kept
+alive by the Python objects that build the descriptor (``keep`` lists).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libmqdmrg.so")
+
+MQ_OK = 0
+MQ_NONE = 0xFFFFFFFF
+MQ_BUFFER_ALIGNMENT = 128
+SYM_U1, SYM_TWOU1, SYM_SU2U1 = 1, 2, 3
+
+
+class mq_charge(C.Structure):
+    _fields_ = [("q", C.c_int32 * 2)]
+
+
+class mq_index_desc(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("charges", C.POINTER(mq_charge)), ("sizes", C.POINTER(C.c_uint32))]
+
+
+class mq_op_desc(C.Structure):
+    _fields_ = [
+        ("spin", C.c_int32), ("n_blocks", C.c_uint32),
+        ("in_charge", C.POINTER(mq_charge)), ("out_charge", C.POINTER(mq_charge)),
+        ("entry_begin", C.POINTER(C.c_uint32)),
+        ("row", C.POINTER(C.c_uint32)), ("col", C.POINTER(C.c_uint32)),
+        ("row_spin", C.POINTER(C.c_uint8)), ("col_spin", C.POINTER(C.c_uint8)),
+        ("coeff", C.POINTER(C.c_double)),
+    ]
+
+
+class mq_mpo_desc(C.Structure):
+    _fields_ = [
+        ("row_dim", C.c_uint32), ("col_dim", C.c_uint32),
+        ("left_spin", C.POINTER(C.c_int32)), ("right_spin", C.POINTER(C.c_int32)),
+        ("left_herm", C.POINTER(C.c_uint32)), ("left_phase", C.POINTER(C.c_int32)),
+        ("right_herm", C.POINTER(C.c_uint32)), ("right_phase", C.POINTER(C.c_int32)),
+        ("n_ops", C.c_uint32), ("ops", C.POINTER(mq_op_desc)),
+        ("n_terms", C.c_uint32),
+        ("term_b1", C.POINTER(C.c_uint32)), ("term_b2", C.POINTER(C.c_uint32)),
+        ("term_op", C.POINTER(C.c_uint32)), ("term_scale", C.POINTER(C.c_double)),
+    ]
+
+
+class mq_site_desc(C.Structure):
+    _fields_ = [("sym", C.c_int32), ("left_i", mq_index_desc), ("phys_i", mq_index_desc), ("right_i", mq_index_desc)]
+
+
+class mq_bindex_desc(C.Structure):
+    _fields_ = [
+        ("bra", mq_index_desc), ("ket", mq_index_desc),
+        ("n_cohorts", C.c_uint32), ("aux_dim", C.c_uint32),
+        ("bra_pos", C.POINTER(C.c_uint32)), ("ket_pos", C.POINTER(C.c_uint32)),
+        ("tr", C.POINTER(C.c_uint8)), ("n_blocks", C.POINTER(C.c_uint32)),
+        ("offsets", C.POINTER(C.c_int64)), ("conj_scales", C.POINTER(C.c_double)),
+        ("transposes", C.POINTER(C.c_uint8)),
+    ]
+
+
+class mq_schedule_desc(C.Structure):
+    _fields_ = [
+        ("n_blocks", C.c_uint32),
+        ("t_begin", C.POINTER(C.c_uint64)),
+        ("t_mps_offset", C.POINTER(C.c_uint32)), ("t_ci", C.POINTER(C.c_uint32)),
+        ("t_ci_eff", C.POINTER(C.c_uint32)), ("t_lb_ket", C.POINTER(C.c_uint32)),
+        ("coh_begin", C.POINTER(C.c_uint64)),
+        ("coh_lb", C.POINTER(C.c_uint32)), ("coh_rb", C.POINTER(C.c_uint32)),
+        ("coh_ls", C.POINTER(C.c_uint32)), ("coh_rs", C.POINTER(C.c_uint32)),
+        ("coh_ci", C.POINTER(C.c_uint32)), ("coh_ci_eff", C.POINTER(C.c_uint32)),
+        ("coh_nSrows", C.POINTER(C.c_uint32)), ("coh_stripe", C.POINTER(C.c_uint32)),
+        ("coh_unit_begin", C.POINTER(C.c_uint64)),
+        ("unit_offset", C.POINTER(C.c_uint32)), ("unit_ms", C.POINTER(C.c_uint32)),
+        ("unit_line_begin", C.POINTER(C.c_uint64)),
+        ("line_b1", C.POINTER(C.c_uint32)), ("line_count", C.POINTER(C.c_uint32)),
+        ("unit_task_begin", C.POINTER(C.c_uint64)),
+        ("task_ti", C.POINTER(C.c_uint32)), ("task_col", C.POINTER(C.c_uint32)),
+        ("task_alpha", C.POINTER(C.c_double)),
+        ("aux_dim", C.c_uint32), ("coh_mpo_offsets", C.POINTER(C.c_int64)),
+    ]
+
+
+P = C.c_void_p
+PP = C.POINTER(C.c_void_p)
+u64p = C.POINTER(C.c_uint64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mq_last_error": (C.c_char_p, []),
+    "mq_ctx_create": (C.c_int, [C.c_int, PP]),
+    "mq_ctx_destroy": (C.c_int, [P]),
+    "mq_ctx_synchronize": (C.c_int, [P]),
+    "mq_ctx_stream": (P, [P]),
+    "mq_site_vector_layout": (C.c_int, [C.POINTER(mq_site_desc), u64p, u64p]),
+    "mq_site_left_layout": (C.c_int, [C.POINTER(mq_site_desc), u64p, u64p]),
+    "mq_reshape_right_to_left": (C.c_int, [P, C.POINTER(mq_site_desc), P, P]),
+    "mq_reshape_left_to_right": (C.c_int, [P, C.POINTER(mq_site_desc), P, P]),
+    "mq_boundary_create_trivial": (C.c_int, [P, C.POINTER(mq_index_desc), C.c_uint32, PP]),
+    "mq_boundary_create": (C.c_int, [P, C.POINTER(mq_bindex_desc), C.POINTER(C.POINTER(C.c_double)), PP]),
+    "mq_boundary_destroy": (C.c_int, [P]),
+    "mq_boundary_index": (C.c_int, [P, C.POINTER(mq_bindex_desc)]),
+    "mq_boundary_download": (C.c_int, [P, C.c_uint32, P]),
+    "mq_boundary_cohort_ptr": (C.c_int, [P, C.c_uint32, PP]),
+    "mq_boundary_traces": (C.c_int, [P, P]),
+    "mq_schedule_build_sigma": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),
+                                          C.POINTER(mq_bindex_desc), C.POINTER(mq_bindex_desc), PP]),
+    "mq_schedule_build_left": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),
+                                         C.POINTER(mq_bindex_desc), C.c_int, PP]),
+    "mq_schedule_build_right": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),
+                                          C.POINTER(mq_bindex_desc), C.c_int, PP]),
+    "mq_schedule_out_index": (C.c_int, [P, C.POINTER(mq_bindex_desc)]),
+    "mq_schedule_export": (C.c_int, [P, C.POINTER(mq_schedule_desc)]),
+    "mq_schedule_flops": (C.c_int, [P, u64p]),
+    "mq_schedule_destroy": (C.c_int, [P]),
+    "mq_sigma_plan_create": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, PP]),
+    "mq_sigma_plan_create_from_schedule": (C.c_int, [P, C.POINTER(mq_schedule_desc), C.c_uint32,
+                                                     C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), P, P, PP]),
+    "mq_sigma_apply": (C.c_int, [P, P, P]),
+    "mq_plan_vector_size": (C.c_int, [P, u64p]),
+    "mq_plan_flops": (C.c_int, [P, u64p]),
+    "mq_plan_workspace_bytes": (C.c_int, [P, u64p]),
+    "mq_plan_destroy": (C.c_int, [P]),
+    "mq_boundary_left_step": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, P, C.c_int, PP]),
+    "mq_boundary_right_step": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, P, C.c_int, PP]),
+    "mq_vec_dot": (C.c_int, [P, C.c_uint64, P, P, C.POINTER(C.c_double)]),
+    "mq_vec_nrm2": (C.c_int, [P, C.c_uint64, P, C.POINTER(C.c_double)]),
+    "mq_vec_axpy": (C.c_int, [P, C.c_uint64, C.c_double, P, P]),
+    "mq_vec_scal": (C.c_int, [P, C.c_uint64, C.c_double, P]),
+    "mq_vec_copy": (C.c_int, [P, C.c_uint64, P, P]),
+    "mq_jd_solve": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+}
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmqdmrg.so (raises if it has not been built: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError("%s is missing: build it with maquis-dmrg_amd/build_ext.py" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != MQ_OK:
+        raise EngineError("mq error %d: %s" % (rc, lib().mq_last_error().decode()))
+
+
+# ---------------------------------------------------------------------------- helpers
+def ptr(a, ctype):
+    """Pointer to a numpy array's data (None for None / empty)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def charges_array(charges):
+    arr = (mq_charge * max(1, len(charges)))()
+    for i, c in enumerate(charges):
+        arr[i].q[0] = int(c[0])
+        arr[i].q[1] = int(c[1])
+    return arr
+
+
+def index_desc(index, keep):
+    """mq_index_desc from an Index (charges: list of 2-tuples, sizes)."""
+    ch = charges_array(index.charges)
+    sz = np.ascontiguousarray(np.asarray(index.sizes, dtype=np.uint32))
+    keep += [ch, sz]
+    d = mq_index_desc()
+    d.n = len(index.charges)
+    d.charges = C.cast(ch, C.POINTER(mq_charge))
+    d.sizes = ptr(sz, C.c_uint32)
+    return d
+
+
+def site_desc(sym, left, phys, right, keep):
+    d = mq_site_desc()
+    d.sym = sym
+    d.left_i = index_desc(left, keep)
+    d.phys_i = index_desc(phys, keep)
+    d.right_i = index_desc(right, keep)
+    return d
+
+
+def read_index(d):
+    from .symmetry import Index
+    ch = [(d.charges[i].q[0], d.charges[i].q[1]) for i in range(d.n)]
+    sz = [int(d.sizes[i]) for i in range(d.n)]
+    return Index(ch, sz, sort=False)
+
+
+def _arr(p, n, dtype):
+    if n == 0 or not p:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True)
+
+
+class BIndex:
+    """Plain-numpy copy of a boundary index (mq_bindex_desc)."""
+
+    def __init__(self, bra, ket, aux_dim, bra_pos, ket_pos, tr, n_blocks, offsets, conj_scales, transposes):
+        self.bra, self.ket, self.aux_dim = bra, ket, int(aux_dim)
+        self.bra_pos = np.asarray(bra_pos, dtype=np.uint32)
+        self.ket_pos = np.asarray(ket_pos, dtype=np.uint32)
+        self.tr = np.asarray(tr, dtype=np.uint8)
+        self.n_blocks = np.asarray(n_blocks, dtype=np.uint32)
+        self.offsets = np.asarray(offsets, dtype=np.int64).reshape(-1, self.aux_dim)
+        self.conj_scales = np.asarray(conj_scales, dtype=np.float64).reshape(-1, self.aux_dim)
+        self.transposes = np.asarray(transposes, dtype=np.uint8).reshape(-1, self.aux_dim)
+
+    @property
+    def n_cohorts(self):
+        return len(self.bra_pos)
+
+    def cohort_size(self, ci):
+        return int(self.n_blocks[ci]) * self.bra.sizes[self.bra_pos[ci]] * self.ket.sizes[self.ket_pos[ci]]
+
+    def cohort_index(self, lc, rc):
+        lb, rb = self.bra.position(lc), self.ket.position(rc)
+        for ci in range(self.n_cohorts):
+            if self.bra_pos[ci] == lb and self.ket_pos[ci] == rb:
+                return ci
+        return self.n_cohorts
+
+    @staticmethod
+    def from_desc(d):
+        n, a = d.n_cohorts, d.aux_dim
+        return BIndex(read_index(d.bra), read_index(d.ket), a,
+                      _arr(d.bra_pos, n, np.uint32), _arr(d.ket_pos, n, np.uint32), _arr(d.tr, n, np.uint8),
+                      _arr(d.n_blocks, n, np.uint32), _arr(d.offsets, n * a, np.int64),
+                      _arr(d.conj_scales, n * a, np.float64),
+                      _arr(d.transposes, n * a, np.uint8) if d.transposes else np.zeros(n * a, np.uint8))
+
+    def desc(self, keep):
+        d = mq_bindex_desc()
+        d.bra = index_desc(self.bra, keep)
+        d.ket = index_desc(self.ket, keep)
+        d.n_cohorts = self.n_cohorts
+        d.aux_dim = self.aux_dim
+        arrs = [np.ascontiguousarray(x) for x in (self.bra_pos, self.ket_pos, self.tr, self.n_blocks,
+                                                   self.offsets.ravel(), self.conj_scales.ravel(),
+                                                   self.transposes.ravel())]
+        keep += arrs
+        d.bra_pos = ptr(arrs[0], C.c_uint32)
+        d.ket_pos = ptr(arrs[1], C.c_uint32)
+        d.tr = ptr(arrs[2], C.c_uint8)
+        d.n_blocks = ptr(arrs[3], C.c_uint32)
+        d.offsets = ptr(arrs[4], C.c_int64)
+        d.conj_scales = ptr(arrs[5], C.c_double)
+        d.transposes = ptr(arrs[6], C.c_uint8)
+        return d
+
+
+SCHEDULE_FIELDS = [
+    ("t_begin", "blocks+1", np.uint64), ("t_mps_offset", "t", np.uint32), ("t_ci", "t", np.uint32),
+    ("t_ci_eff", "t", np.uint32), ("t_lb_ket", "t", np.uint32), ("coh_begin", "blocks+1", np.uint64),
+    ("coh_lb", "c", np.uint32), ("coh_rb", "c", np.uint32), ("coh_ls", "c", np.uint32), ("coh_rs", "c", np.uint32),
+    ("coh_ci", "c", np.uint32), ("coh_ci_eff", "c", np.uint32), ("coh_nSrows", "c", np.uint32),
+    ("coh_stripe", "c", np.uint32), ("coh_unit_begin", "c+1", np.uint64), ("unit_offset", "u", np.uint32),
+    ("unit_ms", "u", np.uint32), ("unit_line_begin", "u+1", np.uint64), ("line_b1", "l", np.uint32),
+    ("line_count", "l", np.uint32), ("unit_task_begin", "u+1", np.uint64), ("task_ti", "k", np.uint32),
+    ("task_col", "k", np.uint32), ("task_alpha", "k", np.float64),
+]
+
+
+def schedule_to_dict(d):
+    """Copy a mq_schedule_desc into a dict of numpy arrays."""
+    nb = d.n_blocks
+    out = {"n_blocks": nb}
+    tb = _arr(d.t_begin, nb + 1, np.uint64)
+    cb = _arr(d.coh_begin, nb + 1, np.uint64)
+    nt, nc = int(tb[-1]), int(cb[-1])
+    ub = _arr(d.coh_unit_begin, nc + 1, np.uint64)
+    nu = int(ub[-1])
+    lb = _arr(d.unit_line_begin, nu + 1, np.uint64)
+    kb = _arr(d.unit_task_begin, nu + 1, np.uint64)
+    sizes = {"blocks+1": nb + 1, "t": nt, "c": nc, "c+1": nc + 1, "u": nu, "u+1": nu + 1, "l": int(lb[-1]),
+             "k": int(kb[-1])}
+    for name, kind, dt in SCHEDULE_FIELDS:
+        out[name] = _arr(getattr(d, name), sizes[kind], dt)
+    out["aux_dim"] = d.aux_dim
+    if d.coh_mpo_offsets:
+        out["coh_mpo_offsets"] = _arr(d.coh_mpo_offsets, nc * d.aux_dim, np.int64)
+    return out
+
+
+def schedule_from_dict(s, keep):
+    d = mq_schedule_desc()
+    d.n_blocks = s["n_blocks"]
+    for name, kind, dt in SCHEDULE_FIELDS:
+        a = np.ascontiguousarray(s[name], dtype=dt)
+        keep.append(a)
+        ct = {np.uint64: C.c_uint64, np.uint32: C.c_uint32, np.float64: C.c_double}[dt]
+        setattr(d, name, ptr(a, ct))
+    d.aux_dim = s.get("aux_dim", 0)
+    if "coh_mpo_offsets" in s:
+        a = np.ascontiguousarray(s["coh_mpo_offsets"], dtype=np.int64)
+        keep.append(a)
+        d.coh_mpo_offsets = ptr(a, C.c_int64)
+    return d

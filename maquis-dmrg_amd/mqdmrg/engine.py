@@ -1,0 +1,301 @@
+"""Python handles over the engine's C-ABI: context, device boundaries, sigma plans, boundary
+steps and the Jacobi-Davidson solver.  Device memory of site tensors is held in torch tensors on
+the engine's own stream (torch is plumbing here: allocation and small dense algebra)."""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import capi
+from .capi import lib, check
+from .models import Op
+from .symmetry import Index
+
+
+class Engine:
+    def __init__(self, device=0):
+        self.device = device
+        self.ctx = C.c_void_p()
+        check(lib().mq_ctx_create(device, C.byref(self.ctx)))
+        self.stream = torch.cuda.ExternalStream(lib().mq_ctx_stream(self.ctx), device="cuda:%d" % device)
+        self.torch_device = torch.device("cuda:%d" % device)
+        # every torch op of the caller is ordered with the engine's launches on one stream
+        torch.cuda.set_stream(self.stream)
+
+    def sync(self):
+        check(lib().mq_ctx_synchronize(self.ctx))
+
+    def close(self):
+        if self.ctx:
+            lib().mq_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # vectors ------------------------------------------------------------------
+    def dot(self, x, y):
+        r = C.c_double()
+        check(lib().mq_vec_dot(self.ctx, x.numel(), x.data_ptr(), y.data_ptr(), C.byref(r)))
+        return r.value
+
+    def nrm2(self, x):
+        r = C.c_double()
+        check(lib().mq_vec_nrm2(self.ctx, x.numel(), x.data_ptr(), C.byref(r)))
+        return r.value
+
+    def axpy(self, a, x, y):
+        check(lib().mq_vec_axpy(self.ctx, x.numel(), float(a), x.data_ptr(), y.data_ptr()))
+
+    def scal(self, a, x):
+        check(lib().mq_vec_scal(self.ctx, x.numel(), float(a), x.data_ptr()))
+
+
+# ---------------------------------------------------------------------------- descriptors
+class MpoDesc:
+    """mq_mpo_desc of an MPOTensor (arrays kept alive by this object)."""
+
+    def __init__(self, W):
+        self.keep = []
+        ops, op_id = [], {}
+        b1s, b2s, tops, scales = [], [], [], []
+        for (b1, b2) in sorted(W.terms):
+            for op, sc in W.terms[(b1, b2)]:
+                if id(op) not in op_id:
+                    op_id[id(op)] = len(ops)
+                    ops.append(op)
+                b1s.append(b1)
+                b2s.append(b2)
+                tops.append(op_id[id(op)])
+                scales.append(sc)
+        opd = (capi.mq_op_desc * max(1, len(ops)))()
+        for k, op in enumerate(ops):
+            keys = list(op.blocks.keys())
+            cin = capi.charges_array([k_[0] for k_ in keys])
+            cout = capi.charges_array([k_[1] for k_ in keys])
+            eb = np.zeros(len(keys) + 1, dtype=np.uint32)
+            rows, cols, rsp, csp, coef = [], [], [], [], []
+            for i, key in enumerate(keys):
+                ents = op.blocks[key]
+                eb[i + 1] = eb[i] + len(ents)
+                for r, c, v, rs, cs in ents:
+                    rows.append(r)
+                    cols.append(c)
+                    coef.append(v)
+                    rsp.append(rs)
+                    csp.append(cs)
+            arrs = [np.asarray(rows, np.uint32), np.asarray(cols, np.uint32), np.asarray(rsp, np.uint8),
+                    np.asarray(csp, np.uint8), np.asarray(coef, np.float64)]
+            self.keep += [cin, cout, eb] + arrs
+            d = opd[k]
+            d.spin = op.spin
+            d.n_blocks = len(keys)
+            d.in_charge = C.cast(cin, C.POINTER(capi.mq_charge))
+            d.out_charge = C.cast(cout, C.POINTER(capi.mq_charge))
+            d.entry_begin = capi.ptr(eb, C.c_uint32)
+            d.row = capi.ptr(arrs[0], C.c_uint32)
+            d.col = capi.ptr(arrs[1], C.c_uint32)
+            d.row_spin = capi.ptr(arrs[2], C.c_uint8)
+            d.col_spin = capi.ptr(arrs[3], C.c_uint8)
+            d.coeff = capi.ptr(arrs[4], C.c_double)
+        self.keep.append(opd)
+        herm = lambda h: np.asarray([capi.MQ_NONE if x is None else x for x in h], np.uint32)
+        a = [np.asarray(W.left_spin, np.int32), np.asarray(W.right_spin, np.int32), herm(W.left_herm),
+             np.asarray(W.left_phase, np.int32), herm(W.right_herm), np.asarray(W.right_phase, np.int32),
+             np.asarray(b1s, np.uint32), np.asarray(b2s, np.uint32), np.asarray(tops, np.uint32),
+             np.asarray(scales, np.float64)]
+        self.keep += a
+        d = capi.mq_mpo_desc()
+        d.row_dim, d.col_dim = W.row_dim, W.col_dim
+        d.left_spin = capi.ptr(a[0], C.c_int32)
+        d.right_spin = capi.ptr(a[1], C.c_int32)
+        d.left_herm = capi.ptr(a[2], C.c_uint32)
+        d.left_phase = capi.ptr(a[3], C.c_int32)
+        d.right_herm = capi.ptr(a[4], C.c_uint32)
+        d.right_phase = capi.ptr(a[5], C.c_int32)
+        d.n_ops = len(ops)
+        d.ops = C.cast(opd, C.POINTER(capi.mq_op_desc))
+        d.n_terms = len(b1s)
+        d.term_b1 = capi.ptr(a[6], C.c_uint32)
+        d.term_b2 = capi.ptr(a[7], C.c_uint32)
+        d.term_op = capi.ptr(a[8], C.c_uint32)
+        d.term_scale = capi.ptr(a[9], C.c_double)
+        self.desc = d
+
+
+class SiteDesc:
+    def __init__(self, sym, left, phys, right):
+        self.keep = []
+        self.desc = capi.site_desc(sym, left, phys, right, self.keep)
+
+
+# ---------------------------------------------------------------------------- boundaries
+class DeviceBoundary:
+    def __init__(self, engine, handle):
+        self.engine = engine
+        self.h = handle
+        d = capi.mq_bindex_desc()
+        check(lib().mq_boundary_index(self.h, C.byref(d)))
+        self.index = capi.BIndex.from_desc(d)
+
+    @staticmethod
+    def trivial(engine, index, aux_dim):
+        keep = []
+        d = capi.index_desc(index, keep)
+        h = C.c_void_p()
+        check(lib().mq_boundary_create_trivial(engine.ctx, C.byref(d), aux_dim, C.byref(h)))
+        return DeviceBoundary(engine, h)
+
+    @staticmethod
+    def upload(engine, bindex, cohorts):
+        keep = []
+        d = bindex.desc(keep)
+        arrs = [None if c is None else np.ascontiguousarray(c, dtype=np.float64) for c in cohorts]
+        ptrs = (C.POINTER(C.c_double) * max(1, len(arrs)))()
+        for i, a in enumerate(arrs):
+            ptrs[i] = capi.ptr(a, C.c_double) if a is not None and a.size else None
+        h = C.c_void_p()
+        check(lib().mq_boundary_create(engine.ctx, C.byref(d), ptrs, C.byref(h)))
+        return DeviceBoundary(engine, h)
+
+    def download(self):
+        out = []
+        for ci in range(self.index.n_cohorts):
+            n = self.index.cohort_size(ci)
+            a = np.zeros(n, dtype=np.float64)
+            if n:
+                check(lib().mq_boundary_download(self.h, ci, a.ctypes.data))
+            out.append(a)
+        return out
+
+    def traces(self):
+        a = np.zeros(self.index.aux_dim, dtype=np.float64)
+        check(lib().mq_boundary_traces(self.h, a.ctypes.data))
+        return a
+
+    def close(self):
+        if self.h:
+            lib().mq_boundary_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------- schedules
+def build_schedule(kind, site, mpo_desc, bindex_l=None, bindex_r=None, symmetric=True):
+    """Host-only schedule builders; returns (dict of arrays, flops, out_index or None)."""
+    keep = []
+    h = C.c_void_p()
+    if kind == "sigma":
+        check(lib().mq_schedule_build_sigma(C.byref(site.desc), C.byref(mpo_desc.desc),
+                                            C.byref(bindex_l.desc(keep)), C.byref(bindex_r.desc(keep)), C.byref(h)))
+    elif kind == "left":
+        check(lib().mq_schedule_build_left(C.byref(site.desc), C.byref(mpo_desc.desc),
+                                           C.byref(bindex_l.desc(keep)), int(symmetric), C.byref(h)))
+    else:
+        check(lib().mq_schedule_build_right(C.byref(site.desc), C.byref(mpo_desc.desc),
+                                            C.byref(bindex_r.desc(keep)), int(symmetric), C.byref(h)))
+    try:
+        d = capi.mq_schedule_desc()
+        check(lib().mq_schedule_export(h, C.byref(d)))
+        s = capi.schedule_to_dict(d)
+        f = C.c_uint64()
+        check(lib().mq_schedule_flops(h, C.byref(f)))
+        out_idx = None
+        if kind != "sigma":
+            bd = capi.mq_bindex_desc()
+            check(lib().mq_schedule_out_index(h, C.byref(bd)))
+            out_idx = capi.BIndex.from_desc(bd)
+        return s, f.value, out_idx
+    finally:
+        lib().mq_schedule_destroy(h)
+
+
+# ---------------------------------------------------------------------------- sigma
+class SigmaPlan:
+    def __init__(self, engine, site, mpo_desc, left, right):
+        self.engine = engine
+        self.left, self.right = left, right   # keep boundaries alive
+        self.h = C.c_void_p()
+        check(lib().mq_sigma_plan_create(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), left.h, right.h,
+                                         C.byref(self.h)))
+        n = C.c_uint64()
+        check(lib().mq_plan_vector_size(self.h, C.byref(n)))
+        self.n = n.value
+        f = C.c_uint64()
+        check(lib().mq_plan_flops(self.h, C.byref(f)))
+        self.flops = f.value
+        b = C.c_uint64()
+        check(lib().mq_plan_workspace_bytes(self.h, C.byref(b)))
+        self.workspace_bytes = b.value
+
+    @staticmethod
+    def from_schedule(engine, sched, ket_rows, ket_cols, left, right):
+        self = SigmaPlan.__new__(SigmaPlan)
+        self.engine, self.left, self.right = engine, left, right
+        keep = []
+        d = capi.schedule_from_dict(sched, keep)
+        rows = np.ascontiguousarray(ket_rows, np.uint32)
+        cols = np.ascontiguousarray(ket_cols, np.uint32)
+        self.h = C.c_void_p()
+        check(lib().mq_sigma_plan_create_from_schedule(engine.ctx, C.byref(d), len(rows), capi.ptr(rows, C.c_uint32),
+                                                       capi.ptr(cols, C.c_uint32), left.h, right.h, C.byref(self.h)))
+        n = C.c_uint64()
+        check(lib().mq_plan_vector_size(self.h, C.byref(n)))
+        self.n = n.value
+        f = C.c_uint64()
+        check(lib().mq_plan_flops(self.h, C.byref(f)))
+        self.flops = f.value
+        self.workspace_bytes = 0
+        return self
+
+    def apply(self, psi, sigma):
+        assert psi.numel() >= self.n and sigma.numel() >= self.n
+        check(lib().mq_sigma_apply(self.h, psi.data_ptr(), sigma.data_ptr()))
+
+    def jd(self, init, out, tol=1e-8, max_iter=8):
+        e = C.c_double()
+        it = C.c_int()
+        check(lib().mq_jd_solve(self.h, init.data_ptr(), out.data_ptr(), float(tol), int(max_iter), 0, C.byref(e),
+                                C.byref(it)))
+        return e.value, it.value
+
+    def close(self):
+        if self.h:
+            lib().mq_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def left_step(engine, site, mpo_desc, bra_lp, ket_rp, left, symmetric=True):
+    h = C.c_void_p()
+    check(lib().mq_boundary_left_step(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_lp.data_ptr(),
+                                      ket_rp.data_ptr(), left.h, int(symmetric), C.byref(h)))
+    return DeviceBoundary(engine, h)
+
+
+def right_step(engine, site, mpo_desc, bra_rp, ket_rp, right, symmetric=True):
+    h = C.c_void_p()
+    check(lib().mq_boundary_right_step(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_rp.data_ptr(),
+                                       ket_rp.data_ptr(), right.h, int(symmetric), C.byref(h)))
+    return DeviceBoundary(engine, h)
+
+
+def reshape_right_to_left(engine, site, src, dst):
+    check(lib().mq_reshape_right_to_left(engine.ctx, C.byref(site.desc), src.data_ptr(), dst.data_ptr()))
+
+
+def reshape_left_to_right(engine, site, src, dst):
+    check(lib().mq_reshape_left_to_right(engine.ctx, C.byref(site.desc), src.data_ptr(), dst.data_ptr()))

@@ -1,0 +1,121 @@
+"""MPS bond indices, random initial states and host-side layout conversions.
+
+allowed_sectors follows the reference's mps_sectors.h:57-150 (left/right reachable charges, each
+sector capped at Mmax); random_bond_dims gives a synthetic bond index of total size ~m whose
+sector sizes follow the multiplicity of each charge (used to build benchmark shapes)."""
+import math
+
+import numpy as np
+
+from .symmetry import Index, SiteLayout, fuse, sub
+
+
+def _fuse_index(a, b, sign=+1, cap=None):
+    acc = {}
+    for ca, sa in zip(a.charges, a.sizes):
+        for cb, sb in zip(b.charges, b.sizes):
+            c = fuse(ca, cb) if sign > 0 else sub(ca, cb)
+            acc[c] = acc.get(c, 0) + sa * sb
+    if cap is not None:
+        acc = {c: min(cap, s) for c, s in acc.items()}
+    return acc
+
+
+def allowed_sectors(phys_list, right_end, Mmax, left_end=(0, 0)):
+    """Bond indices 0..L of a chain with per-site physical indices phys_list (mps_sectors.h:57-150)."""
+    L = len(phys_list)
+    left = [None] * (L + 1)
+    right = [None] * (L + 1)
+    left[0] = {tuple(left_end): 1}
+    for i in range(L):
+        left[i + 1] = _fuse_index(Index.from_dict(left[i]), phys_list[i], +1, cap=None)
+        # cap counts to avoid overflow while keeping exact "reachable" information
+        left[i + 1] = {c: min(s, 10 ** 12) for c, s in left[i + 1].items()}
+    right[L] = {tuple(right_end): 1}
+    for i in range(L - 1, -1, -1):
+        right[i] = _fuse_index(Index.from_dict(right[i + 1]), phys_list[i], -1, cap=None)
+        right[i] = {c: min(s, 10 ** 12) for c, s in right[i].items()}
+    out = []
+    for i in range(L + 1):
+        common = {c: min(Mmax, left[i][c], right[i][c]) for c in left[i] if c in right[i]}
+        out.append(Index.from_dict(common))
+    return out, left, right
+
+
+def random_bond_dims(phys_list, right_end, m, left_end=(0, 0)):
+    """Synthetic bond indices of total dimension ~m: sector sizes proportional to the square root
+    of the number of states reachable from both ends, capped by what both ends allow."""
+    _, left, right = allowed_sectors(phys_list, right_end, 1, left_end)
+    L = len(phys_list)
+    out = []
+    for i in range(L + 1):
+        common = {c: min(left[i][c], right[i][c]) for c in left[i] if c in right[i]}
+        w = {c: math.sqrt(float(left[i][c]) * float(right[i][c])) for c in common}
+        tot = sum(w.values())
+        dims = {}
+        for c, wc in w.items():
+            d = int(round(m * wc / tot))
+            d = min(d, common[c])
+            if d > 0:
+                dims[c] = d
+        if not dims:
+            c = max(w, key=w.get)
+            dims[c] = 1
+        out.append(Index.from_dict(dims))
+    return out
+
+
+def random_site_vector(layout, rng):
+    """Random N(0,1) entries in every block of the right-paired padded layout (padding zero)."""
+    v = np.zeros(layout.rp_total)
+    for lb, ls in enumerate(layout.left.sizes):
+        n = ls * layout.rp_cols[lb]
+        v[layout.rp_offset[lb]:layout.rp_offset[lb] + n] = rng.standard_normal(n)
+    return v
+
+
+def rp_to_lp(layout, vec):
+    """Host reshape right-paired -> left-paired (reshapes.h:197-242 inverse direction)."""
+    out = np.zeros(layout.lp_total)
+    L_, P_, R_ = layout.left, layout.phys, layout.right
+    for lb, (lc, ls) in enumerate(zip(L_.charges, L_.sizes)):
+        cols = layout.rp_cols[lb]
+        blk = vec[layout.rp_offset[lb]:layout.rp_offset[lb] + ls * cols].reshape(cols, ls).T
+        for s, (pc, d) in enumerate(zip(P_.charges, P_.sizes)):
+            rc = fuse(lc, pc)
+            rb = R_.position(rc)
+            if rb == len(R_):
+                continue
+            rs = R_.sizes[rb]
+            rows = layout.lp_rows[rb]
+            lpblk = out[layout.lp_offset[rb]:layout.lp_offset[rb] + rows * rs].reshape(rs, rows).T
+            c0 = layout.rpb(pc, rc)
+            r0 = layout.lpb(pc, lc)
+            for ss in range(d):
+                lpblk[r0 + ss * ls:r0 + (ss + 1) * ls, :] = blk[:, c0 + ss * rs:c0 + (ss + 1) * rs]
+    return out
+
+
+def lp_to_rp(layout, vec):
+    out = np.zeros(layout.rp_total)
+    L_, P_, R_ = layout.left, layout.phys, layout.right
+    for lb, (lc, ls) in enumerate(zip(L_.charges, L_.sizes)):
+        cols = layout.rp_cols[lb]
+        blk = out[layout.rp_offset[lb]:layout.rp_offset[lb] + ls * cols].reshape(cols, ls).T
+        for s, (pc, d) in enumerate(zip(P_.charges, P_.sizes)):
+            rc = fuse(lc, pc)
+            rb = R_.position(rc)
+            if rb == len(R_):
+                continue
+            rs = R_.sizes[rb]
+            rows = layout.lp_rows[rb]
+            lpblk = vec[layout.lp_offset[rb]:layout.lp_offset[rb] + rows * rs].reshape(rs, rows).T
+            c0 = layout.rpb(pc, rc)
+            r0 = layout.lpb(pc, lc)
+            for ss in range(d):
+                blk[:, c0 + ss * rs:c0 + (ss + 1) * rs] = lpblk[r0 + ss * ls:r0 + (ss + 1) * ls, :]
+    return out
+
+
+def layouts_for_chain(bonds, phys_list):
+    return [SiteLayout(bonds[i], phys_list[i], bonds[i + 1]) for i in range(len(phys_list))]

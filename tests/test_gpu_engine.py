@@ -1,0 +1,160 @@
+"""GPU parity of the HIP engine against the CPU oracle (called through the C-ABI).
+
+Tolerances: sigma and boundary steps are FP64 and must agree to 1e-12 relative (the bar the
+engine is specified to); Jacobi-Davidson energies to 1e-10 absolute."""
+import numpy as np
+import pytest
+import torch
+
+from chain import SiteProblem
+from oracle import oracle as O
+from mqdmrg import engine as E, mps
+
+pytestmark = pytest.mark.gpu
+
+SIGMA_RTOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = E.Engine(0)
+    yield e
+    e.close()
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+
+
+def upload(eng, hb):
+    return E.DeviceBoundary.upload(eng, hb.index, hb.cohorts)
+
+
+CASES = [
+    ("heisenberg", 8, 3, 6, 1),
+    ("heisenberg", 10, 0, 8, 2),     # left edge (trivial left boundary)
+    ("heisenberg", 10, 8, 8, 3),     # right edge
+    ("hubbard", 6, 2, 6, 4),
+    ("hubbard", 8, 3, 10, 5),
+]
+
+
+@pytest.mark.parametrize("model,L,site,M,seed", CASES)
+def test_sigma_matches_oracle(eng, model, L, site, M, seed):
+    print("case", model, L, site, M, seed, flush=True)
+    p = SiteProblem(model, L=L, site=site, Mmax=M, seed=seed)
+    ref = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb).apply(p.psi)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb)
+    assert plan.n == p.layout.rp_total
+    psi = dev(p.psi)
+    out = torch.full_like(psi, np.nan)
+    plan.apply(psi, out)
+    eng.sync()
+    got = out.cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err <= SIGMA_RTOL, err
+    # run twice: bitwise deterministic (no atomics)
+    out2 = torch.zeros_like(psi)
+    plan.apply(psi, out2)
+    eng.sync()
+    assert torch.equal(out, out2)
+
+
+def test_sigma_plan_from_caller_schedule(eng):
+    """Drop-in seam: the caller hands its own ScheduleNew (here the oracle's) to the engine."""
+    p = SiteProblem("hubbard", L=6, site=2, Mmax=6, seed=7)
+    ref = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb).apply(p.psi)
+    sched, _ = O.schedule_sigma(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    rows = np.asarray(p.layout.left.sizes, np.uint32)
+    cols = np.asarray(p.layout.rp_cols, np.uint32)
+    plan = E.SigmaPlan.from_schedule(eng, sched, rows, cols, Lb, Rb)
+    psi = dev(p.psi)
+    out = torch.zeros_like(psi)
+    plan.apply(psi, out)
+    eng.sync()
+    err = np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref)
+    assert err <= SIGMA_RTOL, err
+
+
+def _cmp_boundaries(hb, db):
+    assert db.index.n_cohorts == hb.index.n_cohorts
+    np.testing.assert_array_equal(db.index.offsets, hb.index.offsets)
+    np.testing.assert_array_equal(db.index.tr, hb.index.tr)
+    np.testing.assert_allclose(db.index.conj_scales, hb.index.conj_scales, rtol=0, atol=0)
+    got = db.download()
+    num = sum(float(np.sum((g - h) ** 2)) for g, h in zip(got, hb.cohorts))
+    den = sum(float(np.sum(h ** 2)) for h in hb.cohorts)
+    assert np.sqrt(num / max(den, 1e-300)) <= SIGMA_RTOL
+
+
+@pytest.mark.parametrize("model,symmetric", [("heisenberg", True), ("heisenberg", False), ("hubbard", True)])
+def test_left_step_matches_oracle(eng, model, symmetric):
+    p = SiteProblem(model, L=8, site=4, Mmax=8, seed=11)
+    k = 4
+    lay = p.layouts[k]
+    sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+    md = E.MpoDesc(p.mpo[k])
+    bra_lp = mps.rp_to_lp(lay, p.vecs[k])
+    ref = O.left_step(sd, md, bra_lp, p.vecs[k], p.Lb, symmetric)
+    Lb = upload(eng, p.Lb)
+    got = E.left_step(eng, sd, md, dev(bra_lp), dev(p.vecs[k]), Lb, symmetric)
+    _cmp_boundaries(ref, got)
+
+
+@pytest.mark.parametrize("model", ["heisenberg", "hubbard"])
+def test_right_step_matches_oracle(eng, model):
+    p = SiteProblem(model, L=8, site=3, Mmax=8, seed=12)
+    k = 4   # p.Rb is the boundary of sites >= 5
+    lay = p.layouts[k]
+    sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+    md = E.MpoDesc(p.mpo[k])
+    ref = O.right_step(sd, md, p.vecs[k], p.vecs[k], p.Rb, True)
+    Rb = upload(eng, p.Rb)
+    v = dev(p.vecs[k])
+    got = E.right_step(eng, sd, md, v, v, Rb, True)
+    _cmp_boundaries(ref, got)
+
+
+def test_reshape_roundtrip(eng):
+    p = SiteProblem("hubbard", L=6, site=2, Mmax=6, seed=13)
+    lay = p.layouts[2]
+    sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+    rp = dev(p.vecs[2])
+    lp = torch.zeros(lay.lp_total, dtype=torch.float64, device="cuda:0")
+    E.reshape_right_to_left(eng, sd, rp, lp)
+    np.testing.assert_array_equal(lp.cpu().numpy(), mps.rp_to_lp(lay, p.vecs[2]))
+    back = torch.zeros_like(rp)
+    E.reshape_left_to_right(eng, sd, lp, back)
+    np.testing.assert_array_equal(back.cpu().numpy(), p.vecs[2])
+
+
+def test_vector_kernels(eng):
+    rng = np.random.default_rng(5)
+    for n in [1, 127, 128, 100003, 3 * 2 ** 20 + 5]:
+        x, y = rng.standard_normal(n), rng.standard_normal(n)
+        dx, dy = dev(x), dev(y)
+        assert abs(eng.dot(dx, dy) - x @ y) <= 1e-12 * max(1.0, abs(x) @ abs(y))
+        assert abs(eng.nrm2(dx) - np.linalg.norm(x)) <= 1e-12 * np.linalg.norm(x)
+        eng.axpy(0.5, dx, dy)
+        eng.scal(-2.0, dx)
+        eng.sync()
+        np.testing.assert_allclose(dy.cpu().numpy(), y + 0.5 * x, rtol=0, atol=1e-14)
+        np.testing.assert_array_equal(dx.cpu().numpy(), -2.0 * x)
+
+
+@pytest.mark.parametrize("model", ["heisenberg", "hubbard"])
+def test_jacobi_davidson_matches_oracle(eng, model):
+    p = SiteProblem(model, L=8, site=3, Mmax=6, seed=21)
+    so = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    e_ref, v_ref, it_ref = so.jd(p.psi, tol=1e-10, max_iter=8)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb)
+    psi = dev(p.psi)
+    out = torch.zeros_like(psi)
+    e, it = plan.jd(psi, out, tol=1e-10, max_iter=8)
+    assert it == it_ref
+    assert abs(e - e_ref) <= 1e-10 * max(1.0, abs(e_ref))
+    v = out.cpu().numpy()
+    assert min(np.linalg.norm(v - v_ref), np.linalg.norm(v + v_ref)) <= 1e-8
