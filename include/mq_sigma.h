@@ -233,6 +233,13 @@ int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sche
                                        const mq_boundary* right, mq_plan** out);
 /* sigma = H_eff * psi; psi and sigma are device pointers in the padded Davidson-vector layout. */
 int mq_sigma_apply(mq_plan* plan, const double* psi, double* sigma);
+/* Measurement: runs `reps` sigma applications stage by stage with HIP events on the context's
+ * stream and returns the average milliseconds of each stage: [0] S zero-fill, [1] T-stage GEMM,
+ * [2] S gather, [3] final GEMM.  flops_out (may be NULL) receives the algorithmic flops of the
+ * three compute stages (T GEMM, gather, final GEMM) and gather_bytes (may be NULL) the
+ * algorithmic bytes of the gather (T slab reads + S writes). */
+int mq_plan_profile(mq_plan* plan, const double* psi, double* sigma, int reps, double* stage_ms,
+                    uint64_t* flops_out, uint64_t* gather_bytes);
 int mq_plan_vector_size(const mq_plan* plan, uint64_t* n_elements);
 int mq_plan_flops(const mq_plan* plan, uint64_t* flops);
 /* Bytes of device workspace held by the plan (T and S buffers and tables). */

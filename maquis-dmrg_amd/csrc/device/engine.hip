@@ -780,6 +780,69 @@ int mq_sigma_apply(mq_plan* p, const double* psi, double* sigma) {
     });
 }
 
+int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, double* stage_ms, uint64_t* flops_out,
+                    uint64_t* gather_bytes) {
+    return guarded([&] {
+        MQ_REQUIRE(p && psi && sigma && stage_ms && reps > 0, "bad argument");
+        MQ_REQUIRE(psi != sigma, "psi and sigma must not alias");
+        const StagePlan& P = p->st;
+        hipStream_t st = p->ctx->stream;
+        Bases b{};
+        b.p[BASE_WS] = p->ws.as<double>();
+        b.p[BASE_IN] = const_cast<double*>(psi);
+        b.p[BASE_OUT] = sigma;
+        b.p[BASE_LEFT] = p->left->ptr();
+        b.p[BASE_RIGHT] = p->right->ptr();
+        hipEvent_t ev[5];
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        double acc[4] = {0, 0, 0, 0};
+        for (int r = 0; r < reps; ++r) {
+            HIPCHK(hipEventRecord(ev[0], st));
+            HIPCHK(launch_fill(st, sigma, p->vec_size, 0.0));
+            if (P.s_elems) HIPCHK(launch_fill(st, b.p[BASE_WS] + P.s_begin, P.s_elems, 0.0));
+            HIPCHK(hipEventRecord(ev[1], st));
+            HIPCHK(launch_gemm(st, P.d_t_tasks.as<GemmTask>(), uint32_t(P.t_tasks.size()), P.d_t_segs.as<GemmSeg>(),
+                               P.t_tiles, b));
+            HIPCHK(hipEventRecord(ev[2], st));
+            HIPCHK(launch_gather(st, P.d_regions.as<GatherRegion>(), uint32_t(P.regions.size()),
+                                 P.d_src_off.as<uint64_t>(), P.d_alpha.as<double>(), P.g_chunks, b.p[BASE_WS],
+                                 b.p[BASE_WS]));
+            HIPCHK(hipEventRecord(ev[3], st));
+            HIPCHK(launch_gemm(st, P.d_f_tasks.as<GemmTask>(), uint32_t(P.f_tasks.size()), P.d_f_segs.as<GemmSeg>(),
+                               P.f_tiles, b));
+            HIPCHK(hipEventRecord(ev[4], st));
+            HIPCHK(hipEventSynchronize(ev[4]));
+            for (int k = 0; k < 4; ++k) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+                acc[k] += ms;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < 4; ++k) stage_ms[k] = acc[k] / reps;
+        if (flops_out) {
+            uint64_t ft = 0, fg = 0, ff = 0;
+            auto gemm_flops = [](const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs) {
+                uint64_t f = 0;
+                for (const auto& t : tasks)
+                    for (uint32_t s = t.seg0; s < t.seg1; ++s) f += 2ull * t.M * t.N * segs[s].K;
+                return f;
+            };
+            ft = gemm_flops(P.t_tasks, P.t_segs);
+            ff = gemm_flops(P.f_tasks, P.f_segs);
+            for (const auto& r : P.regions) fg += 2ull * r.R * r.C * (r.t1 - r.t0);
+            flops_out[0] = ft;
+            flops_out[1] = fg;
+            flops_out[2] = ff;
+        }
+        if (gather_bytes) {
+            uint64_t by = 0;
+            for (const auto& r : P.regions) by += 8ull * r.R * r.C * (uint64_t(r.t1 - r.t0) + 1);
+            *gather_bytes = by;
+        }
+    });
+}
+
 int mq_plan_vector_size(const mq_plan* p, uint64_t* n) {
     return guarded([&] {
         MQ_REQUIRE(p && n, "bad argument");

@@ -122,6 +122,7 @@ SIGNATURES = {
     "mq_sigma_plan_create_from_schedule": (C.c_int, [P, C.POINTER(mq_schedule_desc), C.c_uint32,
                                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), P, P, PP]),
     "mq_sigma_apply": (C.c_int, [P, P, P]),
+    "mq_plan_profile": (C.c_int, [P, P, P, C.c_int, C.POINTER(C.c_double), u64p, u64p]),
     "mq_plan_vector_size": (C.c_int, [P, u64p]),
     "mq_plan_flops": (C.c_int, [P, u64p]),
     "mq_plan_workspace_bytes": (C.c_int, [P, u64p]),

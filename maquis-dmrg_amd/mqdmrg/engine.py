@@ -260,6 +260,15 @@ class SigmaPlan:
         assert psi.numel() >= self.n and sigma.numel() >= self.n
         check(lib().mq_sigma_apply(self.h, psi.data_ptr(), sigma.data_ptr()))
 
+    def profile(self, psi, sigma, reps=5):
+        """Average ms per stage [fill, T GEMM, gather, final GEMM] and the algorithmic flops of
+        the three compute stages and bytes of the gather."""
+        ms = (C.c_double * 4)()
+        fl = (C.c_uint64 * 3)()
+        gb = C.c_uint64()
+        check(lib().mq_plan_profile(self.h, psi.data_ptr(), sigma.data_ptr(), int(reps), ms, fl, C.byref(gb)))
+        return list(ms), list(fl), gb.value
+
     def jd(self, init, out, tol=1e-8, max_iter=8):
         e = C.c_double()
         it = C.c_int()

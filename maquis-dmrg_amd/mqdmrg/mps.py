@@ -28,13 +28,11 @@ def allowed_sectors(phys_list, right_end, Mmax, left_end=(0, 0)):
     right = [None] * (L + 1)
     left[0] = {tuple(left_end): 1}
     for i in range(L):
+        # exact (arbitrary precision) state counts
         left[i + 1] = _fuse_index(Index.from_dict(left[i]), phys_list[i], +1, cap=None)
-        # cap counts to avoid overflow while keeping exact "reachable" information
-        left[i + 1] = {c: min(s, 10 ** 12) for c, s in left[i + 1].items()}
     right[L] = {tuple(right_end): 1}
     for i in range(L - 1, -1, -1):
         right[i] = _fuse_index(Index.from_dict(right[i + 1]), phys_list[i], -1, cap=None)
-        right[i] = {c: min(s, 10 ** 12) for c, s in right[i].items()}
     out = []
     for i in range(L + 1):
         common = {c: min(Mmax, left[i][c], right[i][c]) for c in left[i] if c in right[i]}
@@ -42,15 +40,22 @@ def allowed_sectors(phys_list, right_end, Mmax, left_end=(0, 0)):
     return out, left, right
 
 
-def random_bond_dims(phys_list, right_end, m, left_end=(0, 0)):
-    """Synthetic bond indices of total dimension ~m: sector sizes proportional to the square root
-    of the number of states reachable from both ends, capped by what both ends allow."""
+def random_bond_dims(phys_list, right_end, m, left_end=(0, 0), sigma=2.0):
+    """Synthetic bond indices of total dimension ~m.  The sector sizes follow a Gaussian in the
+    charge distance (in units of the smallest charge step) from the most populated sector, of
+    width sigma, the way the Schmidt spectrum of a ground state concentrates around the mean
+    charge; each size is capped by the number of states both chain ends can reach."""
     _, left, right = allowed_sectors(phys_list, right_end, 1, left_end)
     L = len(phys_list)
+    steps = [abs(c[k]) for I in phys_list for c in I.charges for k in (0, 1) if c[k] != 0]
+    unit = min(steps) if steps else 1
     out = []
     for i in range(L + 1):
         common = {c: min(left[i][c], right[i][c]) for c in left[i] if c in right[i]}
-        w = {c: math.sqrt(float(left[i][c]) * float(right[i][c])) for c in common}
+        pop = {c: math.log(float(left[i][c])) + math.log(float(right[i][c])) for c in common}
+        c0 = max(pop, key=pop.get)
+        w = {c: math.exp(-(((c[0] - c0[0]) / unit) ** 2 + ((c[1] - c0[1]) / unit) ** 2) / (2 * sigma ** 2))
+             for c in common}
         tot = sum(w.values())
         dims = {}
         for c, wc in w.items():

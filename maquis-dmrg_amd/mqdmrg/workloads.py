@@ -1,0 +1,69 @@
+"""Synthetic site problems at full benchmark shapes.
+
+The boundary *indices* of a chain are propagated through the host planner (left / right step
+schedules, no device work); the boundary data, the MPS tensors and psi are random.  Sigma's cost
+depends only on these shapes, so this gives the benchmark workload of a site in the middle of a
+chain at bond dimension m without running sweeps first."""
+import numpy as np
+
+from . import models, mps
+from .capi import BIndex
+from .engine import MpoDesc, SiteDesc, build_schedule
+from .symmetry import SiteLayout
+
+
+def trivial_bindex(index, aux_dim):
+    """Index of Boundary(ud, ld, ad) (mp_tensors/boundary.h:258-283)."""
+    n = len(index.charges)
+    bs = [s * s for s in index.sizes]
+    offs = np.asarray([[k * bs[i] for k in range(aux_dim)] for i in range(n)], np.int64).ravel()
+    return BIndex(index, index, aux_dim, list(range(n)), list(range(n)), [0] * n, [aux_dim] * n, offs,
+                  np.ones(n * aux_dim), np.zeros(n * aux_dim, np.uint8))
+
+
+def model_chain(model, L, U=4.0):
+    if model == "heisenberg":
+        basis, mpo = models.heisenberg_mpo(L)
+        target = (0, 0)
+    elif model == "hubbard":
+        basis, mpo = models.hubbard_mpo(L, t=1.0, U=U)
+        target = (L // 2, L // 2)
+    else:
+        raise ValueError(model)
+    return basis, mpo, target
+
+
+class SyntheticSite:
+    def __init__(self, model, L, site, m, seed=0):
+        self.model, self.L, self.site, self.m = model, L, site, m
+        basis, mpo, target = model_chain(model, L)
+        self.basis, self.mpo = basis, mpo
+        phys = [basis.phys] * L
+        self.bonds = mps.random_bond_dims(phys, target, m)
+        sym = basis.sym
+        bi = trivial_bindex(self.bonds[0], mpo[0].row_dim)
+        for k in range(site):
+            sd = SiteDesc(sym, self.bonds[k], basis.phys, self.bonds[k + 1])
+            _, _, bi = build_schedule("left", sd, MpoDesc(mpo[k]), bindex_l=bi)
+        self.left_index = bi
+        bi = trivial_bindex(self.bonds[L], mpo[L - 1].col_dim)
+        for k in range(L - 1, site + 1, -1):
+            sd = SiteDesc(sym, self.bonds[k], basis.phys, self.bonds[k + 1])
+            _, _, bi = build_schedule("right", sd, MpoDesc(mpo[k]), bindex_r=bi)
+        self.right_index = bi
+        W12, phys12, _ = models.twosite_mpo(mpo[site], mpo[site + 1], basis.phys, basis.phys)
+        self.W12, self.phys12 = W12, phys12
+        self.mpo_desc = MpoDesc(W12)
+        self.layout = SiteLayout(self.bonds[site], phys12, self.bonds[site + 2])
+        self.site_desc = SiteDesc(sym, self.bonds[site], phys12, self.bonds[site + 2])
+        self.rng = np.random.default_rng(seed)
+
+    def random_cohorts(self, bindex):
+        out = []
+        for ci in range(bindex.n_cohorts):
+            n = bindex.cohort_size(ci)
+            out.append(self.rng.standard_normal(n) if n else None)
+        return out
+
+    def random_psi(self):
+        return mps.random_site_vector(self.layout, self.rng)
