@@ -226,6 +226,66 @@ def hubbard_mpo(L, t=1.0, U=4.0):
     return site, mpo
 
 
+def bosons_u1(nmax):
+    """Soft-core bosons with U1 charge n, 0 <= n <= nmax (models/coded/models_u1.hpp:316-340)."""
+    phys = Index([(n, 0) for n in range(nmax + 1)], [1] * (nmax + 1))
+    ops = {"id": Op(name="id"), "create": Op(name="create"), "destroy": Op(name="destroy"),
+           "count": Op(name="count"), "interaction": Op(name="interaction")}
+    for n in range(nmax + 1):
+        c = (n, 0)
+        ops["id"].add(c, c, 0, 0, 1.0)
+        if n > 0:
+            ops["count"].add(c, c, 0, 0, float(n))
+            ops["interaction"].add(c, c, 0, 0, float(n * n - n))
+            ops["create"].add((n - 1, 0), c, 0, 0, float(np.sqrt(n)))
+            ops["destroy"].add(c, (n - 1, 0), 0, 0, float(np.sqrt(n)))
+    return SiteBasis(SYM_U1, phys, ops)
+
+
+def boson_hubbard_mpo(L, nmax=2, t=1.0, U=8.0, V=0.0):
+    """Coded boson Hubbard (models/coded/models_u1.hpp:359-392): U/2 n(n-1) on every site,
+    -t (b+_p b_{p+1} + b_p b+_{p+1}) and V n_p n_{p+1} on every bond.
+    Bond states: 0 start, 1 b+ placed, 2 b placed, 3 n placed, 4 complete."""
+    site = bosons_u1(nmax)
+    o = site.ops
+    mpo = []
+    for i in range(L):
+        last = (i == L - 1)
+        W = MPOTensor(1 if i == 0 else 5, 1 if last else 5)
+        C = (lambda b: 0) if last else (lambda b: b)
+        if not last:
+            W.add(0, 0, o["id"])
+            W.add(0, 1, o["create"], -t)
+            W.add(0, 2, o["destroy"], -t)
+            if V != 0.0:
+                W.add(0, 3, o["count"], V)
+        if i > 0:
+            W.add(1, C(4), o["destroy"])
+            W.add(2, C(4), o["create"])
+            if V != 0.0:
+                W.add(3, C(4), o["count"])
+            W.add(4, C(4), o["id"])
+        W.add(0, C(4), o["interaction"], U / 2)
+        mpo.append(W)
+    _finish_bonds(mpo, pairs=[(1, 2, 1, 1)], self_adjoint=[0, 3, 4])
+    return site, mpo
+
+
+def chain(model, L, **kw):
+    """(site basis, MPO, total charge) of the benchmark chains."""
+    if model == "heisenberg":
+        basis, mpo = heisenberg_mpo(L, J=kw.get("J", 1.0), Jz=kw.get("Jz", kw.get("J", 1.0)))
+        return basis, mpo, (kw.get("sz2", 0), 0)
+    if model == "hubbard":
+        basis, mpo = hubbard_mpo(L, t=kw.get("t", 1.0), U=kw.get("U", 4.0))
+        return basis, mpo, (kw.get("nup", L // 2), kw.get("ndown", L // 2))
+    if model == "bosehubbard":
+        basis, mpo = boson_hubbard_mpo(L, nmax=kw.get("nmax", 2), t=kw.get("t", 1.0), U=kw.get("U", 8.0),
+                                       V=kw.get("V", 0.0))
+        return basis, mpo, (kw.get("N", L // 2), 0)
+    raise ValueError(model)
+
+
 def _finish_bonds(mpo, pairs, self_adjoint):
     """Register Hermitian partners on every internal bond (bulk bond numbering)."""
     L = len(mpo)

@@ -21,16 +21,8 @@ def trivial_bindex(index, aux_dim):
                   np.ones(n * aux_dim), np.zeros(n * aux_dim, np.uint8))
 
 
-def model_chain(model, L, U=4.0):
-    if model == "heisenberg":
-        basis, mpo = models.heisenberg_mpo(L)
-        target = (0, 0)
-    elif model == "hubbard":
-        basis, mpo = models.hubbard_mpo(L, t=1.0, U=U)
-        target = (L // 2, L // 2)
-    else:
-        raise ValueError(model)
-    return basis, mpo, target
+def model_chain(model, L, **kw):
+    return models.chain(model, L, **kw)
 
 
 class SyntheticSite:

@@ -1,0 +1,81 @@
+"""End-to-end two-site DMRG on the engine: energies against exact diagonalisation and the
+reference's own regression energy, and sigma against the oracle at every site of a sweep."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+from mqdmrg import models
+from mqdmrg.dmrg import TwoSiteDMRG
+from mqdmrg import engine as E
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# dmrg/regression/applications/dmrg/ref/test_coded_bh_ts.h5, /spectrum/results/Energy/mean/value
+# (coded boson Hubbard, open chain L=10, Nmax=2, t=1, U=8, N=5, two-site, m=200, truncation 1e-10)
+BH_REFERENCE_ENERGY = -6.8810903606393472
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = E.Engine(0)
+    yield e
+    e.close()
+
+
+def ed_energy(model, L, target, **kw):
+    basis, mpo, _ = models.chain(model, L, **kw)
+    H = models.mpo_dense_chain(mpo, basis.phys)
+    # restrict to the target charge sector
+    ch = [c for c, s in zip(basis.phys.charges, basis.phys.sizes) for _ in range(s)]
+    tot = np.zeros((1, 2), dtype=int)
+    for _ in range(L):
+        tot = (tot[:, None, :] + np.asarray(ch)[None, :, :]).reshape(-1, 2)
+    sel = np.where((tot[:, 0] == target[0]) & (tot[:, 1] == target[1]))[0]
+    Hs = H[np.ix_(sel, sel)]
+    return float(np.linalg.eigvalsh(Hs)[0])
+
+
+def test_heisenberg_matches_exact(eng):
+    L = 10
+    d = TwoSiteDMRG(eng, "heisenberg", L, m=64, cutoff=1e-12, init_m=8, seed=3, jd_tol=1e-10)
+    e = d.run(3)
+    assert abs(e - ed_energy("heisenberg", L, (0, 0))) < 1e-8
+
+
+def test_hubbard_matches_exact(eng):
+    L = 6
+    d = TwoSiteDMRG(eng, "hubbard", L, m=128, cutoff=1e-12, init_m=8, seed=4, jd_tol=1e-10, U=4.0)
+    e = d.run(3)
+    assert abs(e - ed_energy("hubbard", L, (3, 3), U=4.0)) < 1e-8
+
+
+def test_boson_hubbard_reference_energy(eng):
+    """The reference's regression energy (same model, sizes, bond dimension and truncation)."""
+    d = TwoSiteDMRG(eng, "bosehubbard", 10, m=200, cutoff=1e-10, init_m=8, seed=5, jd_tol=1e-10,
+                    nmax=2, t=1.0, U=8.0, N=5)
+    e = d.run(4)
+    assert abs(e - BH_REFERENCE_ENERGY) < 1e-8, e
+
+
+def test_sigma_parity_at_every_site_heisenberg_L32(eng):
+    """Config 1 (Heisenberg L=32, U1, two-site): sigma of the engine equals the oracle's on the
+    identical psi and boundaries at every site pair of a sweep."""
+    errs = []
+
+    def hook(dm, i, theta, wdesc, plan):
+        Lh = O.HostBoundary(dm.Lb[i].index, dm.Lb[i].download())
+        Rh = O.HostBoundary(dm.Rb[i + 2].index, dm.Rb[i + 2].download())
+        psi = theta.data.cpu().numpy()
+        ref = O.SigmaOracle(theta.desc, wdesc, Lh, Rh).apply(psi)
+        import torch
+        out = torch.zeros_like(theta.data)
+        plan.apply(theta.data, out)
+        got = out.cpu().numpy()
+        errs.append(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    d = TwoSiteDMRG(eng, "heisenberg", 32, m=128, cutoff=1e-10, init_m=16, seed=1, hook=hook)
+    d.init_boundaries()
+    d.sweep()
+    assert len(errs) == 2 * 31
+    assert max(errs) <= 1e-12, max(errs)
