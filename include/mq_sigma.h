@@ -201,6 +201,13 @@ int mq_boundary_cohort_ptr(const mq_boundary* b, uint32_t ci, const double** dev
  * of every block b; out has aux_dim entries. */
 int mq_boundary_traces(const mq_boundary* b, double* out);
 
+/* ---------------------------------------------------------------- SU2 couplings */
+/* SU2::set_coupling (block_matrix/symmetry/gsl_coupling.h:72-92): the four 9j-based coupling
+ * factors (cases 0..3 = site spins (0,0), (2,0), (0,2), (2,2)) for doubled spins a..i, times init. */
+int mq_su2_coupling(int a, int b, int c, int d, int e, int f, int g, int h, int i, double init, double* out4);
+/* Wigner 9j symbol of doubled spins (gsl_sf_coupling_9j semantics). */
+double mq_wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i);
+
 /* ---------------------------------------------------------------- schedules */
 /* Schedule builders run on the host only (no device needed); they take the boundary indices.
  * create_contraction_schedule (site_hamil_schedule.hpp:40-93): right_tasks + site_hamil_tasks. */

@@ -592,6 +592,18 @@ int mq_boundary_traces(const mq_boundary* b, double* out) {
     });
 }
 
+// ---------------------------------------------------------------------------- SU2
+int mq_su2_coupling(int a, int b, int c, int d, int e, int f, int g, int h, int i, double init, double* out4) {
+    return guarded([&] {
+        MQ_REQUIRE(out4, "null output");
+        su2_set_coupling(a, b, c, d, e, f, g, h, i, init, out4);
+    });
+}
+
+double mq_wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
+    return wigner9j(a, b, c, d, e, f, g, h, i);
+}
+
 // ---------------------------------------------------------------------------- schedules
 int mq_schedule_build_sigma(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* left,
                             const mq_bindex_desc* right, mq_schedule** out) {

@@ -108,6 +108,8 @@ SIGNATURES = {
     "mq_boundary_download": (C.c_int, [P, C.c_uint32, P]),
     "mq_boundary_cohort_ptr": (C.c_int, [P, C.c_uint32, PP]),
     "mq_boundary_traces": (C.c_int, [P, P]),
+    "mq_su2_coupling": (C.c_int, [C.c_int] * 9 + [C.c_double, C.POINTER(C.c_double)]),
+    "mq_wigner9j": (C.c_double, [C.c_int] * 9),
     "mq_schedule_build_sigma": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),
                                           C.POINTER(mq_bindex_desc), C.POINTER(mq_bindex_desc), PP]),
     "mq_schedule_build_left": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),

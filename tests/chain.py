@@ -12,14 +12,8 @@ class SiteProblem:
 
     def __init__(self, model="heisenberg", L=8, site=3, Mmax=6, seed=1, symmetric=True, U=4.0):
         rng = np.random.default_rng(seed)
-        if model == "heisenberg":
-            basis, mpo = models.heisenberg_mpo(L)
-            target = (0, 0)
-        elif model == "hubbard":
-            basis, mpo = models.hubbard_mpo(L, t=1.0, U=U)
-            target = (L // 2, L // 2)
-        else:
-            raise ValueError(model)
+        kw = {"U": U} if model == "hubbard" else {}
+        basis, mpo, target = models.chain(model, L, **kw)
         self.basis, self.mpo, self.L, self.site = basis, mpo, L, site
         phys = [basis.phys] * L
         bonds, _, _ = mps.allowed_sectors(phys, target, Mmax)

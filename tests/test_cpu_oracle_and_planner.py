@@ -1,0 +1,218 @@
+"""CPU tests (no GPU): the oracle pinned by dense brute force, closed forms and sympy fixtures;
+the product's host planner against the oracle's restatement; the C-ABI library surface."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from chain import SiteProblem
+from oracle import oracle as O
+from mqdmrg import capi, engine as E, models, mps
+from mqdmrg.dmrg import truncate
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+# --------------------------------------------------------------------------- oracle pins
+@pytest.mark.parametrize("model,L,site,M,seed", [("heisenberg", 6, 2, 5, 1), ("heisenberg", 8, 0, 6, 2),
+                                                 ("heisenberg", 8, 6, 6, 3), ("hubbard", 6, 2, 5, 4),
+                                                 ("bosehubbard", 6, 2, 6, 5)])
+def test_oracle_sigma_equals_dense_brute_force(model, L, site, M, seed):
+    p = SiteProblem(model, L=L, site=site, Mmax=M, seed=seed)
+    sig = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb).apply(p.psi)
+    Ld, Rd = O.dense_boundary(p.Lb), O.dense_boundary(p.Rb)
+    ref = O.dense_to_vec(p.layout, O.dense_sigma(Ld, p.W12.dense(p.phys12), Rd, O.dense_site(p.layout, p.psi)))
+    assert np.linalg.norm(sig - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("model", ["heisenberg", "hubbard", "bosehubbard"])
+def test_oracle_boundary_steps_equal_dense_brute_force(model):
+    p = SiteProblem(model, L=8, site=3, Mmax=6, seed=7)
+    k = 4   # Rb covers sites >= 5, Lb sites < 3
+    lay = p.layouts[k]
+    sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+    md = E.MpoDesc(p.mpo[k])
+    X = O.dense_site(lay, p.vecs[k])
+    Wd = p.mpo[k].dense(lay.phys)
+    r = O.right_step(sd, md, p.vecs[k], p.vecs[k], p.Rb, True)
+    dn = O.dense_right_step(O.dense_boundary(p.Rb), Wd, X, X)
+    assert np.abs(O.dense_boundary(r) - dn).max() <= 1e-13 * np.abs(dn).max()
+    p2 = SiteProblem(model, L=8, site=4, Mmax=6, seed=8)
+    lay = p2.layouts[4]
+    sd = E.SiteDesc(p2.sym, lay.left, lay.phys, lay.right)
+    md = E.MpoDesc(p2.mpo[4])
+    X = O.dense_site(lay, p2.vecs[4])
+    lft = O.left_step(sd, md, mps.rp_to_lp(lay, p2.vecs[4]), p2.vecs[4], p2.Lb, True)
+    dn = O.dense_left_step(O.dense_boundary(p2.Lb), p2.mpo[4].dense(lay.phys), X, X)
+    assert np.abs(O.dense_boundary(lft) - dn).max() <= 1e-13 * np.abs(dn).max()
+
+
+@pytest.mark.parametrize("model", ["heisenberg", "hubbard", "bosehubbard"])
+def test_hermitian_complement_equals_full_computation(model):
+    """complement_transpose's transposed references (boundary.h:161-191) reproduce the blocks a
+    computation without Hermitian skipping produces."""
+    import copy
+    basis, mpo, target = models.chain(model, 6)
+    rng = np.random.default_rng(2)
+    bonds, _, _ = mps.allowed_sectors([basis.phys] * 6, target, 5)
+    lays = mps.layouts_for_chain(bonds, [basis.phys] * 6)
+    vecs = [mps.random_site_vector(l, rng) for l in lays]
+
+    def strip(W):
+        W2 = copy.copy(W)
+        W2.left_herm, W2.right_herm = [None] * W.row_dim, [None] * W.col_dim
+        return W2
+    res = []
+    for herm in (True, False):
+        Lb = O.trivial_boundary(bonds[0], mpo[0].row_dim)
+        for k in range(3):
+            sd = E.SiteDesc(basis.sym, lays[k].left, lays[k].phys, lays[k].right)
+            md = E.MpoDesc(mpo[k] if herm else strip(mpo[k]))
+            Lb = O.left_step(sd, md, mps.rp_to_lp(lays[k], vecs[k]), vecs[k], Lb, True)
+        Rb = O.trivial_boundary(bonds[6], mpo[5].col_dim)
+        for k in range(5, 2, -1):
+            sd = E.SiteDesc(basis.sym, lays[k].left, lays[k].phys, lays[k].right)
+            md = E.MpoDesc(mpo[k] if herm else strip(mpo[k]))
+            Rb = O.right_step(sd, md, vecs[k], vecs[k], Rb, True)
+        res.append((O.dense_boundary(Lb), O.dense_boundary(Rb)))
+    for a, b in zip(res[0], res[1]):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_oracle_ninej_matches_sympy_fixture():
+    rows = json.load(open(os.path.join(GOLDEN, "ninej_sympy.json")))["rows"]
+    err = max(abs(O.wigner9j(*r[:9]) - r[9]) for r in rows)
+    assert err <= 1e-14
+    # a triangle violation gives zero
+    assert O.wigner9j(0, 0, 2, 0, 0, 0, 0, 0, 2) == 0.0
+
+
+def test_oracle_ninej_special_cases():
+    """Closed forms: with one zero spin the 9j reduces to a 6j; {1/2..} known value."""
+    # {a b c; d e f; g h 0} = (-1)^(b+c+d+g) / sqrt((2c+1)(2g+1)) {a b c; e d g}  (c == f, g == h)
+    from sympy import Rational
+    from sympy.physics.wigner import wigner_6j
+    for (a, b, c, d, e, g) in [(1, 1, 2, 1, 1, 0), (2, 1, 1, 1, 2, 1), (2, 2, 2, 2, 2, 2), (3, 1, 2, 1, 3, 2)]:
+        lhs = O.wigner9j(a, b, c, d, e, c, g, g, 0)
+        six = float(wigner_6j(*[Rational(x, 2) for x in (a, b, c, e, d, g)]))
+        sign = (-1) ** ((b + c + d + g) // 2)
+        assert abs(lhs - sign * six / np.sqrt((c + 1) * (g + 1))) <= 1e-14
+
+
+def test_product_su2_coupling_matches_oracle():
+    L = capi.lib()
+    out = (C.c_double * 4)()
+    rows = json.load(open(os.path.join(GOLDEN, "ninej_sympy.json")))["rows"]
+    for r in rows[::7]:
+        assert abs(L.mq_wigner9j(*r[:9]) - r[9]) <= 1e-14
+    for a, c, g, i in [(1, 1, 1, 1), (2, 2, 1, 3), (4, 3, 5, 4), (0, 1, 1, 0), (3, 3, 3, 3)]:
+        for d, e, f in [(0, 1, 1), (1, 1, 0), (2, 1, 1), (1, 2, 1), (0, 0, 0)]:
+            capi.check(L.mq_su2_coupling(a, abs(a - c), c, d, e, f, g, abs(g - i), i, 0.7, out))
+            ref = O.set_coupling(a, abs(a - c), c, d, e, f, g, abs(g - i), i, 0.7)
+            assert np.allclose(list(out), ref, rtol=0, atol=1e-14)
+
+
+def test_oracle_subspace_eigensolver():
+    rng = np.random.default_rng(1)
+    for n in [1, 2, 5, 8]:
+        A = rng.standard_normal((n, n))
+        A = A + A.T
+        val, vec = O.subspace_min(A)
+        w, v = np.linalg.eigh(A)
+        assert abs(val - w[0]) <= 1e-12
+        assert min(np.linalg.norm(vec - v[:, 0]), np.linalg.norm(vec + v[:, 0])) <= 1e-10
+
+
+def test_oracle_jd_converges_to_lowest_eigenvalue():
+    p = SiteProblem("heisenberg", L=6, site=2, Mmax=4, seed=9)
+    so = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    n = p.layout.rp_total
+    # explicit H_eff by applying sigma to unit vectors of the valid (non-padding) entries
+    idx = np.nonzero(p.psi)[0]
+    H = np.zeros((len(idx), len(idx)))
+    for j, k in enumerate(idx):
+        e = np.zeros(n)
+        e[k] = 1.0
+        H[:, j] = so.apply(e)[idx]
+    assert np.allclose(H, H.T, atol=1e-10)
+    e0 = np.linalg.eigvalsh(H)[0]
+    e, v, it = so.jd(p.psi, tol=1e-12, max_iter=40)
+    assert abs(e - e0) <= 1e-9
+
+
+# --------------------------------------------------------------------------- product planner
+@pytest.mark.parametrize("model,L,site,M,seed", [("heisenberg", 8, 3, 6, 1), ("hubbard", 6, 2, 6, 2),
+                                                 ("bosehubbard", 8, 3, 6, 3), ("hubbard", 8, 0, 8, 4)])
+def test_product_sigma_schedule_equals_oracle_schedule(model, L, site, M, seed):
+    p = SiteProblem(model, L=L, site=site, Mmax=M, seed=seed)
+    sp, fp, _ = E.build_schedule("sigma", p.site_desc, p.mpo_desc, p.Lb.index, p.Rb.index)
+    so, fo = O.schedule_sigma(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    assert fp == fo
+    for k in so:
+        if isinstance(so[k], np.ndarray):
+            np.testing.assert_array_equal(sp[k], so[k], err_msg=k)
+
+
+@pytest.mark.parametrize("model", ["heisenberg", "hubbard"])
+def test_product_step_indices_equal_oracle(model):
+    p = SiteProblem(model, L=8, site=3, Mmax=6, seed=5)
+    k = 4
+    lay = p.layouts[k]
+    sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+    md = E.MpoDesc(p.mpo[k])
+    _, _, ri = E.build_schedule("right", sd, md, bindex_r=p.Rb.index)
+    ro = O.right_step(sd, md, p.vecs[k], p.vecs[k], p.Rb, True)
+    np.testing.assert_array_equal(ri.offsets, ro.index.offsets)
+    np.testing.assert_array_equal(ri.tr, ro.index.tr)
+    np.testing.assert_array_equal(ri.n_blocks, ro.index.n_blocks)
+    np.testing.assert_array_equal(ri.conj_scales, ro.index.conj_scales)
+
+
+def test_planner_rejects_mismatched_boundary():
+    p = SiteProblem("heisenberg", L=8, site=3, Mmax=6, seed=1)
+    q = SiteProblem("heisenberg", L=8, site=2, Mmax=6, seed=1)
+    with pytest.raises(capi.EngineError, match="does not match"):
+        E.build_schedule("sigma", p.site_desc, p.mpo_desc, p.Lb.index, q.Rb.index)
+
+
+def test_planner_rejects_bad_descriptors():
+    p = SiteProblem("heisenberg", L=6, site=2, Mmax=4, seed=1)
+    W = p.W12
+    md = E.MpoDesc(W)
+    md.desc.term_b1[0] = W.row_dim + 5    # out of range bond index
+    with pytest.raises(capi.EngineError, match="out of range"):
+        E.build_schedule("sigma", p.site_desc, md, p.Lb.index, p.Rb.index)
+
+
+def test_truncation_rule():
+    """estimate_truncation: cut = max(cutoff*s_max, s[Mmax]); keep s >= cut per block."""
+    s = [np.array([1.0, 0.5, 1e-12]), np.array([0.7, 0.5]), np.array([])]
+    keeps, tw = truncate(s, Mmax=10, cutoff=1e-10)
+    assert keeps == [2, 2, 0]
+    keeps, _ = truncate(s, Mmax=1, cutoff=1e-10)   # cut = s[1] = 0.7
+    assert keeps == [1, 1, 0]
+    keeps, _ = truncate(s, Mmax=2, cutoff=1e-10)   # cut = s[2] = 0.5: the tie keeps both 0.5's
+    assert keeps == [2, 2, 0]
+    keeps, _ = truncate(s, Mmax=10, cutoff=0.6)    # relative cutoff 0.6 * s_max
+    assert keeps == [1, 1, 0]
+
+
+# --------------------------------------------------------------------------- C-ABI surface
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "mq_sigma.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|double|void\*|const char\*)\s+(mq_\w+)\s*\(", hdr, re.M))
+    assert len(declared) >= 40
+    lib = capi.lib()
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert declared <= set(capi.SIGNATURES), declared - set(capi.SIGNATURES)
+
+
+def test_error_codes_not_exit():
+    L = capi.lib()
+    assert L.mq_sigma_apply(None, None, None) == 1
+    assert b"bad argument" in L.mq_last_error()
