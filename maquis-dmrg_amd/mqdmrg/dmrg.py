@@ -127,7 +127,7 @@ class TwoSiteDMRG:
     # ------------------------------------------------------------------ two-site tensors
     def twosite_mpo(self, i):
         if i not in self.ts_mpo:
-            W12, phys12, pb = models.twosite_mpo(self.mpo[i], self.mpo[i + 1], self.phys, self.phys)
+            W12, phys12, pb = models.twosite_sparse(self.mpo[i], self.mpo[i + 1], self.phys, self.phys)
             self.ts_mpo[i] = (W12, E.MpoDesc(W12), phys12, pb)
         return self.ts_mpo[i]
 

@@ -60,17 +60,21 @@ class MpoDesc:
 
     def __init__(self, W):
         self.keep = []
-        ops, op_id = [], {}
-        b1s, b2s, tops, scales = [], [], [], []
-        for (b1, b2) in sorted(W.terms):
-            for op, sc in W.terms[(b1, b2)]:
-                if id(op) not in op_id:
-                    op_id[id(op)] = len(ops)
-                    ops.append(op)
-                b1s.append(b1)
-                b2s.append(b2)
-                tops.append(op_id[id(op)])
-                scales.append(sc)
+        if hasattr(W, "sorted_terms"):
+            ops = W.ops
+            b1s, b2s, tops, scales = W.sorted_terms()
+        else:
+            ops, op_id = [], {}
+            b1s, b2s, tops, scales = [], [], [], []
+            for (b1, b2) in sorted(W.terms):
+                for op, sc in W.terms[(b1, b2)]:
+                    if id(op) not in op_id:
+                        op_id[id(op)] = len(ops)
+                        ops.append(op)
+                    b1s.append(b1)
+                    b2s.append(b2)
+                    tops.append(op_id[id(op)])
+                    scales.append(sc)
         opd = (capi.mq_op_desc * max(1, len(ops)))()
         for k, op in enumerate(ops):
             keys = list(op.blocks.keys())

@@ -157,6 +157,96 @@ class MPOTensor:
         return W
 
 
+class SparseMPOTensor:
+    """W = sum_o C_o (x) o: one sparse (row_dim x col_dim) coefficient matrix per distinct site
+    operator.  Same interface as MPOTensor for the descriptor builder (sorted_terms) and the
+    dense check; scales to the O(L^2) bond dimensions of ab initio Hamiltonians."""
+
+    def __init__(self, row_dim, col_dim, ops, coeffs):
+        import scipy.sparse as sp
+        self.row_dim, self.col_dim = row_dim, col_dim
+        self.ops = list(ops)
+        self.coeffs = [sp.csr_matrix(C, shape=(row_dim, col_dim)) for C in coeffs]
+        self.left_spin = [0] * row_dim
+        self.right_spin = [0] * col_dim
+        self.left_herm = [None] * row_dim
+        self.left_phase = [1] * row_dim
+        self.right_herm = [None] * col_dim
+        self.right_phase = [1] * col_dim
+
+    def sorted_terms(self):
+        """Arrays (b1, b2, op_index, scale) sorted by (b1, b2)."""
+        b1s, b2s, oi, sc = [], [], [], []
+        for k, C in enumerate(self.coeffs):
+            Cc = C.tocoo()
+            b1s.append(Cc.row)
+            b2s.append(Cc.col)
+            oi.append(np.full(Cc.nnz, k, dtype=np.int64))
+            sc.append(Cc.data)
+        if not b1s:
+            z = np.zeros(0, np.int64)
+            return z, z, z, np.zeros(0)
+        b1, b2, oi, sc = (np.concatenate(x) for x in (b1s, b2s, oi, sc))
+        keep = sc != 0
+        b1, b2, oi, sc = b1[keep], b2[keep], oi[keep], sc[keep]
+        order = np.lexsort((oi, b2, b1))
+        return b1[order], b2[order], oi[order], sc[order]
+
+    @property
+    def n_cells(self):
+        return sum(C.nnz for C in self.coeffs)
+
+    def dense(self, phys):
+        n = phys.total()
+        W = np.zeros((self.row_dim, self.col_dim, n, n))
+        for op, C in zip(self.ops, self.coeffs):
+            W += np.einsum("ab,oi->aboi", C.toarray(), op.dense(phys))
+        return W
+
+    @staticmethod
+    def from_mpo_tensor(W):
+        ops, idx, rows, cols, vals = [], {}, [], [], []
+        for (b1, b2), lst in W.terms.items():
+            for op, s in lst:
+                if id(op) not in idx:
+                    idx[id(op)] = len(ops)
+                    ops.append(op)
+                    rows.append([]), cols.append([]), vals.append([])
+                k = idx[id(op)]
+                rows[k].append(b1), cols[k].append(b2), vals[k].append(s)
+        import scipy.sparse as sp
+        C = [sp.coo_matrix((vals[k], (rows[k], cols[k])), shape=(W.row_dim, W.col_dim)) for k in range(len(ops))]
+        T = SparseMPOTensor(W.row_dim, W.col_dim, ops, C)
+        for a in ("left_spin", "right_spin", "left_herm", "left_phase", "right_herm", "right_phase"):
+            setattr(T, a, list(getattr(W, a)))
+        return T
+
+
+def twosite_sparse(W1, W2, phys1, phys2):
+    """Two-site MPO W12 = sum_{o1,o2} (C_o1 C_o2) (x) (o1 (x) o2) (mp_tensors/ts_ops.hpp:96-168)."""
+    if isinstance(W1, MPOTensor):
+        W1 = SparseMPOTensor.from_mpo_tensor(W1)
+    if isinstance(W2, MPOTensor):
+        W2 = SparseMPOTensor.from_mpo_tensor(W2)
+    phys12, pb = product_index(phys1, phys2)
+    ops, coeffs = [], []
+    for o1, C1 in zip(W1.ops, W1.coeffs):
+        for o2, C2 in zip(W2.ops, W2.coeffs):
+            P = (C1 @ C2).tocsr()
+            P.eliminate_zeros()
+            if P.nnz == 0:
+                continue
+            k = _merge(o1.kron(o2, phys1, phys2, phys12, pb))
+            if not k.blocks:
+                continue
+            ops.append(k)
+            coeffs.append(P)
+    T = SparseMPOTensor(W1.row_dim, W2.col_dim, ops, coeffs)
+    T.left_spin, T.left_herm, T.left_phase = list(W1.left_spin), list(W1.left_herm), list(W1.left_phase)
+    T.right_spin, T.right_herm, T.right_phase = list(W2.right_spin), list(W2.right_herm), list(W2.right_phase)
+    return T, phys12, pb
+
+
 def _set_pairs(herm, phase, pairs, self_adjoint):
     for a, b, pa, pb in pairs:
         herm[a], herm[b], phase[a], phase[b] = b, a, pa, pb
@@ -283,6 +373,16 @@ def chain(model, L, **kw):
         basis, mpo = boson_hubbard_mpo(L, nmax=kw.get("nmax", 2), t=kw.get("t", 1.0), U=kw.get("U", 8.0),
                                        V=kw.get("V", 0.0))
         return basis, mpo, (kw.get("N", L // 2), 0)
+    if model == "qc":
+        from . import qc
+        if "fcidump" in kw:
+            h, eri, ecore = qc.read_fcidump(kw["fcidump"])
+        else:
+            h, eri = qc.synthetic_integrals(L, seed=kw.get("integral_seed", 0))
+            ecore = kw.get("ecore", 0.0)
+        basis, mpo, _ = qc.qc_mpo(h, eri, ecore)
+        N = kw.get("N", L)
+        return basis, mpo, (N // 2, N - N // 2)
     raise ValueError(model)
 
 

@@ -43,7 +43,7 @@ class SyntheticSite:
             sd = SiteDesc(sym, self.bonds[k], basis.phys, self.bonds[k + 1])
             _, _, bi = build_schedule("right", sd, MpoDesc(mpo[k]), bindex_r=bi)
         self.right_index = bi
-        W12, phys12, _ = models.twosite_mpo(mpo[site], mpo[site + 1], basis.phys, basis.phys)
+        W12, phys12, _ = models.twosite_sparse(mpo[site], mpo[site + 1], basis.phys, basis.phys)
         self.W12, self.phys12 = W12, phys12
         self.mpo_desc = MpoDesc(W12)
         self.layout = SiteLayout(self.bonds[site], phys12, self.bonds[site + 2])

@@ -34,7 +34,7 @@ class SiteProblem:
             sd = SiteDesc(self.sym, lay.left, lay.phys, lay.right)
             md = MpoDesc(mpo[k])
             self.Rb = O.right_step(sd, md, self.vecs[k], self.vecs[k], self.Rb, symmetric)
-        W12, phys12, pb = models.twosite_mpo(mpo[site], mpo[site + 1], basis.phys, basis.phys)
+        W12, phys12, pb = models.twosite_sparse(mpo[site], mpo[site + 1], basis.phys, basis.phys)
         self.W12, self.phys12 = W12, phys12
         self.mpo_desc = MpoDesc(W12)
         self.layout = SiteLayout(bonds[site], phys12, bonds[site + 2])

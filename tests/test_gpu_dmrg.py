@@ -79,3 +79,16 @@ def test_sigma_parity_at_every_site_heisenberg_L32(eng):
     d.sweep()
     assert len(errs) == 2 * 31
     assert max(errs) <= 1e-12, max(errs)
+
+
+def test_ab_initio_matches_fci(eng):
+    """Synthetic ab initio Hamiltonian (the benchmark integral generator), L=6 orbitals, N=6:
+    two-site DMRG at a bond dimension covering the full space equals FCI."""
+    from mqdmrg import qc
+    L = 6
+    h, eri = qc.synthetic_integrals(L, seed=11)
+    e_fci = qc.fci_energy(h, eri, 3, 3, ecore=0.5)
+    d = TwoSiteDMRG(eng, "qc", L, m=400, cutoff=1e-14, init_m=8, seed=2, jd_tol=1e-11, jd_maxiter=12,
+                    integral_seed=11, ecore=0.5, N=6)
+    e = d.run(3)
+    assert abs(e - e_fci) < 1e-8, (e, e_fci)
