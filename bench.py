@@ -25,6 +25,8 @@ WORKLOADS = {
     # name: (model, L, m)
     "hubbard-L64-m1024": ("hubbard", 64, 1024),
     "heisenberg-L32-m128": ("heisenberg", 32, 128),
+    "qc-L16-m256": ("qc", 16, 256),
+    "qc-L24-m1000": ("qc", 24, 1000),
 }
 
 

@@ -238,6 +238,13 @@ int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sche
                                        uint32_t n_ket_blocks, const uint32_t* ket_rows,
                                        const uint32_t* ket_cols, const mq_boundary* left,
                                        const mq_boundary* right, mq_plan** out);
+/* Sigma algorithms.  SCHEDULE executes the reference's ScheduleNew tables (T GEMMs, alpha-list
+ * gather, L GEMMs).  GROUPED (abelian groups) groups sector pairs by charge transfer and performs
+ * the MPO-term sum as one dense MFMA GEMM per group; same result, no per-sector alpha lists.
+ * AUTO picks GROUPED for abelian groups and SCHEDULE for SU2U1. */
+enum mq_sigma_algorithm { MQ_SIGMA_AUTO = 0, MQ_SIGMA_SCHEDULE = 1, MQ_SIGMA_GROUPED = 2 };
+int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
+                            const mq_boundary* left, const mq_boundary* right, int algorithm, mq_plan** out);
 /* sigma = H_eff * psi; psi and sigma are device pointers in the padded Davidson-vector layout. */
 int mq_sigma_apply(mq_plan* plan, const double* psi, double* sigma);
 /* Measurement: runs `reps` sigma applications stage by stage with HIP events on the context's

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <map>
 #include <memory>
 #include <new>
 #include <string>
@@ -129,30 +130,42 @@ struct mq_schedule {
 struct StagePlan {
     std::vector<GemmTask> t_tasks, f_tasks;
     std::vector<GemmSeg> t_segs, f_segs;
-    std::vector<GatherRegion> regions;
+    std::vector<GatherRegion> regions;      // large regions, chunked
+    std::vector<GatherRegion> small;        // regions of <= GATHER_SMALL elements, one wavefront each
     std::vector<uint64_t> src_off;
     std::vector<double> alpha;
-    uint32_t t_tiles = 0, f_tiles = 0, g_chunks = 0;
+    // grouped path: the middle stage is a GEMM (S = T * W) instead of the gather
+    std::vector<GemmTask> s_tasks;
+    std::vector<GemmSeg> s_segs;
+    std::vector<uint32_t> maps;             // column / row block remaps of the GEMM stages
+    uint32_t t_tiles = 0, f_tiles = 0, s_tiles = 0, g_chunks = 0;
     uint64_t ws_elems = 0;
-    uint64_t s_begin = 0, s_elems = 0;   // S range in the workspace (zero-filled each apply)
-    DevBuf d_t_tasks, d_t_segs, d_f_tasks, d_f_segs, d_regions, d_src_off, d_alpha;
+    uint64_t s_begin = 0, s_elems = 0;   // workspace range zero-filled at the start of each apply
+    DevBuf d_t_tasks, d_t_segs, d_f_tasks, d_f_segs, d_s_tasks, d_s_segs, d_maps, d_regions, d_small, d_src_off,
+        d_alpha;
     uint64_t sizes[N_BASES] = {0, 0, 0, 0, 0, 0};   // element counts of each base for validation
 
+    static uint32_t tile_tasks(std::vector<GemmTask>& tasks) {
+        uint64_t tile = 0;
+        for (auto& t : tasks) {
+            t.tiles_n = (t.N + GEMM_BN - 1) / GEMM_BN;
+            t.tile_begin = uint32_t(tile);
+            tile += uint64_t((t.M + GEMM_BM - 1) / GEMM_BM) * t.tiles_n;
+            MQ_REQUIRE(tile < (1ull << 31), "too many GEMM tiles in one launch");
+        }
+        return uint32_t(tile);
+    }
+
     void finish_tables() {
-        uint32_t tile = 0;
-        for (auto& t : t_tasks) {
-            t.tiles_n = (t.N + GEMM_BN - 1) / GEMM_BN;
-            t.tile_begin = tile;
-            tile += ((t.M + GEMM_BM - 1) / GEMM_BM) * t.tiles_n;
+        t_tiles = tile_tasks(t_tasks);
+        f_tiles = tile_tasks(f_tasks);
+        s_tiles = tile_tasks(s_tasks);
+        std::vector<GatherRegion> big;
+        for (const auto& r : regions) {
+            if (uint64_t(r.R) * r.C <= uint64_t(GATHER_SMALL)) small.push_back(r);
+            else big.push_back(r);
         }
-        t_tiles = tile;
-        tile = 0;
-        for (auto& t : f_tasks) {
-            t.tiles_n = (t.N + GEMM_BN - 1) / GEMM_BN;
-            t.tile_begin = tile;
-            tile += ((t.M + GEMM_BM - 1) / GEMM_BM) * t.tiles_n;
-        }
-        f_tiles = tile;
+        regions.swap(big);
         uint32_t ch = 0;
         for (auto& r : regions) {
             r.chunk_begin = ch;
@@ -182,24 +195,47 @@ struct StagePlan {
         auto gemm = [&](const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs) {
             for (const auto& t : tasks) {
                 MQ_REQUIRE(t.ldc >= t.M || t.N <= 1, "ldc < M");
-                check_in(t.c_base, t.c_off, extent(t.M, t.N, t.ldc), "C");
+                if (t.cmap == MAP_NONE) {
+                    check_in(t.c_base, t.c_off, extent(t.M, t.N, t.ldc), "C");
+                } else {
+                    MQ_REQUIRE(t.cbw > 0 && t.ldc >= t.M, "column remap without block width");
+                    uint32_t nblk = (t.N + t.cbw - 1) / t.cbw;
+                    MQ_REQUIRE(uint64_t(t.cmap) + nblk <= maps.size(), "column map out of range");
+                    for (uint32_t j = 0; j < nblk; ++j) {
+                        uint32_t w = std::min(t.cbw, t.N - j * t.cbw);
+                        check_in(t.c_base, t.c_off + uint64_t(maps[t.cmap + j]) * t.cldk, extent(t.M, w, t.ldc),
+                                 "C block");
+                    }
+                }
                 MQ_REQUIRE(t.seg0 <= t.seg1 && t.seg1 <= segs.size(), "segment range");
                 for (uint32_t s = t.seg0; s < t.seg1; ++s) {
                     const GemmSeg& g = segs[s];
                     if (!g.trans_a) check_in(g.a_base, g.a_off, extent(t.M, g.K, g.lda), "A");
                     else check_in(g.a_base, g.a_off, extent(g.K, t.M, g.lda), "A^T");
-                    if (!g.trans_b) check_in(g.b_base, g.b_off, extent(g.K, t.N, g.ldb), "B");
-                    else check_in(g.b_base, g.b_off, extent(t.N, g.K, g.ldb), "B^T");
+                    if (g.bmap != MAP_NONE) {
+                        MQ_REQUIRE(!g.trans_b && g.bbh > 0 && g.K % g.bbh == 0, "row remap shape");
+                        uint32_t nblk = g.K / g.bbh;
+                        MQ_REQUIRE(uint64_t(g.bmap) + nblk <= maps.size(), "row map out of range");
+                        for (uint32_t j = 0; j < nblk; ++j)
+                            check_in(g.b_base, g.b_off + uint64_t(maps[g.bmap + j]) * g.bldk,
+                                     extent(g.bbh, t.N, g.ldb), "B block");
+                    } else if (!g.trans_b) {
+                        check_in(g.b_base, g.b_off, extent(g.K, t.N, g.ldb), "B");
+                    } else {
+                        check_in(g.b_base, g.b_off, extent(t.N, g.K, g.ldb), "B^T");
+                    }
                 }
             }
         };
         gemm(t_tasks, t_segs);
+        gemm(s_tasks, s_segs);
         gemm(f_tasks, f_segs);
-        for (const auto& r : regions) {
-            check_in(BASE_WS, r.dst_off, extent(r.R, r.C, r.ld_dst), "S region");
-            MQ_REQUIRE(r.t0 <= r.t1 && r.t1 <= src_off.size(), "region task range");
-            for (uint32_t t = r.t0; t < r.t1; ++t) check_in(BASE_WS, src_off[t], uint64_t(r.R) * r.C, "T slab");
-        }
+        for (const auto* list : {&regions, &small})
+            for (const auto& r : *list) {
+                check_in(BASE_WS, r.dst_off, extent(r.R, r.C, r.ld_dst), "S region");
+                MQ_REQUIRE(r.t0 <= r.t1 && r.t1 <= src_off.size(), "region task range");
+                for (uint32_t t = r.t0; t < r.t1; ++t) check_in(BASE_WS, src_off[t], uint64_t(r.R) * r.C, "T slab");
+            }
     }
     void upload_all(hipStream_t st) {
         upload(d_t_tasks, t_tasks, st);
@@ -207,15 +243,49 @@ struct StagePlan {
         upload(d_f_tasks, f_tasks, st);
         upload(d_f_segs, f_segs, st);
         upload(d_regions, regions, st);
+        upload(d_small, small, st);
         upload(d_src_off, src_off, st);
         upload(d_alpha, alpha, st);
+        upload(d_s_tasks, s_tasks, st);
+        upload(d_s_segs, s_segs, st);
+        upload(d_maps, maps, st);
+    }
+    // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM
+    void stage(int k, hipStream_t st, Bases b) const {
+        const uint32_t* mp = d_maps.as<uint32_t>();
+        switch (k) {
+            case 0:
+                if (s_elems) HIPCHK(launch_fill(st, b.p[BASE_WS] + s_begin, s_elems, 0.0));
+                break;
+            case 1:
+                HIPCHK(launch_gemm(st, d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()), d_t_segs.as<GemmSeg>(),
+                                   t_tiles, b, mp));
+                break;
+            case 2:
+                if (!s_tasks.empty())
+                    HIPCHK(launch_gemm(st, d_s_tasks.as<GemmTask>(), uint32_t(s_tasks.size()),
+                                       d_s_segs.as<GemmSeg>(), s_tiles, b, mp));
+                run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
+                break;
+            default:
+                HIPCHK(launch_gemm(st, d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()), d_f_segs.as<GemmSeg>(),
+                                   f_tiles, b, mp));
+        }
     }
     void run(hipStream_t st, Bases b) const {
-        if (s_elems) HIPCHK(launch_fill(st, b.p[BASE_WS] + s_begin, s_elems, 0.0));
-        HIPCHK(launch_gemm(st, d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()), d_t_segs.as<GemmSeg>(), t_tiles, b));
+        for (int k = 0; k < 4; ++k) stage(k, st, b);
+    }
+    // gather of both region lists: src offsets are relative to src, destinations to dst
+    void run_gather(hipStream_t st, double* src, double* dst) const {
         HIPCHK(launch_gather(st, d_regions.as<GatherRegion>(), uint32_t(regions.size()), d_src_off.as<uint64_t>(),
-                             d_alpha.as<double>(), g_chunks, b.p[BASE_WS], b.p[BASE_WS]));
-        HIPCHK(launch_gemm(st, d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()), d_f_segs.as<GemmSeg>(), f_tiles, b));
+                             d_alpha.as<double>(), g_chunks, src, dst));
+        HIPCHK(launch_gather_small(st, d_small.as<GatherRegion>(), uint32_t(small.size()), d_src_off.as<uint64_t>(),
+                                   d_alpha.as<double>(), src, dst));
+    }
+    template <class F>
+    void for_regions(F&& f) const {
+        for (const auto& r : regions) f(r);
+        for (const auto& r : small) f(r);
     }
 };
 
@@ -236,9 +306,10 @@ struct mq_plan {
 // ============================================================================ lowering
 namespace {
 
-void add_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, uint8_t c_base, uint64_t c_off, uint32_t M,
-              uint32_t N, uint32_t ldc, const std::vector<GemmSeg>& s, uint8_t beta = 0) {
-    if (M == 0 || N == 0) return;
+GemmTask& add_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, uint8_t c_base, uint64_t c_off,
+                   uint32_t M, uint32_t N, uint32_t ldc, const std::vector<GemmSeg>& s, uint8_t beta = 0) {
+    static GemmTask dummy;
+    if (M == 0 || N == 0 || s.empty()) return dummy;
     GemmTask t{};
     t.c_off = c_off;
     t.M = M;
@@ -249,10 +320,15 @@ void add_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, uint8_t 
     t.seg1 = uint32_t(segs.size());
     t.c_base = c_base;
     t.beta = beta;
+    t.cmap = MAP_NONE;
+    t.cbw = N;
+    t.cldk = 0;
     tasks.push_back(t);
+    return tasks.back();
 }
 
-GemmSeg seg(uint8_t ab, uint64_t ao, uint32_t lda, bool ta, uint8_t bb, uint64_t bo, uint32_t ldb, bool tb, uint32_t K) {
+GemmSeg seg(uint8_t ab, uint64_t ao, uint32_t lda, bool ta, uint8_t bb, uint64_t bo, uint32_t ldb, bool tb, uint32_t K,
+            double alpha = 1.0) {
     GemmSeg g{};
     g.a_off = ao;
     g.b_off = bo;
@@ -263,6 +339,10 @@ GemmSeg seg(uint8_t ab, uint64_t ao, uint32_t lda, bool ta, uint8_t bb, uint64_t
     g.b_base = bb;
     g.trans_a = ta;
     g.trans_b = tb;
+    g.alpha = alpha;
+    g.bmap = MAP_NONE;
+    g.bbh = 1;
+    g.bldk = 0;
     return g;
 }
 
@@ -452,16 +532,14 @@ int reshape(mq_ctx* ctx, const mq_site_desc* site, const double* src, double* ds
         P.finish_tables();
         // validation: gather reads from src, writes to dst
         uint64_t n_src = to_left ? L.rp_total : L.lp_total, n_dst = to_left ? L.lp_total : L.rp_total;
-        for (const auto& r : P.regions) {
+        P.for_regions([&](const GatherRegion& r) {
             MQ_REQUIRE(StagePlan::extent(r.R, r.C, r.ld_dst) + r.dst_off <= n_dst, "reshape dst out of range");
             for (uint32_t t = r.t0; t < r.t1; ++t)
                 MQ_REQUIRE(P.src_off[t] + uint64_t(r.R) * r.C <= n_src, "reshape src out of range");
-        }
+        });
         P.upload_all(ctx->stream);
         HIPCHK(launch_fill(ctx->stream, dst, n_dst, 0.0));
-        HIPCHK(launch_gather(ctx->stream, P.d_regions.as<GatherRegion>(), uint32_t(P.regions.size()),
-                             P.d_src_off.as<uint64_t>(), P.d_alpha.as<double>(), P.g_chunks, const_cast<double*>(src),
-                             dst));
+        P.run_gather(ctx->stream, const_cast<double*>(src), dst);
         HIPCHK(hipStreamSynchronize(ctx->stream));   // tables are freed on return
     });
 }
@@ -719,6 +797,310 @@ void lower_sigma(mq_plan* p, const Schedule& S, const std::vector<uint32_t>& ket
     P.ws_elems = ws;
 }
 
+// ---------------------------------------------------------------------------- grouped sigma
+// sigma(lc_out; s_out, rc_out) = sum_{b1, lc_in} L_b1(lc_out, lc_in) S_b1(lc_in; s_out, rc_out)
+// S_b1(lc_in; s_out, rc_out)  = sum_{b2, s_in} W_b1b2(s_out <- s_in) T_b2(lc_in; s_in, rc_out)
+// For abelian groups the MPO coefficient matrix W[(b2, s_in, ss1), (b1, s_out, ss2)] does not
+// depend on the sector pair (lc_in, rc_out) beyond its charge transfer D = rc_out - lc_in.  Pairs
+// are grouped by D; per group T_D (rows: pair slab elements, columns k = (b2, s_in, ss1)) is written
+// by the T GEMMs through a column-block remap, S_D = T_D W_D is one dense MFMA GEMM, and the final
+// GEMMs read S_D through a row-block remap.  The conjugation scales of transposed boundary blocks
+// are applied as per-segment alpha.  Same result as the reference's T / S / L stages
+// (solver/tasks.cpp:286-344, 466-494, 650-752) with the MPO-term sum done as a dense product.
+struct GroupedLowering {
+    static uint64_t kkey(uint32_t b, uint32_t s, uint32_t ss) {
+        return (uint64_t(b) << 32) | (uint64_t(s) << 16) | ss;
+    }
+    struct Pair {
+        uint32_t lb_in, rb_out, g;
+        uint64_t m_off;
+        std::vector<uint32_t> kk, nn;   // indices into the group's K / N lists
+    };
+    struct Group {
+        Charge delta;
+        std::vector<uint64_t> K, N;
+        std::unordered_map<uint64_t, uint32_t> kidx, nidx;
+        uint64_t M = 0, t_off = 0, s_off = 0, w_off = 0;
+        std::vector<uint32_t> pairs;
+    };
+    std::vector<Pair> pairs;
+    std::vector<Group> groups;
+    std::unordered_map<uint64_t, uint32_t> pair_of;   // (lb_in, rb_out) -> pair
+    std::vector<double> w_dense;
+    uint64_t ref_flops = 0;
+};
+
+void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::vector<double>& ws_init,
+                         uint64_t& ws_init_off) {
+    const Index &left_i = site.left_i, &right_i = site.right_i, &phys_i = site.phys_i;
+    const BoundaryIndex& LI = p->left->index;
+    const BoundaryIndex& RI = p->right->index;
+    MQ_REQUIRE(!site.symm.su2(), "the grouped sigma path handles abelian symmetries only");
+    MQ_REQUIRE(LI.aux_dim == W.row_dim && RI.aux_dim == W.col_dim, "boundary aux_dim != mpo bond dimension");
+    require_bond(LI, left_i, "left");
+    require_bond(RI, right_i, "right");
+    ProductBasis rpb(phys_i, right_i, -1);
+    GroupedLowering G;
+    std::map<uint64_t, uint32_t> group_of;   // delta key -> group (ordered: deterministic numbering)
+    // stored block j of a (non-transposed) cohort -> MPO index b
+    auto block_order = [](const BoundaryIndex& B, uint32_t ci) {
+        std::vector<uint32_t> jb(B.n_blocks[ci], MQ_NONE);
+        uint64_t bs = B.block_size(ci);
+        for (uint32_t b = 0; b < B.aux_dim; ++b) {
+            int64_t o = B.offset(ci, b);
+            if (o < 0) continue;
+            MQ_REQUIRE(!B.transposes[size_t(ci) * B.aux_dim + b], "transposed block inside a stored cohort");
+            MQ_REQUIRE(B.conjugate_scale(ci, b) == 1.0, "scaled block inside a stored cohort");
+            uint64_t j = uint64_t(o) / bs;
+            MQ_REQUIRE(j < jb.size() && uint64_t(o) % bs == 0 && jb[j] == MQ_NONE, "cohort block offsets");
+            jb[j] = b;
+        }
+        for (auto b : jb) MQ_REQUIRE(b != MQ_NONE, "cohort with unmapped stored block");
+        return jb;
+    };
+    // ---- pairs and groups
+    for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
+        Charge lc_in = left_i.charges[lb];
+        for (uint32_t rb = 0; rb < right_i.size(); ++rb) {
+            Charge rc_out = right_i.charges[rb];
+            std::vector<uint64_t> K, N;
+            for (uint32_t s = 0; s < phys_i.size(); ++s) {
+                Charge rc_in = lc_in + phys_i.charges[s];
+                uint32_t ci = RI.cohort_index(rc_in, rc_out);
+                if (ci < RI.n_cohorts())
+                    for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2)
+                        if (RI.has_block(ci, b2))
+                            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) K.push_back(GroupedLowering::kkey(b2, s, ss));
+                Charge lc_out = rc_out - phys_i.charges[s];
+                uint32_t cl = LI.cohort_index(lc_out, lc_in);
+                if (cl < LI.n_cohorts())
+                    for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1)
+                        if (LI.has_block(cl, b1))
+                            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) N.push_back(GroupedLowering::kkey(b1, s, ss));
+            }
+            if (K.empty() || N.empty()) continue;
+            Charge d = rc_out - lc_in;
+            auto ins = group_of.emplace(d.key(), uint32_t(G.groups.size()));
+            if (ins.second) {
+                G.groups.emplace_back();
+                G.groups.back().delta = d;
+            }
+            GroupedLowering::Group& g = G.groups[ins.first->second];
+            GroupedLowering::Pair pr;
+            pr.lb_in = lb;
+            pr.rb_out = rb;
+            pr.g = ins.first->second;
+            pr.m_off = g.M;
+            g.M += uint64_t(left_i.sizes[lb]) * right_i.sizes[rb];
+            for (auto k : K) {
+                auto it = g.kidx.emplace(k, uint32_t(g.K.size()));
+                if (it.second) g.K.push_back(k);
+                pr.kk.push_back(it.first->second);
+            }
+            for (auto n : N) {
+                auto it = g.nidx.emplace(n, uint32_t(g.N.size()));
+                if (it.second) g.N.push_back(n);
+                pr.nn.push_back(it.first->second);
+            }
+            g.pairs.push_back(uint32_t(G.pairs.size()));
+            G.pair_of[uint64_t(lb) << 32 | rb] = uint32_t(G.pairs.size());
+            G.pairs.push_back(std::move(pr));
+        }
+    }
+    // ---- workspace: T_D, S_D, W_D per group
+    uint64_t ws = 0;
+    for (auto& g : G.groups) {
+        MQ_REQUIRE(g.M < (1ull << 32) && g.K.size() < (1u << 31) && g.N.size() < (1u << 31), "group too large");
+        g.t_off = ws;
+        ws += align32(g.M * g.K.size());
+    }
+    p->st.s_begin = 0;
+    p->st.s_elems = ws;   // T_D must start at zero (absent blocks)
+    for (auto& g : G.groups) {
+        g.s_off = ws;
+        ws += align32(g.M * g.N.size());
+    }
+    ws_init_off = ws;
+    for (auto& g : G.groups) {
+        g.w_off = ws;
+        ws += align32(uint64_t(g.K.size()) * g.N.size());
+    }
+    ws_init.assign(ws - ws_init_off, 0.0);
+    // ---- dense W_D: for every MPO entry, its (k, n) in the one group holding both keys
+    std::unordered_map<uint64_t, std::vector<std::pair<uint32_t, uint32_t>>> kgroups, ngroups;
+    for (uint32_t gi = 0; gi < G.groups.size(); ++gi) {
+        for (uint32_t i = 0; i < G.groups[gi].K.size(); ++i) kgroups[G.groups[gi].K[i]].push_back({gi, i});
+        for (uint32_t i = 0; i < G.groups[gi].N.size(); ++i) ngroups[G.groups[gi].N[i]].push_back({gi, i});
+    }
+    for (uint32_t b1 = 0; b1 < W.row_dim; ++b1)
+        for (const auto& cell : W.rows[b1]) {
+            uint32_t b2 = cell.other;
+            for (uint32_t t = cell.t0; t < cell.t1; ++t) {
+                const Op& op = W.ops[W.terms[t].op];
+                double sc = W.terms[t].scale;
+                for (const OpBlock& wb : op.blocks) {
+                    uint32_t s_in = phys_i.position(wb.in), s_out = phys_i.position(wb.out);
+                    MQ_REQUIRE(s_in < phys_i.size() && s_out < phys_i.size(), "operator charge not in physical index");
+                    for (uint32_t e = wb.e0; e < wb.e1; ++e) {
+                        auto kit = kgroups.find(GroupedLowering::kkey(b2, s_in, op.row[e]));
+                        auto nit = ngroups.find(GroupedLowering::kkey(b1, s_out, op.col[e]));
+                        if (kit == kgroups.end() || nit == ngroups.end()) continue;
+                        for (auto& kg : kit->second)
+                            for (auto& ng : nit->second)
+                                if (kg.first == ng.first) {
+                                    const auto& g = G.groups[kg.first];
+                                    ws_init[g.w_off - ws_init_off + uint64_t(ng.second) * g.K.size() + kg.second] +=
+                                        op.coeff[e] * sc;
+                                }
+                    }
+                }
+            }
+        }
+    // reference flop model of the same sigma (tasks.cpp:388-398, 777-796): T + S (one alpha per
+    // nonzero coupling of a pair) + L
+    {
+        uint64_t f = 0;
+        for (auto& g : G.groups) {
+            uint64_t Kg = g.K.size(), Ng = g.N.size();
+            const double* Wg = ws_init.data() + (g.w_off - ws_init_off);
+            std::vector<std::vector<uint32_t>> col_nz(Ng);
+            for (uint64_t n = 0; n < Ng; ++n)
+                for (uint64_t k = 0; k < Kg; ++k)
+                    if (Wg[n * Kg + k] != 0.0) col_nz[n].push_back(uint32_t(k));
+            std::vector<char> kin(Kg);
+            for (uint32_t pi : g.pairs) {
+                const auto& pr = G.pairs[pi];
+                std::fill(kin.begin(), kin.end(), 0);
+                for (auto k : pr.kk) kin[k] = 1;
+                uint64_t na = 0;
+                for (auto n : pr.nn)
+                    for (auto k : col_nz[n]) na += kin[k];
+                f += 2ull * left_i.sizes[pr.lb_in] * right_i.sizes[pr.rb_out] * na;
+            }
+        }
+        G.ref_flops = f;
+    }
+    // ---- T stage
+    StagePlan& P = p->st;
+    for (auto& pr : G.pairs) {
+        const auto& g = G.groups[pr.g];
+        uint32_t lb = pr.lb_in, rb = pr.rb_out;
+        Charge lc_in = left_i.charges[lb], rc_out = right_i.charges[rb];
+        uint32_t M = left_i.sizes[lb], brs = right_i.sizes[rb];
+        for (uint32_t s = 0; s < phys_i.size(); ++s) {
+            Charge rc_in = lc_in + phys_i.charges[s];
+            uint32_t rbi = right_i.position(rc_in);
+            uint32_t ci = RI.cohort_index(rc_in, rc_out);
+            if (rbi == right_i.size() || ci >= RI.n_cohorts()) continue;
+            uint32_t K = right_i.sizes[rbi];
+            bool tr = RI.tr[ci];
+            uint32_t cie = tr ? RI.cohort_index(rc_out, rc_in) : ci;
+            MQ_REQUIRE(cie < RI.n_cohorts(), "transposed right cohort without partner");
+            uint64_t bs = uint64_t(K) * brs;
+            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
+                uint64_t a_off = site.rp_offset[lb] + (rpb(s, rbi) + uint64_t(ss) * K) * M;
+                if (!tr) {
+                    std::vector<uint32_t> jb = block_order(RI, ci);
+                    uint32_t nb = uint32_t(jb.size());
+                    if (!nb) continue;
+                    GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, g.t_off + pr.m_off, M, nb * brs, M,
+                                           {seg(BASE_IN, a_off, M, false, BASE_RIGHT, p->right->cohort_off[ci], K, false, K)});
+                    t.cmap = uint32_t(P.maps.size());
+                    t.cbw = brs;
+                    t.cldk = g.M;
+                    for (uint32_t j = 0; j < nb; ++j) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(jb[j], s, ss)));
+                } else {
+                    for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2) {
+                        if (!RI.has_block(ci, b2)) continue;
+                        uint64_t o = uint64_t(RI.offset(ci, b2));
+                        uint32_t k = g.kidx.at(GroupedLowering::kkey(b2, s, ss));
+                        add_gemm(P.t_tasks, P.t_segs, BASE_WS, g.t_off + uint64_t(k) * g.M + pr.m_off, M, brs, M,
+                                 {seg(BASE_IN, a_off, M, false, BASE_RIGHT, p->right->cohort_off[cie] + o, brs, true, K,
+                                      RI.conjugate_scale(ci, b2))});
+                        (void)bs;
+                    }
+                }
+            }
+        }
+    }
+    // ---- S stage: S_D = T_D W_D
+    for (auto& g : G.groups) {
+        uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
+        add_gemm(P.s_tasks, P.s_segs, BASE_WS, g.s_off, uint32_t(g.M), Ng, uint32_t(g.M),
+                 {seg(BASE_WS, g.t_off, uint32_t(g.M), false, BASE_WS, g.w_off, Kg, false, Kg)});
+    }
+    // ---- final stage: per output unit (lb_out, s_out, ss2), K segments over (lc_in, b1)
+    for (uint32_t lbo = 0; lbo < left_i.size(); ++lbo) {
+        Charge lc_out = left_i.charges[lbo];
+        uint32_t ls_out = left_i.sizes[lbo];
+        for (uint32_t s = 0; s < phys_i.size(); ++s) {
+            Charge rc_out = lc_out + phys_i.charges[s];
+            uint32_t rb = right_i.position(rc_out);
+            if (rb == right_i.size()) continue;
+            uint32_t rs = right_i.sizes[rb];
+            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
+                std::vector<GemmSeg> segs;
+                for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
+                    auto pit = G.pair_of.find(uint64_t(lb) << 32 | rb);
+                    if (pit == G.pair_of.end()) continue;
+                    const auto& pr = G.pairs[pit->second];
+                    const auto& g = G.groups[pr.g];
+                    Charge lc_in = left_i.charges[lb];
+                    uint32_t cl = LI.cohort_index(lc_out, lc_in);
+                    if (cl >= LI.n_cohorts()) continue;
+                    uint32_t ls_in = left_i.sizes[lb];
+                    uint64_t sbase = g.s_off + pr.m_off;
+                    if (!LI.tr[cl]) {
+                        std::vector<uint32_t> jb = block_order(LI, cl);
+                        if (jb.empty()) continue;
+                        GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cl], ls_out, false, BASE_WS, sbase, ls_in, false,
+                                         uint32_t(jb.size()) * ls_in);
+                        sg.bmap = uint32_t(P.maps.size());
+                        sg.bbh = ls_in;
+                        sg.bldk = g.M;
+                        for (uint32_t j = 0; j < jb.size(); ++j)
+                            P.maps.push_back(g.nidx.at(GroupedLowering::kkey(jb[j], s, ss)));
+                        segs.push_back(sg);
+                    } else {
+                        uint32_t cle = LI.cohort_index(lc_in, lc_out);
+                        MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
+                        for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1) {
+                            if (!LI.has_block(cl, b1)) continue;
+                            uint64_t o = uint64_t(LI.offset(cl, b1));
+                            uint32_t n = g.nidx.at(GroupedLowering::kkey(b1, s, ss));
+                            segs.push_back(seg(BASE_LEFT, p->left->cohort_off[cle] + o, ls_in, true, BASE_WS,
+                                               sbase + uint64_t(n) * g.M, ls_in, false, ls_in,
+                                               LI.conjugate_scale(cl, b1)));
+                        }
+                    }
+                }
+                uint64_t c_off = site.rp_offset[lbo] + (rpb(s, rb) + uint64_t(ss) * rs) * ls_out;
+                add_gemm(P.f_tasks, P.f_segs, BASE_OUT, c_off, ls_out, rs, ls_out, segs);
+            }
+        }
+    }
+    P.ws_elems = ws;
+    p->flops = G.ref_flops;
+    // T and L reference flops
+    for (auto& pr : G.pairs) {
+        uint32_t lb = pr.lb_in, rb = pr.rb_out;
+        for (uint32_t s = 0; s < phys_i.size(); ++s) {
+            Charge rc_in = left_i.charges[lb] + phys_i.charges[s];
+            uint32_t ci = RI.cohort_index(rc_in, right_i.charges[rb]);
+            if (ci >= RI.n_cohorts()) continue;
+            uint32_t cie = RI.tr[ci] ? RI.cohort_index(right_i.charges[rb], rc_in) : ci;
+            p->flops += 2ull * left_i.sizes[lb] * RI.cohort_size(cie) * phys_i.sizes[s];
+        }
+    }
+    for (uint32_t lbo = 0; lbo < left_i.size(); ++lbo)
+        for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
+            uint32_t cl = LI.cohort_index(left_i.charges[lbo], left_i.charges[lb]);
+            if (cl >= LI.n_cohorts()) continue;
+            uint32_t cle = LI.tr[cl] ? LI.cohort_index(left_i.charges[lb], left_i.charges[lbo]) : cl;
+            p->flops += 2ull * left_i.sizes[lbo] * left_i.sizes[lb] * LI.n_blocks[cle] * site.rp_cols[lbo];
+        }
+}
+
 }  // namespace
 
 extern "C" {
@@ -755,6 +1137,44 @@ int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sche
         P.sizes[BASE_RIGHT] = right->total;
         P.validate();
         p->ws.alloc(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
+        P.upload_all(ctx->stream);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = p.release();
+    });
+}
+
+int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_boundary* left,
+                            const mq_boundary* right, int algorithm, mq_plan** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && site && mpo && left && right && out, "bad argument");
+        SiteLayout L(*site);
+        if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
+        if (algorithm == MQ_SIGMA_SCHEDULE) {
+            int rc = mq_sigma_plan_create(ctx, site, mpo, left, right, out);
+            if (rc != MQ_OK) fail(rc, g_err);
+            return;
+        }
+        MQ_REQUIRE(algorithm == MQ_SIGMA_GROUPED, "unknown sigma algorithm");
+        Mpo W(*mpo);
+        auto p = std::make_unique<mq_plan>();
+        p->ctx = ctx;
+        p->left = left;
+        p->right = right;
+        p->vec_size = L.rp_total;
+        std::vector<double> ws_init;
+        uint64_t ws_init_off = 0;
+        lower_sigma_grouped(p.get(), L, W, ws_init, ws_init_off);
+        StagePlan& P = p->st;
+        P.finish_tables();
+        P.sizes[BASE_WS] = P.ws_elems;
+        P.sizes[BASE_IN] = P.sizes[BASE_OUT] = L.rp_total;
+        P.sizes[BASE_LEFT] = left->total;
+        P.sizes[BASE_RIGHT] = right->total;
+        P.validate();
+        p->ws.alloc(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
+        if (!ws_init.empty())
+            HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, ctx->stream));
         P.upload_all(ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
         *out = p.release();
@@ -811,17 +1231,10 @@ int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, doub
         for (int r = 0; r < reps; ++r) {
             HIPCHK(hipEventRecord(ev[0], st));
             HIPCHK(launch_fill(st, sigma, p->vec_size, 0.0));
-            if (P.s_elems) HIPCHK(launch_fill(st, b.p[BASE_WS] + P.s_begin, P.s_elems, 0.0));
-            HIPCHK(hipEventRecord(ev[1], st));
-            HIPCHK(launch_gemm(st, P.d_t_tasks.as<GemmTask>(), uint32_t(P.t_tasks.size()), P.d_t_segs.as<GemmSeg>(),
-                               P.t_tiles, b));
-            HIPCHK(hipEventRecord(ev[2], st));
-            HIPCHK(launch_gather(st, P.d_regions.as<GatherRegion>(), uint32_t(P.regions.size()),
-                                 P.d_src_off.as<uint64_t>(), P.d_alpha.as<double>(), P.g_chunks, b.p[BASE_WS],
-                                 b.p[BASE_WS]));
-            HIPCHK(hipEventRecord(ev[3], st));
-            HIPCHK(launch_gemm(st, P.d_f_tasks.as<GemmTask>(), uint32_t(P.f_tasks.size()), P.d_f_segs.as<GemmSeg>(),
-                               P.f_tiles, b));
+            for (int k = 0; k < 4; ++k) {
+                P.stage(k, st, b);
+                HIPCHK(hipEventRecord(ev[k + 1], st));
+            }
             HIPCHK(hipEventRecord(ev[4], st));
             HIPCHK(hipEventSynchronize(ev[4]));
             for (int k = 0; k < 4; ++k) {
@@ -842,14 +1255,15 @@ int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, doub
             };
             ft = gemm_flops(P.t_tasks, P.t_segs);
             ff = gemm_flops(P.f_tasks, P.f_segs);
-            for (const auto& r : P.regions) fg += 2ull * r.R * r.C * (r.t1 - r.t0);
+            P.for_regions([&](const GatherRegion& r) { fg += 2ull * r.R * r.C * (r.t1 - r.t0); });
+            fg += gemm_flops(P.s_tasks, P.s_segs);
             flops_out[0] = ft;
             flops_out[1] = fg;
             flops_out[2] = ff;
         }
         if (gather_bytes) {
             uint64_t by = 0;
-            for (const auto& r : P.regions) by += 8ull * r.R * r.C * (uint64_t(r.t1 - r.t0) + 1);
+            P.for_regions([&](const GatherRegion& r) { by += 8ull * r.R * r.C * (uint64_t(r.t1 - r.t0) + 1); });
             *gather_bytes = by;
         }
     });
@@ -874,7 +1288,7 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
         MQ_REQUIRE(p && bytes, "bad argument");
         const StagePlan& P = p->st;
         *bytes = p->ws.bytes + P.d_t_tasks.bytes + P.d_t_segs.bytes + P.d_f_tasks.bytes + P.d_f_segs.bytes +
-                 P.d_regions.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
+                 P.d_regions.bytes + P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
     });
 }
 

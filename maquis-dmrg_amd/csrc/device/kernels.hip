@@ -26,7 +26,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* __restrict__ tasks,
                                                                 uint32_t n_tasks,
                                                                 const GemmSeg* __restrict__ segs,
-                                                                Bases bases) {
+                                                                Bases bases,
+                                                                const uint32_t* __restrict__ maps) {
     __shared__ double As[GEMM_BK][GEMM_BM + 1];
     __shared__ double Bs[GEMM_BK][GEMM_BN + 1];
 
@@ -55,6 +56,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* 
         const double* __restrict__ A = bases.p[sg.a_base] + sg.a_off;
         const double* __restrict__ B = bases.p[sg.b_base] + sg.b_off;
         const uint32_t K = sg.K;
+        const double alpha = sg.alpha;
         for (uint32_t k0 = 0; k0 < K; k0 += GEMM_BK) {
             if (!sg.trans_a) {
                 const int m = tid & 63;
@@ -62,7 +64,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* 
                 for (int i = 0; i < 4; ++i) {
                     const int k = (tid >> 6) + 4 * i;
                     const uint32_t gm = m0 + m, gk = k0 + k;
-                    As[k][m] = (gm < M && gk < K) ? A[uint64_t(gk) * sg.lda + gm] : 0.0;
+                    As[k][m] = (gm < M && gk < K) ? alpha * A[uint64_t(gk) * sg.lda + gm] : 0.0;
                 }
             } else {
                 const int k = tid & 15;
@@ -70,10 +72,25 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* 
                 for (int i = 0; i < 4; ++i) {
                     const int m = (tid >> 4) + 16 * i;
                     const uint32_t gm = m0 + m, gk = k0 + k;
-                    As[k][m] = (gm < M && gk < K) ? A[uint64_t(gm) * sg.lda + gk] : 0.0;
+                    As[k][m] = (gm < M && gk < K) ? alpha * A[uint64_t(gm) * sg.lda + gk] : 0.0;
                 }
             }
-            if (!sg.trans_b) {
+            if (sg.bmap != MAP_NONE) {
+                // row-block remapped B: row gk lives in block gk / bbh at column maps[bmap + block]
+                const int k = tid & 15;
+                const uint32_t gk = k0 + k;
+                const double* __restrict__ Bk = nullptr;
+                if (gk < K) {
+                    const uint32_t blk = gk / sg.bbh;
+                    Bk = B + uint64_t(maps[sg.bmap + blk]) * sg.bldk + (gk - blk * sg.bbh);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int n = (tid >> 4) + 16 * i;
+                    const uint32_t gn = n0 + n;
+                    Bs[k][n] = (gn < N && Bk) ? Bk[uint64_t(gn) * sg.ldb] : 0.0;
+                }
+            } else if (!sg.trans_b) {
                 const int k = tid & 15;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -112,12 +129,20 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* 
     for (int q = 0; q < 4; ++q) {
         const int i = q >> 1, j = q & 1;
         const uint32_t gn = n0 + wn + j * 16 + l16;
+        if (gn >= N) continue;
+        double* __restrict__ Cn;
+        if (tk.cmap != MAP_NONE) {
+            const uint32_t blk = gn / tk.cbw;
+            Cn = C + uint64_t(maps[tk.cmap + blk]) * tk.cldk + uint64_t(gn - blk * tk.cbw) * ldc;
+        } else {
+            Cn = C + uint64_t(gn) * ldc;
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const uint32_t gm = m0 + wm + i * 16 + 4 * r + lk;
-            if (gm < M && gn < N) {
+            if (gm < M) {
                 double v = (*accs[q])[r];
-                double* p = C + uint64_t(gn) * ldc + gm;
+                double* p = Cn + gm;
                 if (tk.beta) v += *p;
                 *p = v;
             }
@@ -126,9 +151,9 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* 
 }
 
 hipError_t launch_gemm(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                       uint32_t n_tiles, Bases bases) {
+                       uint32_t n_tiles, Bases bases, const uint32_t* maps) {
     if (n_tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(gemm_f64_kernel, dim3(n_tiles), dim3(GEMM_THREADS), 0, st, tasks, n_tasks, segs, bases);
+    hipLaunchKernelGGL(gemm_f64_kernel, dim3(n_tiles), dim3(GEMM_THREADS), 0, st, tasks, n_tasks, segs, bases, maps);
     return hipGetLastError();
 }
 
@@ -180,6 +205,51 @@ hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n
     if (n_chunks == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_kernel, dim3(n_chunks), dim3(GATHER_THREADS), 0, st, regions, n_regions, src_off,
                        alpha, ws, dst_base);
+    return hipGetLastError();
+}
+
+// Small regions (R*C <= GATHER_SMALL): one wavefront per region, four regions per workgroup.
+// Regions are listed contiguously; chunk_begin is unused.
+__global__ __launch_bounds__(256) void gather_small_kernel(const GatherRegion* __restrict__ regions,
+                                                           uint32_t n_regions,
+                                                           const uint64_t* __restrict__ src_off,
+                                                           const double* __restrict__ alpha,
+                                                           const double* __restrict__ ws,
+                                                           double* __restrict__ dst_base) {
+    const uint32_t rid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (rid >= n_regions) return;
+    const int lane = threadIdx.x & 63;
+    const GatherRegion rg = regions[rid];
+    const uint32_t total = rg.R * rg.C;
+    constexpr int PER = GATHER_SMALL / 64;
+    double acc[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) acc[j] = 0.0;
+    for (uint32_t t = rg.t0; t < rg.t1; ++t) {
+        const double a = alpha[t];
+        const double* __restrict__ src = ws + src_off[t];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t e = lane + 64 * j;
+            if (e < total) acc[j] += a * src[e];
+        }
+    }
+    double* __restrict__ dst = dst_base + rg.dst_off;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const uint32_t e = lane + 64 * j;
+        if (e < total) {
+            const uint32_t c = e / rg.R, r = e - c * rg.R;
+            dst[uint64_t(c) * rg.ld_dst + r] = acc[j];
+        }
+    }
+}
+
+hipError_t launch_gather_small(hipStream_t st, const GatherRegion* regions, uint32_t n_regions,
+                               const uint64_t* src_off, const double* alpha, double* ws, double* dst_base) {
+    if (n_regions == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_small_kernel, dim3((n_regions + 3) / 4), dim3(256), 0, st, regions, n_regions,
+                       src_off, alpha, ws, dst_base);
     return hipGetLastError();
 }
 

@@ -22,19 +22,30 @@ struct Bases {
     double* p[N_BASES];
 };
 
-// C (M x N, ldc) = beta*C + sum_seg op(A_seg) (M x K_seg) * op(B_seg) (K_seg x N)
+// C (M x N, ldc) = beta*C + sum_seg alpha_seg op(A_seg) (M x K_seg) * op(B_seg) (K_seg x N)
+//
+// Optional remaps (MQ_NONE = off):
+//  column-block store remap: column n of C is column n % cbw of block j = n / cbw, stored at
+//      c_off + maps[cmap + j] * cldk + (n % cbw) * ldc
+//  row-block load remap of B (non-transposed B only): row k of B is row k % bbh of block
+//      j = k / bbh, read at b_off + maps[bmap + j] * bldk + (k % bbh) + n * ldb
 struct GemmTask {
     uint64_t c_off;
+    uint64_t cldk;
     uint32_t M, N, ldc;
     uint32_t seg0, seg1;
     uint32_t tiles_n;
     uint32_t tile_begin;
+    uint32_t cmap, cbw;
     uint8_t c_base, beta, pad0, pad1;
 };
 
 struct GemmSeg {
     uint64_t a_off, b_off;
+    uint64_t bldk;
+    double alpha;
     uint32_t K, lda, ldb;
+    uint32_t bmap, bbh;
     uint8_t a_base, b_base, trans_a, trans_b;
 };
 
@@ -47,6 +58,7 @@ struct GatherRegion {
     uint32_t chunk_begin;
 };
 
+constexpr uint32_t MAP_NONE = 0xffffffffu;
 constexpr int GEMM_BM = 64;
 constexpr int GEMM_BN = 64;
 constexpr int GEMM_BK = 16;
@@ -54,9 +66,13 @@ constexpr int GEMM_THREADS = 256;
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
 constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions
+constexpr int GATHER_SMALL = 256;    // regions up to this many elements: one wavefront each
+
+hipError_t launch_gather_small(hipStream_t st, const GatherRegion* regions, uint32_t n_regions,
+                               const uint64_t* src_off, const double* alpha, double* ws, double* dst_base);
 
 hipError_t launch_gemm(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                       uint32_t n_tiles, Bases bases);
+                       uint32_t n_tiles, Bases bases, const uint32_t* maps);
 hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n_regions,
                          const uint64_t* src_off, const double* alpha, uint32_t n_chunks, double* ws,
                          double* dst_base);

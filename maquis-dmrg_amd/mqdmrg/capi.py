@@ -121,6 +121,7 @@ SIGNATURES = {
     "mq_schedule_flops": (C.c_int, [P, u64p]),
     "mq_schedule_destroy": (C.c_int, [P]),
     "mq_sigma_plan_create": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, PP]),
+    "mq_sigma_plan_create_ex": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, C.c_int, PP]),
     "mq_sigma_plan_create_from_schedule": (C.c_int, [P, C.POINTER(mq_schedule_desc), C.c_uint32,
                                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), P, P, PP]),
     "mq_sigma_apply": (C.c_int, [P, P, P]),

@@ -223,13 +223,16 @@ def build_schedule(kind, site, mpo_desc, bindex_l=None, bindex_r=None, symmetric
 
 
 # ---------------------------------------------------------------------------- sigma
+SIGMA_AUTO, SIGMA_SCHEDULE, SIGMA_GROUPED = 0, 1, 2
+
+
 class SigmaPlan:
-    def __init__(self, engine, site, mpo_desc, left, right):
+    def __init__(self, engine, site, mpo_desc, left, right, algorithm=SIGMA_AUTO):
         self.engine = engine
         self.left, self.right = left, right   # keep boundaries alive
         self.h = C.c_void_p()
-        check(lib().mq_sigma_plan_create(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), left.h, right.h,
-                                         C.byref(self.h)))
+        check(lib().mq_sigma_plan_create_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), left.h, right.h,
+                                            int(algorithm), C.byref(self.h)))
         n = C.c_uint64()
         check(lib().mq_plan_vector_size(self.h, C.byref(n)))
         self.n = n.value

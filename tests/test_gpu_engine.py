@@ -36,17 +36,24 @@ CASES = [
     ("heisenberg", 10, 8, 8, 3),     # right edge
     ("hubbard", 6, 2, 6, 4),
     ("hubbard", 8, 3, 10, 5),
+    ("bosehubbard", 8, 3, 6, 8),
+    ("qc", 6, 2, 8, 6),
+    ("qc", 8, 3, 10, 7),
 ]
 
 
+@pytest.mark.parametrize("algorithm", [E.SIGMA_SCHEDULE, E.SIGMA_GROUPED])
 @pytest.mark.parametrize("model,L,site,M,seed", CASES)
-def test_sigma_matches_oracle(eng, model, L, site, M, seed):
-    print("case", model, L, site, M, seed, flush=True)
+def test_sigma_matches_oracle(eng, model, L, site, M, seed, algorithm):
+    print("case", model, L, site, M, seed, algorithm, flush=True)
     p = SiteProblem(model, L=L, site=site, Mmax=M, seed=seed)
-    ref = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb).apply(p.psi)
+    so = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    ref = so.apply(p.psi)
     Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
-    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb, algorithm=algorithm)
     assert plan.n == p.layout.rp_total
+    if algorithm == E.SIGMA_SCHEDULE:
+        assert plan.flops == so.flops
     psi = dev(p.psi)
     out = torch.full_like(psi, np.nan)
     plan.apply(psi, out)
@@ -144,7 +151,7 @@ def test_vector_kernels(eng):
         np.testing.assert_array_equal(dx.cpu().numpy(), -2.0 * x)
 
 
-@pytest.mark.parametrize("model", ["heisenberg", "hubbard"])
+@pytest.mark.parametrize("model", ["heisenberg", "hubbard", "qc"])
 def test_jacobi_davidson_matches_oracle(eng, model):
     p = SiteProblem(model, L=8, site=3, Mmax=6, seed=21)
     so = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
