@@ -54,10 +54,6 @@ enum mq_symmetry {
     MQ_SYM_SU2U1 = 3   /* charge (N, 2S)              (block_matrix/symmetry/su2u1.h) */
 };
 
-enum mq_precision {
-    MQ_PREC_FP64 = 0   /* plain path: every stage in FP64 */
-};
-
 typedef struct { int32_t q[2]; } mq_charge;
 
 /* Index<SymmGroup>: (charge, size) pairs, in the caller's order (block_matrix/indexing_stable.hpp:97-258). */
@@ -167,6 +163,7 @@ typedef struct mq_schedule mq_schedule;
 /* ---------------------------------------------------------------- context */
 const char* mq_last_error(void);
 int mq_ctx_create(int device, mq_ctx** out);
+/* Plans and boundaries made on a context must be destroyed before it. */
 int mq_ctx_destroy(mq_ctx* ctx);
 int mq_ctx_synchronize(mq_ctx* ctx);
 /* Stream the context launches on (a hipStream_t); lets callers time with events on it. */
@@ -247,13 +244,22 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
                             const mq_boundary* left, const mq_boundary* right, int algorithm, mq_plan** out);
 /* sigma = H_eff * psi; psi and sigma are device pointers in the padded Davidson-vector layout. */
 int mq_sigma_apply(mq_plan* plan, const double* psi, double* sigma);
+/* Arithmetic of the sigma contraction.  psi, sigma, boundaries stay FP64 in memory in every mode.
+ *   F64     FP64 MFMA throughout (mq_sigma_apply).
+ *   F32     T = psi R, S = T W and sigma = L S on FP32 MFMA with FP32 accumulation; T and S are
+ *           held in FP32, the final accumulation into sigma is FP64.
+ *   BF16X3  as F32, each product formed from the bf16 split x = hi + lo as hi*hi + hi*lo + lo*hi.
+ * Reduced precisions need a GROUPED plan.  No reference counterpart: the reference's sigma is
+ * FP64 only (solver/tasks.cpp). */
+enum mq_precision { MQ_PREC_F64 = 0, MQ_PREC_F32 = 1, MQ_PREC_BF16X3 = 2 };
+int mq_sigma_apply_ex(mq_plan* plan, const double* psi, double* sigma, int precision);
 /* Measurement: runs `reps` sigma applications stage by stage with HIP events on the context's
  * stream and returns the average milliseconds of each stage: [0] S zero-fill, [1] T-stage GEMM,
  * [2] S gather, [3] final GEMM.  flops_out (may be NULL) receives the algorithmic flops of the
  * three compute stages (T GEMM, gather, final GEMM) and gather_bytes (may be NULL) the
  * algorithmic bytes of the gather (T slab reads + S writes). */
-int mq_plan_profile(mq_plan* plan, const double* psi, double* sigma, int reps, double* stage_ms,
-                    uint64_t* flops_out, uint64_t* gather_bytes);
+int mq_plan_profile(mq_plan* plan, const double* psi, double* sigma, int reps, int precision,
+                    double* stage_ms, uint64_t* flops_out, uint64_t* gather_bytes);
 int mq_plan_vector_size(const mq_plan* plan, uint64_t* n_elements);
 int mq_plan_flops(const mq_plan* plan, uint64_t* flops);
 /* Bytes of device workspace held by the plan (T and S buffers and tables). */
@@ -284,6 +290,17 @@ int mq_vec_copy(mq_ctx* ctx, uint64_t n, const double* x, double* y);
  * plain correction t = -r + (r.u)/(u.u) u (ietl/jacobi.h:150-156). */
 int mq_jd_solve(mq_plan* plan, const double* init, double* out, double tol, int max_iter,
                 int gmres_iter, double* energy, int* iterations);
+/* Jacobi-Davidson as iterative refinement over a reduced-precision sigma.  Each outer step runs
+ * the Jacobi-Davidson iteration above with sigma in `precision` (the first search vector of an
+ * outer step after the first carries its exact FP64 product), then recomputes the Ritz vector's
+ * product, Rayleigh quotient and residual H u - E u with the FP64 sigma; convergence is tested on
+ * that FP64 residual with the rule of ietl/iteration.h:56-58.  Orthogonalisation, subspace
+ * matrix and eigenproblem stay FP64.  max_iter bounds the reduced-precision sigma calls of one
+ * outer step, max_outer the outer steps.  counts (may be NULL) receives {reduced-precision sigma
+ * calls, FP64 sigma calls, outer steps}; residual (may be NULL) the final FP64 residual norm.
+ * precision == MQ_PREC_F64 gives the plain solver with one extra FP64 check per outer step. */
+int mq_jd_solve_refined(mq_plan* plan, const double* init, double* out, double tol, int max_iter,
+                        int precision, int max_outer, double* energy, int* counts, double* residual);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,7 @@ LIB = os.path.join(HERE, "libmqdmrg.so")
 SOURCES = [
     "csrc/host/schedule.cpp",
     "csrc/device/kernels.hip",
+    "csrc/device/gemm.hip",
     "csrc/device/engine.hip",
 ]
 

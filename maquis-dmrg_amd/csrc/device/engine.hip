@@ -250,30 +250,50 @@ struct StagePlan {
         upload(d_s_segs, s_segs, st);
         upload(d_maps, maps, st);
     }
-    // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM
-    void stage(int k, hipStream_t st, Bases b) const {
+    // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM.
+    // mode != MODE_F64 (grouped plans only): T_D, S_D and W_D are FP32 in a float workspace.
+    void stage(int k, hipStream_t st, Bases b, int mode = MODE_F64) const {
         const uint32_t* mp = d_maps.as<uint32_t>();
+        const bool mixed = mode != MODE_F64;
         switch (k) {
             case 0:
-                if (s_elems) HIPCHK(launch_fill(st, b.p[BASE_WS] + s_begin, s_elems, 0.0));
+                if (s_elems) {
+                    if (!mixed) HIPCHK(launch_fill(st, b.p[BASE_WS] + s_begin, s_elems, 0.0));
+                    else   // zero bits: the float range [s_begin, s_begin + s_elems) as doubles (both multiples of 32)
+                        HIPCHK(launch_fill(st, reinterpret_cast<double*>(reinterpret_cast<float*>(b.p[BASE_WS]) + s_begin),
+                                           s_elems / 2, 0.0));
+                }
                 break;
             case 1:
-                HIPCHK(launch_gemm(st, d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()), d_t_segs.as<GemmSeg>(),
-                                   t_tiles, b, mp));
+                if (!mixed)
+                    HIPCHK(launch_gemm(st, d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()), d_t_segs.as<GemmSeg>(),
+                                       t_tiles, b, mp));
+                else
+                    HIPCHK(launch_gemm_mixed(st, mode, "ddf", d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()),
+                                             d_t_segs.as<GemmSeg>(), t_tiles, b, mp));
                 break;
             case 2:
-                if (!s_tasks.empty())
-                    HIPCHK(launch_gemm(st, d_s_tasks.as<GemmTask>(), uint32_t(s_tasks.size()),
-                                       d_s_segs.as<GemmSeg>(), s_tiles, b, mp));
-                run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
+                if (!s_tasks.empty()) {
+                    if (!mixed)
+                        HIPCHK(launch_gemm(st, d_s_tasks.as<GemmTask>(), uint32_t(s_tasks.size()),
+                                           d_s_segs.as<GemmSeg>(), s_tiles, b, mp));
+                    else
+                        HIPCHK(launch_gemm_mixed(st, mode, "fff", d_s_tasks.as<GemmTask>(), uint32_t(s_tasks.size()),
+                                                 d_s_segs.as<GemmSeg>(), s_tiles, b, mp));
+                }
+                if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
             default:
-                HIPCHK(launch_gemm(st, d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()), d_f_segs.as<GemmSeg>(),
-                                   f_tiles, b, mp));
+                if (!mixed)
+                    HIPCHK(launch_gemm(st, d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()), d_f_segs.as<GemmSeg>(),
+                                       f_tiles, b, mp));
+                else
+                    HIPCHK(launch_gemm_mixed(st, mode, "dfd", d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()),
+                                             d_f_segs.as<GemmSeg>(), f_tiles, b, mp));
         }
     }
-    void run(hipStream_t st, Bases b) const {
-        for (int k = 0; k < 4; ++k) stage(k, st, b);
+    void run(hipStream_t st, Bases b, int mode = MODE_F64) const {
+        for (int k = 0; k < 4; ++k) stage(k, st, b, mode);
     }
     // gather of both region lists: src offsets are relative to src, destinations to dst
     void run_gather(hipStream_t st, double* src, double* dst) const {
@@ -289,6 +309,9 @@ struct StagePlan {
     }
 };
 
+static_assert(MQ_PREC_F64 == MODE_F64 && MQ_PREC_F32 == MODE_F32 && MQ_PREC_BF16X3 == MODE_BF16X3,
+              "precision enum and GEMM modes agree");
+
 struct mq_plan {
     mq_ctx* ctx = nullptr;
     StagePlan st;
@@ -298,9 +321,41 @@ struct mq_plan {
     uint64_t vec_size = 0;
     uint64_t flops = 0;
     std::vector<uint64_t> out_offsets;
+    // grouped plans: the dense MPO matrices W_D (host copy) at ws_init_off of the workspace, and the
+    // FP32 workspace of the reduced-precision modes (allocated on first use)
+    bool grouped = false;
+    std::vector<double> ws_init;
+    uint64_t ws_init_off = 0;
+    DevBuf ws32;
     // Jacobi-Davidson vectors (allocated on first use)
     DevBuf jd_buf;
     int jd_cap = 0;
+
+    // the base pointers of one application in the given mode
+    Bases bases(const double* psi, double* sigma, int mode) {
+        Bases b{};
+        if (mode == MODE_F64) {
+            b.p[BASE_WS] = ws.as<double>();
+        } else {
+            MQ_REQUIRE(grouped, "reduced-precision sigma needs a GROUPED plan");
+            if (!ws32.p) {
+                ws32.alloc(std::max<uint64_t>(st.ws_elems, 64) * sizeof(float));
+                if (!ws_init.empty()) {
+                    std::vector<float> w(ws_init.begin(), ws_init.end());
+                    HIPCHK(hipMemcpyAsync(ws32.as<float>() + ws_init_off, w.data(), w.size() * sizeof(float),
+                                          hipMemcpyHostToDevice, ctx->stream));
+                    HIPCHK(hipStreamSynchronize(ctx->stream));
+                }
+            }
+            b.p[BASE_WS] = reinterpret_cast<double*>(ws32.as<float>());
+        }
+        b.p[BASE_IN] = const_cast<double*>(psi);
+        b.p[BASE_OUT] = sigma;
+        b.p[BASE_LEFT] = left->ptr();
+        b.p[BASE_RIGHT] = right->ptr();
+        b.p[BASE_BRA] = nullptr;
+        return b;
+    }
 };
 
 // ============================================================================ lowering
@@ -1177,6 +1232,9 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
                                   hipMemcpyHostToDevice, ctx->stream));
         P.upload_all(ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        p->grouped = true;
+        p->ws_init = std::move(ws_init);
+        p->ws_init_off = ws_init_off;
         *out = p.release();
     });
 }
@@ -1196,35 +1254,32 @@ int mq_sigma_plan_create(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_des
     });
 }
 
-int mq_sigma_apply(mq_plan* p, const double* psi, double* sigma) {
+int mq_sigma_apply_ex(mq_plan* p, const double* psi, double* sigma, int precision) {
     return guarded([&] {
         MQ_REQUIRE(p && psi && sigma, "bad argument");
         MQ_REQUIRE(psi != sigma, "psi and sigma must not alias");
-        Bases b{};
-        b.p[BASE_WS] = p->ws.as<double>();
-        b.p[BASE_IN] = const_cast<double*>(psi);
-        b.p[BASE_OUT] = sigma;
-        b.p[BASE_LEFT] = p->left->ptr();
-        b.p[BASE_RIGHT] = p->right->ptr();
-        b.p[BASE_BRA] = nullptr;
+        MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
+                   "unknown precision");
+        Bases b = p->bases(psi, sigma, precision);
         HIPCHK(launch_fill(p->ctx->stream, sigma, p->vec_size, 0.0));
-        p->st.run(p->ctx->stream, b);
+        p->st.run(p->ctx->stream, b, precision);
     });
 }
 
-int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, double* stage_ms, uint64_t* flops_out,
-                    uint64_t* gather_bytes) {
+int mq_sigma_apply(mq_plan* p, const double* psi, double* sigma) {
+    return mq_sigma_apply_ex(p, psi, sigma, MQ_PREC_F64);
+}
+
+int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, int precision, double* stage_ms,
+                    uint64_t* flops_out, uint64_t* gather_bytes) {
     return guarded([&] {
         MQ_REQUIRE(p && psi && sigma && stage_ms && reps > 0, "bad argument");
         MQ_REQUIRE(psi != sigma, "psi and sigma must not alias");
+        MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
+                   "unknown precision");
         const StagePlan& P = p->st;
         hipStream_t st = p->ctx->stream;
-        Bases b{};
-        b.p[BASE_WS] = p->ws.as<double>();
-        b.p[BASE_IN] = const_cast<double*>(psi);
-        b.p[BASE_OUT] = sigma;
-        b.p[BASE_LEFT] = p->left->ptr();
-        b.p[BASE_RIGHT] = p->right->ptr();
+        Bases b = p->bases(psi, sigma, precision);
         hipEvent_t ev[5];
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         double acc[4] = {0, 0, 0, 0};
@@ -1232,7 +1287,7 @@ int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, doub
             HIPCHK(hipEventRecord(ev[0], st));
             HIPCHK(launch_fill(st, sigma, p->vec_size, 0.0));
             for (int k = 0; k < 4; ++k) {
-                P.stage(k, st, b);
+                P.stage(k, st, b, precision);
                 HIPCHK(hipEventRecord(ev[k + 1], st));
             }
             HIPCHK(hipEventRecord(ev[4], st));
@@ -1555,42 +1610,61 @@ extern "C" {
 
 // ietl::jacobi_davidson::calculate_eigenvalue (ietl/jacobi.h:386-474) with
 // basic_iteration(max_iter, tol, tol) (ietl/iteration.h:40-78, solver/ietl_jacobi_davidson.h:58-64).
-int mq_jd_solve(mq_plan* p, const double* init, double* out, double tol, int max_iter, int gmres_iter,
-                double* energy, int* iterations) {
-    return guarded([&] {
-        MQ_REQUIRE(p && init && out && energy && max_iter > 0, "bad argument");
-        MQ_REQUIRE(gmres_iter == 0, "GMRES correction is not implemented; use gmres_iter = 0");
-        mq_ctx* ctx = p->ctx;
-        const uint64_t n = p->vec_size;
-        const uint64_t stride = align32(n);
-        // V[0..max_iter], VA[0..max_iter-1], u, uA
-        int cap = 2 * max_iter + 3;
+}  // extern "C"
+
+namespace {
+
+// Device-side Jacobi-Davidson state of one plan: V[0..max_iter], VA[0..max_iter-1], u, uA and two
+// spare vectors (x, xA) used by the refinement loop.
+struct JdSpace {
+    mq_plan* p;
+    mq_ctx* ctx;
+    uint64_t n, stride;
+    int max_iter;
+    double* base;
+    JdSpace(mq_plan* p_, int max_iter_) : p(p_), ctx(p_->ctx), n(p_->vec_size), stride(align32(p_->vec_size)),
+                                          max_iter(max_iter_) {
+        int cap = 2 * max_iter + 5;
         if (p->jd_cap < cap) {
             p->jd_buf.alloc(uint64_t(cap) * stride * sizeof(double));
             p->jd_cap = cap;
         }
-        double* base = p->jd_buf.as<double>();
-        auto V = [&](int i) { return base + uint64_t(i) * stride; };
-        auto VA = [&](int i) { return base + uint64_t(max_iter + 1 + i) * stride; };
-        double* u = base + uint64_t(2 * max_iter + 1) * stride;
-        double* uA = base + uint64_t(2 * max_iter + 2) * stride;
-        auto dot = [&](const double* x, const double* y) {
-            double r;
-            int rc = mq_vec_dot(ctx, n, x, y, &r);
-            if (rc) fail(rc, g_err);
-            return r;
-        };
-        auto axpy = [&](double a, const double* x, double* y) { HIPCHK(launch_axpy(ctx->stream, n, a, x, y)); };
-        auto scal = [&](double a, double* x) { HIPCHK(launch_scal(ctx->stream, n, a, x)); };
-        auto copy = [&](const double* x, double* y) {
-            HIPCHK(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        };
+        base = p->jd_buf.as<double>();
+    }
+    double* V(int i) { return base + uint64_t(i) * stride; }
+    double* VA(int i) { return base + uint64_t(max_iter + 1 + i) * stride; }
+    double* u() { return base + uint64_t(2 * max_iter + 1) * stride; }
+    double* uA() { return base + uint64_t(2 * max_iter + 2) * stride; }
+    double* x() { return base + uint64_t(2 * max_iter + 3) * stride; }
+    double* xA() { return base + uint64_t(2 * max_iter + 4) * stride; }
+    double dot(const double* a, const double* b) {
+        double r;
+        int rc = mq_vec_dot(ctx, n, a, b, &r);
+        if (rc) fail(rc, g_err);
+        return r;
+    }
+    void axpy(double a, const double* x, double* y) { HIPCHK(launch_axpy(ctx->stream, n, a, x, y)); }
+    void scal(double a, double* x) { HIPCHK(launch_scal(ctx->stream, n, a, x)); }
+    void copy(const double* x, double* y) {
+        if (x != y) HIPCHK(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    void sigma(const double* x, double* y, int precision) {
+        int rc = mq_sigma_apply_ex(p, x, y, precision);
+        if (rc) fail(rc, g_err);
+    }
 
+    // One Jacobi-Davidson run for the smallest eigenpair (ietl/jacobi.h:386-474) with sigma in
+    // `precision`.  If init_A is given it is the product H*init, used for the first search vector
+    // in place of a sigma call.  Stops when ||r|| <= max(tol |theta|, stop_abs) or ||r|| < tol
+    // (ietl/iteration.h:56-58), or after max_iter search vectors.  On return u() holds the Ritz
+    // vector, uA() its residual uA - theta u; returns theta.
+    double run(const double* init, const double* init_A, double tol, double stop_abs, int precision, int* n_sigma,
+               int* iters, double* rnorm) {
         std::vector<double> M(size_t(max_iter) * max_iter, 0.0), s;
         const double kappa = 0.25;
         copy(init, V(0));
         int it = 0;
-        double theta = 0;
+        double theta = 0, rn = 0;
         while (true) {
             double* t = V(it);
             double tau = std::sqrt(dot(t, t));
@@ -1600,34 +1674,99 @@ int mq_jd_solve(mq_plan* p, const double* init, double* out, double tol, int max
             double nt = std::sqrt(dot(t, t));
             MQ_REQUIRE(nt > 0, "Jacobi-Davidson: zero search vector");
             scal(1.0 / nt, t);
-            int rc = mq_sigma_apply(p, t, VA(it));
-            if (rc) fail(rc, g_err);
-            for (int i = 0; i <= it; ++i) M[i + size_t(it) * max_iter] = dot(V(i), VA(it));
-            theta = smallest_eigenpair(M, max_iter, it + 1, s);
-            copy(V(0), u);
-            scal(s[0], u);
-            copy(VA(0), uA);
-            scal(s[0], uA);
-            for (int j = 1; j <= it; ++j) {
-                axpy(s[j], V(j), u);
-                axpy(s[j], VA(j), uA);
+            if (it == 0 && init_A) {
+                copy(init_A, VA(0));
+                scal(1.0 / nt, VA(0));
+            } else {
+                sigma(t, VA(it), precision);
+                ++*n_sigma;
             }
-            axpy(-theta, u, uA);   // r = uA - theta u
+            for (int i = 0; i <= it; ++i) M[i + size_t(it) * max_iter] = dot(V(i), VA(it));
+            // with an exact first product, the coupling of the first vector to the new one is taken
+            // from it: V(it).(H V(0)) is exact where V(0).(H~ V(it)) carries the reduced-precision
+            // error, and it is this coupling that sets the size of the update of V(0)
+            if (init_A && it > 0) M[size_t(it) * max_iter] = dot(V(it), VA(0));
+            theta = smallest_eigenpair(M, max_iter, it + 1, s);
+            copy(V(0), u());
+            scal(s[0], u());
+            copy(VA(0), uA());
+            scal(s[0], uA());
+            for (int j = 1; j <= it; ++j) {
+                axpy(s[j], V(j), u());
+                axpy(s[j], VA(j), uA());
+            }
+            axpy(-theta, u(), uA());   // r = uA - theta u
             ++it;
-            double rn = std::sqrt(dot(uA, uA));
-            bool conv = (rn <= tol * std::fabs(theta) || rn < tol);
+            rn = std::sqrt(dot(uA(), uA()));
+            bool conv = (rn <= std::max(tol * std::fabs(theta), stop_abs) || rn < tol);
             if (conv || it >= max_iter) break;
             // t = -r + (r.u)/(u.u) u   (jcd_gmres_solver with max_iter 0, ietl/jacobi.h:141-157)
-            double dru = dot(uA, u), duu = dot(u, u);
+            double dru = dot(uA(), u()), duu = dot(u(), u());
             double* tn = V(it);
-            copy(uA, tn);
+            copy(uA(), tn);
             scal(-1.0, tn);
-            axpy(dru / duu, u, tn);
+            axpy(dru / duu, u(), tn);
         }
-        copy(u, out);
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (iters) *iters = it;
+        if (rnorm) *rnorm = rn;
+        return theta;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mq_jd_solve(mq_plan* p, const double* init, double* out, double tol, int max_iter, int gmres_iter,
+                double* energy, int* iterations) {
+    return guarded([&] {
+        MQ_REQUIRE(p && init && out && energy && max_iter > 0, "bad argument");
+        MQ_REQUIRE(gmres_iter == 0, "GMRES correction is not implemented; use gmres_iter = 0");
+        JdSpace J(p, max_iter);
+        int ns = 0;
+        double theta = J.run(init, nullptr, tol, 0.0, MQ_PREC_F64, &ns, iterations, nullptr);
+        J.copy(J.u(), out);
+        HIPCHK(hipStreamSynchronize(p->ctx->stream));
         *energy = theta;
-        if (iterations) *iterations = it;
+    });
+}
+
+int mq_jd_solve_refined(mq_plan* p, const double* init, double* out, double tol, int max_iter, int precision,
+                        int max_outer, double* energy, int* counts, double* residual) {
+    return guarded([&] {
+        MQ_REQUIRE(p && init && out && energy && max_iter > 0 && max_outer > 0, "bad argument");
+        MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
+                   "unknown precision");
+        JdSpace J(p, max_iter);
+        int n_red = 0, n64 = 0, outer = 0;
+        double theta = 0, rn = 0;
+        const double* x_A = nullptr;
+        J.copy(init, J.x());
+        while (outer < max_outer) {
+            ++outer;
+            // inner run: stop once its residual is a decade below the last FP64 one; the FP64 step
+            // then measures the true residual (the reduced-precision products put a floor under the
+            // inner residual, below which inner steps gain nothing)
+            double stop_abs = (outer > 1) ? 1e-1 * rn : 0.0;
+            J.run(J.x(), x_A, tol, stop_abs, precision, &n_red, nullptr, nullptr);
+            // FP64 refinement step: exact product, Rayleigh quotient and residual of the Ritz vector
+            J.copy(J.u(), J.x());
+            double nx = std::sqrt(J.dot(J.x(), J.x()));
+            J.scal(1.0 / nx, J.x());
+            J.sigma(J.x(), J.xA(), MQ_PREC_F64);
+            ++n64;
+            x_A = J.xA();
+            theta = J.dot(J.x(), J.xA());
+            J.copy(J.xA(), J.uA());
+            J.axpy(-theta, J.x(), J.uA());
+            rn = std::sqrt(J.dot(J.uA(), J.uA()));
+            if (rn <= tol * std::fabs(theta) || rn < tol) break;
+        }
+        J.copy(J.x(), out);
+        HIPCHK(hipStreamSynchronize(p->ctx->stream));
+        *energy = theta;
+        if (counts) counts[0] = n_red, counts[1] = n64, counts[2] = outer;
+        if (residual) *residual = rn;
     });
 }
 

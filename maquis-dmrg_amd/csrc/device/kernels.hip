@@ -1,8 +1,6 @@
 // gfx950 kernels of the sigma / boundary engine.
 //
-//  gemm_f64      grouped, K-segmented FP64 GEMM on MFMA (v_mfma_f64_16x16x4f64); one launch
-//                covers every block product of a stage (the reference issues one cuBLAS call per
-//                product, solver/tasks.cpp:226-229, 277-280, 335-336, 630-639, 738-748).
+//  (the grouped MFMA GEMM lives in gemm.hip)
 //  gather        the MPO-term gather/scale of T slabs into S (reference kernels
 //                compute_s_stackedv / compute_s_left_stackedv, solver/numeric/gpu_dgemm_ddot.cu:215-359):
 //                every region is owned by one workgroup chunk and written exactly once, so no
@@ -13,149 +11,6 @@
 #include "kernels.hpp"
 
 namespace mq {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// ------------------------------------------------------------------------------- GEMM
-// Workgroup = 4 wavefronts = one 64x64 tile of C; wave w owns the 32x32 quadrant
-// (w&1, w>>1) as 2x2 MFMA 16x16 sub-tiles.  Operands are staged through LDS in
-// k-major order: As[k][m], Bs[k][n].
-//
-// v_mfma_f64_16x16x4f64 operand layout on gfx950 (lane l = 0..63, measured with one-hot operands):
-//   A: row l%16, k l/16      B: col l%16, k l/16      D[r]: row 4*r + l/16, col l%16
-__global__ __launch_bounds__(GEMM_THREADS) void gemm_f64_kernel(const GemmTask* __restrict__ tasks,
-                                                                uint32_t n_tasks,
-                                                                const GemmSeg* __restrict__ segs,
-                                                                Bases bases,
-                                                                const uint32_t* __restrict__ maps) {
-    __shared__ double As[GEMM_BK][GEMM_BM + 1];
-    __shared__ double Bs[GEMM_BK][GEMM_BN + 1];
-
-    const uint32_t tile = blockIdx.x;
-    uint32_t lo = 0, hi = n_tasks;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (tasks[mid].tile_begin <= tile) lo = mid; else hi = mid;
-    }
-    const GemmTask tk = tasks[lo];
-    const uint32_t local = tile - tk.tile_begin;
-    const uint32_t m0 = (local / tk.tiles_n) * GEMM_BM;
-    const uint32_t n0 = (local % tk.tiles_n) * GEMM_BN;
-    const uint32_t M = tk.M, N = tk.N;
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    const int l16 = lane & 15, lk = lane >> 4;
-
-    d4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
-
-    for (uint32_t sgi = tk.seg0; sgi < tk.seg1; ++sgi) {
-        const GemmSeg sg = segs[sgi];
-        const double* __restrict__ A = bases.p[sg.a_base] + sg.a_off;
-        const double* __restrict__ B = bases.p[sg.b_base] + sg.b_off;
-        const uint32_t K = sg.K;
-        const double alpha = sg.alpha;
-        for (uint32_t k0 = 0; k0 < K; k0 += GEMM_BK) {
-            if (!sg.trans_a) {
-                const int m = tid & 63;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int k = (tid >> 6) + 4 * i;
-                    const uint32_t gm = m0 + m, gk = k0 + k;
-                    As[k][m] = (gm < M && gk < K) ? alpha * A[uint64_t(gk) * sg.lda + gm] : 0.0;
-                }
-            } else {
-                const int k = tid & 15;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int m = (tid >> 4) + 16 * i;
-                    const uint32_t gm = m0 + m, gk = k0 + k;
-                    As[k][m] = (gm < M && gk < K) ? alpha * A[uint64_t(gm) * sg.lda + gk] : 0.0;
-                }
-            }
-            if (sg.bmap != MAP_NONE) {
-                // row-block remapped B: row gk lives in block gk / bbh at column maps[bmap + block]
-                const int k = tid & 15;
-                const uint32_t gk = k0 + k;
-                const double* __restrict__ Bk = nullptr;
-                if (gk < K) {
-                    const uint32_t blk = gk / sg.bbh;
-                    Bk = B + uint64_t(maps[sg.bmap + blk]) * sg.bldk + (gk - blk * sg.bbh);
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int n = (tid >> 4) + 16 * i;
-                    const uint32_t gn = n0 + n;
-                    Bs[k][n] = (gn < N && Bk) ? Bk[uint64_t(gn) * sg.ldb] : 0.0;
-                }
-            } else if (!sg.trans_b) {
-                const int k = tid & 15;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int n = (tid >> 4) + 16 * i;
-                    const uint32_t gn = n0 + n, gk = k0 + k;
-                    Bs[k][n] = (gn < N && gk < K) ? B[uint64_t(gn) * sg.ldb + gk] : 0.0;
-                }
-            } else {
-                const int n = tid & 63;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int k = (tid >> 6) + 4 * i;
-                    const uint32_t gn = n0 + n, gk = k0 + k;
-                    Bs[k][n] = (gn < N && gk < K) ? B[uint64_t(gk) * sg.ldb + gn] : 0.0;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kk = 0; kk < GEMM_BK; kk += 4) {
-                const int kr = kk + lk;
-                const double a0 = As[kr][wm + l16], a1 = As[kr][wm + 16 + l16];
-                const double b0 = Bs[kr][wn + l16], b1 = Bs[kr][wn + 16 + l16];
-                acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-                acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-                acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-                acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
-            }
-            __syncthreads();
-        }
-    }
-
-    double* __restrict__ C = bases.p[tk.c_base] + tk.c_off;
-    const uint64_t ldc = tk.ldc;
-    const d4* accs[4] = {&acc00, &acc01, &acc10, &acc11};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = q >> 1, j = q & 1;
-        const uint32_t gn = n0 + wn + j * 16 + l16;
-        if (gn >= N) continue;
-        double* __restrict__ Cn;
-        if (tk.cmap != MAP_NONE) {
-            const uint32_t blk = gn / tk.cbw;
-            Cn = C + uint64_t(maps[tk.cmap + blk]) * tk.cldk + uint64_t(gn - blk * tk.cbw) * ldc;
-        } else {
-            Cn = C + uint64_t(gn) * ldc;
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const uint32_t gm = m0 + wm + i * 16 + 4 * r + lk;
-            if (gm < M) {
-                double v = (*accs[q])[r];
-                double* p = Cn + gm;
-                if (tk.beta) v += *p;
-                *p = v;
-            }
-        }
-    }
-}
-
-hipError_t launch_gemm(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                       uint32_t n_tiles, Bases bases, const uint32_t* maps) {
-    if (n_tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(gemm_f64_kernel, dim3(n_tiles), dim3(GEMM_THREADS), 0, st, tasks, n_tasks, segs, bases, maps);
-    return hipGetLastError();
-}
 
 // ------------------------------------------------------------------------------- gather
 // One workgroup = GATHER_CHUNK consecutive elements of one region.  The (src, alpha) list of

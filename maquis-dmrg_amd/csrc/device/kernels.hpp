@@ -18,6 +18,8 @@ enum Base : uint8_t {
     N_BASES = 6
 };
 
+// Base pointers of one launch.  They are carried as double*; a GEMM reads each operand in its
+// own storage type (double or float, a template argument of the kernel).
 struct Bases {
     double* p[N_BASES];
 };
@@ -59,6 +61,15 @@ struct GatherRegion {
 };
 
 constexpr uint32_t MAP_NONE = 0xffffffffu;
+enum GemmMode { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
+
+// MFMA output-row / bf16 k-slot layouts on gfx950 (see gemm.hip)
+#define MQ_F32_DROW(r, lk) (4 * (lk) + (r))
+#define MQ_BF16_DROW(r, lk) (4 * (lk) + (r))
+#define MQ_BF16_K(lk, j) (8 * (lk) + (j))
+
+hipError_t launch_gemm_mixed(hipStream_t st, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
+                             const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps);
 constexpr int GEMM_BM = 64;
 constexpr int GEMM_BN = 64;
 constexpr int GEMM_BK = 16;

@@ -125,7 +125,8 @@ SIGNATURES = {
     "mq_sigma_plan_create_from_schedule": (C.c_int, [P, C.POINTER(mq_schedule_desc), C.c_uint32,
                                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), P, P, PP]),
     "mq_sigma_apply": (C.c_int, [P, P, P]),
-    "mq_plan_profile": (C.c_int, [P, P, P, C.c_int, C.POINTER(C.c_double), u64p, u64p]),
+    "mq_sigma_apply_ex": (C.c_int, [P, P, P, C.c_int]),
+    "mq_plan_profile": (C.c_int, [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_double), u64p, u64p]),
     "mq_plan_vector_size": (C.c_int, [P, u64p]),
     "mq_plan_flops": (C.c_int, [P, u64p]),
     "mq_plan_workspace_bytes": (C.c_int, [P, u64p]),
@@ -138,6 +139,8 @@ SIGNATURES = {
     "mq_vec_scal": (C.c_int, [P, C.c_uint64, C.c_double, P]),
     "mq_vec_copy": (C.c_int, [P, C.c_uint64, P, P]),
     "mq_jd_solve": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "mq_jd_solve_refined": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_double)]),
 }
 
 _lib = None

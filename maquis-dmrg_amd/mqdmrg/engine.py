@@ -2,6 +2,7 @@
 steps and the Jacobi-Davidson solver.  Device memory of site tensors is held in torch tensors on
 the engine's own stream (torch is plumbing here: allocation and small dense algebra)."""
 import ctypes as C
+import weakref
 
 import numpy as np
 import torch
@@ -21,12 +22,20 @@ class Engine:
         self.torch_device = torch.device("cuda:%d" % device)
         # every torch op of the caller is ordered with the engine's launches on one stream
         torch.cuda.set_stream(self.stream)
+        # boundaries and plans made on this context: freed before the context itself
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        self._children.add(obj)
+        return obj
 
     def sync(self):
         check(lib().mq_ctx_synchronize(self.ctx))
 
     def close(self):
         if self.ctx:
+            for obj in list(self._children):
+                obj.close()
             lib().mq_ctx_destroy(self.ctx)
             self.ctx = C.c_void_p()
 
@@ -141,6 +150,7 @@ class DeviceBoundary:
     def __init__(self, engine, handle):
         self.engine = engine
         self.h = handle
+        engine.adopt(self)
         d = capi.mq_bindex_desc()
         check(lib().mq_boundary_index(self.h, C.byref(d)))
         self.index = capi.BIndex.from_desc(d)
@@ -224,11 +234,16 @@ def build_schedule(kind, site, mpo_desc, bindex_l=None, bindex_r=None, symmetric
 
 # ---------------------------------------------------------------------------- sigma
 SIGMA_AUTO, SIGMA_SCHEDULE, SIGMA_GROUPED = 0, 1, 2
+# arithmetic of the sigma contraction (mq_precision)
+PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
 
 
 class SigmaPlan:
     def __init__(self, engine, site, mpo_desc, left, right, algorithm=SIGMA_AUTO):
         self.engine = engine
+        self.h = None
+        engine.adopt(self)
         self.left, self.right = left, right   # keep boundaries alive
         self.h = C.c_void_p()
         check(lib().mq_sigma_plan_create_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), left.h, right.h,
@@ -247,6 +262,8 @@ class SigmaPlan:
     def from_schedule(engine, sched, ket_rows, ket_cols, left, right):
         self = SigmaPlan.__new__(SigmaPlan)
         self.engine, self.left, self.right = engine, left, right
+        self.h = None
+        engine.adopt(self)
         keep = []
         d = capi.schedule_from_dict(sched, keep)
         rows = np.ascontiguousarray(ket_rows, np.uint32)
@@ -263,17 +280,18 @@ class SigmaPlan:
         self.workspace_bytes = 0
         return self
 
-    def apply(self, psi, sigma):
+    def apply(self, psi, sigma, precision=PREC_F64):
         assert psi.numel() >= self.n and sigma.numel() >= self.n
-        check(lib().mq_sigma_apply(self.h, psi.data_ptr(), sigma.data_ptr()))
+        check(lib().mq_sigma_apply_ex(self.h, psi.data_ptr(), sigma.data_ptr(), int(precision)))
 
-    def profile(self, psi, sigma, reps=5):
+    def profile(self, psi, sigma, reps=5, precision=PREC_F64):
         """Average ms per stage [fill, T GEMM, gather, final GEMM] and the algorithmic flops of
         the three compute stages and bytes of the gather."""
         ms = (C.c_double * 4)()
         fl = (C.c_uint64 * 3)()
         gb = C.c_uint64()
-        check(lib().mq_plan_profile(self.h, psi.data_ptr(), sigma.data_ptr(), int(reps), ms, fl, C.byref(gb)))
+        check(lib().mq_plan_profile(self.h, psi.data_ptr(), sigma.data_ptr(), int(reps), int(precision), ms, fl,
+                                    C.byref(gb)))
         return list(ms), list(fl), gb.value
 
     def jd(self, init, out, tol=1e-8, max_iter=8):
@@ -282,6 +300,16 @@ class SigmaPlan:
         check(lib().mq_jd_solve(self.h, init.data_ptr(), out.data_ptr(), float(tol), int(max_iter), 0, C.byref(e),
                                 C.byref(it)))
         return e.value, it.value
+
+    def jd_refined(self, init, out, tol=1e-8, max_iter=8, precision=PREC_F32, max_outer=4):
+        """Jacobi-Davidson with sigma in `precision` and an FP64 residual each outer step.
+        Returns (energy, {"reduced": n, "fp64": n, "outer": n}, final FP64 residual norm)."""
+        e = C.c_double()
+        cnt = (C.c_int * 3)()
+        r = C.c_double()
+        check(lib().mq_jd_solve_refined(self.h, init.data_ptr(), out.data_ptr(), float(tol), int(max_iter),
+                                        int(precision), int(max_outer), C.byref(e), cnt, C.byref(r)))
+        return e.value, {"reduced": cnt[0], "fp64": cnt[1], "outer": cnt[2]}, r.value
 
     def close(self):
         if self.h:

@@ -165,3 +165,82 @@ def test_jacobi_davidson_matches_oracle(eng, model):
     assert abs(e - e_ref) <= 1e-10 * max(1.0, abs(e_ref))
     v = out.cpu().numpy()
     assert min(np.linalg.norm(v - v_ref), np.linalg.norm(v + v_ref)) <= 1e-8
+
+
+# Reduced-precision sigma: FP32 products carry ~1e-7 relative rounding per term, the bf16 split
+# x = hi + lo leaves ~2^-17 relative per operand; sums over a few hundred terms stay well inside
+# these bounds.
+MIXED_RTOL = {E.PREC_F32: 1e-5, E.PREC_BF16X3: 1e-4}
+
+
+@pytest.mark.parametrize("precision", [E.PREC_F32, E.PREC_BF16X3])
+@pytest.mark.parametrize("model,L,site,M,seed", [("hubbard", 8, 3, 10, 5), ("qc", 8, 3, 10, 7),
+                                                  ("heisenberg", 10, 0, 8, 2)])
+def test_mixed_sigma_error(eng, model, L, site, M, seed, precision):
+    print("case", model, L, site, M, seed, precision, flush=True)
+    p = SiteProblem(model, L=L, site=site, Mmax=M, seed=seed)
+    ref = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb).apply(p.psi)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    psi = dev(p.psi)
+    out = torch.full_like(psi, np.nan)
+    plan.apply(psi, out, precision=precision)
+    eng.sync()
+    got = out.cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print("relative error", err, flush=True)
+    assert err <= MIXED_RTOL[precision], err
+    assert err > 1e-13   # the reduced-precision kernels ran, not the FP64 ones
+    # the FP64 path of the same plan is unaffected by the FP32 workspace
+    plan.apply(psi, out)
+    eng.sync()
+    assert np.linalg.norm(out.cpu().numpy() - ref) <= SIGMA_RTOL * np.linalg.norm(ref)
+
+
+def test_mixed_precision_rejected_on_schedule_plan(eng):
+    p = SiteProblem("hubbard", L=6, site=2, Mmax=6, seed=4)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb, algorithm=E.SIGMA_SCHEDULE)
+    psi = dev(p.psi)
+    out = torch.zeros_like(psi)
+    with pytest.raises(RuntimeError, match="GROUPED"):
+        plan.apply(psi, out, precision=E.PREC_F32)
+
+
+def _dense_lowest(so, layout):
+    """Lowest eigenvalue of H_eff assembled column by column from the oracle's sigma, over the
+    stored (non-padding) entries of the padded layout."""
+    live = np.concatenate([off + np.arange(ls * c) for off, ls, c in
+                           zip(layout.rp_offset, layout.left.sizes, layout.rp_cols)]).astype(np.int64)
+    H = np.empty((live.size, live.size))
+    e = np.zeros(layout.rp_total)
+    for j, k in enumerate(live):
+        e[k] = 1.0
+        H[:, j] = so.apply(e)[live]
+        e[k] = 0.0
+    return np.linalg.eigvalsh(0.5 * (H + H.T))[0]
+
+
+@pytest.mark.parametrize("precision", [E.PREC_F32, E.PREC_BF16X3])
+@pytest.mark.parametrize("model", ["hubbard", "qc"])
+def test_refined_jd_reaches_fp64_energy(eng, model, precision):
+    """Iterative refinement over the reduced-precision sigma converges to the FP64 eigenvalue."""
+    p = SiteProblem(model, L=8, site=3, Mmax=6, seed=21)
+    so = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    assert p.layout.rp_total <= 4096
+    e_exact = _dense_lowest(so, p.layout)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    psi = dev(p.psi)
+    out = torch.zeros_like(psi)
+    tol = 1e-9
+    e, counts, rn = plan.jd_refined(psi, out, tol=tol, max_iter=12, precision=precision, max_outer=8)
+    print("refined", model, precision, e, e_exact, counts, rn, flush=True)
+    assert rn <= tol * abs(e) or rn < tol
+    # these boundaries come from random MPS, so the spectrum's scale is arbitrary: relative bar
+    assert abs(e - e_exact) <= 1e-9 * max(1.0, abs(e_exact))
+    assert counts["fp64"] == counts["outer"] and counts["reduced"] >= counts["fp64"]
+    # the returned vector is the normalised Ritz vector whose FP64 Rayleigh quotient is e
+    v = out.cpu().numpy()
+    assert abs(np.linalg.norm(v) - 1.0) <= 1e-12
+    assert abs(v @ so.apply(v) - e) <= 1e-10 * max(1.0, abs(e))
