@@ -216,6 +216,14 @@ int mq_schedule_build_left(const mq_site_desc* site, const mq_mpo_desc* mpo, con
 /* Schedule of overlap_mpo_right_step (move_boundary.hpp:281-337; right_tasks.hpp:39-167). */
 int mq_schedule_build_right(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* right,
                             int symmetric, mq_schedule** out);
+/* Index-only forms of the two step schedules: the result holds the new boundary's index
+ * (mq_schedule_out_index, identical to the full builders') and no task lists.  They let a caller
+ * lay out boundaries along a chain without building the steps' schedules.  No reference
+ * counterpart (the reference always builds the full schedule, move_boundary.hpp:147-404). */
+int mq_schedule_build_left_index(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* left,
+                                 int symmetric, mq_schedule** out);
+int mq_schedule_build_right_index(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* right,
+                                  int symmetric, mq_schedule** out);
 /* Index of the boundary a left/right-step schedule produces (BoundaryIndex after add_cohort and
  * complement_transpose, move_boundary.hpp:202-209, 331-337). */
 int mq_schedule_out_index(const mq_schedule* s, mq_bindex_desc* out);

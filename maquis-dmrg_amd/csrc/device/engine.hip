@@ -127,6 +127,45 @@ struct mq_schedule {
 };
 
 // Lowered stage tables (device) + the host copies used for validation.
+// One GEMM stage on the device: the lowering's tasks split by tile configuration (one launch
+// each), sharing one segment table.
+struct GemmStage {
+    std::vector<GemmTask> cfg[GEMM_N_CFG];
+    uint32_t tiles[GEMM_N_CFG] = {};
+    DevBuf d_tasks[GEMM_N_CFG];
+    DevBuf d_segs;
+
+    void build(const std::vector<GemmTask>& tasks) {
+        for (auto& c : cfg) c.clear();
+        for (const auto& t : tasks) cfg[gemm_pick_cfg(t.M, t.N)].push_back(t);
+        for (int c = 0; c < GEMM_N_CFG; ++c) {
+            uint64_t tile = 0;
+            for (auto& t : cfg[c]) {
+                t.tiles_n = (t.N + GEMM_CFG_BN[c] - 1) / GEMM_CFG_BN[c];
+                t.tile_begin = uint32_t(tile);
+                tile += uint64_t((t.M + GEMM_CFG_BM[c] - 1) / GEMM_CFG_BM[c]) * t.tiles_n;
+                MQ_REQUIRE(tile < (1ull << 31), "too many GEMM tiles in one launch");
+            }
+            tiles[c] = uint32_t(tile);
+        }
+    }
+    void upload_all(const std::vector<GemmSeg>& segs, hipStream_t st) {
+        upload(d_segs, segs, st);
+        for (int c = 0; c < GEMM_N_CFG; ++c) upload(d_tasks[c], cfg[c], st);
+    }
+    void run(hipStream_t st, int mode, const char* types, Bases b, const uint32_t* maps) const {
+        for (int c = 0; c < GEMM_N_CFG; ++c)
+            if (tiles[c])
+                HIPCHK(launch_gemm(st, c, mode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
+                                   d_segs.as<GemmSeg>(), tiles[c], b, maps));
+    }
+    uint64_t device_bytes() const {
+        uint64_t n = d_segs.bytes;
+        for (const auto& d : d_tasks) n += d.bytes;
+        return n;
+    }
+};
+
 struct StagePlan {
     std::vector<GemmTask> t_tasks, f_tasks;
     std::vector<GemmSeg> t_segs, f_segs;
@@ -138,28 +177,17 @@ struct StagePlan {
     std::vector<GemmTask> s_tasks;
     std::vector<GemmSeg> s_segs;
     std::vector<uint32_t> maps;             // column / row block remaps of the GEMM stages
-    uint32_t t_tiles = 0, f_tiles = 0, s_tiles = 0, g_chunks = 0;
+    GemmStage gT, gS, gF;   // device form of the three GEMM stages
+    uint32_t g_chunks = 0;
     uint64_t ws_elems = 0;
     uint64_t s_begin = 0, s_elems = 0;   // workspace range zero-filled at the start of each apply
-    DevBuf d_t_tasks, d_t_segs, d_f_tasks, d_f_segs, d_s_tasks, d_s_segs, d_maps, d_regions, d_small, d_src_off,
-        d_alpha;
+    DevBuf d_maps, d_regions, d_small, d_src_off, d_alpha;
     uint64_t sizes[N_BASES] = {0, 0, 0, 0, 0, 0};   // element counts of each base for validation
 
-    static uint32_t tile_tasks(std::vector<GemmTask>& tasks) {
-        uint64_t tile = 0;
-        for (auto& t : tasks) {
-            t.tiles_n = (t.N + GEMM_BN - 1) / GEMM_BN;
-            t.tile_begin = uint32_t(tile);
-            tile += uint64_t((t.M + GEMM_BM - 1) / GEMM_BM) * t.tiles_n;
-            MQ_REQUIRE(tile < (1ull << 31), "too many GEMM tiles in one launch");
-        }
-        return uint32_t(tile);
-    }
-
     void finish_tables() {
-        t_tiles = tile_tasks(t_tasks);
-        f_tiles = tile_tasks(f_tasks);
-        s_tiles = tile_tasks(s_tasks);
+        gT.build(t_tasks);
+        gS.build(s_tasks);
+        gF.build(f_tasks);
         std::vector<GatherRegion> big;
         for (const auto& r : regions) {
             if (uint64_t(r.R) * r.C <= uint64_t(GATHER_SMALL)) small.push_back(r);
@@ -238,16 +266,13 @@ struct StagePlan {
             }
     }
     void upload_all(hipStream_t st) {
-        upload(d_t_tasks, t_tasks, st);
-        upload(d_t_segs, t_segs, st);
-        upload(d_f_tasks, f_tasks, st);
-        upload(d_f_segs, f_segs, st);
+        gT.upload_all(t_segs, st);
+        gS.upload_all(s_segs, st);
+        gF.upload_all(f_segs, st);
         upload(d_regions, regions, st);
         upload(d_small, small, st);
         upload(d_src_off, src_off, st);
         upload(d_alpha, alpha, st);
-        upload(d_s_tasks, s_tasks, st);
-        upload(d_s_segs, s_segs, st);
         upload(d_maps, maps, st);
     }
     // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM.
@@ -265,31 +290,14 @@ struct StagePlan {
                 }
                 break;
             case 1:
-                if (!mixed)
-                    HIPCHK(launch_gemm(st, d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()), d_t_segs.as<GemmSeg>(),
-                                       t_tiles, b, mp));
-                else
-                    HIPCHK(launch_gemm_mixed(st, mode, "ddf", d_t_tasks.as<GemmTask>(), uint32_t(t_tasks.size()),
-                                             d_t_segs.as<GemmSeg>(), t_tiles, b, mp));
+                gT.run(st, mode, "ddf", b, mp);
                 break;
             case 2:
-                if (!s_tasks.empty()) {
-                    if (!mixed)
-                        HIPCHK(launch_gemm(st, d_s_tasks.as<GemmTask>(), uint32_t(s_tasks.size()),
-                                           d_s_segs.as<GemmSeg>(), s_tiles, b, mp));
-                    else
-                        HIPCHK(launch_gemm_mixed(st, mode, "fff", d_s_tasks.as<GemmTask>(), uint32_t(s_tasks.size()),
-                                                 d_s_segs.as<GemmSeg>(), s_tiles, b, mp));
-                }
+                gS.run(st, mode, "fff", b, mp);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
             default:
-                if (!mixed)
-                    HIPCHK(launch_gemm(st, d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()), d_f_segs.as<GemmSeg>(),
-                                       f_tiles, b, mp));
-                else
-                    HIPCHK(launch_gemm_mixed(st, mode, "dfd", d_f_tasks.as<GemmTask>(), uint32_t(f_tasks.size()),
-                                             d_f_segs.as<GemmSeg>(), f_tiles, b, mp));
+                gF.run(st, mode, "dfd", b, mp);
         }
     }
     void run(hipStream_t st, Bases b, int mode = MODE_F64) const {
@@ -780,6 +788,36 @@ int mq_schedule_build_right(const mq_site_desc* site, const mq_mpo_desc* mpo, co
     });
 }
 
+int mq_schedule_build_left_index(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* left,
+                                 int symmetric, mq_schedule** out) {
+    return guarded([&] {
+        MQ_REQUIRE(site && mpo && left && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        BoundaryIndex LI(*left);
+        auto s = std::make_unique<mq_schedule>();
+        s->s.kind = Schedule::LEFT;
+        s->s.aux_dim = W.col_dim;
+        s->s.out_index = left_step_index(L, W, LI, symmetric != 0);
+        *out = s.release();
+    });
+}
+
+int mq_schedule_build_right_index(const mq_site_desc* site, const mq_mpo_desc* mpo, const mq_bindex_desc* right,
+                                  int symmetric, mq_schedule** out) {
+    return guarded([&] {
+        MQ_REQUIRE(site && mpo && right && out, "bad argument");
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        BoundaryIndex RI(*right);
+        auto s = std::make_unique<mq_schedule>();
+        s->s.kind = Schedule::RIGHT;
+        s->s.aux_dim = W.row_dim;
+        s->s.out_index = right_step_index(L, W, RI, symmetric != 0);
+        *out = s.release();
+    });
+}
+
 int mq_schedule_out_index(const mq_schedule* s, mq_bindex_desc* out) {
     return guarded([&] {
         MQ_REQUIRE(s && out, "bad argument");
@@ -962,6 +1000,25 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             G.pairs.push_back(std::move(pr));
         }
     }
+    // ---- order each group's keys by (b, s, ss): W_D then follows the MPO's block structure, and
+    // its all-zero tiles are skipped in the S GEMM
+    for (auto& g : G.groups) {
+        auto reorder = [](std::vector<uint64_t>& keys, std::unordered_map<uint64_t, uint32_t>& idx) {
+            std::vector<uint64_t> sorted = keys;
+            std::sort(sorted.begin(), sorted.end());
+            idx.clear();
+            for (uint32_t i = 0; i < sorted.size(); ++i) idx[sorted[i]] = i;
+            std::vector<uint32_t> perm(keys.size());
+            for (uint32_t i = 0; i < keys.size(); ++i) perm[i] = idx[keys[i]];
+            keys.swap(sorted);
+            return perm;
+        };
+        std::vector<uint32_t> pk = reorder(g.K, g.kidx), pn = reorder(g.N, g.nidx);
+        for (uint32_t pi : g.pairs) {
+            for (auto& k : G.pairs[pi].kk) k = pk[k];
+            for (auto& n : G.pairs[pi].nn) n = pn[n];
+        }
+    }
     // ---- workspace: T_D, S_D, W_D per group
     uint64_t ws = 0;
     for (auto& g : G.groups) {
@@ -1078,11 +1135,37 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             }
         }
     }
-    // ---- S stage: S_D = T_D W_D
+    // ---- S stage: S_D = T_D W_D, one task per 128-column slice of S_D whose K segments are the
+    // runs of 16-row chunks of W_D holding a nonzero in that slice (all-zero chunks add nothing)
     for (auto& g : G.groups) {
-        uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
-        add_gemm(P.s_tasks, P.s_segs, BASE_WS, g.s_off, uint32_t(g.M), Ng, uint32_t(g.M),
-                 {seg(BASE_WS, g.t_off, uint32_t(g.M), false, BASE_WS, g.w_off, Kg, false, Kg)});
+        const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size()), Mg = uint32_t(g.M);
+        const double* Wg = ws_init.data() + (g.w_off - ws_init_off);
+        const uint32_t NT = 128, KC = 16;
+        for (uint32_t n0 = 0; n0 < Ng; n0 += NT) {
+            const uint32_t nw = std::min(NT, Ng - n0);
+            std::vector<GemmSeg> segs;
+            uint32_t run0 = MQ_NONE;
+            for (uint32_t k0 = 0; k0 < Kg + KC; k0 += KC) {
+                bool nz = false;
+                for (uint32_t n = n0; n < n0 + nw && !nz && k0 < Kg; ++n)
+                    for (uint32_t k = k0; k < std::min(k0 + KC, Kg); ++k)
+                        if (Wg[uint64_t(n) * Kg + k] != 0.0) {
+                            nz = true;
+                            break;
+                        }
+                if (nz && run0 == MQ_NONE) run0 = k0;
+                if (!nz && run0 != MQ_NONE) {
+                    const uint32_t k1 = std::min(k0, Kg);
+                    segs.push_back(seg(BASE_WS, g.t_off + uint64_t(run0) * Mg, Mg, false, BASE_WS,
+                                       g.w_off + uint64_t(n0) * Kg + run0, Kg, false, k1 - run0));
+                    run0 = MQ_NONE;
+                }
+            }
+            // an all-zero slice still has its S_D columns written (as zeros): one empty segment
+            if (segs.empty()) segs.push_back(seg(BASE_WS, g.t_off, Mg, false, BASE_WS, g.w_off + uint64_t(n0) * Kg, Kg,
+                                                 false, 0));
+            add_gemm(P.s_tasks, P.s_segs, BASE_WS, g.s_off + uint64_t(n0) * Mg, Mg, nw, Mg, segs);
+        }
     }
     // ---- final stage: per output unit (lb_out, s_out, ss2), K segments over (lc_in, b1)
     for (uint32_t lbo = 0; lbo < left_i.size(); ++lbo) {
@@ -1342,8 +1425,8 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
     return guarded([&] {
         MQ_REQUIRE(p && bytes, "bad argument");
         const StagePlan& P = p->st;
-        *bytes = p->ws.bytes + P.d_t_tasks.bytes + P.d_t_segs.bytes + P.d_f_tasks.bytes + P.d_f_segs.bytes +
-                 P.d_regions.bytes + P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
+        *bytes = p->ws.bytes + p->ws32.bytes + P.gT.device_bytes() + P.gS.device_bytes() + P.gF.device_bytes() +
+                 P.d_maps.bytes + P.d_regions.bytes + P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
     });
 }
 

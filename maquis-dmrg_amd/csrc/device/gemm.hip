@@ -1,18 +1,28 @@
-// Grouped, K-segmented GEMM on gfx950 MFMA, templated on operand storage types and compute mode.
+// Grouped, K-segmented GEMM on gfx950 MFMA, templated on operand storage types, compute mode and
+// tile shape.
 //
 //   MODE_F64    v_mfma_f64_16x16x4f64, FP64 operands and accumulation (the plain path)
 //   MODE_F32    v_mfma_f32_16x16x4f32, FP32 operands and accumulation
 //   MODE_BF16X3 v_mfma_f32_16x16x32_bf16 on a two-term split of each FP32 operand,
 //               x = hi + lo (hi, lo bf16): a*b ~ a_hi*b_hi + a_hi*b_lo + a_lo*b_hi, FP32 accumulation
 //
-// Operands are read in their storage type (TA, TB: double or float) and converted when staged
-// into LDS; C is written in TC.  One launch covers every tile of every task (the reference issues
-// one cuBLAS call per block product, solver/tasks.cpp:226-229, 277-280, 335-336, 630-639, 738-748).
-// Workgroup = 4 wavefronts = one 64x64 tile of C; wave w owns the 32x32 quadrant (w&1, w>>1) as
-// 2x2 MFMA 16x16 sub-tiles.
+// Operands are read in their storage type (TA, TB: double or float) and converted when staged;
+// C is written in TC.  One launch covers every tile of every task of one tile configuration (the
+// reference issues one cuBLAS call per block product, solver/tasks.cpp:226-229, 277-280,
+// 335-336, 630-639, 738-748).  The symmetry blocks of a contraction range from a few rows to
+// thousands, so each task is run with the tile configuration that fits its (M, N):
 //
-// MFMA lane layouts on gfx950 (lane l = 0..63), measured with one-hot operands:
-//   f64 16x16x4 : A row l%16, k l/16;  B col l%16, k l/16;  D[r] row 4*r + l/16, col l%16
+//   GEMM_CFG_128x128  4 waves (2x2), each 64x64  = 4x4 MFMA blocks   large blocks
+//   GEMM_CFG_64x64    4 waves (2x2), each 32x32  = 2x2 MFMA blocks
+//   GEMM_CFG_32x128   4 waves (1x4), each 32x32                      M <= 32
+//   GEMM_CFG_16x128   4 waves (1x4), each 16x32  = 1x2 MFMA blocks   M <= 16
+//   GEMM_CFG_16x16    4 waves split K of one 16x16 tile, operands read straight from memory
+//                     in the MFMA lane layout, partial sums reduced through LDS    M, N <= 16
+//
+// MFMA lane layouts on gfx950 (lane l = 0..63, lk = l/16), measured with one-hot operands:
+//   f64 16x16x4      : A row l%16, k lk;            B col l%16, k lk;   D[r] row 4r + lk, col l%16
+//   f32 16x16x4      : A row l%16, k lk;            B col l%16, k lk;   D[r] row 4lk + r, col l%16
+//   bf16 16x16x32    : A row l%16, k 8lk + j (j<8); B col l%16, same k; D[r] row 4lk + r, col l%16
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -24,15 +34,12 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 
 template <int MODE> struct ModeTraits;
-template <> struct ModeTraits<MODE_F64> { typedef double T; typedef d4 Acc; static constexpr int BK = 16; };
-template <> struct ModeTraits<MODE_F32> { typedef float T; typedef f4 Acc; static constexpr int BK = 16; };
-template <> struct ModeTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; static constexpr int BK = 32; };
+template <> struct ModeTraits<MODE_F64> { typedef double T; typedef d4 Acc; static constexpr int BK = 16, KS = 4; };
+template <> struct ModeTraits<MODE_F32> { typedef float T; typedef f4 Acc; static constexpr int BK = 16, KS = 4; };
+template <> struct ModeTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; static constexpr int BK = 32, KS = 32; };
 
 // output row of accumulator register r for lanes of k-group lk = l/16
-template <int MODE> __device__ inline int d_row(int r, int lk);
-template <> __device__ inline int d_row<MODE_F64>(int r, int lk) { return 4 * r + lk; }
-template <> __device__ inline int d_row<MODE_F32>(int r, int lk) { return MQ_F32_DROW(r, lk); }
-template <> __device__ inline int d_row<MODE_BF16X3>(int r, int lk) { return MQ_BF16_DROW(r, lk); }
+template <int MODE> __device__ inline int d_row(int r, int lk) { return MODE == MODE_F64 ? 4 * r + lk : 4 * lk + r; }
 
 // bf16 of x, round to nearest even (finite inputs)
 __device__ inline short bf16_bits(float x) {
@@ -41,204 +48,375 @@ __device__ inline short bf16_bits(float x) {
     return short(u >> 16);
 }
 __device__ inline float bf16_value(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
-
-template <int MODE, typename T, typename Acc>
-__device__ inline void mma_tile(const T (*As)[GEMM_BM + 1], const T (*Bs)[GEMM_BN + 1], int wm, int wn, int l16,
-                                int lk, Acc& acc00, Acc& acc01, Acc& acc10, Acc& acc11) {
-    constexpr int BK = ModeTraits<MODE>::BK;
-    if constexpr (MODE == MODE_F64) {
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int kr = kk + lk;
-            const double a0 = As[kr][wm + l16], a1 = As[kr][wm + 16 + l16];
-            const double b0 = Bs[kr][wn + l16], b1 = Bs[kr][wn + 16 + l16];
-            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
-        }
-    } else if constexpr (MODE == MODE_F32) {
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int kr = kk + lk;
-            const float a0 = As[kr][wm + l16], a1 = As[kr][wm + 16 + l16];
-            const float b0 = Bs[kr][wn + l16], b1 = Bs[kr][wn + 16 + l16];
-            acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc01, 0, 0, 0);
-            acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc10, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc11, 0, 0, 0);
-        }
-    } else {
-        // one 16x16x32 step: lane holds k = 8*lk + j, j = 0..7
-        s8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int kr = MQ_BF16_K(lk, j);
-                const float a = As[kr][wm + 16 * q + l16], b = Bs[kr][wn + 16 * q + l16];
-                const short ahi = bf16_bits(a), bhi = bf16_bits(b);
-                ah[q][j] = ahi;
-                al[q][j] = bf16_bits(a - bf16_value(ahi));
-                bh[q][j] = bhi;
-                bl[q][j] = bf16_bits(b - bf16_value(bhi));
-            }
-        Acc* accs[4] = {&acc00, &acc01, &acc10, &acc11};
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                Acc& c = *accs[2 * i + j];
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
-            }
-    }
+__device__ inline void bf16_split(float x, short& hi, short& lo) {
+    hi = bf16_bits(x);
+    lo = bf16_bits(x - bf16_value(hi));
 }
 
-template <typename TA, typename TB, typename TC, int MODE>
-__global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
-                                                            const GemmSeg* __restrict__ segs, Bases bases,
-                                                            const uint32_t* __restrict__ maps) {
-    typedef typename ModeTraits<MODE>::T T;
-    typedef typename ModeTraits<MODE>::Acc Acc;
-    constexpr int BK = ModeTraits<MODE>::BK;
-    constexpr int KSTEP = GEMM_THREADS / GEMM_BM;   // k rows per pass of the m-contiguous loaders
-    __shared__ T As[BK][GEMM_BM + 1];
-    __shared__ T Bs[BK][GEMM_BN + 1];
+__device__ inline d4 mfma_step(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline f4 mfma_step(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ inline f4 mfma3(const s8& ah, const s8& al, const s8& bh, const s8& bl, f4 c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+}
 
-    const uint32_t tile = blockIdx.x;
+// ------------------------------------------------------------------------------ operand access
+// element (m, k) of op(A) and (k, n) of op(B) of one segment; the caller checks the bounds
+template <typename TA>
+__device__ inline double a_at(const TA* A, const GemmSeg& sg, uint32_t m, uint32_t k) {
+    return sg.trans_a ? double(A[uint64_t(m) * sg.lda + k]) : double(A[uint64_t(k) * sg.lda + m]);
+}
+template <typename TB>
+__device__ inline double b_at(const TB* B, const GemmSeg& sg, const uint32_t* maps, uint32_t k, uint32_t n) {
+    if (sg.bmap != MAP_NONE) {
+        const uint32_t blk = k / sg.bbh;
+        return double(B[uint64_t(maps[sg.bmap + blk]) * sg.bldk + (k - blk * sg.bbh) + uint64_t(n) * sg.ldb]);
+    }
+    return sg.trans_b ? double(B[uint64_t(k) * sg.ldb + n]) : double(B[uint64_t(n) * sg.ldb + k]);
+}
+
+template <typename TC>
+__device__ inline TC* c_column(const GemmTask& tk, const Bases& bases, const uint32_t* maps, uint32_t gn) {
+    TC* C = reinterpret_cast<TC*>(bases.p[tk.c_base]) + tk.c_off;
+    if (tk.cmap != MAP_NONE) {
+        const uint32_t blk = gn / tk.cbw;
+        return C + uint64_t(maps[tk.cmap + blk]) * tk.cldk + uint64_t(gn - blk * tk.cbw) * tk.ldc;
+    }
+    return C + uint64_t(gn) * tk.ldc;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), each XCD with its
+// own L2.  Logical tiles are renumbered so that each XCD runs one contiguous range of them:
+// neighbouring tiles (same A row panel, same task) then share an L2.  A bijection on [0, n).
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    constexpr uint32_t X = 8;
+    const uint32_t q = n / X, r = n % X, x = b % X, i = b / X;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+__device__ inline uint32_t find_task(const GemmTask* tasks, uint32_t n_tasks, uint32_t tile) {
     uint32_t lo = 0, hi = n_tasks;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
         if (tasks[mid].tile_begin <= tile) lo = mid; else hi = mid;
     }
-    const GemmTask tk = tasks[lo];
+    return lo;
+}
+
+// ------------------------------------------------------------------------------ LDS-tiled kernel
+// Software pipelined: the K steps of all segments form one sequence; while the MFMAs consume
+// step t from one LDS buffer, the global loads of step t+1 are in flight into registers, which
+// are then converted and stored into the other buffer (one barrier per step).
+template <int MODE, int BM, int BN>
+struct Smem {
+    typedef typename ModeTraits<MODE>::T T;
+    static constexpr int BK = ModeTraits<MODE>::BK;
+    static constexpr int RK = BK + 8;   // bf16 rows: k contiguous, 16-byte aligned, padded
+    static constexpr size_t plain = sizeof(T) * size_t(BK) * (BM + 1 + BN + 1);
+    static constexpr size_t split = sizeof(short) * size_t(2) * RK * (BM + BN);
+    static constexpr size_t buf = ((MODE == MODE_BF16X3 ? split : plain) + 15) / 16 * 16;
+};
+
+template <typename TA, typename TB, typename TC, int MODE, int PM, int PN, int WM, int WN>
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
+                                                            const GemmSeg* __restrict__ segs, Bases bases,
+                                                            const uint32_t* __restrict__ maps) {
+    static_assert(PM * PN * 64 == GEMM_THREADS, "four waves");
+    typedef typename ModeTraits<MODE>::T T;
+    typedef typename ModeTraits<MODE>::Acc Acc;
+    constexpr int BK = ModeTraits<MODE>::BK;
+    constexpr int BM = 16 * WM * PM, BN = 16 * WN * PN;
+    constexpr int NA = BK * BM / GEMM_THREADS, NB = BK * BN / GEMM_THREADS;
+    static_assert(NA * GEMM_THREADS == BK * BM && NB * GEMM_THREADS == BK * BN, "whole staging passes");
+    typedef Smem<MODE, BM, BN> SM;
+    constexpr int RK = SM::RK;
+    __shared__ __align__(16) char smem[2 * SM::buf];
+
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const GemmTask tk = tasks[find_task(tasks, n_tasks, tile)];
     const uint32_t local = tile - tk.tile_begin;
-    const uint32_t m0 = (local / tk.tiles_n) * GEMM_BM;
-    const uint32_t n0 = (local % tk.tiles_n) * GEMM_BN;
+    const uint32_t m0 = (local / tk.tiles_n) * BM;
+    const uint32_t n0 = (local % tk.tiles_n) * BN;
     const uint32_t M = tk.M, N = tk.N;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int wm = (wave % PM) * 16 * WM, wn = (wave / PM) * 16 * WN;
     const int l16 = lane & 15, lk = lane >> 4;
 
-    Acc acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
+    Acc acc[WM][WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
+    // cursor over the K steps of all segments: (sgi, k0) with k0 < segs[sgi].K
+    uint32_t sgi = tk.seg0, k0 = 0;
+    GemmSeg sg;
+    auto settle = [&]() {
+        while (sgi < tk.seg1) {
+            sg = segs[sgi];
+            if (k0 < sg.K) return true;
+            ++sgi;
+            k0 = 0;
+        }
+        return false;
+    };
+    // staged operands of one step, in registers, with the index order they were loaded in
+    double ra[NA], rb[NB];
+    bool r_trans_a = false, r_kfast_b = true;
+    auto gload = [&]() {
+        const TA* __restrict__ A = reinterpret_cast<const TA*>(bases.p[sg.a_base]) + sg.a_off;
+        const TB* __restrict__ B = reinterpret_cast<const TB*>(bases.p[sg.b_base]) + sg.b_off;
+        r_trans_a = sg.trans_a;
+        r_kfast_b = sg.bmap != MAP_NONE || !sg.trans_b;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            const int m = r_trans_a ? e / BK : e % BM;
+            const int k = r_trans_a ? e % BK : e / BM;
+            const uint32_t gm = m0 + m, gk = k0 + k;
+            ra[i] = (gm < M && gk < sg.K) ? sg.alpha * a_at(A, sg, gm, gk) : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            const int n = r_kfast_b ? e / BK : e % BN;
+            const int k = r_kfast_b ? e % BK : e / BN;
+            const uint32_t gn = n0 + n, gk = k0 + k;
+            rb[i] = (gn < N && gk < sg.K) ? b_at(B, sg, maps, gk, gn) : 0.0;
+        }
+    };
+    auto lstore = [&](int buf) {
+        char* base = smem + buf * SM::buf;
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            const int m = r_trans_a ? e / BK : e % BM;
+            const int k = r_trans_a ? e % BK : e / BM;
+            if constexpr (MODE == MODE_BF16X3) {
+                short(*Ah)[RK] = reinterpret_cast<short(*)[RK]>(base);
+                bf16_split(float(ra[i]), Ah[m][k], Ah[BM + m][k]);
+            } else {
+                reinterpret_cast<T(*)[BM + 1]>(base)[k][m] = T(ra[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            const int n = r_kfast_b ? e / BK : e % BN;
+            const int k = r_kfast_b ? e % BK : e / BN;
+            if constexpr (MODE == MODE_BF16X3) {
+                short(*Bh)[RK] = reinterpret_cast<short(*)[RK]>(base) + 2 * BM;
+                bf16_split(float(rb[i]), Bh[n][k], Bh[BN + n][k]);
+            } else {
+                reinterpret_cast<T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1))[k][n] = T(rb[i]);
+            }
+        }
+    };
+    auto compute = [&](int buf) {
+        const char* base = smem + buf * SM::buf;
+        if constexpr (MODE == MODE_BF16X3) {
+            const short(*Ah)[RK] = reinterpret_cast<const short(*)[RK]>(base);
+            const short(*Bh)[RK] = Ah + 2 * BM;
+            s8 ah[WM], al[WM], bh[WN], bl[WN];
+#pragma unroll
+            for (int i = 0; i < WM; ++i) {
+                ah[i] = *reinterpret_cast<const s8*>(&Ah[wm + 16 * i + l16][8 * lk]);
+                al[i] = *reinterpret_cast<const s8*>(&Ah[BM + wm + 16 * i + l16][8 * lk]);
+            }
+#pragma unroll
+            for (int j = 0; j < WN; ++j) {
+                bh[j] = *reinterpret_cast<const s8*>(&Bh[wn + 16 * j + l16][8 * lk]);
+                bl[j] = *reinterpret_cast<const s8*>(&Bh[BN + wn + 16 * j + l16][8 * lk]);
+            }
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j) acc[i][j] = mfma3(ah[i], al[i], bh[j], bl[j], acc[i][j]);
+        } else {
+            const T(*As)[BM + 1] = reinterpret_cast<const T(*)[BM + 1]>(base);
+            const T(*Bs)[BN + 1] = reinterpret_cast<const T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1));
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                const int kr = kk + lk;
+                T a[WM], b[WN];
+#pragma unroll
+                for (int i = 0; i < WM; ++i) a[i] = As[kr][wm + 16 * i + l16];
+#pragma unroll
+                for (int j = 0; j < WN; ++j) b[j] = Bs[kr][wn + 16 * j + l16];
+#pragma unroll
+                for (int i = 0; i < WM; ++i)
+#pragma unroll
+                    for (int j = 0; j < WN; ++j) acc[i][j] = mfma_step(a[i], b[j], acc[i][j]);
+            }
+        }
+    };
+
+    if (settle()) {
+        gload();
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        while (true) {
+            k0 += BK;
+            const bool more = settle();
+            if (more) gload();   // next step's loads overlap this step's MFMAs
+            compute(buf);
+            if (!more) break;
+            lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+        const uint32_t gn = n0 + wn + 16 * j + l16;
+        if (gn >= N) continue;
+        TC* __restrict__ Cn = c_column<TC>(tk, bases, maps, gn);
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t gm = m0 + wm + 16 * i + d_row<MODE>(r, lk);
+                if (gm < M) {
+                    TC v = TC(acc[i][j][r]);
+                    if (tk.beta) v += Cn[gm];
+                    Cn[gm] = v;
+                }
+            }
+    }
+}
+
+// ------------------------------------------------------------------------------ split-K 16x16 kernel
+// One workgroup per task (M, N <= 16).  The K steps of all segments are dealt round-robin to the
+// four waves; each wave reads its operand fragments directly from memory in the MFMA lane layout.
+template <typename TA, typename TB, typename TC, int MODE>
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
+                                                                 const GemmSeg* __restrict__ segs, Bases bases,
+                                                                 const uint32_t* __restrict__ maps) {
+    typedef typename ModeTraits<MODE>::T T;
+    typedef typename ModeTraits<MODE>::Acc Acc;
+    constexpr int KS = ModeTraits<MODE>::KS;
+    constexpr int NW = GEMM_THREADS / 64;
+    __shared__ T red[NW - 1][64][4];
+
+    const GemmTask tk = tasks[find_task(tasks, n_tasks, xcd_tile(blockIdx.x, gridDim.x))];
+    const uint32_t M = tk.M, N = tk.N;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int l16 = lane & 15, lk = lane >> 4;
+    const uint32_t row = l16, col = l16;
+
+    Acc acc = {0, 0, 0, 0};
+    uint32_t step = 0;   // K steps of the previous segments: deals continue across segments
     for (uint32_t sgi = tk.seg0; sgi < tk.seg1; ++sgi) {
         const GemmSeg sg = segs[sgi];
         const TA* __restrict__ A = reinterpret_cast<const TA*>(bases.p[sg.a_base]) + sg.a_off;
         const TB* __restrict__ B = reinterpret_cast<const TB*>(bases.p[sg.b_base]) + sg.b_off;
         const uint32_t K = sg.K;
-        const T alpha = T(sg.alpha);
-        for (uint32_t k0 = 0; k0 < K; k0 += BK) {
-            if (!sg.trans_a) {
-                const int m = tid & 63;
+        const double alpha = sg.alpha;
+        const uint32_t nsteps = (K + KS - 1) / KS;
+        for (uint32_t s = (uint32_t(wave) + NW - step % NW) % NW; s < nsteps; s += NW) {
+            const uint32_t k0 = s * KS;
+            if constexpr (MODE == MODE_BF16X3) {
+                s8 ah, al, bh, bl;
 #pragma unroll
-                for (int i = 0; i < BK / KSTEP; ++i) {
-                    const int k = (tid >> 6) + KSTEP * i;
-                    const uint32_t gm = m0 + m, gk = k0 + k;
-                    As[k][m] = (gm < M && gk < K) ? alpha * T(A[uint64_t(gk) * sg.lda + gm]) : T(0);
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t k = k0 + 8 * lk + j;
+                    const float a = (row < M && k < K) ? float(alpha * a_at(A, sg, row, k)) : 0.f;
+                    const float b = (col < N && k < K) ? float(b_at(B, sg, maps, k, col)) : 0.f;
+                    short h, l;
+                    bf16_split(a, h, l);
+                    ah[j] = h;
+                    al[j] = l;
+                    bf16_split(b, h, l);
+                    bh[j] = h;
+                    bl[j] = l;
                 }
+                acc = mfma3(ah, al, bh, bl, acc);
             } else {
-#pragma unroll
-                for (int i = 0; i < (BK * GEMM_BM) / GEMM_THREADS; ++i) {
-                    const int e = tid + GEMM_THREADS * i;
-                    const int k = e % BK, m = e / BK;
-                    const uint32_t gm = m0 + m, gk = k0 + k;
-                    As[k][m] = (gm < M && gk < K) ? alpha * T(A[uint64_t(gm) * sg.lda + gk]) : T(0);
-                }
+                const uint32_t k = k0 + lk;
+                const T a = (row < M && k < K) ? T(alpha * a_at(A, sg, row, k)) : T(0);
+                const T b = (col < N && k < K) ? T(b_at(B, sg, maps, k, col)) : T(0);
+                acc = mfma_step(a, b, acc);
             }
-            if (sg.bmap != MAP_NONE) {
-#pragma unroll
-                for (int i = 0; i < (BK * GEMM_BN) / GEMM_THREADS; ++i) {
-                    const int e = tid + GEMM_THREADS * i;
-                    const int k = e % BK, n = e / BK;
-                    const uint32_t gk = k0 + k, gn = n0 + n;
-                    T v = T(0);
-                    if (gk < K && gn < N) {
-                        const uint32_t blk = gk / sg.bbh;
-                        v = T(B[uint64_t(maps[sg.bmap + blk]) * sg.bldk + (gk - blk * sg.bbh) + uint64_t(gn) * sg.ldb]);
-                    }
-                    Bs[k][n] = v;
-                }
-            } else if (!sg.trans_b) {
-#pragma unroll
-                for (int i = 0; i < (BK * GEMM_BN) / GEMM_THREADS; ++i) {
-                    const int e = tid + GEMM_THREADS * i;
-                    const int k = e % BK, n = e / BK;
-                    const uint32_t gn = n0 + n, gk = k0 + k;
-                    Bs[k][n] = (gn < N && gk < K) ? T(B[uint64_t(gn) * sg.ldb + gk]) : T(0);
-                }
-            } else {
-                const int n = tid & 63;
-#pragma unroll
-                for (int i = 0; i < BK / KSTEP; ++i) {
-                    const int k = (tid >> 6) + KSTEP * i;
-                    const uint32_t gn = n0 + n, gk = k0 + k;
-                    Bs[k][n] = (gn < N && gk < K) ? T(B[uint64_t(gk) * sg.ldb + gn]) : T(0);
-                }
-            }
-            __syncthreads();
-            mma_tile<MODE, T, Acc>(As, Bs, wm, wn, l16, lk, acc00, acc01, acc10, acc11);
-            __syncthreads();
         }
+        step += nsteps;
     }
-
-    TC* __restrict__ C = reinterpret_cast<TC*>(bases.p[tk.c_base]) + tk.c_off;
-    const uint64_t ldc = tk.ldc;
-    const Acc* accs[4] = {&acc00, &acc01, &acc10, &acc11};
+    if (wave > 0)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int i = q >> 1, j = q & 1;
-        const uint32_t gn = n0 + wn + j * 16 + l16;
-        if (gn >= N) continue;
-        TC* __restrict__ Cn;
-        if (tk.cmap != MAP_NONE) {
-            const uint32_t blk = gn / tk.cbw;
-            Cn = C + uint64_t(maps[tk.cmap + blk]) * tk.cldk + uint64_t(gn - blk * tk.cbw) * ldc;
-        } else {
-            Cn = C + uint64_t(gn) * ldc;
-        }
+        for (int r = 0; r < 4; ++r) red[wave - 1][lane][r] = acc[r];
+    __syncthreads();
+    if (wave == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const uint32_t gm = m0 + wm + i * 16 + d_row<MODE>(r, lk);
-            if (gm < M) {
-                TC v = TC((*accs[q])[r]);
-                TC* p = Cn + gm;
-                if (tk.beta) v += *p;
-                *p = v;
+        for (int w = 0; w < NW - 1; ++w)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] += red[w][lane][r];
+        if (col < N) {
+            TC* __restrict__ Cn = c_column<TC>(tk, bases, maps, col);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t gm = d_row<MODE>(r, lk);
+                if (gm < M) {
+                    TC v = TC(acc[r]);
+                    if (tk.beta) v += Cn[gm];
+                    Cn[gm] = v;
+                }
             }
         }
     }
 }
 
+// ------------------------------------------------------------------------------ dispatch
 template <typename TA, typename TB, typename TC, int MODE>
-static hipError_t launch(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                         uint32_t n_tiles, Bases bases, const uint32_t* maps) {
-    hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE>), dim3(n_tiles), dim3(GEMM_THREADS), 0, st, tasks, n_tasks,
-                       segs, bases, maps);
+static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
+                               uint32_t n_tiles, Bases bases, const uint32_t* maps) {
+    const dim3 grid(n_tiles), block(GEMM_THREADS);
+    switch (cfg) {
+        case GEMM_CFG_128x128:
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 4, 4>), grid, block, 0, st, tasks, n_tasks, segs,
+                               bases, maps);
+            break;
+        case GEMM_CFG_64x64:
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
+                               bases, maps);
+            break;
+        case GEMM_CFG_32x128:
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
+                               bases, maps);
+            break;
+        case GEMM_CFG_16x128:
+            hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 1, 2>), grid, block, 0, st, tasks, n_tasks, segs,
+                               bases, maps);
+            break;
+        case GEMM_CFG_16x16:
+            hipLaunchKernelGGL((gemm_tiny_kernel<TA, TB, TC, MODE>), grid, block, 0, st, tasks, n_tasks, segs, bases,
+                               maps);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_gemm(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                       uint32_t n_tiles, Bases bases, const uint32_t* maps) {
+hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
+                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps) {
     if (n_tiles == 0) return hipSuccess;
-    return launch<double, double, double, MODE_F64>(st, tasks, n_tasks, segs, n_tiles, bases, maps);
-}
-
-// Mixed-precision stages.  types: operand storage of A, B, C as 'd' (double) or 'f' (float).
-hipError_t launch_gemm_mixed(hipStream_t st, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
-                             const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps) {
-    if (n_tiles == 0) return hipSuccess;
+    if (mode == MODE_F64)
+        return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);
     const bool da = types[0] == 'd', db = types[1] == 'd', dc = types[2] == 'd';
-#define MQ_DISPATCH(MD)                                                                                       \
-    if (da && db && !dc) return launch<double, double, float, MD>(st, tasks, n_tasks, segs, n_tiles, bases, maps); \
-    if (!da && !db && !dc) return launch<float, float, float, MD>(st, tasks, n_tasks, segs, n_tiles, bases, maps); \
-    if (da && !db && dc) return launch<double, float, double, MD>(st, tasks, n_tasks, segs, n_tiles, bases, maps); \
-    if (da && db && dc) return launch<double, double, double, MD>(st, tasks, n_tasks, segs, n_tiles, bases, maps);
+#define MQ_DISPATCH(MD)                                                                                          \
+    if (da && db && !dc)                                                                                         \
+        return launch_typed<double, double, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);     \
+    if (!da && !db && !dc)                                                                                       \
+        return launch_typed<float, float, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);       \
+    if (da && !db && dc)                                                                                         \
+        return launch_typed<double, float, double, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);
     if (mode == MODE_F32) { MQ_DISPATCH(MODE_F32) }
     if (mode == MODE_BF16X3) { MQ_DISPATCH(MODE_BF16X3) }
 #undef MQ_DISPATCH

@@ -63,17 +63,24 @@ struct GatherRegion {
 constexpr uint32_t MAP_NONE = 0xffffffffu;
 enum GemmMode { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
 
-// MFMA output-row / bf16 k-slot layouts on gfx950 (see gemm.hip)
-#define MQ_F32_DROW(r, lk) (4 * (lk) + (r))
-#define MQ_BF16_DROW(r, lk) (4 * (lk) + (r))
-#define MQ_BF16_K(lk, j) (8 * (lk) + (j))
-
-hipError_t launch_gemm_mixed(hipStream_t st, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
-                             const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps);
-constexpr int GEMM_BM = 64;
-constexpr int GEMM_BN = 64;
-constexpr int GEMM_BK = 16;
+// Tile configurations of the GEMM (gemm.hip); a task runs with the one that fits its (M, N).
+enum GemmCfg { GEMM_CFG_128x128 = 0, GEMM_CFG_64x64 = 1, GEMM_CFG_32x128 = 2, GEMM_CFG_16x128 = 3,
+               GEMM_CFG_16x16 = 4, GEMM_N_CFG = 5 };
+constexpr int GEMM_CFG_BM[GEMM_N_CFG] = {128, 64, 32, 16, 16};
+constexpr int GEMM_CFG_BN[GEMM_N_CFG] = {128, 64, 128, 128, 16};
+inline int gemm_pick_cfg(uint32_t M, uint32_t N) {
+    if (M <= 16) return N <= 16 ? GEMM_CFG_16x16 : GEMM_CFG_16x128;
+    if (M <= 32) return GEMM_CFG_32x128;
+    if (M >= 128 && N >= 128) return GEMM_CFG_128x128;
+    return GEMM_CFG_64x64;
+}
 constexpr int GEMM_THREADS = 256;
+
+// One launch of all tasks of one tile configuration.  mode MODE_F64: all operands double.
+// Otherwise `types` gives the storage of A, B and C as 'd' (double) or 'f' (float): "ddf",
+// "fff" or "dfd".
+hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
+                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps);
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
 constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions
@@ -82,8 +89,6 @@ constexpr int GATHER_SMALL = 256;    // regions up to this many elements: one wa
 hipError_t launch_gather_small(hipStream_t st, const GatherRegion* regions, uint32_t n_regions,
                                const uint64_t* src_off, const double* alpha, double* ws, double* dst_base);
 
-hipError_t launch_gemm(hipStream_t st, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                       uint32_t n_tiles, Bases bases, const uint32_t* maps);
 hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n_regions,
                          const uint64_t* src_off, const double* alpha, uint32_t n_chunks, double* ws,
                          double* dst_base);

@@ -253,6 +253,9 @@ Schedule build_left_schedule(const SiteLayout& site, const Mpo& mpo, const Bound
                              bool symmetric);
 Schedule build_right_schedule(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& right,
                               bool symmetric);
+// The boundary index those two step schedules produce, without their task lists.
+BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& left, bool symmetric);
+BoundaryIndex right_step_index(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& right, bool symmetric);
 uint64_t schedule_flops(const Schedule& s, const BoundaryIndex& t_index, const std::vector<uint32_t>& lr_ket_sizes);
 
 }  // namespace mq

@@ -9,6 +9,7 @@
 //   right : right_tasks.hpp:39-78 + right_tasks.hpp:80-167 (rshtm_tasks)
 // T tuples are looked up through a hash map instead of the reference's linear get_ti scan
 // (tasks.cpp:766-775); MPS blocks are built in parallel and concatenated in block order.
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -797,6 +798,152 @@ Schedule build_right_schedule(const SiteLayout& site, const Mpo& mpo, const Boun
     }
     S.out_index.complement_transpose(mpo.left_herm, false, sy);
     return S;
+}
+
+// ============================================================================ index-only steps
+// The BoundaryIndex that build_left_schedule / build_right_schedule produce, without building their
+// task lists.  A line b of a cohort is present iff some sparse-operator entry reaches it: entries
+// always find their T tuple, so presence does not depend on the entries' values.  Lookups go
+// through dense (sector, sector) and (sector, physical sector) tables.
+namespace {
+
+struct Flow {
+    uint32_t b, s_out, s_in;   // the other MPO index, physical sectors of the bra and ket side
+};
+
+// per MPO index (col b2 for the left step, row b1 for the right step) the distinct flows of
+// non-empty operator blocks
+std::vector<std::vector<Flow>> mpo_flows(const Mpo& mpo, const Index& phys_i, bool by_col) {
+    uint32_t n = by_col ? mpo.col_dim : mpo.row_dim;
+    const auto& cells = by_col ? mpo.cols : mpo.rows;
+    std::vector<std::vector<Flow>> out(n);
+    for (uint32_t b = 0; b < n; ++b) {
+        std::vector<uint64_t> seen;
+        for (const auto& cell : cells[b])
+            for (uint32_t t = cell.t0; t < cell.t1; ++t) {
+                const Op& W = mpo.ops[mpo.terms[t].op];
+                for (const OpBlock& wb : W.blocks) {
+                    if (wb.e1 <= wb.e0) continue;
+                    uint32_t so = phys_i.position(wb.out), si = phys_i.position(wb.in);
+                    MQ_REQUIRE(so < phys_i.size() && si < phys_i.size(), "operator charge not in physical index");
+                    uint64_t key = (uint64_t(cell.other) << 32) | (uint64_t(so) << 16) | si;
+                    if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                    seen.push_back(key);
+                    out[b].push_back({cell.other, so, si});
+                }
+            }
+    }
+    return out;
+}
+
+// table[x * nphys + s] = position of charges[x] + sign * phys[s] in target (target.size() if absent)
+std::vector<uint32_t> shift_table(const Index& from, const Index& phys, int sign, const Index& target) {
+    std::vector<uint32_t> t(size_t(from.size()) * phys.size());
+    for (uint32_t x = 0; x < from.size(); ++x)
+        for (uint32_t s = 0; s < phys.size(); ++s)
+            t[size_t(x) * phys.size() + s] =
+                target.position(sign > 0 ? from.charges[x] + phys.charges[s] : from.charges[x] - phys.charges[s]);
+    return t;
+}
+
+std::vector<uint32_t> cohort_table(const BoundaryIndex& B, const Index& bra, const Index& ket) {
+    std::vector<uint32_t> t(size_t(bra.size()) * ket.size());
+    for (uint32_t i = 0; i < bra.size(); ++i)
+        for (uint32_t j = 0; j < ket.size(); ++j) t[size_t(i) * ket.size() + j] = B.cohort_index(bra.charges[i], ket.charges[j]);
+    return t;
+}
+
+}  // namespace
+
+BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& left, bool symmetric) {
+    const Index &left_i = site.left_i, &right_i = site.right_i, &phys_i = site.phys_i;
+    MQ_REQUIRE(left.aux_dim == mpo.row_dim, "left boundary aux_dim != mpo row_dim");
+    require_bond(left, left_i, "left");
+    const Symm sy = site.symm;
+    const uint32_t nr = right_i.size(), np = phys_i.size(), w = mpo.col_dim;
+    auto flows = mpo_flows(mpo, phys_i, true);
+    auto down = shift_table(right_i, phys_i, -1, left_i);   // rc - phys -> left sector
+    auto coh = cohort_table(left, left_i, left_i);
+    // lines[(rb_ket * nr + rb_bra) * w + b2]
+    std::vector<uint8_t> lines(size_t(nr) * nr * w, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int rbk = 0; rbk < int(nr); ++rbk) {
+        Charge rc_ket = right_i.charges[rbk];
+        for (uint32_t rbb = 0; rbb < nr; ++rbb) {
+            Charge rc_bra = right_i.charges[rbb];
+            if (std::abs(sy.particle_number(rc_bra) - sy.particle_number(rc_ket)) > 2) continue;
+            uint8_t* ln = &lines[(size_t(rbk) * nr + rbb) * w];
+            for (uint32_t b2 = 0; b2 < w; ++b2) {
+                if (symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) continue;
+                if (sy.su2() && !triangle(sy.spin(rc_ket), mpo.right_spin[b2], sy.spin(rc_bra))) continue;
+                for (const Flow& f : flows[b2]) {
+                    uint32_t lbb = down[size_t(rbb) * np + f.s_out], lbk = down[size_t(rbk) * np + f.s_in];
+                    if (lbb == left_i.size() || lbk == left_i.size()) continue;
+                    if (left.has_block(coh[size_t(lbb) * left_i.size() + lbk], f.b)) {
+                        ln[b2] = 1;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    BoundaryIndex out(right_i, right_i, w);
+    std::vector<int64_t> off(w);
+    for (uint32_t rbk = 0; rbk < nr; ++rbk)
+        for (uint32_t rbb = 0; rbb < nr; ++rbb) {
+            const uint8_t* ln = &lines[(size_t(rbk) * nr + rbb) * w];
+            uint64_t bs = uint64_t(right_i.sizes[rbb]) * right_i.sizes[rbk];
+            uint32_t cnt = 0;
+            for (uint32_t b = 0; b < w; ++b) off[b] = ln[b] ? int64_t(bs * cnt++) : -1;
+            if (cnt) out.add_cohort(rbb, rbk, off);
+        }
+    if (symmetric) out.complement_transpose(mpo.right_herm, true, sy);
+    return out;
+}
+
+BoundaryIndex right_step_index(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& right, bool symmetric) {
+    const Index &left_i = site.left_i, &right_i = site.right_i, &phys_i = site.phys_i;
+    MQ_REQUIRE(right.aux_dim == mpo.col_dim, "right boundary aux_dim != mpo col_dim");
+    require_bond(right, right_i, "right");
+    const Symm sy = site.symm;
+    const uint32_t nl = left_i.size(), np = phys_i.size(), w = mpo.row_dim;
+    auto flows = mpo_flows(mpo, phys_i, false);
+    auto up = shift_table(left_i, phys_i, +1, right_i);   // lc + phys -> right sector
+    auto coh = cohort_table(right, right_i, right_i);     // (ket sector, bra sector) of the right boundary
+    std::vector<uint8_t> lines(size_t(nl) * nl * w, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int lbk = 0; lbk < int(nl); ++lbk) {
+        Charge lc_ket = left_i.charges[lbk];
+        for (uint32_t lbb = 0; lbb < nl; ++lbb) {
+            Charge lc_bra = left_i.charges[lbb];
+            if (std::abs(sy.particle_number(lc_ket) - sy.particle_number(lc_bra)) > 2) continue;
+            uint8_t* ln = &lines[(size_t(lbk) * nl + lbb) * w];
+            for (uint32_t b1 = 0; b1 < w; ++b1) {
+                if (symmetric && mpo.left_herm.skip(b1, lc_ket, lc_bra)) continue;
+                if (sy.su2() && !triangle(sy.spin(lc_ket), mpo.left_spin[b1], sy.spin(lc_bra))) continue;
+                for (const Flow& f : flows[b1]) {
+                    uint32_t rbb = up[size_t(lbb) * np + f.s_out], rbk = up[size_t(lbk) * np + f.s_in];
+                    if (rbb == right_i.size() || rbk == right_i.size()) continue;
+                    if (right.has_block(coh[size_t(rbk) * right_i.size() + rbb], f.b)) {
+                        ln[b1] = 1;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    BoundaryIndex out(left_i, left_i, w);
+    std::vector<int64_t> off(w);
+    for (uint32_t lbk = 0; lbk < nl; ++lbk)
+        for (uint32_t lbb = 0; lbb < nl; ++lbb) {
+            const uint8_t* ln = &lines[(size_t(lbk) * nl + lbb) * w];
+            uint64_t bs = uint64_t(left_i.sizes[lbk]) * left_i.sizes[lbb];
+            uint32_t cnt = 0;
+            for (uint32_t b = 0; b < w; ++b) off[b] = ln[b] ? int64_t(bs * cnt++) : -1;
+            if (cnt) out.add_cohort(lbk, lbb, off);
+        }
+    out.complement_transpose(mpo.left_herm, false, sy);
+    return out;
 }
 
 // ============================================================================ flops

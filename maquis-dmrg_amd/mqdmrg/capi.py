@@ -116,6 +116,10 @@ SIGNATURES = {
                                          C.POINTER(mq_bindex_desc), C.c_int, PP]),
     "mq_schedule_build_right": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),
                                           C.POINTER(mq_bindex_desc), C.c_int, PP]),
+    "mq_schedule_build_left_index": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),
+                                               C.POINTER(mq_bindex_desc), C.c_int, PP]),
+    "mq_schedule_build_right_index": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),
+                                                C.POINTER(mq_bindex_desc), C.c_int, PP]),
     "mq_schedule_out_index": (C.c_int, [P, C.POINTER(mq_bindex_desc)]),
     "mq_schedule_export": (C.c_int, [P, C.POINTER(mq_schedule_desc)]),
     "mq_schedule_flops": (C.c_int, [P, u64p]),

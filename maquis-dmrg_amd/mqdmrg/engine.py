@@ -217,19 +217,37 @@ def build_schedule(kind, site, mpo_desc, bindex_l=None, bindex_r=None, symmetric
         check(lib().mq_schedule_build_right(C.byref(site.desc), C.byref(mpo_desc.desc),
                                             C.byref(bindex_r.desc(keep)), int(symmetric), C.byref(h)))
     try:
-        d = capi.mq_schedule_desc()
-        check(lib().mq_schedule_export(h, C.byref(d)))
-        s = capi.schedule_to_dict(d)
-        f = C.c_uint64()
-        check(lib().mq_schedule_flops(h, C.byref(f)))
-        out_idx = None
-        if kind != "sigma":
-            bd = capi.mq_bindex_desc()
-            check(lib().mq_schedule_out_index(h, C.byref(bd)))
-            out_idx = capi.BIndex.from_desc(bd)
-        return s, f.value, out_idx
+        return _schedule_result(kind, h)
     finally:
         lib().mq_schedule_destroy(h)
+
+
+def step_index(kind, site, mpo_desc, bindex, symmetric=True):
+    """Index of the boundary a left / right step produces, without building the step's schedule."""
+    keep = []
+    h = C.c_void_p()
+    fn = lib().mq_schedule_build_left_index if kind == "left" else lib().mq_schedule_build_right_index
+    check(fn(C.byref(site.desc), C.byref(mpo_desc.desc), C.byref(bindex.desc(keep)), int(symmetric), C.byref(h)))
+    try:
+        bd = capi.mq_bindex_desc()
+        check(lib().mq_schedule_out_index(h, C.byref(bd)))
+        return capi.BIndex.from_desc(bd)
+    finally:
+        lib().mq_schedule_destroy(h)
+
+
+def _schedule_result(kind, h):
+    d = capi.mq_schedule_desc()
+    check(lib().mq_schedule_export(h, C.byref(d)))
+    s = capi.schedule_to_dict(d)
+    f = C.c_uint64()
+    check(lib().mq_schedule_flops(h, C.byref(f)))
+    out_idx = None
+    if kind != "sigma":
+        bd = capi.mq_bindex_desc()
+        check(lib().mq_schedule_out_index(h, C.byref(bd)))
+        out_idx = capi.BIndex.from_desc(bd)
+    return s, f.value, out_idx
 
 
 # ---------------------------------------------------------------------------- sigma

@@ -1,14 +1,14 @@
 """Synthetic site problems at full benchmark shapes.
 
-The boundary *indices* of a chain are propagated through the host planner (left / right step
-schedules, no device work); the boundary data, the MPS tensors and psi are random.  Sigma's cost
+The boundary *indices* of a chain are propagated through the host planner (index-only left /
+right steps, no device work); the boundary data, the MPS tensors and psi are random.  Sigma's cost
 depends only on these shapes, so this gives the benchmark workload of a site in the middle of a
 chain at bond dimension m without running sweeps first."""
 import numpy as np
 
 from . import models, mps
 from .capi import BIndex
-from .engine import MpoDesc, SiteDesc, build_schedule
+from .engine import MpoDesc, SiteDesc, step_index
 from .symmetry import SiteLayout
 
 
@@ -36,12 +36,12 @@ class SyntheticSite:
         bi = trivial_bindex(self.bonds[0], mpo[0].row_dim)
         for k in range(site):
             sd = SiteDesc(sym, self.bonds[k], basis.phys, self.bonds[k + 1])
-            _, _, bi = build_schedule("left", sd, MpoDesc(mpo[k]), bindex_l=bi)
+            bi = step_index("left", sd, MpoDesc(mpo[k]), bi)
         self.left_index = bi
         bi = trivial_bindex(self.bonds[L], mpo[L - 1].col_dim)
         for k in range(L - 1, site + 1, -1):
             sd = SiteDesc(sym, self.bonds[k], basis.phys, self.bonds[k + 1])
-            _, _, bi = build_schedule("right", sd, MpoDesc(mpo[k]), bindex_r=bi)
+            bi = step_index("right", sd, MpoDesc(mpo[k]), bi)
         self.right_index = bi
         W12, phys12, _ = models.twosite_sparse(mpo[site], mpo[site + 1], basis.phys, basis.phys)
         self.W12, self.phys12 = W12, phys12

@@ -172,6 +172,39 @@ def test_product_step_indices_equal_oracle(model):
     np.testing.assert_array_equal(ri.conj_scales, ro.index.conj_scales)
 
 
+def _same_bindex(a, b):
+    assert a.n_cohorts == b.n_cohorts
+    for k in ("bra_pos", "ket_pos", "tr", "n_blocks", "offsets", "conj_scales", "transposes"):
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k), err_msg=k)
+
+
+@pytest.mark.parametrize("model,symmetric", [("heisenberg", True), ("heisenberg", False), ("hubbard", True),
+                                             ("qc", True), ("bosehubbard", True)])
+def test_index_only_steps_equal_full_builders(model, symmetric):
+    """The index-only left / right steps give the same boundary index as the full schedules, at
+    every site of a chain (the right step always complements, so it runs with symmetric=True)."""
+    p = SiteProblem(model, L=8, site=3, Mmax=8, seed=9)
+    import mqdmrg.workloads as W
+    left = W.trivial_bindex(p.bonds[0], p.mpo[0].row_dim)
+    right = W.trivial_bindex(p.bonds[p.L], p.mpo[p.L - 1].col_dim)
+    for k in range(p.L):
+        lay = p.layouts[k]
+        sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+        md = E.MpoDesc(p.mpo[k])
+        _, _, full = E.build_schedule("left", sd, md, bindex_l=left, symmetric=symmetric)
+        fast = E.step_index("left", sd, md, left, symmetric=symmetric)
+        _same_bindex(full, fast)
+        left = full
+    for k in range(p.L - 1, -1, -1):
+        lay = p.layouts[k]
+        sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+        md = E.MpoDesc(p.mpo[k])
+        _, _, full = E.build_schedule("right", sd, md, bindex_r=right)
+        fast = E.step_index("right", sd, md, right)
+        _same_bindex(full, fast)
+        right = full
+
+
 def test_planner_rejects_mismatched_boundary():
     p = SiteProblem("heisenberg", L=8, site=3, Mmax=6, seed=1)
     q = SiteProblem("heisenberg", L=8, site=2, Mmax=6, seed=1)
