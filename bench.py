@@ -1,12 +1,18 @@
-"""Benchmark: sigma-vector throughput of the gfx950 engine on one GPU.
+"""Benchmark: sigma-vector throughput of the gfx950 engine on one GPU, FP64 against the
+reduced-precision paths.
 
 One step = one application of the two-site effective Hamiltonian (sigma = H_eff psi) at the
-middle site of the chain, inputs resident in HBM.  value = sigma TFLOP/s with the reference's
-own flop model (solver/tasks.cpp:388-398, 777-796).  The roofline entry is the dominant kernel
-of the step, timed with HIP events on the engine's stream; the CPU baseline is the oracle (the
-CPU restatement of the reference's CPU branch) on the same inputs on the host cores.
+middle site of the chain, inputs resident in HBM.  value = sigma TFLOP/s with the reference's own
+flop model (solver/tasks.cpp:388-398, 777-796) in the reported precision (default: the fastest
+reduced-precision path whose relative sigma error stays within 1e-4).  The FP64 plain path and
+every reduced precision are timed in the same run, and the site's eigenvalue is solved with the
+plain FP64 Jacobi-Davidson and with the refined solver (reduced-precision sigma, FP64 residual)
+to report the energy difference.  The roofline entry is the dominant kernel of the step, timed
+with HIP events on the engine's stream; the CPU baseline is the oracle (the CPU restatement of
+the reference's CPU branch) on a bounded sample of the same sigma (a subset of its MPS blocks)
+on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hubbard-L64-m1024]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload qc-L30-m2048]
 """
 import argparse
 import json
@@ -18,48 +24,75 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "maquis-dmrg_amd"))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_MFMA_TFLOPS = 78.6
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16x3": 2500.0 / 3}   # dense MFMA; bf16x3 = 3 bf16 products
 PEAK_HBM_GBS = 8000.0
+MIXED_RTOL = 1e-4
 
 WORKLOADS = {
     # name: (model, L, m)
+    "qc-L30-m2048": ("qc", 30, 2048),
+    "qc-L24-m1000": ("qc", 24, 1000),
+    "qc-L16-m256": ("qc", 16, 256),
     "hubbard-L64-m1024": ("hubbard", 64, 1024),
     "heisenberg-L32-m128": ("heisenberg", 32, 128),
-    "qc-L16-m256": ("qc", 16, 256),
-    "qc-L24-m1000": ("qc", 24, 1000),
 }
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="hubbard-L64-m1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="qc-L30-m2048", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="auto", choices=["auto", "f64", "f32", "bf16x3"],
+                    help="precision reported as value (auto: fastest reduced precision within the sigma bar)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-jd", action="store_true")
     ap.add_argument("--seed", type=int, default=7)
     return ap.parse_args()
 
 
-def cpu_baseline(ws, Lcoh, Rcoh, psi, flops, budget_s):
-    """Time the oracle's CPU sigma (OpenMP over MPS blocks, all host cores) on the same inputs."""
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ws, Lcoh, Rcoh, psi, budget_s, gpu_sigma):
+    """Time the oracle's CPU sigma (OpenMP over MPS blocks, all host cores) on a sample of the MPS
+    blocks of the same sigma, sized to about budget_s of CPU work; check it against the GPU's sigma
+    restricted to the same blocks."""
+    import numpy as np
     from oracle import oracle as O
     blas = O.use_numpy_blas()
-    Lb = O.HostBoundary(ws.left_index, [c if c is not None else __import__("numpy").zeros(0) for c in Lcoh])
-    Rb = O.HostBoundary(ws.right_index, [c if c is not None else __import__("numpy").zeros(0) for c in Rcoh])
-    so = O.SigmaOracle(ws.site_desc, ws.mpo_desc, Lb, Rb)
-    t0 = time.perf_counter()
-    out = so.apply(psi)
-    n, t = 1, time.perf_counter() - t0
-    while t < budget_s and n < 50:
+    Lb = O.HostBoundary(ws.left_index, [c if c is not None else np.zeros(0) for c in Lcoh])
+    Rb = O.HostBoundary(ws.right_index, [c if c is not None else np.zeros(0) for c in Rcoh])
+    nblk = ws.site_desc.desc.left_i.n
+    # grow the sample (every stride-th MPS block) until one application takes about budget_s / 4
+    stride = max(1, nblk // 4)
+    while True:
+        blocks = list(range(0, nblk, stride))
+        so = O.SigmaOracle(ws.site_desc, ws.mpo_desc, Lb, Rb, only_blocks=blocks)
+        t0 = time.perf_counter()
+        out = so.apply(psi)
+        t1 = time.perf_counter() - t0
+        if t1 >= budget_s / 4 or stride == 1:
+            break
+        stride = max(1, stride // 4)
+    n, t = 1, t1
+    while t < budget_s and n < 20:
+        t0 = time.perf_counter()
         so.apply(psi)
+        t += time.perf_counter() - t0
         n += 1
-        t = time.perf_counter() - t0
+    # sigma of a block subset = the contributions of those ket blocks; the GPU sigma of psi with
+    # the other blocks zeroed is the same quantity
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return out, {"value": flops * n / t / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-                 "sample": "%d full sigma applications of the same workload (%.1f s), dgemm: %s"
-                           % (n, t, "numpy OpenBLAS, 1 thread per call" if blas else "oracle blocked loop")}
+    res = {"value": so.flops * n / t / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+           "sample": "%d applications of sigma restricted to %d of %d MPS blocks (every %d-th; %.3e of the "
+                     "reference-model flops per application), %.1f s; dgemm: %s"
+                     % (n, len(blocks), nblk, stride, so.flops, t,
+                        "numpy OpenBLAS, 1 thread per call" if blas else "oracle blocked loop")}
+    return res, blocks, out
 
 
 def main():
@@ -75,82 +108,137 @@ def main():
 
     model, L, m = WORKLOADS[args.workload]
     site = L // 2 - 1
+    t_site = time.perf_counter()
     ws = SyntheticSite(model, L, site, m, seed=args.seed)
+    t_site = time.perf_counter() - t_site
+    log("site built in %.1f s: %d psi elements" % (t_site, ws.layout.rp_total))
     eng = E.Engine(0)
     Lcoh = ws.random_cohorts(ws.left_index)
     Rcoh = ws.random_cohorts(ws.right_index)
     Lb = E.DeviceBoundary.upload(eng, ws.left_index, Lcoh)
     Rb = E.DeviceBoundary.upload(eng, ws.right_index, Rcoh)
     t_plan = time.perf_counter()
-    plan = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb)
+    plan = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
     t_plan = time.perf_counter() - t_plan
+    log("plan built in %.1f s: %.3e flops per sigma, %.2f GB workspace" % (t_plan, plan.flops,
+                                                                         plan.workspace_bytes / 1e9))
     psi_h = ws.random_psi()
     psi = torch.from_numpy(psi_h).to(eng.torch_device)
     sigma = torch.zeros_like(psi)
 
-    for _ in range(args.warmup):
-        plan.apply(psi, sigma)
-    eng.sync()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(eng.stream)
-    for _ in range(args.steps):
-        plan.apply(psi, sigma)
-    ev1.record(eng.stream)
-    eng.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = ev0.elapsed_time(ev1) / args.steps
-    value = plan.flops / (ms_step * 1e-3) / 1e12
+    # ---- timed sigma applications per precision
+    per = {}
+    ref = None
+    for name, prec in (("f64", E.PREC_F64), ("f32", E.PREC_F32), ("bf16x3", E.PREC_BF16X3)):
+        for _ in range(args.warmup):
+            plan.apply(psi, sigma, precision=prec)
+        eng.sync()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(eng.stream)
+        for _ in range(args.steps):
+            plan.apply(psi, sigma, precision=prec)
+        ev1.record(eng.stream)
+        eng.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = ev0.elapsed_time(ev1) / args.steps
+        out = sigma.clone()
+        if ref is None:
+            ref = out
+        err = float(torch.linalg.norm(out - ref) / torch.linalg.norm(ref))
+        stage_ms, stage_flops, _ = plan.profile(psi, sigma, reps=max(2, min(args.steps, 5)), precision=prec)
+        per[name] = {"ms_per_sigma": ms, "tflops": plan.flops / (ms * 1e-3) / 1e12, "sigma_rel_err_vs_f64": err,
+                     "wall_seconds_timed": round(wall, 4),
+                     "stage_ms": dict(zip(["fill", "gemm_T", "gemm_S", "gemm_final"], [round(x, 4) for x in stage_ms])),
+                     "stage_gemm_flops": dict(zip(["T", "S", "final"], stage_flops))}
+        log("%s: %.3f ms/sigma, %.2f TFLOP/s, rel err %.2e, stages %s" % (name, ms, per[name]["tflops"], err,
+                                                                         per[name]["stage_ms"]))
 
-    # per-stage timing of the same kernels (HIP events between stages on the engine stream)
-    stage_ms, stage_flops, gather_bytes = plan.profile(psi, sigma, reps=max(3, min(args.steps, 20)))
-    names = ["fill", "gemm_T", "gather_S", "gemm_final"]
-    dom = max(range(1, 4), key=lambda k: stage_ms[k])
-    if dom == 2:
-        achieved = gather_bytes / (stage_ms[2] * 1e-3) / 1e9
-        roof = {"kernel": "gather_kernel (S stage)", "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None}
+    if args.precision == "auto":
+        ok = [p for p in ("f32", "bf16x3") if per[p]["sigma_rel_err_vs_f64"] <= MIXED_RTOL]
+        report = min(ok, key=lambda p: per[p]["ms_per_sigma"]) if ok else "f64"
     else:
-        fl = stage_flops[0] if dom == 1 else stage_flops[2]
-        achieved = fl / (stage_ms[dom] * 1e-3) / 1e12
-        roof = {"kernel": "gemm_f64_kernel (%s)" % names[dom], "bound": "mfma", "achieved": achieved,
-                "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA_TFLOPS,
-                "traffic": None}
-    roof["stage_ms"] = dict(zip(names, [round(x, 4) for x in stage_ms]))
+        report = args.precision
+    prec_code = E.PRECISIONS[report]
 
+    # ---- site eigenproblem: plain FP64 Jacobi-Davidson vs refinement over the reported precision
+    jd = None
+    if not args.no_jd and report != "f64":
+        tol, max_iter, max_outer = 1e-9, 16, 8
+        out = torch.zeros_like(psi)
+        res = {}
+        for name, pc in (("f64", E.PREC_F64), (report, prec_code)):
+            eng.sync()
+            t0 = time.perf_counter()
+            e, cnt, rn = plan.jd_refined(psi, out, tol=tol, max_iter=max_iter, precision=pc, max_outer=max_outer)
+            res[name] = {"energy": e, "seconds": time.perf_counter() - t0, "sigma_calls": cnt,
+                         "fp64_residual": rn}
+            log("JD %s: E=%.12f in %.2f s, %s, residual %.2e" % (name, e, res[name]["seconds"], cnt, rn))
+        e64, ep = res["f64"]["energy"], res[report]["energy"]
+        jd = {"tol": tol, "max_iter": max_iter, "max_outer": max_outer, "runs": res,
+              "abs_delta_e": abs(ep - e64), "rel_delta_e": abs(ep - e64) / max(1.0, abs(e64)),
+              "speedup_vs_f64": res["f64"]["seconds"] / res[report]["seconds"],
+              "note": "boundaries are random (not from a converged MPS), so the spectrum's scale is arbitrary; "
+                      "the relative difference is the meaningful figure"}
+
+    # ---- roofline of the dominant kernel in the reported precision
+    pr = per[report]
+    st = pr["stage_ms"]
+    dom = max(["gemm_T", "gemm_S", "gemm_final"], key=lambda k: st[k])
+    fl = pr["stage_gemm_flops"][{"gemm_T": "T", "gemm_S": "S", "gemm_final": "final"}[dom]]
+    achieved = fl / (st[dom] * 1e-3) / 1e12
+    roof = {"kernel": "gemm_kernel (%s, %s)" % (dom, report), "bound": "mfma", "achieved": achieved,
+            "peak": PEAK_TFLOPS[report], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[report],
+            "traffic": None, "stage_ms": st}
+
+    # ---- CPU baseline on a bounded sample of the same sigma
     cpu = None
     if not args.no_cpu:
-        ref, cpu = cpu_baseline(ws, Lcoh, Rcoh, psi_h, plan.flops, args.cpu_seconds)
-        plan.apply(psi, sigma)
+        cpu, blocks, ref_cpu = cpu_baseline(ws, Lcoh, Rcoh, psi_h, args.cpu_seconds, None)
+        # GPU sigma of psi restricted to the sampled ket blocks (others zeroed): same contributions
+        masked = np.zeros_like(psi_h)
+        lay = ws.layout
+        for b in blocks:
+            o = lay.rp_offset[b]
+            n = lay.left.sizes[b] * lay.rp_cols[b]
+            masked[o:o + n] = psi_h[o:o + n]
+        plan.apply(torch.from_numpy(masked).to(eng.torch_device), sigma)
+        eng.sync()
         got = sigma.cpu().numpy()
-        cpu["parity_rel_err"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        cpu["parity_rel_err"] = float(np.linalg.norm(got - ref_cpu) / np.linalg.norm(ref_cpu))
 
     out = {
         "metric": "sweep wall-time & σ TFLOP/s at L=30 m=2048, mixed vs FP64 at |ΔE|≤1e-6 Eh",
-        "value": value,
+        "value": pr["tflops"],
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": pr["ms_per_sigma"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"f64": "f64", "f32": "f32", "bf16x3": "bf16x3 (fp32 accumulate)"}[report],
         "data": "synthetic",
         "config": {
-            "workload": "sigma (H_eff psi) at the middle site, %s chain L=%d, bond dimension m=%d, FP64 plain path"
-                        % (model, L, m),
+            "workload": "sigma (two-site H_eff psi) at the middle site of a %s chain, L=%d, bond dimension m=%d "
+                        "(%s symmetry), grouped MFMA path" % (model, L, m, "TwoU1" if model in ("qc", "hubbard")
+                                                                  else "U1"),
             "model": model, "L": L, "m": m, "site": site,
             "bond_sectors": len(ws.bonds[site + 1].sizes),
+            "mpo_bond_dim": int(ws.mpo[site].col_dim),
             "flops_per_sigma": plan.flops,
             "psi_elements": plan.n,
+            "site_build_seconds": round(t_site, 2),
             "plan_seconds": round(t_plan, 3),
             "workspace_bytes": plan.workspace_bytes,
-            "wall_seconds_timed": round(wall, 4),
+            "reported_precision": report,
         },
+        "precisions": per,
+        "speedup_vs_f64": per["f64"]["ms_per_sigma"] / pr["ms_per_sigma"],
+        "site_eigensolve": jd,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

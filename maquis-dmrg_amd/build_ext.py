@@ -1,7 +1,8 @@
 """Build the gfx950 engine library ``libmqdmrg.so`` in-tree with hipcc.
 
 All sources (host planner in C++, kernels and C-ABI in HIP) go into one shared library next to
-this file.  HIP code is compiled for gfx950 only, with packed FP32 instructions disabled.
+this file.  HIP code is compiled for gfx950 only, with packed FP32 instructions disabled.  The
+plain dense GEMMs of the grouped S stage call rocBLAS (linked from /opt/rocm/lib).
 """
 import os
 import subprocess
@@ -23,6 +24,7 @@ SOURCES = [
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ROCM_LLVM_LIB = "/opt/rocm/llvm/lib"
+ROCM_LIB = "/opt/rocm/lib"
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fopenmp",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -45,7 +47,8 @@ def build(verbose=False):
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", tmp] + objs + [
-        "-Wl,-rpath," + ROCM_LLVM_LIB, "-L" + ROCM_LLVM_LIB]
+        "-Wl,-rpath," + ROCM_LLVM_LIB, "-L" + ROCM_LLVM_LIB,
+        "-Wl,-rpath," + ROCM_LIB, "-L" + ROCM_LIB, "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr))

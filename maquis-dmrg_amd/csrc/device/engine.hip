@@ -8,14 +8,18 @@
 //              no atomics), or the prop_l / prop_r products of the boundary steps.
 // All operand addresses are checked against their buffers on the host before any launch.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../host/host_types.hpp"
@@ -96,9 +100,17 @@ inline uint64_t align32(uint64_t x) { return round_up(x, 32); }
 struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;   // plain dense GEMMs of the grouped S stage, on `stream`
     DevBuf red_partial, red_result;
     DevBuf step_ws;   // grows; reused by boundary steps
 };
+
+#define BLASCHK(x)                                                                                            \
+    do {                                                                                                      \
+        rocblas_status s_ = (x);                                                                              \
+        if (s_ != rocblas_status_success)                                                                     \
+            fail(MQ_ERR_HIP, std::string("rocBLAS: ") + rocblas_status_to_string(s_) + " at " #x);         \
+    } while (0)
 
 struct mq_boundary {
     mq_ctx* ctx = nullptr;
@@ -137,7 +149,11 @@ struct GemmStage {
 
     void build(const std::vector<GemmTask>& tasks) {
         for (auto& c : cfg) c.clear();
-        for (const auto& t : tasks) cfg[gemm_pick_cfg(t.M, t.N)].push_back(t);
+        for (const auto& t : tasks) {
+            int c = gemm_pick_cfg(t.M, t.N);
+            if (c == GEMM_CFG_16x16 && t.colunit != MAP_NONE) c = GEMM_CFG_16x128;   // units: LDS-tiled kernels
+            cfg[c].push_back(t);
+        }
         for (int c = 0; c < GEMM_N_CFG; ++c) {
             uint64_t tile = 0;
             for (auto& t : cfg[c]) {
@@ -153,17 +169,55 @@ struct GemmStage {
         upload(d_segs, segs, st);
         for (int c = 0; c < GEMM_N_CFG; ++c) upload(d_tasks[c], cfg[c], st);
     }
-    void run(hipStream_t st, int mode, const char* types, Bases b, const uint32_t* maps) const {
+    void run(hipStream_t st, int mode, const char* types, Bases b, const uint32_t* maps, const GemmUnit* units) const {
         for (int c = 0; c < GEMM_N_CFG; ++c)
             if (tiles[c])
                 HIPCHK(launch_gemm(st, c, mode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
-                                   d_segs.as<GemmSeg>(), tiles[c], b, maps));
+                                   d_segs.as<GemmSeg>(), tiles[c], b, maps, units));
     }
     uint64_t device_bytes() const {
         uint64_t n = d_segs.bytes;
         for (const auto& d : d_tasks) n += d.bytes;
         return n;
     }
+};
+
+// A list of gather regions with its own source table: region dst = sum_t alpha_t src_t.
+struct GatherSet {
+    std::vector<GatherRegion> regions, small;
+    std::vector<uint64_t> src_off;
+    std::vector<double> alpha;
+    uint32_t chunks = 0;
+    DevBuf d_regions, d_small, d_src_off, d_alpha;
+
+    bool empty() const { return regions.empty() && small.empty(); }
+    void finish() {
+        std::vector<GatherRegion> big;
+        for (const auto& r : regions) (uint64_t(r.R) * r.C <= uint64_t(GATHER_SMALL) ? small : big).push_back(r);
+        regions.swap(big);
+        uint32_t ch = 0;
+        for (auto& r : regions) {
+            r.chunk_begin = ch;
+            uint64_t c = std::max<uint64_t>(1, (uint64_t(r.R) * r.C + GATHER_CHUNK - 1) / GATHER_CHUNK);
+            MQ_REQUIRE(uint64_t(ch) + c < (1ull << 31), "too many gather chunks");
+            ch += uint32_t(c);
+        }
+        chunks = ch;
+    }
+    void upload_all(hipStream_t st) {
+        upload(d_regions, regions, st);
+        upload(d_small, small, st);
+        upload(d_src_off, src_off, st);
+        upload(d_alpha, alpha, st);
+    }
+    void run(hipStream_t st, double* src, double* dst) const {
+        if (empty()) return;
+        HIPCHK(launch_gather(st, d_regions.as<GatherRegion>(), uint32_t(regions.size()), d_src_off.as<uint64_t>(),
+                             d_alpha.as<double>(), chunks, src, dst));
+        HIPCHK(launch_gather_small(st, d_small.as<GatherRegion>(), uint32_t(small.size()), d_src_off.as<uint64_t>(),
+                                   d_alpha.as<double>(), src, dst));
+    }
+    uint64_t device_bytes() const { return d_regions.bytes + d_small.bytes + d_src_off.bytes + d_alpha.bytes; }
 };
 
 struct StagePlan {
@@ -177,17 +231,35 @@ struct StagePlan {
     std::vector<GemmTask> s_tasks;
     std::vector<GemmSeg> s_segs;
     std::vector<uint32_t> maps;             // column / row block remaps of the GEMM stages
+    std::vector<GemmUnit> units;            // column units of B (grouped final stage)
     GemmStage gT, gS, gF;   // device form of the three GEMM stages
+    // grouped path: split-K partial sums of the final stage (FP64, base BASE_BRA) and their
+    // reduction into sigma
+    GatherSet red;
+    uint64_t part_elems = 0;
+    // grouped path: the S stage S_D = T_D W_D as plain dense GEMMs (rocBLAS), one per group, on the
+    // workspace (element offsets).  BF16x3 runs them as three bf16 products of the hi/lo splits of
+    // T_D and W_D (bf_hi / bf_lo, same element offsets) accumulated in FP32.
+    struct LibGemm {
+        uint64_t a_off, b_off, c_off;
+        uint32_t M, N, K, lda, ldb, ldc;
+    };
+    std::vector<LibGemm> lib;
+    rocblas_handle blas = nullptr;
+    uint64_t t_end = 0;   // T_D occupies workspace elements [0, t_end)
+    uint16_t* bf_hi = nullptr;
+    uint16_t* bf_lo = nullptr;
     uint32_t g_chunks = 0;
     uint64_t ws_elems = 0;
     uint64_t s_begin = 0, s_elems = 0;   // workspace range zero-filled at the start of each apply
-    DevBuf d_maps, d_regions, d_small, d_src_off, d_alpha;
+    DevBuf d_maps, d_units, d_regions, d_small, d_src_off, d_alpha;
     uint64_t sizes[N_BASES] = {0, 0, 0, 0, 0, 0};   // element counts of each base for validation
 
     void finish_tables() {
         gT.build(t_tasks);
         gS.build(s_tasks);
         gF.build(f_tasks);
+        red.finish();
         std::vector<GatherRegion> big;
         for (const auto& r : regions) {
             if (uint64_t(r.R) * r.C <= uint64_t(GATHER_SMALL)) small.push_back(r);
@@ -224,23 +296,49 @@ struct StagePlan {
             for (const auto& t : tasks) {
                 MQ_REQUIRE(t.ldc >= t.M || t.N <= 1, "ldc < M");
                 if (t.cmap == MAP_NONE) {
-                    check_in(t.c_base, t.c_off, extent(t.M, t.N, t.ldc), "C");
+                    check_in(t.c_base, t.c_off + t.m_off, extent(t.M, t.N, t.ldc), "C");
                 } else {
                     MQ_REQUIRE(t.cbw > 0 && t.ldc >= t.M, "column remap without block width");
                     uint32_t nblk = (t.N + t.cbw - 1) / t.cbw;
                     MQ_REQUIRE(uint64_t(t.cmap) + nblk <= maps.size(), "column map out of range");
                     for (uint32_t j = 0; j < nblk; ++j) {
                         uint32_t w = std::min(t.cbw, t.N - j * t.cbw);
-                        check_in(t.c_base, t.c_off + uint64_t(maps[t.cmap + j]) * t.cldk, extent(t.M, w, t.ldc),
-                                 "C block");
+                        check_in(t.c_base, t.c_off + t.m_off + uint64_t(maps[t.cmap + j]) * t.cldk,
+                                 extent(t.M, w, t.ldc), "C block");
                     }
                 }
                 MQ_REQUIRE(t.seg0 <= t.seg1 && t.seg1 <= segs.size(), "segment range");
+                // column units: unit of every column, and the column count of each unit
+                std::vector<uint32_t> ucols;
+                if (t.colunit != MAP_NONE) {
+                    MQ_REQUIRE(uint64_t(t.colunit) + t.N <= maps.size() && t.nunits > 0, "column unit table range");
+                    ucols.assign(t.nunits, 0);
+                    for (uint32_t n = 0; n < t.N; ++n) {
+                        MQ_REQUIRE(maps[t.colunit + n] < t.nunits, "column unit out of range");
+                        ++ucols[maps[t.colunit + n]];
+                    }
+                }
                 for (uint32_t s = t.seg0; s < t.seg1; ++s) {
                     const GemmSeg& g = segs[s];
-                    if (!g.trans_a) check_in(g.a_base, g.a_off, extent(t.M, g.K, g.lda), "A");
-                    else check_in(g.a_base, g.a_off, extent(g.K, t.M, g.lda), "A^T");
-                    if (g.bmap != MAP_NONE) {
+                    if (!g.trans_a) check_in(g.a_base, g.a_off + t.m_off, extent(t.M, g.K, g.lda), "A");
+                    else check_in(g.a_base, g.a_off + uint64_t(t.m_off) * g.lda, extent(g.K, t.M, g.lda), "A^T");
+                    if (g.units != MAP_NONE) {
+                        MQ_REQUIRE(t.colunit != MAP_NONE && g.bmap == MAP_NONE, "unit segment without column units");
+                        MQ_REQUIRE(g.bbh > 0 && g.K % g.bbh == 0, "unit segment shape");
+                        MQ_REQUIRE(uint64_t(g.units) + t.nunits <= units.size(), "unit table range");
+                        const uint32_t nblk = g.K / g.bbh;
+                        for (uint32_t u = 0; u < t.nunits; ++u) {
+                            const GemmUnit& U = units[g.units + u];
+                            if (!U.valid) continue;
+                            MQ_REQUIRE(U.col0 + ucols[u] <= t.N, "unit columns");
+                            for (uint32_t n = U.col0; n < U.col0 + ucols[u]; ++n)
+                                MQ_REQUIRE(maps[t.colunit + n] == u, "unit columns are not contiguous");
+                            MQ_REQUIRE(uint64_t(U.map) + nblk <= maps.size(), "unit map range");
+                            for (uint32_t j = 0; j < nblk; ++j)
+                                check_in(g.b_base, g.b_off + U.src + uint64_t(maps[U.map + j]) * U.ld,
+                                         uint64_t(g.bbh) * ucols[u], "B unit block");
+                        }
+                    } else if (g.bmap != MAP_NONE) {
                         MQ_REQUIRE(!g.trans_b && g.bbh > 0 && g.K % g.bbh == 0, "row remap shape");
                         uint32_t nblk = g.K / g.bbh;
                         MQ_REQUIRE(uint64_t(g.bmap) + nblk <= maps.size(), "row map out of range");
@@ -258,6 +356,20 @@ struct StagePlan {
         gemm(t_tasks, t_segs);
         gemm(s_tasks, s_segs);
         gemm(f_tasks, f_segs);
+        for (const auto& g : lib) {
+            MQ_REQUIRE(g.lda >= g.M && g.ldb >= g.K && g.ldc >= g.M, "library GEMM leading dimensions");
+            MQ_REQUIRE(g.a_off + extent(g.M, g.K, g.lda) <= t_end, "library GEMM A outside T_D");
+            check_in(BASE_WS, g.a_off, extent(g.M, g.K, g.lda), "library GEMM A");
+            check_in(BASE_WS, g.b_off, extent(g.K, g.N, g.ldb), "library GEMM B");
+            check_in(BASE_WS, g.c_off, extent(g.M, g.N, g.ldc), "library GEMM C");
+        }
+        for (const auto* list : {&red.regions, &red.small})
+            for (const auto& r : *list) {
+                check_in(BASE_OUT, r.dst_off, extent(r.R, r.C, r.ld_dst), "reduced block");
+                MQ_REQUIRE(r.t0 <= r.t1 && r.t1 <= red.src_off.size(), "reduction source range");
+                for (uint32_t t = r.t0; t < r.t1; ++t)
+                    check_in(BASE_BRA, red.src_off[t], uint64_t(r.R) * r.C, "partial sum");
+            }
         for (const auto* list : {&regions, &small})
             for (const auto& r : *list) {
                 check_in(BASE_WS, r.dst_off, extent(r.R, r.C, r.ld_dst), "S region");
@@ -274,11 +386,14 @@ struct StagePlan {
         upload(d_src_off, src_off, st);
         upload(d_alpha, alpha, st);
         upload(d_maps, maps, st);
+        upload(d_units, units, st);
+        red.upload_all(st);
     }
     // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM.
     // mode != MODE_F64 (grouped plans only): T_D, S_D and W_D are FP32 in a float workspace.
     void stage(int k, hipStream_t st, Bases b, int mode = MODE_F64) const {
         const uint32_t* mp = d_maps.as<uint32_t>();
+        const GemmUnit* un = d_units.as<GemmUnit>();
         const bool mixed = mode != MODE_F64;
         switch (k) {
             case 0:
@@ -290,18 +405,54 @@ struct StagePlan {
                 }
                 break;
             case 1:
-                gT.run(st, mode, "ddf", b, mp);
+                // the T and final stages are bandwidth-bound block products: BF16x3 runs them in FP32
+                // and uses the split products in the S stage only
+                gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, mp, un);
                 break;
             case 2:
-                gS.run(st, mode, "fff", b, mp);
+                gS.run(st, mode, "fff", b, mp, un);
+                run_lib(st, b, mode);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
             default:
-                gF.run(st, mode, "dfd", b, mp);
+                gF.run(st, mixed ? MODE_F32 : MODE_F64, "dfd", b, mp, un);
+                red.run(st, b.p[BASE_BRA], b.p[BASE_OUT]);
         }
     }
     void run(hipStream_t st, Bases b, int mode = MODE_F64) const {
         for (int k = 0; k < 4; ++k) stage(k, st, b, mode);
+    }
+    void run_lib(hipStream_t st, Bases b, int mode) const {
+        if (lib.empty()) return;
+        const rocblas_operation nn = rocblas_operation_none;
+        if (mode == MODE_F64) {
+            const double one = 1.0, zero = 0.0;
+            const double* ws = b.p[BASE_WS];
+            for (const auto& g : lib)
+                BLASCHK(rocblas_dgemm(blas, nn, nn, g.M, g.N, g.K, &one, ws + g.a_off, g.lda, ws + g.b_off, g.ldb, &zero,
+                                      const_cast<double*>(ws) + g.c_off, g.ldc));
+        } else if (mode == MODE_F32) {
+            const float one = 1.f, zero = 0.f;
+            float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
+            for (const auto& g : lib)
+                BLASCHK(rocblas_sgemm(blas, nn, nn, g.M, g.N, g.K, &one, ws + g.a_off, g.lda, ws + g.b_off, g.ldb, &zero,
+                                      ws + g.c_off, g.ldc));
+        } else {
+            MQ_REQUIRE(bf_hi && bf_lo, "bf16 split buffers not allocated");
+            float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
+            HIPCHK(launch_split_bf16(st, ws, bf_hi, bf_lo, t_end));
+            const float one = 1.f, zero = 0.f;
+            const rocblas_datatype bf = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
+            for (const auto& g : lib) {
+                // the two cross terms first, then the leading product: small terms accumulate first
+                const uint16_t* A[3] = {bf_hi + g.a_off, bf_lo + g.a_off, bf_hi + g.a_off};
+                const uint16_t* B[3] = {bf_lo + g.b_off, bf_hi + g.b_off, bf_hi + g.b_off};
+                for (int q = 0; q < 3; ++q)
+                    BLASCHK(rocblas_gemm_ex(blas, nn, nn, g.M, g.N, g.K, &one, A[q], bf, g.lda, B[q], bf, g.ldb,
+                                            q ? &one : &zero, ws + g.c_off, f32, g.ldc, ws + g.c_off, f32, g.ldc, f32,
+                                            rocblas_gemm_algo_standard, 0, 0));
+            }
+        }
     }
     // gather of both region lists: src offsets are relative to src, destinations to dst
     void run_gather(hipStream_t st, double* src, double* dst) const {
@@ -316,6 +467,21 @@ struct StagePlan {
         for (const auto& r : small) f(r);
     }
 };
+
+// x = hi + lo in bf16, round to nearest even (the device split_bf16_kernel's rule)
+static inline uint16_t bf16_rne_host(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+static inline void bf16_split_host(float x, uint16_t& hi, uint16_t& lo) {
+    hi = bf16_rne_host(x);
+    uint32_t h = uint32_t(hi) << 16;
+    float hv;
+    std::memcpy(&hv, &h, 4);
+    lo = bf16_rne_host(x - hv);
+}
 
 static_assert(MQ_PREC_F64 == MODE_F64 && MQ_PREC_F32 == MODE_F32 && MQ_PREC_BF16X3 == MODE_BF16X3,
               "precision enum and GEMM modes agree");
@@ -335,9 +501,23 @@ struct mq_plan {
     std::vector<double> ws_init;
     uint64_t ws_init_off = 0;
     DevBuf ws32;
+    DevBuf part;   // split-K partial sums of the final stage (FP64)
+    DevBuf bf;     // bf16 hi / lo splits of the workspace (BF16x3 S stage)
     // Jacobi-Davidson vectors (allocated on first use)
     DevBuf jd_buf;
     int jd_cap = 0;
+    // replayable HIP graphs of whole applications, one per (psi, sigma, precision): a Davidson
+    // iteration replays the graph of its operands instead of re-issuing every launch
+    bool use_graphs = true;
+    std::map<std::tuple<const double*, double*, int>, hipGraphExec_t> graphs;
+    std::set<std::tuple<const double*, double*, int>> seen;   // applied once eagerly
+    static constexpr size_t MAX_GRAPHS = 64;
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    ~mq_plan() { drop_graphs(); }
 
     // the base pointers of one application in the given mode
     Bases bases(const double* psi, double* sigma, int mode) {
@@ -356,12 +536,28 @@ struct mq_plan {
                 }
             }
             b.p[BASE_WS] = reinterpret_cast<double*>(ws32.as<float>());
+            if (mode == MODE_BF16X3 && !st.lib.empty() && !bf.p) {
+                // hi/lo bf16 copies of T_D (written per application) and W_D (constant, split here)
+                const uint64_t n = std::max<uint64_t>(st.ws_elems, 64);
+                bf.alloc(2 * n * sizeof(uint16_t));
+                st.bf_hi = bf.as<uint16_t>();
+                st.bf_lo = st.bf_hi + n;
+                if (!ws_init.empty()) {
+                    std::vector<uint16_t> hi(ws_init.size()), lo(ws_init.size());
+                    for (size_t i = 0; i < ws_init.size(); ++i) bf16_split_host(float(ws_init[i]), hi[i], lo[i]);
+                    HIPCHK(hipMemcpyAsync(st.bf_hi + ws_init_off, hi.data(), hi.size() * 2, hipMemcpyHostToDevice,
+                                          ctx->stream));
+                    HIPCHK(hipMemcpyAsync(st.bf_lo + ws_init_off, lo.data(), lo.size() * 2, hipMemcpyHostToDevice,
+                                          ctx->stream));
+                    HIPCHK(hipStreamSynchronize(ctx->stream));
+                }
+            }
         }
         b.p[BASE_IN] = const_cast<double*>(psi);
         b.p[BASE_OUT] = sigma;
         b.p[BASE_LEFT] = left->ptr();
         b.p[BASE_RIGHT] = right->ptr();
-        b.p[BASE_BRA] = nullptr;
+        b.p[BASE_BRA] = part.as<double>();
         return b;
     }
 };
@@ -386,8 +582,31 @@ GemmTask& add_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, uin
     t.cmap = MAP_NONE;
     t.cbw = N;
     t.cldk = 0;
+    t.colunit = MAP_NONE;
+    t.nunits = 0;
+    t.m_off = 0;
     tasks.push_back(t);
     return tasks.back();
+}
+
+// A task over an existing segment range (tasks of one slice share their segments), rows
+// [m_off, m_off + M) of op(A) and C.
+void add_gemm_range(std::vector<GemmTask>& tasks, uint8_t c_base, uint64_t c_off, uint32_t M, uint32_t N, uint32_t ldc,
+                    uint32_t seg0, uint32_t seg1, uint32_t m_off) {
+    if (M == 0 || N == 0 || seg0 == seg1) return;
+    GemmTask t{};
+    t.c_off = c_off;
+    t.M = M;
+    t.N = N;
+    t.ldc = ldc;
+    t.seg0 = seg0;
+    t.seg1 = seg1;
+    t.c_base = c_base;
+    t.cmap = MAP_NONE;
+    t.cbw = N;
+    t.colunit = MAP_NONE;
+    t.m_off = m_off;
+    tasks.push_back(t);
 }
 
 GemmSeg seg(uint8_t ab, uint64_t ao, uint32_t lda, bool ta, uint8_t bb, uint64_t bo, uint32_t ldb, bool tb, uint32_t K,
@@ -406,6 +625,7 @@ GemmSeg seg(uint8_t ab, uint64_t ao, uint32_t lda, bool ta, uint8_t bb, uint64_t
     g.bmap = MAP_NONE;
     g.bbh = 1;
     g.bldk = 0;
+    g.units = MAP_NONE;
     return g;
 }
 
@@ -490,6 +710,8 @@ int mq_ctx_create(int device, mq_ctx** out) {
         c->device = device;
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        BLASCHK(rocblas_create_handle(&c->blas));
+        BLASCHK(rocblas_set_stream(c->blas, c->stream));
         c->red_partial.alloc(RED_BLOCKS * sizeof(double));
         c->red_result.alloc(64 * sizeof(double));
         *out = c.release();
@@ -500,6 +722,7 @@ int mq_ctx_destroy(mq_ctx* ctx) {
     return guarded([&] {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -1135,87 +1358,134 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             }
         }
     }
-    // ---- S stage: S_D = T_D W_D, one task per 128-column slice of S_D whose K segments are the
-    // runs of 16-row chunks of W_D holding a nonzero in that slice (all-zero chunks add nothing)
+    // ---- S stage: S_D = T_D W_D, one plain dense GEMM per group (rocBLAS; the MPO coefficient
+    // matrices are 10-30% dense, and the library GEMM runs at near the MFMA peak)
     for (auto& g : G.groups) {
-        const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size()), Mg = uint32_t(g.M);
-        const double* Wg = ws_init.data() + (g.w_off - ws_init_off);
-        const uint32_t NT = 128, KC = 16;
-        for (uint32_t n0 = 0; n0 < Ng; n0 += NT) {
-            const uint32_t nw = std::min(NT, Ng - n0);
-            std::vector<GemmSeg> segs;
-            uint32_t run0 = MQ_NONE;
-            for (uint32_t k0 = 0; k0 < Kg + KC; k0 += KC) {
-                bool nz = false;
-                for (uint32_t n = n0; n < n0 + nw && !nz && k0 < Kg; ++n)
-                    for (uint32_t k = k0; k < std::min(k0 + KC, Kg); ++k)
-                        if (Wg[uint64_t(n) * Kg + k] != 0.0) {
-                            nz = true;
-                            break;
-                        }
-                if (nz && run0 == MQ_NONE) run0 = k0;
-                if (!nz && run0 != MQ_NONE) {
-                    const uint32_t k1 = std::min(k0, Kg);
-                    segs.push_back(seg(BASE_WS, g.t_off + uint64_t(run0) * Mg, Mg, false, BASE_WS,
-                                       g.w_off + uint64_t(n0) * Kg + run0, Kg, false, k1 - run0));
-                    run0 = MQ_NONE;
-                }
-            }
-            // an all-zero slice still has its S_D columns written (as zeros): one empty segment
-            if (segs.empty()) segs.push_back(seg(BASE_WS, g.t_off, Mg, false, BASE_WS, g.w_off + uint64_t(n0) * Kg, Kg,
-                                                 false, 0));
-            add_gemm(P.s_tasks, P.s_segs, BASE_WS, g.s_off + uint64_t(n0) * Mg, Mg, nw, Mg, segs);
-        }
+        if (!g.M || g.K.empty() || g.N.empty()) continue;
+        MQ_REQUIRE(g.M < (1ull << 31), "group too large for one library GEMM");
+        P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(g.N.size()), uint32_t(g.K.size()),
+                         uint32_t(g.M), uint32_t(g.K.size()), uint32_t(g.M)});
     }
-    // ---- final stage: per output unit (lb_out, s_out, ss2), K segments over (lc_in, b1)
+    P.t_end = P.s_begin + P.s_elems;
+    // ---- final stage: one task per output block lb_out (ls_out x all its columns), K segments over
+    // the left cohorts (lc_out, lc_in).  The block's columns fall into units (s, ss2) of rs columns;
+    // unit u of segment lb_in reads its B from S_D of the pair (lb_in, rb(s)) through the unit table,
+    // so each left cohort is read once per column tile (not once per physical state).
     for (uint32_t lbo = 0; lbo < left_i.size(); ++lbo) {
         Charge lc_out = left_i.charges[lbo];
         uint32_t ls_out = left_i.sizes[lbo];
+        const uint32_t Nb = site.rp_cols[lbo];
+        if (!ls_out || !Nb) continue;
+        struct Unit { uint32_t s, ss, rb, col0, rs; };
+        std::vector<Unit> us;
         for (uint32_t s = 0; s < phys_i.size(); ++s) {
             Charge rc_out = lc_out + phys_i.charges[s];
             uint32_t rb = right_i.position(rc_out);
             if (rb == right_i.size()) continue;
             uint32_t rs = right_i.sizes[rb];
-            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
-                std::vector<GemmSeg> segs;
-                for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
-                    auto pit = G.pair_of.find(uint64_t(lb) << 32 | rb);
-                    if (pit == G.pair_of.end()) continue;
+            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss)
+                us.push_back({s, ss, rb, uint32_t(rpb(s, rb)) + ss * rs, rs});
+        }
+        const uint32_t colunit = uint32_t(P.maps.size());
+        P.maps.resize(P.maps.size() + Nb, MQ_NONE);
+        for (uint32_t u = 0; u < us.size(); ++u)
+            for (uint32_t r = 0; r < us[u].rs; ++r) {
+                MQ_REQUIRE(us[u].col0 + r < Nb && P.maps[colunit + us[u].col0 + r] == MQ_NONE, "output column units");
+                P.maps[colunit + us[u].col0 + r] = u;
+            }
+        for (uint32_t n = 0; n < Nb; ++n) MQ_REQUIRE(P.maps[colunit + n] != MQ_NONE, "output column without unit");
+        // unit descriptors of one segment: the S_D block of the pair (lb_in, rb(s)) and the S_D column
+        // of every b1 block of the segment's K range
+        auto add_units = [&](uint32_t lb_in, const std::vector<uint32_t>& b1s) {
+            const uint32_t first = uint32_t(P.units.size());
+            for (const Unit& U : us) {
+                GemmUnit d{};
+                d.col0 = U.col0;
+                auto pit = G.pair_of.find(uint64_t(lb_in) << 32 | U.rb);
+                if (pit != G.pair_of.end()) {
                     const auto& pr = G.pairs[pit->second];
                     const auto& g = G.groups[pr.g];
-                    Charge lc_in = left_i.charges[lb];
-                    uint32_t cl = LI.cohort_index(lc_out, lc_in);
-                    if (cl >= LI.n_cohorts()) continue;
-                    uint32_t ls_in = left_i.sizes[lb];
-                    uint64_t sbase = g.s_off + pr.m_off;
-                    if (!LI.tr[cl]) {
-                        std::vector<uint32_t> jb = block_order(LI, cl);
-                        if (jb.empty()) continue;
-                        GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cl], ls_out, false, BASE_WS, sbase, ls_in, false,
-                                         uint32_t(jb.size()) * ls_in);
-                        sg.bmap = uint32_t(P.maps.size());
-                        sg.bbh = ls_in;
-                        sg.bldk = g.M;
-                        for (uint32_t j = 0; j < jb.size(); ++j)
-                            P.maps.push_back(g.nidx.at(GroupedLowering::kkey(jb[j], s, ss)));
-                        segs.push_back(sg);
-                    } else {
-                        uint32_t cle = LI.cohort_index(lc_in, lc_out);
-                        MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
-                        for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1) {
-                            if (!LI.has_block(cl, b1)) continue;
-                            uint64_t o = uint64_t(LI.offset(cl, b1));
-                            uint32_t n = g.nidx.at(GroupedLowering::kkey(b1, s, ss));
-                            segs.push_back(seg(BASE_LEFT, p->left->cohort_off[cle] + o, ls_in, true, BASE_WS,
-                                               sbase + uint64_t(n) * g.M, ls_in, false, ls_in,
-                                               LI.conjugate_scale(cl, b1)));
-                        }
-                    }
+                    d.src = g.s_off + pr.m_off;
+                    d.ld = uint32_t(g.M);
+                    d.map = uint32_t(P.maps.size());
+                    d.valid = 1;
+                    for (uint32_t b1 : b1s) P.maps.push_back(g.nidx.at(GroupedLowering::kkey(b1, U.s, U.ss)));
                 }
-                uint64_t c_off = site.rp_offset[lbo] + (rpb(s, rb) + uint64_t(ss) * rs) * ls_out;
-                add_gemm(P.f_tasks, P.f_segs, BASE_OUT, c_off, ls_out, rs, ls_out, segs);
+                P.units.push_back(d);
+            }
+            return first;
+        };
+        std::vector<GemmSeg> segs;
+        for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
+            Charge lc_in = left_i.charges[lb];
+            uint32_t cl = LI.cohort_index(lc_out, lc_in);
+            if (cl >= LI.n_cohorts()) continue;
+            uint32_t ls_in = left_i.sizes[lb];
+            if (!LI.tr[cl]) {
+                std::vector<uint32_t> jb = block_order(LI, cl);
+                if (jb.empty()) continue;
+                GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cl], ls_out, false, BASE_WS, 0, ls_in, false,
+                                 uint32_t(jb.size()) * ls_in);
+                sg.bbh = ls_in;
+                sg.units = add_units(lb, jb);
+                segs.push_back(sg);
+            } else {
+                uint32_t cle = LI.cohort_index(lc_in, lc_out);
+                MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
+                for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1) {
+                    if (!LI.has_block(cl, b1)) continue;
+                    uint64_t o = uint64_t(LI.offset(cl, b1));
+                    GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cle] + o, ls_in, true, BASE_WS, 0, ls_in, false,
+                                     ls_in, LI.conjugate_scale(cl, b1));
+                    sg.bbh = ls_in;
+                    sg.units = add_units(lb, {b1});
+                    segs.push_back(sg);
+                }
             }
         }
+        if (segs.empty()) continue;
+        // split K over several tasks when it is long, so that the few output blocks fill the GPU:
+        // each part writes its own FP64 partial block, and one gather region sums the parts into
+        // sigma in a fixed order (deterministic, no atomics)
+        uint64_t ktot = 0;
+        for (const auto& sg : segs) ktot += sg.K;
+        const uint64_t KPART = 2048;
+        const uint32_t nparts = uint32_t(std::min<uint64_t>({uint64_t(segs.size()), 32, (ktot + KPART - 1) / KPART}));
+        auto emit = [&](uint8_t c_base, uint64_t c_off, size_t a0, size_t a1) {
+            std::vector<GemmSeg> part(segs.begin() + a0, segs.begin() + a1);
+            GemmTask& t = add_gemm(P.f_tasks, P.f_segs, c_base, c_off, ls_out, Nb, ls_out, part);
+            t.colunit = colunit;
+            t.nunits = uint32_t(us.size());
+        };
+        if (nparts <= 1) {
+            emit(BASE_OUT, site.rp_offset[lbo], 0, segs.size());
+            continue;
+        }
+        GatherRegion r{};
+        r.dst_off = site.rp_offset[lbo];
+        r.R = ls_out;
+        r.C = Nb;
+        r.ld_dst = ls_out;
+        r.t0 = uint32_t(P.red.src_off.size());
+        size_t a0 = 0;
+        uint64_t acc = 0;
+        for (uint32_t q = 0; q < nparts && a0 < segs.size(); ++q) {
+            // parts of about ktot / nparts each; the last takes the rest
+            const uint64_t target = (ktot * (q + 1)) / nparts;
+            size_t a1 = a0;
+            while (a1 < segs.size() && (acc < target || a1 == a0)) acc += segs[a1++].K;
+            if (q + 1 == nparts) {
+                while (a1 < segs.size()) acc += segs[a1++].K;
+            }
+            const uint64_t off = P.part_elems;
+            P.part_elems += align32(uint64_t(ls_out) * Nb);
+            emit(BASE_BRA, off, a0, a1);
+            P.red.src_off.push_back(off);
+            P.red.alpha.push_back(1.0);
+            a0 = a1;
+        }
+        r.t1 = uint32_t(P.red.src_off.size());
+        P.red.regions.push_back(r);
     }
     P.ws_elems = ws;
     p->flops = G.ref_flops;
@@ -1303,12 +1573,15 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         uint64_t ws_init_off = 0;
         lower_sigma_grouped(p.get(), L, W, ws_init, ws_init_off);
         StagePlan& P = p->st;
+        P.blas = ctx->blas;
         P.finish_tables();
         P.sizes[BASE_WS] = P.ws_elems;
         P.sizes[BASE_IN] = P.sizes[BASE_OUT] = L.rp_total;
         P.sizes[BASE_LEFT] = left->total;
         P.sizes[BASE_RIGHT] = right->total;
+        P.sizes[BASE_BRA] = P.part_elems;
         P.validate();
+        if (P.part_elems) p->part.alloc(P.part_elems * sizeof(double));
         p->ws.alloc(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
         if (!ws_init.empty())
             HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
@@ -1343,9 +1616,54 @@ int mq_sigma_apply_ex(mq_plan* p, const double* psi, double* sigma, int precisio
         MQ_REQUIRE(psi != sigma, "psi and sigma must not alias");
         MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
                    "unknown precision");
-        Bases b = p->bases(psi, sigma, precision);
-        HIPCHK(launch_fill(p->ctx->stream, sigma, p->vec_size, 0.0));
-        p->st.run(p->ctx->stream, b, precision);
+        Bases b = p->bases(psi, sigma, precision);   // allocates the FP32 workspace before any capture
+        hipStream_t st = p->ctx->stream;
+        auto issue = [&] {
+            HIPCHK(launch_fill(st, sigma, p->vec_size, 0.0));
+            p->st.run(st, b, precision);
+        };
+        if (!p->use_graphs) {
+            issue();
+            return;
+        }
+        auto key = std::make_tuple(psi, sigma, precision);
+        auto it = p->graphs.find(key);
+        if (it == p->graphs.end() && !p->seen.count(key)) {
+            // the first application with these operands runs eagerly: libraries load their code and
+            // workspaces on first use, which a capture must not contain
+            if (p->seen.size() >= mq_plan::MAX_GRAPHS) p->seen.clear();
+            p->seen.insert(key);
+            issue();
+            return;
+        }
+        if (it == p->graphs.end()) {
+            if (p->graphs.size() >= mq_plan::MAX_GRAPHS) p->drop_graphs();
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                issue();
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(st, &g));
+            hipGraphExec_t ge = nullptr;
+            hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(e);
+            it = p->graphs.emplace(key, ge).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second, st));
+    });
+}
+
+int mq_plan_set_graphs(mq_plan* p, int enable) {
+    return guarded([&] {
+        MQ_REQUIRE(p, "bad argument");
+        HIPCHK(hipStreamSynchronize(p->ctx->stream));
+        p->use_graphs = enable != 0;
+        if (!p->use_graphs) p->drop_graphs();
     });
 }
 
@@ -1395,6 +1713,7 @@ int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, int 
             ff = gemm_flops(P.f_tasks, P.f_segs);
             P.for_regions([&](const GatherRegion& r) { fg += 2ull * r.R * r.C * (r.t1 - r.t0); });
             fg += gemm_flops(P.s_tasks, P.s_segs);
+            for (const auto& g : P.lib) fg += 2ull * g.M * g.N * g.K;
             flops_out[0] = ft;
             flops_out[1] = fg;
             flops_out[2] = ff;
@@ -1425,8 +1744,9 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
     return guarded([&] {
         MQ_REQUIRE(p && bytes, "bad argument");
         const StagePlan& P = p->st;
-        *bytes = p->ws.bytes + p->ws32.bytes + P.gT.device_bytes() + P.gS.device_bytes() + P.gF.device_bytes() +
-                 P.d_maps.bytes + P.d_regions.bytes + P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
+        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gS.device_bytes() +
+                 P.gF.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_regions.bytes +
+                 P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
     });
 }
 

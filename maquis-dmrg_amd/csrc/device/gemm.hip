@@ -62,18 +62,30 @@ __device__ inline f4 mfma3(const s8& ah, const s8& al, const s8& bh, const s8& b
 }
 
 // ------------------------------------------------------------------------------ operand access
-// element (m, k) of op(A) and (k, n) of op(B) of one segment; the caller checks the bounds
+// Element loads are branch-free: an out-of-range element reads offset 0 of its operand (the first
+// element of a stored block: a valid address holding finite data) and is scaled by zero.  The value
+// is always used, so the compiler keeps the load unconditional and the loads of a step stay in
+// flight together instead of each sitting in a divergent branch behind a full wait.
 template <typename TA>
-__device__ inline double a_at(const TA* A, const GemmSeg& sg, uint32_t m, uint32_t k) {
-    return sg.trans_a ? double(A[uint64_t(m) * sg.lda + k]) : double(A[uint64_t(k) * sg.lda + m]);
+__device__ inline double a_load(const TA* A, const GemmSeg& sg, bool ok, uint32_t m, uint32_t k) {
+    const uint64_t sm = sg.trans_a ? sg.lda : 1u, sk = sg.trans_a ? 1u : sg.lda;
+    const double v = double(A[(uint64_t(m) * sm + uint64_t(k) * sk) & (0ull - uint64_t(ok))]);
+    return v * (ok ? sg.alpha : 0.0);
 }
 template <typename TB>
-__device__ inline double b_at(const TB* B, const GemmSeg& sg, const uint32_t* maps, uint32_t k, uint32_t n) {
+__device__ inline double b_load(const TB* B, const GemmSeg& sg, const uint32_t* maps, bool ok, uint32_t k,
+                                uint32_t n) {
+    uint64_t off;
     if (sg.bmap != MAP_NONE) {
-        const uint32_t blk = k / sg.bbh;
-        return double(B[uint64_t(maps[sg.bmap + blk]) * sg.bldk + (k - blk * sg.bbh) + uint64_t(n) * sg.ldb]);
+        const uint32_t blk = min(k, sg.K - 1) / sg.bbh;
+        const uint64_t row = maps[sg.bmap + blk];
+        off = row * sg.bldk + (k - blk * sg.bbh) + uint64_t(n) * sg.ldb;
+    } else {
+        const uint64_t sk = sg.trans_b ? sg.ldb : 1u, sn = sg.trans_b ? 1u : sg.ldb;
+        off = uint64_t(k) * sk + uint64_t(n) * sn;
     }
-    return sg.trans_b ? double(B[uint64_t(k) * sg.ldb + n]) : double(B[uint64_t(n) * sg.ldb + k]);
+    const double v = double(B[off & (0ull - uint64_t(ok))]);
+    return v * (ok ? 1.0 : 0.0);
 }
 
 template <typename TC>
@@ -96,6 +108,7 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
 }
 
 __device__ inline uint32_t find_task(const GemmTask* tasks, uint32_t n_tasks, uint32_t tile) {
+    if (n_tasks == gridDim.x) return tile;   // one tile per task
     uint32_t lo = 0, hi = n_tasks;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
@@ -121,7 +134,8 @@ struct Smem {
 template <typename TA, typename TB, typename TC, int MODE, int PM, int PN, int WM, int WN>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                             const GemmSeg* __restrict__ segs, Bases bases,
-                                                            const uint32_t* __restrict__ maps) {
+                                                            const uint32_t* __restrict__ maps,
+                                                            const GemmUnit* __restrict__ units) {
     static_assert(PM * PN * 64 == GEMM_THREADS, "four waves");
     typedef typename ModeTraits<MODE>::T T;
     typedef typename ModeTraits<MODE>::Acc Acc;
@@ -164,29 +178,106 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
         }
         return false;
     };
-    // staged operands of one step, in registers, with the index order they were loaded in
-    double ra[NA], rb[NB];
+    // staged operands of one step, in registers as loaded (storage type), with their in-range masks
+    // and the index order they were loaded in.  Nothing is computed on the loaded values until
+    // lstore, after the MFMAs of the current step: the loads stay in flight across them.
+    TA va[NA];
+    TB vb[NB];
+    uint32_t oka = 0, okb = 0;
+    double r_alpha = 1.0;
     bool r_trans_a = false, r_kfast_b = true;
-    auto gload = [&]() {
-        const TA* __restrict__ A = reinterpret_cast<const TA*>(bases.p[sg.a_base]) + sg.a_off;
-        const TB* __restrict__ B = reinterpret_cast<const TB*>(bases.p[sg.b_base]) + sg.b_off;
-        r_trans_a = sg.trans_a;
-        r_kfast_b = sg.bmap != MAP_NONE || !sg.trans_b;
+    auto load_a = [&](const TA* __restrict__ A) {
+        const uint64_t sm = sg.trans_a ? sg.lda : 1u, sk = sg.trans_a ? 1u : sg.lda;
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
             const int e = tid + GEMM_THREADS * i;
             const int m = r_trans_a ? e / BK : e % BM;
             const int k = r_trans_a ? e % BK : e / BM;
             const uint32_t gm = m0 + m, gk = k0 + k;
-            ra[i] = (gm < M && gk < sg.K) ? sg.alpha * a_at(A, sg, gm, gk) : 0.0;
+            const uint32_t ok = uint32_t(gm < M) & uint32_t(gk < sg.K);
+            oka |= ok << i;
+            va[i] = A[(uint64_t(gm + tk.m_off) * sm + uint64_t(gk) * sk) & (0ull - uint64_t(ok))];
         }
+    };
+    // column units: the B source of this thread's columns for the current segment (the columns of
+    // a thread's elements, n0 + (tid + 256 i) / BK, are fixed across the K steps)
+    uint64_t u_src[NB];
+    uint32_t u_ld[NB], u_map[NB];
+    uint32_t u_ok = 0, u_sgi = 0xffffffffu;
+    auto resolve_units = [&]() {
+        u_ok = 0;
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             const int e = tid + GEMM_THREADS * i;
-            const int n = r_kfast_b ? e / BK : e % BN;
-            const int k = r_kfast_b ? e % BK : e / BN;
-            const uint32_t gn = n0 + n, gk = k0 + k;
-            rb[i] = (gn < N && gk < sg.K) ? b_at(B, sg, maps, gk, gn) : 0.0;
+            const uint32_t n = n0 + e / BK;
+            const uint32_t nc = min(n, N - 1);
+            const GemmUnit U = units[sg.units + maps[tk.colunit + nc]];
+            const uint32_t ok = uint32_t(n < N) & uint32_t(U.valid != 0);
+            u_ok |= ok << i;
+            u_src[i] = U.src + uint64_t(sg.bbh) * (nc - U.col0);
+            u_ld[i] = U.ld;
+            u_map[i] = ok ? U.map : 0u;
+        }
+        u_sgi = sgi;
+    };
+    auto gload = [&]() {
+        const TA* __restrict__ A = reinterpret_cast<const TA*>(bases.p[sg.a_base]) + sg.a_off;
+        const TB* __restrict__ B = reinterpret_cast<const TB*>(bases.p[sg.b_base]) + sg.b_off;
+        r_trans_a = sg.trans_a;
+        r_kfast_b = sg.bmap != MAP_NONE || sg.units != MAP_NONE || !sg.trans_b;
+        r_alpha = sg.alpha;
+        oka = okb = 0;
+        if (sg.units != MAP_NONE) {
+            // every element of this thread has k = k0 + tid % BK: one block row for all of them
+            if (u_sgi != sgi) resolve_units();
+            const uint32_t gk = k0 + uint32_t(tid % BK);
+            const uint32_t okk = uint32_t(gk < sg.K);
+            const uint32_t blk = min(gk, sg.K - 1) / sg.bbh;
+            const uint32_t kin = gk - blk * sg.bbh;
+            uint32_t row[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) row[i] = maps[(u_map[i] + blk) & (0u - ((u_ok >> i) & 1u))];
+            load_a(A);
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const uint32_t ok = ((u_ok >> i) & 1u) & okk;
+                okb |= ok << i;
+                vb[i] = B[(u_src[i] + uint64_t(row[i]) * u_ld[i] + kin) & (0ull - uint64_t(ok))];
+            }
+        } else if (sg.bmap != MAP_NONE) {
+            // row-block remapped B: the map entries of all elements first, then the A loads, then
+            // the B loads (only these wait for the map entries)
+            uint32_t row[NB], kin[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int e = tid + GEMM_THREADS * i;
+                const uint32_t gk = k0 + e % BK;
+                const uint32_t blk = min(gk, sg.K - 1) / sg.bbh;
+                kin[i] = gk - blk * sg.bbh;
+                row[i] = maps[sg.bmap + blk];
+            }
+            load_a(A);
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int e = tid + GEMM_THREADS * i;
+                const uint32_t gn = n0 + e / BK, gk = k0 + e % BK;
+                const uint32_t ok = uint32_t(gn < N) & uint32_t(gk < sg.K);
+                okb |= ok << i;
+                vb[i] = B[(uint64_t(row[i]) * sg.bldk + kin[i] + uint64_t(gn) * sg.ldb) & (0ull - uint64_t(ok))];
+            }
+        } else {
+            const uint64_t bsk = sg.trans_b ? sg.ldb : 1u, bsn = sg.trans_b ? 1u : sg.ldb;
+            load_a(A);
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int e = tid + GEMM_THREADS * i;
+                const int n = r_kfast_b ? e / BK : e % BN;
+                const int k = r_kfast_b ? e % BK : e / BN;
+                const uint32_t gn = n0 + n, gk = k0 + k;
+                const uint32_t ok = uint32_t(gn < N) & uint32_t(gk < sg.K);
+                okb |= ok << i;
+                vb[i] = B[(uint64_t(gk) * bsk + uint64_t(gn) * bsn) & (0ull - uint64_t(ok))];
+            }
         }
     };
     auto lstore = [&](int buf) {
@@ -196,11 +287,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             const int e = tid + GEMM_THREADS * i;
             const int m = r_trans_a ? e / BK : e % BM;
             const int k = r_trans_a ? e % BK : e / BM;
+            const double v = double(va[i]) * (((oka >> i) & 1u) ? r_alpha : 0.0);
             if constexpr (MODE == MODE_BF16X3) {
                 short(*Ah)[RK] = reinterpret_cast<short(*)[RK]>(base);
-                bf16_split(float(ra[i]), Ah[m][k], Ah[BM + m][k]);
+                bf16_split(float(v), Ah[m][k], Ah[BM + m][k]);
             } else {
-                reinterpret_cast<T(*)[BM + 1]>(base)[k][m] = T(ra[i]);
+                reinterpret_cast<T(*)[BM + 1]>(base)[k][m] = T(v);
             }
         }
 #pragma unroll
@@ -208,11 +300,12 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             const int e = tid + GEMM_THREADS * i;
             const int n = r_kfast_b ? e / BK : e % BN;
             const int k = r_kfast_b ? e % BK : e / BN;
+            const double v = double(vb[i]) * (((okb >> i) & 1u) ? 1.0 : 0.0);
             if constexpr (MODE == MODE_BF16X3) {
                 short(*Bh)[RK] = reinterpret_cast<short(*)[RK]>(base) + 2 * BM;
-                bf16_split(float(rb[i]), Bh[n][k], Bh[BN + n][k]);
+                bf16_split(float(v), Bh[n][k], Bh[BN + n][k]);
             } else {
-                reinterpret_cast<T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1))[k][n] = T(rb[i]);
+                reinterpret_cast<T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1))[k][n] = T(v);
             }
         }
     };
@@ -284,8 +377,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                 const uint32_t gm = m0 + wm + 16 * i + d_row<MODE>(r, lk);
                 if (gm < M) {
                     TC v = TC(acc[i][j][r]);
-                    if (tk.beta) v += Cn[gm];
-                    Cn[gm] = v;
+                    if (tk.beta) v += Cn[gm + tk.m_off];
+                    Cn[gm + tk.m_off] = v;
                 }
             }
     }
@@ -297,7 +390,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
 template <typename TA, typename TB, typename TC, int MODE>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                                  const GemmSeg* __restrict__ segs, Bases bases,
-                                                                 const uint32_t* __restrict__ maps) {
+                                                                 const uint32_t* __restrict__ maps,
+                                                                 const GemmUnit* __restrict__ units) {
     typedef typename ModeTraits<MODE>::T T;
     typedef typename ModeTraits<MODE>::Acc Acc;
     constexpr int KS = ModeTraits<MODE>::KS;
@@ -319,17 +413,32 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask*
         const TA* __restrict__ A = reinterpret_cast<const TA*>(bases.p[sg.a_base]) + sg.a_off;
         const TB* __restrict__ B = reinterpret_cast<const TB*>(bases.p[sg.b_base]) + sg.b_off;
         const uint32_t K = sg.K;
-        const double alpha = sg.alpha;
         const uint32_t nsteps = (K + KS - 1) / KS;
-        for (uint32_t s = (uint32_t(wave) + NW - step % NW) % NW; s < nsteps; s += NW) {
+        const uint32_t first = (uint32_t(wave) + NW - step % NW) % NW;
+        if constexpr (MODE != MODE_BF16X3) {
+            // U steps per pass: all their loads are issued before the MFMAs consume them
+            constexpr int U = 8;
+            for (uint32_t s = first; s < nsteps; s += NW * U) {
+                T a[U], b[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t k = (s + NW * u) * KS + lk;
+                    a[u] = T(a_load(A, sg, (row < M) & (k < K), row + tk.m_off, k));
+                    b[u] = T(b_load(B, sg, maps, (col < N) & (k < K), k, col));
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc = mfma_step(a[u], b[u], acc);
+            }
+        }
+        for (uint32_t s = first; MODE == MODE_BF16X3 && s < nsteps; s += NW) {
             const uint32_t k0 = s * KS;
             if constexpr (MODE == MODE_BF16X3) {
                 s8 ah, al, bh, bl;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const uint32_t k = k0 + 8 * lk + j;
-                    const float a = (row < M && k < K) ? float(alpha * a_at(A, sg, row, k)) : 0.f;
-                    const float b = (col < N && k < K) ? float(b_at(B, sg, maps, k, col)) : 0.f;
+                    const float a = float(a_load(A, sg, (row < M) & (k < K), row + tk.m_off, k));
+                    const float b = float(b_load(B, sg, maps, (col < N) & (k < K), k, col));
                     short h, l;
                     bf16_split(a, h, l);
                     ah[j] = h;
@@ -339,11 +448,6 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask*
                     bl[j] = l;
                 }
                 acc = mfma3(ah, al, bh, bl, acc);
-            } else {
-                const uint32_t k = k0 + lk;
-                const T a = (row < M && k < K) ? T(alpha * a_at(A, sg, row, k)) : T(0);
-                const T b = (col < N && k < K) ? T(b_at(B, sg, maps, k, col)) : T(0);
-                acc = mfma_step(a, b, acc);
             }
         }
         step += nsteps;
@@ -364,8 +468,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask*
                 const uint32_t gm = d_row<MODE>(r, lk);
                 if (gm < M) {
                     TC v = TC(acc[r]);
-                    if (tk.beta) v += Cn[gm];
-                    Cn[gm] = v;
+                    if (tk.beta) v += Cn[gm + tk.m_off];
+                    Cn[gm + tk.m_off] = v;
                 }
             }
         }
@@ -375,28 +479,28 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask*
 // ------------------------------------------------------------------------------ dispatch
 template <typename TA, typename TB, typename TC, int MODE>
 static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                               uint32_t n_tiles, Bases bases, const uint32_t* maps) {
+                               uint32_t n_tiles, Bases bases, const uint32_t* maps, const GemmUnit* units) {
     const dim3 grid(n_tiles), block(GEMM_THREADS);
     switch (cfg) {
         case GEMM_CFG_128x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 4, 4>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps);
+                               bases, maps, units);
             break;
         case GEMM_CFG_64x64:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps);
+                               bases, maps, units);
             break;
         case GEMM_CFG_32x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps);
+                               bases, maps, units);
             break;
         case GEMM_CFG_16x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 1, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps);
+                               bases, maps, units);
             break;
         case GEMM_CFG_16x16:
             hipLaunchKernelGGL((gemm_tiny_kernel<TA, TB, TC, MODE>), grid, block, 0, st, tasks, n_tasks, segs, bases,
-                               maps);
+                               maps, units);
             break;
         default:
             return hipErrorInvalidValue;
@@ -405,18 +509,19 @@ static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, u
 }
 
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
-                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps) {
+                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps,
+                       const GemmUnit* units) {
     if (n_tiles == 0) return hipSuccess;
     if (mode == MODE_F64)
-        return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);
+        return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);
     const bool da = types[0] == 'd', db = types[1] == 'd', dc = types[2] == 'd';
 #define MQ_DISPATCH(MD)                                                                                          \
     if (da && db && !dc)                                                                                         \
-        return launch_typed<double, double, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);     \
+        return launch_typed<double, double, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);     \
     if (!da && !db && !dc)                                                                                       \
-        return launch_typed<float, float, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);       \
+        return launch_typed<float, float, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);       \
     if (da && !db && dc)                                                                                         \
-        return launch_typed<double, float, double, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps);
+        return launch_typed<double, float, double, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);
     if (mode == MODE_F32) { MQ_DISPATCH(MODE_F32) }
     if (mode == MODE_BF16X3) { MQ_DISPATCH(MODE_BF16X3) }
 #undef MQ_DISPATCH

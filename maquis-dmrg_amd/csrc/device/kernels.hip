@@ -127,6 +127,29 @@ hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v) {
     return hipGetLastError();
 }
 
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even (finite inputs): the
+// operands of the three bf16 products of the BF16x3 GEMMs.
+__device__ inline uint16_t bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__global__ void split_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                  uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const float v = x[i];
+        const uint16_t h = bf16_rne(v);
+        hi[i] = h;
+        lo[i] = bf16_rne(v - __uint_as_float(uint32_t(h) << 16));
+    }
+}
+
+hipError_t launch_split_bf16(hipStream_t st, const float* x, uint16_t* hi, uint16_t* lo, uint64_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, x, hi, lo, n);
+    return hipGetLastError();
+}
+
 __global__ void axpy_kernel(uint64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         y[i] += a * x[i];

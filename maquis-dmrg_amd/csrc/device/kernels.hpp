@@ -31,6 +31,12 @@ struct Bases {
 //      c_off + maps[cmap + j] * cldk + (n % cbw) * ldc
 //  row-block load remap of B (non-transposed B only): row k of B is row k % bbh of block
 //      j = k / bbh, read at b_off + maps[bmap + j] * bldk + (k % bbh) + n * ldb
+//  column units of B: output column n of a task with colunit != MAP_NONE belongs to unit
+//      u = maps[colunit + n] (u < nunits); a segment with units != MAP_NONE reads, for unit u,
+//      the descriptor U = unit table[units + u] and
+//          B(k, n) = b_off + U.src + maps[U.map + k / bbh] * U.ld + (k % bbh) + bbh * (n - U.col0)
+//      (zero if U.valid == 0).  Used where the output columns of one task gather their B blocks
+//      from different workspace matrices.
 struct GemmTask {
     uint64_t c_off;
     uint64_t cldk;
@@ -39,6 +45,8 @@ struct GemmTask {
     uint32_t tiles_n;
     uint32_t tile_begin;
     uint32_t cmap, cbw;
+    uint32_t colunit, nunits;
+    uint32_t m_off;   // row offset of the task in op(A) and C (tasks sharing one segment range)
     uint8_t c_base, beta, pad0, pad1;
 };
 
@@ -48,7 +56,13 @@ struct GemmSeg {
     double alpha;
     uint32_t K, lda, ldb;
     uint32_t bmap, bbh;
+    uint32_t units;
     uint8_t a_base, b_base, trans_a, trans_b;
+};
+
+struct GemmUnit {
+    uint64_t src;
+    uint32_t ld, map, col0, valid;
 };
 
 // S-stage region: dst (R x C, column stride ld_dst) = sum_{t in [t0,t1)} alpha_t * src_t,
@@ -80,7 +94,8 @@ constexpr int GEMM_THREADS = 256;
 // Otherwise `types` gives the storage of A, B and C as 'd' (double) or 'f' (float): "ddf",
 // "fff" or "dfd".
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
-                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps);
+                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps,
+                       const GemmUnit* units);
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
 constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions
@@ -93,6 +108,8 @@ hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n
                          const uint64_t* src_off, const double* alpha, uint32_t n_chunks, double* ws,
                          double* dst_base);
 hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v);
+// hi[i] = bf16(x[i]), lo[i] = bf16(x[i] - hi[i]) (round to nearest even)
+hipError_t launch_split_bf16(hipStream_t st, const float* x, uint16_t* hi, uint16_t* lo, uint64_t n);
 hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial,
                       double* result);
 hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y);

@@ -130,6 +130,7 @@ SIGNATURES = {
                                                      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), P, P, PP]),
     "mq_sigma_apply": (C.c_int, [P, P, P]),
     "mq_sigma_apply_ex": (C.c_int, [P, P, P, C.c_int]),
+    "mq_plan_set_graphs": (C.c_int, [P, C.c_int]),
     "mq_plan_profile": (C.c_int, [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_double), u64p, u64p]),
     "mq_plan_vector_size": (C.c_int, [P, u64p]),
     "mq_plan_flops": (C.c_int, [P, u64p]),

@@ -302,6 +302,10 @@ class SigmaPlan:
         assert psi.numel() >= self.n and sigma.numel() >= self.n
         check(lib().mq_sigma_apply_ex(self.h, psi.data_ptr(), sigma.data_ptr(), int(precision)))
 
+    def set_graphs(self, enable):
+        """Replay captured HIP graphs of whole applications (default on)."""
+        check(lib().mq_plan_set_graphs(self.h, int(bool(enable))))
+
     def profile(self, psi, sigma, reps=5, precision=PREC_F64):
         """Average ms per stage [fill, T GEMM, gather, final GEMM] and the algorithmic flops of
         the three compute stages and bytes of the gather."""

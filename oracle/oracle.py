@@ -34,6 +34,7 @@ def lib():
         L.mqo_wigner9j.argtypes = [C.c_int] * 9
         L.mqo_set_coupling.argtypes = [C.c_int] * 9 + [C.c_double, P]
         L.mqo_schedule_sigma.argtypes = [P, P, P, P, C.POINTER(C.c_void_p)]
+        L.mqo_schedule_sigma_blocks.argtypes = [P, P, P, P, P, C.POINTER(C.c_void_p)]
         L.mqo_schedule_export.argtypes = [P, P, C.POINTER(C.c_uint64)]
         L.mqo_schedule_free.argtypes = [P]
         L.mqo_sigma_apply.argtypes = [P, P, P, P, P, P, P, P]
@@ -97,7 +98,9 @@ def schedule_sigma(site, mpo, left, right):
 class SigmaOracle:
     """create_contraction_schedule + super_hamil_mv (CPU branch) on host data."""
 
-    def __init__(self, site, mpo, left, right):
+    def __init__(self, site, mpo, left, right, only_blocks=None):
+        """only_blocks: optional iterable of ket block (left sector) positions; sigma is then the
+        contribution of those MPS blocks alone (a bounded sample for timing)."""
         self.site, self.left, self.right = site, left, right
         self.keep = []
         self.ld = left.index.desc(self.keep)
@@ -105,8 +108,14 @@ class SigmaOracle:
         self.lp = _cohort_ptrs(left.cohorts)
         self.rp = _cohort_ptrs(right.cohorts)
         self.h = C.c_void_p()
-        _chk(lib().mqo_schedule_sigma(C.byref(site.desc), C.byref(mpo.desc), C.byref(self.ld), C.byref(self.rd),
-                                      C.byref(self.h)))
+        only = None
+        if only_blocks is not None:
+            mask = np.zeros(site.desc.left_i.n, np.uint8)
+            mask[list(only_blocks)] = 1
+            self.keep.append(mask)
+            only = mask.ctypes.data
+        _chk(lib().mqo_schedule_sigma_blocks(C.byref(site.desc), C.byref(mpo.desc), C.byref(self.ld),
+                                             C.byref(self.rd), only, C.byref(self.h)))
         d = capi.mq_schedule_desc()
         f = C.c_uint64()
         _chk(lib().mqo_schedule_export(self.h, C.byref(d), C.byref(f)))

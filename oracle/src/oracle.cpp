@@ -909,9 +909,11 @@ void mqo_set_coupling(int a, int b, int c, int d, int e, int f, int g, int h, in
     set_coupling(a, b, c, d, e, f, g, h, i, init, out4);
 }
 
-// create_contraction_schedule (site_hamil_schedule.hpp:40-93)
-int mqo_schedule_sigma(const mq_site_desc* sd, const mq_mpo_desc* md, const mq_bindex_desc* ld,
-                       const mq_bindex_desc* rd, void** out) {
+// create_contraction_schedule (site_hamil_schedule.hpp:40-93).  only (may be NULL): per MPS block
+// a flag; blocks without it get an empty schedule, so that sigma is the contribution of the
+// flagged ket blocks alone (a bounded sample of the full work for timing).
+int mqo_schedule_sigma_blocks(const mq_site_desc* sd, const mq_mpo_desc* md, const mq_bindex_desc* ld,
+                              const mq_bindex_desc* rd, const uint8_t* only, void** out) {
     return guard([&] {
         Site st(*sd);
         MPO mpo(*md);
@@ -922,6 +924,7 @@ int mqo_schedule_sigma(const mq_site_desc* sd, const mq_mpo_desc* md, const mq_b
         s->blocks.resize(st.left_i.size());
 #pragma omp parallel for schedule(dynamic, 1)
         for (long mb = 0; mb < long(st.left_i.size()); ++mb) {
+            if (only && !only[mb]) continue;
             rshtm_t_tasks(right, st.left_i, st.right_i, st.phys_i, rpb, unsigned(mb), s->blocks[mb]);
             shtm_tasks(mpo, left, right, st, rpb, unsigned(mb), s->blocks[mb]);
         }
@@ -968,6 +971,11 @@ int mqo_schedule_export(void* h, mq_schedule_desc* d, uint64_t* flops) {
         d->coh_mpo_offsets = s->coh_mpo_offsets.empty() ? nullptr : s->coh_mpo_offsets.data();
         if (flops) *flops = s->flops;
     });
+}
+
+int mqo_schedule_sigma(const mq_site_desc* sd, const mq_mpo_desc* md, const mq_bindex_desc* ld,
+                       const mq_bindex_desc* rd, void** out) {
+    return mqo_schedule_sigma_blocks(sd, md, ld, rd, nullptr, out);
 }
 
 void mqo_schedule_free(void* h) { delete (Sched*)h; }

@@ -205,6 +205,18 @@ def test_index_only_steps_equal_full_builders(model, symmetric):
         right = full
 
 
+def test_oracle_block_sample_is_additive():
+    """The bounded CPU sample (sigma of a subset of MPS blocks) partitions sigma and its flops."""
+    p = SiteProblem("hubbard", L=8, site=3, Mmax=10, seed=5)
+    full = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    n = p.site_desc.desc.left_i.n
+    a = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb, only_blocks=range(0, n, 2))
+    b = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb, only_blocks=range(1, n, 2))
+    r = full.apply(p.psi)
+    np.testing.assert_allclose(a.apply(p.psi) + b.apply(p.psi), r, rtol=0, atol=1e-13 * np.abs(r).max())
+    assert a.flops + b.flops == full.flops and 0 < a.flops < full.flops
+
+
 def test_planner_rejects_mismatched_boundary():
     p = SiteProblem("heisenberg", L=8, site=3, Mmax=6, seed=1)
     q = SiteProblem("heisenberg", L=8, site=2, Mmax=6, seed=1)

@@ -276,3 +276,33 @@ def test_refined_jd_reaches_fp64_energy(eng, model, precision):
     v = out.cpu().numpy()
     assert abs(np.linalg.norm(v) - 1.0) <= 1e-12
     assert abs(v @ so.apply(v) - e) <= 1e-10 * max(1.0, abs(e))
+
+
+@pytest.mark.parametrize("precision", [E.PREC_F64, E.PREC_F32])
+def test_graph_replay_equals_eager(eng, precision):
+    """Captured applications replay to the eager result, bit for bit, on every replay, with eager
+    work and synchronisations of the program between the replays."""
+    p = SiteProblem("hubbard", L=8, site=3, Mmax=10, seed=5)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    psi = dev(p.psi)
+    psi2 = dev(p.psi[::-1].copy())
+    plan.set_graphs(False)
+    eager = torch.zeros_like(psi)
+    plan.apply(psi, eager, precision=precision)
+    eager2 = torch.zeros_like(psi)
+    plan.apply(psi2, eager2, precision=precision)
+    eng.sync()
+    plan.set_graphs(True)
+    out = torch.zeros_like(psi)
+    out2 = torch.zeros_like(psi)
+    for rep in range(4):
+        out.fill_(np.nan)
+        plan.apply(psi, out, precision=precision)      # captured on the first pass, replayed after
+        eng.sync()
+        assert torch.equal(out, eager), rep
+        plan.set_graphs(False) if rep == 2 else None  # eager work between replays
+        plan.apply(psi2, out2, precision=precision)
+        eng.sync()
+        assert torch.equal(out2, eager2), rep
+        plan.set_graphs(True)
