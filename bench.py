@@ -5,9 +5,10 @@ One step = one application of the two-site effective Hamiltonian (sigma = H_eff 
 middle site of the chain, inputs resident in HBM.  value = sigma TFLOP/s with the reference's own
 flop model (solver/tasks.cpp:388-398, 777-796) in the reported precision (default: the fastest
 reduced-precision path whose relative sigma error stays within 1e-4).  The FP64 plain path and
-every reduced precision are timed in the same run, and the site's eigenvalue is solved with the
-plain FP64 Jacobi-Davidson and with the refined solver (reduced-precision sigma, FP64 residual)
-to report the energy difference.  The roofline entry is the dominant kernel of the step, timed
+every reduced precision are timed in the same run.  A two-site DMRG run of the same ab initio
+Hamiltonian at a smaller size is then swept with the FP64 sigma and with the reported precision
+(refined Jacobi-Davidson: reduced-precision products, FP64 Rayleigh quotient and residual) to
+report the energy difference in Eh and the sweep wall times.  The roofline entry is the dominant kernel of the step, timed
 with HIP events on the engine's stream; the CPU baseline is the oracle (the CPU restatement of
 the reference's CPU branch) on a bounded sample of the same sigma (a subset of its MPS blocks)
 on the host cores.
@@ -48,7 +49,10 @@ def parse():
                     help="precision reported as value (auto: fastest reduced precision within the sigma bar)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-jd", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-L", type=int, default=16, help="chain length of the sweep comparison")
+    ap.add_argument("--sweep-m", type=int, default=256, help="bond dimension of the sweep comparison")
+    ap.add_argument("--sweep-n", type=int, default=3, help="sweeps of the sweep comparison")
     ap.add_argument("--seed", type=int, default=7)
     return ap.parse_args()
 
@@ -163,25 +167,30 @@ def main():
         report = args.precision
     prec_code = E.PRECISIONS[report]
 
-    # ---- site eigenproblem: plain FP64 Jacobi-Davidson vs refinement over the reported precision
-    jd = None
-    if not args.no_jd and report != "f64":
-        tol, max_iter, max_outer = 1e-9, 16, 8
-        out = torch.zeros_like(psi)
-        res = {}
+    # ---- sweeps: the same two-site DMRG run (ab initio Hamiltonian from the benchmark integral
+    # generator) with the FP64 sigma and with the reported precision; energies in Eh and wall times
+    sweep = None
+    if not args.no_sweep and report != "f64":
+        from mqdmrg.dmrg import TwoSiteDMRG
+        sL, sm, nsw = args.sweep_L, args.sweep_m, args.sweep_n
+        kw = dict(m=sm, cutoff=1e-10, init_m=16, seed=3, jd_tol=1e-8, jd_maxiter=8, integral_seed=0, ecore=0.0,
+                  N=sL)
+        runs = {}
         for name, pc in (("f64", E.PREC_F64), (report, prec_code)):
-            eng.sync()
+            d = TwoSiteDMRG(eng, "qc", sL, precision=pc, **kw)
+            torch.cuda.synchronize()
             t0 = time.perf_counter()
-            e, cnt, rn = plan.jd_refined(psi, out, tol=tol, max_iter=max_iter, precision=pc, max_outer=max_outer)
-            res[name] = {"energy": e, "seconds": time.perf_counter() - t0, "sigma_calls": cnt,
-                         "fp64_residual": rn}
-            log("JD %s: E=%.12f in %.2f s, %s, residual %.2e" % (name, e, res[name]["seconds"], cnt, rn))
-        e64, ep = res["f64"]["energy"], res[report]["energy"]
-        jd = {"tol": tol, "max_iter": max_iter, "max_outer": max_outer, "runs": res,
-              "abs_delta_e": abs(ep - e64), "rel_delta_e": abs(ep - e64) / max(1.0, abs(e64)),
-              "speedup_vs_f64": res["f64"]["seconds"] / res[report]["seconds"],
-              "note": "boundaries are random (not from a converged MPS), so the spectrum's scale is arbitrary; "
-                      "the relative difference is the meaningful figure"}
+            e = d.run(nsw)
+            torch.cuda.synchronize()
+            runs[name] = {"energy_Eh": e, "energies_per_sweep": d.energies, "seconds": time.perf_counter() - t0,
+                          "sweep_seconds": d.timings["sweep"], "jd_seconds": d.timings["jd"],
+                          "sigma_calls": d.sigma_calls}
+            log("sweeps %s: E=%.10f  %s s  calls %s" % (name, e, [round(x, 2) for x in d.timings["sweep"]],
+                                                       d.sigma_calls))
+        sweep = {"model": "ab initio (synthetic integrals), TwoU1, L=%d, N=%d, m=%d, %d sweeps, JD tol 1e-8, "
+                          "max_iter 8; reduced precision: one FP64 refinement step per site" % (sL, sL, sm, nsw),
+                 "runs": runs, "abs_delta_e_Eh": abs(runs[report]["energy_Eh"] - runs["f64"]["energy_Eh"]),
+                 "speedup_vs_f64": runs["f64"]["seconds"] / runs[report]["seconds"]}
 
     # ---- roofline of the dominant kernel in the reported precision
     pr = per[report]
@@ -189,9 +198,18 @@ def main():
     dom = max(["gemm_T", "gemm_S", "gemm_final"], key=lambda k: st[k])
     fl = pr["stage_gemm_flops"][{"gemm_T": "T", "gemm_S": "S", "gemm_final": "final"}[dom]]
     achieved = fl / (st[dom] * 1e-3) / 1e12
-    roof = {"kernel": "gemm_kernel (%s, %s)" % (dom, report), "bound": "mfma", "achieved": achieved,
-            "peak": PEAK_TFLOPS[report], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[report],
-            "traffic": None, "stage_ms": st}
+    if dom == "gemm_S":
+        # the S stage is one dense library GEMM per charge-transfer group, in the reported precision
+        kernel = {"f64": "rocBLAS dgemm", "f32": "rocBLAS sgemm",
+                  "bf16x3": "rocBLAS gemm_ex bf16 (3 products per group)"}[report] + " (S stage)"
+        peak_key = report
+    else:
+        # T and final stages: the hand-written MFMA GEMM, FP32 in the reduced precisions
+        kernel = "mq::gemm_kernel (%s stage)" % ("T" if dom == "gemm_T" else "final")
+        peak_key = "f64" if report == "f64" else "f32"
+    roof = {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[peak_key],
+            "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[peak_key], "traffic": None, "stage_ms": st,
+            "flops_per_launch_set": fl}
 
     # ---- CPU baseline on a bounded sample of the same sigma
     cpu = None
@@ -238,7 +256,7 @@ def main():
         },
         "precisions": per,
         "speedup_vs_f64": per["f64"]["ms_per_sigma"] / pr["ms_per_sigma"],
-        "site_eigensolve": jd,
+        "sweeps": sweep,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

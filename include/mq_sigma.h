@@ -261,12 +261,13 @@ int mq_sigma_apply(mq_plan* plan, const double* psi, double* sigma);
  * FP64 only (solver/tasks.cpp). */
 enum mq_precision { MQ_PREC_F64 = 0, MQ_PREC_F32 = 1, MQ_PREC_BF16X3 = 2 };
 int mq_sigma_apply_ex(mq_plan* plan, const double* psi, double* sigma, int precision);
-/* Graph replay of applications (default on).  The first application with a given (psi, sigma,
- * precision) runs eagerly, the second is captured into a HIP graph on the context's stream, and
- * later ones with the same operands replay it (one launch instead of one per stage and tile
- * configuration).  Up to 64
- * graphs are kept per plan.  enable = 0 issues every launch directly and drops the graphs.  No
- * reference counterpart (the reference issues cuBLAS calls per block, solver/tasks.cpp). */
+/* Graph replay of applications (default on).  Each precision has one HIP graph of the whole
+ * application over plan-owned operand slots: psi is copied into the input slot, the graph is
+ * replayed, and the output slot is copied to sigma, so every application (any operands, e.g. each
+ * Jacobi-Davidson iteration) replays it.  The first application of a precision runs eagerly, the
+ * second is captured on the context's stream.  enable = 0 issues every launch directly on the
+ * caller's operands and drops the graphs.  No reference counterpart (the reference issues cuBLAS
+ * calls per block, solver/tasks.cpp). */
 int mq_plan_set_graphs(mq_plan* plan, int enable);
 /* Measurement: runs `reps` sigma applications stage by stage with HIP events on the context's
  * stream and returns the average milliseconds of each stage: [0] S zero-fill, [1] T-stage GEMM,

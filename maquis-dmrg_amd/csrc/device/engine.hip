@@ -15,11 +15,9 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
-#include <set>
 #include <memory>
 #include <new>
 #include <string>
-#include <tuple>
 #include <vector>
 
 #include "../host/host_types.hpp"
@@ -506,16 +504,17 @@ struct mq_plan {
     // Jacobi-Davidson vectors (allocated on first use)
     DevBuf jd_buf;
     int jd_cap = 0;
-    // replayable HIP graphs of whole applications, one per (psi, sigma, precision): a Davidson
-    // iteration replays the graph of its operands instead of re-issuing every launch
+    // replayable HIP graphs of whole applications, one per precision, over the plan's own operand
+    // slots: every Davidson iteration replays it instead of re-issuing every launch
     bool use_graphs = true;
-    std::map<std::tuple<const double*, double*, int>, hipGraphExec_t> graphs;
-    std::set<std::tuple<const double*, double*, int>> seen;   // applied once eagerly
-    static constexpr size_t MAX_GRAPHS = 64;
+    hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};   // per precision
+    bool seen[3] = {false, false, false};                     // precision applied once eagerly
+    DevBuf io;                                                // operand slots of the graphs: in, out
 
     void drop_graphs() {
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
+        for (auto& g : graphs)
+            if (g) (void)hipGraphExecDestroy(g), g = nullptr;
+        for (auto& b : seen) b = false;
     }
     ~mq_plan() { drop_graphs(); }
 
@@ -1616,45 +1615,50 @@ int mq_sigma_apply_ex(mq_plan* p, const double* psi, double* sigma, int precisio
         MQ_REQUIRE(psi != sigma, "psi and sigma must not alias");
         MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
                    "unknown precision");
-        Bases b = p->bases(psi, sigma, precision);   // allocates the FP32 workspace before any capture
         hipStream_t st = p->ctx->stream;
-        auto issue = [&] {
+        if (!p->use_graphs) {
+            Bases b = p->bases(psi, sigma, precision);
             HIPCHK(launch_fill(st, sigma, p->vec_size, 0.0));
             p->st.run(st, b, precision);
+            return;
+        }
+        // Graph replay: one graph per precision over plan-owned operand slots, so that every
+        // application (any psi / sigma, e.g. each Jacobi-Davidson iteration) replays it; the
+        // operands are copied in and out around the replay.  The first application of a precision
+        // runs eagerly (libraries load code and workspaces on first use, which a capture must not
+        // contain), the second is captured.
+        if (!p->io.p) p->io.alloc(2 * std::max<uint64_t>(p->vec_size, 1) * sizeof(double));
+        double* in = p->io.as<double>();
+        double* out = in + std::max<uint64_t>(p->vec_size, 1);
+        Bases b = p->bases(in, out, precision);   // allocates the FP32 / bf16 workspaces before any capture
+        auto issue = [&] {
+            HIPCHK(launch_fill(st, out, p->vec_size, 0.0));
+            p->st.run(st, b, precision);
         };
-        if (!p->use_graphs) {
+        HIPCHK(hipMemcpyAsync(in, psi, p->vec_size * sizeof(double), hipMemcpyDeviceToDevice, st));
+        hipGraphExec_t& ge = p->graphs[precision];
+        if (!ge && !p->seen[precision]) {
+            p->seen[precision] = true;
             issue();
-            return;
-        }
-        auto key = std::make_tuple(psi, sigma, precision);
-        auto it = p->graphs.find(key);
-        if (it == p->graphs.end() && !p->seen.count(key)) {
-            // the first application with these operands runs eagerly: libraries load their code and
-            // workspaces on first use, which a capture must not contain
-            if (p->seen.size() >= mq_plan::MAX_GRAPHS) p->seen.clear();
-            p->seen.insert(key);
-            issue();
-            return;
-        }
-        if (it == p->graphs.end()) {
-            if (p->graphs.size() >= mq_plan::MAX_GRAPHS) p->drop_graphs();
-            hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            try {
-                issue();
-            } catch (...) {
-                (void)hipStreamEndCapture(st, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
+        } else {
+            if (!ge) {
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                try {
+                    issue();
+                } catch (...) {
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(st, &g));
+                hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPCHK(e);
             }
-            HIPCHK(hipStreamEndCapture(st, &g));
-            hipGraphExec_t ge = nullptr;
-            hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            HIPCHK(e);
-            it = p->graphs.emplace(key, ge).first;
+            HIPCHK(hipGraphLaunch(ge, st));
         }
-        HIPCHK(hipGraphLaunch(it->second, st));
+        HIPCHK(hipMemcpyAsync(sigma, out, p->vec_size * sizeof(double), hipMemcpyDeviceToDevice, st));
     });
 }
 

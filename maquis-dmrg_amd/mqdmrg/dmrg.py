@@ -73,9 +73,17 @@ def truncate(svals, Mmax, cutoff):
 
 class TwoSiteDMRG:
     def __init__(self, eng, model, L, m, cutoff=1e-10, init_m=16, seed=1, jd_tol=1e-8, jd_maxiter=8,
-                 mpo=None, basis=None, target=None, verbose=False, hook=None, **model_kw):
+                 mpo=None, basis=None, target=None, verbose=False, hook=None, precision=E.PREC_F64, max_outer=1,
+                 **model_kw):
+        """precision != PREC_F64: sigma in that precision inside Jacobi-Davidson, with the FP64
+        residual refinement of SigmaPlan.jd_refined (at most max_outer FP64 steps per site).  With
+        max_outer = 1 a site costs jd_maxiter reduced-precision products and one FP64 product (the
+        Rayleigh quotient and residual), and the sweep follows the FP64 sweep's trajectory; more
+        outer steps restart the iteration and suit solves to a tight tolerance."""
         self.eng, self.L, self.m, self.cutoff = eng, L, m, cutoff
         self.jd_tol, self.jd_maxiter, self.verbose = jd_tol, jd_maxiter, verbose
+        self.precision, self.max_outer = precision, max_outer
+        self.sigma_calls = {"reduced": 0, "fp64": 0}
         self.hook = hook
         if mpo is None:
             basis, mpo, target = models.chain(model, L, **model_kw)
@@ -277,7 +285,15 @@ class TwoSiteDMRG:
         if self.hook is not None:
             self.hook(self, i, theta, wdesc, plan)
         out = torch.zeros_like(theta.data)
-        e, it = plan.jd(theta.data, out, tol=self.jd_tol, max_iter=self.jd_maxiter)
+        if self.precision == E.PREC_F64:
+            e, it = plan.jd(theta.data, out, tol=self.jd_tol, max_iter=self.jd_maxiter)
+            self.sigma_calls["fp64"] += it
+        else:
+            e, cnt, _ = plan.jd_refined(theta.data, out, tol=self.jd_tol, max_iter=self.jd_maxiter,
+                                        precision=self.precision, max_outer=self.max_outer)
+            it = cnt["reduced"]
+            self.sigma_calls["reduced"] += cnt["reduced"]
+            self.sigma_calls["fp64"] += cnt["fp64"]
         plan.close()
         self.timings["jd"] += time.perf_counter() - t0
         theta.data = out

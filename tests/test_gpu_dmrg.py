@@ -92,3 +92,17 @@ def test_ab_initio_matches_fci(eng):
                     integral_seed=11, ecore=0.5, N=6)
     e = d.run(3)
     assert abs(e - e_fci) < 1e-8, (e, e_fci)
+
+
+@pytest.mark.parametrize("precision", [E.PREC_F32, E.PREC_BF16X3])
+def test_mixed_precision_dmrg_energy(eng, precision):
+    """Two-site DMRG on the ab initio Hamiltonian (synthetic integrals, L=8, N=8) with sigma in FP32
+    or BF16x3 and FP64 residual refinement ends within 1e-6 Eh of the same run in FP64."""
+    L = 8
+    kw = dict(m=64, cutoff=1e-12, init_m=8, seed=3, jd_tol=1e-10, jd_maxiter=12, integral_seed=5, ecore=0.0, N=8)
+    e64 = TwoSiteDMRG(eng, "qc", L, **kw).run(3)
+    d = TwoSiteDMRG(eng, "qc", L, precision=precision, max_outer=4, **kw)
+    e = d.run(3)
+    print("E64 %.12f  E(%d) %.12f  dE %.2e  sigma calls %s" % (e64, precision, e, e - e64, d.sigma_calls), flush=True)
+    assert abs(e - e64) <= 1e-6, (e, e64)
+    assert d.sigma_calls["reduced"] > d.sigma_calls["fp64"]
