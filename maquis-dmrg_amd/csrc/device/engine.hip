@@ -225,12 +225,9 @@ struct StagePlan {
     std::vector<GatherRegion> small;        // regions of <= GATHER_SMALL elements, one wavefront each
     std::vector<uint64_t> src_off;
     std::vector<double> alpha;
-    // grouped path: the middle stage is a GEMM (S = T * W) instead of the gather
-    std::vector<GemmTask> s_tasks;
-    std::vector<GemmSeg> s_segs;
     std::vector<uint32_t> maps;             // column / row block remaps of the GEMM stages
     std::vector<GemmUnit> units;            // column units of B (grouped final stage)
-    GemmStage gT, gS, gF;   // device form of the three GEMM stages
+    GemmStage gT, gF;   // device form of the T and final GEMM stages
     // grouped path: split-K partial sums of the final stage (FP64, base BASE_BRA) and their
     // reduction into sigma
     GatherSet red;
@@ -238,15 +235,19 @@ struct StagePlan {
     // grouped path: the S stage S_D = T_D W_D as plain dense GEMMs (rocBLAS), one per group, on the
     // workspace (element offsets).  BF16x3 runs them as three bf16 products of the hi/lo splits of
     // T_D and W_D (bf_hi / bf_lo, same element offsets) accumulated in FP32.
+    // BF16x3 uses one bf16 GEMM per group with K' = 3K: A' = [A_hi A_lo A_hi] (split per
+    // application from T_D) and B' = [B_lo; B_hi; B_hi] (split once from W_D), at bfa_off / bfb_off of
+    // the bf16 buffer.
     struct LibGemm {
         uint64_t a_off, b_off, c_off;
         uint32_t M, N, K, lda, ldb, ldc;
+        uint64_t bfa_off, bfb_off;
     };
     std::vector<LibGemm> lib;
     rocblas_handle blas = nullptr;
-    uint64_t t_end = 0;   // T_D occupies workspace elements [0, t_end)
-    uint16_t* bf_hi = nullptr;
-    uint16_t* bf_lo = nullptr;
+    uint64_t t_end = 0;     // T_D occupies workspace elements [0, t_end)
+    uint64_t bf_elems = 0;  // size of the bf16 buffer
+    uint16_t* bf = nullptr;
     uint32_t g_chunks = 0;
     uint64_t ws_elems = 0;
     uint64_t s_begin = 0, s_elems = 0;   // workspace range zero-filled at the start of each apply
@@ -255,7 +256,6 @@ struct StagePlan {
 
     void finish_tables() {
         gT.build(t_tasks);
-        gS.build(s_tasks);
         gF.build(f_tasks);
         red.finish();
         std::vector<GatherRegion> big;
@@ -352,7 +352,6 @@ struct StagePlan {
             }
         };
         gemm(t_tasks, t_segs);
-        gemm(s_tasks, s_segs);
         gemm(f_tasks, f_segs);
         for (const auto& g : lib) {
             MQ_REQUIRE(g.lda >= g.M && g.ldb >= g.K && g.ldc >= g.M, "library GEMM leading dimensions");
@@ -377,7 +376,6 @@ struct StagePlan {
     }
     void upload_all(hipStream_t st) {
         gT.upload_all(t_segs, st);
-        gS.upload_all(s_segs, st);
         gF.upload_all(f_segs, st);
         upload(d_regions, regions, st);
         upload(d_small, small, st);
@@ -408,7 +406,6 @@ struct StagePlan {
                 gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, mp, un);
                 break;
             case 2:
-                gS.run(st, mode, "fff", b, mp, un);
                 run_lib(st, b, mode);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
@@ -436,19 +433,15 @@ struct StagePlan {
                 BLASCHK(rocblas_sgemm(blas, nn, nn, g.M, g.N, g.K, &one, ws + g.a_off, g.lda, ws + g.b_off, g.ldb, &zero,
                                       ws + g.c_off, g.ldc));
         } else {
-            MQ_REQUIRE(bf_hi && bf_lo, "bf16 split buffers not allocated");
+            MQ_REQUIRE(bf, "bf16 split buffer not allocated");
             float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
-            HIPCHK(launch_split_bf16(st, ws, bf_hi, bf_lo, t_end));
             const float one = 1.f, zero = 0.f;
-            const rocblas_datatype bf = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
+            const rocblas_datatype b16 = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
             for (const auto& g : lib) {
-                // the two cross terms first, then the leading product: small terms accumulate first
-                const uint16_t* A[3] = {bf_hi + g.a_off, bf_lo + g.a_off, bf_hi + g.a_off};
-                const uint16_t* B[3] = {bf_lo + g.b_off, bf_hi + g.b_off, bf_hi + g.b_off};
-                for (int q = 0; q < 3; ++q)
-                    BLASCHK(rocblas_gemm_ex(blas, nn, nn, g.M, g.N, g.K, &one, A[q], bf, g.lda, B[q], bf, g.ldb,
-                                            q ? &one : &zero, ws + g.c_off, f32, g.ldc, ws + g.c_off, f32, g.ldc, f32,
-                                            rocblas_gemm_algo_standard, 0, 0));
+                HIPCHK(launch_split3_bf16(st, ws + g.a_off, bf + g.bfa_off, uint64_t(g.M) * g.K));
+                BLASCHK(rocblas_gemm_ex(blas, nn, nn, g.M, g.N, 3 * g.K, &one, bf + g.bfa_off, b16, g.M,
+                                        bf + g.bfb_off, b16, 3 * g.K, &zero, ws + g.c_off, f32, g.ldc, ws + g.c_off,
+                                        f32, g.ldc, f32, rocblas_gemm_algo_standard, 0, 0));
             }
         }
     }
@@ -500,7 +493,7 @@ struct mq_plan {
     uint64_t ws_init_off = 0;
     DevBuf ws32;
     DevBuf part;   // split-K partial sums of the final stage (FP64)
-    DevBuf bf;     // bf16 hi / lo splits of the workspace (BF16x3 S stage)
+    DevBuf bf;     // bf16 operands of the BF16x3 S stage
     // Jacobi-Davidson vectors (allocated on first use)
     DevBuf jd_buf;
     int jd_cap = 0;
@@ -536,18 +529,24 @@ struct mq_plan {
             }
             b.p[BASE_WS] = reinterpret_cast<double*>(ws32.as<float>());
             if (mode == MODE_BF16X3 && !st.lib.empty() && !bf.p) {
-                // hi/lo bf16 copies of T_D (written per application) and W_D (constant, split here)
-                const uint64_t n = std::max<uint64_t>(st.ws_elems, 64);
-                bf.alloc(2 * n * sizeof(uint16_t));
-                st.bf_hi = bf.as<uint16_t>();
-                st.bf_lo = st.bf_hi + n;
-                if (!ws_init.empty()) {
-                    std::vector<uint16_t> hi(ws_init.size()), lo(ws_init.size());
-                    for (size_t i = 0; i < ws_init.size(); ++i) bf16_split_host(float(ws_init[i]), hi[i], lo[i]);
-                    HIPCHK(hipMemcpyAsync(st.bf_hi + ws_init_off, hi.data(), hi.size() * 2, hipMemcpyHostToDevice,
-                                          ctx->stream));
-                    HIPCHK(hipMemcpyAsync(st.bf_lo + ws_init_off, lo.data(), lo.size() * 2, hipMemcpyHostToDevice,
-                                          ctx->stream));
+                // bf16 operands of the one-GEMM BF16x3 form: A' regions (written per application) and
+                // B' = [W_lo; W_hi; W_hi] per group (constant, split here)
+                bf.alloc(std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t));
+                st.bf = bf.as<uint16_t>();
+                for (const auto& g : st.lib) {
+                    const double* W = ws_init.data() + (g.b_off - ws_init_off);
+                    std::vector<uint16_t> Bp(uint64_t(3) * g.K * g.N);
+                    for (uint64_t n = 0; n < g.N; ++n)
+                        for (uint64_t k = 0; k < g.K; ++k) {
+                            uint16_t hi, lo;
+                            bf16_split_host(float(W[n * g.ldb + k]), hi, lo);
+                            uint16_t* col = Bp.data() + n * 3 * g.K;
+                            col[k] = lo;
+                            col[g.K + k] = hi;
+                            col[2 * g.K + k] = hi;
+                        }
+                    HIPCHK(hipMemcpyAsync(st.bf + g.bfb_off, Bp.data(), Bp.size() * sizeof(uint16_t),
+                                          hipMemcpyHostToDevice, ctx->stream));
                     HIPCHK(hipStreamSynchronize(ctx->stream));
                 }
             }
@@ -1362,8 +1361,18 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
     for (auto& g : G.groups) {
         if (!g.M || g.K.empty() || g.N.empty()) continue;
         MQ_REQUIRE(g.M < (1ull << 31), "group too large for one library GEMM");
-        P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(g.N.size()), uint32_t(g.K.size()),
-                         uint32_t(g.M), uint32_t(g.K.size()), uint32_t(g.M)});
+        const uint64_t Kg = g.K.size(), Ng = g.N.size();
+        MQ_REQUIRE(3 * Kg < (1ull << 31), "group too large for one library GEMM");
+        P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(Ng), uint32_t(Kg), uint32_t(g.M),
+                         uint32_t(Kg), uint32_t(g.M), 0, 0});
+    }
+    for (auto& L : P.lib) {   // bf16 buffer: A' regions, then B' regions
+        L.bfa_off = P.bf_elems;
+        P.bf_elems += align32(3ull * L.M * L.K);
+    }
+    for (auto& L : P.lib) {
+        L.bfb_off = P.bf_elems;
+        P.bf_elems += align32(3ull * L.K * L.N);
     }
     P.t_end = P.s_begin + P.s_elems;
     // ---- final stage: one task per output block lb_out (ls_out x all its columns), K segments over
@@ -1716,7 +1725,6 @@ int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, int 
             ft = gemm_flops(P.t_tasks, P.t_segs);
             ff = gemm_flops(P.f_tasks, P.f_segs);
             P.for_regions([&](const GatherRegion& r) { fg += 2ull * r.R * r.C * (r.t1 - r.t0); });
-            fg += gemm_flops(P.s_tasks, P.s_segs);
             for (const auto& g : P.lib) fg += 2ull * g.M * g.N * g.K;
             flops_out[0] = ft;
             flops_out[1] = fg;
@@ -1748,8 +1756,7 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
     return guarded([&] {
         MQ_REQUIRE(p && bytes, "bad argument");
         const StagePlan& P = p->st;
-        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gS.device_bytes() +
-                 P.gF.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_regions.bytes +
+        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gF.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_regions.bytes +
                  P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
     });
 }

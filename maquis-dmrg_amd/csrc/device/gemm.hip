@@ -2,9 +2,9 @@
 // tile shape.
 //
 //   MODE_F64    v_mfma_f64_16x16x4f64, FP64 operands and accumulation (the plain path)
-//   MODE_F32    v_mfma_f32_16x16x4f32, FP32 operands and accumulation
-//   MODE_BF16X3 v_mfma_f32_16x16x32_bf16 on a two-term split of each FP32 operand,
-//               x = hi + lo (hi, lo bf16): a*b ~ a_hi*b_hi + a_hi*b_lo + a_lo*b_hi, FP32 accumulation
+//   MODE_F32    v_mfma_f32_16x16x4f32, FP32 operands and accumulation (the reduced-precision
+//               paths: their T and final stages are bandwidth-bound, and the BF16x3 products
+//               of the S stage are plain dense GEMMs done by the library)
 //
 // Operands are read in their storage type (TA, TB: double or float) and converted when staged;
 // C is written in TC.  One launch covers every tile of every task of one tile configuration (the
@@ -20,9 +20,8 @@
 //                     in the MFMA lane layout, partial sums reduced through LDS    M, N <= 16
 //
 // MFMA lane layouts on gfx950 (lane l = 0..63, lk = l/16), measured with one-hot operands:
-//   f64 16x16x4      : A row l%16, k lk;            B col l%16, k lk;   D[r] row 4r + lk, col l%16
-//   f32 16x16x4      : A row l%16, k lk;            B col l%16, k lk;   D[r] row 4lk + r, col l%16
-//   bf16 16x16x32    : A row l%16, k 8lk + j (j<8); B col l%16, same k; D[r] row 4lk + r, col l%16
+//   f64 16x16x4 : A row l%16, k lk;  B col l%16, k lk;  D[r] row 4r + lk, col l%16
+//   f32 16x16x4 : A row l%16, k lk;  B col l%16, k lk;  D[r] row 4lk + r, col l%16
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -31,36 +30,16 @@ namespace mq {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-typedef short s8 __attribute__((ext_vector_type(8)));
 
 template <int MODE> struct ModeTraits;
 template <> struct ModeTraits<MODE_F64> { typedef double T; typedef d4 Acc; static constexpr int BK = 16, KS = 4; };
 template <> struct ModeTraits<MODE_F32> { typedef float T; typedef f4 Acc; static constexpr int BK = 16, KS = 4; };
-template <> struct ModeTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; static constexpr int BK = 32, KS = 32; };
 
 // output row of accumulator register r for lanes of k-group lk = l/16
 template <int MODE> __device__ inline int d_row(int r, int lk) { return MODE == MODE_F64 ? 4 * r + lk : 4 * lk + r; }
 
-// bf16 of x, round to nearest even (finite inputs)
-__device__ inline short bf16_bits(float x) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return short(u >> 16);
-}
-__device__ inline float bf16_value(short h) { return __uint_as_float(uint32_t(uint16_t(h)) << 16); }
-__device__ inline void bf16_split(float x, short& hi, short& lo) {
-    hi = bf16_bits(x);
-    lo = bf16_bits(x - bf16_value(hi));
-}
-
 __device__ inline d4 mfma_step(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ inline f4 mfma_step(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-__device__ inline f4 mfma3(const s8& ah, const s8& al, const s8& bh, const s8& bl, f4 c) {
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
-}
-
 // ------------------------------------------------------------------------------ operand access
 // Element loads are branch-free: an out-of-range element reads offset 0 of its operand (the first
 // element of a stored block: a valid address holding finite data) and is scaled by zero.  The value
@@ -125,10 +104,7 @@ template <int MODE, int BM, int BN>
 struct Smem {
     typedef typename ModeTraits<MODE>::T T;
     static constexpr int BK = ModeTraits<MODE>::BK;
-    static constexpr int RK = BK + 8;   // bf16 rows: k contiguous, 16-byte aligned, padded
-    static constexpr size_t plain = sizeof(T) * size_t(BK) * (BM + 1 + BN + 1);
-    static constexpr size_t split = sizeof(short) * size_t(2) * RK * (BM + BN);
-    static constexpr size_t buf = ((MODE == MODE_BF16X3 ? split : plain) + 15) / 16 * 16;
+    static constexpr size_t buf = (sizeof(T) * size_t(BK) * (BM + 1 + BN + 1) + 15) / 16 * 16;
 };
 
 template <typename TA, typename TB, typename TC, int MODE, int PM, int PN, int WM, int WN>
@@ -144,7 +120,6 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     constexpr int NA = BK * BM / GEMM_THREADS, NB = BK * BN / GEMM_THREADS;
     static_assert(NA * GEMM_THREADS == BK * BM && NB * GEMM_THREADS == BK * BN, "whole staging passes");
     typedef Smem<MODE, BM, BN> SM;
-    constexpr int RK = SM::RK;
     __shared__ __align__(16) char smem[2 * SM::buf];
 
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -288,12 +263,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             const int m = r_trans_a ? e / BK : e % BM;
             const int k = r_trans_a ? e % BK : e / BM;
             const double v = double(va[i]) * (((oka >> i) & 1u) ? r_alpha : 0.0);
-            if constexpr (MODE == MODE_BF16X3) {
-                short(*Ah)[RK] = reinterpret_cast<short(*)[RK]>(base);
-                bf16_split(float(v), Ah[m][k], Ah[BM + m][k]);
-            } else {
-                reinterpret_cast<T(*)[BM + 1]>(base)[k][m] = T(v);
-            }
+            reinterpret_cast<T(*)[BM + 1]>(base)[k][m] = T(v);
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
@@ -301,50 +271,25 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             const int n = r_kfast_b ? e / BK : e % BN;
             const int k = r_kfast_b ? e % BK : e / BN;
             const double v = double(vb[i]) * (((okb >> i) & 1u) ? 1.0 : 0.0);
-            if constexpr (MODE == MODE_BF16X3) {
-                short(*Bh)[RK] = reinterpret_cast<short(*)[RK]>(base) + 2 * BM;
-                bf16_split(float(v), Bh[n][k], Bh[BN + n][k]);
-            } else {
-                reinterpret_cast<T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1))[k][n] = T(v);
-            }
+            reinterpret_cast<T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1))[k][n] = T(v);
         }
     };
     auto compute = [&](int buf) {
         const char* base = smem + buf * SM::buf;
-        if constexpr (MODE == MODE_BF16X3) {
-            const short(*Ah)[RK] = reinterpret_cast<const short(*)[RK]>(base);
-            const short(*Bh)[RK] = Ah + 2 * BM;
-            s8 ah[WM], al[WM], bh[WN], bl[WN];
+        const T(*As)[BM + 1] = reinterpret_cast<const T(*)[BM + 1]>(base);
+        const T(*Bs)[BN + 1] = reinterpret_cast<const T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1));
 #pragma unroll
-            for (int i = 0; i < WM; ++i) {
-                ah[i] = *reinterpret_cast<const s8*>(&Ah[wm + 16 * i + l16][8 * lk]);
-                al[i] = *reinterpret_cast<const s8*>(&Ah[BM + wm + 16 * i + l16][8 * lk]);
-            }
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + lk;
+            T a[WM], b[WN];
 #pragma unroll
-            for (int j = 0; j < WN; ++j) {
-                bh[j] = *reinterpret_cast<const s8*>(&Bh[wn + 16 * j + l16][8 * lk]);
-                bl[j] = *reinterpret_cast<const s8*>(&Bh[BN + wn + 16 * j + l16][8 * lk]);
-            }
+            for (int i = 0; i < WM; ++i) a[i] = As[kr][wm + 16 * i + l16];
+#pragma unroll
+            for (int j = 0; j < WN; ++j) b[j] = Bs[kr][wn + 16 * j + l16];
 #pragma unroll
             for (int i = 0; i < WM; ++i)
 #pragma unroll
-                for (int j = 0; j < WN; ++j) acc[i][j] = mfma3(ah[i], al[i], bh[j], bl[j], acc[i][j]);
-        } else {
-            const T(*As)[BM + 1] = reinterpret_cast<const T(*)[BM + 1]>(base);
-            const T(*Bs)[BN + 1] = reinterpret_cast<const T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1));
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 4) {
-                const int kr = kk + lk;
-                T a[WM], b[WN];
-#pragma unroll
-                for (int i = 0; i < WM; ++i) a[i] = As[kr][wm + 16 * i + l16];
-#pragma unroll
-                for (int j = 0; j < WN; ++j) b[j] = Bs[kr][wn + 16 * j + l16];
-#pragma unroll
-                for (int i = 0; i < WM; ++i)
-#pragma unroll
-                    for (int j = 0; j < WN; ++j) acc[i][j] = mfma_step(a[i], b[j], acc[i][j]);
-            }
+                for (int j = 0; j < WN; ++j) acc[i][j] = mfma_step(a[i], b[j], acc[i][j]);
         }
     };
 
@@ -415,40 +360,18 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask*
         const uint32_t K = sg.K;
         const uint32_t nsteps = (K + KS - 1) / KS;
         const uint32_t first = (uint32_t(wave) + NW - step % NW) % NW;
-        if constexpr (MODE != MODE_BF16X3) {
-            // U steps per pass: all their loads are issued before the MFMAs consume them
-            constexpr int U = 8;
-            for (uint32_t s = first; s < nsteps; s += NW * U) {
-                T a[U], b[U];
+        // U steps per pass: all their loads are issued before the MFMAs consume them
+        constexpr int U = 8;
+        for (uint32_t s = first; s < nsteps; s += NW * U) {
+            T a[U], b[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t k = (s + NW * u) * KS + lk;
-                    a[u] = T(a_load(A, sg, (row < M) & (k < K), row + tk.m_off, k));
-                    b[u] = T(b_load(B, sg, maps, (col < N) & (k < K), k, col));
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) acc = mfma_step(a[u], b[u], acc);
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = (s + NW * u) * KS + lk;
+                a[u] = T(a_load(A, sg, (row < M) & (k < K), row + tk.m_off, k));
+                b[u] = T(b_load(B, sg, maps, (col < N) & (k < K), k, col));
             }
-        }
-        for (uint32_t s = first; MODE == MODE_BF16X3 && s < nsteps; s += NW) {
-            const uint32_t k0 = s * KS;
-            if constexpr (MODE == MODE_BF16X3) {
-                s8 ah, al, bh, bl;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t k = k0 + 8 * lk + j;
-                    const float a = float(a_load(A, sg, (row < M) & (k < K), row + tk.m_off, k));
-                    const float b = float(b_load(B, sg, maps, (col < N) & (k < K), k, col));
-                    short h, l;
-                    bf16_split(a, h, l);
-                    ah[j] = h;
-                    al[j] = l;
-                    bf16_split(b, h, l);
-                    bh[j] = h;
-                    bl[j] = l;
-                }
-                acc = mfma3(ah, al, bh, bl, acc);
-            }
+            for (int u = 0; u < U; ++u) acc = mfma_step(a[u], b[u], acc);
         }
         step += nsteps;
     }
@@ -523,7 +446,6 @@ hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, con
     if (da && !db && dc)                                                                                         \
         return launch_typed<double, float, double, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);
     if (mode == MODE_F32) { MQ_DISPATCH(MODE_F32) }
-    if (mode == MODE_BF16X3) { MQ_DISPATCH(MODE_BF16X3) }
 #undef MQ_DISPATCH
     return hipErrorInvalidValue;
 }

@@ -128,25 +128,27 @@ hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v) {
 }
 
 // x = hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even (finite inputs): the
-// operands of the three bf16 products of the BF16x3 GEMMs.
+// operands of the bf16 products of the BF16x3 GEMMs.
 __device__ inline uint16_t bf16_rne(float x) {
     uint32_t u = __float_as_uint(x);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return uint16_t(u >> 16);
 }
-__global__ void split_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
-                                  uint64_t n) {
+// dst = [hi | lo | hi] of x (three consecutive n-element runs): the K-concatenated left operand of
+// the one-GEMM form of the BF16x3 product [A_hi A_lo A_hi] [B_lo; B_hi; B_hi].
+__global__ void split3_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict__ dst, uint64_t n) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
         const float v = x[i];
         const uint16_t h = bf16_rne(v);
-        hi[i] = h;
-        lo[i] = bf16_rne(v - __uint_as_float(uint32_t(h) << 16));
+        dst[i] = h;
+        dst[n + i] = bf16_rne(v - __uint_as_float(uint32_t(h) << 16));
+        dst[2 * n + i] = h;
     }
 }
 
-hipError_t launch_split_bf16(hipStream_t st, const float* x, uint16_t* hi, uint16_t* lo, uint64_t n) {
+hipError_t launch_split3_bf16(hipStream_t st, const float* x, uint16_t* dst, uint64_t n) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, x, hi, lo, n);
+    hipLaunchKernelGGL(split3_bf16_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, x, dst, n);
     return hipGetLastError();
 }
 

@@ -75,6 +75,9 @@ struct GatherRegion {
 };
 
 constexpr uint32_t MAP_NONE = 0xffffffffu;
+// Arithmetic of an application (mq_precision).  The hand-written GEMM runs MODE_F64 and MODE_F32;
+// MODE_BF16X3 applications run their T and final stages in MODE_F32 and the S stage as bf16
+// library GEMMs.
 enum GemmMode { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
 
 // Tile configurations of the GEMM (gemm.hip); a task runs with the one that fits its (M, N).
@@ -108,8 +111,9 @@ hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n
                          const uint64_t* src_off, const double* alpha, uint32_t n_chunks, double* ws,
                          double* dst_base);
 hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v);
-// hi[i] = bf16(x[i]), lo[i] = bf16(x[i] - hi[i]) (round to nearest even)
-hipError_t launch_split_bf16(hipStream_t st, const float* x, uint16_t* hi, uint16_t* lo, uint64_t n);
+// dst[0, n) = hi, dst[n, 2n) = lo, dst[2n, 3n) = hi of x, with hi = bf16(x), lo = bf16(x - hi)
+// (round to nearest even)
+hipError_t launch_split3_bf16(hipStream_t st, const float* x, uint16_t* dst, uint64_t n);
 hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial,
                       double* result);
 hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y);
