@@ -142,14 +142,16 @@ struct mq_schedule {
 struct GemmStage {
     std::vector<GemmTask> cfg[GEMM_N_CFG];
     uint32_t tiles[GEMM_N_CFG] = {};
-    DevBuf d_tasks[GEMM_N_CFG];
+    std::vector<uint32_t> tile_task[GEMM_N_CFG];   // launches with many multi-tile tasks: task of each tile
+    DevBuf d_tasks[GEMM_N_CFG], d_tile_task[GEMM_N_CFG];
     DevBuf d_segs;
 
     void build(const std::vector<GemmTask>& tasks) {
         for (auto& c : cfg) c.clear();
         for (const auto& t : tasks) {
             int c = gemm_pick_cfg(t.M, t.N);
-            if (c == GEMM_CFG_16x16 && t.colunit != MAP_NONE) c = GEMM_CFG_16x128;   // units: LDS-tiled kernels
+            // column units and remapped A are implemented in the LDS-tiled kernels only
+            if (c == GEMM_CFG_16x16 && t.colunit != MAP_NONE) c = GEMM_CFG_16x128;
             cfg[c].push_back(t);
         }
         for (int c = 0; c < GEMM_N_CFG; ++c) {
@@ -161,21 +163,36 @@ struct GemmStage {
                 MQ_REQUIRE(tile < (1ull << 31), "too many GEMM tiles in one launch");
             }
             tiles[c] = uint32_t(tile);
+            tile_task[c].clear();
+            if (cfg[c].size() > 64 && tile != cfg[c].size() && tile < (1ull << 28)) {
+                tile_task[c].reserve(tile);
+                for (uint32_t t = 0; t < cfg[c].size(); ++t) {
+                    const GemmTask& k = cfg[c][t];
+                    const uint64_t n = uint64_t((k.M + GEMM_CFG_BM[c] - 1) / GEMM_CFG_BM[c]) * k.tiles_n;
+                    tile_task[c].insert(tile_task[c].end(), n, t);
+                }
+            }
         }
     }
     void upload_all(const std::vector<GemmSeg>& segs, hipStream_t st) {
         upload(d_segs, segs, st);
-        for (int c = 0; c < GEMM_N_CFG; ++c) upload(d_tasks[c], cfg[c], st);
+        for (int c = 0; c < GEMM_N_CFG; ++c) {
+            upload(d_tasks[c], cfg[c], st);
+            upload(d_tile_task[c], tile_task[c], st);
+        }
     }
-    void run(hipStream_t st, int mode, const char* types, Bases b, const uint32_t* maps, const GemmUnit* units) const {
+    void run(hipStream_t st, int mode, const char* types, Bases b, const uint32_t* maps, const GemmUnit* units,
+             const double* scales) const {
         for (int c = 0; c < GEMM_N_CFG; ++c)
             if (tiles[c])
                 HIPCHK(launch_gemm(st, c, mode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
-                                   d_segs.as<GemmSeg>(), tiles[c], b, maps, units));
+                                   d_segs.as<GemmSeg>(), tiles[c], b, maps, units,
+                                   tile_task[c].empty() ? nullptr : d_tile_task[c].as<uint32_t>(), scales));
     }
     uint64_t device_bytes() const {
         uint64_t n = d_segs.bytes;
         for (const auto& d : d_tasks) n += d.bytes;
+        for (const auto& d : d_tile_task) n += d.bytes;
         return n;
     }
 };
@@ -227,6 +244,7 @@ struct StagePlan {
     std::vector<double> alpha;
     std::vector<uint32_t> maps;             // column / row block remaps of the GEMM stages
     std::vector<GemmUnit> units;            // column units of B (grouped final stage)
+    std::vector<double> scales;             // block scales of remapped transposed A
     GemmStage gT, gF;   // device form of the T and final GEMM stages
     // grouped path: split-K partial sums of the final stage (FP64, base BASE_BRA) and their
     // reduction into sigma
@@ -251,7 +269,7 @@ struct StagePlan {
     uint32_t g_chunks = 0;
     uint64_t ws_elems = 0;
     uint64_t s_begin = 0, s_elems = 0;   // workspace range zero-filled at the start of each apply
-    DevBuf d_maps, d_units, d_regions, d_small, d_src_off, d_alpha;
+    DevBuf d_maps, d_units, d_scales, d_regions, d_small, d_src_off, d_alpha;
     uint64_t sizes[N_BASES] = {0, 0, 0, 0, 0, 0};   // element counts of each base for validation
 
     void finish_tables() {
@@ -318,8 +336,19 @@ struct StagePlan {
                 }
                 for (uint32_t s = t.seg0; s < t.seg1; ++s) {
                     const GemmSeg& g = segs[s];
-                    if (!g.trans_a) check_in(g.a_base, g.a_off + t.m_off, extent(t.M, g.K, g.lda), "A");
-                    else check_in(g.a_base, g.a_off + uint64_t(t.m_off) * g.lda, extent(g.K, t.M, g.lda), "A^T");
+                    if (g.amap != MAP_NONE) {
+                        MQ_REQUIRE(g.trans_a && g.bbh > 0 && g.K % g.bbh == 0, "remapped A shape");
+                        const uint32_t nblk = g.K / g.bbh;
+                        MQ_REQUIRE(uint64_t(g.amap) + nblk <= maps.size() && uint64_t(g.ascale) + nblk <= scales.size(),
+                                   "A block map range");
+                        for (uint32_t j = 0; j < nblk; ++j)
+                            check_in(g.a_base, g.a_off + maps[g.amap + j] + uint64_t(t.m_off) * g.lda,
+                                     extent(g.bbh, t.M, g.lda), "A^T block");
+                    } else if (!g.trans_a) {
+                        check_in(g.a_base, g.a_off + t.m_off, extent(t.M, g.K, g.lda), "A");
+                    } else {
+                        check_in(g.a_base, g.a_off + uint64_t(t.m_off) * g.lda, extent(g.K, t.M, g.lda), "A^T");
+                    }
                     if (g.units != MAP_NONE) {
                         MQ_REQUIRE(t.colunit != MAP_NONE && g.bmap == MAP_NONE, "unit segment without column units");
                         MQ_REQUIRE(g.bbh > 0 && g.K % g.bbh == 0, "unit segment shape");
@@ -383,6 +412,7 @@ struct StagePlan {
         upload(d_alpha, alpha, st);
         upload(d_maps, maps, st);
         upload(d_units, units, st);
+        upload(d_scales, scales, st);
         red.upload_all(st);
     }
     // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM.
@@ -390,6 +420,7 @@ struct StagePlan {
     void stage(int k, hipStream_t st, Bases b, int mode = MODE_F64) const {
         const uint32_t* mp = d_maps.as<uint32_t>();
         const GemmUnit* un = d_units.as<GemmUnit>();
+        const double* sc = d_scales.as<double>();
         const bool mixed = mode != MODE_F64;
         switch (k) {
             case 0:
@@ -403,14 +434,14 @@ struct StagePlan {
             case 1:
                 // the T and final stages are bandwidth-bound block products: BF16x3 runs them in FP32
                 // and uses the split products in the S stage only
-                gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, mp, un);
+                gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, mp, un, sc);
                 break;
             case 2:
                 run_lib(st, b, mode);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
             default:
-                gF.run(st, mixed ? MODE_F32 : MODE_F64, "dfd", b, mp, un);
+                gF.run(st, mixed ? MODE_F32 : MODE_F64, "dfd", b, mp, un, sc);
                 red.run(st, b.p[BASE_BRA], b.p[BASE_OUT]);
         }
     }
@@ -624,6 +655,8 @@ GemmSeg seg(uint8_t ab, uint64_t ao, uint32_t lda, bool ta, uint8_t bb, uint64_t
     g.bbh = 1;
     g.bldk = 0;
     g.units = MAP_NONE;
+    g.amap = MAP_NONE;
+    g.ascale = MAP_NONE;
     return g;
 }
 
@@ -1438,17 +1471,28 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 sg.units = add_units(lb, jb);
                 segs.push_back(sg);
             } else {
+                // Hermitian partner: the blocks are the partner cohort's, transposed, each with its
+                // conjugation scale; one segment over all of them (A block map + block scales)
                 uint32_t cle = LI.cohort_index(lc_in, lc_out);
                 MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
+                std::vector<uint32_t> b1s;
+                const uint32_t amap = uint32_t(P.maps.size()), ascale = uint32_t(P.scales.size());
                 for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1) {
                     if (!LI.has_block(cl, b1)) continue;
-                    uint64_t o = uint64_t(LI.offset(cl, b1));
-                    GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cle] + o, ls_in, true, BASE_WS, 0, ls_in, false,
-                                     ls_in, LI.conjugate_scale(cl, b1));
-                    sg.bbh = ls_in;
-                    sg.units = add_units(lb, {b1});
-                    segs.push_back(sg);
+                    const int64_t o = LI.offset(cl, b1);
+                    MQ_REQUIRE(o >= 0 && o < (int64_t(1) << 32), "partner block offset");
+                    b1s.push_back(b1);
+                    P.maps.push_back(uint32_t(o));
+                    P.scales.push_back(LI.conjugate_scale(cl, b1));
                 }
+                if (b1s.empty()) continue;
+                GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cle], ls_in, true, BASE_WS, 0, ls_in, false,
+                                 uint32_t(b1s.size()) * ls_in);
+                sg.bbh = ls_in;
+                sg.amap = amap;
+                sg.ascale = ascale;
+                sg.units = add_units(lb, b1s);
+                segs.push_back(sg);
             }
         }
         if (segs.empty()) continue;

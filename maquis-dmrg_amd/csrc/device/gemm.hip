@@ -86,8 +86,11 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-__device__ inline uint32_t find_task(const GemmTask* tasks, uint32_t n_tasks, uint32_t tile) {
-    if (n_tasks == gridDim.x) return tile;   // one tile per task
+// Task of a tile: a direct table where the launch has one (many small tasks), else the identity
+// (one tile per task) or a binary search over the tile prefixes.
+__device__ inline uint32_t find_task(const GemmTask* tasks, uint32_t n_tasks, const uint32_t* tile_task, uint32_t tile) {
+    if (tile_task) return tile_task[tile];
+    if (n_tasks == gridDim.x) return tile;
     uint32_t lo = 0, hi = n_tasks;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
@@ -111,7 +114,9 @@ template <typename TA, typename TB, typename TC, int MODE, int PM, int PN, int W
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                             const GemmSeg* __restrict__ segs, Bases bases,
                                                             const uint32_t* __restrict__ maps,
-                                                            const GemmUnit* __restrict__ units) {
+                                                            const GemmUnit* __restrict__ units,
+                                                            const uint32_t* __restrict__ tile_task,
+                                                            const double* __restrict__ scales) {
     static_assert(PM * PN * 64 == GEMM_THREADS, "four waves");
     typedef typename ModeTraits<MODE>::T T;
     typedef typename ModeTraits<MODE>::Acc Acc;
@@ -123,7 +128,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     __shared__ __align__(16) char smem[2 * SM::buf];
 
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const GemmTask tk = tasks[find_task(tasks, n_tasks, tile)];
+    const GemmTask tk = tasks[find_task(tasks, n_tasks, tile_task, tile)];
     const uint32_t local = tile - tk.tile_begin;
     const uint32_t m0 = (local / tk.tiles_n) * BM;
     const uint32_t n0 = (local % tk.tiles_n) * BN;
@@ -161,7 +166,24 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     uint32_t oka = 0, okb = 0;
     double r_alpha = 1.0;
     bool r_trans_a = false, r_kfast_b = true;
+    double a_scale = 1.0;   // block scale of a remapped transposed A (all of a thread's elements share k)
     auto load_a = [&](const TA* __restrict__ A) {
+        if (sg.amap != MAP_NONE) {
+            const uint32_t gk = k0 + uint32_t(tid % BK);   // trans_a: k = e % BK = tid % BK
+            const uint32_t blk = min(gk, sg.K - 1) / sg.bbh;
+            const uint64_t boff = uint64_t(maps[sg.amap + blk]) + (gk - blk * sg.bbh);
+            a_scale = scales[sg.ascale + blk];
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+                const int e = tid + GEMM_THREADS * i;
+                const uint32_t gm = m0 + e / BK;
+                const uint32_t ok = uint32_t(gm < M) & uint32_t(gk < sg.K);
+                oka |= ok << i;
+                va[i] = A[(boff + uint64_t(gm + tk.m_off) * sg.lda) & (0ull - uint64_t(ok))];
+            }
+            return;
+        }
+        a_scale = 1.0;
         const uint64_t sm = sg.trans_a ? sg.lda : 1u, sk = sg.trans_a ? 1u : sg.lda;
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
@@ -262,7 +284,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             const int e = tid + GEMM_THREADS * i;
             const int m = r_trans_a ? e / BK : e % BM;
             const int k = r_trans_a ? e % BK : e / BM;
-            const double v = double(va[i]) * (((oka >> i) & 1u) ? r_alpha : 0.0);
+            const double v = double(va[i]) * (((oka >> i) & 1u) ? r_alpha * a_scale : 0.0);
             reinterpret_cast<T(*)[BM + 1]>(base)[k][m] = T(v);
         }
 #pragma unroll
@@ -336,14 +358,16 @@ template <typename TA, typename TB, typename TC, int MODE>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                                  const GemmSeg* __restrict__ segs, Bases bases,
                                                                  const uint32_t* __restrict__ maps,
-                                                                 const GemmUnit* __restrict__ units) {
+                                                                 const GemmUnit* __restrict__ units,
+                                                                 const uint32_t* __restrict__ tile_task,
+                                                                 const double* __restrict__ scales) {
     typedef typename ModeTraits<MODE>::T T;
     typedef typename ModeTraits<MODE>::Acc Acc;
     constexpr int KS = ModeTraits<MODE>::KS;
     constexpr int NW = GEMM_THREADS / 64;
     __shared__ T red[NW - 1][64][4];
 
-    const GemmTask tk = tasks[find_task(tasks, n_tasks, xcd_tile(blockIdx.x, gridDim.x))];
+    const GemmTask tk = tasks[find_task(tasks, n_tasks, tile_task, xcd_tile(blockIdx.x, gridDim.x))];
     const uint32_t M = tk.M, N = tk.N;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -402,28 +426,29 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask*
 // ------------------------------------------------------------------------------ dispatch
 template <typename TA, typename TB, typename TC, int MODE>
 static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                               uint32_t n_tiles, Bases bases, const uint32_t* maps, const GemmUnit* units) {
+                               uint32_t n_tiles, Bases bases, const uint32_t* maps, const GemmUnit* units,
+                               const uint32_t* tile_task, const double* scales) {
     const dim3 grid(n_tiles), block(GEMM_THREADS);
     switch (cfg) {
         case GEMM_CFG_128x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 4, 4>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units);
+                               bases, maps, units, tile_task, scales);
             break;
         case GEMM_CFG_64x64:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units);
+                               bases, maps, units, tile_task, scales);
             break;
         case GEMM_CFG_32x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units);
+                               bases, maps, units, tile_task, scales);
             break;
         case GEMM_CFG_16x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 1, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units);
+                               bases, maps, units, tile_task, scales);
             break;
         case GEMM_CFG_16x16:
             hipLaunchKernelGGL((gemm_tiny_kernel<TA, TB, TC, MODE>), grid, block, 0, st, tasks, n_tasks, segs, bases,
-                               maps, units);
+                               maps, units, tile_task, scales);
             break;
         default:
             return hipErrorInvalidValue;
@@ -433,18 +458,18 @@ static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, u
 
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
                        const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps,
-                       const GemmUnit* units) {
+                       const GemmUnit* units, const uint32_t* tile_task, const double* scales) {
     if (n_tiles == 0) return hipSuccess;
     if (mode == MODE_F64)
-        return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);
+        return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);
     const bool da = types[0] == 'd', db = types[1] == 'd', dc = types[2] == 'd';
 #define MQ_DISPATCH(MD)                                                                                          \
     if (da && db && !dc)                                                                                         \
-        return launch_typed<double, double, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);     \
+        return launch_typed<double, double, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);     \
     if (!da && !db && !dc)                                                                                       \
-        return launch_typed<float, float, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);       \
+        return launch_typed<float, float, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);       \
     if (da && !db && dc)                                                                                         \
-        return launch_typed<double, float, double, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units);
+        return launch_typed<double, float, double, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);
     if (mode == MODE_F32) { MQ_DISPATCH(MODE_F32) }
 #undef MQ_DISPATCH
     return hipErrorInvalidValue;

@@ -50,6 +50,9 @@ struct GemmTask {
     uint8_t c_base, beta, pad0, pad1;
 };
 
+//  row-block remap of a transposed A (trans_a only): column k of op(A) lies in block j = k / bbh,
+//      op(A)(m, k) = alpha * scales[ascale + j] * A[a_off + maps[amap + j] + (k % bbh) + m * lda]
+//      (the transposed Hermitian partners of a cohort's blocks, each with its conjugation scale)
 struct GemmSeg {
     uint64_t a_off, b_off;
     uint64_t bldk;
@@ -57,6 +60,7 @@ struct GemmSeg {
     uint32_t K, lda, ldb;
     uint32_t bmap, bbh;
     uint32_t units;
+    uint32_t amap, ascale;
     uint8_t a_base, b_base, trans_a, trans_b;
 };
 
@@ -96,9 +100,10 @@ constexpr int GEMM_THREADS = 256;
 // One launch of all tasks of one tile configuration.  mode MODE_F64: all operands double.
 // Otherwise `types` gives the storage of A, B and C as 'd' (double) or 'f' (float): "ddf",
 // "fff" or "dfd".
+// tile_task (may be null): task index of every tile of the launch.
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
                        const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps,
-                       const GemmUnit* units);
+                       const GemmUnit* units, const uint32_t* tile_task, const double* scales);
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
 constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions
