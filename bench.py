@@ -201,7 +201,7 @@ def main():
     if dom == "gemm_S":
         # the S stage is one dense library GEMM per charge-transfer group, in the reported precision
         kernel = {"f64": "rocBLAS dgemm", "f32": "rocBLAS sgemm",
-                  "bf16x3": "rocBLAS gemm_ex bf16 (3 products per group)"}[report] + " (S stage)"
+                  "bf16x3": "split3_bf16_kernel + rocBLAS gemm_ex bf16, K = 3K per group"}[report] + " (S stage)"
         peak_key = report
     else:
         # T and final stages: the hand-written MFMA GEMM, FP32 in the reduced precisions
@@ -210,6 +210,14 @@ def main():
     roof = {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[peak_key],
             "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[peak_key], "traffic": None, "stage_ms": st,
             "flops_per_launch_set": fl}
+    # HBM bytes of the same launch set, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+    # this workload (profiles/), when they cover the reported stage and precision
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (args.workload.replace("-", "_"), report))
+    if dom == "gemm_S" and os.path.exists(tf):
+        with open(tf) as fh:
+            t = json.load(fh)
+        roof["traffic"] = t["traffic_bytes"]
+        roof["traffic_source"] = os.path.relpath(tf, ROOT)
 
     # ---- CPU baseline on a bounded sample of the same sigma
     cpu = None
