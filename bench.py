@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--sweep-L", type=int, default=16, help="chain length of the sweep comparison")
     ap.add_argument("--sweep-m", type=int, default=256, help="bond dimension of the sweep comparison")
     ap.add_argument("--sweep-n", type=int, default=3, help="sweeps of the sweep comparison")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also time N independent ab initio instances (L uniform in [12, 24], m=256) "
+                         "applied one by one and as one batch")
     ap.add_argument("--seed", type=int, default=7)
     return ap.parse_args()
 
@@ -235,6 +238,49 @@ def main():
         got = sigma.cpu().numpy()
         cpu["parity_rel_err"] = float(np.linalg.norm(got - ref_cpu) / np.linalg.norm(ref_cpu))
 
+    # ---- batch of independent instances: sequential applications vs one batched launch
+    batch = None
+    if args.batch > 0:
+        rng = np.random.default_rng(args.seed)
+        plans, psis, sigs, keep = [], [], [], []
+        flops = 0
+        for i in range(args.batch):
+            bL = int(rng.integers(12, 25))
+            inst = SyntheticSite("qc", bL, bL // 2 - 1, 256, seed=100 + i)
+            lb = E.DeviceBoundary.upload(eng, inst.left_index, inst.random_cohorts(inst.left_index))
+            rb = E.DeviceBoundary.upload(eng, inst.right_index, inst.random_cohorts(inst.right_index))
+            pl = E.SigmaPlan(eng, inst.site_desc, inst.mpo_desc, lb, rb, algorithm=E.SIGMA_GROUPED)
+            x = torch.from_numpy(inst.random_psi()).to(eng.torch_device)
+            plans.append(pl)
+            psis.append(x)
+            sigs.append(torch.zeros_like(x))
+            keep += [lb, rb, inst]
+            flops += pl.flops
+        res = {}
+        for name, pc in (("f64", E.PREC_F64), (report, prec_code)):
+            for mode in ("sequential", "batched"):
+                def once():
+                    if mode == "batched":
+                        E.apply_batched(eng, plans, psis, sigs, precision=pc)
+                    else:
+                        for pl, x, y in zip(plans, psis, sigs):
+                            pl.apply(x, y, precision=pc)
+                for _ in range(3):   # eager, capture, warm replay
+                    once()
+                eng.sync()
+                ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ev0.record(eng.stream)
+                for _ in range(args.steps):
+                    once()
+                ev1.record(eng.stream)
+                eng.sync()
+                ms = ev0.elapsed_time(ev1) / args.steps
+                res["%s_%s" % (name, mode)] = {"ms_per_batch_step": ms, "tflops": flops / (ms * 1e-3) / 1e12}
+                log("batch %s %s: %.3f ms per step of %d instances, %.2f TFLOP/s" % (name, mode, ms, args.batch,
+                                                                                   res["%s_%s" % (name, mode)]["tflops"]))
+        batch = {"instances": args.batch, "model": "ab initio (synthetic integrals), L uniform in [12, 24], m=256, "
+                 "middle site", "flops_per_step": flops, "runs": res}
+
     out = {
         "metric": "sweep wall-time & σ TFLOP/s at L=30 m=2048, mixed vs FP64 at |ΔE|≤1e-6 Eh",
         "value": pr["tflops"],
@@ -265,6 +311,7 @@ def main():
         "precisions": per,
         "speedup_vs_f64": per["f64"]["ms_per_sigma"] / pr["ms_per_sigma"],
         "sweeps": sweep,
+        "batch": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

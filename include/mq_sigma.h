@@ -269,6 +269,14 @@ int mq_sigma_apply_ex(mq_plan* plan, const double* psi, double* sigma, int preci
  * caller's operands and drops the graphs.  No reference counterpart (the reference issues cuBLAS
  * calls per block, solver/tasks.cpp). */
 int mq_plan_set_graphs(mq_plan* plan, int enable);
+/* n independent applications (plans of one context, e.g. the sites of independent chains) issued
+ * as one batch: each instance on one of four side streams forked from and joined back to the
+ * context's stream, so small instances run concurrently.  The batch of a given (plans, precision)
+ * runs eagerly on first use and is captured into one HIP graph on the second, replayed after.
+ * psi[i] / sigma[i] are device vectors of plans[i].  Destroying a plan drops the batch graphs that
+ * contain it.  No reference counterpart (the reference applies one site Hamiltonian at a time). */
+int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const double* const* psi,
+                           double* const* sigma, int precision);
 /* Measurement: runs `reps` sigma applications stage by stage with HIP events on the context's
  * stream and returns the average milliseconds of each stage: [0] S zero-fill, [1] T-stage GEMM,
  * [2] S gather, [3] final GEMM.  flops_out (may be NULL) receives the algorithmic flops of the

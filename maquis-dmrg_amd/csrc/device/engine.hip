@@ -95,12 +95,32 @@ inline uint64_t align32(uint64_t x) { return round_up(x, 32); }
 }  // namespace
 
 // ============================================================================ context
+struct mq_plan;
+
+// A captured batch of applications (mq_sigma_apply_batched): one graph over the operand slots of
+// its plans, with one branch per plan.
+struct BatchGraph {
+    hipGraphExec_t exec = nullptr;
+    bool seen = false;
+};
+
 struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;   // plain dense GEMMs of the grouped S stage, on `stream`
     DevBuf red_partial, red_result;
     DevBuf step_ws;   // grows; reused by boundary steps
+    // batched applications: side streams (default priority) the instances of a batch fork onto,
+    // and the captured graphs keyed by (plans, precision)
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;
+    hipEvent_t fork_ev = nullptr;
+    std::map<std::pair<std::vector<const mq_plan*>, int>, BatchGraph> batches;
+    void drop_batches() {
+        for (auto& kv : batches)
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        batches.clear();
+    }
 };
 
 #define BLASCHK(x)                                                                                            \
@@ -753,6 +773,10 @@ int mq_ctx_destroy(mq_ctx* ctx) {
     return guarded([&] {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->stream);
+        ctx->drop_batches();
+        for (auto sd : ctx->side) (void)hipStreamDestroy(sd);
+        for (auto e : ctx->side_ev) (void)hipEventDestroy(e);
+        if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -1715,6 +1739,90 @@ int mq_sigma_apply_ex(mq_plan* p, const double* psi, double* sigma, int precisio
     });
 }
 
+int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const double* const* psi,
+                           double* const* sigma, int precision) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && n > 0 && plans && psi && sigma, "bad argument");
+        MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
+                   "unknown precision");
+        std::vector<const mq_plan*> key(plans, plans + n);
+        for (int i = 0; i < n; ++i) {
+            MQ_REQUIRE(plans[i] && plans[i]->ctx == ctx && psi[i] && sigma[i] && psi[i] != sigma[i],
+                       "batch operand " + std::to_string(i));
+            for (int j = 0; j < i; ++j) MQ_REQUIRE(plans[j] != plans[i], "a plan appears twice in the batch");
+        }
+        hipStream_t st = ctx->stream;
+        constexpr int NSIDE = 4;
+        if (ctx->side.empty()) {
+            ctx->side.resize(NSIDE);
+            ctx->side_ev.resize(NSIDE);
+            for (int k = 0; k < NSIDE; ++k) {
+                HIPCHK(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+                HIPCHK(hipEventCreateWithFlags(&ctx->side_ev[k], hipEventDisableTiming));
+            }
+            HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+        }
+        // operand slots and workspaces of every plan exist before any capture
+        std::vector<Bases> bs(n);
+        std::vector<double*> ins(n), outs(n);
+        for (int i = 0; i < n; ++i) {
+            mq_plan* p = plans[i];
+            if (!p->io.p) p->io.alloc(2 * std::max<uint64_t>(p->vec_size, 1) * sizeof(double));
+            ins[i] = p->io.as<double>();
+            outs[i] = ins[i] + std::max<uint64_t>(p->vec_size, 1);
+            bs[i] = p->bases(ins[i], outs[i], precision);
+        }
+        // the batch: fork the instances over the side streams, join back on the context's stream
+        auto issue = [&] {
+            HIPCHK(hipEventRecord(ctx->fork_ev, st));
+            for (int k = 0; k < NSIDE; ++k) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
+            for (int i = 0; i < n; ++i) {
+                hipStream_t s = ctx->side[i % NSIDE];
+                BLASCHK(rocblas_set_stream(ctx->blas, s));
+                HIPCHK(launch_fill(s, outs[i], plans[i]->vec_size, 0.0));
+                plans[i]->st.run(s, bs[i], precision);
+            }
+            BLASCHK(rocblas_set_stream(ctx->blas, st));
+            for (int k = 0; k < NSIDE; ++k) {
+                HIPCHK(hipEventRecord(ctx->side_ev[k], ctx->side[k]));
+                HIPCHK(hipStreamWaitEvent(st, ctx->side_ev[k], 0));
+            }
+        };
+        for (int i = 0; i < n; ++i)
+            HIPCHK(hipMemcpyAsync(ins[i], psi[i], plans[i]->vec_size * sizeof(double), hipMemcpyDeviceToDevice, st));
+        BatchGraph& bg = ctx->batches[std::make_pair(key, precision)];
+        if (!bg.exec && !bg.seen) {
+            bg.seen = true;   // first use runs eagerly (library code loads on first use)
+            try {
+                issue();
+            } catch (...) {
+                (void)rocblas_set_stream(ctx->blas, st);
+                throw;
+            }
+        } else {
+            if (!bg.exec) {
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                try {
+                    issue();
+                } catch (...) {
+                    (void)rocblas_set_stream(ctx->blas, st);
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(st, &g));
+                hipError_t e = hipGraphInstantiate(&bg.exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIPCHK(e);
+            }
+            HIPCHK(hipGraphLaunch(bg.exec, st));
+        }
+        for (int i = 0; i < n; ++i)
+            HIPCHK(hipMemcpyAsync(sigma[i], outs[i], plans[i]->vec_size * sizeof(double), hipMemcpyDeviceToDevice, st));
+    });
+}
+
 int mq_plan_set_graphs(mq_plan* p, int enable) {
     return guarded([&] {
         MQ_REQUIRE(p, "bad argument");
@@ -1807,7 +1915,20 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
 
 int mq_plan_destroy(mq_plan* p) {
     return guarded([&] {
-        if (p && p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+        if (p && p->ctx) {
+            (void)hipStreamSynchronize(p->ctx->stream);
+            // batch graphs that contain the plan refer to its buffers
+            auto& B = p->ctx->batches;
+            for (auto it = B.begin(); it != B.end();) {
+                const auto& ps = it->first.first;
+                if (std::find(ps.begin(), ps.end(), p) != ps.end()) {
+                    if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
+                    it = B.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
         delete p;
     });
 }

@@ -345,6 +345,19 @@ class SigmaPlan:
             pass
 
 
+def apply_batched(engine, plans, psis, sigmas, precision=PREC_F64):
+    """sigma_i = H_i psi_i for independent plans in one batch (concurrent side streams, captured
+    into one HIP graph from the second call with the same plans on)."""
+    n = len(plans)
+    assert n == len(psis) == len(sigmas) and n > 0
+    for p, x, y in zip(plans, psis, sigmas):
+        assert x.numel() >= p.n and y.numel() >= p.n
+    hp = (C.c_void_p * n)(*[p.h for p in plans])
+    xp = (C.c_void_p * n)(*[x.data_ptr() for x in psis])
+    yp = (C.c_void_p * n)(*[y.data_ptr() for y in sigmas])
+    check(lib().mq_sigma_apply_batched(engine.ctx, n, hp, xp, yp, int(precision)))
+
+
 def left_step(engine, site, mpo_desc, bra_lp, ket_rp, left, symmetric=True):
     h = C.c_void_p()
     check(lib().mq_boundary_left_step(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_lp.data_ptr(),

@@ -306,3 +306,32 @@ def test_graph_replay_equals_eager(eng, precision):
         eng.sync()
         assert torch.equal(out2, eager2), rep
         plan.set_graphs(True)
+
+
+@pytest.mark.parametrize("precision", [E.PREC_F64, E.PREC_BF16X3])
+def test_batched_apply_equals_individual(eng, precision):
+    """Independent instances (different models and chains) applied as one batch, eagerly and then as
+    a captured graph replayed several times, equal their individual applications bit for bit."""
+    probs = [SiteProblem("hubbard", L=8, site=3, Mmax=10, seed=5), SiteProblem("qc", L=8, site=3, Mmax=8, seed=7),
+             SiteProblem("heisenberg", L=10, site=0, Mmax=8, seed=2), SiteProblem("qc", L=6, site=2, Mmax=8, seed=6),
+             SiteProblem("bosehubbard", L=8, site=3, Mmax=6, seed=8)]
+    plans, psis, refs = [], [], []
+    for p in probs:
+        Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+        plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+        psi = dev(p.psi)
+        ref = torch.zeros_like(psi)
+        plan.set_graphs(False)
+        plan.apply(psi, ref, precision=precision)
+        plans.append(plan)
+        psis.append(psi)
+        refs.append(ref)
+    eng.sync()
+    outs = [torch.zeros_like(x) for x in psis]
+    for rep in range(4):   # eager, capture, replay, replay
+        for o in outs:
+            o.fill_(np.nan)
+        E.apply_batched(eng, plans, psis, outs, precision=precision)
+        eng.sync()
+        for i, (o, r) in enumerate(zip(outs, refs)):
+            assert torch.equal(o, r), (rep, i)
