@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=0,
                     help="also time N independent ab initio instances (L uniform in [12, 24], m=256) "
                          "applied one by one and as one batch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile configurations (no timing of candidates before the run)")
     ap.add_argument("--seed", type=int, default=7)
     return ap.parse_args()
 
@@ -137,6 +139,16 @@ def main():
     per = {}
     ref = None
     for name, prec in (("f64", E.PREC_F64), ("f32", E.PREC_F32), ("bf16x3", E.PREC_BF16X3)):
+        tune = None
+        if not args.no_autotune:
+            # tile configurations of the T and final stages: stage times before and after tuning
+            untuned, _, _ = plan.profile(psi, sigma, reps=3, precision=prec)
+            t0 = time.perf_counter()
+            n_timed = plan.autotune(prec)
+            tune = {"timings": n_timed, "seconds": round(time.perf_counter() - t0, 3),
+                    "untuned_stage_ms": dict(zip(["fill", "gemm_T", "gemm_S", "gemm_final"],
+                                                 [round(x, 4) for x in untuned]))}
+            log("%s autotune: %d stage timings in %.2f s" % (name, n_timed, tune["seconds"]))
         for _ in range(args.warmup):
             plan.apply(psi, sigma, precision=prec)
         eng.sync()
@@ -159,7 +171,7 @@ def main():
         per[name] = {"ms_per_sigma": ms, "tflops": plan.flops / (ms * 1e-3) / 1e12, "sigma_rel_err_vs_f64": err,
                      "wall_seconds_timed": round(wall, 4),
                      "stage_ms": dict(zip(["fill", "gemm_T", "gemm_S", "gemm_final"], [round(x, 4) for x in stage_ms])),
-                     "stage_gemm_flops": dict(zip(["T", "S", "final"], stage_flops))}
+                     "stage_gemm_flops": dict(zip(["T", "S", "final"], stage_flops)), "autotune": tune}
         log("%s: %.3f ms/sigma, %.2f TFLOP/s, rel err %.2e, stages %s" % (name, ms, per[name]["tflops"], err,
                                                                          per[name]["stage_ms"]))
 

@@ -269,6 +269,14 @@ int mq_sigma_apply_ex(mq_plan* plan, const double* psi, double* sigma, int preci
  * caller's operands and drops the graphs.  No reference counterpart (the reference issues cuBLAS
  * calls per block, solver/tasks.cpp). */
 int mq_plan_set_graphs(mq_plan* plan, int enable);
+/* Autotune the GEMM tile configurations of the plan's T and final stages for `precision`.  Tasks
+ * fall into row-size classes (M <= 16, 32, 64, 128, larger); for each class present, candidate tile
+ * shapes are timed on the plan's own stage (one warm-up and three repetitions) and the fastest is
+ * kept.  Decisions are cached in the context, keyed by the stage's flop histogram over the classes
+ * (log2 buckets): a plan whose histogram was tuned before applies the cached choice without timing.
+ * timed (may be NULL) receives the number of stage timings made.  Captured graphs of the plan are
+ * dropped.  No reference counterpart. */
+int mq_plan_autotune(mq_plan* plan, int precision, int* timed);
 /* n independent applications (plans of one context, e.g. the sites of independent chains) issued
  * as one batch: each instance on one of four side streams forked from and joined back to the
  * context's stream, so small instances run concurrently.  The batch of a given (plans, precision)

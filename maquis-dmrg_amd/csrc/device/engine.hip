@@ -11,6 +11,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -116,6 +117,8 @@ struct mq_ctx {
     std::vector<hipEvent_t> side_ev;
     hipEvent_t fork_ev = nullptr;
     std::map<std::pair<std::vector<const mq_plan*>, int>, BatchGraph> batches;
+    // autotuned tile policies, keyed by (stage, GEMM mode, log2 flop histogram over size classes)
+    std::map<std::vector<int>, std::array<int, 5>> tuned;
     void drop_batches() {
         for (auto& kv : batches)
             if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -159,19 +162,48 @@ struct mq_schedule {
 // Lowered stage tables (device) + the host copies used for validation.
 // One GEMM stage on the device: the lowering's tasks split by tile configuration (one launch
 // each), sharing one segment table.
+// Row-size classes of GEMM tasks and the tile configurations the autotuner tries for each.  The
+// class of a task is fixed by its M; -1 in a policy means the default choice of gemm_pick_cfg.
+constexpr int N_SIZE_CLASS = 5;
+inline int size_class(uint32_t M) { return M <= 16 ? 0 : M <= 32 ? 1 : M <= 64 ? 2 : M <= 128 ? 3 : 4; }
+const std::vector<int> CLASS_CANDIDATES[N_SIZE_CLASS] = {
+    {GEMM_CFG_16x128, GEMM_CFG_64x64},
+    {GEMM_CFG_32x128, GEMM_CFG_64x64, GEMM_CFG_16x128},
+    {GEMM_CFG_64x64, GEMM_CFG_32x128, GEMM_CFG_128x128},
+    {GEMM_CFG_64x64, GEMM_CFG_128x128, GEMM_CFG_32x128},
+    {GEMM_CFG_128x128, GEMM_CFG_64x64}};
+struct TilePolicy {
+    int cls[N_SIZE_CLASS] = {-1, -1, -1, -1, -1};
+};
+
 struct GemmStage {
+    std::vector<GemmTask> all;                     // the lowering's tasks (rebuilt per policy)
+    TilePolicy policy;
     std::vector<GemmTask> cfg[GEMM_N_CFG];
     uint32_t tiles[GEMM_N_CFG] = {};
     std::vector<uint32_t> tile_task[GEMM_N_CFG];   // launches with many multi-tile tasks: task of each tile
     DevBuf d_tasks[GEMM_N_CFG], d_tile_task[GEMM_N_CFG];
     DevBuf d_segs;
 
+    // Flops per size class: the block-size histogram the autotuner's decisions are keyed on.
+    std::vector<uint64_t> class_flops(const std::vector<GemmSeg>& segs) const {
+        std::vector<uint64_t> f(N_SIZE_CLASS, 0);
+        for (const auto& t : all)
+            for (uint32_t s = t.seg0; s < t.seg1; ++s) f[size_class(t.M)] += 2ull * t.M * t.N * segs[s].K;
+        return f;
+    }
     void build(const std::vector<GemmTask>& tasks) {
+        all = tasks;
+        rebuild();
+    }
+    void rebuild() {
         for (auto& c : cfg) c.clear();
-        for (const auto& t : tasks) {
-            int c = gemm_pick_cfg(t.M, t.N);
-            // column units and remapped A are implemented in the LDS-tiled kernels only
-            if (c == GEMM_CFG_16x16 && t.colunit != MAP_NONE) c = GEMM_CFG_16x128;
+        for (const auto& t : all) {
+            int c = policy.cls[size_class(t.M)];
+            if (c < 0) c = gemm_pick_cfg(t.M, t.N);
+            // the split-K 16x16 kernel covers one 16x16 tile; column units and remapped A are
+            // implemented in the LDS-tiled kernels only
+            if (c == GEMM_CFG_16x16 && (t.M > 16 || t.N > 16 || t.colunit != MAP_NONE)) c = GEMM_CFG_16x128;
             cfg[c].push_back(t);
         }
         for (int c = 0; c < GEMM_N_CFG; ++c) {
@@ -1820,6 +1852,94 @@ int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const doub
         }
         for (int i = 0; i < n; ++i)
             HIPCHK(hipMemcpyAsync(sigma[i], outs[i], plans[i]->vec_size * sizeof(double), hipMemcpyDeviceToDevice, st));
+    });
+}
+
+int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
+    return guarded([&] {
+        MQ_REQUIRE(p, "bad argument");
+        MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
+                   "unknown precision");
+        mq_ctx* ctx = p->ctx;
+        hipStream_t st = ctx->stream;
+        StagePlan& P = p->st;
+        const int mode = precision == MQ_PREC_F64 ? MODE_F64 : MODE_F32;   // the hand-written stages' mode
+        if (!p->io.p) p->io.alloc(2 * std::max<uint64_t>(p->vec_size, 1) * sizeof(double));
+        double* in = p->io.as<double>();
+        double* out = in + std::max<uint64_t>(p->vec_size, 1);
+        Bases b = p->bases(in, out, precision);
+        HIPCHK(launch_fill(st, in, p->vec_size, 0.0));   // values do not change the timing
+        HIPCHK(launch_fill(st, out, p->vec_size, 0.0));
+        P.run(st, b, precision);   // a full application: the final stage then reads a valid S
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        int n_timed = 0;
+        auto time_stage = [&](int k) {
+            P.stage(k, st, b, precision);   // warm-up
+            HIPCHK(hipEventRecord(e0, st));
+            for (int r = 0; r < 3; ++r) P.stage(k, st, b, precision);
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ++n_timed;
+            return ms;
+        };
+        struct StageRef { int k; GemmStage* g; const std::vector<GemmSeg>* segs; };
+        for (StageRef sr : {StageRef{1, &P.gT, &P.t_segs}, StageRef{3, &P.gF, &P.f_segs}}) {
+            GemmStage& G = *sr.g;
+            std::vector<uint64_t> cf = G.class_flops(*sr.segs);
+            std::vector<int> key = {sr.k, mode, int(p->grouped)};
+            for (auto f : cf) key.push_back(f ? int(std::log2(double(f))) : -1);
+            auto apply = [&](const TilePolicy& pol) {
+                G.policy = pol;
+                G.rebuild();
+                G.upload_all(*sr.segs, st);
+            };
+            auto it = ctx->tuned.find(key);
+            if (it != ctx->tuned.end()) {
+                TilePolicy pol;
+                for (int c = 0; c < N_SIZE_CLASS; ++c) pol.cls[c] = it->second[c];
+                apply(pol);
+                continue;
+            }
+            TilePolicy pol;
+            for (int c = 0; c < N_SIZE_CLASS; ++c) {
+                if (!cf[c]) continue;
+                apply(pol);
+                float best = time_stage(sr.k);
+                int best_cfg = -1;
+                for (int cand : CLASS_CANDIDATES[c]) {
+                    TilePolicy trial = pol;
+                    trial.cls[c] = cand;
+                    apply(trial);
+                    float t = time_stage(sr.k);
+                    if (t < 0.97f * best) best = t, best_cfg = cand;
+                }
+                pol.cls[c] = best_cfg;
+            }
+            apply(pol);
+            std::array<int, 5> rec;
+            for (int c = 0; c < N_SIZE_CLASS; ++c) rec[c] = pol.cls[c];
+            ctx->tuned[key] = rec;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCHK(hipStreamSynchronize(st));
+        // the device task tables were re-uploaded: graphs that captured the old ones are stale
+        p->drop_graphs();
+        auto& B = ctx->batches;
+        for (auto it = B.begin(); it != B.end();) {
+            const auto& ps = it->first.first;
+            if (std::find(ps.begin(), ps.end(), p) != ps.end()) {
+                if (it->second.exec) (void)hipGraphExecDestroy(it->second.exec);
+                it = B.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        if (timed) *timed = n_timed;
     });
 }
 

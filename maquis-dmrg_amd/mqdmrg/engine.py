@@ -302,6 +302,14 @@ class SigmaPlan:
         assert psi.numel() >= self.n and sigma.numel() >= self.n
         check(lib().mq_sigma_apply_ex(self.h, psi.data_ptr(), sigma.data_ptr(), int(precision)))
 
+    def autotune(self, precision=PREC_F64):
+        """Time the tile configurations of the T and final stages and keep the fastest per row-size
+        class (cached per engine by the stage's block-size histogram).  Returns the number of stage
+        timings made (0 when the decision came from the cache)."""
+        n = C.c_int()
+        check(lib().mq_plan_autotune(self.h, int(precision), C.byref(n)))
+        return n.value
+
     def set_graphs(self, enable):
         """Replay captured HIP graphs of whole applications (default on)."""
         check(lib().mq_plan_set_graphs(self.h, int(bool(enable))))

@@ -100,6 +100,57 @@ def test_sigma_synthetic_shapes_match_oracle(eng, model, L, m):
         plan.close()
 
 
+def test_autotuned_plan_matches_oracle(eng):
+    """Tile configurations chosen by the autotuner change the schedule of the T and final stages,
+    not the result: sigma still matches the oracle (FP64 to 1e-13 of the untuned result); a second
+    plan of the same block-size histogram takes the cached decision without timing; graphs captured
+    after tuning replay to the tuned eager result."""
+    from mqdmrg.workloads import SyntheticSite
+    ws = SyntheticSite("qc", 12, 5, 128, seed=3)
+    Lc, Rc = ws.random_cohorts(ws.left_index), ws.random_cohorts(ws.right_index)
+    hl = O.HostBoundary(ws.left_index, [c if c is not None else np.zeros(0) for c in Lc])
+    hr = O.HostBoundary(ws.right_index, [c if c is not None else np.zeros(0) for c in Rc])
+    psi_h = ws.random_psi()
+    ref = O.SigmaOracle(ws.site_desc, ws.mpo_desc, hl, hr).apply(psi_h)
+    Lb = E.DeviceBoundary.upload(eng, ws.left_index, Lc)
+    Rb = E.DeviceBoundary.upload(eng, ws.right_index, Rc)
+    psi = dev(psi_h)
+    for alg in (E.SIGMA_SCHEDULE, E.SIGMA_GROUPED):
+        precs = [E.PREC_F64] if alg == E.SIGMA_SCHEDULE else [E.PREC_F64, E.PREC_F32, E.PREC_BF16X3]
+        for prec in precs:
+            print("case", alg, prec, flush=True)
+            plan = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=alg)
+            plan.set_graphs(False)
+            before = torch.full_like(psi, np.nan)
+            plan.apply(psi, before, precision=prec)
+            n_timed = plan.autotune(prec)
+            # BF16x3 runs its T and final stages as FP32 does: the FP32 decision is reused
+            assert n_timed > 0 or prec == E.PREC_BF16X3
+            after = torch.full_like(psi, np.nan)
+            plan.apply(psi, after, precision=prec)
+            eng.sync()
+            got = after.cpu().numpy()
+            err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+            assert err <= (SIGMA_RTOL if prec == E.PREC_F64 else MIXED_RTOL[prec]), (alg, prec, err)
+            if prec == E.PREC_F64:
+                d = np.linalg.norm(got - before.cpu().numpy()) / np.linalg.norm(ref)
+                assert d <= 1e-13, d
+            plan.set_graphs(True)
+            for rep in range(3):
+                out = torch.full_like(psi, np.nan)
+                plan.apply(psi, out, precision=prec)
+                eng.sync()
+                assert torch.equal(out, after), rep
+            twin = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=alg)
+            assert twin.autotune(prec) == 0   # same histogram: cached decision
+            out = torch.full_like(psi, np.nan)
+            twin.apply(psi, out, precision=prec)
+            eng.sync()
+            assert torch.equal(out, after)
+            twin.close()
+            plan.close()
+
+
 def test_sigma_plan_from_caller_schedule(eng):
     """Drop-in seam: the caller hands its own ScheduleNew (here the oracle's) to the engine."""
     p = SiteProblem("hubbard", L=6, site=2, Mmax=6, seed=7)
