@@ -201,9 +201,10 @@ struct GemmStage {
         for (const auto& t : all) {
             int c = policy.cls[size_class(t.M)];
             if (c < 0) c = gemm_pick_cfg(t.M, t.N);
-            // the split-K 16x16 kernel covers one 16x16 tile; column units and remapped A are
-            // implemented in the LDS-tiled kernels only
-            if (c == GEMM_CFG_16x16 && (t.M > 16 || t.N > 16 || t.colunit != MAP_NONE)) c = GEMM_CFG_16x128;
+            // the split-K 16x16 kernel covers one 16x16 tile; column units, row units and remapped
+            // A are implemented in the LDS-tiled kernels only
+            if (c == GEMM_CFG_16x16 && (t.M > 16 || t.N > 16 || t.colunit != MAP_NONE || t.rowunit != MAP_NONE))
+                c = GEMM_CFG_16x128;
             cfg[c].push_back(t);
         }
         for (int c = 0; c < GEMM_N_CFG; ++c) {
@@ -233,13 +234,12 @@ struct GemmStage {
             upload(d_tile_task[c], tile_task[c], st);
         }
     }
-    void run(hipStream_t st, int mode, const char* types, Bases b, const uint32_t* maps, const GemmUnit* units,
-             const double* scales) const {
+    void run(hipStream_t st, int mode, const char* types, Bases b, GemmTables tb) const {
         for (int c = 0; c < GEMM_N_CFG; ++c)
             if (tiles[c])
                 HIPCHK(launch_gemm(st, c, mode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
-                                   d_segs.as<GemmSeg>(), tiles[c], b, maps, units,
-                                   tile_task[c].empty() ? nullptr : d_tile_task[c].as<uint32_t>(), scales));
+                                   d_segs.as<GemmSeg>(), tiles[c], b, tb,
+                                   tile_task[c].empty() ? nullptr : d_tile_task[c].as<uint32_t>()));
     }
     uint64_t device_bytes() const {
         uint64_t n = d_segs.bytes;
@@ -297,17 +297,16 @@ struct StagePlan {
     std::vector<uint32_t> maps;             // column / row block remaps of the GEMM stages
     std::vector<GemmUnit> units;            // column units of B (grouped final stage)
     std::vector<double> scales;             // block scales of remapped transposed A
+    std::vector<GemmRow> rows;              // row units of stacked tasks (grouped T stage)
     GemmStage gT, gF;   // device form of the T and final GEMM stages
     // grouped path: split-K partial sums of the final stage (FP64, base BASE_BRA) and their
     // reduction into sigma
     GatherSet red;
     uint64_t part_elems = 0;
     // grouped path: the S stage S_D = T_D W_D as plain dense GEMMs (rocBLAS), one per group, on the
-    // workspace (element offsets).  BF16x3 runs them as three bf16 products of the hi/lo splits of
-    // T_D and W_D (bf_hi / bf_lo, same element offsets) accumulated in FP32.
-    // BF16x3 uses one bf16 GEMM per group with K' = 3K: A' = [A_hi A_lo A_hi] (split per
-    // application from T_D) and B' = [B_lo; B_hi; B_hi] (split once from W_D), at bfa_off / bfb_off of
-    // the bf16 buffer.
+    // workspace (element offsets).  BF16x3 uses one bf16 GEMM per group with K' = 3K and FP32
+    // accumulation: A' = [A_hi A_lo A_hi] (split per application from T_D) and B' = [B_lo; B_hi; B_hi]
+    // (split once from W_D), at bfa_off / bfb_off of the bf16 buffer.
     struct LibGemm {
         uint64_t a_off, b_off, c_off;
         uint32_t M, N, K, lda, ldb, ldc;
@@ -321,7 +320,7 @@ struct StagePlan {
     uint32_t g_chunks = 0;
     uint64_t ws_elems = 0;
     uint64_t s_begin = 0, s_elems = 0;   // workspace range zero-filled at the start of each apply
-    DevBuf d_maps, d_units, d_scales, d_regions, d_small, d_src_off, d_alpha;
+    DevBuf d_maps, d_units, d_scales, d_rows, d_regions, d_small, d_src_off, d_alpha;
     uint64_t sizes[N_BASES] = {0, 0, 0, 0, 0, 0};   // element counts of each base for validation
 
     void finish_tables() {
@@ -359,9 +358,48 @@ struct StagePlan {
                                       " off " + std::to_string(off) + " ext " + std::to_string(ext) + " size " +
                                       std::to_string(sizes[base]));
     }
+    // A stacked task: its rows are partitioned into contiguous row units; every unit's A block and
+    // C blocks lie inside their buffers; one plain (non-transposed, unmapped) A segment.
+    void check_stacked(const GemmTask& t, const std::vector<GemmSeg>& segs) const {
+        MQ_REQUIRE(uint64_t(t.rowunit) + t.M <= maps.size(), "row unit table range");
+        MQ_REQUIRE(t.seg0 + 1 == t.seg1 && t.seg1 <= segs.size(), "stacked task with other than one segment");
+        MQ_REQUIRE(t.colunit == MAP_NONE && t.cbw > 0, "stacked task shape");
+        const GemmSeg& g = segs[t.seg0];
+        MQ_REQUIRE(!g.trans_a && g.amap == MAP_NONE && g.units == MAP_NONE, "stacked task with a remapped A");
+        if (g.bmap != MAP_NONE) fail(MQ_ERR_INTERNAL, "stacked task with a remapped B");
+        check_in(g.b_base, g.b_off, g.trans_b ? extent(t.N, g.K, g.ldb) : extent(g.K, t.N, g.ldb), "stacked B");
+        MQ_REQUIRE(t.M % 16 == 0, "stacked task rows not a multiple of 16");
+        uint32_t m = 0;
+        while (m < t.M) {
+            const uint32_t u = maps[t.rowunit + m];
+            MQ_REQUIRE(u < rows.size(), "row unit out of range");
+            const GemmRow& R = rows[u];
+            MQ_REQUIRE(R.row0 == m && m % 16 == 0, "row unit does not start at its first row");
+            uint32_t e = m;
+            while (e < t.M && maps[t.rowunit + e] == u) ++e;
+            MQ_REQUIRE((e - m) % 16 == 0 && R.nrows <= e - m && e - m < R.nrows + 16, "row unit span");
+            const uint32_t nr = R.nrows;
+            MQ_REQUIRE(R.lda >= nr && R.ldc >= nr, "row unit leading dimensions");
+            check_in(g.a_base, g.a_off + R.a_src, extent(nr, g.K, R.lda), "stacked A block");
+            if (R.cmap == MAP_NONE) {
+                check_in(t.c_base, t.c_off + R.c_src, extent(nr, t.N, R.ldc), "stacked C block");
+            } else {
+                const uint32_t nblk = (t.N + t.cbw - 1) / t.cbw;
+                MQ_REQUIRE(uint64_t(R.cmap) + nblk <= maps.size(), "row unit column map range");
+                for (uint32_t j = 0; j < nblk; ++j)
+                    check_in(t.c_base, t.c_off + R.c_src + uint64_t(maps[R.cmap + j]) * R.cldk,
+                             extent(nr, std::min(t.cbw, t.N - j * t.cbw), R.ldc), "stacked C block");
+            }
+            m = e;
+        }
+    }
     void validate() const {
         auto gemm = [&](const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs) {
             for (const auto& t : tasks) {
+                if (t.rowunit != MAP_NONE) {
+                    check_stacked(t, segs);
+                    continue;
+                }
                 MQ_REQUIRE(t.ldc >= t.M || t.N <= 1, "ldc < M");
                 if (t.cmap == MAP_NONE) {
                     check_in(t.c_base, t.c_off + t.m_off, extent(t.M, t.N, t.ldc), "C");
@@ -465,14 +503,13 @@ struct StagePlan {
         upload(d_maps, maps, st);
         upload(d_units, units, st);
         upload(d_scales, scales, st);
+        upload(d_rows, rows, st);
         red.upload_all(st);
     }
     // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM.
     // mode != MODE_F64 (grouped plans only): T_D, S_D and W_D are FP32 in a float workspace.
     void stage(int k, hipStream_t st, Bases b, int mode = MODE_F64) const {
-        const uint32_t* mp = d_maps.as<uint32_t>();
-        const GemmUnit* un = d_units.as<GemmUnit>();
-        const double* sc = d_scales.as<double>();
+        const GemmTables tb{d_maps.as<uint32_t>(), d_units.as<GemmUnit>(), d_rows.as<GemmRow>(), d_scales.as<double>()};
         const bool mixed = mode != MODE_F64;
         switch (k) {
             case 0:
@@ -486,14 +523,14 @@ struct StagePlan {
             case 1:
                 // the T and final stages are bandwidth-bound block products: BF16x3 runs them in FP32
                 // and uses the split products in the S stage only
-                gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, mp, un, sc);
+                gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, tb);
                 break;
             case 2:
                 run_lib(st, b, mode);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
             default:
-                gF.run(st, mixed ? MODE_F32 : MODE_F64, "dfd", b, mp, un, sc);
+                gF.run(st, mixed ? MODE_F32 : MODE_F64, "dfd", b, tb);
                 red.run(st, b.p[BASE_BRA], b.p[BASE_OUT]);
         }
     }
@@ -666,6 +703,7 @@ GemmTask& add_gemm(std::vector<GemmTask>& tasks, std::vector<GemmSeg>& segs, uin
     t.colunit = MAP_NONE;
     t.nunits = 0;
     t.m_off = 0;
+    t.rowunit = MAP_NONE;
     tasks.push_back(t);
     return tasks.back();
 }
@@ -687,6 +725,7 @@ void add_gemm_range(std::vector<GemmTask>& tasks, uint8_t c_base, uint64_t c_off
     t.cbw = N;
     t.colunit = MAP_NONE;
     t.m_off = m_off;
+    t.rowunit = MAP_NONE;
     tasks.push_back(t);
 }
 
@@ -1402,9 +1441,17 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         }
         G.ref_flops = f;
     }
-    // ---- T stage
+    // ---- T stage.  The psi blocks that meet one stored right cohort (every (lb_in, s, ss) with
+    // lc_in + q_s = rc_in, for the cohort's rc_out) are stacked into the rows of one task, so the
+    // cohort is read once per column tile for all of them; each block's rows are written to its own
+    // pair slab and keys through a row unit.  Transposed cohorts keep one task per stored block
+    // (the partner's block, transposed and scaled).
     StagePlan& P = p->st;
-    for (auto& pr : G.pairs) {
+    struct RowBlock { uint32_t pair, s, ss; };
+    struct Stack { uint32_t K = 0, brs = 0, rbi = 0; std::vector<RowBlock> blocks; };
+    std::map<uint32_t, Stack> stacks;   // stored right cohort -> its psi blocks (deterministic order)
+    for (uint32_t pi = 0; pi < G.pairs.size(); ++pi) {
+        const auto& pr = G.pairs[pi];
         const auto& g = G.groups[pr.g];
         uint32_t lb = pr.lb_in, rb = pr.rb_out;
         Charge lc_in = left_i.charges[lb], rc_out = right_i.charges[rb];
@@ -1415,35 +1462,62 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             uint32_t ci = RI.cohort_index(rc_in, rc_out);
             if (rbi == right_i.size() || ci >= RI.n_cohorts()) continue;
             uint32_t K = right_i.sizes[rbi];
-            bool tr = RI.tr[ci];
-            uint32_t cie = tr ? RI.cohort_index(rc_out, rc_in) : ci;
+            if (!RI.tr[ci]) {
+                Stack& st = stacks[ci];
+                st.K = K;
+                st.brs = brs;
+                st.rbi = rbi;
+                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) st.blocks.push_back({pi, s, ss});
+                continue;
+            }
+            uint32_t cie = RI.cohort_index(rc_out, rc_in);
             MQ_REQUIRE(cie < RI.n_cohorts(), "transposed right cohort without partner");
-            uint64_t bs = uint64_t(K) * brs;
             for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
                 uint64_t a_off = site.rp_offset[lb] + (rpb(s, rbi) + uint64_t(ss) * K) * M;
-                if (!tr) {
-                    std::vector<uint32_t> jb = block_order(RI, ci);
-                    uint32_t nb = uint32_t(jb.size());
-                    if (!nb) continue;
-                    GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, g.t_off + pr.m_off, M, nb * brs, M,
-                                           {seg(BASE_IN, a_off, M, false, BASE_RIGHT, p->right->cohort_off[ci], K, false, K)});
-                    t.cmap = uint32_t(P.maps.size());
-                    t.cbw = brs;
-                    t.cldk = g.M;
-                    for (uint32_t j = 0; j < nb; ++j) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(jb[j], s, ss)));
-                } else {
-                    for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2) {
-                        if (!RI.has_block(ci, b2)) continue;
-                        uint64_t o = uint64_t(RI.offset(ci, b2));
-                        uint32_t k = g.kidx.at(GroupedLowering::kkey(b2, s, ss));
-                        add_gemm(P.t_tasks, P.t_segs, BASE_WS, g.t_off + uint64_t(k) * g.M + pr.m_off, M, brs, M,
-                                 {seg(BASE_IN, a_off, M, false, BASE_RIGHT, p->right->cohort_off[cie] + o, brs, true, K,
-                                      RI.conjugate_scale(ci, b2))});
-                        (void)bs;
-                    }
+                for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2) {
+                    if (!RI.has_block(ci, b2)) continue;
+                    uint64_t o = uint64_t(RI.offset(ci, b2));
+                    uint32_t k = g.kidx.at(GroupedLowering::kkey(b2, s, ss));
+                    add_gemm(P.t_tasks, P.t_segs, BASE_WS, g.t_off + uint64_t(k) * g.M + pr.m_off, M, brs, M,
+                             {seg(BASE_IN, a_off, M, false, BASE_RIGHT, p->right->cohort_off[cie] + o, brs, true, K,
+                                  RI.conjugate_scale(ci, b2))});
                 }
             }
         }
+    }
+    for (const auto& kv : stacks) {
+        const uint32_t ci = kv.first;
+        const Stack& st = kv.second;
+        std::vector<uint32_t> jb = block_order(RI, ci);
+        const uint32_t nb = uint32_t(jb.size());
+        if (!nb || st.blocks.empty()) continue;
+        std::vector<uint32_t> unit_of_row;
+        for (const RowBlock& rbk : st.blocks) {
+            const auto& pr = G.pairs[rbk.pair];
+            const auto& g = G.groups[pr.g];
+            const uint32_t lb = pr.lb_in, M = left_i.sizes[lb];
+            if (!M) continue;
+            GemmRow R{};
+            R.a_src = site.rp_offset[lb] + (rpb(rbk.s, st.rbi) + uint64_t(rbk.ss) * st.K) * M;
+            R.lda = M;
+            R.row0 = uint32_t(unit_of_row.size());
+            R.c_src = g.t_off + pr.m_off;
+            R.cldk = uint32_t(g.M);
+            R.ldc = M;
+            R.cmap = uint32_t(P.maps.size());
+            for (uint32_t j = 0; j < nb; ++j) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(jb[j], rbk.s, rbk.ss)));
+            MQ_REQUIRE(g.M < (1ull << 32), "group too large for a row unit");
+            R.nrows = M;
+            unit_of_row.insert(unit_of_row.end(), (M + 15) / 16 * 16, uint32_t(P.rows.size()));
+            P.rows.push_back(R);
+        }
+        if (unit_of_row.empty()) continue;
+        const uint32_t Mt = uint32_t(unit_of_row.size());
+        GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, nb * st.brs, Mt,
+                               {seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[ci], st.K, false, st.K)});
+        t.cbw = st.brs;
+        t.rowunit = uint32_t(P.maps.size());
+        P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
     }
     // ---- S stage: S_D = T_D W_D, one plain dense GEMM per group (rocBLAS; the MPO coefficient
     // matrices are 10-30% dense, and the library GEMM runs at near the MFMA peak)
@@ -1512,23 +1586,31 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             }
             return first;
         };
+        // segments: the left cohorts (lc_out, lc_in), each cut into pieces of whole blocks of about
+        // KPART columns of op(A), so that split-K parts below can be balanced however the cohort
+        // sizes are distributed
+        const uint64_t KPART = 2048;
         std::vector<GemmSeg> segs;
         for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
             Charge lc_in = left_i.charges[lb];
             uint32_t cl = LI.cohort_index(lc_out, lc_in);
             if (cl >= LI.n_cohorts()) continue;
             uint32_t ls_in = left_i.sizes[lb];
+            if (!ls_in) continue;
+            const uint32_t per = uint32_t(std::max<uint64_t>(1, KPART / ls_in));   // blocks per piece
             if (!LI.tr[cl]) {
                 std::vector<uint32_t> jb = block_order(LI, cl);
-                if (jb.empty()) continue;
-                GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cl], ls_out, false, BASE_WS, 0, ls_in, false,
-                                 uint32_t(jb.size()) * ls_in);
-                sg.bbh = ls_in;
-                sg.units = add_units(lb, jb);
-                segs.push_back(sg);
+                for (uint32_t j0 = 0; j0 < jb.size(); j0 += per) {
+                    const uint32_t j1 = std::min<uint32_t>(uint32_t(jb.size()), j0 + per);
+                    GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cl] + uint64_t(j0) * ls_in * ls_out, ls_out, false,
+                                     BASE_WS, 0, ls_in, false, (j1 - j0) * ls_in);
+                    sg.bbh = ls_in;
+                    sg.units = add_units(lb, std::vector<uint32_t>(jb.begin() + j0, jb.begin() + j1));
+                    segs.push_back(sg);
+                }
             } else {
                 // Hermitian partner: the blocks are the partner cohort's, transposed, each with its
-                // conjugation scale; one segment over all of them (A block map + block scales)
+                // conjugation scale (A block map + block scales)
                 uint32_t cle = LI.cohort_index(lc_in, lc_out);
                 MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
                 std::vector<uint32_t> b1s;
@@ -1541,14 +1623,16 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                     P.maps.push_back(uint32_t(o));
                     P.scales.push_back(LI.conjugate_scale(cl, b1));
                 }
-                if (b1s.empty()) continue;
-                GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cle], ls_in, true, BASE_WS, 0, ls_in, false,
-                                 uint32_t(b1s.size()) * ls_in);
-                sg.bbh = ls_in;
-                sg.amap = amap;
-                sg.ascale = ascale;
-                sg.units = add_units(lb, b1s);
-                segs.push_back(sg);
+                for (uint32_t j0 = 0; j0 < b1s.size(); j0 += per) {
+                    const uint32_t j1 = std::min<uint32_t>(uint32_t(b1s.size()), j0 + per);
+                    GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cle], ls_in, true, BASE_WS, 0, ls_in, false,
+                                     (j1 - j0) * ls_in);
+                    sg.bbh = ls_in;
+                    sg.amap = amap + j0;
+                    sg.ascale = ascale + j0;
+                    sg.units = add_units(lb, std::vector<uint32_t>(b1s.begin() + j0, b1s.begin() + j1));
+                    segs.push_back(sg);
+                }
             }
         }
         if (segs.empty()) continue;
@@ -1557,8 +1641,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         // sigma in a fixed order (deterministic, no atomics)
         uint64_t ktot = 0;
         for (const auto& sg : segs) ktot += sg.K;
-        const uint64_t KPART = 2048;
-        const uint32_t nparts = uint32_t(std::min<uint64_t>({uint64_t(segs.size()), 32, (ktot + KPART - 1) / KPART}));
+        const uint32_t nparts = uint32_t(std::min<uint64_t>({uint64_t(segs.size()), 64, (ktot + KPART - 1) / KPART}));
         auto emit = [&](uint8_t c_base, uint64_t c_off, size_t a0, size_t a1) {
             std::vector<GemmSeg> part(segs.begin() + a0, segs.begin() + a1);
             GemmTask& t = add_gemm(P.f_tasks, P.f_segs, c_base, c_off, ls_out, Nb, ls_out, part);
@@ -2028,7 +2111,7 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
     return guarded([&] {
         MQ_REQUIRE(p && bytes, "bad argument");
         const StagePlan& P = p->st;
-        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gF.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_regions.bytes +
+        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gF.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_rows.bytes + P.d_regions.bytes +
                  P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
     });
 }

@@ -113,10 +113,12 @@ struct Smem {
 template <typename TA, typename TB, typename TC, int MODE, int PM, int PN, int WM, int WN>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                             const GemmSeg* __restrict__ segs, Bases bases,
-                                                            const uint32_t* __restrict__ maps,
-                                                            const GemmUnit* __restrict__ units,
-                                                            const uint32_t* __restrict__ tile_task,
-                                                            const double* __restrict__ scales) {
+                                                            GemmTables tables,
+                                                            const uint32_t* __restrict__ tile_task) {
+    const uint32_t* __restrict__ maps = tables.maps;
+    const GemmUnit* __restrict__ units = tables.units;
+    const GemmRow* __restrict__ rows = tables.rows;
+    const double* __restrict__ scales = tables.scales;
     static_assert(PM * PN * 64 == GEMM_THREADS, "four waves");
     typedef typename ModeTraits<MODE>::T T;
     typedef typename ModeTraits<MODE>::Acc Acc;
@@ -130,9 +132,11 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const GemmTask tk = tasks[find_task(tasks, n_tasks, tile_task, tile)];
     const uint32_t local = tile - tk.tile_begin;
-    const uint32_t m0 = (local / tk.tiles_n) * BM;
-    const uint32_t n0 = (local % tk.tiles_n) * BN;
+    const bool stacked = tk.rowunit != MAP_NONE;
     const uint32_t M = tk.M, N = tk.N;
+    const uint32_t tiles_m = (M + BM - 1) / BM;
+    const uint32_t m0 = (stacked ? local % tiles_m : local / tk.tiles_n) * BM;
+    const uint32_t n0 = (stacked ? local / tiles_m : local % tk.tiles_n) * BN;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -146,6 +150,18 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
 #pragma unroll
         for (int j = 0; j < WN; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
+    // stacked task: the A row of this thread (m = tid % BM for every staged element of a
+    // non-transposed A) and its unit's column stride
+    uint64_t a_row = 0;
+    uint32_t a_ld = 0, a_ok = 0;
+    if (stacked) {
+        const uint32_t m = m0 + uint32_t(tid % BM);
+        const GemmRow R = rows[maps[tk.rowunit + min(m, M - 1)]];
+        const uint32_t i = m - R.row0;
+        a_ok = uint32_t(m < M) & uint32_t(i < R.nrows);
+        a_row = R.a_src + i;
+        a_ld = R.lda;
+    }
     // cursor over the K steps of all segments: (sgi, k0) with k0 < segs[sgi].K
     uint32_t sgi = tk.seg0, k0 = 0;
     GemmSeg sg;
@@ -184,6 +200,17 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             return;
         }
         a_scale = 1.0;
+        if (stacked) {
+#pragma unroll
+            for (int i = 0; i < NA; ++i) {
+                const int e = tid + GEMM_THREADS * i;
+                const uint32_t gk = k0 + e / BM;
+                const uint32_t ok = a_ok & uint32_t(gk < sg.K);
+                oka |= ok << i;
+                va[i] = A[(a_row + uint64_t(gk) * a_ld) & (0ull - uint64_t(ok))];
+            }
+            return;
+        }
         const uint64_t sm = sg.trans_a ? sg.lda : 1u, sk = sg.trans_a ? 1u : sg.lda;
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
@@ -332,6 +359,47 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
         }
     }
 
+    if (stacked) {
+        // row units: each 16-row block of this wave lies in one unit, with its own C block and
+        // column remap.  The unit ids, then the descriptors, then the column offsets are loaded
+        // for all blocks before the first store, so the lookups overlap.
+        TC* __restrict__ C = reinterpret_cast<TC*>(bases.p[tk.c_base]) + tk.c_off;
+        uint32_t uid[WM];
+#pragma unroll
+        for (int i = 0; i < WM; ++i) uid[i] = maps[tk.rowunit + min(m0 + wm + 16 * i, M - 1)];
+        GemmRow R[WM];
+#pragma unroll
+        for (int i = 0; i < WM; ++i) R[i] = rows[uid[i]];
+#pragma unroll
+        for (int i = 0; i < WM; ++i) {
+            uint64_t coff[WN];
+#pragma unroll
+            for (int j = 0; j < WN; ++j) {
+                const uint32_t gn = min(n0 + wn + 16 * j + l16, N - 1);
+                if (R[i].cmap != MAP_NONE) {
+                    const uint32_t blk = gn / tk.cbw;
+                    coff[j] = uint64_t(maps[R[i].cmap + blk]) * R[i].cldk + uint64_t(gn - blk * tk.cbw) * R[i].ldc;
+                } else {
+                    coff[j] = uint64_t(gn) * R[i].ldc;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t gm = m0 + wm + 16 * i + d_row<MODE>(r, lk);
+                const uint32_t li = gm - R[i].row0;
+                if (gm >= M || li >= R[i].nrows) continue;
+                TC* __restrict__ Cm = C + R[i].c_src + li;
+#pragma unroll
+                for (int j = 0; j < WN; ++j) {
+                    if (n0 + wn + 16 * j + l16 >= N) continue;
+                    TC v = TC(acc[i][j][r]);
+                    if (tk.beta) v += Cm[coff[j]];
+                    Cm[coff[j]] = v;
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < WN; ++j) {
         const uint32_t gn = n0 + wn + 16 * j + l16;
@@ -357,10 +425,9 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
 template <typename TA, typename TB, typename TC, int MODE>
 __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                                  const GemmSeg* __restrict__ segs, Bases bases,
-                                                                 const uint32_t* __restrict__ maps,
-                                                                 const GemmUnit* __restrict__ units,
-                                                                 const uint32_t* __restrict__ tile_task,
-                                                                 const double* __restrict__ scales) {
+                                                                 GemmTables tables,
+                                                                 const uint32_t* __restrict__ tile_task) {
+    const uint32_t* __restrict__ maps = tables.maps;
     typedef typename ModeTraits<MODE>::T T;
     typedef typename ModeTraits<MODE>::Acc Acc;
     constexpr int KS = ModeTraits<MODE>::KS;
@@ -426,29 +493,28 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_tiny_kernel(const GemmTask*
 // ------------------------------------------------------------------------------ dispatch
 template <typename TA, typename TB, typename TC, int MODE>
 static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
-                               uint32_t n_tiles, Bases bases, const uint32_t* maps, const GemmUnit* units,
-                               const uint32_t* tile_task, const double* scales) {
+                               uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task) {
     const dim3 grid(n_tiles), block(GEMM_THREADS);
     switch (cfg) {
         case GEMM_CFG_128x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 4, 4>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units, tile_task, scales);
+                               bases, tb, tile_task);
             break;
         case GEMM_CFG_64x64:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 2, 2, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units, tile_task, scales);
+                               bases, tb, tile_task);
             break;
         case GEMM_CFG_32x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 2, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units, tile_task, scales);
+                               bases, tb, tile_task);
             break;
         case GEMM_CFG_16x128:
             hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, MODE, 1, 4, 1, 2>), grid, block, 0, st, tasks, n_tasks, segs,
-                               bases, maps, units, tile_task, scales);
+                               bases, tb, tile_task);
             break;
         case GEMM_CFG_16x16:
             hipLaunchKernelGGL((gemm_tiny_kernel<TA, TB, TC, MODE>), grid, block, 0, st, tasks, n_tasks, segs, bases,
-                               maps, units, tile_task, scales);
+                               tb, tile_task);
             break;
         default:
             return hipErrorInvalidValue;
@@ -457,21 +523,19 @@ static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, u
 }
 
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
-                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps,
-                       const GemmUnit* units, const uint32_t* tile_task, const double* scales) {
+                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task) {
     if (n_tiles == 0) return hipSuccess;
     if (mode == MODE_F64)
-        return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);
+        return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
     const bool da = types[0] == 'd', db = types[1] == 'd', dc = types[2] == 'd';
-#define MQ_DISPATCH(MD)                                                                                          \
-    if (da && db && !dc)                                                                                         \
-        return launch_typed<double, double, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);     \
-    if (!da && !db && !dc)                                                                                       \
-        return launch_typed<float, float, float, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);       \
-    if (da && !db && dc)                                                                                         \
-        return launch_typed<double, float, double, MD>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, maps, units, tile_task, scales);
-    if (mode == MODE_F32) { MQ_DISPATCH(MODE_F32) }
-#undef MQ_DISPATCH
+    if (mode == MODE_F32) {
+        if (da && db && !dc)
+            return launch_typed<double, double, float, MODE_F32>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
+        if (!da && !db && !dc)
+            return launch_typed<float, float, float, MODE_F32>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
+        if (da && !db && dc)
+            return launch_typed<double, float, double, MODE_F32>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
+    }
     return hipErrorInvalidValue;
 }
 

@@ -47,7 +47,23 @@ struct GemmTask {
     uint32_t cmap, cbw;
     uint32_t colunit, nunits;
     uint32_t m_off;   // row offset of the task in op(A) and C (tasks sharing one segment range)
+    uint32_t rowunit; // row units (MAP_NONE = off), below
     uint8_t c_base, beta, pad0, pad1;
+};
+
+// Row units of a stacked task (rowunit != MAP_NONE): several A blocks that multiply the same B are
+// stacked into the rows of one task, so the B panel of a column tile is read once for all of them.
+// Unit u occupies rows [R.row0, R.row0 + span) of the task, span = nrows rounded up to a multiple
+// of 16 (row0 too is a multiple of 16, so every 16-row MFMA block lies in one unit); row m belongs
+// to unit u = maps[rowunit + m], with descriptor R = row table[u] and local row i = m - R.row0:
+//     A(m, k)  = A[a_off + R.a_src + i + k * R.lda]    i < R.nrows, else 0  (non-transposed A only)
+//     C(m, n)  = C[c_off + R.c_src + maps[R.cmap + n / cbw] * R.cldk + (n % cbw) * R.ldc + i]
+// (R.cmap == MAP_NONE: C[c_off + R.c_src + n * R.ldc + i]); padding rows are not stored.  The
+// task's own cmap, ldc and m_off are unused.  Tiles of a stacked task are numbered row tile
+// fastest: the row tiles that read one B panel run next to each other and share it in L2.
+struct GemmRow {
+    uint64_t a_src, c_src;
+    uint32_t lda, row0, cldk, ldc, cmap, nrows;
 };
 
 //  row-block remap of a transposed A (trans_a only): column k of op(A) lies in block j = k / bbh,
@@ -97,13 +113,21 @@ inline int gemm_pick_cfg(uint32_t M, uint32_t N) {
 }
 constexpr int GEMM_THREADS = 256;
 
+// Side tables of a launch: remap entries, column units, row units, block scales.
+struct GemmTables {
+    const uint32_t* maps;
+    const GemmUnit* units;
+    const GemmRow* rows;
+    const double* scales;
+};
+
 // One launch of all tasks of one tile configuration.  mode MODE_F64: all operands double.
 // Otherwise `types` gives the storage of A, B and C as 'd' (double) or 'f' (float): "ddf",
 // "fff" or "dfd".
 // tile_task (may be null): task index of every tile of the launch.
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
-                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, const uint32_t* maps,
-                       const GemmUnit* units, const uint32_t* tile_task, const double* scales);
+                       const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tables,
+                       const uint32_t* tile_task);
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
 constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions
