@@ -272,8 +272,9 @@ int mq_plan_set_graphs(mq_plan* plan, int enable);
 /* Autotune the GEMM tile configurations of the plan's T and final stages for `precision`.  Tasks
  * fall into row-size classes (M <= 16, 32, 64, 128, larger); for each class present, candidate tile
  * shapes are timed on the plan's own stage (one warm-up and three repetitions) and the fastest is
- * kept.  Decisions are cached in the context, keyed by the stage's flop histogram over the classes
- * (log2 buckets): a plan whose histogram was tuned before applies the cached choice without timing.
+ * kept.  Decisions are cached in the context, keyed by stage, precision and the stage's flop
+ * histogram over the classes (log2 buckets): a plan whose histogram was tuned before applies the
+ * cached choice without timing.
  * timed (may be NULL) receives the number of stage timings made.  Captured graphs of the plan are
  * dropped.  No reference counterpart. */
 int mq_plan_autotune(mq_plan* plan, int precision, int* timed);

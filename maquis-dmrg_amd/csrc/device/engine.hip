@@ -80,6 +80,14 @@ struct DevBuf {
     void ensure(size_t n) {
         if (n > bytes) alloc(n);
     }
+    // allocation cleared to zero bits: library GEMMs called with beta = 0 may still read their C
+    // operand (0 * NaN of stale memory is NaN), so workspaces never hold non-finite garbage
+    void alloc_zeroed(size_t n, hipStream_t st) {
+        alloc((n + 7) / 8 * 8);
+        if (!p) return;
+        HIPCHK(launch_fill(st, static_cast<double*>(p), bytes / 8, 0.0));
+        HIPCHK(hipStreamSynchronize(st));
+    }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
@@ -117,7 +125,7 @@ struct mq_ctx {
     std::vector<hipEvent_t> side_ev;
     hipEvent_t fork_ev = nullptr;
     std::map<std::pair<std::vector<const mq_plan*>, int>, BatchGraph> batches;
-    // autotuned tile policies, keyed by (stage, GEMM mode, log2 flop histogram over size classes)
+    // autotuned tile policies, keyed by (stage, precision, grouped, log2 flop histogram over size classes)
     std::map<std::vector<int>, std::array<int, 5>> tuned;
     void drop_batches() {
         for (auto& kv : batches)
@@ -298,6 +306,11 @@ struct StagePlan {
     std::vector<GemmUnit> units;            // column units of B (grouped final stage)
     std::vector<double> scales;             // block scales of remapped transposed A
     std::vector<GemmRow> rows;              // row units of stacked tasks (grouped T stage)
+    // grouped path: T_D ranges no T task writes, cleared at the start of an application (in the
+    // workspace, or in the three A' copies of the bf16 buffer for BF16x3)
+    std::vector<ZeroChunk> zero_ws, zero_bf;
+    bool split_t = false;       // all T tasks stacked: BF16x3 T stage writes A' directly
+    uint64_t bf_a_elems = 0;    // A' part of the bf16 buffer
     GemmStage gT, gF;   // device form of the T and final GEMM stages
     // grouped path: split-K partial sums of the final stage (FP64, base BASE_BRA) and their
     // reduction into sigma
@@ -320,7 +333,7 @@ struct StagePlan {
     uint32_t g_chunks = 0;
     uint64_t ws_elems = 0;
     uint64_t s_begin = 0, s_elems = 0;   // workspace range zero-filled at the start of each apply
-    DevBuf d_maps, d_units, d_scales, d_rows, d_regions, d_small, d_src_off, d_alpha;
+    DevBuf d_maps, d_units, d_scales, d_rows, d_zero_ws, d_zero_bf, d_regions, d_small, d_src_off, d_alpha;
     uint64_t sizes[N_BASES] = {0, 0, 0, 0, 0, 0};   // element counts of each base for validation
 
     void finish_tables() {
@@ -381,14 +394,19 @@ struct StagePlan {
             const uint32_t nr = R.nrows;
             MQ_REQUIRE(R.lda >= nr && R.ldc >= nr, "row unit leading dimensions");
             check_in(g.a_base, g.a_off + R.a_src, extent(nr, g.K, R.lda), "stacked A block");
+            // C blocks: [off, off + ext) of the workspace, and of the A' copies when split
+            auto c_block = [&](uint64_t rel, uint64_t ext) {
+                check_in(t.c_base, t.c_off + R.c_src + rel, ext, "stacked C block");
+                if (split_t && ext && t.c_off + R.c_split + rel + 2 * R.split_stride + ext > bf_a_elems)
+                    fail(MQ_ERR_INTERNAL, "stacked split output out of bounds");
+            };
             if (R.cmap == MAP_NONE) {
-                check_in(t.c_base, t.c_off + R.c_src, extent(nr, t.N, R.ldc), "stacked C block");
+                c_block(0, extent(nr, t.N, R.ldc));
             } else {
                 const uint32_t nblk = (t.N + t.cbw - 1) / t.cbw;
                 MQ_REQUIRE(uint64_t(R.cmap) + nblk <= maps.size(), "row unit column map range");
                 for (uint32_t j = 0; j < nblk; ++j)
-                    check_in(t.c_base, t.c_off + R.c_src + uint64_t(maps[R.cmap + j]) * R.cldk,
-                             extent(nr, std::min(t.cbw, t.N - j * t.cbw), R.ldc), "stacked C block");
+                    c_block(uint64_t(maps[R.cmap + j]) * R.cldk, extent(nr, std::min(t.cbw, t.N - j * t.cbw), R.ldc));
             }
             m = e;
         }
@@ -472,6 +490,12 @@ struct StagePlan {
         };
         gemm(t_tasks, t_segs);
         gemm(f_tasks, f_segs);
+        if (split_t)
+            for (const auto& t : t_tasks)
+                MQ_REQUIRE(t.rowunit != MAP_NONE && t.c_base == BASE_WS && !t.beta, "split T stage needs stacked tasks");
+        for (const auto& z : zero_ws) check_in(BASE_WS, z.off, z.len, "zero chunk");
+        for (const auto& z : zero_bf)
+            if (z.off + z.len > bf_a_elems) fail(MQ_ERR_INTERNAL, "bf16 zero chunk out of bounds");
         for (const auto& g : lib) {
             MQ_REQUIRE(g.lda >= g.M && g.ldb >= g.K && g.ldc >= g.M, "library GEMM leading dimensions");
             MQ_REQUIRE(g.a_off + extent(g.M, g.K, g.lda) <= t_end, "library GEMM A outside T_D");
@@ -504,6 +528,8 @@ struct StagePlan {
         upload(d_units, units, st);
         upload(d_scales, scales, st);
         upload(d_rows, rows, st);
+        upload(d_zero_ws, zero_ws, st);
+        upload(d_zero_bf, zero_bf, st);
         red.upload_all(st);
     }
     // the four stages of an application: zero-fill, T GEMM, middle (gather or S GEMM), final GEMM.
@@ -519,11 +545,22 @@ struct StagePlan {
                         HIPCHK(launch_fill(st, reinterpret_cast<double*>(reinterpret_cast<float*>(b.p[BASE_WS]) + s_begin),
                                            s_elems / 2, 0.0));
                 }
+                if (mode == MODE_BF16X3 && split_t)
+                    HIPCHK(launch_zero_chunks(st, d_zero_bf.as<ZeroChunk>(), uint32_t(zero_bf.size()), bf, 2));
+                else
+                    HIPCHK(launch_zero_chunks(st, d_zero_ws.as<ZeroChunk>(), uint32_t(zero_ws.size()), b.p[BASE_WS],
+                                              mixed ? 4 : 8));
                 break;
             case 1:
                 // the T and final stages are bandwidth-bound block products: BF16x3 runs them in FP32
                 // and uses the split products in the S stage only
-                gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, tb);
+                if (mode == MODE_BF16X3 && split_t) {
+                    Bases bs = b;   // C of the stacked T tasks: the A' copies in the bf16 buffer
+                    bs.p[BASE_WS] = reinterpret_cast<double*>(bf);
+                    gT.run(st, MODE_F32, "dds", bs, tb);
+                } else {
+                    gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, tb);
+                }
                 break;
             case 2:
                 run_lib(st, b, mode);
@@ -558,7 +595,7 @@ struct StagePlan {
             const float one = 1.f, zero = 0.f;
             const rocblas_datatype b16 = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
             for (const auto& g : lib) {
-                HIPCHK(launch_split3_bf16(st, ws + g.a_off, bf + g.bfa_off, uint64_t(g.M) * g.K));
+                if (!split_t) HIPCHK(launch_split3_bf16(st, ws + g.a_off, bf + g.bfa_off, uint64_t(g.M) * g.K));
                 BLASCHK(rocblas_gemm_ex(blas, nn, nn, g.M, g.N, 3 * g.K, &one, bf + g.bfa_off, b16, g.M,
                                         bf + g.bfb_off, b16, 3 * g.K, &zero, ws + g.c_off, f32, g.ldc, ws + g.c_off,
                                         f32, g.ldc, f32, rocblas_gemm_algo_standard, 0, 0));
@@ -639,7 +676,7 @@ struct mq_plan {
         } else {
             MQ_REQUIRE(grouped, "reduced-precision sigma needs a GROUPED plan");
             if (!ws32.p) {
-                ws32.alloc(std::max<uint64_t>(st.ws_elems, 64) * sizeof(float));
+                ws32.alloc_zeroed(std::max<uint64_t>(st.ws_elems, 64) * sizeof(float), ctx->stream);
                 if (!ws_init.empty()) {
                     std::vector<float> w(ws_init.begin(), ws_init.end());
                     HIPCHK(hipMemcpyAsync(ws32.as<float>() + ws_init_off, w.data(), w.size() * sizeof(float),
@@ -648,10 +685,10 @@ struct mq_plan {
                 }
             }
             b.p[BASE_WS] = reinterpret_cast<double*>(ws32.as<float>());
-            if (mode == MODE_BF16X3 && !st.lib.empty() && !bf.p) {
+            if (mode == MODE_BF16X3 && st.bf_elems && !bf.p) {
                 // bf16 operands of the one-GEMM BF16x3 form: A' regions (written per application) and
                 // B' = [W_lo; W_hi; W_hi] per group (constant, split here)
-                bf.alloc(std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t));
+                bf.alloc_zeroed(std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t), ctx->stream);
                 st.bf = bf.as<uint16_t>();
                 for (const auto& g : st.lib) {
                     const double* W = ws_init.data() + (g.b_off - ws_init_off);
@@ -1262,7 +1299,7 @@ struct GroupedLowering {
         Charge delta;
         std::vector<uint64_t> K, N;
         std::unordered_map<uint64_t, uint32_t> kidx, nidx;
-        uint64_t M = 0, t_off = 0, s_off = 0, w_off = 0;
+        uint64_t M = 0, t_off = 0, s_off = 0, w_off = 0, bf_off = 0;
         std::vector<uint32_t> pairs;
     };
     std::vector<Pair> pairs;
@@ -1375,8 +1412,18 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         g.t_off = ws;
         ws += align32(g.M * g.K.size());
     }
+    // T_D columns that no pair writes are cleared per application by zero chunks (below), not by
+    // a dense fill
     p->st.s_begin = 0;
-    p->st.s_elems = ws;   // T_D must start at zero (absent blocks)
+    p->st.s_elems = 0;
+    p->st.t_end = ws;
+    // bf16 buffer: per group A' = [T_hi | T_lo | T_hi] (M x 3K), written by the T stage of BF16x3
+    // applications; the B' regions follow (S stage)
+    for (auto& g : G.groups) {
+        g.bf_off = p->st.bf_elems;
+        p->st.bf_elems += align32(3ull * g.M * g.K.size());
+    }
+    p->st.bf_a_elems = p->st.bf_elems;
     for (auto& g : G.groups) {
         g.s_off = ws;
         ws += align32(g.M * g.N.size());
@@ -1441,83 +1488,115 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         }
         G.ref_flops = f;
     }
-    // ---- T stage.  The psi blocks that meet one stored right cohort (every (lb_in, s, ss) with
+    // ---- T stage.  The psi blocks that meet one right cohort (every (lb_in, s, ss) with
     // lc_in + q_s = rc_in, for the cohort's rc_out) are stacked into the rows of one task, so the
     // cohort is read once per column tile for all of them; each block's rows are written to its own
-    // pair slab and keys through a row unit.  Transposed cohorts keep one task per stored block
-    // (the partner's block, transposed and scaled).
+    // pair slab and keys through a row unit.  A stored cohort is one task over all its blocks
+    // (columns (j, rs) of block j); a transposed cohort is one task per block: the partner's block,
+    // transposed and scaled by its conjugation factor.
     StagePlan& P = p->st;
     struct RowBlock { uint32_t pair, s, ss; };
     struct Stack { uint32_t K = 0, brs = 0, rbi = 0; std::vector<RowBlock> blocks; };
-    std::map<uint32_t, Stack> stacks;   // stored right cohort -> its psi blocks (deterministic order)
+    std::map<uint32_t, Stack> stacks;   // right cohort -> its psi blocks (deterministic order)
     for (uint32_t pi = 0; pi < G.pairs.size(); ++pi) {
         const auto& pr = G.pairs[pi];
-        const auto& g = G.groups[pr.g];
         uint32_t lb = pr.lb_in, rb = pr.rb_out;
         Charge lc_in = left_i.charges[lb], rc_out = right_i.charges[rb];
-        uint32_t M = left_i.sizes[lb], brs = right_i.sizes[rb];
         for (uint32_t s = 0; s < phys_i.size(); ++s) {
             Charge rc_in = lc_in + phys_i.charges[s];
             uint32_t rbi = right_i.position(rc_in);
             uint32_t ci = RI.cohort_index(rc_in, rc_out);
             if (rbi == right_i.size() || ci >= RI.n_cohorts()) continue;
-            uint32_t K = right_i.sizes[rbi];
-            if (!RI.tr[ci]) {
-                Stack& st = stacks[ci];
-                st.K = K;
-                st.brs = brs;
-                st.rbi = rbi;
-                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) st.blocks.push_back({pi, s, ss});
-                continue;
-            }
-            uint32_t cie = RI.cohort_index(rc_out, rc_in);
-            MQ_REQUIRE(cie < RI.n_cohorts(), "transposed right cohort without partner");
-            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
-                uint64_t a_off = site.rp_offset[lb] + (rpb(s, rbi) + uint64_t(ss) * K) * M;
-                for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2) {
-                    if (!RI.has_block(ci, b2)) continue;
-                    uint64_t o = uint64_t(RI.offset(ci, b2));
-                    uint32_t k = g.kidx.at(GroupedLowering::kkey(b2, s, ss));
-                    add_gemm(P.t_tasks, P.t_segs, BASE_WS, g.t_off + uint64_t(k) * g.M + pr.m_off, M, brs, M,
-                             {seg(BASE_IN, a_off, M, false, BASE_RIGHT, p->right->cohort_off[cie] + o, brs, true, K,
-                                  RI.conjugate_scale(ci, b2))});
-                }
-            }
+            Stack& st = stacks[ci];
+            st.K = right_i.sizes[rbi];
+            st.brs = right_i.sizes[rb];
+            st.rbi = rbi;
+            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) st.blocks.push_back({pi, s, ss});
         }
     }
-    for (const auto& kv : stacks) {
-        const uint32_t ci = kv.first;
-        const Stack& st = kv.second;
-        std::vector<uint32_t> jb = block_order(RI, ci);
-        const uint32_t nb = uint32_t(jb.size());
-        if (!nb || st.blocks.empty()) continue;
+    // one stacked task over the given psi blocks: row units with the column keys of `b2s`
+    auto stacked_task = [&](const Stack& st, const std::vector<uint32_t>& b2s, const GemmSeg& sg) {
         std::vector<uint32_t> unit_of_row;
         for (const RowBlock& rbk : st.blocks) {
             const auto& pr = G.pairs[rbk.pair];
             const auto& g = G.groups[pr.g];
             const uint32_t lb = pr.lb_in, M = left_i.sizes[lb];
             if (!M) continue;
+            MQ_REQUIRE(g.M < (1ull << 32), "group too large for a row unit");
             GemmRow R{};
             R.a_src = site.rp_offset[lb] + (rpb(rbk.s, st.rbi) + uint64_t(rbk.ss) * st.K) * M;
             R.lda = M;
             R.row0 = uint32_t(unit_of_row.size());
+            R.nrows = M;
             R.c_src = g.t_off + pr.m_off;
+            R.c_split = g.bf_off + pr.m_off;
+            R.split_stride = g.M * g.K.size();
             R.cldk = uint32_t(g.M);
             R.ldc = M;
             R.cmap = uint32_t(P.maps.size());
-            for (uint32_t j = 0; j < nb; ++j) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(jb[j], rbk.s, rbk.ss)));
-            MQ_REQUIRE(g.M < (1ull << 32), "group too large for a row unit");
-            R.nrows = M;
+            for (uint32_t b2 : b2s) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(b2, rbk.s, rbk.ss)));
             unit_of_row.insert(unit_of_row.end(), (M + 15) / 16 * 16, uint32_t(P.rows.size()));
             P.rows.push_back(R);
         }
-        if (unit_of_row.empty()) continue;
+        if (unit_of_row.empty() || !st.brs) return;
         const uint32_t Mt = uint32_t(unit_of_row.size());
-        GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, nb * st.brs, Mt,
-                               {seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[ci], st.K, false, st.K)});
+        GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, uint32_t(b2s.size()) * st.brs, Mt, {sg});
         t.cbw = st.brs;
         t.rowunit = uint32_t(P.maps.size());
         P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+    };
+    for (const auto& kv : stacks) {
+        const uint32_t ci = kv.first;
+        const Stack& st = kv.second;
+        if (st.blocks.empty()) continue;
+        if (!RI.tr[ci]) {
+            std::vector<uint32_t> jb = block_order(RI, ci);
+            if (jb.empty()) continue;
+            stacked_task(st, jb, seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[ci], st.K, false, st.K));
+        } else {
+            const uint32_t cie = RI.cohort_index(RI.ket.charges[RI.ket_pos[ci]], RI.bra.charges[RI.bra_pos[ci]]);
+            MQ_REQUIRE(cie < RI.n_cohorts(), "transposed right cohort without partner");
+            for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2) {
+                if (!RI.has_block(ci, b2)) continue;
+                const uint64_t o = uint64_t(RI.offset(ci, b2));
+                stacked_task(st, {b2}, seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[cie] + o, st.brs, true,
+                                           st.K, RI.conjugate_scale(ci, b2)));
+            }
+        }
+    }
+    P.split_t = true;   // every T task is stacked: BF16x3 writes A' from the T epilogue
+    // ---- zero chunks: the (pair, key) slabs of T_D that no T task writes, merged along the rows
+    // of each key column (consecutive pairs of a group are consecutive rows)
+    auto add_zero = [&](std::vector<ZeroChunk>& list, uint64_t off, uint64_t len) {
+        for (uint64_t o = 0; o < len; o += ZERO_CHUNK)
+            list.push_back({off + o, uint32_t(std::min<uint64_t>(ZERO_CHUNK, len - o)), 0});
+    };
+    for (const auto& g : G.groups) {
+        const uint64_t Kg = g.K.size();
+        std::vector<std::vector<char>> has(g.pairs.size(), std::vector<char>(Kg, 0));
+        for (size_t q = 0; q < g.pairs.size(); ++q)
+            for (auto k : G.pairs[g.pairs[q]].kk) has[q][k] = 1;
+        for (uint64_t k = 0; k < Kg; ++k) {
+            uint64_t run0 = 0, run = 0;
+            auto flush = [&]() {
+                if (!run) return;
+                add_zero(P.zero_ws, g.t_off + k * g.M + run0, run);
+                for (uint64_t c = 0; c < 3; ++c) add_zero(P.zero_bf, g.bf_off + (k + c * Kg) * g.M + run0, run);
+                run = 0;
+            };
+            for (size_t q = 0; q < g.pairs.size(); ++q) {
+                const auto& pr = G.pairs[g.pairs[q]];
+                const uint64_t slab = uint64_t(left_i.sizes[pr.lb_in]) * right_i.sizes[pr.rb_out];
+                if (has[q][k] || !slab) {
+                    if (slab) flush();
+                    continue;
+                }
+                if (!run) run0 = pr.m_off;
+                MQ_REQUIRE(run0 + run == pr.m_off, "pair slabs of a group are not consecutive");
+                run += slab;
+            }
+            flush();
+        }
     }
     // ---- S stage: S_D = T_D W_D, one plain dense GEMM per group (rocBLAS; the MPO coefficient
     // matrices are 10-30% dense, and the library GEMM runs at near the MFMA peak)
@@ -1527,17 +1606,12 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         const uint64_t Kg = g.K.size(), Ng = g.N.size();
         MQ_REQUIRE(3 * Kg < (1ull << 31), "group too large for one library GEMM");
         P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(Ng), uint32_t(Kg), uint32_t(g.M),
-                         uint32_t(Kg), uint32_t(g.M), 0, 0});
+                         uint32_t(Kg), uint32_t(g.M), g.bf_off, 0});
     }
-    for (auto& L : P.lib) {   // bf16 buffer: A' regions, then B' regions
-        L.bfa_off = P.bf_elems;
-        P.bf_elems += align32(3ull * L.M * L.K);
-    }
-    for (auto& L : P.lib) {
+    for (auto& L : P.lib) {   // B' regions after the A' regions
         L.bfb_off = P.bf_elems;
         P.bf_elems += align32(3ull * L.K * L.N);
     }
-    P.t_end = P.s_begin + P.s_elems;
     // ---- final stage: one task per output block lb_out (ls_out x all its columns), K segments over
     // the left cohorts (lc_out, lc_in).  The block's columns fall into units (s, ss2) of rs columns;
     // unit u of segment lb_in reads its B from S_D of the pair (lb_in, rb(s)) through the unit table,
@@ -1735,7 +1809,7 @@ int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sche
         P.sizes[BASE_LEFT] = left->total;
         P.sizes[BASE_RIGHT] = right->total;
         P.validate();
-        p->ws.alloc(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
+        p->ws.alloc_zeroed(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double), ctx->stream);
         P.upload_all(ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
         *out = p.release();
@@ -1773,7 +1847,7 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         P.sizes[BASE_BRA] = P.part_elems;
         P.validate();
         if (P.part_elems) p->part.alloc(P.part_elems * sizeof(double));
-        p->ws.alloc(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
+        p->ws.alloc_zeroed(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double), ctx->stream);
         if (!ws_init.empty())
             HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
                                   hipMemcpyHostToDevice, ctx->stream));
@@ -1946,7 +2020,6 @@ int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
         mq_ctx* ctx = p->ctx;
         hipStream_t st = ctx->stream;
         StagePlan& P = p->st;
-        const int mode = precision == MQ_PREC_F64 ? MODE_F64 : MODE_F32;   // the hand-written stages' mode
         if (!p->io.p) p->io.alloc(2 * std::max<uint64_t>(p->vec_size, 1) * sizeof(double));
         double* in = p->io.as<double>();
         double* out = in + std::max<uint64_t>(p->vec_size, 1);
@@ -1973,7 +2046,7 @@ int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
         for (StageRef sr : {StageRef{1, &P.gT, &P.t_segs}, StageRef{3, &P.gF, &P.f_segs}}) {
             GemmStage& G = *sr.g;
             std::vector<uint64_t> cf = G.class_flops(*sr.segs);
-            std::vector<int> key = {sr.k, mode, int(p->grouped)};
+            std::vector<int> key = {sr.k, precision, int(p->grouped)};
             for (auto f : cf) key.push_back(f ? int(std::log2(double(f))) : -1);
             auto apply = [&](const TilePolicy& pol) {
                 G.policy = pol;

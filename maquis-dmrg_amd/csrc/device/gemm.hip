@@ -24,6 +24,8 @@
 //   f32 16x16x4 : A row l%16, k lk;  B col l%16, k lk;  D[r] row 4lk + r, col l%16
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace mq {
@@ -120,6 +122,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     const GemmRow* __restrict__ rows = tables.rows;
     const double* __restrict__ scales = tables.scales;
     static_assert(PM * PN * 64 == GEMM_THREADS, "four waves");
+    constexpr bool SPLIT = std::is_same<TC, SplitBf16>::value;
+    typedef typename std::conditional<SPLIT, uint16_t, TC>::type CT;   // C element storage
     typedef typename ModeTraits<MODE>::T T;
     typedef typename ModeTraits<MODE>::Acc Acc;
     constexpr int BK = ModeTraits<MODE>::BK;
@@ -363,7 +367,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
         // row units: each 16-row block of this wave lies in one unit, with its own C block and
         // column remap.  The unit ids, then the descriptors, then the column offsets are loaded
         // for all blocks before the first store, so the lookups overlap.
-        TC* __restrict__ C = reinterpret_cast<TC*>(bases.p[tk.c_base]) + tk.c_off;
+        CT* __restrict__ C = reinterpret_cast<CT*>(bases.p[tk.c_base]) + tk.c_off;
         uint32_t uid[WM];
 #pragma unroll
         for (int i = 0; i < WM; ++i) uid[i] = maps[tk.rowunit + min(m0 + wm + 16 * i, M - 1)];
@@ -388,18 +392,35 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                 const uint32_t gm = m0 + wm + 16 * i + d_row<MODE>(r, lk);
                 const uint32_t li = gm - R[i].row0;
                 if (gm >= M || li >= R[i].nrows) continue;
-                TC* __restrict__ Cm = C + R[i].c_src + li;
+                if constexpr (SPLIT) {
+                    uint16_t* __restrict__ Cm = C + R[i].c_split + li;
+                    const uint64_t sd = R[i].split_stride;
 #pragma unroll
-                for (int j = 0; j < WN; ++j) {
-                    if (n0 + wn + 16 * j + l16 >= N) continue;
-                    TC v = TC(acc[i][j][r]);
-                    if (tk.beta) v += Cm[coff[j]];
-                    Cm[coff[j]] = v;
+                    for (int j = 0; j < WN; ++j) {
+                        if (n0 + wn + 16 * j + l16 >= N) continue;
+                        const float v = float(acc[i][j][r]);
+                        const uint16_t hi = bf16_rne(v);
+                        const uint16_t lo = bf16_rne(v - __uint_as_float(uint32_t(hi) << 16));
+                        Cm[coff[j]] = hi;
+                        Cm[coff[j] + sd] = lo;
+                        Cm[coff[j] + 2 * sd] = hi;
+                    }
+                } else {
+                    CT* __restrict__ Cm = C + R[i].c_src + li;
+#pragma unroll
+                    for (int j = 0; j < WN; ++j) {
+                        if (n0 + wn + 16 * j + l16 >= N) continue;
+                        CT v = CT(acc[i][j][r]);
+                        if (tk.beta) v += Cm[coff[j]];
+                        Cm[coff[j]] = v;
+                    }
                 }
             }
         }
         return;
     }
+    if constexpr (SPLIT) return;   // split output: stacked tasks only
+    else {
 #pragma unroll
     for (int j = 0; j < WN; ++j) {
         const uint32_t gn = n0 + wn + 16 * j + l16;
@@ -416,6 +437,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                     Cn[gm + tk.m_off] = v;
                 }
             }
+    }
     }
 }
 
@@ -513,8 +535,10 @@ static hipError_t launch_typed(hipStream_t st, int cfg, const GemmTask* tasks, u
                                bases, tb, tile_task);
             break;
         case GEMM_CFG_16x16:
-            hipLaunchKernelGGL((gemm_tiny_kernel<TA, TB, TC, MODE>), grid, block, 0, st, tasks, n_tasks, segs, bases,
-                               tb, tile_task);
+            if constexpr (std::is_same<TC, SplitBf16>::value) return hipErrorInvalidValue;
+            else
+                hipLaunchKernelGGL((gemm_tiny_kernel<TA, TB, TC, MODE>), grid, block, 0, st, tasks, n_tasks, segs,
+                                   bases, tb, tile_task);
             break;
         default:
             return hipErrorInvalidValue;
@@ -527,14 +551,16 @@ hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, con
     if (n_tiles == 0) return hipSuccess;
     if (mode == MODE_F64)
         return launch_typed<double, double, double, MODE_F64>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
-    const bool da = types[0] == 'd', db = types[1] == 'd', dc = types[2] == 'd';
+    const bool da = types[0] == 'd', db = types[1] == 'd', dc = types[2] == 'd', fc = types[2] == 'f';
     if (mode == MODE_F32) {
-        if (da && db && !dc)
+        if (da && db && fc)
             return launch_typed<double, double, float, MODE_F32>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
-        if (!da && !db && !dc)
+        if (!da && !db && fc)
             return launch_typed<float, float, float, MODE_F32>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
         if (da && !db && dc)
             return launch_typed<double, float, double, MODE_F32>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
+        if (da && db && types[2] == 's' && cfg != GEMM_CFG_16x16)
+            return launch_typed<double, double, SplitBf16, MODE_F32>(st, cfg, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
     }
     return hipErrorInvalidValue;
 }

@@ -127,13 +127,6 @@ hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v) {
     return hipGetLastError();
 }
 
-// x = hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even (finite inputs): the
-// operands of the bf16 products of the BF16x3 GEMMs.
-__device__ inline uint16_t bf16_rne(float x) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
 // dst = [hi | lo | hi] of x (three consecutive n-element runs): the K-concatenated left operand of
 // the one-GEMM form of the BF16x3 product [A_hi A_lo A_hi] [B_lo; B_hi; B_hi].
 __global__ void split3_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict__ dst, uint64_t n) {
@@ -149,6 +142,27 @@ __global__ void split3_bf16_kernel(const float* __restrict__ x, uint16_t* __rest
 hipError_t launch_split3_bf16(hipStream_t st, const float* x, uint16_t* dst, uint64_t n) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(split3_bf16_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, x, dst, n);
+    return hipGetLastError();
+}
+
+template <typename E>
+__global__ void zero_chunks_kernel(const ZeroChunk* __restrict__ chunks, E* __restrict__ base) {
+    const ZeroChunk c = chunks[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < c.len; i += blockDim.x) base[c.off + i] = E(0);
+}
+hipError_t launch_zero_chunks(hipStream_t st, const ZeroChunk* chunks, uint32_t n_chunks, void* base, int elem_bytes) {
+    if (n_chunks == 0) return hipSuccess;
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(zero_chunks_kernel<uint64_t>, dim3(n_chunks), dim3(256), 0, st, chunks,
+                           static_cast<uint64_t*>(base));
+    else if (elem_bytes == 4)
+        hipLaunchKernelGGL(zero_chunks_kernel<uint32_t>, dim3(n_chunks), dim3(256), 0, st, chunks,
+                           static_cast<uint32_t*>(base));
+    else if (elem_bytes == 2)
+        hipLaunchKernelGGL(zero_chunks_kernel<uint16_t>, dim3(n_chunks), dim3(256), 0, st, chunks,
+                           static_cast<uint16_t*>(base));
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -61,10 +61,32 @@ struct GemmTask {
 // (R.cmap == MAP_NONE: C[c_off + R.c_src + n * R.ldc + i]); padding rows are not stored.  The
 // task's own cmap, ldc and m_off are unused.  Tiles of a stacked task are numbered row tile
 // fastest: the row tiles that read one B panel run next to each other and share it in L2.
+// Split output (C type SplitBf16, stacked tasks only): the element goes to three bf16 copies,
+//     C[c_off + R.c_split + o] = hi,  C[... + o + R.split_stride] = lo,  C[... + o + 2 R.split_stride] = hi
+// with o the offset above without c_src, hi = bf16(x), lo = bf16(x - hi).
 struct GemmRow {
-    uint64_t a_src, c_src;
+    uint64_t a_src, c_src, c_split, split_stride;
     uint32_t lda, row0, cldk, ldc, cmap, nrows;
 };
+
+// C storage tag of the split output (three bf16 runs [hi | lo | hi]: the K-concatenated left
+// operand of the one-GEMM BF16x3 product)
+struct SplitBf16 {};
+
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi), round to nearest even (finite inputs)
+__device__ inline uint16_t bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+// Zeroing of scattered ranges: chunk c clears elements [off, off + len) of the base array
+// (len <= ZERO_CHUNK).
+struct ZeroChunk {
+    uint64_t off;
+    uint32_t len, pad;
+};
+constexpr uint32_t ZERO_CHUNK = 4096;
 
 //  row-block remap of a transposed A (trans_a only): column k of op(A) lies in block j = k / bbh,
 //      op(A)(m, k) = alpha * scales[ascale + j] * A[a_off + maps[amap + j] + (k % bbh) + m * lda]
@@ -122,8 +144,8 @@ struct GemmTables {
 };
 
 // One launch of all tasks of one tile configuration.  mode MODE_F64: all operands double.
-// Otherwise `types` gives the storage of A, B and C as 'd' (double) or 'f' (float): "ddf",
-// "fff" or "dfd".
+// Otherwise `types` gives the storage of A, B and C as 'd' (double), 'f' (float) or 's' (split
+// bf16 output of stacked tasks): "ddf", "fff", "dfd" or "dds".
 // tile_task (may be null): task index of every tile of the launch.
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
                        const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tables,
@@ -140,6 +162,8 @@ hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n
                          const uint64_t* src_off, const double* alpha, uint32_t n_chunks, double* ws,
                          double* dst_base);
 hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v);
+// clear every chunk's range of base, elements of elem_bytes (2, 4 or 8) bytes
+hipError_t launch_zero_chunks(hipStream_t st, const ZeroChunk* chunks, uint32_t n_chunks, void* base, int elem_bytes);
 // dst[0, n) = hi, dst[n, 2n) = lo, dst[2n, 3n) = hi of x, with hi = bf16(x), lo = bf16(x - hi)
 // (round to nearest even)
 hipError_t launch_split3_bf16(hipStream_t st, const float* x, uint16_t* dst, uint64_t n);

@@ -123,9 +123,7 @@ def test_autotuned_plan_matches_oracle(eng):
             plan.set_graphs(False)
             before = torch.full_like(psi, np.nan)
             plan.apply(psi, before, precision=prec)
-            n_timed = plan.autotune(prec)
-            # BF16x3 runs its T and final stages as FP32 does: the FP32 decision is reused
-            assert n_timed > 0 or prec == E.PREC_BF16X3
+            assert plan.autotune(prec) > 0
             after = torch.full_like(psi, np.nan)
             plan.apply(psi, after, precision=prec)
             eng.sync()
@@ -274,6 +272,27 @@ def test_mixed_sigma_error(eng, model, L, site, M, seed, precision):
     plan.apply(psi, out)
     eng.sync()
     assert np.linalg.norm(out.cpu().numpy() - ref) <= SIGMA_RTOL * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("precision", [E.PREC_F32, E.PREC_BF16X3])
+def test_reduced_precision_first_use_after_fp64(eng, precision):
+    """A plan's FP32 workspace is allocated at its first reduced-precision application; device
+    memory released before that (here filled with NaN) must not leak into sigma."""
+    poison = torch.full((1 << 25,), np.nan, dtype=torch.float64, device="cuda:0")
+    del poison
+    torch.cuda.empty_cache()
+    p = SiteProblem("qc", L=8, site=0, Mmax=64, seed=7)
+    ref = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb).apply(p.psi)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    psi = dev(p.psi)
+    out = torch.zeros_like(psi)
+    plan.apply(psi, out)
+    plan.apply(psi, out, precision=precision)
+    eng.sync()
+    got = out.cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert np.isfinite(got).all() and err <= MIXED_RTOL[precision], err
 
 
 def test_mixed_precision_rejected_on_schedule_plan(eng):
