@@ -248,6 +248,9 @@ int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sche
  * the MPO-term sum as one dense MFMA GEMM per group; same result, no per-sector alpha lists.
  * AUTO picks GROUPED for abelian groups and SCHEDULE for SU2U1. */
 enum mq_sigma_algorithm { MQ_SIGMA_AUTO = 0, MQ_SIGMA_SCHEDULE = 1, MQ_SIGMA_GROUPED = 2 };
+/* OR-ed into the algorithm of mq_sigma_plan_create_ex: keep what mq_sigma_adjoint needs (the map
+ * from MPO terms to W_D entries and the adjoint task tables).  GROUPED plans only. */
+enum { MQ_SIGMA_ADJOINT = 0x100 };
 int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
                             const mq_boundary* left, const mq_boundary* right, int algorithm, mq_plan** out);
 /* sigma = H_eff * psi; psi and sigma are device pointers in the padded Davidson-vector layout. */
@@ -269,6 +272,15 @@ int mq_sigma_apply_ex(mq_plan* plan, const double* psi, double* sigma, int preci
  * caller's operands and drops the graphs.  No reference counterpart (the reference issues cuBLAS
  * calls per block, solver/tasks.cpp). */
 int mq_plan_set_graphs(mq_plan* plan, int enable);
+/* Adjoint of the sigma contraction with respect to the MPO coefficients (the gradient with respect
+ * to the reference's alpha list, tasks.cpp:466-494 and op_iterate.hpp:58-63, taken per MPO term):
+ *   grad[t] = d(psi . H_eff psi) / d term_scale[t] = psi . (L_b1 (x) op_t (x) R_b2) psi
+ * for every term t of the plan's mq_mpo_desc, in its order.  For a normalized eigenvector of H_eff
+ * this is dE/d scale_t (Hellmann-Feynman); summed over the sites where an integral enters the MPO
+ * it gives dE/dh_ij and dE/d(ij|kl), the 1- and 2-RDM elements.  psi: device pointer (padded
+ * layout); grad: host array of n_terms doubles.  FP64.  The plan must have been created with
+ * MQ_SIGMA_ADJOINT. */
+int mq_sigma_adjoint(mq_plan* plan, const double* psi, double* grad);
 /* Autotune the GEMM tile configurations of the plan's T and final stages for `precision`.  Tasks
  * fall into row-size classes (M <= 16, 32, 64, 128, larger); for each class present, candidate tile
  * shapes are timed on the plan's own stage (one warm-up and three repetitions) and the fastest is

@@ -312,6 +312,10 @@ struct StagePlan {
     bool split_t = false;       // all T tasks stacked: BF16x3 T stage writes A' directly
     uint64_t bf_a_elems = 0;    // A' part of the bf16 buffer
     GemmStage gT, gF;   // device form of the T and final GEMM stages
+    // adjoint (plans created with MQ_SIGMA_ADJOINT): U_D = L^T psi in the S_D layout, as stacked tasks
+    std::vector<GemmTask> u_tasks;
+    std::vector<GemmSeg> u_segs;
+    GemmStage gU;
     // grouped path: split-K partial sums of the final stage (FP64, base BASE_BRA) and their
     // reduction into sigma
     GatherSet red;
@@ -339,6 +343,7 @@ struct StagePlan {
     void finish_tables() {
         gT.build(t_tasks);
         gF.build(f_tasks);
+        gU.build(u_tasks);
         red.finish();
         std::vector<GatherRegion> big;
         for (const auto& r : regions) {
@@ -373,7 +378,7 @@ struct StagePlan {
     }
     // A stacked task: its rows are partitioned into contiguous row units; every unit's A block and
     // C blocks lie inside their buffers; one plain (non-transposed, unmapped) A segment.
-    void check_stacked(const GemmTask& t, const std::vector<GemmSeg>& segs) const {
+    void check_stacked(const GemmTask& t, const std::vector<GemmSeg>& segs, bool split) const {
         MQ_REQUIRE(uint64_t(t.rowunit) + t.M <= maps.size(), "row unit table range");
         MQ_REQUIRE(t.seg0 + 1 == t.seg1 && t.seg1 <= segs.size(), "stacked task with other than one segment");
         MQ_REQUIRE(t.colunit == MAP_NONE && t.cbw > 0, "stacked task shape");
@@ -392,12 +397,14 @@ struct StagePlan {
             while (e < t.M && maps[t.rowunit + e] == u) ++e;
             MQ_REQUIRE((e - m) % 16 == 0 && R.nrows <= e - m && e - m < R.nrows + 16, "row unit span");
             const uint32_t nr = R.nrows;
-            MQ_REQUIRE(R.lda >= nr && R.ldc >= nr, "row unit leading dimensions");
-            check_in(g.a_base, g.a_off + R.a_src, extent(nr, g.K, R.lda), "stacked A block");
+            MQ_REQUIRE(R.ldc >= nr, "row unit leading dimensions");
+            if (nr && g.K)
+                check_in(g.a_base, g.a_off + R.a_src, uint64_t(nr - 1) * R.a_si + uint64_t(g.K - 1) * R.a_sk + 1,
+                         "stacked A block");
             // C blocks: [off, off + ext) of the workspace, and of the A' copies when split
             auto c_block = [&](uint64_t rel, uint64_t ext) {
                 check_in(t.c_base, t.c_off + R.c_src + rel, ext, "stacked C block");
-                if (split_t && ext && t.c_off + R.c_split + rel + 2 * R.split_stride + ext > bf_a_elems)
+                if (split && ext && t.c_off + R.c_split + rel + 2 * R.split_stride + ext > bf_a_elems)
                     fail(MQ_ERR_INTERNAL, "stacked split output out of bounds");
             };
             if (R.cmap == MAP_NONE) {
@@ -412,10 +419,10 @@ struct StagePlan {
         }
     }
     void validate() const {
-        auto gemm = [&](const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs) {
+        auto gemm = [&](const std::vector<GemmTask>& tasks, const std::vector<GemmSeg>& segs, bool split) {
             for (const auto& t : tasks) {
                 if (t.rowunit != MAP_NONE) {
-                    check_stacked(t, segs);
+                    check_stacked(t, segs, split);
                     continue;
                 }
                 MQ_REQUIRE(t.ldc >= t.M || t.N <= 1, "ldc < M");
@@ -488,8 +495,9 @@ struct StagePlan {
                 }
             }
         };
-        gemm(t_tasks, t_segs);
-        gemm(f_tasks, f_segs);
+        gemm(t_tasks, t_segs, split_t);
+        gemm(f_tasks, f_segs, false);
+        gemm(u_tasks, u_segs, false);
         if (split_t)
             for (const auto& t : t_tasks)
                 MQ_REQUIRE(t.rowunit != MAP_NONE && t.c_base == BASE_WS && !t.beta, "split T stage needs stacked tasks");
@@ -520,6 +528,7 @@ struct StagePlan {
     void upload_all(hipStream_t st) {
         gT.upload_all(t_segs, st);
         gF.upload_all(f_segs, st);
+        gU.upload_all(u_segs, st);
         upload(d_regions, regions, st);
         upload(d_small, small, st);
         upload(d_src_off, src_off, st);
@@ -573,6 +582,11 @@ struct StagePlan {
     }
     void run(hipStream_t st, Bases b, int mode = MODE_F64) const {
         for (int k = 0; k < 4; ++k) stage(k, st, b, mode);
+    }
+    // U stage of the adjoint (FP64)
+    void run_u(hipStream_t st, Bases b) const {
+        const GemmTables tb{d_maps.as<uint32_t>(), d_units.as<GemmUnit>(), d_rows.as<GemmRow>(), d_scales.as<double>()};
+        gU.run(st, MODE_F64, "ddd", b, tb);
     }
     void run_lib(hipStream_t st, Bases b, int mode) const {
         if (lib.empty()) return;
@@ -660,6 +674,15 @@ struct mq_plan {
     hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};   // per precision
     bool seen[3] = {false, false, false};                     // precision applied once eagerly
     DevBuf io;                                                // operand slots of the graphs: in, out
+    // adjoint (MQ_SIGMA_ADJOINT): per MPO term t, the entries [adj_ptr[t], adj_ptr[t+1]) of
+    // (offset in the W_D region, coefficient) through which the term's scale enters W_D
+    bool adjoint = false;
+    uint32_t n_terms = 0;
+    std::vector<uint32_t> adj_ptr;
+    std::vector<uint64_t> adj_idx;
+    std::vector<double> adj_coef;
+    uint64_t s_region0 = 0, s_region1 = 0;   // the S_D groups' workspace range
+    DevBuf adj_g, d_adj_ptr, d_adj_idx, d_adj_coef, d_grad;
 
     void drop_graphs() {
         for (auto& g : graphs)
@@ -1310,7 +1333,7 @@ struct GroupedLowering {
 };
 
 void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::vector<double>& ws_init,
-                         uint64_t& ws_init_off) {
+                         uint64_t& ws_init_off, bool adjoint) {
     const Index &left_i = site.left_i, &right_i = site.right_i, &phys_i = site.phys_i;
     const BoundaryIndex& LI = p->left->index;
     const BoundaryIndex& RI = p->right->index;
@@ -1424,16 +1447,20 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         p->st.bf_elems += align32(3ull * g.M * g.K.size());
     }
     p->st.bf_a_elems = p->st.bf_elems;
+    p->s_region0 = ws;
     for (auto& g : G.groups) {
         g.s_off = ws;
         ws += align32(g.M * g.N.size());
     }
+    p->s_region1 = ws;
     ws_init_off = ws;
     for (auto& g : G.groups) {
         g.w_off = ws;
         ws += align32(uint64_t(g.K.size()) * g.N.size());
     }
     ws_init.assign(ws - ws_init_off, 0.0);
+    struct AdjEntry { uint32_t t; uint64_t w; double c; };
+    std::vector<AdjEntry> adj;
     // ---- dense W_D: for every MPO entry, its (k, n) in the one group holding both keys
     std::unordered_map<uint64_t, std::vector<std::pair<uint32_t, uint32_t>>> kgroups, ngroups;
     for (uint32_t gi = 0; gi < G.groups.size(); ++gi) {
@@ -1457,8 +1484,9 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                             for (auto& ng : nit->second)
                                 if (kg.first == ng.first) {
                                     const auto& g = G.groups[kg.first];
-                                    ws_init[g.w_off - ws_init_off + uint64_t(ng.second) * g.K.size() + kg.second] +=
-                                        op.coeff[e] * sc;
+                                    const uint64_t w = g.w_off - ws_init_off + uint64_t(ng.second) * g.K.size() + kg.second;
+                                    ws_init[w] += op.coeff[e] * sc;
+                                    if (adjoint) adj.push_back({t, w, op.coeff[e]});
                                 }
                     }
                 }
@@ -1525,7 +1553,9 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             MQ_REQUIRE(g.M < (1ull << 32), "group too large for a row unit");
             GemmRow R{};
             R.a_src = site.rp_offset[lb] + (rpb(rbk.s, st.rbi) + uint64_t(rbk.ss) * st.K) * M;
-            R.lda = M;
+            R.a_si = 1;
+            R.a_sk = M;
+            R.a_scale = 1.0;
             R.row0 = uint32_t(unit_of_row.size());
             R.nrows = M;
             R.c_src = g.t_off + pr.m_off;
@@ -1752,6 +1782,82 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         r.t1 = uint32_t(P.red.src_off.size());
         P.red.regions.push_back(r);
     }
+    if (adjoint) {
+        // ---- term map: CSR over the MPO terms, entries in their accumulation order
+        std::stable_sort(adj.begin(), adj.end(), [](const AdjEntry& a, const AdjEntry& b) { return a.t < b.t; });
+        p->adjoint = true;
+        p->n_terms = uint32_t(W.terms.size());
+        p->adj_ptr.assign(p->n_terms + 1, 0);
+        for (const auto& a : adj) ++p->adj_ptr[a.t + 1];
+        for (uint32_t t = 0; t < p->n_terms; ++t) p->adj_ptr[t + 1] += p->adj_ptr[t];
+        p->adj_idx.resize(adj.size());
+        p->adj_coef.resize(adj.size());
+        for (size_t i = 0; i < adj.size(); ++i) p->adj_idx[i] = adj[i].w, p->adj_coef[i] = adj[i].c;
+        // ---- U stage: for every output block lb_out and unit u = (s, ss2) of its columns, the
+        // slabs U(pair(lb_in, rb(s)), key(b1, s, ss2)) = L_b1^T psi(lb_out)[:, unit u] of all left
+        // cohorts (lc_out, lc_in) and their blocks b1, stacked as row units (the adjoint of the
+        // final stage: psi . sigma = sum_g <U_g, T_g W_g>)
+        for (uint32_t lbo = 0; lbo < left_i.size(); ++lbo) {
+            Charge lc_out = left_i.charges[lbo];
+            const uint32_t ls_out = left_i.sizes[lbo];
+            if (!ls_out || !site.rp_cols[lbo]) continue;
+            for (uint32_t s = 0; s < phys_i.size(); ++s) {
+                Charge rc_out = lc_out + phys_i.charges[s];
+                const uint32_t rb = right_i.position(rc_out);
+                if (rb == right_i.size()) continue;
+                const uint32_t rs = right_i.sizes[rb];
+                if (!rs) continue;
+                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
+                    const uint32_t col0 = uint32_t(rpb(s, rb)) + ss * rs;
+                    std::vector<uint32_t> unit_of_row;
+                    for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
+                        const uint32_t cl = LI.cohort_index(lc_out, left_i.charges[lb]);
+                        const uint32_t ls_in = left_i.sizes[lb];
+                        if (cl >= LI.n_cohorts() || !ls_in) continue;
+                        auto pit = G.pair_of.find(uint64_t(lb) << 32 | rb);
+                        if (pit == G.pair_of.end()) continue;
+                        const auto& pr = G.pairs[pit->second];
+                        const auto& g = G.groups[pr.g];
+                        auto add_row = [&](uint32_t b1, uint64_t a_src, uint32_t a_si, uint32_t a_sk, double scale) {
+                            GemmRow R{};
+                            R.a_src = a_src;
+                            R.a_si = a_si;
+                            R.a_sk = a_sk;
+                            R.a_scale = scale;
+                            R.row0 = uint32_t(unit_of_row.size());
+                            R.nrows = ls_in;
+                            R.c_src = g.s_off + pr.m_off + uint64_t(g.nidx.at(GroupedLowering::kkey(b1, s, ss))) * g.M;
+                            R.ldc = ls_in;
+                            R.cmap = MAP_NONE;
+                            unit_of_row.insert(unit_of_row.end(), (ls_in + 15) / 16 * 16, uint32_t(P.rows.size()));
+                            P.rows.push_back(R);
+                        };
+                        if (!LI.tr[cl]) {
+                            std::vector<uint32_t> jb = block_order(LI, cl);
+                            for (uint32_t j = 0; j < jb.size(); ++j)
+                                add_row(jb[j], p->left->cohort_off[cl] + uint64_t(j) * ls_out * ls_in, ls_out, 1, 1.0);
+                        } else {
+                            const uint32_t cle = LI.cohort_index(left_i.charges[lb], lc_out);
+                            MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
+                            for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1) {
+                                if (!LI.has_block(cl, b1)) continue;
+                                add_row(b1, p->left->cohort_off[cle] + uint64_t(LI.offset(cl, b1)), 1, ls_in,
+                                        LI.conjugate_scale(cl, b1));
+                            }
+                        }
+                    }
+                    if (unit_of_row.empty()) continue;
+                    const uint32_t Mt = uint32_t(unit_of_row.size());
+                    GemmTask& t = add_gemm(P.u_tasks, P.u_segs, BASE_WS, 0, Mt, rs, Mt,
+                                           {seg(BASE_LEFT, 0, 1, false, BASE_IN,
+                                                site.rp_offset[lbo] + uint64_t(col0) * ls_out, ls_out, false, ls_out)});
+                    t.cbw = rs;
+                    t.rowunit = uint32_t(P.maps.size());
+                    P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+                }
+            }
+        }
+    }
     P.ws_elems = ws;
     p->flops = G.ref_flops;
     // T and L reference flops
@@ -1822,12 +1928,16 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         MQ_REQUIRE(ctx && site && mpo && left && right && out, "bad argument");
         SiteLayout L(*site);
         if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
+        if (algorithm == (MQ_SIGMA_AUTO | MQ_SIGMA_ADJOINT)) algorithm = MQ_SIGMA_GROUPED | MQ_SIGMA_ADJOINT;
         if (algorithm == MQ_SIGMA_SCHEDULE) {
             int rc = mq_sigma_plan_create(ctx, site, mpo, left, right, out);
             if (rc != MQ_OK) fail(rc, g_err);
             return;
         }
-        MQ_REQUIRE(algorithm == MQ_SIGMA_GROUPED, "unknown sigma algorithm");
+        const bool adjoint = (algorithm & MQ_SIGMA_ADJOINT) != 0;
+        algorithm &= ~int(MQ_SIGMA_ADJOINT);
+        MQ_REQUIRE(algorithm == MQ_SIGMA_GROUPED || (algorithm == MQ_SIGMA_AUTO && !L.symm.su2()),
+                   adjoint ? "the sigma adjoint needs a GROUPED plan (abelian symmetry)" : "unknown sigma algorithm");
         Mpo W(*mpo);
         auto p = std::make_unique<mq_plan>();
         p->ctx = ctx;
@@ -1836,7 +1946,7 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         p->vec_size = L.rp_total;
         std::vector<double> ws_init;
         uint64_t ws_init_off = 0;
-        lower_sigma_grouped(p.get(), L, W, ws_init, ws_init_off);
+        lower_sigma_grouped(p.get(), L, W, ws_init, ws_init_off, adjoint);
         StagePlan& P = p->st;
         P.blas = ctx->blas;
         P.finish_tables();
@@ -2096,6 +2206,43 @@ int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
             }
         }
         if (timed) *timed = n_timed;
+    });
+}
+
+int mq_sigma_adjoint(mq_plan* p, const double* psi, double* grad) {
+    return guarded([&] {
+        MQ_REQUIRE(p && psi && grad, "bad argument");
+        MQ_REQUIRE(p->adjoint, "plan created without MQ_SIGMA_ADJOINT");
+        mq_ctx* ctx = p->ctx;
+        hipStream_t st = ctx->stream;
+        StagePlan& P = p->st;
+        if (!p->d_grad.p) {
+            upload(p->d_adj_ptr, p->adj_ptr, st);
+            upload(p->d_adj_idx, p->adj_idx, st);
+            upload(p->d_adj_coef, p->adj_coef, st);
+            p->d_grad.alloc(std::max<uint32_t>(p->n_terms, 1) * sizeof(double));
+            p->adj_g.alloc_zeroed(std::max<uint64_t>(P.ws_elems - p->ws_init_off, 1) * sizeof(double), st);
+        }
+        Bases b = p->bases(psi, nullptr, MODE_F64);
+        double* ws = b.p[BASE_WS];
+        // T_D = psi R (FP64), then U_D = L^T psi in the S_D region (cleared first: keys without a
+        // left block stay zero)
+        P.stage(0, st, b, MODE_F64);
+        P.stage(1, st, b, MODE_F64);
+        HIPCHK(launch_fill(st, ws + p->s_region0, p->s_region1 - p->s_region0, 0.0));
+        P.run_u(st, b);
+        // G_g = T_g^T U_g in the W_D layout: the derivative of psi . sigma with respect to W_D
+        BLASCHK(rocblas_set_stream(P.blas, st));
+        const double one = 1.0, zero = 0.0;
+        double* G = p->adj_g.as<double>();
+        for (const auto& g : P.lib)
+            BLASCHK(rocblas_dgemm(P.blas, rocblas_operation_transpose, rocblas_operation_none, g.K, g.N, g.M, &one,
+                                  ws + g.a_off, g.lda, ws + g.c_off, g.ldc, &zero, G + (g.b_off - p->ws_init_off), g.K));
+        // per term: the coefficients of its op entries against G
+        HIPCHK(launch_term_grad(st, p->n_terms, p->d_adj_ptr.as<uint32_t>(), p->d_adj_idx.as<uint64_t>(),
+                                p->d_adj_coef.as<double>(), G, p->d_grad.as<double>()));
+        HIPCHK(hipMemcpyAsync(grad, p->d_grad.p, p->n_terms * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
     });
 }
 

@@ -158,13 +158,15 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     // non-transposed A) and its unit's column stride
     uint64_t a_row = 0;
     uint32_t a_ld = 0, a_ok = 0;
+    double a_rscale = 1.0;
     if (stacked) {
         const uint32_t m = m0 + uint32_t(tid % BM);
         const GemmRow R = rows[maps[tk.rowunit + min(m, M - 1)]];
         const uint32_t i = m - R.row0;
         a_ok = uint32_t(m < M) & uint32_t(i < R.nrows);
-        a_row = R.a_src + i;
-        a_ld = R.lda;
+        a_row = R.a_src + uint64_t(i) * R.a_si;
+        a_ld = R.a_sk;
+        a_rscale = R.a_scale;
     }
     // cursor over the K steps of all segments: (sgi, k0) with k0 < segs[sgi].K
     uint32_t sgi = tk.seg0, k0 = 0;
@@ -205,6 +207,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
         }
         a_scale = 1.0;
         if (stacked) {
+            a_scale = a_rscale;
 #pragma unroll
             for (int i = 0; i < NA; ++i) {
                 const int e = tid + GEMM_THREADS * i;

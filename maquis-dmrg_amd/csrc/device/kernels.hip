@@ -166,6 +166,22 @@ hipError_t launch_zero_chunks(hipStream_t st, const ZeroChunk* chunks, uint32_t 
     return hipGetLastError();
 }
 
+__global__ void term_grad_kernel(uint32_t n_terms, const uint32_t* __restrict__ ptr, const uint64_t* __restrict__ idx,
+                                 const double* __restrict__ coef, const double* __restrict__ g, double* __restrict__ grad) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_terms) return;
+    double acc = 0.0;
+    for (uint32_t e = ptr[t]; e < ptr[t + 1]; ++e) acc += coef[e] * g[idx[e]];
+    grad[t] = acc;
+}
+hipError_t launch_term_grad(hipStream_t st, uint32_t n_terms, const uint32_t* ptr, const uint64_t* idx,
+                            const double* coef, const double* g, double* grad) {
+    if (n_terms == 0) return hipSuccess;
+    hipLaunchKernelGGL(term_grad_kernel, dim3((n_terms + 255) / 256), dim3(256), 0, st, n_terms, ptr, idx, coef, g,
+                       grad);
+    return hipGetLastError();
+}
+
 __global__ void axpy_kernel(uint64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         y[i] += a * x[i];

@@ -56,7 +56,7 @@ struct GemmTask {
 // Unit u occupies rows [R.row0, R.row0 + span) of the task, span = nrows rounded up to a multiple
 // of 16 (row0 too is a multiple of 16, so every 16-row MFMA block lies in one unit); row m belongs
 // to unit u = maps[rowunit + m], with descriptor R = row table[u] and local row i = m - R.row0:
-//     A(m, k)  = A[a_off + R.a_src + i + k * R.lda]    i < R.nrows, else 0  (non-transposed A only)
+//     A(m, k)  = R.a_scale * A[a_off + R.a_src + i * R.a_si + k * R.a_sk]   i < R.nrows, else 0
 //     C(m, n)  = C[c_off + R.c_src + maps[R.cmap + n / cbw] * R.cldk + (n % cbw) * R.ldc + i]
 // (R.cmap == MAP_NONE: C[c_off + R.c_src + n * R.ldc + i]); padding rows are not stored.  The
 // task's own cmap, ldc and m_off are unused.  Tiles of a stacked task are numbered row tile
@@ -66,7 +66,8 @@ struct GemmTask {
 // with o the offset above without c_src, hi = bf16(x), lo = bf16(x - hi).
 struct GemmRow {
     uint64_t a_src, c_src, c_split, split_stride;
-    uint32_t lda, row0, cldk, ldc, cmap, nrows;
+    double a_scale;
+    uint32_t a_si, a_sk, row0, cldk, ldc, cmap, nrows, pad;
 };
 
 // C storage tag of the split output (three bf16 runs [hi | lo | hi]: the K-concatenated left
@@ -162,6 +163,9 @@ hipError_t launch_gather(hipStream_t st, const GatherRegion* regions, uint32_t n
                          const uint64_t* src_off, const double* alpha, uint32_t n_chunks, double* ws,
                          double* dst_base);
 hipError_t launch_fill(hipStream_t st, double* x, uint64_t n, double v);
+// grad[t] = sum_{e in [ptr[t], ptr[t+1])} coef[e] * g[idx[e]]   (one thread per term, fixed order)
+hipError_t launch_term_grad(hipStream_t st, uint32_t n_terms, const uint32_t* ptr, const uint64_t* idx,
+                            const double* coef, const double* g, double* grad);
 // clear every chunk's range of base, elements of elem_bytes (2, 4 or 8) bytes
 hipError_t launch_zero_chunks(hipStream_t st, const ZeroChunk* chunks, uint32_t n_chunks, void* base, int elem_bytes);
 // dst[0, n) = hi, dst[n, 2n) = lo, dst[2n, 3n) = hi of x, with hi = bf16(x), lo = bf16(x - hi)

@@ -132,6 +132,7 @@ SIGNATURES = {
     "mq_sigma_apply_ex": (C.c_int, [P, P, P, C.c_int]),
     "mq_plan_set_graphs": (C.c_int, [P, C.c_int]),
     "mq_plan_autotune": (C.c_int, [P, C.c_int, C.POINTER(C.c_int)]),
+    "mq_sigma_adjoint": (C.c_int, [P, P, P]),
     "mq_sigma_apply_batched": (C.c_int, [P, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                          C.POINTER(C.c_void_p), C.c_int]),
     "mq_plan_profile": (C.c_int, [P, P, P, C.c_int, C.c_int, C.POINTER(C.c_double), u64p, u64p]),

@@ -132,6 +132,52 @@ class TwoSiteDMRG:
         nrm = torch.linalg.norm(s0.data)
         s0.data /= nrm
 
+    def move_center_right(self, j):
+        """QR of site j (left-paired, per right sector): site j becomes left-canonical and R moves
+        into site j+1 (mps.hpp canonize_left_step)."""
+        A, B = self.sites[j], self.sites[j + 1]
+        lp = A.lp_data()
+        dims, Qs, Rs = {}, {}, {}
+        for rb, rc in enumerate(A.right.charges):
+            blk = A.lp_block(lp, rb)
+            if blk.numel() == 0:
+                continue
+            q, r = torch.linalg.qr(blk.contiguous(), mode="reduced")
+            dims[rc], Qs[rc], Rs[rc] = q.shape[1], q, r
+        newright = Index(list(dims.keys()), list(dims.values()))
+        self.sites[j] = SiteTensor.from_lp(self.eng, self.sym, A.left, A.phys, newright,
+                                           {newright.position(c): Qs[c] for c in dims})
+        blocks = {}
+        for lb, lc in enumerate(B.left.charges):
+            if lc in dims:
+                blocks[newright.position(lc)] = Rs[lc] @ B.rp_block(lb)
+        self.sites[j + 1] = SiteTensor.from_rp(self.eng, self.sym, newright, B.phys, B.right, blocks)
+
+    def mpo_gradients(self):
+        """Derivatives of the energy with respect to every MPO term scale of every site, at the
+        current state (after run(): orthogonality center at site 0): the center walks left to
+        right by QR, and at each site j the one-site adjoint (SigmaPlan.adjoint) of
+        psi_j . H_eff psi_j / psi_j . psi_j with the boundaries L_j, R_{j+1} gives dE/d scale for
+        the terms of mpo[j] (Hellmann-Feynman at the variational optimum).  Returns (per-site
+        gradient arrays in mpo[j].sorted_terms() order, per-site energies psi_j . H psi_j)."""
+        grads, energies = [], []
+        for j in range(self.L):
+            s = self.sites[j]
+            nrm2 = float(torch.dot(s.data, s.data))
+            plan = E.SigmaPlan(self.eng, s.desc, self.mpo_desc[j], self.Lb[j], self.Rb[j + 1],
+                               algorithm=E.SIGMA_GROUPED, adjoint=True)
+            grads.append(plan.adjoint(s.data) / nrm2)
+            out = torch.zeros_like(s.data)
+            plan.apply(s.data, out)
+            self.eng.sync()
+            energies.append(float(torch.dot(s.data, out)) / nrm2)
+            plan.close()
+            if j < self.L - 1:
+                self.move_center_right(j)
+                a = self.sites[j]
+                self.Lb[j + 1] = E.left_step(self.eng, a.desc, self.mpo_desc[j], a.lp_data(), a.data, self.Lb[j])
+        return grads, energies
+
     # ------------------------------------------------------------------ two-site tensors
     def twosite_mpo(self, i):
         if i not in self.ts_mpo:

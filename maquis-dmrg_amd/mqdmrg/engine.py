@@ -137,6 +137,19 @@ class MpoDesc:
         d.term_op = capi.ptr(a[8], C.c_uint32)
         d.term_scale = capi.ptr(a[9], C.c_double)
         self.desc = d
+        self.term_b1, self.term_b2, self.term_op, self.term_scale = a[6], a[7], a[8], a[9]
+
+    def with_scales(self, scales):
+        """The same MPO with other term scales (same order): a view sharing the op tables."""
+        new = MpoDesc.__new__(MpoDesc)
+        sc = np.ascontiguousarray(scales, np.float64)
+        assert sc.shape == self.term_scale.shape
+        new.keep = self.keep + [sc]
+        new.term_b1, new.term_b2, new.term_op, new.term_scale = self.term_b1, self.term_b2, self.term_op, sc
+        d = capi.mq_mpo_desc.from_buffer_copy(self.desc)
+        d.term_scale = capi.ptr(sc, C.c_double)
+        new.desc = d
+        return new
 
 
 class SiteDesc:
@@ -252,20 +265,24 @@ def _schedule_result(kind, h):
 
 # ---------------------------------------------------------------------------- sigma
 SIGMA_AUTO, SIGMA_SCHEDULE, SIGMA_GROUPED = 0, 1, 2
+SIGMA_ADJOINT = 0x100   # OR-ed into the algorithm: the plan also supports SigmaPlan.adjoint
 # arithmetic of the sigma contraction (mq_precision)
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
 PRECISIONS = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
 
 
 class SigmaPlan:
-    def __init__(self, engine, site, mpo_desc, left, right, algorithm=SIGMA_AUTO):
+    def __init__(self, engine, site, mpo_desc, left, right, algorithm=SIGMA_AUTO, adjoint=False):
+        """adjoint: keep the MPO term map for SigmaPlan.adjoint (GROUPED plans)."""
         self.engine = engine
         self.h = None
         engine.adopt(self)
         self.left, self.right = left, right   # keep boundaries alive
+        self.n_terms = mpo_desc.desc.n_terms
         self.h = C.c_void_p()
+        alg = int(algorithm) | (SIGMA_ADJOINT if adjoint else 0)
         check(lib().mq_sigma_plan_create_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), left.h, right.h,
-                                            int(algorithm), C.byref(self.h)))
+                                            alg, C.byref(self.h)))
         n = C.c_uint64()
         check(lib().mq_plan_vector_size(self.h, C.byref(n)))
         self.n = n.value
@@ -301,6 +318,14 @@ class SigmaPlan:
     def apply(self, psi, sigma, precision=PREC_F64):
         assert psi.numel() >= self.n and sigma.numel() >= self.n
         check(lib().mq_sigma_apply_ex(self.h, psi.data_ptr(), sigma.data_ptr(), int(precision)))
+
+    def adjoint(self, psi):
+        """d(psi . H_eff psi) / d term_scale[t] for every term of the plan's MPO descriptor (numpy,
+        FP64): for a normalized eigenvector, dE/d scale_t."""
+        assert psi.dtype == torch.float64 and psi.is_cuda and psi.numel() >= self.n
+        g = np.zeros(max(1, self.n_terms), np.float64)
+        check(lib().mq_sigma_adjoint(self.h, C.c_void_p(psi.data_ptr()), g.ctypes.data_as(C.c_void_p)))
+        return g[:self.n_terms]
 
     def autotune(self, precision=PREC_F64):
         """Time the tile configurations of the T and final stages and keep the fastest per row-size

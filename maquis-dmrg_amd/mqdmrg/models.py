@@ -173,6 +173,7 @@ class SparseMPOTensor:
         self.left_phase = [1] * row_dim
         self.right_herm = [None] * col_dim
         self.right_phase = [1] * col_dim
+        self.keep_zero = False   # keep explicit zero coefficients as terms (derivatives)
 
     def sorted_terms(self):
         """Arrays (b1, b2, op_index, scale) sorted by (b1, b2)."""
@@ -187,8 +188,9 @@ class SparseMPOTensor:
             z = np.zeros(0, np.int64)
             return z, z, z, np.zeros(0)
         b1, b2, oi, sc = (np.concatenate(x) for x in (b1s, b2s, oi, sc))
-        keep = sc != 0
-        b1, b2, oi, sc = b1[keep], b2[keep], oi[keep], sc[keep]
+        if not getattr(self, "keep_zero", False):
+            keep = sc != 0
+            b1, b2, oi, sc = b1[keep], b2[keep], oi[keep], sc[keep]
         order = np.lexsort((oi, b2, b1))
         return b1[order], b2[order], oi[order], sc[order]
 

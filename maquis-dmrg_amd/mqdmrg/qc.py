@@ -110,14 +110,17 @@ def fermion_site():
 
 
 # ---------------------------------------------------------------------------- terms
-def qc_terms(h, eri, tol=1e-14):
-    """Normal-ordered operator strings of H: list of (coeff, ((orb, spin, dagger), ...))."""
+def qc_terms(h, eri, tol=1e-14, with_source=False):
+    """Normal-ordered operator strings of H: list of (coeff, ((orb, spin, dagger), ...)).
+    with_source: each entry also carries (kind, indices, factor) with coeff = factor * integral,
+    kind "h" (p, q) or "eri" (p, q, r, s)."""
     L = h.shape[0]
     terms = []
     for p, q in itertools.product(range(L), repeat=2):
         if abs(h[p, q]) > tol:
             for s in range(2):
-                terms.append((h[p, q], ((p, s, 1), (q, s, 0))))
+                t = (h[p, q], ((p, s, 1), (q, s, 0)))
+                terms.append(t + (("h", (p, q), 1.0),) if with_source else t)
     nz = np.argwhere(np.abs(eri) > tol)
     for p, q, r, s_ in nz:
         v = 0.5 * eri[p, q, r, s_]
@@ -127,7 +130,8 @@ def qc_terms(h, eri, tol=1e-14):
                     continue        # c+_pa c+_pa = 0
                 if s_ == q and a == b:
                     continue        # c_qa c_qa = 0
-                terms.append((v, ((p, a, 1), (r, b, 1), (s_, b, 0), (q, a, 0))))
+                t = (v, ((p, a, 1), (r, b, 1), (s_, b, 0), (q, a, 0)))
+                terms.append(t + (("eri", (int(p), int(q), int(r), int(s_)), 0.5),) if with_source else t)
     return terms
 
 
@@ -158,8 +162,14 @@ def _site_order(ops):
     return sign, ops
 
 
-def qc_mpo(h, eri, ecore=0.0, tol=1e-14):
-    """MPO of the ab initio Hamiltonian in the TwoU1 site basis (see module docstring)."""
+def qc_mpo(h, eri, ecore=0.0, tol=1e-14, sources=False):
+    """MPO of the ab initio Hamiltonian in the TwoU1 site basis (see module docstring).
+
+    sources=True: also return, per site j, {(b1, b2, op_id): [(kind, indices, factor), ...]}: the
+    integrals summed into each switch coefficient (coefficient = sum factor * integral); each
+    tensor then carries op_ids (the table id of each of its ops) and keeps zero coefficients, so
+    that the derivative with respect to every MPO coefficient (SigmaPlan.adjoint) maps back to the
+    integrals (mpo_gradient_to_integrals)."""
     L = h.shape[0]
     phys, cdag, c, F = fermion_site()
     I4 = np.eye(4)
@@ -181,7 +191,9 @@ def qc_mpo(h, eri, ecore=0.0, tol=1e-14):
             local_cache[k] = table.id(X) if np.any(X) else None
         return local_cache[k]
 
-    for coef, ops in qc_terms(h, eri, tol):
+    key_src = {}
+    for item in qc_terms(h, eri, tol, with_source=sources):
+        coef, ops = item[0], item[1]
         sign, sops = _site_order(ops)
         loc = []
         n = len(sops)
@@ -201,8 +213,13 @@ def qc_mpo(h, eri, ecore=0.0, tol=1e-14):
             continue
         key = tuple(loc)
         site_terms[key] = site_terms.get(key, 0.0) + sign * coef
+        if sources:
+            kind, idx, fac = item[2]
+            key_src.setdefault(key, []).append((kind, idx, sign * fac))
     if abs(ecore) > 0:
         site_terms[((0, idI),)] = site_terms.get(((0, idI),), 0.0) + ecore
+        if sources:
+            key_src.setdefault(((0, idI),), []).append(("ecore", (), 1.0))
     # fermionic parity of each distinct local operator
     parities = np.array([(a + b) % 2 for (a, b) in phys.charges])
 
@@ -234,17 +251,23 @@ def qc_mpo(h, eri, ecore=0.0, tol=1e-14):
     def key_parity(key):
         return sum(op_par[o] for _, o in key) % 2
 
+    switch_src = [dict() for _ in range(L)]   # (a, b, op) -> integral sources of its coefficient
+    cur_src = []
+
     def trans(j, prev, nxt, oid, coef):
         a, b = state(j, prev), state(j + 1, nxt)
         if prev[0] in ("start", "P") and nxt[0] in ("S", "done"):
             k3 = (a, b, oid)
             switch[j][k3] = switch[j].get(k3, 0.0) + coef
+            if sources:
+                switch_src[j].setdefault(k3, []).extend(cur_src)
         else:
             fixed[j][(a, b)] = oid
 
     for key, coef in site_terms.items():
-        if abs(coef) < tol:
+        if abs(coef) < tol and not sources:
             continue
+        cur_src = key_src.get(key, [])
         n = len(key)
         prev = START
         for t in range(n):
@@ -301,16 +324,45 @@ def qc_mpo(h, eri, ecore=0.0, tol=1e-14):
             per_op.setdefault(oid, []).append((a, b, 1.0))
         for (a, b, oid), v in switch[j].items():
             per_op.setdefault(oid, []).append((a, b, v))
-        used, coeffs = [], []
+        used, coeffs, op_ids = [], [], []
         for oid, ents in sorted(per_op.items()):
             r, c_, v = zip(*ents)
             C = sp.coo_matrix((v, (r, c_)), shape=(rows, cols)).tocsr()
             C.sum_duplicates()
             used.append(ops[oid])
             coeffs.append(C)
-        mpo.append(models.SparseMPOTensor(rows, cols, used, coeffs))
+            op_ids.append(oid)
+        T = models.SparseMPOTensor(rows, cols, used, coeffs)
+        if sources:
+            T.op_ids = op_ids
+            T.keep_zero = True
+        mpo.append(T)
     basis = models.SiteBasis(SYM_TWOU1, phys, {})
+    if sources:
+        return basis, mpo, [len(b) for b in bonds], switch_src
     return basis, mpo, [len(b) for b in bonds]
+
+
+def mpo_gradient_to_integrals(mpo, switch_src, term_grads, L):
+    """dE/dh (L x L), dE/d(pq|rs) (L^4) and dE/decore from the derivatives with respect to the
+    MPO term scales of every site (term_grads[j] in the order of mpo[j].sorted_terms()), through
+    the integral sources of each switch coefficient (qc_mpo(..., sources=True)).  Coefficients
+    that are 1 by construction (operator-string pass-through) carry no integral."""
+    dh = np.zeros((L, L))
+    deri = np.zeros((L, L, L, L))
+    decore = 0.0
+    for j, (W, g) in enumerate(zip(mpo, term_grads)):
+        b1s, b2s, ois, _ = W.sorted_terms()
+        assert len(g) == len(b1s)
+        for t in range(len(b1s)):
+            for kind, idx, fac in switch_src[j].get((int(b1s[t]), int(b2s[t]), W.op_ids[int(ois[t])]), ()):
+                if kind == "h":
+                    dh[idx] += fac * g[t]
+                elif kind == "eri":
+                    deri[idx] += fac * g[t]
+                else:
+                    decore += fac * g[t]
+    return dh, deri, decore
 
 
 # ---------------------------------------------------------------------------- exact check
@@ -344,3 +396,22 @@ def fci_energy(h, eri, n_up, n_down, ecore=0.0):
             if sg:
                 H[index[nb], k] += coef * sg
     return float(np.linalg.eigvalsh(H)[0]) + ecore
+
+
+def fci_energy_gradient_fd(h, eri, n_up, n_down, step=1e-5, entries=None):
+    """Central finite differences of the FCI energy with respect to single integral entries
+    (each h[p, q] and (pq|rs) entry an independent parameter).  entries: list of ("h", (p, q)) /
+    ("eri", (p, q, r, s)); default all.  Returns {entry: dE/dx}."""
+    L = h.shape[0]
+    if entries is None:
+        entries = [("h", ij) for ij in itertools.product(range(L), repeat=2)] + \
+                  [("eri", ijkl) for ijkl in itertools.product(range(L), repeat=4)]
+    out = {}
+    for kind, idx in entries:
+        vals = []
+        for sgn in (+1, -1):
+            hh, ee = h.copy(), eri.copy()
+            (hh if kind == "h" else ee)[idx] += sgn * step
+            vals.append(fci_energy(hh, ee, n_up, n_down))
+        out[(kind, idx)] = (vals[0] - vals[1]) / (2 * step)
+    return out

@@ -144,6 +144,39 @@ class SigmaOracle:
             pass
 
 
+class _TermMpo:
+    """The MPO descriptor of `mpo` restricted to its term t (scale 1): H_eff of this MPO is the
+    derivative of H_eff with respect to term_scale[t]."""
+
+    def __init__(self, mpo, t):
+        d = capi.mq_mpo_desc.from_buffer_copy(mpo.desc)
+        assert 0 <= t < d.n_terms
+        self.one = np.ones(1, np.float64)
+
+        def at(ptr, ctype, size):
+            return C.cast(C.cast(ptr, C.c_void_p).value + size * t, C.POINTER(ctype))
+        d.n_terms = 1
+        d.term_b1 = at(d.term_b1, C.c_uint32, 4)
+        d.term_b2 = at(d.term_b2, C.c_uint32, 4)
+        d.term_op = at(d.term_op, C.c_uint32, 4)
+        d.term_scale = C.cast(self.one.ctypes.data, C.POINTER(C.c_double))
+        self.parent = mpo
+        self.desc = d
+
+
+def sigma_term_gradient(site, mpo, left, right, psi):
+    """grad[t] = psi . (dH_eff / d term_scale[t]) psi for every MPO term, by brute force: the
+    oracle sigma of the single-term MPO (scale 1) dotted with psi.  H_eff is linear in each term
+    scale (the reference's alpha = coefficient x coupling x scale, op_iterate.hpp:58-63,
+    site_hamil_tasks.hpp:120-125), so this is the exact derivative of psi . H_eff psi."""
+    psi = np.ascontiguousarray(psi, np.float64)
+    n = mpo.desc.n_terms
+    g = np.zeros(n)
+    for t in range(n):
+        g[t] = float(np.dot(psi, SigmaOracle(site, _TermMpo(mpo, t), left, right).apply(psi)))
+    return g
+
+
 def _take_boundary(h):
     d = capi.mq_bindex_desc()
     _chk(lib().mqo_boundary_index(h, C.byref(d)))
