@@ -56,6 +56,15 @@ def parse():
     ap.add_argument("--batch", type=int, default=0,
                     help="also time N independent ab initio instances (L uniform in [12, 24], m=256) "
                          "applied one by one and as one batch")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also run the differentiation workload: two-site DMRG of the ab initio chain "
+                         "(--adjoint-L orbitals, N = L, bond dimension --adjoint-m), the sigma-adjoint sweep "
+                         "giving dE/dh and dE/d(pq|rs), its identities, and central finite differences of the "
+                         "re-optimised energy for a few integrals")
+    ap.add_argument("--adjoint-L", type=int, default=16)
+    ap.add_argument("--adjoint-m", type=int, default=256)
+    ap.add_argument("--adjoint-sweeps", type=int, default=6)
+    ap.add_argument("--adjoint-fd", type=int, default=3, help="integrals checked by finite differences")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile configurations (no timing of candidates before the run)")
     ap.add_argument("--seed", type=int, default=7)
@@ -102,6 +111,73 @@ def cpu_baseline(ws, Lcoh, Rcoh, psi, budget_s, gpu_sigma):
                      % (n, len(blocks), nblk, stride, so.flops, t,
                         "numpy OpenBLAS, 1 thread per call" if blas else "oracle blocked loop")}
     return res, blocks, out
+
+
+def adjoint_workload(eng, args):
+    """dE/dh_pq and dE/d(pq|rs) of the DMRG ground state from the sigma adjoint (one-site adjoint at
+    every site, Hellmann-Feynman), with size-independent identities and finite differences of the
+    re-optimised energy (warm start from the converged state, step 1e-5)."""
+    import numpy as np
+    import torch
+    from mqdmrg import qc
+    from mqdmrg.dmrg import TwoSiteDMRG, SiteTensor
+    L, m = args.adjoint_L, args.adjoint_m
+    N = L
+    h, eri = qc.synthetic_integrals(L, seed=0)
+    kw = dict(m=m, cutoff=1e-12, init_m=16, seed=3, jd_tol=1e-10, jd_maxiter=12)
+
+    def make(hh, ee):
+        basis, mpo, _, src = qc.qc_mpo(hh, ee, sources=True)
+        return basis, mpo, src
+    basis, mpo, src = make(h, eri)
+    d = TwoSiteDMRG(eng, "qc", L, mpo=mpo, basis=basis, target=(N // 2, N - N // 2), **kw)
+    t0 = time.perf_counter()
+    e = d.run(args.adjoint_sweeps)
+    t_dmrg = time.perf_counter() - t0
+    log("adjoint workload: L=%d m=%d E=%.12f after %d sweeps (%.1f s)" % (L, m, e, args.adjoint_sweeps, t_dmrg))
+    state = [SiteTensor(eng, s.sym, s.left, s.phys, s.right, s.data.clone()) for s in d.sites]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads, energies = d.mpo_gradients()
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    dh, deri, _ = qc.mpo_gradient_to_integrals(mpo, src, grads, L)
+    t_map = time.perf_counter() - t0 - t_adj
+    euler = float(np.sum(h * dh) + np.sum(eri * deri))
+    contr = np.einsum("pqrr->pq", deri)
+    res = {
+        "model": "ab initio (synthetic integrals), TwoU1, L=%d, N=%d, m=%d, %d sweeps, FP64" % (L, N, m,
+                                                                                              args.adjoint_sweeps),
+        "energy_Eh": e, "dmrg_seconds": round(t_dmrg, 2),
+        "adjoint_sweep_seconds": round(t_adj, 3), "integral_map_seconds": round(t_map, 3),
+        "site_energy_spread_Eh": float(max(energies) - min(energies)),
+        "checks": {
+            "euler_sum_x_dEdx_minus_E": euler - e,
+            "trace_1rdm_minus_N": float(np.trace(dh) - N),
+            "1rdm_asymmetry": float(np.linalg.norm(dh - dh.T)),
+            "2rdm_partial_trace_residual": float(np.linalg.norm(contr - 0.5 * (N - 1) * dh) / np.linalg.norm(dh)),
+        },
+    }
+    log("adjoint sweep %.2f s; checks %s" % (t_adj, res["checks"]))
+    # finite differences of the re-optimised energy, warm-started from the converged state
+    fd = []
+    picks = [("h", (0, 0)), ("h", (L // 2, L // 2 + 1)), ("eri", (1, 1, 2, 2)), ("eri", (0, 1, 1, 0))][:args.adjoint_fd]
+    for kind, idx in picks:
+        vals = []
+        for sgn in (+1, -1):
+            hh, ee = h.copy(), eri.copy()
+            (hh if kind == "h" else ee)[idx] += sgn * 1e-5
+            b2, mpo2, _ = make(hh, ee)
+            d2 = TwoSiteDMRG(eng, "qc", L, mpo=mpo2, basis=b2, target=(N // 2, N - N // 2), **kw)
+            d2.sites = [SiteTensor(eng, s.sym, s.left, s.phys, s.right, s.data.clone()) for s in state]
+            vals.append(d2.run(2))
+        num = (vals[0] - vals[1]) / 2e-5
+        ana = float((dh if kind == "h" else deri)[idx])
+        fd.append({"integral": kind, "index": list(idx), "adjoint": ana, "finite_difference": num,
+                   "rel_err": abs(ana - num) / max(abs(num), 1e-300)})
+        log("  d E / d %s%s: adjoint %.10f  FD %.10f" % (kind, idx, ana, num))
+    res["finite_differences"] = fd
+    return res
 
 
 def main():
@@ -293,6 +369,8 @@ def main():
         batch = {"instances": args.batch, "model": "ab initio (synthetic integrals), L uniform in [12, 24], m=256, "
                  "middle site", "flops_per_step": flops, "runs": res}
 
+    adjoint = adjoint_workload(eng, args) if args.adjoint else None
+
     out = {
         "metric": "sweep wall-time & σ TFLOP/s at L=30 m=2048, mixed vs FP64 at |ΔE|≤1e-6 Eh",
         "value": pr["tflops"],
@@ -324,6 +402,7 @@ def main():
         "speedup_vs_f64": per["f64"]["ms_per_sigma"] / pr["ms_per_sigma"],
         "sweeps": sweep,
         "batch": batch,
+        "adjoint": adjoint,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
