@@ -390,6 +390,41 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                     coff[j] = uint64_t(gn) * R[i].ldc;
                 }
             }
+            // FP32 accumulators: registers r = 0..3 of a lane are 4 consecutive rows of one column.
+            // With all four inside the unit, each output copy is one 8-byte (bf16) or 16-byte
+            // (float) store, so a wave writes whole 32/64-byte row runs per column instead of
+            // scattered 2/4-byte pieces (global memory accepts element-aligned vector stores)
+            if constexpr (MODE == MODE_F32 && (SPLIT || std::is_same<CT, float>::value)) {
+                const uint32_t gm0 = m0 + wm + 16 * i + 4 * lk;
+                const uint32_t li0 = gm0 - R[i].row0;
+                if (!tk.beta && gm0 + 3 < M && li0 + 3 < R[i].nrows) {
+#pragma unroll
+                    for (int j = 0; j < WN; ++j) {
+                        if (n0 + wn + 16 * j + l16 >= N) continue;
+                        if constexpr (SPLIT) {
+                            typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+                            typedef u16x4 __attribute__((aligned(2))) u16x4_a2;
+                            u16x4 hi, lo;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const float v = float(acc[i][j][r]);
+                                hi[r] = bf16_rne(v);
+                                lo[r] = bf16_rne(v - __uint_as_float(uint32_t(hi[r]) << 16));
+                            }
+                            uint16_t* __restrict__ Cm = C + R[i].c_split + li0 + coff[j];
+                            const uint64_t sd = R[i].split_stride;
+                            *reinterpret_cast<u16x4_a2*>(Cm) = hi;
+                            *reinterpret_cast<u16x4_a2*>(Cm + sd) = lo;
+                            *reinterpret_cast<u16x4_a2*>(Cm + 2 * sd) = hi;
+                        } else {
+                            typedef float __attribute__((ext_vector_type(4), aligned(4))) f4_a4;
+                            *reinterpret_cast<f4_a4*>(C + R[i].c_src + li0 + coff[j]) =
+                                f4_a4{float(acc[i][j][0]), float(acc[i][j][1]), float(acc[i][j][2]), float(acc[i][j][3])};
+                        }
+                    }
+                    continue;
+                }
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const uint32_t gm = m0 + wm + 16 * i + d_row<MODE>(r, lk);
