@@ -48,7 +48,7 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", tmp] + objs + [
         "-Wl,-rpath," + ROCM_LLVM_LIB, "-L" + ROCM_LLVM_LIB,
-        "-Wl,-rpath," + ROCM_LIB, "-L" + ROCM_LIB, "-lrocblas"]
+        "-Wl,-rpath," + ROCM_LIB, "-L" + ROCM_LIB, "-lrocblas", "-lhipblaslt"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr))

@@ -8,6 +8,7 @@
 //              no atomics), or the prop_l / prop_r products of the boundary steps.
 // All operand addresses are checked against their buffers on the host before any launch.
 #include <hip/hip_runtime.h>
+#include <hipblaslt/hipblaslt.h>
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -113,10 +114,34 @@ struct BatchGraph {
     bool seen = false;
 };
 
+// One library GEMM of the reduced-precision S stage through hipBLASLt: descriptors and the
+// algorithm chosen for its shape (the library's first heuristic, or the fastest of its candidates
+// after mq_plan_autotune).
+struct LtGemm {
+    bool ready = false, ok = false;
+    hipblasLtMatmulDesc_t op = nullptr;
+    hipblasLtMatrixLayout_t a = nullptr, b = nullptr, c = nullptr;
+    hipblasLtMatmulAlgo_t algo{};
+    void release() {
+        if (a) (void)hipblasLtMatrixLayoutDestroy(a);
+        if (b) (void)hipblasLtMatrixLayoutDestroy(b);
+        if (c) (void)hipblasLtMatrixLayoutDestroy(c);
+        if (op) (void)hipblasLtMatmulDescDestroy(op);
+        a = b = c = nullptr;
+        op = nullptr;
+        ready = ok = false;
+    }
+};
+constexpr size_t LT_WORKSPACE = size_t(64) << 20;
+
 struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;   // plain dense GEMMs of the grouped S stage, on `stream`
+    hipblasLtHandle_t lt = nullptr;  // reduced-precision S stage GEMMs
+    DevBuf lt_ws;
+    // hipBLASLt algorithms timed by mq_plan_autotune, keyed by (precision, M, N, K)
+    std::map<std::array<uint64_t, 4>, hipblasLtMatmulAlgo_t> lt_tuned;
     DevBuf red_partial, red_result;
     DevBuf step_ws;   // grows; reused by boundary steps
     // batched applications: side streams (default priority) the instances of a batch fork onto,
@@ -133,6 +158,13 @@ struct mq_ctx {
         batches.clear();
     }
 };
+
+#define LTCHK(x)                                                                                              \
+    do {                                                                                                      \
+        hipblasStatus_t s_ = (x);                                                                             \
+        if (s_ != HIPBLAS_STATUS_SUCCESS)                                                                     \
+            fail(MQ_ERR_HIP, std::string("hipBLASLt: status ") + std::to_string(int(s_)) + " at " #x);        \
+    } while (0)
 
 #define BLASCHK(x)                                                                                            \
     do {                                                                                                      \
@@ -331,6 +363,117 @@ struct StagePlan {
     };
     std::vector<LibGemm> lib;
     rocblas_handle blas = nullptr;
+    // hipBLASLt form of the FP32 (index 0) and BF16x3 (index 1) library GEMMs, prepared on first
+    // use outside graph capture; a GEMM without a hipBLASLt algorithm runs on rocBLAS
+    mq_ctx* lt_ctx = nullptr;
+    mutable std::vector<LtGemm> ltg[2];
+    StagePlan() = default;
+    StagePlan(const StagePlan&) = delete;
+    StagePlan& operator=(const StagePlan&) = delete;
+    ~StagePlan() {
+        for (auto& v : ltg)
+            for (auto& g : v) g.release();
+    }
+    // descriptors of lib GEMM i in mode mi; returns whether hipBLASLt runs it
+    bool lt_prepare(int mi, size_t i, hipStream_t st) const {
+        if (!lt_ctx || !lt_ctx->lt) return false;
+        if (ltg[mi].size() != lib.size()) ltg[mi].resize(lib.size());
+        LtGemm& L = ltg[mi][i];
+        if (L.ready) return L.ok;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone) return false;   // never query the library inside a capture
+        const LibGemm& g = lib[i];
+        const hipDataType in = mi == 0 ? HIP_R_32F : HIP_R_16BF;
+        const uint64_t K = mi == 0 ? g.K : 3ull * g.K;
+        const uint64_t lda = g.lda, ldb = mi == 0 ? g.ldb : 3ull * g.K;
+        LTCHK(hipblasLtMatmulDescCreate(&L.op, HIPBLAS_COMPUTE_32F, HIP_R_32F));
+        hipblasOperation_t nn = HIPBLAS_OP_N;
+        LTCHK(hipblasLtMatmulDescSetAttribute(L.op, HIPBLASLT_MATMUL_DESC_TRANSA, &nn, sizeof(nn)));
+        LTCHK(hipblasLtMatmulDescSetAttribute(L.op, HIPBLASLT_MATMUL_DESC_TRANSB, &nn, sizeof(nn)));
+        LTCHK(hipblasLtMatrixLayoutCreate(&L.a, in, g.M, K, lda));
+        LTCHK(hipblasLtMatrixLayoutCreate(&L.b, in, K, g.N, ldb));
+        LTCHK(hipblasLtMatrixLayoutCreate(&L.c, HIP_R_32F, g.M, g.N, g.ldc));
+        std::array<uint64_t, 4> key{uint64_t(mi), g.M, g.N, K};
+        auto it = lt_ctx->lt_tuned.find(key);
+        if (it != lt_ctx->lt_tuned.end()) {
+            L.algo = it->second;
+            L.ok = true;
+        } else {
+            hipblasLtMatmulPreference_t pref;
+            LTCHK(hipblasLtMatmulPreferenceCreate(&pref));
+            size_t wsz = LT_WORKSPACE;
+            LTCHK(hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz, sizeof(wsz)));
+            hipblasLtMatmulHeuristicResult_t res[1];
+            int n = 0;
+            hipblasStatus_t hs = hipblasLtMatmulAlgoGetHeuristic(lt_ctx->lt, L.op, L.a, L.b, L.c, L.c, pref, 1, res, &n);
+            (void)hipblasLtMatmulPreferenceDestroy(pref);
+            L.ok = hs == HIPBLAS_STATUS_SUCCESS && n > 0;
+            if (L.ok) L.algo = res[0].algo;
+        }
+        L.ready = true;
+        return L.ok;
+    }
+    // operands of lib GEMM i in mode mi (0: FP32 workspace, 1: bf16 split buffer)
+    void lt_operands(int mi, size_t i, Bases b, const void*& A, const void*& B, float*& C) const {
+        float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
+        const LibGemm& g = lib[i];
+        A = mi == 0 ? static_cast<const void*>(ws + g.a_off) : static_cast<const void*>(bf + g.bfa_off);
+        B = mi == 0 ? static_cast<const void*>(ws + g.b_off) : static_cast<const void*>(bf + g.bfb_off);
+        C = ws + g.c_off;
+    }
+    // time up to 8 of the library's candidate algorithms for lib GEMM i and keep the fastest
+    // (cached in the context by shape); returns the number of timings
+    int lt_tune(int mi, size_t i, hipStream_t st, Bases b, hipEvent_t e0, hipEvent_t e1) const {
+        if (!lt_prepare(mi, i, st)) return 0;
+        LtGemm& L = ltg[mi][i];
+        const LibGemm& g = lib[i];
+        const uint64_t K = mi == 0 ? g.K : 3ull * g.K;
+        std::array<uint64_t, 4> key{uint64_t(mi), g.M, g.N, K};
+        auto it = lt_ctx->lt_tuned.find(key);
+        if (it != lt_ctx->lt_tuned.end()) {
+            L.algo = it->second;
+            return 0;
+        }
+        hipblasLtMatmulPreference_t pref;
+        LTCHK(hipblasLtMatmulPreferenceCreate(&pref));
+        size_t wsz = LT_WORKSPACE;
+        LTCHK(hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz, sizeof(wsz)));
+        hipblasLtMatmulHeuristicResult_t res[8];
+        int n = 0;
+        hipblasStatus_t hs = hipblasLtMatmulAlgoGetHeuristic(lt_ctx->lt, L.op, L.a, L.b, L.c, L.c, pref, 8, res, &n);
+        (void)hipblasLtMatmulPreferenceDestroy(pref);
+        if (hs != HIPBLAS_STATUS_SUCCESS || n <= 0) return 0;
+        const void *A, *B;
+        float* C;
+        lt_operands(mi, i, b, A, B, C);
+        const float one = 1.f, zero = 0.f;
+        float best = 0.f;
+        int timed = 0;
+        for (int c = 0; c < n; ++c) {
+            auto run = [&]() {
+                return hipblasLtMatmul(lt_ctx->lt, L.op, &one, A, L.a, B, L.b, &zero, C, L.c, C, L.c, &res[c].algo,
+                                       lt_ctx->lt_ws.p, LT_WORKSPACE, st);
+            };
+            if (run() != HIPBLAS_STATUS_SUCCESS) continue;   // warm-up; a failing candidate is skipped
+            HIPCHK(hipEventRecord(e0, st));
+            for (int r = 0; r < 2; ++r) LTCHK(run());
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            ++timed;
+            if (timed == 1 || ms < best) best = ms, L.algo = res[c].algo;
+        }
+        if (timed) lt_ctx->lt_tuned[key] = L.algo;
+        return timed;
+    }
+    void lt_run(int mi, size_t i, hipStream_t st, const void* A, const void* B, float* C) const {
+        const LtGemm& L = ltg[mi][i];
+        const float one = 1.f, zero = 0.f;
+        LTCHK(hipblasLtMatmul(lt_ctx->lt, L.op, &one, A, L.a, B, L.b, &zero, C, L.c, C, L.c, &L.algo,
+                              lt_ctx->lt_ws.p, LT_WORKSPACE, st));
+    }
     uint64_t t_end = 0;     // T_D occupies workspace elements [0, t_end)
     uint64_t bf_elems = 0;  // size of the bf16 buffer
     uint16_t* bf = nullptr;
@@ -600,16 +743,26 @@ struct StagePlan {
         } else if (mode == MODE_F32) {
             const float one = 1.f, zero = 0.f;
             float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
-            for (const auto& g : lib)
-                BLASCHK(rocblas_sgemm(blas, nn, nn, g.M, g.N, g.K, &one, ws + g.a_off, g.lda, ws + g.b_off, g.ldb, &zero,
-                                      ws + g.c_off, g.ldc));
+            for (size_t i = 0; i < lib.size(); ++i) {
+                const auto& g = lib[i];
+                if (lt_prepare(0, i, st))
+                    lt_run(0, i, st, ws + g.a_off, ws + g.b_off, ws + g.c_off);
+                else
+                    BLASCHK(rocblas_sgemm(blas, nn, nn, g.M, g.N, g.K, &one, ws + g.a_off, g.lda, ws + g.b_off, g.ldb,
+                                          &zero, ws + g.c_off, g.ldc));
+            }
         } else {
             MQ_REQUIRE(bf, "bf16 split buffer not allocated");
             float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
             const float one = 1.f, zero = 0.f;
             const rocblas_datatype b16 = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
-            for (const auto& g : lib) {
+            for (size_t i = 0; i < lib.size(); ++i) {
+                const auto& g = lib[i];
                 if (!split_t) HIPCHK(launch_split3_bf16(st, ws + g.a_off, bf + g.bfa_off, uint64_t(g.M) * g.K));
+                if (lt_prepare(1, i, st)) {
+                    lt_run(1, i, st, bf + g.bfa_off, bf + g.bfb_off, ws + g.c_off);
+                    continue;
+                }
                 BLASCHK(rocblas_gemm_ex(blas, nn, nn, g.M, g.N, 3 * g.K, &one, bf + g.bfa_off, b16, g.M,
                                         bf + g.bfb_off, b16, 3 * g.K, &zero, ws + g.c_off, f32, g.ldc, ws + g.c_off,
                                         f32, g.ldc, f32, rocblas_gemm_algo_standard, 0, 0));
@@ -894,6 +1047,8 @@ int mq_ctx_create(int device, mq_ctx** out) {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         BLASCHK(rocblas_create_handle(&c->blas));
         BLASCHK(rocblas_set_stream(c->blas, c->stream));
+        LTCHK(hipblasLtCreate(&c->lt));
+        c->lt_ws.alloc(LT_WORKSPACE);
         c->red_partial.alloc(RED_BLOCKS * sizeof(double));
         c->red_result.alloc(64 * sizeof(double));
         *out = c.release();
@@ -909,6 +1064,7 @@ int mq_ctx_destroy(mq_ctx* ctx) {
         for (auto e : ctx->side_ev) (void)hipEventDestroy(e);
         if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+        if (ctx->lt) (void)hipblasLtDestroy(ctx->lt);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -1949,6 +2105,7 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         lower_sigma_grouped(p.get(), L, W, ws_init, ws_init_off, adjoint);
         StagePlan& P = p->st;
         P.blas = ctx->blas;
+        P.lt_ctx = ctx;
         P.finish_tables();
         P.sizes[BASE_WS] = P.ws_elems;
         P.sizes[BASE_IN] = P.sizes[BASE_OUT] = L.rp_total;
@@ -2190,6 +2347,9 @@ int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
             for (int c = 0; c < N_SIZE_CLASS; ++c) rec[c] = pol.cls[c];
             ctx->tuned[key] = rec;
         }
+        if (precision != MQ_PREC_F64)   // the library GEMMs of the S stage: fastest hipBLASLt candidate
+            for (size_t i = 0; i < P.lib.size(); ++i)
+                n_timed += P.lt_tune(precision == MQ_PREC_F32 ? 0 : 1, i, st, b, e0, e1);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         HIPCHK(hipStreamSynchronize(st));
