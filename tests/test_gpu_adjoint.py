@@ -134,8 +134,9 @@ def test_rdm_from_adjoint_matches_fci_finite_differences(eng):
     et = torch.tensor(eri, requires_grad=True)
     e = DMRGEnergy.apply(ht, et, eng, nup, ndn)
     e.backward()
-    print("E", float(e), "FCI", efci, flush=True)
-    assert abs(float(e) - efci) < 1e-9
+    ev = float(e.detach())
+    print("E", ev, "FCI", efci, flush=True)
+    assert abs(ev - efci) < 1e-9
     fd = qc.fci_energy_gradient_fd(h, eri, nup, ndn)
     fh = np.array([[fd[("h", (p, q))] for q in range(L)] for p in range(L)])
     fe = np.zeros((L,) * 4)
