@@ -291,8 +291,8 @@ def main():
     achieved = fl / (st[dom] * 1e-3) / 1e12
     if dom == "gemm_S":
         # the S stage is one dense library GEMM per charge-transfer group, in the reported precision
-        kernel = {"f64": "rocBLAS dgemm", "f32": "rocBLAS sgemm",
-                  "bf16x3": "split3_bf16_kernel + rocBLAS gemm_ex bf16, K = 3K per group"}[report] + " (S stage)"
+        kernel = {"f64": "rocBLAS dgemm", "f32": "hipBLASLt FP32 GEMM",
+                  "bf16x3": "hipBLASLt bf16 GEMM over the split operands, K' = 3K per group"}[report] + " (S stage)"
         peak_key = report
     else:
         # T and final stages: the hand-written MFMA GEMM, FP32 in the reduced precisions
