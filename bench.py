@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-L", type=int, default=16, help="chain length of the sweep comparison")
     ap.add_argument("--sweep-m", type=int, default=256, help="bond dimension of the sweep comparison")
-    ap.add_argument("--sweep-n", type=int, default=3, help="sweeps of the sweep comparison")
+    ap.add_argument("--sweep-n", type=int, default=6, help="sweeps of the sweep comparison (converged energies)")
     ap.add_argument("--batch", type=int, default=0,
                     help="also time N independent ab initio instances (L uniform in [12, 24], m=256) "
                          "applied one by one and as one batch")
