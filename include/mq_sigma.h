@@ -321,6 +321,17 @@ int mq_boundary_left_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_de
 int mq_boundary_right_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
                            const double* bra_rp, const double* ket_rp, const mq_boundary* right,
                            int symmetric, mq_boundary** out);
+/* As mq_boundary_left_step / mq_boundary_right_step with the algorithm chosen (mq_sigma_algorithm).
+ * GROUPED (abelian groups): the new boundary's index comes from the index-only builder; the step
+ * is T (psi R, or L psi), S_D = T_D W_D (one library GEMM per charge-transfer group) and the
+ * contraction with the bra, as in the grouped sigma, without the per-sector alpha lists.  The
+ * plain entry points use AUTO (GROUPED for U1 / TwoU1, SCHEDULE for SU2U1). */
+int mq_boundary_left_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
+                             const double* bra_lp, const double* ket_rp, const mq_boundary* left,
+                             int symmetric, int algorithm, mq_boundary** out);
+int mq_boundary_right_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
+                              const double* bra_rp, const double* ket_rp, const mq_boundary* right,
+                              int symmetric, int algorithm, mq_boundary** out);
 
 /* ---------------------------------------------------------------- Davidson */
 /* Device-resident vector kernels (DavidsonVector ops, solver/davidson_vector.cpp:197-254). */

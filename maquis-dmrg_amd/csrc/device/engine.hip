@@ -523,12 +523,15 @@ struct StagePlan {
     // C blocks lie inside their buffers; one plain (non-transposed, unmapped) A segment.
     void check_stacked(const GemmTask& t, const std::vector<GemmSeg>& segs, bool split) const {
         MQ_REQUIRE(uint64_t(t.rowunit) + t.M <= maps.size(), "row unit table range");
-        MQ_REQUIRE(t.seg0 + 1 == t.seg1 && t.seg1 <= segs.size(), "stacked task with other than one segment");
+        MQ_REQUIRE(t.seg0 < t.seg1 && t.seg1 <= segs.size(), "stacked task segment range");
         MQ_REQUIRE(t.colunit == MAP_NONE && t.cbw > 0, "stacked task shape");
-        const GemmSeg& g = segs[t.seg0];
-        MQ_REQUIRE(!g.trans_a && g.amap == MAP_NONE && g.units == MAP_NONE, "stacked task with a remapped A");
-        if (g.bmap != MAP_NONE) fail(MQ_ERR_INTERNAL, "stacked task with a remapped B");
-        check_in(g.b_base, g.b_off, g.trans_b ? extent(t.N, g.K, g.ldb) : extent(g.K, t.N, g.ldb), "stacked B");
+        for (uint32_t q = t.seg0; q < t.seg1; ++q) {
+            const GemmSeg& g = segs[q];
+            MQ_REQUIRE(g.units == MAP_NONE && g.bmap == MAP_NONE, "stacked task with remapped B");
+            MQ_REQUIRE(g.amap != MAP_NONE || !g.trans_a, "stacked task with a transposed A outside a block map");
+            MQ_REQUIRE(g.amap == MAP_NONE || t.seg1 - t.seg0 >= 1, "stacked block map");
+            check_in(g.b_base, g.b_off, g.trans_b ? extent(t.N, g.K, g.ldb) : extent(g.K, t.N, g.ldb), "stacked B");
+        }
         MQ_REQUIRE(t.M % 16 == 0, "stacked task rows not a multiple of 16");
         uint32_t m = 0;
         while (m < t.M) {
@@ -540,23 +543,37 @@ struct StagePlan {
             while (e < t.M && maps[t.rowunit + e] == u) ++e;
             MQ_REQUIRE((e - m) % 16 == 0 && R.nrows <= e - m && e - m < R.nrows + 16, "row unit span");
             const uint32_t nr = R.nrows;
-            MQ_REQUIRE(R.ldc >= nr, "row unit leading dimensions");
-            if (nr && g.K)
-                check_in(g.a_base, g.a_off + R.a_src, uint64_t(nr - 1) * R.a_si + uint64_t(g.K - 1) * R.a_sk + 1,
-                         "stacked A block");
+            MQ_REQUIRE(R.ldc >= 1 && R.c_si >= 1, "row unit output strides");
+            for (uint32_t q = t.seg0; q < t.seg1 && nr; ++q) {
+                const GemmSeg& g = segs[q];
+                if (!g.K) continue;
+                if (g.amap == MAP_NONE) {
+                    check_in(g.a_base, g.a_off + R.a_src, uint64_t(nr - 1) * R.a_si + uint64_t(g.K - 1) * R.a_sk + 1,
+                             "stacked A block");
+                } else {
+                    MQ_REQUIRE(uint64_t(g.amap) + R.jblk < maps.size(), "stacked A block map range");
+                    const uint64_t si = g.trans_a ? g.lda : 1, sk = g.trans_a ? 1 : g.lda;
+                    check_in(g.a_base, g.a_off + uint64_t(maps[g.amap + R.jblk]) * g.bldk,
+                             uint64_t(nr - 1) * si + uint64_t(g.K - 1) * sk + 1, "stacked A unit block");
+                }
+            }
             // C blocks: [off, off + ext) of the workspace, and of the A' copies when split
-            auto c_block = [&](uint64_t rel, uint64_t ext) {
+            auto c_block = [&](uint64_t rel, uint32_t ncols) {
+                if (!nr || !ncols) return;
+                const uint64_t ext = uint64_t(nr - 1) * R.c_si + uint64_t(ncols - 1) * R.ldc + 1;
                 check_in(t.c_base, t.c_off + R.c_src + rel, ext, "stacked C block");
-                if (split && ext && t.c_off + R.c_split + rel + 2 * R.split_stride + ext > bf_a_elems)
+                if (split && t.c_off + R.c_split + rel + 2 * R.split_stride + ext > bf_a_elems)
                     fail(MQ_ERR_INTERNAL, "stacked split output out of bounds");
             };
+            if (split) MQ_REQUIRE(R.c_si == 1, "split output with a row stride");
             if (R.cmap == MAP_NONE) {
-                c_block(0, extent(nr, t.N, R.ldc));
+                c_block(0, t.N);
             } else {
+                MQ_REQUIRE(R.c_si == 1, "column-remapped stacked output with a row stride");
                 const uint32_t nblk = (t.N + t.cbw - 1) / t.cbw;
                 MQ_REQUIRE(uint64_t(R.cmap) + nblk <= maps.size(), "row unit column map range");
                 for (uint32_t j = 0; j < nblk; ++j)
-                    c_block(uint64_t(maps[R.cmap + j]) * R.cldk, extent(nr, std::min(t.cbw, t.N - j * t.cbw), R.ldc));
+                    c_block(uint64_t(maps[R.cmap + j]) * R.cldk, std::min(t.cbw, t.N - j * t.cbw));
             }
             m = e;
         }
@@ -1488,15 +1505,35 @@ struct GroupedLowering {
     uint64_t ref_flops = 0;
 };
 
+// A boundary step lowered by the grouped path: kind Schedule::LEFT (T = L psi) or RIGHT (T = psi R),
+// the new boundary's index and cohort offsets (the final stage's output).
+struct GroupedStep {
+    int kind;
+    const BoundaryIndex* out;
+    const std::vector<uint64_t>* out_off;
+};
+
 void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::vector<double>& ws_init,
-                         uint64_t& ws_init_off, bool adjoint) {
+                         uint64_t& ws_init_off, bool adjoint, const GroupedStep* step = nullptr) {
     const Index &left_i = site.left_i, &right_i = site.right_i, &phys_i = site.phys_i;
-    const BoundaryIndex& LI = p->left->index;
-    const BoundaryIndex& RI = p->right->index;
+    const int kind = step ? step->kind : Schedule::SIGMA;
+    const bool sigma = kind == Schedule::SIGMA, rstep = kind == Schedule::RIGHT, lstep = kind == Schedule::LEFT;
     MQ_REQUIRE(!site.symm.su2(), "the grouped sigma path handles abelian symmetries only");
-    MQ_REQUIRE(LI.aux_dim == W.row_dim && RI.aux_dim == W.col_dim, "boundary aux_dim != mpo bond dimension");
-    require_bond(LI, left_i, "left");
-    require_bond(RI, right_i, "right");
+    MQ_REQUIRE(!(adjoint && !sigma), "the adjoint is a sigma-plan option");
+    static const BoundaryIndex none;
+    const BoundaryIndex& LI = p->left ? p->left->index : none;
+    const BoundaryIndex& RI = p->right ? p->right->index : none;
+    const BoundaryIndex& NI = step ? *step->out : none;   // the new boundary of a step
+    if (!rstep) {
+        MQ_REQUIRE(p->left && LI.aux_dim == W.row_dim, "left boundary aux_dim != mpo row dimension");
+        require_bond(LI, left_i, "left");
+    }
+    if (!lstep) {
+        MQ_REQUIRE(p->right && RI.aux_dim == W.col_dim, "right boundary aux_dim != mpo column dimension");
+        require_bond(RI, right_i, "right");
+    }
+    if (rstep) MQ_REQUIRE(NI.aux_dim == W.row_dim, "new right boundary aux_dim != mpo row dimension");
+    if (lstep) MQ_REQUIRE(NI.aux_dim == W.col_dim, "new left boundary aux_dim != mpo column dimension");
     ProductBasis rpb(phys_i, right_i, -1);
     GroupedLowering G;
     std::map<uint64_t, uint32_t> group_of;   // delta key -> group (ordered: deterministic numbering)
@@ -1521,20 +1558,30 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         Charge lc_in = left_i.charges[lb];
         for (uint32_t rb = 0; rb < right_i.size(); ++rb) {
             Charge rc_out = right_i.charges[rb];
+            // keys of the pair: K = (b, s_in, ss) of the boundary multiplied in the T stage, N = (b,
+            // s_out, ss) of the boundary the final stage contracts with (sigma: L; steps: the new
+            // boundary's stored cohorts).  For the left step the pair is (bra left sector, ket right
+            // sector) and T = L psi; otherwise (ket left sector, output right sector) and T = psi R.
             std::vector<uint64_t> K, N;
+            auto add_keys = [&](std::vector<uint64_t>& out, const BoundaryIndex& B, uint32_t c, uint32_t s,
+                                bool stored_only) {
+                if (c >= B.n_cohorts() || (stored_only && B.tr[c])) return;
+                for (uint32_t b = 0; b < B.aux_dim; ++b)
+                    if (B.has_block(c, b))
+                        for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) out.push_back(GroupedLowering::kkey(b, s, ss));
+            };
             for (uint32_t s = 0; s < phys_i.size(); ++s) {
-                Charge rc_in = lc_in + phys_i.charges[s];
-                uint32_t ci = RI.cohort_index(rc_in, rc_out);
-                if (ci < RI.n_cohorts())
-                    for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2)
-                        if (RI.has_block(ci, b2))
-                            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) K.push_back(GroupedLowering::kkey(b2, s, ss));
-                Charge lc_out = rc_out - phys_i.charges[s];
-                uint32_t cl = LI.cohort_index(lc_out, lc_in);
-                if (cl < LI.n_cohorts())
-                    for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1)
-                        if (LI.has_block(cl, b1))
-                            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) N.push_back(GroupedLowering::kkey(b1, s, ss));
+                if (lstep) {
+                    // lc_in is the bra's left sector, rc_out the ket's right sector
+                    add_keys(K, LI, LI.cohort_index(lc_in, rc_out - phys_i.charges[s]), s, false);
+                    add_keys(N, NI, NI.cohort_index(lc_in + phys_i.charges[s], rc_out), s, true);
+                    continue;
+                }
+                add_keys(K, RI, RI.cohort_index(lc_in + phys_i.charges[s], rc_out), s, false);
+                if (sigma)
+                    add_keys(N, LI, LI.cohort_index(rc_out - phys_i.charges[s], lc_in), s, false);
+                else
+                    add_keys(N, NI, NI.cohort_index(lc_in, rc_out - phys_i.charges[s]), s, true);
             }
             if (K.empty() || N.empty()) continue;
             Charge d = rc_out - lc_in;
@@ -1633,8 +1680,9 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                     uint32_t s_in = phys_i.position(wb.in), s_out = phys_i.position(wb.out);
                     MQ_REQUIRE(s_in < phys_i.size() && s_out < phys_i.size(), "operator charge not in physical index");
                     for (uint32_t e = wb.e0; e < wb.e1; ++e) {
-                        auto kit = kgroups.find(GroupedLowering::kkey(b2, s_in, op.row[e]));
-                        auto nit = ngroups.find(GroupedLowering::kkey(b1, s_out, op.col[e]));
+                        // K keys carry the MPO index of the T-stage boundary (b2 for psi R, b1 for L psi)
+                        auto kit = kgroups.find(GroupedLowering::kkey(lstep ? b1 : b2, s_in, op.row[e]));
+                        auto nit = ngroups.find(GroupedLowering::kkey(lstep ? b2 : b1, s_out, op.col[e]));
                         if (kit == kgroups.end() || nit == ngroups.end()) continue;
                         for (auto& kg : kit->second)
                             for (auto& ng : nit->second)
@@ -1672,83 +1720,147 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         }
         G.ref_flops = f;
     }
-    // ---- T stage.  The psi blocks that meet one right cohort (every (lb_in, s, ss) with
-    // lc_in + q_s = rc_in, for the cohort's rc_out) are stacked into the rows of one task, so the
-    // cohort is read once per column tile for all of them; each block's rows are written to its own
-    // pair slab and keys through a row unit.  A stored cohort is one task over all its blocks
-    // (columns (j, rs) of block j); a transposed cohort is one task per block: the partner's block,
-    // transposed and scaled by its conjugation factor.
     StagePlan& P = p->st;
-    struct RowBlock { uint32_t pair, s, ss; };
-    struct Stack { uint32_t K = 0, brs = 0, rbi = 0; std::vector<RowBlock> blocks; };
-    std::map<uint32_t, Stack> stacks;   // right cohort -> its psi blocks (deterministic order)
-    for (uint32_t pi = 0; pi < G.pairs.size(); ++pi) {
-        const auto& pr = G.pairs[pi];
-        uint32_t lb = pr.lb_in, rb = pr.rb_out;
-        Charge lc_in = left_i.charges[lb], rc_out = right_i.charges[rb];
-        for (uint32_t s = 0; s < phys_i.size(); ++s) {
-            Charge rc_in = lc_in + phys_i.charges[s];
-            uint32_t rbi = right_i.position(rc_in);
-            uint32_t ci = RI.cohort_index(rc_in, rc_out);
-            if (rbi == right_i.size() || ci >= RI.n_cohorts()) continue;
-            Stack& st = stacks[ci];
-            st.K = right_i.sizes[rbi];
-            st.brs = right_i.sizes[rb];
-            st.rbi = rbi;
-            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) st.blocks.push_back({pi, s, ss});
-        }
-    }
-    // one stacked task over the given psi blocks: row units with the column keys of `b2s`
-    auto stacked_task = [&](const Stack& st, const std::vector<uint32_t>& b2s, const GemmSeg& sg) {
-        std::vector<uint32_t> unit_of_row;
-        for (const RowBlock& rbk : st.blocks) {
-            const auto& pr = G.pairs[rbk.pair];
-            const auto& g = G.groups[pr.g];
-            const uint32_t lb = pr.lb_in, M = left_i.sizes[lb];
-            if (!M) continue;
-            MQ_REQUIRE(g.M < (1ull << 32), "group too large for a row unit");
-            GemmRow R{};
-            R.a_src = site.rp_offset[lb] + (rpb(rbk.s, st.rbi) + uint64_t(rbk.ss) * st.K) * M;
-            R.a_si = 1;
-            R.a_sk = M;
-            R.a_scale = 1.0;
-            R.row0 = uint32_t(unit_of_row.size());
-            R.nrows = M;
-            R.c_src = g.t_off + pr.m_off;
-            R.c_split = g.bf_off + pr.m_off;
-            R.split_stride = g.M * g.K.size();
-            R.cldk = uint32_t(g.M);
-            R.ldc = M;
-            R.cmap = uint32_t(P.maps.size());
-            for (uint32_t b2 : b2s) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(b2, rbk.s, rbk.ss)));
-            unit_of_row.insert(unit_of_row.end(), (M + 15) / 16 * 16, uint32_t(P.rows.size()));
-            P.rows.push_back(R);
-        }
-        if (unit_of_row.empty() || !st.brs) return;
-        const uint32_t Mt = uint32_t(unit_of_row.size());
-        GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, uint32_t(b2s.size()) * st.brs, Mt, {sg});
-        t.cbw = st.brs;
-        t.rowunit = uint32_t(P.maps.size());
-        P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
-    };
-    for (const auto& kv : stacks) {
-        const uint32_t ci = kv.first;
-        const Stack& st = kv.second;
-        if (st.blocks.empty()) continue;
-        if (!RI.tr[ci]) {
-            std::vector<uint32_t> jb = block_order(RI, ci);
-            if (jb.empty()) continue;
-            stacked_task(st, jb, seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[ci], st.K, false, st.K));
-        } else {
-            const uint32_t cie = RI.cohort_index(RI.ket.charges[RI.ket_pos[ci]], RI.bra.charges[RI.bra_pos[ci]]);
-            MQ_REQUIRE(cie < RI.n_cohorts(), "transposed right cohort without partner");
-            for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2) {
-                if (!RI.has_block(ci, b2)) continue;
-                const uint64_t o = uint64_t(RI.offset(ci, b2));
-                stacked_task(st, {b2}, seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[cie] + o, st.brs, true,
-                                           st.K, RI.conjugate_scale(ci, b2)));
+    if (!lstep) {
+        // ---- T stage.  The psi blocks that meet one right cohort (every (lb_in, s, ss) with
+        // lc_in + q_s = rc_in, for the cohort's rc_out) are stacked into the rows of one task, so the
+        // cohort is read once per column tile for all of them; each block's rows are written to its own
+        // pair slab and keys through a row unit.  A stored cohort is one task over all its blocks
+        // (columns (j, rs) of block j); a transposed cohort is one task per block: the partner's block,
+        // transposed and scaled by its conjugation factor.
+        struct RowBlock { uint32_t pair, s, ss; };
+        struct Stack { uint32_t K = 0, brs = 0, rbi = 0; std::vector<RowBlock> blocks; };
+        std::map<uint32_t, Stack> stacks;   // right cohort -> its psi blocks (deterministic order)
+        for (uint32_t pi = 0; pi < G.pairs.size(); ++pi) {
+            const auto& pr = G.pairs[pi];
+            uint32_t lb = pr.lb_in, rb = pr.rb_out;
+            Charge lc_in = left_i.charges[lb], rc_out = right_i.charges[rb];
+            for (uint32_t s = 0; s < phys_i.size(); ++s) {
+                Charge rc_in = lc_in + phys_i.charges[s];
+                uint32_t rbi = right_i.position(rc_in);
+                uint32_t ci = RI.cohort_index(rc_in, rc_out);
+                if (rbi == right_i.size() || ci >= RI.n_cohorts()) continue;
+                Stack& st = stacks[ci];
+                st.K = right_i.sizes[rbi];
+                st.brs = right_i.sizes[rb];
+                st.rbi = rbi;
+                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) st.blocks.push_back({pi, s, ss});
             }
         }
+        // one stacked task over the given psi blocks: row units with the column keys of `b2s`
+        auto stacked_task = [&](const Stack& st, const std::vector<uint32_t>& b2s, const GemmSeg& sg) {
+            std::vector<uint32_t> unit_of_row;
+            for (const RowBlock& rbk : st.blocks) {
+                const auto& pr = G.pairs[rbk.pair];
+                const auto& g = G.groups[pr.g];
+                const uint32_t lb = pr.lb_in, M = left_i.sizes[lb];
+                if (!M) continue;
+                MQ_REQUIRE(g.M < (1ull << 32), "group too large for a row unit");
+                GemmRow R{};
+                R.a_src = site.rp_offset[lb] + (rpb(rbk.s, st.rbi) + uint64_t(rbk.ss) * st.K) * M;
+                R.c_si = 1;
+                R.a_si = 1;
+                R.a_sk = M;
+                R.a_scale = 1.0;
+                R.row0 = uint32_t(unit_of_row.size());
+                R.nrows = M;
+                R.c_src = g.t_off + pr.m_off;
+                R.c_split = g.bf_off + pr.m_off;
+                R.split_stride = g.M * g.K.size();
+                R.cldk = uint32_t(g.M);
+                R.ldc = M;
+                R.cmap = uint32_t(P.maps.size());
+                for (uint32_t b2 : b2s) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(b2, rbk.s, rbk.ss)));
+                unit_of_row.insert(unit_of_row.end(), (M + 15) / 16 * 16, uint32_t(P.rows.size()));
+                P.rows.push_back(R);
+            }
+            if (unit_of_row.empty() || !st.brs) return;
+            const uint32_t Mt = uint32_t(unit_of_row.size());
+            GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, uint32_t(b2s.size()) * st.brs, Mt, {sg});
+            t.cbw = st.brs;
+            t.rowunit = uint32_t(P.maps.size());
+            P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+        };
+        for (const auto& kv : stacks) {
+            const uint32_t ci = kv.first;
+            const Stack& st = kv.second;
+            if (st.blocks.empty()) continue;
+            if (!RI.tr[ci]) {
+                std::vector<uint32_t> jb = block_order(RI, ci);
+                if (jb.empty()) continue;
+                stacked_task(st, jb, seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[ci], st.K, false, st.K));
+            } else {
+                const uint32_t cie = RI.cohort_index(RI.ket.charges[RI.ket_pos[ci]], RI.bra.charges[RI.bra_pos[ci]]);
+                MQ_REQUIRE(cie < RI.n_cohorts(), "transposed right cohort without partner");
+                for (uint32_t b2 = 0; b2 < RI.aux_dim; ++b2) {
+                    if (!RI.has_block(ci, b2)) continue;
+                    const uint64_t o = uint64_t(RI.offset(ci, b2));
+                    stacked_task(st, {b2}, seg(BASE_IN, 0, 1, false, BASE_RIGHT, p->right->cohort_off[cie] + o, st.brs, true,
+                                               st.K, RI.conjugate_scale(ci, b2)));
+                }
+            }
+        }
+    } else {
+        // ---- T stage of the left step: X(pair (lb_bra, rb_ket), key (b1, s, ss)) = L_b1(lb_bra, lb_ket)
+        // ket(lb_ket)[:, columns (s, ss, rb_ket)], stacked per left cohort and (s, ss): its blocks b1
+        // are row units sharing the ket columns
+        for (uint32_t lb = 0; lb < left_i.size(); ++lb)
+            for (uint32_t lbk = 0; lbk < left_i.size(); ++lbk) {
+                const uint32_t cl = LI.cohort_index(left_i.charges[lb], left_i.charges[lbk]);
+                const uint32_t lsb = left_i.sizes[lb], lsk = left_i.sizes[lbk];
+                if (cl >= LI.n_cohorts() || !lsb || !lsk) continue;
+                // the cohort's blocks: (b1, A address, strides, scale)
+                struct LBlk { uint32_t b1; uint64_t a; uint32_t si, sk; double sc; };
+                std::vector<LBlk> blks;
+                if (!LI.tr[cl]) {
+                    std::vector<uint32_t> jb = block_order(LI, cl);
+                    for (uint32_t j = 0; j < jb.size(); ++j)
+                        blks.push_back({jb[j], p->left->cohort_off[cl] + uint64_t(j) * lsb * lsk, 1, lsb, 1.0});
+                } else {
+                    const uint32_t cle = LI.cohort_index(left_i.charges[lbk], left_i.charges[lb]);
+                    MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
+                    for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1)
+                        if (LI.has_block(cl, b1))
+                            blks.push_back({b1, p->left->cohort_off[cle] + uint64_t(LI.offset(cl, b1)), lsk, 1,
+                                            LI.conjugate_scale(cl, b1)});
+                }
+                if (blks.empty()) continue;
+                for (uint32_t s = 0; s < phys_i.size(); ++s) {
+                    const uint32_t rb = right_i.position(left_i.charges[lbk] + phys_i.charges[s]);
+                    if (rb == right_i.size() || !right_i.sizes[rb]) continue;
+                    auto pit = G.pair_of.find(uint64_t(lb) << 32 | rb);
+                    if (pit == G.pair_of.end()) continue;
+                    const auto& pr = G.pairs[pit->second];
+                    const auto& g = G.groups[pr.g];
+                    const uint32_t rs = right_i.sizes[rb];
+                    for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
+                        std::vector<uint32_t> unit_of_row;
+                        for (const LBlk& bk : blks) {
+                            GemmRow R{};
+                            R.a_src = bk.a;
+                            R.a_si = bk.si;
+                            R.a_sk = bk.sk;
+                            R.a_scale = bk.sc;
+                            R.row0 = uint32_t(unit_of_row.size());
+                            R.nrows = lsb;
+                            R.c_src = g.t_off + pr.m_off + uint64_t(g.kidx.at(GroupedLowering::kkey(bk.b1, s, ss))) * g.M;
+                            R.ldc = lsb;
+                            R.c_si = 1;
+                            R.cmap = MAP_NONE;
+                            unit_of_row.insert(unit_of_row.end(), (lsb + 15) / 16 * 16, uint32_t(P.rows.size()));
+                            P.rows.push_back(R);
+                        }
+                        const uint32_t Mt = uint32_t(unit_of_row.size());
+                        const uint64_t col0 = rpb(s, rb) + uint64_t(ss) * rs;
+                        GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, rs, Mt,
+                                               {seg(BASE_LEFT, 0, 1, false, BASE_IN, site.rp_offset[lbk] + col0 * lsk,
+                                                    lsk, false, lsk)});
+                        t.cbw = rs;
+                        t.rowunit = uint32_t(P.maps.size());
+                        P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+                    }
+                }
+            }
     }
     P.split_t = true;   // every T task is stacked: BF16x3 writes A' from the T epilogue
     // ---- zero chunks: the (pair, key) slabs of T_D that no T task writes, merged along the rows
@@ -1798,145 +1910,227 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         L.bfb_off = P.bf_elems;
         P.bf_elems += align32(3ull * L.K * L.N);
     }
-    // ---- final stage: one task per output block lb_out (ls_out x all its columns), K segments over
-    // the left cohorts (lc_out, lc_in).  The block's columns fall into units (s, ss2) of rs columns;
-    // unit u of segment lb_in reads its B from S_D of the pair (lb_in, rb(s)) through the unit table,
-    // so each left cohort is read once per column tile (not once per physical state).
-    for (uint32_t lbo = 0; lbo < left_i.size(); ++lbo) {
-        Charge lc_out = left_i.charges[lbo];
-        uint32_t ls_out = left_i.sizes[lbo];
-        const uint32_t Nb = site.rp_cols[lbo];
-        if (!ls_out || !Nb) continue;
-        struct Unit { uint32_t s, ss, rb, col0, rs; };
-        std::vector<Unit> us;
-        for (uint32_t s = 0; s < phys_i.size(); ++s) {
-            Charge rc_out = lc_out + phys_i.charges[s];
-            uint32_t rb = right_i.position(rc_out);
-            if (rb == right_i.size()) continue;
-            uint32_t rs = right_i.sizes[rb];
-            for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss)
-                us.push_back({s, ss, rb, uint32_t(rpb(s, rb)) + ss * rs, rs});
-        }
-        const uint32_t colunit = uint32_t(P.maps.size());
-        P.maps.resize(P.maps.size() + Nb, MQ_NONE);
-        for (uint32_t u = 0; u < us.size(); ++u)
-            for (uint32_t r = 0; r < us[u].rs; ++r) {
-                MQ_REQUIRE(us[u].col0 + r < Nb && P.maps[colunit + us[u].col0 + r] == MQ_NONE, "output column units");
-                P.maps[colunit + us[u].col0 + r] = u;
+    if (sigma) {
+        // ---- final stage: one task per output block lb_out (ls_out x all its columns), K segments over
+        // the left cohorts (lc_out, lc_in).  The block's columns fall into units (s, ss2) of rs columns;
+        // unit u of segment lb_in reads its B from S_D of the pair (lb_in, rb(s)) through the unit table,
+        // so each left cohort is read once per column tile (not once per physical state).
+        for (uint32_t lbo = 0; lbo < left_i.size(); ++lbo) {
+            Charge lc_out = left_i.charges[lbo];
+            uint32_t ls_out = left_i.sizes[lbo];
+            const uint32_t Nb = site.rp_cols[lbo];
+            if (!ls_out || !Nb) continue;
+            struct Unit { uint32_t s, ss, rb, col0, rs; };
+            std::vector<Unit> us;
+            for (uint32_t s = 0; s < phys_i.size(); ++s) {
+                Charge rc_out = lc_out + phys_i.charges[s];
+                uint32_t rb = right_i.position(rc_out);
+                if (rb == right_i.size()) continue;
+                uint32_t rs = right_i.sizes[rb];
+                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss)
+                    us.push_back({s, ss, rb, uint32_t(rpb(s, rb)) + ss * rs, rs});
             }
-        for (uint32_t n = 0; n < Nb; ++n) MQ_REQUIRE(P.maps[colunit + n] != MQ_NONE, "output column without unit");
-        // unit descriptors of one segment: the S_D block of the pair (lb_in, rb(s)) and the S_D column
-        // of every b1 block of the segment's K range
-        auto add_units = [&](uint32_t lb_in, const std::vector<uint32_t>& b1s) {
-            const uint32_t first = uint32_t(P.units.size());
-            for (const Unit& U : us) {
-                GemmUnit d{};
-                d.col0 = U.col0;
-                auto pit = G.pair_of.find(uint64_t(lb_in) << 32 | U.rb);
-                if (pit != G.pair_of.end()) {
-                    const auto& pr = G.pairs[pit->second];
-                    const auto& g = G.groups[pr.g];
-                    d.src = g.s_off + pr.m_off;
-                    d.ld = uint32_t(g.M);
-                    d.map = uint32_t(P.maps.size());
-                    d.valid = 1;
-                    for (uint32_t b1 : b1s) P.maps.push_back(g.nidx.at(GroupedLowering::kkey(b1, U.s, U.ss)));
+            const uint32_t colunit = uint32_t(P.maps.size());
+            P.maps.resize(P.maps.size() + Nb, MQ_NONE);
+            for (uint32_t u = 0; u < us.size(); ++u)
+                for (uint32_t r = 0; r < us[u].rs; ++r) {
+                    MQ_REQUIRE(us[u].col0 + r < Nb && P.maps[colunit + us[u].col0 + r] == MQ_NONE, "output column units");
+                    P.maps[colunit + us[u].col0 + r] = u;
                 }
-                P.units.push_back(d);
+            for (uint32_t n = 0; n < Nb; ++n) MQ_REQUIRE(P.maps[colunit + n] != MQ_NONE, "output column without unit");
+            // unit descriptors of one segment: the S_D block of the pair (lb_in, rb(s)) and the S_D column
+            // of every b1 block of the segment's K range
+            auto add_units = [&](uint32_t lb_in, const std::vector<uint32_t>& b1s) {
+                const uint32_t first = uint32_t(P.units.size());
+                for (const Unit& U : us) {
+                    GemmUnit d{};
+                    d.col0 = U.col0;
+                    auto pit = G.pair_of.find(uint64_t(lb_in) << 32 | U.rb);
+                    if (pit != G.pair_of.end()) {
+                        const auto& pr = G.pairs[pit->second];
+                        const auto& g = G.groups[pr.g];
+                        d.src = g.s_off + pr.m_off;
+                        d.ld = uint32_t(g.M);
+                        d.map = uint32_t(P.maps.size());
+                        d.valid = 1;
+                        for (uint32_t b1 : b1s) P.maps.push_back(g.nidx.at(GroupedLowering::kkey(b1, U.s, U.ss)));
+                    }
+                    P.units.push_back(d);
+                }
+                return first;
+            };
+            // segments: the left cohorts (lc_out, lc_in), each cut into pieces of whole blocks of about
+            // KPART columns of op(A), so that split-K parts below can be balanced however the cohort
+            // sizes are distributed
+            const uint64_t KPART = 2048;
+            std::vector<GemmSeg> segs;
+            for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
+                Charge lc_in = left_i.charges[lb];
+                uint32_t cl = LI.cohort_index(lc_out, lc_in);
+                if (cl >= LI.n_cohorts()) continue;
+                uint32_t ls_in = left_i.sizes[lb];
+                if (!ls_in) continue;
+                const uint32_t per = uint32_t(std::max<uint64_t>(1, KPART / ls_in));   // blocks per piece
+                if (!LI.tr[cl]) {
+                    std::vector<uint32_t> jb = block_order(LI, cl);
+                    for (uint32_t j0 = 0; j0 < jb.size(); j0 += per) {
+                        const uint32_t j1 = std::min<uint32_t>(uint32_t(jb.size()), j0 + per);
+                        GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cl] + uint64_t(j0) * ls_in * ls_out, ls_out, false,
+                                         BASE_WS, 0, ls_in, false, (j1 - j0) * ls_in);
+                        sg.bbh = ls_in;
+                        sg.units = add_units(lb, std::vector<uint32_t>(jb.begin() + j0, jb.begin() + j1));
+                        segs.push_back(sg);
+                    }
+                } else {
+                    // Hermitian partner: the blocks are the partner cohort's, transposed, each with its
+                    // conjugation scale (A block map + block scales)
+                    uint32_t cle = LI.cohort_index(lc_in, lc_out);
+                    MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
+                    std::vector<uint32_t> b1s;
+                    const uint32_t amap = uint32_t(P.maps.size()), ascale = uint32_t(P.scales.size());
+                    for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1) {
+                        if (!LI.has_block(cl, b1)) continue;
+                        const int64_t o = LI.offset(cl, b1);
+                        MQ_REQUIRE(o >= 0 && o < (int64_t(1) << 32), "partner block offset");
+                        b1s.push_back(b1);
+                        P.maps.push_back(uint32_t(o));
+                        P.scales.push_back(LI.conjugate_scale(cl, b1));
+                    }
+                    for (uint32_t j0 = 0; j0 < b1s.size(); j0 += per) {
+                        const uint32_t j1 = std::min<uint32_t>(uint32_t(b1s.size()), j0 + per);
+                        GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cle], ls_in, true, BASE_WS, 0, ls_in, false,
+                                         (j1 - j0) * ls_in);
+                        sg.bbh = ls_in;
+                        sg.amap = amap + j0;
+                        sg.ascale = ascale + j0;
+                        sg.units = add_units(lb, std::vector<uint32_t>(b1s.begin() + j0, b1s.begin() + j1));
+                        segs.push_back(sg);
+                    }
+                }
             }
-            return first;
-        };
-        // segments: the left cohorts (lc_out, lc_in), each cut into pieces of whole blocks of about
-        // KPART columns of op(A), so that split-K parts below can be balanced however the cohort
-        // sizes are distributed
-        const uint64_t KPART = 2048;
-        std::vector<GemmSeg> segs;
-        for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
-            Charge lc_in = left_i.charges[lb];
-            uint32_t cl = LI.cohort_index(lc_out, lc_in);
-            if (cl >= LI.n_cohorts()) continue;
-            uint32_t ls_in = left_i.sizes[lb];
-            if (!ls_in) continue;
-            const uint32_t per = uint32_t(std::max<uint64_t>(1, KPART / ls_in));   // blocks per piece
-            if (!LI.tr[cl]) {
-                std::vector<uint32_t> jb = block_order(LI, cl);
-                for (uint32_t j0 = 0; j0 < jb.size(); j0 += per) {
-                    const uint32_t j1 = std::min<uint32_t>(uint32_t(jb.size()), j0 + per);
-                    GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cl] + uint64_t(j0) * ls_in * ls_out, ls_out, false,
-                                     BASE_WS, 0, ls_in, false, (j1 - j0) * ls_in);
-                    sg.bbh = ls_in;
-                    sg.units = add_units(lb, std::vector<uint32_t>(jb.begin() + j0, jb.begin() + j1));
+            if (segs.empty()) continue;
+            // split K over several tasks when it is long, so that the few output blocks fill the GPU:
+            // each part writes its own FP64 partial block, and one gather region sums the parts into
+            // sigma in a fixed order (deterministic, no atomics)
+            uint64_t ktot = 0;
+            for (const auto& sg : segs) ktot += sg.K;
+            const uint32_t nparts = uint32_t(std::min<uint64_t>({uint64_t(segs.size()), 64, (ktot + KPART - 1) / KPART}));
+            auto emit = [&](uint8_t c_base, uint64_t c_off, size_t a0, size_t a1) {
+                std::vector<GemmSeg> part(segs.begin() + a0, segs.begin() + a1);
+                GemmTask& t = add_gemm(P.f_tasks, P.f_segs, c_base, c_off, ls_out, Nb, ls_out, part);
+                t.colunit = colunit;
+                t.nunits = uint32_t(us.size());
+            };
+            if (nparts <= 1) {
+                emit(BASE_OUT, site.rp_offset[lbo], 0, segs.size());
+                continue;
+            }
+            GatherRegion r{};
+            r.dst_off = site.rp_offset[lbo];
+            r.R = ls_out;
+            r.C = Nb;
+            r.ld_dst = ls_out;
+            r.t0 = uint32_t(P.red.src_off.size());
+            size_t a0 = 0;
+            uint64_t acc = 0;
+            for (uint32_t q = 0; q < nparts && a0 < segs.size(); ++q) {
+                // parts of about ktot / nparts each; the last takes the rest
+                const uint64_t target = (ktot * (q + 1)) / nparts;
+                size_t a1 = a0;
+                while (a1 < segs.size() && (acc < target || a1 == a0)) acc += segs[a1++].K;
+                if (q + 1 == nparts) {
+                    while (a1 < segs.size()) acc += segs[a1++].K;
+                }
+                const uint64_t off = P.part_elems;
+                P.part_elems += align32(uint64_t(ls_out) * Nb);
+                emit(BASE_BRA, off, a0, a1);
+                P.red.src_off.push_back(off);
+                P.red.alpha.push_back(1.0);
+                a0 = a1;
+            }
+            r.t1 = uint32_t(P.red.src_off.size());
+            P.red.regions.push_back(r);
+        }
+    } else {
+        // ---- final stage of a step: one stacked task per stored cohort of the new boundary, its
+        // blocks as row units, one K segment per (bra sector, s, ss) unit of columns; each segment
+        // reads the S_D slab of every block's key through its A block map.
+        //   right step: R'_b1(ket, bra) = sum S_b1(ket; s, ss, r) bra(bra; s, ss, r)
+        //   left step:  L'_b2(bra, ket)^T = sum S_b2(l; s, ss, ket)^T bra(l; s, ss, bra), written
+        //               through the units' row stride
+        for (uint32_t cn = 0; cn < NI.n_cohorts(); ++cn) {
+            if (NI.tr[cn]) continue;
+            const uint32_t a_sec = NI.bra_pos[cn], b_sec = NI.ket_pos[cn];
+            // rows of each block (ket sector size for both steps), and the output's other dimension
+            const uint32_t rows_u = rstep ? left_i.sizes[a_sec] : right_i.sizes[b_sec];
+            const uint32_t ncol = rstep ? left_i.sizes[b_sec] : right_i.sizes[a_sec];
+            if (!rows_u || !ncol) continue;
+            const uint64_t bs = NI.block_size(cn);
+            std::vector<uint32_t> bidx;   // MPO index of each stored block, in storage order
+            std::vector<uint64_t> boff;
+            for (uint32_t b = 0; b < NI.aux_dim; ++b)
+                if (NI.has_block(cn, b)) bidx.push_back(b), boff.push_back(uint64_t(NI.offset(cn, b)));
+            if (bidx.empty()) continue;
+            MQ_REQUIRE(bs == uint64_t(rows_u) * ncol, "new cohort block shape");
+            std::vector<uint32_t> unit_of_row;
+            for (uint32_t j = 0; j < bidx.size(); ++j) {
+                GemmRow R{};
+                R.a_scale = 1.0;
+                R.row0 = uint32_t(unit_of_row.size());
+                R.nrows = rows_u;
+                R.jblk = j;
+                R.c_src = (*step->out_off)[cn] + boff[j];
+                R.ldc = rstep ? rows_u : 1;      // right: (ket x bra) column-major; left: the transpose
+                R.c_si = rstep ? 1 : ncol;
+                R.cmap = MAP_NONE;
+                unit_of_row.insert(unit_of_row.end(), (rows_u + 15) / 16 * 16, uint32_t(P.rows.size()));
+                P.rows.push_back(R);
+            }
+            std::vector<GemmSeg> segs;
+            for (uint32_t s = 0; s < phys_i.size(); ++s) {
+                // the pair whose S_D slab this unit of columns reads, and the bra block
+                uint32_t pl, pr_sec, bra_blk;
+                uint64_t col0;
+                if (rstep) {
+                    const uint32_t rb = right_i.position(left_i.charges[b_sec] + phys_i.charges[s]);
+                    if (rb == right_i.size()) continue;
+                    pl = a_sec, pr_sec = rb, bra_blk = b_sec;
+                    col0 = rpb(s, rb);
+                } else {
+                    const uint32_t lb = left_i.position(right_i.charges[a_sec] - phys_i.charges[s]);
+                    if (lb == left_i.size()) continue;
+                    pl = lb, pr_sec = b_sec, bra_blk = lb;
+                    col0 = rpb(s, a_sec);
+                }
+                auto pit = G.pair_of.find(uint64_t(pl) << 32 | pr_sec);
+                if (pit == G.pair_of.end()) continue;
+                const auto& pr = G.pairs[pit->second];
+                const auto& g = G.groups[pr.g];
+                const uint32_t lsb = left_i.sizes[bra_blk];
+                const uint32_t width = rstep ? right_i.sizes[pr_sec] : right_i.sizes[a_sec];   // unit columns
+                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
+                    const uint32_t amap = uint32_t(P.maps.size());
+                    for (uint32_t b : bidx) P.maps.push_back(g.nidx.at(GroupedLowering::kkey(b, s, ss)));
+                    const uint64_t c0 = col0 + uint64_t(ss) * width;
+                    GemmSeg sg;
+                    if (rstep)   // A = S slab (ket rows x r columns), B = bra columns transposed
+                        sg = seg(BASE_WS, g.s_off + pr.m_off, rows_u, false, BASE_BRA, site.rp_offset[bra_blk] + c0 * lsb,
+                                 lsb, true, width);
+                    else         // A = S slab transposed (ket columns x l rows), B = bra columns
+                        sg = seg(BASE_WS, g.s_off + pr.m_off, lsb, true, BASE_BRA, site.rp_offset[bra_blk] + c0 * lsb,
+                                 lsb, false, lsb);
+                    sg.amap = amap;
+                    sg.bldk = g.M;
                     segs.push_back(sg);
                 }
-            } else {
-                // Hermitian partner: the blocks are the partner cohort's, transposed, each with its
-                // conjugation scale (A block map + block scales)
-                uint32_t cle = LI.cohort_index(lc_in, lc_out);
-                MQ_REQUIRE(cle < LI.n_cohorts(), "transposed left cohort without partner");
-                std::vector<uint32_t> b1s;
-                const uint32_t amap = uint32_t(P.maps.size()), ascale = uint32_t(P.scales.size());
-                for (uint32_t b1 = 0; b1 < LI.aux_dim; ++b1) {
-                    if (!LI.has_block(cl, b1)) continue;
-                    const int64_t o = LI.offset(cl, b1);
-                    MQ_REQUIRE(o >= 0 && o < (int64_t(1) << 32), "partner block offset");
-                    b1s.push_back(b1);
-                    P.maps.push_back(uint32_t(o));
-                    P.scales.push_back(LI.conjugate_scale(cl, b1));
-                }
-                for (uint32_t j0 = 0; j0 < b1s.size(); j0 += per) {
-                    const uint32_t j1 = std::min<uint32_t>(uint32_t(b1s.size()), j0 + per);
-                    GemmSeg sg = seg(BASE_LEFT, p->left->cohort_off[cle], ls_in, true, BASE_WS, 0, ls_in, false,
-                                     (j1 - j0) * ls_in);
-                    sg.bbh = ls_in;
-                    sg.amap = amap + j0;
-                    sg.ascale = ascale + j0;
-                    sg.units = add_units(lb, std::vector<uint32_t>(b1s.begin() + j0, b1s.begin() + j1));
-                    segs.push_back(sg);
-                }
             }
+            if (segs.empty()) continue;
+            const uint32_t Mt = uint32_t(unit_of_row.size());
+            GemmTask& t = add_gemm(P.f_tasks, P.f_segs, BASE_OUT, 0, Mt, ncol, Mt, segs);
+            t.cbw = ncol;
+            t.rowunit = uint32_t(P.maps.size());
+            P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
         }
-        if (segs.empty()) continue;
-        // split K over several tasks when it is long, so that the few output blocks fill the GPU:
-        // each part writes its own FP64 partial block, and one gather region sums the parts into
-        // sigma in a fixed order (deterministic, no atomics)
-        uint64_t ktot = 0;
-        for (const auto& sg : segs) ktot += sg.K;
-        const uint32_t nparts = uint32_t(std::min<uint64_t>({uint64_t(segs.size()), 64, (ktot + KPART - 1) / KPART}));
-        auto emit = [&](uint8_t c_base, uint64_t c_off, size_t a0, size_t a1) {
-            std::vector<GemmSeg> part(segs.begin() + a0, segs.begin() + a1);
-            GemmTask& t = add_gemm(P.f_tasks, P.f_segs, c_base, c_off, ls_out, Nb, ls_out, part);
-            t.colunit = colunit;
-            t.nunits = uint32_t(us.size());
-        };
-        if (nparts <= 1) {
-            emit(BASE_OUT, site.rp_offset[lbo], 0, segs.size());
-            continue;
-        }
-        GatherRegion r{};
-        r.dst_off = site.rp_offset[lbo];
-        r.R = ls_out;
-        r.C = Nb;
-        r.ld_dst = ls_out;
-        r.t0 = uint32_t(P.red.src_off.size());
-        size_t a0 = 0;
-        uint64_t acc = 0;
-        for (uint32_t q = 0; q < nparts && a0 < segs.size(); ++q) {
-            // parts of about ktot / nparts each; the last takes the rest
-            const uint64_t target = (ktot * (q + 1)) / nparts;
-            size_t a1 = a0;
-            while (a1 < segs.size() && (acc < target || a1 == a0)) acc += segs[a1++].K;
-            if (q + 1 == nparts) {
-                while (a1 < segs.size()) acc += segs[a1++].K;
-            }
-            const uint64_t off = P.part_elems;
-            P.part_elems += align32(uint64_t(ls_out) * Nb);
-            emit(BASE_BRA, off, a0, a1);
-            P.red.src_off.push_back(off);
-            P.red.alpha.push_back(1.0);
-            a0 = a1;
-        }
-        r.t1 = uint32_t(P.red.src_off.size());
-        P.red.regions.push_back(r);
     }
     if (adjoint) {
         // ---- term map: CSR over the MPO terms, entries in their accumulation order
@@ -1984,6 +2178,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                             R.nrows = ls_in;
                             R.c_src = g.s_off + pr.m_off + uint64_t(g.nidx.at(GroupedLowering::kkey(b1, s, ss))) * g.M;
                             R.ldc = ls_in;
+                            R.c_si = 1;
                             R.cmap = MAP_NONE;
                             unit_of_row.insert(unit_of_row.end(), (ls_in + 15) / 16 * 16, uint32_t(P.rows.size()));
                             P.rows.push_back(R);
@@ -2016,6 +2211,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
     }
     P.ws_elems = ws;
     p->flops = G.ref_flops;
+    if (!sigma) return;
     // T and L reference flops
     for (auto& pr : G.pairs) {
         uint32_t lb = pr.lb_in, rb = pr.rb_out;
@@ -2540,8 +2736,8 @@ extern "C" {
 // overlap_mpo_left_step (move_boundary.hpp:147-279): T_ti = L_b * ket slab (create_T_left,
 // tasks.cpp:545-648), S_left gather (create_s, tasks.cpp:425-464), L'(ci) = bra_lp^T * S (prop_l,
 // tasks.cpp:198-241).
-int mq_boundary_left_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_lp,
-                          const double* ket_rp, const mq_boundary* left, int symmetric, mq_boundary** out) {
+static int left_step_schedule(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_lp,
+                              const double* ket_rp, const mq_boundary* left, int symmetric, mq_boundary** out) {
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && bra_lp && ket_rp && left && out, "bad argument");
         SiteLayout L(*site);
@@ -2633,8 +2829,8 @@ int mq_boundary_left_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_de
 
 // overlap_mpo_right_step (move_boundary.hpp:281-404): T as in sigma, S = create_s_r, then
 // R'_b = S_b * bra_rp^T for every b-row (prop_r, tasks.cpp:243-284).
-int mq_boundary_right_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_rp,
-                           const double* ket_rp, const mq_boundary* right, int symmetric, mq_boundary** out) {
+static int right_step_schedule(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_rp,
+                               const double* ket_rp, const mq_boundary* right, int symmetric, mq_boundary** out) {
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && bra_rp && ket_rp && right && out, "bad argument");
         SiteLayout L(*site);
@@ -2674,6 +2870,108 @@ int mq_boundary_right_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_d
         HIPCHK(hipStreamSynchronize(ctx->stream));
         *out = nb.release();
     });
+}
+
+// Grouped boundary step (abelian groups): the new boundary's index from the index-only builder,
+// then T (psi R or L psi), S_D = T_D W_D and the final contraction with the bra as stacked tasks.
+// bra_rp: the bra in the right-paired layout.
+static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_rp,
+                        const double* ket_rp, const mq_boundary* bnd, int symmetric, mq_boundary** out) {
+    return guarded([&] {
+        SiteLayout L(*site);
+        Mpo W(*mpo);
+        auto nb = std::make_unique<mq_boundary>();
+        nb->ctx = ctx;
+        nb->index = kind == Schedule::LEFT ? left_step_index(L, W, bnd->index, symmetric != 0)
+                                           : right_step_index(L, W, bnd->index, symmetric != 0);
+        nb->layout();
+        nb->data.alloc(std::max<uint64_t>(nb->total, 1) * sizeof(double));
+        HIPCHK(launch_fill(ctx->stream, nb->ptr(), std::max<uint64_t>(nb->total, 1), 0.0));
+        mq_plan p;
+        p.ctx = ctx;
+        p.left = kind == Schedule::LEFT ? bnd : nullptr;
+        p.right = kind == Schedule::RIGHT ? bnd : nullptr;
+        std::vector<double> ws_init;
+        uint64_t ws_init_off = 0;
+        const GroupedStep spec{kind, &nb->index, &nb->cohort_off};
+        lower_sigma_grouped(&p, L, W, ws_init, ws_init_off, false, &spec);
+        StagePlan& P = p.st;
+        P.blas = ctx->blas;
+        P.finish_tables();
+        P.sizes[BASE_WS] = P.ws_elems;
+        P.sizes[BASE_IN] = P.sizes[BASE_BRA] = L.rp_total;
+        P.sizes[BASE_OUT] = nb->total;
+        P.sizes[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = bnd->total;
+        P.validate();
+        P.upload_all(ctx->stream);
+        // workspace: the context's step buffer (cleared whenever it grows), W_D copied in
+        const size_t need = std::max<uint64_t>(P.ws_elems, 32) * sizeof(double);
+        if (need > ctx->step_ws.bytes) ctx->step_ws.alloc_zeroed(need, ctx->stream);
+        double* ws = ctx->step_ws.as<double>();
+        if (!ws_init.empty())
+            HIPCHK(hipMemcpyAsync(ws + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, ctx->stream));
+        Bases bs{};
+        bs.p[BASE_WS] = ws;
+        bs.p[BASE_IN] = const_cast<double*>(ket_rp);
+        bs.p[BASE_OUT] = nb->ptr();
+        bs.p[BASE_BRA] = const_cast<double*>(bra_rp);
+        bs.p[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = const_cast<double*>(bnd->ptr());
+        BLASCHK(rocblas_set_stream(ctx->blas, ctx->stream));
+        P.run(ctx->stream, bs, MODE_F64);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *out = nb.release();
+    });
+}
+
+int mq_boundary_left_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_lp,
+                             const double* ket_rp, const mq_boundary* left, int symmetric, int algorithm,
+                             mq_boundary** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && site && mpo && bra_lp && ket_rp && left && out, "bad argument");
+        SiteLayout L(*site);
+        if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
+        int rc;
+        if (algorithm == MQ_SIGMA_SCHEDULE) {
+            rc = left_step_schedule(ctx, site, mpo, bra_lp, ket_rp, left, symmetric, out);
+        } else {
+            MQ_REQUIRE(algorithm == MQ_SIGMA_GROUPED, "unknown step algorithm");
+            // the grouped step reads the bra right-paired
+            DevBuf bra_rp;
+            bra_rp.alloc(std::max<uint64_t>(L.rp_total, 1) * sizeof(double));
+            rc = mq_reshape_left_to_right(ctx, site, bra_lp, bra_rp.as<double>());
+            if (rc == MQ_OK) rc = step_grouped(ctx, Schedule::LEFT, site, mpo, bra_rp.as<double>(), ket_rp, left, symmetric, out);
+        }
+        if (rc != MQ_OK) fail(rc, g_err);
+    });
+}
+
+int mq_boundary_left_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_lp,
+                          const double* ket_rp, const mq_boundary* left, int symmetric, mq_boundary** out) {
+    return mq_boundary_left_step_ex(ctx, site, mpo, bra_lp, ket_rp, left, symmetric, MQ_SIGMA_AUTO, out);
+}
+
+int mq_boundary_right_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_rp,
+                              const double* ket_rp, const mq_boundary* right, int symmetric, int algorithm,
+                              mq_boundary** out) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && site && mpo && bra_rp && ket_rp && right && out, "bad argument");
+        SiteLayout L(*site);
+        if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
+        int rc;
+        if (algorithm == MQ_SIGMA_SCHEDULE) {
+            rc = right_step_schedule(ctx, site, mpo, bra_rp, ket_rp, right, symmetric, out);
+        } else {
+            MQ_REQUIRE(algorithm == MQ_SIGMA_GROUPED, "unknown step algorithm");
+            rc = step_grouped(ctx, Schedule::RIGHT, site, mpo, bra_rp, ket_rp, right, symmetric, out);
+        }
+        if (rc != MQ_OK) fail(rc, g_err);
+    });
+}
+
+int mq_boundary_right_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_rp,
+                           const double* ket_rp, const mq_boundary* right, int symmetric, mq_boundary** out) {
+    return mq_boundary_right_step_ex(ctx, site, mpo, bra_rp, ket_rp, right, symmetric, MQ_SIGMA_AUTO, out);
 }
 
 // ---------------------------------------------------------------------------- vectors

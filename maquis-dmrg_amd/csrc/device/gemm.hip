@@ -157,7 +157,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     // stacked task: the A row of this thread (m = tid % BM for every staged element of a
     // non-transposed A) and its unit's column stride
     uint64_t a_row = 0;
-    uint32_t a_ld = 0, a_ok = 0;
+    uint32_t a_ld = 0, a_ok = 0, a_i = 0, a_j = 0;
     double a_rscale = 1.0;
     if (stacked) {
         const uint32_t m = m0 + uint32_t(tid % BM);
@@ -167,6 +167,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
         a_row = R.a_src + uint64_t(i) * R.a_si;
         a_ld = R.a_sk;
         a_rscale = R.a_scale;
+        a_i = i;
+        a_j = R.jblk;
     }
     // cursor over the K steps of all segments: (sgi, k0) with k0 < segs[sgi].K
     uint32_t sgi = tk.seg0, k0 = 0;
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     bool r_trans_a = false, r_kfast_b = true;
     double a_scale = 1.0;   // block scale of a remapped transposed A (all of a thread's elements share k)
     auto load_a = [&](const TA* __restrict__ A) {
-        if (sg.amap != MAP_NONE) {
+        if (sg.amap != MAP_NONE && !stacked) {
             const uint32_t gk = k0 + uint32_t(tid % BK);   // trans_a: k = e % BK = tid % BK
             const uint32_t blk = min(gk, sg.K - 1) / sg.bbh;
             const uint64_t boff = uint64_t(maps[sg.amap + blk]) + (gk - blk * sg.bbh);
@@ -208,13 +210,19 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
         a_scale = 1.0;
         if (stacked) {
             a_scale = a_rscale;
+            // this thread's row: unit strides, or the segment's block of the unit (A block map)
+            uint64_t row = a_row, ldk = a_ld;
+            if (sg.amap != MAP_NONE) {
+                row = uint64_t(maps[sg.amap + a_j]) * sg.bldk + uint64_t(a_i) * (sg.trans_a ? sg.lda : 1u);
+                ldk = sg.trans_a ? 1u : sg.lda;
+            }
 #pragma unroll
             for (int i = 0; i < NA; ++i) {
                 const int e = tid + GEMM_THREADS * i;
                 const uint32_t gk = k0 + e / BM;
                 const uint32_t ok = a_ok & uint32_t(gk < sg.K);
                 oka |= ok << i;
-                va[i] = A[(a_row + uint64_t(gk) * a_ld) & (0ull - uint64_t(ok))];
+                va[i] = A[(row + uint64_t(gk) * ldk) & (0ull - uint64_t(ok))];
             }
             return;
         }
@@ -254,7 +262,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     auto gload = [&]() {
         const TA* __restrict__ A = reinterpret_cast<const TA*>(bases.p[sg.a_base]) + sg.a_off;
         const TB* __restrict__ B = reinterpret_cast<const TB*>(bases.p[sg.b_base]) + sg.b_off;
-        r_trans_a = sg.trans_a;
+        r_trans_a = sg.trans_a && !stacked;   // stacked tasks stage A rows m-fast in every case
         r_kfast_b = sg.bmap != MAP_NONE || sg.units != MAP_NONE || !sg.trans_b;
         r_alpha = sg.alpha;
         oka = okb = 0;
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             if constexpr (MODE == MODE_F32 && (SPLIT || std::is_same<CT, float>::value)) {
                 const uint32_t gm0 = m0 + wm + 16 * i + 4 * lk;
                 const uint32_t li0 = gm0 - R[i].row0;
-                if (!tk.beta && gm0 + 3 < M && li0 + 3 < R[i].nrows) {
+                if (!tk.beta && R[i].c_si == 1 && gm0 + 3 < M && li0 + 3 < R[i].nrows) {
 #pragma unroll
                     for (int j = 0; j < WN; ++j) {
                         if (n0 + wn + 16 * j + l16 >= N) continue;
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                         Cm[coff[j] + 2 * sd] = hi;
                     }
                 } else {
-                    CT* __restrict__ Cm = C + R[i].c_src + li;
+                    CT* __restrict__ Cm = C + R[i].c_src + uint64_t(li) * R[i].c_si;
 #pragma unroll
                     for (int j = 0; j < WN; ++j) {
                         if (n0 + wn + 16 * j + l16 >= N) continue;

@@ -57,17 +57,22 @@ struct GemmTask {
 // of 16 (row0 too is a multiple of 16, so every 16-row MFMA block lies in one unit); row m belongs
 // to unit u = maps[rowunit + m], with descriptor R = row table[u] and local row i = m - R.row0:
 //     A(m, k)  = R.a_scale * A[a_off + R.a_src + i * R.a_si + k * R.a_sk]   i < R.nrows, else 0
-//     C(m, n)  = C[c_off + R.c_src + maps[R.cmap + n / cbw] * R.cldk + (n % cbw) * R.ldc + i]
-// (R.cmap == MAP_NONE: C[c_off + R.c_src + n * R.ldc + i]); padding rows are not stored.  The
-// task's own cmap, ldc and m_off are unused.  Tiles of a stacked task are numbered row tile
-// fastest: the row tiles that read one B panel run next to each other and share it in L2.
+//     C(m, n)  = C[c_off + R.c_src + maps[R.cmap + n / cbw] * R.cldk + (n % cbw) * R.ldc + i * R.c_si]
+// (R.cmap == MAP_NONE: C[c_off + R.c_src + n * R.ldc + i * R.c_si]); padding rows are not stored.
+// A segment with amap != MAP_NONE addresses A per unit through its own block map instead: unit j
+// = R.jblk reads block maps[amap + j] (stride bldk) of the segment's operand, op(A) with the
+// segment's lda / trans_a,
+//     A(m, k)  = R.a_scale * A[a_off + maps[amap + j] * bldk + (trans_a ? i * lda + k : i + k * lda)]
+// so one task can sum segments whose blocks sit at per-unit, per-segment places.  The task's own
+// cmap, ldc and m_off are unused.  Tiles of a stacked task are numbered row tile fastest: the row
+// tiles that read one B panel run next to each other and share it in L2.
 // Split output (C type SplitBf16, stacked tasks only): the element goes to three bf16 copies,
 //     C[c_off + R.c_split + o] = hi,  C[... + o + R.split_stride] = lo,  C[... + o + 2 R.split_stride] = hi
 // with o the offset above without c_src, hi = bf16(x), lo = bf16(x - hi).
 struct GemmRow {
     uint64_t a_src, c_src, c_split, split_stride;
     double a_scale;
-    uint32_t a_si, a_sk, row0, cldk, ldc, cmap, nrows, pad;
+    uint32_t a_si, a_sk, row0, cldk, ldc, cmap, nrows, jblk, c_si, pad;
 };
 
 // C storage tag of the split output (three bf16 runs [hi | lo | hi]: the K-concatenated left

@@ -142,6 +142,10 @@ SIGNATURES = {
     "mq_plan_destroy": (C.c_int, [P]),
     "mq_boundary_left_step": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, P, C.c_int, PP]),
     "mq_boundary_right_step": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, P, C.c_int, PP]),
+    "mq_boundary_left_step_ex": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, P, C.c_int,
+                                           C.c_int, PP]),
+    "mq_boundary_right_step_ex": (C.c_int, [P, C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc), P, P, P, C.c_int,
+                                            C.c_int, PP]),
     "mq_vec_dot": (C.c_int, [P, C.c_uint64, P, P, C.POINTER(C.c_double)]),
     "mq_vec_nrm2": (C.c_int, [P, C.c_uint64, P, C.POINTER(C.c_double)]),
     "mq_vec_axpy": (C.c_int, [P, C.c_uint64, C.c_double, P, P]),

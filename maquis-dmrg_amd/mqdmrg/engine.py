@@ -391,17 +391,20 @@ def apply_batched(engine, plans, psis, sigmas, precision=PREC_F64):
     check(lib().mq_sigma_apply_batched(engine.ctx, n, hp, xp, yp, int(precision)))
 
 
-def left_step(engine, site, mpo_desc, bra_lp, ket_rp, left, symmetric=True):
+def left_step(engine, site, mpo_desc, bra_lp, ket_rp, left, symmetric=True, algorithm=SIGMA_AUTO):
+    """overlap_mpo_left_step; algorithm SIGMA_AUTO (grouped for abelian groups), SIGMA_SCHEDULE or
+    SIGMA_GROUPED."""
     h = C.c_void_p()
-    check(lib().mq_boundary_left_step(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_lp.data_ptr(),
-                                      ket_rp.data_ptr(), left.h, int(symmetric), C.byref(h)))
+    check(lib().mq_boundary_left_step_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_lp.data_ptr(),
+                                         ket_rp.data_ptr(), left.h, int(symmetric), int(algorithm), C.byref(h)))
     return DeviceBoundary(engine, h)
 
 
-def right_step(engine, site, mpo_desc, bra_rp, ket_rp, right, symmetric=True):
+def right_step(engine, site, mpo_desc, bra_rp, ket_rp, right, symmetric=True, algorithm=SIGMA_AUTO):
+    """overlap_mpo_right_step; algorithm as for left_step."""
     h = C.c_void_p()
-    check(lib().mq_boundary_right_step(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_rp.data_ptr(),
-                                       ket_rp.data_ptr(), right.h, int(symmetric), C.byref(h)))
+    check(lib().mq_boundary_right_step_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_rp.data_ptr(),
+                                          ket_rp.data_ptr(), right.h, int(symmetric), int(algorithm), C.byref(h)))
     return DeviceBoundary(engine, h)
 
 

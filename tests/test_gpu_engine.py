@@ -9,6 +9,7 @@ import torch
 from chain import SiteProblem
 from oracle import oracle as O
 from mqdmrg import engine as E, mps
+from mqdmrg.symmetry import SiteLayout
 
 pytestmark = pytest.mark.gpu
 
@@ -177,8 +178,14 @@ def _cmp_boundaries(hb, db):
     assert np.sqrt(num / max(den, 1e-300)) <= SIGMA_RTOL
 
 
-@pytest.mark.parametrize("model,symmetric", [("heisenberg", True), ("heisenberg", False), ("hubbard", True)])
-def test_left_step_matches_oracle(eng, model, symmetric):
+STEP_ALGS = [E.SIGMA_SCHEDULE, E.SIGMA_GROUPED]
+
+
+@pytest.mark.parametrize("algorithm", STEP_ALGS)
+@pytest.mark.parametrize("model,symmetric", [("heisenberg", True), ("heisenberg", False), ("hubbard", True),
+                                             ("hubbard", False), ("qc", True), ("bosehubbard", True)])
+def test_left_step_matches_oracle(eng, model, symmetric, algorithm):
+    print("case", model, symmetric, algorithm, flush=True)
     p = SiteProblem(model, L=8, site=4, Mmax=8, seed=11)
     k = 4
     lay = p.layouts[k]
@@ -187,22 +194,54 @@ def test_left_step_matches_oracle(eng, model, symmetric):
     bra_lp = mps.rp_to_lp(lay, p.vecs[k])
     ref = O.left_step(sd, md, bra_lp, p.vecs[k], p.Lb, symmetric)
     Lb = upload(eng, p.Lb)
-    got = E.left_step(eng, sd, md, dev(bra_lp), dev(p.vecs[k]), Lb, symmetric)
+    got = E.left_step(eng, sd, md, dev(bra_lp), dev(p.vecs[k]), Lb, symmetric, algorithm=algorithm)
     _cmp_boundaries(ref, got)
 
 
-@pytest.mark.parametrize("model", ["heisenberg", "hubbard"])
-def test_right_step_matches_oracle(eng, model):
-    p = SiteProblem(model, L=8, site=3, Mmax=8, seed=12)
-    k = 4   # p.Rb is the boundary of sites >= 5
+@pytest.mark.parametrize("algorithm", STEP_ALGS)
+@pytest.mark.parametrize("model,site", [("heisenberg", 3), ("hubbard", 3), ("qc", 3), ("bosehubbard", 3),
+                                        ("heisenberg", 0), ("qc", 5)])
+def test_right_step_matches_oracle(eng, model, site, algorithm):
+    print("case", model, site, algorithm, flush=True)
+    p = SiteProblem(model, L=8, site=site, Mmax=8, seed=12)
+    k = site + 1   # p.Rb is the boundary of sites >= site + 2
     lay = p.layouts[k]
     sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
     md = E.MpoDesc(p.mpo[k])
     ref = O.right_step(sd, md, p.vecs[k], p.vecs[k], p.Rb, True)
     Rb = upload(eng, p.Rb)
     v = dev(p.vecs[k])
-    got = E.right_step(eng, sd, md, v, v, Rb, True)
+    got = E.right_step(eng, sd, md, v, v, Rb, True, algorithm=algorithm)
     _cmp_boundaries(ref, got)
+
+
+@pytest.mark.parametrize("model,L,m", [("qc", 12, 128), ("hubbard", 16, 256)])
+def test_grouped_steps_match_schedule_steps_at_synthetic_shapes(eng, model, L, m):
+    """Middle-of-chain shapes: the grouped left and right steps equal the schedule steps (random
+    boundary data and site tensor; the oracle is too slow here, the schedule path is pinned by it
+    at the small cases above)."""
+    from mqdmrg.workloads import SyntheticSite
+    print("case", model, L, m, flush=True)
+    ws = SyntheticSite(model, L, L // 2 - 1, m, seed=3)
+    sym, phys = ws.basis.sym, ws.basis.phys
+    rng = np.random.default_rng(1)
+    Lb = E.DeviceBoundary.upload(eng, ws.left_index, ws.random_cohorts(ws.left_index))
+    Rb = E.DeviceBoundary.upload(eng, ws.right_index, ws.random_cohorts(ws.right_index))
+    for k, side in ((ws.site, "left"), (ws.site + 1, "right")):
+        sd = E.SiteDesc(sym, ws.bonds[k], phys, ws.bonds[k + 1])
+        md = E.MpoDesc(ws.mpo[k])
+        lay = SiteLayout(ws.bonds[k], phys, ws.bonds[k + 1])
+        dv = dev(rng.standard_normal(max(lay.rp_total, 1)))
+        if side == "left":
+            lp = torch.zeros(max(lay.lp_total, 1), dtype=torch.float64, device="cuda:0")
+            E.reshape_right_to_left(eng, sd, dv, lp)
+            a = E.left_step(eng, sd, md, lp, dv, Lb, True, algorithm=E.SIGMA_SCHEDULE)
+            b = E.left_step(eng, sd, md, lp, dv, Lb, True, algorithm=E.SIGMA_GROUPED)
+        else:
+            a = E.right_step(eng, sd, md, dv, dv, Rb, True, algorithm=E.SIGMA_SCHEDULE)
+            b = E.right_step(eng, sd, md, dv, dv, Rb, True, algorithm=E.SIGMA_GROUPED)
+        print("  ", side, "cohorts", a.index.n_cohorts, flush=True)
+        _cmp_boundaries(O.HostBoundary(a.index, a.download()), b)
 
 
 def test_reshape_roundtrip(eng):
