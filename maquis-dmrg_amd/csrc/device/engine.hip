@@ -1511,6 +1511,9 @@ struct GroupedStep {
     int kind;
     const BoundaryIndex* out;
     const std::vector<uint64_t>* out_off;
+    // only sector pairs whose ket sector (right step: left sector of the pair; left step: right
+    // sector) lies in [lo, hi): a step is lowered in such chunks to bound its workspace
+    uint32_t lo = 0, hi = 0xffffffffu;
 };
 
 void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::vector<double>& ws_init,
@@ -1556,8 +1559,10 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
     // ---- pairs and groups
     for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
         Charge lc_in = left_i.charges[lb];
+        if (rstep && (lb < step->lo || lb >= step->hi)) continue;
         for (uint32_t rb = 0; rb < right_i.size(); ++rb) {
             Charge rc_out = right_i.charges[rb];
+            if (lstep && (rb < step->lo || rb >= step->hi)) continue;
             // keys of the pair: K = (b, s_in, ss) of the boundary multiplied in the T stage, N = (b,
             // s_out, ss) of the boundary the final stage contracts with (sigma: L; steps: the new
             // boundary's stored cohorts).  For the left step the pair is (bra left sector, ket right
@@ -2061,6 +2066,8 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         for (uint32_t cn = 0; cn < NI.n_cohorts(); ++cn) {
             if (NI.tr[cn]) continue;
             const uint32_t a_sec = NI.bra_pos[cn], b_sec = NI.ket_pos[cn];
+            const uint32_t ket_sec = rstep ? a_sec : b_sec;
+            if (ket_sec < step->lo || ket_sec >= step->hi) continue;
             // rows of each block (ket sector size for both steps), and the output's other dimension
             const uint32_t rows_u = rstep ? left_i.sizes[a_sec] : right_i.sizes[b_sec];
             const uint32_t ncol = rstep ? left_i.sizes[b_sec] : right_i.sizes[a_sec];
@@ -2887,39 +2894,61 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
         nb->layout();
         nb->data.alloc(std::max<uint64_t>(nb->total, 1) * sizeof(double));
         HIPCHK(launch_fill(ctx->stream, nb->ptr(), std::max<uint64_t>(nb->total, 1), 0.0));
-        mq_plan p;
-        p.ctx = ctx;
-        p.left = kind == Schedule::LEFT ? bnd : nullptr;
-        p.right = kind == Schedule::RIGHT ? bnd : nullptr;
-        std::vector<double> ws_init;
-        uint64_t ws_init_off = 0;
-        const GroupedStep spec{kind, &nb->index, &nb->cohort_off};
-        lower_sigma_grouped(&p, L, W, ws_init, ws_init_off, false, &spec);
-        StagePlan& P = p.st;
-        P.blas = ctx->blas;
-        P.finish_tables();
-        P.sizes[BASE_WS] = P.ws_elems;
-        P.sizes[BASE_IN] = P.sizes[BASE_BRA] = L.rp_total;
-        P.sizes[BASE_OUT] = nb->total;
-        P.sizes[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = bnd->total;
-        P.validate();
-        P.upload_all(ctx->stream);
-        // workspace: the context's step buffer (cleared whenever it grows), W_D copied in
-        const size_t need = std::max<uint64_t>(P.ws_elems, 32) * sizeof(double);
-        if (need > ctx->step_ws.bytes) ctx->step_ws.alloc_zeroed(need, ctx->stream);
-        double* ws = ctx->step_ws.as<double>();
-        if (!ws_init.empty())
-            HIPCHK(hipMemcpyAsync(ws + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
-                                  hipMemcpyHostToDevice, ctx->stream));
-        Bases bs{};
-        bs.p[BASE_WS] = ws;
-        bs.p[BASE_IN] = const_cast<double*>(ket_rp);
-        bs.p[BASE_OUT] = nb->ptr();
-        bs.p[BASE_BRA] = const_cast<double*>(bra_rp);
-        bs.p[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = const_cast<double*>(bnd->ptr());
-        BLASCHK(rocblas_set_stream(ctx->blas, ctx->stream));
-        P.run(ctx->stream, bs, MODE_F64);
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        // The step's intermediates (T_D, S_D) are several times the new boundary.  The ket sectors
+        // are processed in chunks whose workspace fits a budget: 40% of the free device memory
+        // (at least 1 GiB); a chunk that does not fit is halved.
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t budget = std::max<uint64_t>(uint64_t(1) << 27, uint64_t(0.4 * double(free_b + ctx->step_ws.bytes)) / 8);
+        const uint32_t n_ket = kind == Schedule::LEFT ? L.right_i.size() : L.left_i.size();
+        std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, n_ket}};
+        while (!todo.empty()) {
+            const auto range = todo.back();
+            todo.pop_back();
+            if (range.first >= range.second) continue;
+            mq_plan p;
+            p.ctx = ctx;
+            p.left = kind == Schedule::LEFT ? bnd : nullptr;
+            p.right = kind == Schedule::RIGHT ? bnd : nullptr;
+            std::vector<double> ws_init;
+            uint64_t ws_init_off = 0;
+            GroupedStep spec{kind, &nb->index, &nb->cohort_off};
+            spec.lo = range.first;
+            spec.hi = range.second;
+            lower_sigma_grouped(&p, L, W, ws_init, ws_init_off, false, &spec);
+            StagePlan& P = p.st;
+            if (P.ws_elems > budget && range.second - range.first > 1) {
+                const uint32_t mid = range.first + (range.second - range.first) / 2;
+                todo.push_back({mid, range.second});
+                todo.push_back({range.first, mid});
+                continue;
+            }
+            P.blas = ctx->blas;
+            P.finish_tables();
+            P.sizes[BASE_WS] = P.ws_elems;
+            P.sizes[BASE_IN] = P.sizes[BASE_BRA] = L.rp_total;
+            P.sizes[BASE_OUT] = nb->total;
+            P.sizes[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = bnd->total;
+            P.validate();
+            P.upload_all(ctx->stream);
+            // workspace: the context's step buffer (cleared whenever it grows), W_D copied in
+            const size_t need = std::max<uint64_t>(P.ws_elems, 32) * sizeof(double);
+            if (need > ctx->step_ws.bytes) ctx->step_ws.alloc_zeroed(need, ctx->stream);
+            double* ws = ctx->step_ws.as<double>();
+            if (!ws_init.empty())
+                HIPCHK(hipMemcpyAsync(ws + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, ctx->stream));
+            Bases bs{};
+            bs.p[BASE_WS] = ws;
+            bs.p[BASE_IN] = const_cast<double*>(ket_rp);
+            bs.p[BASE_OUT] = nb->ptr();
+            bs.p[BASE_BRA] = const_cast<double*>(bra_rp);
+            bs.p[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = const_cast<double*>(bnd->ptr());
+            BLASCHK(rocblas_set_stream(ctx->blas, ctx->stream));
+            P.run(ctx->stream, bs, MODE_F64);
+            HIPCHK(hipStreamSynchronize(ctx->stream));   // the chunk's tables and ws_init are freed next
+        }
+        ctx->step_ws.release();   // steps run once per site: keep no workspace between them
         *out = nb.release();
     });
 }

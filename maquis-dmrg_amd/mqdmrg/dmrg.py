@@ -74,12 +74,16 @@ def truncate(svals, Mmax, cutoff):
 class TwoSiteDMRG:
     def __init__(self, eng, model, L, m, cutoff=1e-10, init_m=16, seed=1, jd_tol=1e-8, jd_maxiter=8,
                  mpo=None, basis=None, target=None, verbose=False, hook=None, precision=E.PREC_F64, max_outer=1,
-                 **model_kw):
+                 init_bonds="sectors", **model_kw):
         """precision != PREC_F64: sigma in that precision inside Jacobi-Davidson, with the FP64
         residual refinement of SigmaPlan.jd_refined (at most max_outer FP64 steps per site).  With
         max_outer = 1 a site costs jd_maxiter reduced-precision products and one FP64 product (the
         Rayleigh quotient and residual), and the sweep follows the FP64 sweep's trajectory; more
-        outer steps restart the iteration and suit solves to a tight tolerance."""
+        outer steps restart the iteration and suit solves to a tight tolerance.
+
+        init_bonds: "sectors" caps every charge sector of the initial MPS at init_m (the reference's
+        mps initialisation, mps_sectors.h); "total" draws bond indices of total dimension ~init_m
+        (mps.random_bond_dims), the shape a converged state at that bond dimension has."""
         self.eng, self.L, self.m, self.cutoff = eng, L, m, cutoff
         self.jd_tol, self.jd_maxiter, self.verbose = jd_tol, jd_maxiter, verbose
         self.precision, self.max_outer = precision, max_outer
@@ -93,14 +97,17 @@ class TwoSiteDMRG:
         self.mpo_desc = [E.MpoDesc(W) for W in mpo]
         self.ts_mpo = {}
         rng = np.random.default_rng(seed)
-        bonds, _, _ = M.allowed_sectors([self.phys] * L, target, init_m)
+        if init_bonds == "total":
+            bonds = M.random_bond_dims([self.phys] * L, target, init_m)
+        else:
+            bonds, _, _ = M.allowed_sectors([self.phys] * L, target, init_m)
         self.sites = []
         for i in range(L):
             lay = SiteLayout(bonds[i], self.phys, bonds[i + 1])
             v = torch.from_numpy(M.random_site_vector(lay, rng)).to(eng.torch_device)
             self.sites.append(SiteTensor(eng, self.sym, bonds[i], self.phys, bonds[i + 1], v))
         self.energies = []
-        self.timings = {"jd": 0.0, "split": 0.0, "boundary": 0.0, "sweep": []}
+        self.timings = {"merge": 0.0, "plan": 0.0, "jd": 0.0, "split": 0.0, "boundary": 0.0, "sweep": []}
 
     # ------------------------------------------------------------------ canonical form
     def right_normalize(self):
@@ -324,10 +331,14 @@ class TwoSiteDMRG:
             self.Rb[k] = E.right_step(self.eng, s.desc, self.mpo_desc[k], s.data, s.data, self.Rb[k + 1])
 
     def optimize_pair(self, i, direction):
+        t0 = time.perf_counter()
         W12, wdesc, phys12, pb = self.twosite_mpo(i)
         theta = self.merge(i)
+        self.timings["merge"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         plan = E.SigmaPlan(self.eng, theta.desc, wdesc, self.Lb[i], self.Rb[i + 2])
+        self.timings["plan"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
         if self.hook is not None:
             self.hook(self, i, theta, wdesc, plan)
         out = torch.zeros_like(theta.data)
