@@ -55,6 +55,10 @@ int guarded(F&& f) {
 }
 
 // Device allocation owned by RAII.
+// Releases the workspace pools of every context (set up with the contexts): an allocation that
+// fails retries once after it.
+static void (*g_release_pools)() = nullptr;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -71,6 +75,11 @@ struct DevBuf {
         release();
         if (n == 0) return;
         hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess && g_release_pools) {
+            (void)hipGetLastError();
+            g_release_pools();
+            e = hipMalloc(&p, n);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             p = nullptr;
@@ -80,6 +89,10 @@ struct DevBuf {
     }
     void ensure(size_t n) {
         if (n > bytes) alloc(n);
+    }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
     }
     // allocation cleared to zero bits: library GEMMs called with beta = 0 may still read their C
     // operand (0 * NaN of stale memory is NaN), so workspaces never hold non-finite garbage
@@ -142,6 +155,44 @@ struct mq_ctx {
     DevBuf lt_ws;
     // hipBLASLt algorithms timed by mq_plan_autotune, keyed by (precision, M, N, K)
     std::map<std::array<uint64_t, 4>, hipblasLtMatmulAlgo_t> lt_tuned;
+    // workspaces of destroyed plans, reused by the next plans: a sweep creates one plan per site,
+    // and allocating and releasing tens of GB per site costs more than the site's products
+    std::vector<std::unique_ptr<DevBuf>> pool;
+    static constexpr size_t POOL_MAX = 4;
+    void pool_put(DevBuf& b) {
+        if (!b.p) return;
+        auto keep = std::make_unique<DevBuf>();
+        keep->swap(b);
+        pool.push_back(std::move(keep));
+        if (pool.size() > POOL_MAX) pool.erase(pool.begin());   // oldest first
+    }
+    // a workspace of at least n bytes cleared to zero: the smallest pooled buffer that fits (and is
+    // not more than 4x larger), else a new allocation; when that fails the pool is released and
+    // the allocation retried
+    void take(DevBuf& dst, size_t n) {
+        n = (n + 7) / 8 * 8;
+        size_t best = pool.size();
+        for (size_t i = 0; i < pool.size(); ++i)
+            if (pool[i]->bytes >= n && pool[i]->bytes <= 4 * n && (best == pool.size() || pool[i]->bytes < pool[best]->bytes))
+                best = i;
+        if (best < pool.size()) {
+            dst.release();
+            dst.swap(*pool[best]);
+            pool.erase(pool.begin() + best);
+            HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), dst.bytes / 8, 0.0));
+            return;
+        }
+        hipError_t e = hipMalloc(&dst.p, n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            dst.p = nullptr;
+            pool.clear();
+            dst.alloc_zeroed(n, stream);
+            return;
+        }
+        dst.bytes = n;
+        HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), n / 8, 0.0));
+    }
     DevBuf red_partial, red_result;
     DevBuf step_ws;   // grows; reused by boundary steps
     // batched applications: side streams (default priority) the instances of a batch fork onto,
@@ -800,21 +851,6 @@ struct StagePlan {
     }
 };
 
-// x = hi + lo in bf16, round to nearest even (the device split_bf16_kernel's rule)
-static inline uint16_t bf16_rne_host(float x) {
-    uint32_t u;
-    std::memcpy(&u, &x, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-static inline void bf16_split_host(float x, uint16_t& hi, uint16_t& lo) {
-    hi = bf16_rne_host(x);
-    uint32_t h = uint32_t(hi) << 16;
-    float hv;
-    std::memcpy(&hv, &h, 4);
-    lo = bf16_rne_host(x - hv);
-}
-
 static_assert(MQ_PREC_F64 == MODE_F64 && MQ_PREC_F32 == MODE_F32 && MQ_PREC_BF16X3 == MODE_BF16X3,
               "precision enum and GEMM modes agree");
 
@@ -869,36 +905,21 @@ struct mq_plan {
         } else {
             MQ_REQUIRE(grouped, "reduced-precision sigma needs a GROUPED plan");
             if (!ws32.p) {
-                ws32.alloc_zeroed(std::max<uint64_t>(st.ws_elems, 64) * sizeof(float), ctx->stream);
-                if (!ws_init.empty()) {
-                    std::vector<float> w(ws_init.begin(), ws_init.end());
-                    HIPCHK(hipMemcpyAsync(ws32.as<float>() + ws_init_off, w.data(), w.size() * sizeof(float),
-                                          hipMemcpyHostToDevice, ctx->stream));
-                    HIPCHK(hipStreamSynchronize(ctx->stream));
-                }
+                // the FP32 workspace; W_D converted on the device from the FP64 copy
+                ctx->take(ws32, std::max<uint64_t>(st.ws_elems, 64) * sizeof(float));
+                if (!ws_init.empty())
+                    HIPCHK(launch_to_float(ctx->stream, ws.as<double>() + ws_init_off, ws32.as<float>() + ws_init_off,
+                                           ws_init.size()));
             }
             b.p[BASE_WS] = reinterpret_cast<double*>(ws32.as<float>());
             if (mode == MODE_BF16X3 && st.bf_elems && !bf.p) {
                 // bf16 operands of the one-GEMM BF16x3 form: A' regions (written per application) and
-                // B' = [W_lo; W_hi; W_hi] per group (constant, split here)
-                bf.alloc_zeroed(std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t), ctx->stream);
+                // B' = [W_lo; W_hi; W_hi] per group (constant, split here on the device from W_D)
+                ctx->take(bf, std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t));
                 st.bf = bf.as<uint16_t>();
-                for (const auto& g : st.lib) {
-                    const double* W = ws_init.data() + (g.b_off - ws_init_off);
-                    std::vector<uint16_t> Bp(uint64_t(3) * g.K * g.N);
-                    for (uint64_t n = 0; n < g.N; ++n)
-                        for (uint64_t k = 0; k < g.K; ++k) {
-                            uint16_t hi, lo;
-                            bf16_split_host(float(W[n * g.ldb + k]), hi, lo);
-                            uint16_t* col = Bp.data() + n * 3 * g.K;
-                            col[k] = lo;
-                            col[g.K + k] = hi;
-                            col[2 * g.K + k] = hi;
-                        }
-                    HIPCHK(hipMemcpyAsync(st.bf + g.bfb_off, Bp.data(), Bp.size() * sizeof(uint16_t),
-                                          hipMemcpyHostToDevice, ctx->stream));
-                    HIPCHK(hipStreamSynchronize(ctx->stream));
-                }
+                for (const auto& g : st.lib)
+                    HIPCHK(launch_split_w_bf16(ctx->stream, ws.as<double>() + g.b_off, g.ldb, g.K, g.N,
+                                               st.bf + g.bfb_off));
             }
         }
         b.p[BASE_IN] = const_cast<double*>(psi);
@@ -1055,6 +1076,14 @@ extern "C" {
 
 const char* mq_last_error(void) { return g_err.c_str(); }
 
+static std::vector<mq_ctx*> g_contexts;
+static void release_all_pools() {
+    for (mq_ctx* c : g_contexts) {
+        (void)hipStreamSynchronize(c->stream);
+        c->pool.clear();
+    }
+}
+
 int mq_ctx_create(int device, mq_ctx** out) {
     return guarded([&] {
         MQ_REQUIRE(out, "null output");
@@ -1068,6 +1097,8 @@ int mq_ctx_create(int device, mq_ctx** out) {
         c->lt_ws.alloc(LT_WORKSPACE);
         c->red_partial.alloc(RED_BLOCKS * sizeof(double));
         c->red_result.alloc(64 * sizeof(double));
+        g_release_pools = release_all_pools;
+        g_contexts.push_back(c.get());
         *out = c.release();
     });
 }
@@ -1080,6 +1111,8 @@ int mq_ctx_destroy(mq_ctx* ctx) {
         for (auto sd : ctx->side) (void)hipStreamDestroy(sd);
         for (auto e : ctx->side_ev) (void)hipEventDestroy(e);
         if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+        ctx->pool.clear();
+        g_contexts.erase(std::remove(g_contexts.begin(), g_contexts.end(), ctx), g_contexts.end());
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
         if (ctx->lt) (void)hipblasLtDestroy(ctx->lt);
         (void)hipStreamDestroy(ctx->stream);
@@ -2316,8 +2349,8 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         P.sizes[BASE_RIGHT] = right->total;
         P.sizes[BASE_BRA] = P.part_elems;
         P.validate();
-        if (P.part_elems) p->part.alloc(P.part_elems * sizeof(double));
-        p->ws.alloc_zeroed(std::max<uint64_t>(P.ws_elems, 32) * sizeof(double), ctx->stream);
+        if (P.part_elems) ctx->take(p->part, P.part_elems * sizeof(double));
+        ctx->take(p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
         if (!ws_init.empty())
             HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
                                   hipMemcpyHostToDevice, ctx->stream));
@@ -2703,6 +2736,8 @@ int mq_plan_destroy(mq_plan* p) {
     return guarded([&] {
         if (p && p->ctx) {
             (void)hipStreamSynchronize(p->ctx->stream);
+            if (p->grouped)   // the large workspaces go back to the context for the next plans
+                for (DevBuf* b : {&p->ws, &p->ws32, &p->bf, &p->part}) p->ctx->pool_put(*b);
             // batch graphs that contain the plan refer to its buffers
             auto& B = p->ctx->batches;
             for (auto it = B.begin(); it != B.end();) {

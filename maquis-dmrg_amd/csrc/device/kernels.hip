@@ -182,6 +182,35 @@ hipError_t launch_term_grad(hipStream_t st, uint32_t n_terms, const uint32_t* pt
     return hipGetLastError();
 }
 
+__global__ void to_float_kernel(const double* __restrict__ src, float* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = float(src[i]);
+}
+hipError_t launch_to_float(hipStream_t st, const double* src, float* dst, uint64_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(to_float_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+__global__ void split_w_bf16_kernel(const double* __restrict__ W, uint64_t ldw, uint64_t K, uint64_t N,
+                                    uint16_t* __restrict__ dst) {
+    const uint64_t total = K * N;
+    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t n = e / K, k = e - n * K;
+        const float v = float(W[n * ldw + k]);
+        const uint16_t hi = bf16_rne(v);
+        const uint16_t lo = bf16_rne(v - __uint_as_float(uint32_t(hi) << 16));
+        uint16_t* col = dst + n * 3 * K;
+        col[k] = lo;
+        col[K + k] = hi;
+        col[2 * K + k] = hi;
+    }
+}
+hipError_t launch_split_w_bf16(hipStream_t st, const double* W, uint64_t ldw, uint64_t K, uint64_t N, uint16_t* dst) {
+    if (K == 0 || N == 0) return hipSuccess;
+    hipLaunchKernelGGL(split_w_bf16_kernel, dim3(grid_for(K * N, 256)), dim3(256), 0, st, W, ldw, K, N, dst);
+    return hipGetLastError();
+}
+
 __global__ void axpy_kernel(uint64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         y[i] += a * x[i];

@@ -176,6 +176,11 @@ hipError_t launch_zero_chunks(hipStream_t st, const ZeroChunk* chunks, uint32_t 
 // dst[0, n) = hi, dst[n, 2n) = lo, dst[2n, 3n) = hi of x, with hi = bf16(x), lo = bf16(x - hi)
 // (round to nearest even)
 hipError_t launch_split3_bf16(hipStream_t st, const float* x, uint16_t* dst, uint64_t n);
+// dst[i] = float(src[i])
+hipError_t launch_to_float(hipStream_t st, const double* src, float* dst, uint64_t n);
+// B' = [W_lo; W_hi; W_hi] (3K x N, column-major, ld 3K) of W (K x N, column stride ldw), with
+// hi = bf16(float(w)), lo = bf16(float(w) - hi)
+hipError_t launch_split_w_bf16(hipStream_t st, const double* W, uint64_t ldw, uint64_t K, uint64_t N, uint16_t* dst);
 hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial,
                       double* result);
 hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y);

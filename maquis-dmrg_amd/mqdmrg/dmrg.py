@@ -235,6 +235,32 @@ class TwoSiteDMRG:
             theta.rp_block(lb)[:, col12:col12 + rs] = prod[mc][r:r + ls, c:c + rs]
         return theta
 
+    def _svds(self, mats, keys):
+        """FP64 SVD of every sector matrix on the GPU.  Each call is latency-bound (small
+        bidiagonalisation kernels with host round trips), so the sectors are spread over a few
+        host threads, each on its own stream (default priority), and the results joined before
+        use."""
+        if len(keys) < 4:
+            return {c: torch.linalg.svd(mats[c], full_matrices=False) for c in keys}
+        if not hasattr(self, "_svd_pool"):
+            from concurrent.futures import ThreadPoolExecutor
+            self._svd_pool = ThreadPoolExecutor(max_workers=8)
+            self._svd_streams = [torch.cuda.Stream(device=self.eng.torch_device) for _ in range(8)]
+        torch.cuda.current_stream(self.eng.torch_device).synchronize()   # the matrices are written
+        order = sorted(keys, key=lambda c: -mats[c].numel())              # largest first
+        chunks = [order[k::8] for k in range(8)]
+
+        def run(k):
+            st = self._svd_streams[k]
+            with torch.cuda.stream(st):
+                out = {c: torch.linalg.svd(mats[c], full_matrices=False) for c in chunks[k]}
+            st.synchronize()
+            return out
+        res = {}
+        for part in self._svd_pool.map(run, range(8)):
+            res.update(part)
+        return res
+
     def split(self, i, theta, direction):
         """Truncated SVD split of the two-site tensor (split_mps_l2r / split_mps_r2l)."""
         A1, A2 = self.sites[i], self.sites[i + 1]
@@ -278,7 +304,7 @@ class TwoSiteDMRG:
                             sub_i = pb(p1, p2) + ss1 * d2 + ss2
                             mats[mc][r0 + ss1 * ls:r0 + (ss1 + 1) * ls, c0 + ss2 * rs:c0 + (ss2 + 1) * rs] = \
                                 theta.rp_block(lb)[:, col12 + sub_i * rs:col12 + (sub_i + 1) * rs]
-        svd = {c: torch.linalg.svd(mats[c], full_matrices=False) for c in mids}
+        svd = self._svds(mats, mids)
         svals = [svd[c][1].cpu().numpy() for c in mids]
         keeps, tw = truncate(svals, self.m, self.cutoff)
         kept = [(c, k) for c, k in zip(mids, keeps) if k > 0]
