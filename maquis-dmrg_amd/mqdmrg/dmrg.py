@@ -6,7 +6,8 @@ Jacobi-Davidson (solver_interface.hpp:30-77 -> engine mq_jd_solve), the result i
 truncated SVD (split_mps_l2r / r2l, twositetensor.hpp:153-200 with estimate_truncation,
 block_matrix_algorithms.h:135-183), and the boundary is moved (optimize.h:134-152 -> engine
 left/right steps).  Site tensors live on the device in the right-paired padded layout; the small
-dense algebra of the split (SVD, QR) uses torch on the engine's stream."""
+dense algebra of the split (batched Gram matrices and eigh in place of per-sector SVDs) uses torch
+on the engine's stream."""
 import time
 
 import numpy as np
@@ -60,15 +61,205 @@ class SiteTensor:
         return t
 
 
-def truncate(svals, Mmax, cutoff):
-    """estimate_truncation (block_matrix_algorithms.h:135-183): per-block keep counts."""
+def truncation_cut(svals, Mmax, cutoff):
+    """The threshold of estimate_truncation (block_matrix_algorithms.h:135-183):
+    max(cutoff * s_max, s[Mmax]) over all blocks' singular values, sorted descending."""
     allv = np.sort(np.concatenate([s for s in svals if len(s)]))[::-1]
     cut = cutoff * allv[0]
     if len(allv) > Mmax:
         cut = max(cut, allv[Mmax])
+    return cut, allv
+
+
+def truncate(svals, Mmax, cutoff):
+    """estimate_truncation (block_matrix_algorithms.h:135-183): per-block keep counts."""
+    cut, allv = truncation_cut(svals, Mmax, cutoff)
     keeps = [int(np.sum(s >= cut)) for s in svals]
     tw = float(np.sum(allv[allv < cut] ** 2) / np.sum(allv ** 2))
     return keeps, tw
+
+
+def _gram_buckets(sizes):
+    """Group Gram sizes into padded batches for torch.linalg.eigh.  A batched eigh call has a
+    fixed cost that grows slowly with the padded size n (~1.5 + 0.02 n ms on MI355X) and a
+    per-matrix cost ~0.08 (n/256)^2 ms, so a few large batches beat one call per size.  Sizes are
+    rounded up to multiples of 32 and split into contiguous ranges by dynamic programming on that
+    cost.  Returns a list of (padded size, [positions into sizes])."""
+    order = sorted(range(len(sizes)), key=lambda i: sizes[i])
+    rounded = [max(32, -(-sizes[i] // 32) * 32) for i in order]
+    cost = lambda nb, cnt: 1.5 + 0.02 * nb + cnt * 0.08 * (nb / 256.0) ** 2
+    K = len(order)
+    best, cut = [0.0] + [float("inf")] * K, [0] * (K + 1)
+    for j in range(1, K + 1):
+        for i in range(j):
+            c = best[i] + cost(rounded[j - 1], j - i)
+            if c < best[j]:
+                best[j], cut[j] = c, i
+    groups, j = [], K
+    while j > 0:
+        i = cut[j]
+        groups.append((rounded[j - 1], order[i:j]))
+        j = i
+    return groups[::-1]
+
+
+def gram_spectrum(mats, keys, direction, device):
+    """Singular values of the mid-sector matrices of a two-site split from batched Gram matrices.
+
+    X is the sector matrix M (rows s1 x left, columns s2 x right) for a left-to-right split and
+    M^T for a right-to-left one, so the factor that must come out orthonormal (U, or V for the
+    right-normalised site) spans X's rows.  Per sector the Gram matrix of the smaller side,
+    G = Y Y^T with Y = X (rows <= columns) or Y = X^T, is formed with batched GEMMs; all of them
+    are diagonalised by a few padded batched eigh calls (one SVD per sector is instead a chain of
+    small latency-bound kernels).  Z = P^T Y with P the eigenvectors; the singular values are the
+    row norms of Z, the weights the vectors actually carry.  Padded diagonal entries are set to
+    -(1 + trace G), far below the non-negative spectrum, so padding eigenvectors separate exactly
+    and sort last.  The eigenvectors are orthonormal to working precision whatever the
+    conditioning, but a singular value s carries an error of ~eps s_max^2 / s, so values below
+    ~GRAM_RESOLVED s_max are not resolved (split_factors falls back to exact SVDs when the
+    truncation threshold lies there).
+
+    Returns (state, {c: singular values (numpy, descending, min(rows, cols) of them)})."""
+    f64 = torch.float64
+    xs = {c: (mats[c] if direction > 0 else mats[c].t()) for c in keys}
+    svals, live = {}, []
+    for c in keys:
+        n, w = xs[c].shape
+        if n == 0 or w == 0:
+            svals[c] = np.zeros(0)
+        else:
+            live.append(c)
+    buckets = []
+    for nb, pos in _gram_buckets([min(xs[c].shape) for c in live]):
+        cs = [live[p] for p in pos]
+        trans = [xs[c].shape[0] > xs[c].shape[1] for c in cs]
+        wb = max(max(xs[c].shape) for c in cs)
+        nbat = len(cs)
+        Y = torch.zeros(nbat, nb, wb, dtype=f64, device=device)
+        for i, (c, t) in enumerate(zip(cs, trans)):
+            y = xs[c].t() if t else xs[c]
+            Y[i, :y.shape[0], :y.shape[1]] = y
+        G = torch.bmm(Y, Y.transpose(1, 2).contiguous())
+        r = torch.tensor([min(xs[c].shape) for c in cs], device=device)
+        pos_n = torch.arange(nb, device=device)[None, :]
+        diag = G.diagonal(dim1=1, dim2=2)
+        trace = diag.sum(dim=1, keepdim=True)
+        diag.add_(torch.where(pos_n >= r[:, None], -(1.0 + trace), torch.zeros_like(trace)))
+        _, P = torch.linalg.eigh(G)
+        P = P.flip(-1)                                   # descending eigenvalues, padding last
+        Z = torch.bmm(P.transpose(1, 2).contiguous(), Y)
+        sv = torch.where(pos_n < r[:, None], Z.norm(dim=2), torch.full_like(Z[:, :, 0], -1.0))
+        sv, order = sv.sort(dim=1, descending=True)
+        P = P.gather(2, order[:, None, :].expand(nbat, nb, nb))
+        Z = Z.gather(1, order[:, :, None].expand(nbat, nb, wb))
+        s_host = sv.cpu().numpy()
+        for i, c in enumerate(cs):
+            svals[c] = s_host[i, :min(xs[c].shape)]
+        buckets.append((cs, trans, Y, P, Z, sv))
+    return (xs, buckets), svals
+
+
+def gram_truncated(state, keeps, device):
+    """The kept factors {c: (Q_k, B_k)} of gram_spectrum's sectors, keeps = {c: k}: Q_k (rows of
+    X x k) with orthonormal columns and B_k = Q_k^T X (k x columns of X), so that Q_k B_k is the
+    rank-k truncation of X.  Where the Gram was taken on X's rows side, Q_k is its leading
+    eigenvectors and B_k the leading rows of Z.  Where it was taken on the columns side (X has
+    more rows), Q_k = X P_k / s_k, whose columns are orthogonal only to ~eps s_max^2 / (s_i s_j);
+    one Cholesky QR pass (C = Q^T Q = L L^T, Q <- Q L^-T, well conditioned because every kept
+    s >= GRAM_RESOLVED s_max) makes them orthonormal to working precision, and B_k = Q_k^T X is
+    then the exact projection."""
+    f64 = torch.float64
+    xs, buckets = state
+    out = {}
+    for cs, trans, Y, P, Z, sv in buckets:
+        tb = []
+        for i, (c, t) in enumerate(zip(cs, trans)):
+            k = keeps.get(c, 0)
+            if k == 0:
+                continue
+            n, w = xs[c].shape
+            if not t:
+                out[c] = (P[i, :n, :k], Z[i, :k, :w])
+            else:
+                tb.append((i, c, k))
+        if not tb:
+            continue
+        kmax = max(k for _, _, k in tb)
+        nmax = max(xs[c].shape[0] for _, c, _ in tb)
+        nt = len(tb)
+        idx = torch.tensor([i for i, _, _ in tb], device=device)
+        kk = torch.tensor([k for _, _, k in tb], device=device)
+        keep = torch.arange(kmax, device=device)[None, :] < kk[:, None]                    # (nt, kmax)
+        s_k = torch.where(keep, sv[idx, :kmax], torch.ones_like(sv[idx, :kmax]))
+        U = Z[idx, :kmax, :nmax].transpose(1, 2) / s_k[:, None, :]                       # (nt, nmax, kmax)
+        U = U * keep[:, None, :]
+        C = torch.bmm(U.transpose(1, 2).contiguous(), U)
+        C.diagonal(dim1=1, dim2=2).add_((~keep).to(f64))
+        Lc = torch.linalg.cholesky(C)
+        U = torch.linalg.solve_triangular(Lc.transpose(1, 2).contiguous(), U, upper=True, left=False)
+        Bt = torch.bmm(Y[idx, :, :nmax], U)                                               # (nt, w_pad, kmax) = X^T Q
+        for j, (i, c, k) in enumerate(tb):
+            n, w = xs[c].shape
+            out[c] = (U[j, :n, :k], Bt[j, :w, :k].t())
+    return out
+
+
+def svd_factors(mats, keys, direction, device, threads=8):
+    """Singular values and full factors {c: (Q, B)} (X = Q B, Q = U, B = S V^T) from one
+    LAPACK-style SVD per sector (torch.linalg.svd).  The calls are latency-bound chains of small
+    kernels, so on a GPU the sectors are spread over a few host threads, each on its own stream
+    (default priority)."""
+    def one(c):
+        X = mats[c] if direction > 0 else mats[c].t()
+        U, S, Vh = torch.linalg.svd(X, full_matrices=False)
+        return c, (U, S[:, None] * Vh, S)
+
+    if torch.device(device).type != "cuda" or len(keys) < 4:
+        res = dict(one(c) for c in keys)
+    else:
+        from concurrent.futures import ThreadPoolExecutor
+        torch.cuda.current_stream(device).synchronize()        # the matrices are written
+        order = sorted(keys, key=lambda c: -mats[c].numel())   # largest first
+        chunks = [order[k::threads] for k in range(threads)]
+
+        def run(chunk):
+            st = torch.cuda.Stream(device=device)
+            with torch.cuda.stream(st):
+                out = [one(c) for c in chunk]
+            st.synchronize()
+            return out
+        res = {}
+        with ThreadPoolExecutor(max_workers=threads) as pool:
+            for part in pool.map(run, chunks):
+                res.update(part)
+    return {c: (U, B) for c, (U, B, _) in res.items()}, {c: S.cpu().numpy() for c, (_, _, S) in res.items()}
+
+
+GRAM_RESOLVED = 1e-5
+
+
+def split_factors(mats, keys, direction, device, Mmax, cutoff):
+    """Truncated factors of a two-site split (twositetensor.hpp:153-200 with estimate_truncation,
+    block_matrix_algorithms.h:135-183).  The spectrum comes from gram_spectrum; if the truncation
+    threshold falls below GRAM_RESOLVED times the largest singular value, where the Gram spectrum
+    cannot place it, every sector is factorised by svd_factors instead.  Returns
+    ({c: (Q_k, B_k)} for the sectors that keep k > 0 states, keeps (per key), singular values
+    (per key), truncated weight, method), with Q_k B_k the rank-k truncation of X (M, or M^T
+    right-to-left) and Q_k orthonormal."""
+    state, sv = gram_spectrum(mats, keys, direction, device)
+    svals = [sv[c] for c in keys]
+    if not any(len(v) for v in svals):
+        return {}, [0] * len(keys), svals, 0.0, "gram"
+    cut, allv = truncation_cut(svals, Mmax, cutoff)
+    if cut >= GRAM_RESOLVED * allv[0]:
+        keeps, tw = truncate(svals, Mmax, cutoff)
+        return gram_truncated(state, dict(zip(keys, keeps)), device), keeps, svals, tw, "gram"
+    del state
+    full, sv = svd_factors(mats, keys, direction, device)
+    svals = [sv[c] for c in keys]
+    keeps, tw = truncate(svals, Mmax, cutoff)
+    fac = {c: (full[c][0][:, :k], full[c][1][:k]) for c, k in zip(keys, keeps) if k > 0}
+    return fac, keeps, svals, tw, "svd"
 
 
 class TwoSiteDMRG:
@@ -88,6 +279,7 @@ class TwoSiteDMRG:
         self.jd_tol, self.jd_maxiter, self.verbose = jd_tol, jd_maxiter, verbose
         self.precision, self.max_outer = precision, max_outer
         self.sigma_calls = {"reduced": 0, "fp64": 0}
+        self.split_methods = {}
         self.hook = hook
         if mpo is None:
             basis, mpo, target = models.chain(model, L, **model_kw)
@@ -235,32 +427,6 @@ class TwoSiteDMRG:
             theta.rp_block(lb)[:, col12:col12 + rs] = prod[mc][r:r + ls, c:c + rs]
         return theta
 
-    def _svds(self, mats, keys):
-        """FP64 SVD of every sector matrix on the GPU.  Each call is latency-bound (small
-        bidiagonalisation kernels with host round trips), so the sectors are spread over a few
-        host threads, each on its own stream (default priority), and the results joined before
-        use."""
-        if len(keys) < 4:
-            return {c: torch.linalg.svd(mats[c], full_matrices=False) for c in keys}
-        if not hasattr(self, "_svd_pool"):
-            from concurrent.futures import ThreadPoolExecutor
-            self._svd_pool = ThreadPoolExecutor(max_workers=8)
-            self._svd_streams = [torch.cuda.Stream(device=self.eng.torch_device) for _ in range(8)]
-        torch.cuda.current_stream(self.eng.torch_device).synchronize()   # the matrices are written
-        order = sorted(keys, key=lambda c: -mats[c].numel())              # largest first
-        chunks = [order[k::8] for k in range(8)]
-
-        def run(k):
-            st = self._svd_streams[k]
-            with torch.cuda.stream(st):
-                out = {c: torch.linalg.svd(mats[c], full_matrices=False) for c in chunks[k]}
-            st.synchronize()
-            return out
-        res = {}
-        for part in self._svd_pool.map(run, range(8)):
-            res.update(part)
-        return res
-
     def split(self, i, theta, direction):
         """Truncated SVD split of the two-site tensor (split_mps_l2r / split_mps_r2l)."""
         A1, A2 = self.sites[i], self.sites[i + 1]
@@ -304,20 +470,19 @@ class TwoSiteDMRG:
                             sub_i = pb(p1, p2) + ss1 * d2 + ss2
                             mats[mc][r0 + ss1 * ls:r0 + (ss1 + 1) * ls, c0 + ss2 * rs:c0 + (ss2 + 1) * rs] = \
                                 theta.rp_block(lb)[:, col12 + sub_i * rs:col12 + (sub_i + 1) * rs]
-        svd = self._svds(mats, mids)
-        svals = [svd[c][1].cpu().numpy() for c in mids]
-        keeps, tw = truncate(svals, self.m, self.cutoff)
+        fac, keeps, svals, tw, method = split_factors(mats, mids, direction, self.eng.torch_device, self.m,
+                                                      self.cutoff)
+        self.split_methods[method] = self.split_methods.get(method, 0) + 1
         kept = [(c, k) for c, k in zip(mids, keeps) if k > 0]
         newmid = Index([c for c, _ in kept], [k for _, k in kept])
         norm = np.sqrt(sum(float(np.sum(s[:k] ** 2)) for s, k in zip(svals, keeps)))
         b1, b2 = {}, {}
         for c, k in kept:
-            U, S, Vh = svd[c]
-            U, S, Vh = U[:, :k], S[:k] / norm, Vh[:k]
+            orth, other = fac[c][0], fac[c][1] / norm
             if direction > 0:
-                b1[c], b2[c] = U, S[:, None] * Vh
+                b1[c], b2[c] = orth, other
             else:
-                b1[c], b2[c] = U * S[None, :], Vh
+                b1[c], b2[c] = other.t(), orth.t()
         # scatter back into the factor layouts: A1 left-paired rows (s1, lc), A2 right-paired
         N1 = SiteLayout(A1.left, self.phys, newmid)
         N2 = SiteLayout(newmid, self.phys, A2.right)

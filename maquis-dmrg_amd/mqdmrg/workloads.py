@@ -26,9 +26,11 @@ def model_chain(model, L, **kw):
 
 
 class SyntheticSite:
-    def __init__(self, model, L, site, m, seed=0):
+    def __init__(self, model, L, site, m, seed=0, chain=None):
+        """chain: a prebuilt (basis, mpo, target) of models.chain(model, L) to reuse."""
         self.model, self.L, self.site, self.m = model, L, site, m
-        basis, mpo, target = model_chain(model, L)
+        basis, mpo, target = chain if chain is not None else model_chain(model, L)
+        self.chain = (basis, mpo, target)
         self.basis, self.mpo = basis, mpo
         phys = [basis.phys] * L
         self.bonds = mps.random_bond_dims(phys, target, m)

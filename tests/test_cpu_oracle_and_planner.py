@@ -7,10 +7,12 @@ import re
 
 import numpy as np
 import pytest
+import torch
 
 from chain import SiteProblem
 from oracle import oracle as O
 from mqdmrg import capi, engine as E, models, mps
+from mqdmrg import dmrg as DM
 from mqdmrg.dmrg import truncate
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -244,6 +246,95 @@ def test_truncation_rule():
     assert keeps == [2, 2, 0]
     keeps, _ = truncate(s, Mmax=10, cutoff=0.6)    # relative cutoff 0.6 * s_max
     assert keeps == [1, 1, 0]
+
+
+def _graded(rng, n, w, smin_exp, rank=None):
+    r = min(n, w) if rank is None else rank
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((w, r)))
+    s = np.logspace(0, smin_exp, r) * rng.uniform(0.5, 1.0)
+    return (U * s) @ V.T
+
+
+def _sector_mats(seed):
+    rng = np.random.default_rng(seed)
+    shapes = {0: (37, 20, -13, None), 1: (20, 37, -9, None), 2: (64, 64, -6, 10), 3: (1, 1, 0, None),
+              4: (0, 5, 0, None), 5: (90, 3, -2, None), 6: (3, 90, -2, None), 7: (200, 150, -12, None),
+              8: (33, 40, -11, 25)}
+    return {c: torch.from_numpy(_graded(rng, n, w, e, r)) if n and w else torch.zeros(n, w, dtype=torch.float64)
+            for c, (n, w, e, r) in shapes.items()}
+
+
+@pytest.mark.parametrize("direction", [1, -1])
+def test_gram_spectrum_and_factors_against_lapack_svd(direction):
+    """The batched Gram-eigh factorisation of the split against numpy's SVD (LAPACK gesdd) on
+    graded spectra down to 1e-13 of the largest value, sectors wider and taller than they are long
+    (Gram on either side, so both the eigenvector and the Cholesky-QR branch), ragged sizes sharing
+    a batch, rank-deficient, empty and 1x1 sectors: each singular value within the Gram bound
+    8 eps s_max^2 / s above GRAM_RESOLVED s_max; kept factors with Q_k^T Q_k = 1 to 1e-14 and
+    Q_k B_k equal to LAPACK's rank-k truncation."""
+    mats = _sector_mats(11)
+    keys = list(mats)
+    state, sv = DM.gram_spectrum(mats, keys, direction, "cpu")
+    ref = {c: np.linalg.svd(mats[c].numpy(), compute_uv=False) if mats[c].numel() else np.zeros(0) for c in keys}
+    smax = max(v[0] for v in ref.values() if len(v))
+    eps = np.finfo(np.float64).eps
+    keeps = {}
+    for c in keys:
+        s = sv[c]
+        assert s.shape == ref[c].shape and np.all(np.diff(s) <= 0)
+        big = ref[c] >= DM.GRAM_RESOLVED * smax
+        assert np.all(np.abs(s - ref[c])[big] <= 8 * eps * smax ** 2 / ref[c][big]), c
+        keeps[c] = int(np.sum(ref[c] >= 1e-4 * smax))
+    assert sum(keeps.values()) > 30
+    fac = DM.gram_truncated(state, keeps, "cpu")
+    assert set(fac) == {c for c, k in keeps.items() if k}
+    for c, (Q, B) in fac.items():
+        k = keeps[c]
+        Q, B = Q.numpy(), B.numpy()
+        assert Q.shape[1] == k and B.shape[0] == k
+        assert np.abs(Q.T @ Q - np.eye(k)).max() <= 1e-14, c
+        U, S, Vh = np.linalg.svd(mats[c].numpy(), full_matrices=False)
+        trunc = (U[:, :k] * S[:k]) @ Vh[:k]
+        got = Q @ B if direction > 0 else (Q @ B).T
+        assert np.abs(got - trunc).max() <= 1e-12 * smax, c
+
+
+@pytest.mark.parametrize("direction", [1, -1])
+@pytest.mark.parametrize("Mmax,cutoff,method", [(40, 1e-10, "gram"), (12, 1e-10, "gram"), (1000, 1e-3, "gram"),
+                                                (1000, 1e-10, "svd"), (1000, 1e-7, "svd")])
+def test_split_factors_truncate_like_lapack_svd(direction, Mmax, cutoff, method):
+    """split_factors keeps the same number of states per sector as a LAPACK SVD with the
+    reference's truncation rule, the same truncated weight, orthonormal Q_k and the same truncated
+    product U_k S_k V_k^T (to 1e-12 of s_max), on the Gram path when the threshold is resolved and
+    on the SVD fallback when it is not."""
+    mats = _sector_mats(12)
+    keys = list(mats)
+    fac, keeps, svals, tw, used = DM.split_factors(mats, keys, direction, "cpu", Mmax, cutoff)
+    assert used == method
+    ref_s = [np.linalg.svd(mats[c].numpy(), compute_uv=False) if mats[c].numel() else np.zeros(0) for c in keys]
+    k_ref, tw_ref = truncate(ref_s, Mmax, cutoff)
+    smax = max(v[0] for v in ref_s if len(v))
+    assert keeps == k_ref and abs(tw - tw_ref) <= 1e-13
+    for c, kc in zip(keys, keeps):
+        if kc == 0:
+            assert c not in fac
+            continue
+        U, S, Vh = np.linalg.svd(mats[c].numpy(), full_matrices=False)
+        ref = (U[:, :kc] * S[:kc]) @ Vh[:kc]
+        Q, B = (t.numpy() for t in fac[c])
+        assert np.abs(Q.T @ Q - np.eye(kc)).max() <= 1e-13
+        got = Q @ B if direction > 0 else (Q @ B).T
+        assert np.abs(got - ref).max() <= 1e-12 * smax, (c, kc)
+
+
+def test_gram_buckets_cover_every_size_once():
+    sizes = [3, 700, 64, 65, 300, 290, 31, 1, 1200, 256]
+    groups = DM._gram_buckets(sizes)
+    seen = sorted(p for _, ps in groups for p in ps)
+    assert seen == list(range(len(sizes)))
+    for nb, ps in groups:
+        assert nb % 32 == 0 and all(sizes[p] <= nb for p in ps)
 
 
 # --------------------------------------------------------------------------- C-ABI surface
