@@ -13,9 +13,13 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <new>
@@ -29,6 +33,8 @@ using namespace mq;
 
 namespace {
 thread_local std::string g_err;
+
+const bool g_trace = trace_enabled();
 
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
@@ -159,6 +165,25 @@ struct mq_ctx {
     // and allocating and releasing tens of GB per site costs more than the site's products
     std::vector<std::unique_ptr<DevBuf>> pool;
     static constexpr size_t POOL_MAX = 4;
+    // pinned host staging of the dense MPO matrices W_D of the plan (or step chunk) being lowered;
+    // grows, and is reused by the next lowering once the previous upload has been synchronized
+    double* wd_host = nullptr;
+    size_t wd_cap = 0;
+    double* wd_stage(size_t n) {
+        if (n > wd_cap) {
+            if (wd_host) (void)hipHostFree(wd_host);
+            wd_host = nullptr;
+            wd_cap = 0;
+            const size_t cap = (n + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+            HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&wd_host), cap * sizeof(double), hipHostMallocDefault));
+            wd_cap = cap;
+        }
+        const int64_t chunk = int64_t(1) << 18;
+#pragma omp parallel for schedule(static)
+        for (int64_t o = 0; o < int64_t(n); o += chunk)
+            std::memset(wd_host + o, 0, size_t(std::min<int64_t>(chunk, int64_t(n) - o)) * sizeof(double));
+        return wd_host;
+    }
     void pool_put(DevBuf& b) {
         if (!b.p) return;
         auto keep = std::make_unique<DevBuf>();
@@ -179,7 +204,7 @@ struct mq_ctx {
             dst.release();
             dst.swap(*pool[best]);
             pool.erase(pool.begin() + best);
-            HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), dst.bytes / 8, 0.0));
+            HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), n / 8, 0.0));   // the part in use
             return;
         }
         hipError_t e = hipMalloc(&dst.p, n);
@@ -573,7 +598,7 @@ struct StagePlan {
     // A stacked task: its rows are partitioned into contiguous row units; every unit's A block and
     // C blocks lie inside their buffers; one plain (non-transposed, unmapped) A segment.
     void check_stacked(const GemmTask& t, const std::vector<GemmSeg>& segs, bool split) const {
-        MQ_REQUIRE(uint64_t(t.rowunit) + t.M <= maps.size(), "row unit table range");
+        MQ_REQUIRE(uint64_t(t.rowunit) + t.M / 16 <= maps.size(), "row unit table range");
         MQ_REQUIRE(t.seg0 < t.seg1 && t.seg1 <= segs.size(), "stacked task segment range");
         MQ_REQUIRE(t.colunit == MAP_NONE && t.cbw > 0, "stacked task shape");
         for (uint32_t q = t.seg0; q < t.seg1; ++q) {
@@ -586,12 +611,12 @@ struct StagePlan {
         MQ_REQUIRE(t.M % 16 == 0, "stacked task rows not a multiple of 16");
         uint32_t m = 0;
         while (m < t.M) {
-            const uint32_t u = maps[t.rowunit + m];
+            const uint32_t u = maps[t.rowunit + m / 16];
             MQ_REQUIRE(u < rows.size(), "row unit out of range");
             const GemmRow& R = rows[u];
             MQ_REQUIRE(R.row0 == m && m % 16 == 0, "row unit does not start at its first row");
             uint32_t e = m;
-            while (e < t.M && maps[t.rowunit + e] == u) ++e;
+            while (e < t.M && maps[t.rowunit + e / 16] == u) e += 16;
             MQ_REQUIRE((e - m) % 16 == 0 && R.nrows <= e - m && e - m < R.nrows + 16, "row unit span");
             const uint32_t nr = R.nrows;
             MQ_REQUIRE(R.ldc >= 1 && R.c_si >= 1, "row unit output strides");
@@ -861,12 +886,13 @@ struct mq_plan {
     const mq_boundary* left = nullptr;
     const mq_boundary* right = nullptr;
     uint64_t vec_size = 0;
-    uint64_t flops = 0;
+    mutable uint64_t flops = 0;
+    mutable std::function<uint64_t()> s_flops;   // grouped plans: the S-stage flops, evaluated on demand
     std::vector<uint64_t> out_offsets;
     // grouped plans: the dense MPO matrices W_D (host copy) at ws_init_off of the workspace, and the
     // FP32 workspace of the reduced-precision modes (allocated on first use)
     bool grouped = false;
-    std::vector<double> ws_init;
+    uint64_t ws_init_elems = 0;
     uint64_t ws_init_off = 0;
     DevBuf ws32;
     DevBuf part;   // split-K partial sums of the final stage (FP64)
@@ -907,9 +933,9 @@ struct mq_plan {
             if (!ws32.p) {
                 // the FP32 workspace; W_D converted on the device from the FP64 copy
                 ctx->take(ws32, std::max<uint64_t>(st.ws_elems, 64) * sizeof(float));
-                if (!ws_init.empty())
+                if (ws_init_elems)
                     HIPCHK(launch_to_float(ctx->stream, ws.as<double>() + ws_init_off, ws32.as<float>() + ws_init_off,
-                                           ws_init.size()));
+                                           ws_init_elems));
             }
             b.p[BASE_WS] = reinterpret_cast<double*>(ws32.as<float>());
             if (mode == MODE_BF16X3 && st.bf_elems && !bf.p) {
@@ -1112,6 +1138,7 @@ int mq_ctx_destroy(mq_ctx* ctx) {
         for (auto e : ctx->side_ev) (void)hipEventDestroy(e);
         if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
         ctx->pool.clear();
+        if (ctx->wd_host) (void)hipHostFree(ctx->wd_host);
         g_contexts.erase(std::remove(g_contexts.begin(), g_contexts.end(), ctx), g_contexts.end());
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
         if (ctx->lt) (void)hipblasLtDestroy(ctx->lt);
@@ -1516,9 +1543,11 @@ void lower_sigma(mq_plan* p, const Schedule& S, const std::vector<uint32_t>& ket
 // are applied as per-segment alpha.  Same result as the reference's T / S / L stages
 // (solver/tasks.cpp:286-344, 466-494, 650-752) with the MPO-term sum done as a dense product.
 struct GroupedLowering {
-    static uint64_t kkey(uint32_t b, uint32_t s, uint32_t ss) {
-        return (uint64_t(b) << 32) | (uint64_t(s) << 16) | ss;
-    }
+    // keys (b, s, ss) (MPO index, physical sector, state in the sector) numbered densely as
+    // b * nst + soff[s] + ss, which is ascending in (b, s, ss)
+    uint32_t nst = 0, nkeys = 0;
+    std::vector<uint32_t> soff;
+    uint32_t key(uint32_t b, uint32_t s, uint32_t ss) const { return b * nst + soff[s] + ss; }
     struct Pair {
         uint32_t lb_in, rb_out, g;
         uint64_t m_off;
@@ -1526,17 +1555,50 @@ struct GroupedLowering {
     };
     struct Group {
         Charge delta;
-        std::vector<uint64_t> K, N;
-        std::unordered_map<uint64_t, uint32_t> kidx, nidx;
+        std::vector<uint32_t> K, N;          // the group's keys, ascending
+        std::vector<int32_t> kpos, npos;     // key -> position in K / N, -1 if absent
         uint64_t M = 0, t_off = 0, s_off = 0, w_off = 0, bf_off = 0;
         std::vector<uint32_t> pairs;
+        uint32_t kid(uint32_t k) const {
+            MQ_REQUIRE(kpos[k] >= 0, "key not in its group");
+            return uint32_t(kpos[k]);
+        }
+        uint32_t nid(uint32_t k) const {
+            MQ_REQUIRE(npos[k] >= 0, "key not in its group");
+            return uint32_t(npos[k]);
+        }
     };
     std::vector<Pair> pairs;
     std::vector<Group> groups;
     std::unordered_map<uint64_t, uint32_t> pair_of;   // (lb_in, rb_out) -> pair
-    std::vector<double> w_dense;
-    uint64_t ref_flops = 0;
 };
+
+// The S-stage part of the reference's flop model of a grouped sigma (tasks.cpp:388-398, 777-796):
+// one alpha (2 flops per element of the pair's slab) per nonzero MPO coupling (k, n) between a
+// key k the pair multiplies in the T stage and a key n it contracts in the final stage.
+uint64_t grouped_s_flops(const GroupedLowering& G, const double* wd, uint64_t wd_off, const std::vector<uint32_t>& lsz,
+                         const std::vector<uint32_t>& rsz) {
+    uint64_t f = 0;
+    for (const auto& g : G.groups) {
+        const uint64_t Kg = g.K.size(), Ng = g.N.size();
+        const double* Wg = wd + (g.w_off - wd_off);
+        std::vector<std::vector<uint32_t>> col_nz(Ng);
+        for (uint64_t n = 0; n < Ng; ++n)
+            for (uint64_t k = 0; k < Kg; ++k)
+                if (Wg[n * Kg + k] != 0.0) col_nz[n].push_back(uint32_t(k));
+        std::vector<char> kin(Kg);
+        for (uint32_t pi : g.pairs) {
+            const auto& pr = G.pairs[pi];
+            std::fill(kin.begin(), kin.end(), 0);
+            for (auto k : pr.kk) kin[k] = 1;
+            uint64_t na = 0;
+            for (auto n : pr.nn)
+                for (auto k : col_nz[n]) na += kin[k];
+            f += 2ull * lsz[pr.lb_in] * rsz[pr.rb_out] * na;
+        }
+    }
+    return f;
+}
 
 // A boundary step lowered by the grouped path: kind Schedule::LEFT (T = L psi) or RIGHT (T = psi R),
 // the new boundary's index and cohort offsets (the final stage's output).
@@ -1549,7 +1611,9 @@ struct GroupedStep {
     uint32_t lo = 0, hi = 0xffffffffu;
 };
 
-void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::vector<double>& ws_init,
+// The dense W_D go to p->ctx's pinned staging buffer (wd_elems of them, at ws_init_off of the
+// workspace), which the caller uploads and synchronizes before the next lowering.
+void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint64_t& wd_elems,
                          uint64_t& ws_init_off, bool adjoint, const GroupedStep* step = nullptr) {
     const Index &left_i = site.left_i, &right_i = site.right_i, &phys_i = site.phys_i;
     const int kind = step ? step->kind : Schedule::SIGMA;
@@ -1572,6 +1636,16 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
     if (lstep) MQ_REQUIRE(NI.aux_dim == W.col_dim, "new left boundary aux_dim != mpo column dimension");
     ProductBasis rpb(phys_i, right_i, -1);
     GroupedLowering G;
+    G.soff.resize(phys_i.size());
+    for (uint32_t s = 0; s < phys_i.size(); ++s) {
+        G.soff[s] = G.nst;
+        G.nst += phys_i.sizes[s];
+    }
+    {
+        const uint64_t nk = uint64_t(std::max(W.row_dim, W.col_dim)) * G.nst;
+        MQ_REQUIRE(nk < (1ull << 31), "MPO bond dimension x physical dimension too large");
+        G.nkeys = uint32_t(nk);
+    }
     std::map<uint64_t, uint32_t> group_of;   // delta key -> group (ordered: deterministic numbering)
     // stored block j of a (non-transposed) cohort -> MPO index b
     auto block_order = [](const BoundaryIndex& B, uint32_t ci) {
@@ -1600,13 +1674,13 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             // s_out, ss) of the boundary the final stage contracts with (sigma: L; steps: the new
             // boundary's stored cohorts).  For the left step the pair is (bra left sector, ket right
             // sector) and T = L psi; otherwise (ket left sector, output right sector) and T = psi R.
-            std::vector<uint64_t> K, N;
-            auto add_keys = [&](std::vector<uint64_t>& out, const BoundaryIndex& B, uint32_t c, uint32_t s,
+            std::vector<uint32_t> K, N;
+            auto add_keys = [&](std::vector<uint32_t>& out, const BoundaryIndex& B, uint32_t c, uint32_t s,
                                 bool stored_only) {
                 if (c >= B.n_cohorts() || (stored_only && B.tr[c])) return;
                 for (uint32_t b = 0; b < B.aux_dim; ++b)
                     if (B.has_block(c, b))
-                        for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) out.push_back(GroupedLowering::kkey(b, s, ss));
+                        for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) out.push_back(G.key(b, s, ss));
             };
             for (uint32_t s = 0; s < phys_i.size(); ++s) {
                 if (lstep) {
@@ -1627,6 +1701,8 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             if (ins.second) {
                 G.groups.emplace_back();
                 G.groups.back().delta = d;
+                G.groups.back().kpos.assign(G.nkeys, -1);
+                G.groups.back().npos.assign(G.nkeys, -1);
             }
             GroupedLowering::Group& g = G.groups[ins.first->second];
             GroupedLowering::Pair pr;
@@ -1635,38 +1711,33 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             pr.g = ins.first->second;
             pr.m_off = g.M;
             g.M += uint64_t(left_i.sizes[lb]) * right_i.sizes[rb];
-            for (auto k : K) {
-                auto it = g.kidx.emplace(k, uint32_t(g.K.size()));
-                if (it.second) g.K.push_back(k);
-                pr.kk.push_back(it.first->second);
-            }
-            for (auto n : N) {
-                auto it = g.nidx.emplace(n, uint32_t(g.N.size()));
-                if (it.second) g.N.push_back(n);
-                pr.nn.push_back(it.first->second);
-            }
+            // keys for now; numbered within the group once every pair is known
+            for (auto k : K) g.kpos[k] = 0;
+            for (auto n : N) g.npos[n] = 0;
+            pr.kk = std::move(K);
+            pr.nn = std::move(N);
             g.pairs.push_back(uint32_t(G.pairs.size()));
             G.pair_of[uint64_t(lb) << 32 | rb] = uint32_t(G.pairs.size());
             G.pairs.push_back(std::move(pr));
         }
     }
-    // ---- order each group's keys by (b, s, ss): W_D then follows the MPO's block structure, and
-    // its all-zero tiles are skipped in the S GEMM
+    trace_phase("lower: pairs and keys");
+    // ---- number each group's keys in ascending (b, s, ss) order: W_D then follows the MPO's block
+    // structure, and its all-zero tiles are skipped in the S GEMM
     for (auto& g : G.groups) {
-        auto reorder = [](std::vector<uint64_t>& keys, std::unordered_map<uint64_t, uint32_t>& idx) {
-            std::vector<uint64_t> sorted = keys;
-            std::sort(sorted.begin(), sorted.end());
-            idx.clear();
-            for (uint32_t i = 0; i < sorted.size(); ++i) idx[sorted[i]] = i;
-            std::vector<uint32_t> perm(keys.size());
-            for (uint32_t i = 0; i < keys.size(); ++i) perm[i] = idx[keys[i]];
-            keys.swap(sorted);
-            return perm;
+        auto number = [&](std::vector<int32_t>& pos, std::vector<uint32_t>& keys) {
+            keys.clear();
+            for (uint32_t k = 0; k < G.nkeys; ++k)
+                if (pos[k] >= 0) {
+                    pos[k] = int32_t(keys.size());
+                    keys.push_back(k);
+                }
         };
-        std::vector<uint32_t> pk = reorder(g.K, g.kidx), pn = reorder(g.N, g.nidx);
+        number(g.kpos, g.K);
+        number(g.npos, g.N);
         for (uint32_t pi : g.pairs) {
-            for (auto& k : G.pairs[pi].kk) k = pk[k];
-            for (auto& n : G.pairs[pi].nn) n = pn[n];
+            for (auto& k : G.pairs[pi].kk) k = uint32_t(g.kpos[k]);
+            for (auto& n : G.pairs[pi].nn) n = uint32_t(g.npos[n]);
         }
     }
     // ---- workspace: T_D, S_D, W_D per group
@@ -1699,16 +1770,35 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         g.w_off = ws;
         ws += align32(uint64_t(g.K.size()) * g.N.size());
     }
-    ws_init.assign(ws - ws_init_off, 0.0);
+    wd_elems = ws - ws_init_off;
+    double* const wd = p->ctx->wd_stage(wd_elems);
+    trace_phase("lower: key order, layout");
+    if (g_trace) {
+        uint64_t tk = 0, tn = 0, tm = 0;
+        for (const auto& g : G.groups) tk += g.K.size(), tn += g.N.size(), tm += g.M;
+        std::fprintf(stderr, "[mq] groups %zu pairs %zu keys K %llu N %llu rows %llu, W_D %.1f MB, workspace %.1f GB\n",
+                     G.groups.size(), G.pairs.size(), (unsigned long long)tk, (unsigned long long)tn,
+                     (unsigned long long)tm, 8e-6 * double(ws - ws_init_off), 8e-9 * double(ws));
+    }
     struct AdjEntry { uint32_t t; uint64_t w; double c; };
     std::vector<AdjEntry> adj;
-    // ---- dense W_D: for every MPO entry, its (k, n) in the one group holding both keys
-    std::unordered_map<uint64_t, std::vector<std::pair<uint32_t, uint32_t>>> kgroups, ngroups;
-    for (uint32_t gi = 0; gi < G.groups.size(); ++gi) {
-        for (uint32_t i = 0; i < G.groups[gi].K.size(); ++i) kgroups[G.groups[gi].K[i]].push_back({gi, i});
-        for (uint32_t i = 0; i < G.groups[gi].N.size(); ++i) ngroups[G.groups[gi].N[i]].push_back({gi, i});
+    // ---- dense W_D: for every MPO entry, its (k, n) in each group holding both keys.  The groups of
+    // a K key are listed per key (CSR over the dense key numbering)
+    std::vector<uint32_t> kg_off(size_t(G.nkeys) + 1, 0);
+    for (const auto& g : G.groups)
+        for (uint32_t k : g.K) ++kg_off[k + 1];
+    for (uint32_t k = 0; k < G.nkeys; ++k) kg_off[k + 1] += kg_off[k];
+    std::vector<std::pair<uint32_t, uint32_t>> kg_ent(kg_off[G.nkeys]);
+    {
+        std::vector<uint32_t> fill(kg_off.begin(), kg_off.end() - 1);
+        for (uint32_t gi = 0; gi < G.groups.size(); ++gi)
+            for (uint32_t i = 0; i < G.groups[gi].K.size(); ++i) kg_ent[fill[G.groups[gi].K[i]]++] = {gi, i};
     }
-    for (uint32_t b1 = 0; b1 < W.row_dim; ++b1)
+    // Rows b1 of the MPO write disjoint W_D entries (b1 is part of every N key of sigma and the
+    // right step, and of every K key of the left step), so they are assembled in parallel; each
+    // entry still sums its contributions in MPO order.  The adjoint's entry list keeps MPO order.
+    std::atomic<int> bad{0};
+    auto mpo_row = [&](uint32_t b1, std::vector<AdjEntry>* adj_out) {
         for (const auto& cell : W.rows[b1]) {
             uint32_t b2 = cell.other;
             for (uint32_t t = cell.t0; t < cell.t1; ++t) {
@@ -1716,48 +1806,40 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 double sc = W.terms[t].scale;
                 for (const OpBlock& wb : op.blocks) {
                     uint32_t s_in = phys_i.position(wb.in), s_out = phys_i.position(wb.out);
-                    MQ_REQUIRE(s_in < phys_i.size() && s_out < phys_i.size(), "operator charge not in physical index");
+                    if (s_in >= phys_i.size() || s_out >= phys_i.size()) {
+                        bad = 1;
+                        return;
+                    }
                     for (uint32_t e = wb.e0; e < wb.e1; ++e) {
+                        if (op.row[e] >= phys_i.sizes[s_in] || op.col[e] >= phys_i.sizes[s_out]) {
+                            bad = 2;
+                            return;
+                        }
                         // K keys carry the MPO index of the T-stage boundary (b2 for psi R, b1 for L psi)
-                        auto kit = kgroups.find(GroupedLowering::kkey(lstep ? b1 : b2, s_in, op.row[e]));
-                        auto nit = ngroups.find(GroupedLowering::kkey(lstep ? b2 : b1, s_out, op.col[e]));
-                        if (kit == kgroups.end() || nit == ngroups.end()) continue;
-                        for (auto& kg : kit->second)
-                            for (auto& ng : nit->second)
-                                if (kg.first == ng.first) {
-                                    const auto& g = G.groups[kg.first];
-                                    const uint64_t w = g.w_off - ws_init_off + uint64_t(ng.second) * g.K.size() + kg.second;
-                                    ws_init[w] += op.coeff[e] * sc;
-                                    if (adjoint) adj.push_back({t, w, op.coeff[e]});
-                                }
+                        const uint32_t kd = G.key(lstep ? b1 : b2, s_in, op.row[e]);
+                        const uint32_t nd = G.key(lstep ? b2 : b1, s_out, op.col[e]);
+                        for (uint32_t q = kg_off[kd]; q < kg_off[kd + 1]; ++q) {
+                            const auto& g = G.groups[kg_ent[q].first];
+                            const int32_t ni = g.npos[nd];
+                            if (ni < 0) continue;
+                            const uint64_t w = g.w_off - ws_init_off + uint64_t(ni) * g.K.size() + kg_ent[q].second;
+                            wd[w] += op.coeff[e] * sc;
+                            if (adj_out) adj_out->push_back({t, w, op.coeff[e]});
+                        }
                     }
                 }
             }
         }
-    // reference flop model of the same sigma (tasks.cpp:388-398, 777-796): T + S (one alpha per
-    // nonzero coupling of a pair) + L
-    {
-        uint64_t f = 0;
-        for (auto& g : G.groups) {
-            uint64_t Kg = g.K.size(), Ng = g.N.size();
-            const double* Wg = ws_init.data() + (g.w_off - ws_init_off);
-            std::vector<std::vector<uint32_t>> col_nz(Ng);
-            for (uint64_t n = 0; n < Ng; ++n)
-                for (uint64_t k = 0; k < Kg; ++k)
-                    if (Wg[n * Kg + k] != 0.0) col_nz[n].push_back(uint32_t(k));
-            std::vector<char> kin(Kg);
-            for (uint32_t pi : g.pairs) {
-                const auto& pr = G.pairs[pi];
-                std::fill(kin.begin(), kin.end(), 0);
-                for (auto k : pr.kk) kin[k] = 1;
-                uint64_t na = 0;
-                for (auto n : pr.nn)
-                    for (auto k : col_nz[n]) na += kin[k];
-                f += 2ull * left_i.sizes[pr.lb_in] * right_i.sizes[pr.rb_out] * na;
-            }
-        }
-        G.ref_flops = f;
+    };
+    if (adjoint) {
+        for (uint32_t b1 = 0; b1 < W.row_dim; ++b1) mpo_row(b1, &adj);
+    } else {
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int b1 = 0; b1 < int(W.row_dim); ++b1) mpo_row(uint32_t(b1), nullptr);
     }
+    MQ_REQUIRE(bad != 1, "operator charge not in physical index");
+    MQ_REQUIRE(bad != 2, "operator element outside its physical sector");
+    trace_phase("lower: W_D");
     StagePlan& P = p->st;
     if (!lstep) {
         // ---- T stage.  The psi blocks that meet one right cohort (every (lb_in, s, ss) with
@@ -1787,7 +1869,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
         }
         // one stacked task over the given psi blocks: row units with the column keys of `b2s`
         auto stacked_task = [&](const Stack& st, const std::vector<uint32_t>& b2s, const GemmSeg& sg) {
-            std::vector<uint32_t> unit_of_row;
+            std::vector<uint32_t> unit_of_blk;
             for (const RowBlock& rbk : st.blocks) {
                 const auto& pr = G.pairs[rbk.pair];
                 const auto& g = G.groups[pr.g];
@@ -1800,7 +1882,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 R.a_si = 1;
                 R.a_sk = M;
                 R.a_scale = 1.0;
-                R.row0 = uint32_t(unit_of_row.size());
+                R.row0 = uint32_t(unit_of_blk.size()) * 16;
                 R.nrows = M;
                 R.c_src = g.t_off + pr.m_off;
                 R.c_split = g.bf_off + pr.m_off;
@@ -1808,16 +1890,16 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 R.cldk = uint32_t(g.M);
                 R.ldc = M;
                 R.cmap = uint32_t(P.maps.size());
-                for (uint32_t b2 : b2s) P.maps.push_back(g.kidx.at(GroupedLowering::kkey(b2, rbk.s, rbk.ss)));
-                unit_of_row.insert(unit_of_row.end(), (M + 15) / 16 * 16, uint32_t(P.rows.size()));
+                for (uint32_t b2 : b2s) P.maps.push_back(g.kid(G.key(b2, rbk.s, rbk.ss)));
+                unit_of_blk.insert(unit_of_blk.end(), (M + 15) / 16, uint32_t(P.rows.size()));
                 P.rows.push_back(R);
             }
-            if (unit_of_row.empty() || !st.brs) return;
-            const uint32_t Mt = uint32_t(unit_of_row.size());
+            if (unit_of_blk.empty() || !st.brs) return;
+            const uint32_t Mt = uint32_t(unit_of_blk.size()) * 16;
             GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, uint32_t(b2s.size()) * st.brs, Mt, {sg});
             t.cbw = st.brs;
             t.rowunit = uint32_t(P.maps.size());
-            P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+            P.maps.insert(P.maps.end(), unit_of_blk.begin(), unit_of_blk.end());
         };
         for (const auto& kv : stacks) {
             const uint32_t ci = kv.first;
@@ -1872,34 +1954,38 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                     const auto& g = G.groups[pr.g];
                     const uint32_t rs = right_i.sizes[rb];
                     for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
-                        std::vector<uint32_t> unit_of_row;
+                        std::vector<uint32_t> unit_of_blk;
                         for (const LBlk& bk : blks) {
                             GemmRow R{};
                             R.a_src = bk.a;
                             R.a_si = bk.si;
                             R.a_sk = bk.sk;
                             R.a_scale = bk.sc;
-                            R.row0 = uint32_t(unit_of_row.size());
+                            R.row0 = uint32_t(unit_of_blk.size()) * 16;
                             R.nrows = lsb;
-                            R.c_src = g.t_off + pr.m_off + uint64_t(g.kidx.at(GroupedLowering::kkey(bk.b1, s, ss))) * g.M;
+                            R.c_src = g.t_off + pr.m_off + uint64_t(g.kid(G.key(bk.b1, s, ss))) * g.M;
                             R.ldc = lsb;
                             R.c_si = 1;
                             R.cmap = MAP_NONE;
-                            unit_of_row.insert(unit_of_row.end(), (lsb + 15) / 16 * 16, uint32_t(P.rows.size()));
+                            unit_of_blk.insert(unit_of_blk.end(), (lsb + 15) / 16, uint32_t(P.rows.size()));
                             P.rows.push_back(R);
                         }
-                        const uint32_t Mt = uint32_t(unit_of_row.size());
+                        const uint32_t Mt = uint32_t(unit_of_blk.size()) * 16;
                         const uint64_t col0 = rpb(s, rb) + uint64_t(ss) * rs;
                         GemmTask& t = add_gemm(P.t_tasks, P.t_segs, BASE_WS, 0, Mt, rs, Mt,
                                                {seg(BASE_LEFT, 0, 1, false, BASE_IN, site.rp_offset[lbk] + col0 * lsk,
                                                     lsk, false, lsk)});
                         t.cbw = rs;
                         t.rowunit = uint32_t(P.maps.size());
-                        P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+                        P.maps.insert(P.maps.end(), unit_of_blk.begin(), unit_of_blk.end());
                     }
                 }
             }
     }
+    trace_phase("lower: T stage");
+    if (g_trace)
+        std::fprintf(stderr, "[mq] T tasks %zu segs %zu rows %zu maps %zu\n", P.t_tasks.size(), P.t_segs.size(),
+                     P.rows.size(), P.maps.size());
     P.split_t = true;   // every T task is stacked: BF16x3 writes A' from the T epilogue
     // ---- zero chunks: the (pair, key) slabs of T_D that no T task writes, merged along the rows
     // of each key column (consecutive pairs of a group are consecutive rows)
@@ -1934,6 +2020,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             flush();
         }
     }
+    trace_phase("lower: zero chunks");
     // ---- S stage: S_D = T_D W_D, one plain dense GEMM per group (rocBLAS; the MPO coefficient
     // matrices are 10-30% dense, and the library GEMM runs at near the MFMA peak)
     for (auto& g : G.groups) {
@@ -1991,7 +2078,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                         d.ld = uint32_t(g.M);
                         d.map = uint32_t(P.maps.size());
                         d.valid = 1;
-                        for (uint32_t b1 : b1s) P.maps.push_back(g.nidx.at(GroupedLowering::kkey(b1, U.s, U.ss)));
+                        for (uint32_t b1 : b1s) P.maps.push_back(g.nid(G.key(b1, U.s, U.ss)));
                     }
                     P.units.push_back(d);
                 }
@@ -2112,18 +2199,18 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 if (NI.has_block(cn, b)) bidx.push_back(b), boff.push_back(uint64_t(NI.offset(cn, b)));
             if (bidx.empty()) continue;
             MQ_REQUIRE(bs == uint64_t(rows_u) * ncol, "new cohort block shape");
-            std::vector<uint32_t> unit_of_row;
+            std::vector<uint32_t> unit_of_blk;
             for (uint32_t j = 0; j < bidx.size(); ++j) {
                 GemmRow R{};
                 R.a_scale = 1.0;
-                R.row0 = uint32_t(unit_of_row.size());
+                R.row0 = uint32_t(unit_of_blk.size()) * 16;
                 R.nrows = rows_u;
                 R.jblk = j;
                 R.c_src = (*step->out_off)[cn] + boff[j];
                 R.ldc = rstep ? rows_u : 1;      // right: (ket x bra) column-major; left: the transpose
                 R.c_si = rstep ? 1 : ncol;
                 R.cmap = MAP_NONE;
-                unit_of_row.insert(unit_of_row.end(), (rows_u + 15) / 16 * 16, uint32_t(P.rows.size()));
+                unit_of_blk.insert(unit_of_blk.end(), (rows_u + 15) / 16, uint32_t(P.rows.size()));
                 P.rows.push_back(R);
             }
             std::vector<GemmSeg> segs;
@@ -2150,7 +2237,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 const uint32_t width = rstep ? right_i.sizes[pr_sec] : right_i.sizes[a_sec];   // unit columns
                 for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
                     const uint32_t amap = uint32_t(P.maps.size());
-                    for (uint32_t b : bidx) P.maps.push_back(g.nidx.at(GroupedLowering::kkey(b, s, ss)));
+                    for (uint32_t b : bidx) P.maps.push_back(g.nid(G.key(b, s, ss)));
                     const uint64_t c0 = col0 + uint64_t(ss) * width;
                     GemmSeg sg;
                     if (rstep)   // A = S slab (ket rows x r columns), B = bra columns transposed
@@ -2165,11 +2252,11 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 }
             }
             if (segs.empty()) continue;
-            const uint32_t Mt = uint32_t(unit_of_row.size());
+            const uint32_t Mt = uint32_t(unit_of_blk.size()) * 16;
             GemmTask& t = add_gemm(P.f_tasks, P.f_segs, BASE_OUT, 0, Mt, ncol, Mt, segs);
             t.cbw = ncol;
             t.rowunit = uint32_t(P.maps.size());
-            P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+            P.maps.insert(P.maps.end(), unit_of_blk.begin(), unit_of_blk.end());
         }
     }
     if (adjoint) {
@@ -2199,7 +2286,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                 if (!rs) continue;
                 for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) {
                     const uint32_t col0 = uint32_t(rpb(s, rb)) + ss * rs;
-                    std::vector<uint32_t> unit_of_row;
+                    std::vector<uint32_t> unit_of_blk;
                     for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
                         const uint32_t cl = LI.cohort_index(lc_out, left_i.charges[lb]);
                         const uint32_t ls_in = left_i.sizes[lb];
@@ -2214,13 +2301,13 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                             R.a_si = a_si;
                             R.a_sk = a_sk;
                             R.a_scale = scale;
-                            R.row0 = uint32_t(unit_of_row.size());
+                            R.row0 = uint32_t(unit_of_blk.size()) * 16;
                             R.nrows = ls_in;
-                            R.c_src = g.s_off + pr.m_off + uint64_t(g.nidx.at(GroupedLowering::kkey(b1, s, ss))) * g.M;
+                            R.c_src = g.s_off + pr.m_off + uint64_t(g.nid(G.key(b1, s, ss))) * g.M;
                             R.ldc = ls_in;
                             R.c_si = 1;
                             R.cmap = MAP_NONE;
-                            unit_of_row.insert(unit_of_row.end(), (ls_in + 15) / 16 * 16, uint32_t(P.rows.size()));
+                            unit_of_blk.insert(unit_of_blk.end(), (ls_in + 15) / 16, uint32_t(P.rows.size()));
                             P.rows.push_back(R);
                         };
                         if (!LI.tr[cl]) {
@@ -2237,22 +2324,22 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
                             }
                         }
                     }
-                    if (unit_of_row.empty()) continue;
-                    const uint32_t Mt = uint32_t(unit_of_row.size());
+                    if (unit_of_blk.empty()) continue;
+                    const uint32_t Mt = uint32_t(unit_of_blk.size()) * 16;
                     GemmTask& t = add_gemm(P.u_tasks, P.u_segs, BASE_WS, 0, Mt, rs, Mt,
                                            {seg(BASE_LEFT, 0, 1, false, BASE_IN,
                                                 site.rp_offset[lbo] + uint64_t(col0) * ls_out, ls_out, false, ls_out)});
                     t.cbw = rs;
                     t.rowunit = uint32_t(P.maps.size());
-                    P.maps.insert(P.maps.end(), unit_of_row.begin(), unit_of_row.end());
+                    P.maps.insert(P.maps.end(), unit_of_blk.begin(), unit_of_blk.end());
                 }
             }
         }
     }
     P.ws_elems = ws;
-    p->flops = G.ref_flops;
     if (!sigma) return;
-    // T and L reference flops
+    // T and L reference flops now; the S part (a scan of W_D per pair) when first asked for
+    p->flops = 0;
     for (auto& pr : G.pairs) {
         uint32_t lb = pr.lb_in, rb = pr.rb_out;
         for (uint32_t s = 0; s < phys_i.size(); ++s) {
@@ -2270,6 +2357,21 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, std::
             uint32_t cle = LI.tr[cl] ? LI.cohort_index(left_i.charges[lb], left_i.charges[lbo]) : cl;
             p->flops += 2ull * left_i.sizes[lbo] * left_i.sizes[lb] * LI.n_blocks[cle] * site.rp_cols[lbo];
         }
+    for (auto& g : G.groups) {
+        std::vector<int32_t>().swap(g.kpos);
+        std::vector<int32_t>().swap(g.npos);
+    }
+    auto lowered = std::make_shared<const GroupedLowering>(std::move(G));
+    p->s_flops = [lowered, p, lsz = left_i.sizes, rsz = right_i.sizes] {
+        // W_D from the plan's workspace (no application writes that region)
+        std::vector<double> wd(p->ws_init_elems);
+        if (!wd.empty()) {
+            HIPCHK(hipMemcpyAsync(wd.data(), p->ws.as<double>() + p->ws_init_off, wd.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, p->ctx->stream));
+            HIPCHK(hipStreamSynchronize(p->ctx->stream));
+        }
+        return grouped_s_flops(*lowered, wd.data(), p->ws_init_off, lsz, rsz);
+    };
 }
 
 }  // namespace
@@ -2318,6 +2420,7 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
                             const mq_boundary* right, int algorithm, mq_plan** out) {
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && left && right && out, "bad argument");
+        trace_phase("(caller)");
         SiteLayout L(*site);
         if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
         if (algorithm == (MQ_SIGMA_AUTO | MQ_SIGMA_ADJOINT)) algorithm = MQ_SIGMA_GROUPED | MQ_SIGMA_ADJOINT;
@@ -2336,13 +2439,15 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         p->left = left;
         p->right = right;
         p->vec_size = L.rp_total;
-        std::vector<double> ws_init;
-        uint64_t ws_init_off = 0;
-        lower_sigma_grouped(p.get(), L, W, ws_init, ws_init_off, adjoint);
+        uint64_t wd_elems = 0, ws_init_off = 0;
+        trace_phase("plan: layout, mpo");
+        lower_sigma_grouped(p.get(), L, W, wd_elems, ws_init_off, adjoint);
+        trace_phase("lower: final stage");
         StagePlan& P = p->st;
         P.blas = ctx->blas;
         P.lt_ctx = ctx;
         P.finish_tables();
+        trace_phase("plan: tables");
         P.sizes[BASE_WS] = P.ws_elems;
         P.sizes[BASE_IN] = P.sizes[BASE_OUT] = L.rp_total;
         P.sizes[BASE_LEFT] = left->total;
@@ -2351,13 +2456,15 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         P.validate();
         if (P.part_elems) ctx->take(p->part, P.part_elems * sizeof(double));
         ctx->take(p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
-        if (!ws_init.empty())
-            HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
+        if (wd_elems)
+            HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ctx->wd_host, wd_elems * sizeof(double),
                                   hipMemcpyHostToDevice, ctx->stream));
+        trace_phase("plan: workspaces");
         P.upload_all(ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        trace_phase("plan: upload");
         p->grouped = true;
-        p->ws_init = std::move(ws_init);
+        p->ws_init_elems = wd_elems;
         p->ws_init_off = ws_init_off;
         *out = p.release();
     });
@@ -2719,6 +2826,10 @@ int mq_plan_vector_size(const mq_plan* p, uint64_t* n) {
 int mq_plan_flops(const mq_plan* p, uint64_t* f) {
     return guarded([&] {
         MQ_REQUIRE(p && f, "bad argument");
+        if (p->s_flops) {
+            p->flops += p->s_flops();
+            p->s_flops = nullptr;
+        }
         *f = p->flops;
     });
 }
@@ -2920,13 +3031,16 @@ static int right_step_schedule(mq_ctx* ctx, const mq_site_desc* site, const mq_m
 static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const mq_mpo_desc* mpo, const double* bra_rp,
                         const double* ket_rp, const mq_boundary* bnd, int symmetric, mq_boundary** out) {
     return guarded([&] {
+        trace_phase("(caller)");
         SiteLayout L(*site);
         Mpo W(*mpo);
+        trace_phase("step: layout, mpo");
         auto nb = std::make_unique<mq_boundary>();
         nb->ctx = ctx;
         nb->index = kind == Schedule::LEFT ? left_step_index(L, W, bnd->index, symmetric != 0)
                                            : right_step_index(L, W, bnd->index, symmetric != 0);
         nb->layout();
+        trace_phase("step: new index");
         nb->data.alloc(std::max<uint64_t>(nb->total, 1) * sizeof(double));
         HIPCHK(launch_fill(ctx->stream, nb->ptr(), std::max<uint64_t>(nb->total, 1), 0.0));
         // The step's intermediates (T_D, S_D) are several times the new boundary.  The ket sectors
@@ -2945,12 +3059,11 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
             p.ctx = ctx;
             p.left = kind == Schedule::LEFT ? bnd : nullptr;
             p.right = kind == Schedule::RIGHT ? bnd : nullptr;
-            std::vector<double> ws_init;
-            uint64_t ws_init_off = 0;
+            uint64_t wd_elems = 0, ws_init_off = 0;
             GroupedStep spec{kind, &nb->index, &nb->cohort_off};
             spec.lo = range.first;
             spec.hi = range.second;
-            lower_sigma_grouped(&p, L, W, ws_init, ws_init_off, false, &spec);
+            lower_sigma_grouped(&p, L, W, wd_elems, ws_init_off, false, &spec);
             StagePlan& P = p.st;
             if (P.ws_elems > budget && range.second - range.first > 1) {
                 const uint32_t mid = range.first + (range.second - range.first) / 2;
@@ -2967,12 +3080,13 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
             P.validate();
             P.upload_all(ctx->stream);
             // workspace: the context's step buffer (cleared whenever it grows), W_D copied in
+            trace_phase("step: tables");
             const size_t need = std::max<uint64_t>(P.ws_elems, 32) * sizeof(double);
             if (need > ctx->step_ws.bytes) ctx->step_ws.alloc_zeroed(need, ctx->stream);
             double* ws = ctx->step_ws.as<double>();
-            if (!ws_init.empty())
-                HIPCHK(hipMemcpyAsync(ws + ws_init_off, ws_init.data(), ws_init.size() * sizeof(double),
-                                      hipMemcpyHostToDevice, ctx->stream));
+            if (wd_elems)
+                HIPCHK(hipMemcpyAsync(ws + ws_init_off, ctx->wd_host, wd_elems * sizeof(double), hipMemcpyHostToDevice,
+                                      ctx->stream));
             Bases bs{};
             bs.p[BASE_WS] = ws;
             bs.p[BASE_IN] = const_cast<double*>(ket_rp);
@@ -2980,8 +3094,11 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
             bs.p[BASE_BRA] = const_cast<double*>(bra_rp);
             bs.p[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = const_cast<double*>(bnd->ptr());
             BLASCHK(rocblas_set_stream(ctx->blas, ctx->stream));
+            if (g_trace) HIPCHK(hipStreamSynchronize(ctx->stream));
+            trace_phase("step: workspace");
             P.run(ctx->stream, bs, MODE_F64);
-            HIPCHK(hipStreamSynchronize(ctx->stream));   // the chunk's tables and ws_init are freed next
+            HIPCHK(hipStreamSynchronize(ctx->stream));   // the chunk's tables and the W_D staging are reused next
+            trace_phase("step: run");
         }
         ctx->step_ws.release();   // steps run once per site: keep no workspace between them
         *out = nb.release();

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
     double a_rscale = 1.0;
     if (stacked) {
         const uint32_t m = m0 + uint32_t(tid % BM);
-        const GemmRow R = rows[maps[tk.rowunit + min(m, M - 1)]];
+        const GemmRow R = rows[maps[tk.rowunit + min(m, M - 1) / 16]];
         const uint32_t i = m - R.row0;
         a_ok = uint32_t(m < M) & uint32_t(i < R.nrows);
         a_row = R.a_src + uint64_t(i) * R.a_si;
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
         CT* __restrict__ C = reinterpret_cast<CT*>(bases.p[tk.c_base]) + tk.c_off;
         uint32_t uid[WM];
 #pragma unroll
-        for (int i = 0; i < WM; ++i) uid[i] = maps[tk.rowunit + min(m0 + wm + 16 * i, M - 1)];
+        for (int i = 0; i < WM; ++i) uid[i] = maps[tk.rowunit + min(m0 + wm + 16 * i, M - 1) / 16];
         GemmRow R[WM];
 #pragma unroll
         for (int i = 0; i < WM; ++i) R[i] = rows[uid[i]];

@@ -55,7 +55,8 @@ struct GemmTask {
 // stacked into the rows of one task, so the B panel of a column tile is read once for all of them.
 // Unit u occupies rows [R.row0, R.row0 + span) of the task, span = nrows rounded up to a multiple
 // of 16 (row0 too is a multiple of 16, so every 16-row MFMA block lies in one unit); row m belongs
-// to unit u = maps[rowunit + m], with descriptor R = row table[u] and local row i = m - R.row0:
+// to unit u = maps[rowunit + m / 16] (one entry per 16-row block), with descriptor R = row
+// table[u] and local row i = m - R.row0:
 //     A(m, k)  = R.a_scale * A[a_off + R.a_src + i * R.a_si + k * R.a_sk]   i < R.nrows, else 0
 //     C(m, n)  = C[c_off + R.c_src + maps[R.cmap + n / cbw] * R.cldk + (n % cbw) * R.ldc + i * R.c_si]
 // (R.cmap == MAP_NONE: C[c_off + R.c_src + n * R.ldc + i * R.c_si]); padding rows are not stored.

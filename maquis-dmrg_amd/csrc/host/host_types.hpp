@@ -254,6 +254,11 @@ Schedule build_left_schedule(const SiteLayout& site, const Mpo& mpo, const Bound
 Schedule build_right_schedule(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& right,
                               bool symmetric);
 // The boundary index those two step schedules produce, without their task lists.
+// Host-side phase timings of plan construction, printed to stderr when MQ_TRACE is set in the
+// environment: trace_phase(name) reports the time since the previous call on this thread.
+bool trace_enabled();
+void trace_phase(const char* name);
+
 BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& left, bool symmetric);
 BoundaryIndex right_step_index(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& right, bool symmetric);
 uint64_t schedule_flops(const Schedule& s, const BoundaryIndex& t_index, const std::vector<uint32_t>& lr_ket_sizes);

@@ -10,6 +10,9 @@
 // T tuples are looked up through a hash map instead of the reference's linear get_ti scan
 // (tasks.cpp:766-775); MPS blocks are built in parallel and concatenated in block order.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -800,6 +803,19 @@ Schedule build_right_schedule(const SiteLayout& site, const Mpo& mpo, const Boun
     return S;
 }
 
+bool trace_enabled() {
+    static const bool on = std::getenv("MQ_TRACE") != nullptr;
+    return on;
+}
+
+void trace_phase(const char* name) {
+    static thread_local std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    if (!trace_enabled()) return;
+    const auto t = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[mq] %-28s %9.3f ms\n", name, std::chrono::duration<double, std::milli>(t - t0).count());
+    t0 = t;
+}
+
 // ============================================================================ index-only steps
 // The BoundaryIndex that build_left_schedule / build_right_schedule produce, without building their
 // task lists.  A line b of a cohort is present iff some sparse-operator entry reaches it: entries
@@ -811,28 +827,35 @@ struct Flow {
     uint32_t b, s_out, s_in;   // the other MPO index, physical sectors of the bra and ket side
 };
 
+constexpr uint32_t MAX_PHYS_PAIRS = 1024;   // (bra, ket) physical sector pairs of one site
+
 // per MPO index (col b2 for the left step, row b1 for the right step) the distinct flows of
 // non-empty operator blocks
 std::vector<std::vector<Flow>> mpo_flows(const Mpo& mpo, const Index& phys_i, bool by_col) {
-    uint32_t n = by_col ? mpo.col_dim : mpo.row_dim;
+    const uint32_t n = by_col ? mpo.col_dim : mpo.row_dim, n_other = by_col ? mpo.row_dim : mpo.col_dim;
+    const uint32_t np = phys_i.size();
     const auto& cells = by_col ? mpo.cols : mpo.rows;
+    // operator -> its flows (so, si) of non-empty blocks, once
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> op_flows(mpo.ops.size());
+    for (size_t o = 0; o < mpo.ops.size(); ++o)
+        for (const OpBlock& wb : mpo.ops[o].blocks) {
+            if (wb.e1 <= wb.e0) continue;
+            uint32_t so = phys_i.position(wb.out), si = phys_i.position(wb.in);
+            MQ_REQUIRE(so < np && si < np, "operator charge not in physical index");
+            op_flows[o].push_back({so, si});
+        }
+    // first-seen order per b, duplicates found through a stamp per (other, so, si)
+    std::vector<uint32_t> stamp(size_t(n_other) * np * np, 0);
     std::vector<std::vector<Flow>> out(n);
-    for (uint32_t b = 0; b < n; ++b) {
-        std::vector<uint64_t> seen;
+    for (uint32_t b = 0; b < n; ++b)
         for (const auto& cell : cells[b])
-            for (uint32_t t = cell.t0; t < cell.t1; ++t) {
-                const Op& W = mpo.ops[mpo.terms[t].op];
-                for (const OpBlock& wb : W.blocks) {
-                    if (wb.e1 <= wb.e0) continue;
-                    uint32_t so = phys_i.position(wb.out), si = phys_i.position(wb.in);
-                    MQ_REQUIRE(so < phys_i.size() && si < phys_i.size(), "operator charge not in physical index");
-                    uint64_t key = (uint64_t(cell.other) << 32) | (uint64_t(so) << 16) | si;
-                    if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
-                    seen.push_back(key);
-                    out[b].push_back({cell.other, so, si});
+            for (uint32_t t = cell.t0; t < cell.t1; ++t)
+                for (const auto& f : op_flows[mpo.terms[t].op]) {
+                    uint32_t& st = stamp[(size_t(cell.other) * np + f.first) * np + f.second];
+                    if (st == b + 1) continue;
+                    st = b + 1;
+                    out[b].push_back({cell.other, f.first, f.second});
                 }
-            }
-    }
     return out;
 }
 
@@ -861,9 +884,11 @@ BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const Boun
     require_bond(left, left_i, "left");
     const Symm sy = site.symm;
     const uint32_t nr = right_i.size(), np = phys_i.size(), w = mpo.col_dim;
+    MQ_REQUIRE(np * np <= MAX_PHYS_PAIRS, "too many physical sectors");
     auto flows = mpo_flows(mpo, phys_i, true);
     auto down = shift_table(right_i, phys_i, -1, left_i);   // rc - phys -> left sector
     auto coh = cohort_table(left, left_i, left_i);
+    trace_phase("left index: flows");
     // lines[(rb_ket * nr + rb_bra) * w + b2]
     std::vector<uint8_t> lines(size_t(nr) * nr * w, 0);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -873,13 +898,21 @@ BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const Boun
             Charge rc_bra = right_i.charges[rbb];
             if (std::abs(sy.particle_number(rc_bra) - sy.particle_number(rc_ket)) > 2) continue;
             uint8_t* ln = &lines[(size_t(rbk) * nr + rbb) * w];
+            // the left cohort's offsets row of every flow sector pair (so, si), or null
+            const int64_t* crow[MAX_PHYS_PAIRS];
+            for (uint32_t so = 0; so < np; ++so)
+                for (uint32_t si = 0; si < np; ++si) {
+                    uint32_t lbb = down[size_t(rbb) * np + so], lbk = down[size_t(rbk) * np + si];
+                    uint32_t c = (lbb == left_i.size() || lbk == left_i.size()) ? left.n_cohorts()
+                                                                                 : coh[size_t(lbb) * left_i.size() + lbk];
+                    crow[so * np + si] = c < left.n_cohorts() ? &left.offsets[size_t(c) * left.aux_dim] : nullptr;
+                }
             for (uint32_t b2 = 0; b2 < w; ++b2) {
                 if (symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) continue;
                 if (sy.su2() && !triangle(sy.spin(rc_ket), mpo.right_spin[b2], sy.spin(rc_bra))) continue;
                 for (const Flow& f : flows[b2]) {
-                    uint32_t lbb = down[size_t(rbb) * np + f.s_out], lbk = down[size_t(rbk) * np + f.s_in];
-                    if (lbb == left_i.size() || lbk == left_i.size()) continue;
-                    if (left.has_block(coh[size_t(lbb) * left_i.size() + lbk], f.b)) {
+                    const int64_t* row = crow[f.s_out * np + f.s_in];
+                    if (row && row[f.b] > -1) {
                         ln[b2] = 1;
                         break;
                     }
@@ -887,6 +920,7 @@ BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const Boun
             }
         }
     }
+    trace_phase("left index: lines");
     BoundaryIndex out(right_i, right_i, w);
     std::vector<int64_t> off(w);
     for (uint32_t rbk = 0; rbk < nr; ++rbk)
@@ -897,7 +931,9 @@ BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const Boun
             for (uint32_t b = 0; b < w; ++b) off[b] = ln[b] ? int64_t(bs * cnt++) : -1;
             if (cnt) out.add_cohort(rbb, rbk, off);
         }
+    trace_phase("left index: cohorts");
     if (symmetric) out.complement_transpose(mpo.right_herm, true, sy);
+    trace_phase("left index: transposes");
     return out;
 }
 
@@ -907,6 +943,7 @@ BoundaryIndex right_step_index(const SiteLayout& site, const Mpo& mpo, const Bou
     require_bond(right, right_i, "right");
     const Symm sy = site.symm;
     const uint32_t nl = left_i.size(), np = phys_i.size(), w = mpo.row_dim;
+    MQ_REQUIRE(np * np <= MAX_PHYS_PAIRS, "too many physical sectors");
     auto flows = mpo_flows(mpo, phys_i, false);
     auto up = shift_table(left_i, phys_i, +1, right_i);   // lc + phys -> right sector
     auto coh = cohort_table(right, right_i, right_i);     // (ket sector, bra sector) of the right boundary
@@ -918,13 +955,21 @@ BoundaryIndex right_step_index(const SiteLayout& site, const Mpo& mpo, const Bou
             Charge lc_bra = left_i.charges[lbb];
             if (std::abs(sy.particle_number(lc_ket) - sy.particle_number(lc_bra)) > 2) continue;
             uint8_t* ln = &lines[(size_t(lbk) * nl + lbb) * w];
+            // the right cohort's offsets row of every flow sector pair (so, si), or null
+            const int64_t* crow[MAX_PHYS_PAIRS];
+            for (uint32_t so = 0; so < np; ++so)
+                for (uint32_t si = 0; si < np; ++si) {
+                    uint32_t rbb = up[size_t(lbb) * np + so], rbk = up[size_t(lbk) * np + si];
+                    uint32_t c = (rbb == right_i.size() || rbk == right_i.size()) ? right.n_cohorts()
+                                                                                   : coh[size_t(rbk) * right_i.size() + rbb];
+                    crow[so * np + si] = c < right.n_cohorts() ? &right.offsets[size_t(c) * right.aux_dim] : nullptr;
+                }
             for (uint32_t b1 = 0; b1 < w; ++b1) {
                 if (symmetric && mpo.left_herm.skip(b1, lc_ket, lc_bra)) continue;
                 if (sy.su2() && !triangle(sy.spin(lc_ket), mpo.left_spin[b1], sy.spin(lc_bra))) continue;
                 for (const Flow& f : flows[b1]) {
-                    uint32_t rbb = up[size_t(lbb) * np + f.s_out], rbk = up[size_t(lbk) * np + f.s_in];
-                    if (rbb == right_i.size() || rbk == right_i.size()) continue;
-                    if (right.has_block(coh[size_t(rbk) * right_i.size() + rbb], f.b)) {
+                    const int64_t* row = crow[f.s_out * np + f.s_in];
+                    if (row && row[f.b] > -1) {
                         ln[b1] = 1;
                         break;
                     }

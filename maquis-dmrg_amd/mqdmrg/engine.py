@@ -286,12 +286,20 @@ class SigmaPlan:
         n = C.c_uint64()
         check(lib().mq_plan_vector_size(self.h, C.byref(n)))
         self.n = n.value
-        f = C.c_uint64()
-        check(lib().mq_plan_flops(self.h, C.byref(f)))
-        self.flops = f.value
+        self._flops = None
         b = C.c_uint64()
         check(lib().mq_plan_workspace_bytes(self.h, C.byref(b)))
         self.workspace_bytes = b.value
+
+    @property
+    def flops(self):
+        """The reference's flop model of one application (the S-stage part of a grouped plan is
+        counted on first access: it scans the dense MPO matrices once per sector pair)."""
+        if self._flops is None:
+            f = C.c_uint64()
+            check(lib().mq_plan_flops(self.h, C.byref(f)))
+            self._flops = f.value
+        return self._flops
 
     @staticmethod
     def from_schedule(engine, sched, ket_rows, ket_cols, left, right):
@@ -309,9 +317,7 @@ class SigmaPlan:
         n = C.c_uint64()
         check(lib().mq_plan_vector_size(self.h, C.byref(n)))
         self.n = n.value
-        f = C.c_uint64()
-        check(lib().mq_plan_flops(self.h, C.byref(f)))
-        self.flops = f.value
+        self._flops = None
         self.workspace_bytes = 0
         return self
 
