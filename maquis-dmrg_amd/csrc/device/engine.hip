@@ -164,7 +164,7 @@ struct mq_ctx {
     // workspaces of destroyed plans, reused by the next plans: a sweep creates one plan per site,
     // and allocating and releasing tens of GB per site costs more than the site's products
     std::vector<std::unique_ptr<DevBuf>> pool;
-    static constexpr size_t POOL_MAX = 4;
+    static constexpr size_t POOL_MAX = 8;
     // pinned host staging of the dense MPO matrices W_D of the plan (or step chunk) being lowered;
     // grows, and is reused by the next lowering once the previous upload has been synchronized
     double* wd_host = nullptr;
@@ -207,6 +207,7 @@ struct mq_ctx {
             HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), n / 8, 0.0));   // the part in use
             return;
         }
+        if (trace_enabled()) std::fprintf(stderr, "[mq] workspace pool miss: %.2f GB allocated\n", 1e-9 * double(n));
         hipError_t e = hipMalloc(&dst.p, n);
         if (e != hipSuccess) {
             (void)hipGetLastError();
