@@ -161,10 +161,12 @@ struct mq_ctx {
     DevBuf lt_ws;
     // hipBLASLt algorithms timed by mq_plan_autotune, keyed by (precision, M, N, K)
     std::map<std::array<uint64_t, 4>, hipblasLtMatmulAlgo_t> lt_tuned;
-    // workspaces of destroyed plans, reused by the next plans: a sweep creates one plan per site,
-    // and allocating and releasing tens of GB per site costs more than the site's products
-    std::vector<std::unique_ptr<DevBuf>> pool;
-    static constexpr size_t POOL_MAX = 8;
+    // workspaces of destroyed plans, reused by the next plans, one slot per kind: a sweep creates
+    // one plan per site, and allocating and releasing tens of GB per site costs more than the
+    // site's products.  A slot keeps the largest buffer returned to it; a request it cannot serve
+    // allocates 25% headroom, so that the slightly larger plans of the next sites still fit.
+    enum PoolKind { POOL_WS, POOL_PART, POOL_WS32, POOL_BF, POOL_KINDS };
+    DevBuf pool[POOL_KINDS];
     // pinned host staging of the dense MPO matrices W_D of the plan (or step chunk) being lowered;
     // grows, and is reused by the next lowering once the previous upload has been synchronized
     double* wd_host = nullptr;
@@ -184,39 +186,35 @@ struct mq_ctx {
             std::memset(wd_host + o, 0, size_t(std::min<int64_t>(chunk, int64_t(n) - o)) * sizeof(double));
         return wd_host;
     }
-    void pool_put(DevBuf& b) {
+    void pool_put(PoolKind kind, DevBuf& b) {
         if (!b.p) return;
-        auto keep = std::make_unique<DevBuf>();
-        keep->swap(b);
-        pool.push_back(std::move(keep));
-        if (pool.size() > POOL_MAX) pool.erase(pool.begin());   // oldest first
+        if (pool[kind].bytes < b.bytes) pool[kind].swap(b);
+        b.release();
     }
-    // a workspace of at least n bytes cleared to zero: the smallest pooled buffer that fits (and is
-    // not more than 4x larger), else a new allocation; when that fails the pool is released and
-    // the allocation retried
-    void take(DevBuf& dst, size_t n) {
+    void pool_clear() {
+        for (auto& b : pool) b.release();
+    }
+    // a workspace of at least n bytes whose first n bytes are cleared to zero: the slot's buffer
+    // if it is large enough, else a new allocation (with headroom; exactly n bytes after the
+    // pools are released when that fails)
+    void take(PoolKind kind, DevBuf& dst, size_t n) {
         n = (n + 7) / 8 * 8;
-        size_t best = pool.size();
-        for (size_t i = 0; i < pool.size(); ++i)
-            if (pool[i]->bytes >= n && pool[i]->bytes <= 4 * n && (best == pool.size() || pool[i]->bytes < pool[best]->bytes))
-                best = i;
-        if (best < pool.size()) {
-            dst.release();
-            dst.swap(*pool[best]);
-            pool.erase(pool.begin() + best);
-            HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), n / 8, 0.0));   // the part in use
-            return;
+        dst.release();
+        if (pool[kind].p && pool[kind].bytes >= n) {
+            dst.swap(pool[kind]);
+        } else {
+            pool[kind].release();
+            const size_t want = (n + n / 4 + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+            if (trace_enabled()) std::fprintf(stderr, "[mq] workspace pool miss: %.2f GB allocated\n", 1e-9 * double(want));
+            if (hipMalloc(&dst.p, want) == hipSuccess) {
+                dst.bytes = want;
+            } else {
+                (void)hipGetLastError();
+                dst.p = nullptr;
+                pool_clear();
+                dst.alloc(n);
+            }
         }
-        if (trace_enabled()) std::fprintf(stderr, "[mq] workspace pool miss: %.2f GB allocated\n", 1e-9 * double(n));
-        hipError_t e = hipMalloc(&dst.p, n);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            dst.p = nullptr;
-            pool.clear();
-            dst.alloc_zeroed(n, stream);
-            return;
-        }
-        dst.bytes = n;
         HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), n / 8, 0.0));
     }
     DevBuf red_partial, red_result;
@@ -933,7 +931,7 @@ struct mq_plan {
             MQ_REQUIRE(grouped, "reduced-precision sigma needs a GROUPED plan");
             if (!ws32.p) {
                 // the FP32 workspace; W_D converted on the device from the FP64 copy
-                ctx->take(ws32, std::max<uint64_t>(st.ws_elems, 64) * sizeof(float));
+                ctx->take(mq_ctx::POOL_WS32, ws32, std::max<uint64_t>(st.ws_elems, 64) * sizeof(float));
                 if (ws_init_elems)
                     HIPCHK(launch_to_float(ctx->stream, ws.as<double>() + ws_init_off, ws32.as<float>() + ws_init_off,
                                            ws_init_elems));
@@ -942,7 +940,7 @@ struct mq_plan {
             if (mode == MODE_BF16X3 && st.bf_elems && !bf.p) {
                 // bf16 operands of the one-GEMM BF16x3 form: A' regions (written per application) and
                 // B' = [W_lo; W_hi; W_hi] per group (constant, split here on the device from W_D)
-                ctx->take(bf, std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t));
+                ctx->take(mq_ctx::POOL_BF, bf, std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t));
                 st.bf = bf.as<uint16_t>();
                 for (const auto& g : st.lib)
                     HIPCHK(launch_split_w_bf16(ctx->stream, ws.as<double>() + g.b_off, g.ldb, g.K, g.N,
@@ -1107,7 +1105,7 @@ static std::vector<mq_ctx*> g_contexts;
 static void release_all_pools() {
     for (mq_ctx* c : g_contexts) {
         (void)hipStreamSynchronize(c->stream);
-        c->pool.clear();
+        c->pool_clear();
     }
 }
 
@@ -1138,7 +1136,7 @@ int mq_ctx_destroy(mq_ctx* ctx) {
         for (auto sd : ctx->side) (void)hipStreamDestroy(sd);
         for (auto e : ctx->side_ev) (void)hipEventDestroy(e);
         if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-        ctx->pool.clear();
+        ctx->pool_clear();
         if (ctx->wd_host) (void)hipHostFree(ctx->wd_host);
         g_contexts.erase(std::remove(g_contexts.begin(), g_contexts.end(), ctx), g_contexts.end());
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
@@ -2455,8 +2453,8 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         P.sizes[BASE_RIGHT] = right->total;
         P.sizes[BASE_BRA] = P.part_elems;
         P.validate();
-        if (P.part_elems) ctx->take(p->part, P.part_elems * sizeof(double));
-        ctx->take(p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
+        if (P.part_elems) ctx->take(mq_ctx::POOL_PART, p->part, P.part_elems * sizeof(double));
+        ctx->take(mq_ctx::POOL_WS, p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
         if (wd_elems)
             HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ctx->wd_host, wd_elems * sizeof(double),
                                   hipMemcpyHostToDevice, ctx->stream));
@@ -2848,8 +2846,13 @@ int mq_plan_destroy(mq_plan* p) {
     return guarded([&] {
         if (p && p->ctx) {
             (void)hipStreamSynchronize(p->ctx->stream);
-            if (p->grouped)   // the large workspaces go back to the context for the next plans
-                for (DevBuf* b : {&p->ws, &p->ws32, &p->bf, &p->part}) p->ctx->pool_put(*b);
+            if (p->grouped) {   // the large workspaces go back to the context for the next plans
+                mq_ctx* c = p->ctx;
+                c->pool_put(mq_ctx::POOL_WS, p->ws);
+                c->pool_put(mq_ctx::POOL_WS32, p->ws32);
+                c->pool_put(mq_ctx::POOL_BF, p->bf);
+                c->pool_put(mq_ctx::POOL_PART, p->part);
+            }
             // batch graphs that contain the plan refer to its buffers
             auto& B = p->ctx->batches;
             for (auto it = B.begin(); it != B.end();) {
