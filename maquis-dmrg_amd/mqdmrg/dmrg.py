@@ -19,6 +19,24 @@ from . import models
 from .symmetry import Index, SiteLayout, fuse, sub, product_index
 
 
+def slab_copy(dst, src, slabs):
+    """dst[d0 + i dsi + j dsj] = src[s0 + i ssi + j ssj] for i < ni, j < nj, for every slab
+    (d0, dsi, dsj, s0, ssi, ssj, ni, nj) of `slabs` (flat tensors on one device; the slabs' targets
+    are disjoint).  The element indices are expanded on the device and the copy is one gather and
+    one scatter, instead of one small copy per block."""
+    S = np.asarray(slabs, np.int64).reshape(-1, 8)
+    total = int((S[:, 6] * S[:, 7]).sum()) if len(S) else 0
+    if total == 0:
+        return
+    T = torch.from_numpy(S).to(dst.device)
+    sz = T[:, 6] * T[:, 7]
+    sid = torch.repeat_interleave(torch.arange(len(S), device=dst.device), sz, output_size=total)
+    loc = torch.arange(total, device=dst.device) - (torch.cumsum(sz, 0) - sz)[sid]
+    P = T[sid]
+    i, j = loc % P[:, 6], loc // P[:, 6]
+    dst[P[:, 0] + i * P[:, 1] + j * P[:, 2]] = src[P[:, 3] + i * P[:, 4] + j * P[:, 5]]
+
+
 class SiteTensor:
     def __init__(self, eng, sym, left, phys, right, data=None):
         self.eng, self.sym = eng, sym
@@ -415,16 +433,22 @@ class TwoSiteDMRG:
         W12, wdesc, phys12, pb = self.twosite_mpo(i)
         theta = SiteTensor(self.eng, self.sym, A1.left, phys12, A2.right)
         lp1 = A1.lp_data()
-        prod = {}
+        prods, base, ncols, off = [], {}, {}, 0
         for mb, mc in enumerate(A1.right.charges):
             lb2 = A2.left.position(mc)
             if lb2 == len(A2.left):
                 continue
-            prod[mc] = A1.lp_block(lp1, mb) @ A2.rp_block(lb2)
-        for (mc, lb, ls, r, rb, rs, c, col12) in self._pairs(A1, A2, phys12, pb):
-            if mc not in prod:
-                continue
-            theta.rp_block(lb)[:, col12:col12 + rs] = prod[mc][r:r + ls, c:c + rs]
+            P = (A1.lp_block(lp1, mb) @ A2.rp_block(lb2)).contiguous()
+            prods.append(P.reshape(-1))
+            base[mc], ncols[mc] = off, P.shape[1]
+            off += P.numel()
+        if not prods:
+            return theta
+        flat = torch.cat(prods)
+        # theta's right-paired block lb holds (row i, column col) at rp_offset[lb] + col * ls + i
+        slabs = [(theta.layout.rp_offset[lb] + col12 * ls, 1, ls, base[mc] + r * ncols[mc] + c, ncols[mc], 1, ls, rs)
+                 for (mc, lb, ls, r, rb, rs, c, col12) in self._pairs(A1, A2, phys12, pb) if mc in base]
+        slab_copy(theta.data, flat, slabs)
         return theta
 
     def split(self, i, theta, direction):
@@ -447,8 +471,14 @@ class TwoSiteDMRG:
         # layouts of the unsplit factors with a full mid index (only row/col offsets are used)
         L1 = SiteLayout(A1.left, self.phys, Index(mids, [lidx[c] for c in mids]))
         L2 = SiteLayout(Index(mids, [ridx[c] for c in mids]), self.phys, A2.right)
-        mats = {c: torch.zeros(lidx[c], ridx[c], dtype=torch.float64, device=self.eng.torch_device) for c in mids}
+        mbase, off = {}, 0
+        for c in mids:
+            mbase[c] = off
+            off += lidx[c] * ridx[c]
+        mflat = torch.zeros(max(off, 1), dtype=torch.float64, device=self.eng.torch_device)
+        mats = {c: mflat[mbase[c]:mbase[c] + lidx[c] * ridx[c]].view(lidx[c], ridx[c]) for c in mids}
         lay12 = SiteLayout(A1.left, phys12, A2.right)
+        slabs = []
         for mc in mids:
             for p1, d1 in zip(self.phys.charges, self.phys.sizes):
                 lc = sub(mc, p1)
@@ -465,11 +495,14 @@ class TwoSiteDMRG:
                     rs = A2.right.sizes[rb]
                     c0 = L2.rpb(p2, rc)
                     col12 = lay12.rpb(fuse(p1, p2), rc)
+                    o = theta.layout.rp_offset[lb]
                     for ss1 in range(d1):
                         for ss2 in range(d2):
                             sub_i = pb(p1, p2) + ss1 * d2 + ss2
-                            mats[mc][r0 + ss1 * ls:r0 + (ss1 + 1) * ls, c0 + ss2 * rs:c0 + (ss2 + 1) * rs] = \
-                                theta.rp_block(lb)[:, col12 + sub_i * rs:col12 + (sub_i + 1) * rs]
+                            # mats[mc][r0 + ss1 ls + i, c0 + ss2 rs + j] = theta block lb (i, col12 + sub_i rs + j)
+                            slabs.append((mbase[mc] + (r0 + ss1 * ls) * ridx[mc] + c0 + ss2 * rs, ridx[mc], 1,
+                                          o + (col12 + sub_i * rs) * ls, 1, ls, ls, rs))
+        slab_copy(mflat, theta.data, slabs)
         fac, keeps, svals, tw, method = split_factors(mats, mids, direction, self.eng.torch_device, self.m,
                                                       self.cutoff)
         self.split_methods[method] = self.split_methods.get(method, 0) + 1

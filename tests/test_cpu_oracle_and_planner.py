@@ -328,6 +328,26 @@ def test_split_factors_truncate_like_lapack_svd(direction, Mmax, cutoff, method)
         assert np.abs(got - ref).max() <= 1e-12 * smax, (c, kc)
 
 
+def test_slab_copy_matches_blockwise_copies():
+    """slab_copy (the merge / split data movement) against one slice copy per block, for row-major
+    and column-major slabs, empty slabs and a slab list with no elements."""
+    rng = np.random.default_rng(5)
+    src = torch.from_numpy(rng.standard_normal(4000))
+    dst, ref = torch.zeros(3000, dtype=torch.float64), torch.zeros(3000, dtype=torch.float64)
+    slabs = [(0, 1, 7, 100, 9, 1, 7, 5),      # column-major target, row-major source
+             (40, 11, 1, 0, 1, 13, 4, 11),    # row-major target, column-major source
+             (500, 1, 3, 2000, 1, 3, 3, 0),   # empty
+             (900, 1, 20, 3000, 20, 1, 20, 30)]
+    for d0, dsi, dsj, s0, ssi, ssj, ni, nj in slabs:
+        for i in range(ni):
+            for j in range(nj):
+                ref[d0 + i * dsi + j * dsj] = src[s0 + i * ssi + j * ssj]
+    DM.slab_copy(dst, src, slabs)
+    assert torch.equal(dst, ref)
+    DM.slab_copy(dst, src, [])
+    assert torch.equal(dst, ref)
+
+
 def test_gram_buckets_cover_every_size_once():
     sizes = [3, 700, 64, 65, 300, 290, 31, 1, 1200, 256]
     groups = DM._gram_buckets(sizes)
