@@ -554,6 +554,28 @@ class TwoSiteDMRG:
             s = self.sites[k]
             self.Rb[k] = E.right_step(self.eng, s.desc, self.mpo_desc[k], s.data, s.data, self.Rb[k + 1])
 
+    def multi_expval(self, mpo=None):
+        """<psi|W|psi> per column index of the last MPO tensor for an MPO W on the chain (default:
+        the Hamiltonian), by left boundary propagation over every site on the device
+        (multi_expval, mps_mpo_ops.h:80-100): L_{k+1} = overlap_mpo_left_step(A_k, A_k, L_k, W_k),
+        then the traces of the last boundary."""
+        mpo = self.mpo if mpo is None else mpo
+        descs = self.mpo_desc if mpo is self.mpo else [E.MpoDesc(W) for W in mpo]
+        Lb = E.DeviceBoundary.trivial(self.eng, self.sites[0].left, mpo[0].row_dim)
+        for k in range(self.L):
+            s = self.sites[k]
+            Lb = E.left_step(self.eng, s.desc, descs[k], s.lp_data(), s.data, Lb)
+        return Lb.traces()
+
+    def expval(self, mpo=None):
+        """<psi|W|psi> (unnormalised, as expval, mps_mpo_ops.h:52-77): the trace of the last left
+        boundary of multi_expval."""
+        return float(np.sum(self.multi_expval(mpo)))
+
+    def norm2(self):
+        """<psi|psi>: expval of the identity MPO."""
+        return self.expval(models.identity_mpo(self.phys, self.L))
+
     def optimize_pair(self, i, direction):
         t0 = time.perf_counter()
         W12, wdesc, phys12, pb = self.twosite_mpo(i)

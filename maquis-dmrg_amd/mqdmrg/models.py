@@ -363,6 +363,17 @@ def boson_hubbard_mpo(L, nmax=2, t=1.0, U=8.0, V=0.0):
     return site, mpo
 
 
+def identity_mpo(phys, L):
+    """The identity MPO (bond dimension 1) on L sites of the physical basis phys."""
+    ident = Op.from_dense(np.eye(phys.total()), phys, name="id")
+    out = []
+    for _ in range(L):
+        W = MPOTensor(1, 1)
+        W.add(0, 0, ident)
+        out.append(W)
+    return out
+
+
 def chain(model, L, **kw):
     """(site basis, MPO, total charge) of the benchmark chains."""
     if model == "heisenberg":

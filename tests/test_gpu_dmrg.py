@@ -147,3 +147,20 @@ def test_gram_split_matches_svd_split_in_a_truncating_sweep(eng, monkeypatch):
     monkeypatch.setattr(DM, "GRAM_RESOLVED", float("inf"))
     e_svd = TwoSiteDMRG(eng, "qc", 12, **kw).run(2)
     assert abs(e - e_svd) <= 1e-9, (e, e_svd)
+
+
+def test_expval_by_left_boundary_propagation(eng):
+    """expval / multi_expval / norm2 over the whole chain on the device: after converged DMRG of
+    the Hubbard chain (L=6, exact at m=128) <psi|psi> = 1, <psi|H|psi> equals the sweep energy and
+    the exact ground-state energy, and <psi|2 I|psi> = 2 (an MPO other than the Hamiltonian)."""
+    L = 6
+    d = TwoSiteDMRG(eng, "hubbard", L, m=128, cutoff=1e-12, init_m=8, seed=4, jd_tol=1e-10, U=4.0)
+    e = d.run(3)
+    assert abs(d.norm2() - 1.0) < 1e-12
+    h = d.expval()
+    assert abs(h - e) < 1e-10, (h, e)
+    assert abs(h - ed_energy("hubbard", L, (3, 3), U=4.0)) < 1e-8
+    two = models.identity_mpo(d.phys, L)
+    two[0].terms[(0, 0)] = [(op, 2.0) for op, _ in two[0].terms[(0, 0)]]
+    assert abs(d.expval(two) - 2.0) < 1e-12
+    assert d.multi_expval(two).shape == (1,)
