@@ -287,8 +287,16 @@ int mq_sigma_adjoint(mq_plan* plan, const double* psi, double* grad);
  * kept.  Decisions are cached in the context, keyed by stage, precision and the stage's flop
  * histogram over the classes (log2 buckets): a plan whose histogram was tuned before applies the
  * cached choice without timing.
+ * In FP32 and BF16x3 the S-stage library GEMMs (hipBLASLt) are tuned too: up to 8 library
+ * candidates per GEMM are timed and the fastest kept, cached by exact shape and, as a library
+ * solution index, by shape class (half-octave buckets of M, N, K).  Any later GEMM of that class,
+ * in any plan of the context, adopts the class's solution when the library reports that it
+ * supports the shape; such GEMMs are not timed again.
+ * precision | MQ_AUTOTUNE_LIBRARY_ONLY tunes only the library GEMMs not already covered by the
+ * caches (the cheap per-site form for sweeps: after the first sites nearly every GEMM is covered).
  * timed (may be NULL) receives the number of stage timings made.  Captured graphs of the plan are
  * dropped.  No reference counterpart. */
+enum { MQ_AUTOTUNE_LIBRARY_ONLY = 0x10 };
 int mq_plan_autotune(mq_plan* plan, int precision, int* timed);
 /* n independent applications (plans of one context, e.g. the sites of independent chains) issued
  * as one batch: each instance on one of four side streams forked from and joined back to the

@@ -9,6 +9,7 @@
 // All operand addresses are checked against their buffers on the host before any launch.
 #include <hip/hip_runtime.h>
 #include <hipblaslt/hipblaslt.h>
+#include <hipblaslt/hipblaslt-ext.hpp>
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -138,6 +139,7 @@ struct BatchGraph {
 // after mq_plan_autotune).
 struct LtGemm {
     bool ready = false, ok = false;
+    bool chosen = false;   // algorithm from the exact-shape or shape-class cache (no retiming)
     hipblasLtMatmulDesc_t op = nullptr;
     hipblasLtMatrixLayout_t a = nullptr, b = nullptr, c = nullptr;
     hipblasLtMatmulAlgo_t algo{};
@@ -159,8 +161,16 @@ struct mq_ctx {
     rocblas_handle blas = nullptr;   // plain dense GEMMs of the grouped S stage, on `stream`
     hipblasLtHandle_t lt = nullptr;  // reduced-precision S stage GEMMs
     DevBuf lt_ws;
-    // hipBLASLt algorithms timed by mq_plan_autotune, keyed by (precision, M, N, K)
+    // hipBLASLt algorithms timed by mq_plan_autotune, keyed by (precision, M, N, K), and the
+    // library solution index of each winner keyed by its shape class (precision and half-octave
+    // buckets of M, N, K): a later GEMM of another shape in the same class adopts that solution
+    // when the library confirms it supports the shape
     std::map<std::array<uint64_t, 4>, hipblasLtMatmulAlgo_t> lt_tuned;
+    std::map<std::array<int, 4>, int> lt_class;
+    static std::array<int, 4> lt_class_key(int mi, uint64_t M, uint64_t N, uint64_t K) {
+        auto h = [](uint64_t x) { return int(std::floor(2.0 * std::log2(double(std::max<uint64_t>(x, 1))))); };
+        return {mi, h(M), h(N), h(K)};
+    }
     // workspaces of destroyed plans, reused by the next plans, one slot per kind: a sweep creates
     // one plan per site, and allocating and releasing tens of GB per site costs more than the
     // site's products.  A slot keeps the largest buffer returned to it; a request it cannot serve
@@ -473,7 +483,9 @@ struct StagePlan {
         auto it = lt_ctx->lt_tuned.find(key);
         if (it != lt_ctx->lt_tuned.end()) {
             L.algo = it->second;
-            L.ok = true;
+            L.ok = L.chosen = true;
+        } else if (lt_adopt_class(mi, g.M, g.N, K, L)) {
+            L.ok = L.chosen = true;
         } else {
             hipblasLtMatmulPreference_t pref;
             LTCHK(hipblasLtMatmulPreferenceCreate(&pref));
@@ -489,6 +501,22 @@ struct StagePlan {
         L.ready = true;
         return L.ok;
     }
+    // the solution tuned for this GEMM's shape class, if the library supports it for this shape
+    bool lt_adopt_class(int mi, uint64_t M, uint64_t N, uint64_t K, LtGemm& L) const {
+        auto c = lt_ctx->lt_class.find(mq_ctx::lt_class_key(mi, M, N, K));
+        if (c == lt_ctx->lt_class.end()) return false;
+        std::vector<int> idx{c->second};
+        std::vector<hipblasLtMatmulHeuristicResult_t> res;
+        if (hipblaslt_ext::getAlgosFromIndex(lt_ctx->lt, idx, res) != HIPBLAS_STATUS_SUCCESS || res.empty()) return false;
+        const float one = 1.f, zero = 0.f;
+        size_t need = 0;
+        if (hipblaslt_ext::matmulIsAlgoSupported(lt_ctx->lt, L.op, &one, L.a, L.b, &zero, L.c, L.c, res[0].algo, need) !=
+                HIPBLAS_STATUS_SUCCESS ||
+            need > LT_WORKSPACE)
+            return false;
+        L.algo = res[0].algo;
+        return true;
+    }
     // operands of lib GEMM i in mode mi (0: FP32 workspace, 1: bf16 split buffer)
     void lt_operands(int mi, size_t i, Bases b, const void*& A, const void*& B, float*& C) const {
         float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
@@ -498,10 +526,12 @@ struct StagePlan {
         C = ws + g.c_off;
     }
     // time up to 8 of the library's candidate algorithms for lib GEMM i and keep the fastest
-    // (cached in the context by shape); returns the number of timings
+    // (cached in the context by shape and by shape class); returns the number of timings.  A GEMM
+    // whose algorithm came from either cache is not timed again.
     int lt_tune(int mi, size_t i, hipStream_t st, Bases b, hipEvent_t e0, hipEvent_t e1) const {
         if (!lt_prepare(mi, i, st)) return 0;
         LtGemm& L = ltg[mi][i];
+        if (L.chosen) return 0;
         const LibGemm& g = lib[i];
         const uint64_t K = mi == 0 ? g.K : 3ull * g.K;
         std::array<uint64_t, 4> key{uint64_t(mi), g.M, g.N, K};
@@ -540,7 +570,12 @@ struct StagePlan {
             ++timed;
             if (timed == 1 || ms < best) best = ms, L.algo = res[c].algo;
         }
-        if (timed) lt_ctx->lt_tuned[key] = L.algo;
+        if (timed) {
+            lt_ctx->lt_tuned[key] = L.algo;
+            const int sol = hipblaslt_ext::getIndexFromAlgo(L.algo);
+            if (sol >= 0) lt_ctx->lt_class[mq_ctx::lt_class_key(mi, g.M, g.N, K)] = sol;
+            L.chosen = true;
+        }
         return timed;
     }
     void lt_run(int mi, size_t i, hipStream_t st, const void* A, const void* B, float* C) const {
@@ -2624,11 +2659,27 @@ int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const doub
 int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
     return guarded([&] {
         MQ_REQUIRE(p, "bad argument");
+        const bool library_only = (precision & MQ_AUTOTUNE_LIBRARY_ONLY) != 0;
+        precision &= ~int(MQ_AUTOTUNE_LIBRARY_ONLY);
         MQ_REQUIRE(precision == MQ_PREC_F64 || precision == MQ_PREC_F32 || precision == MQ_PREC_BF16X3,
                    "unknown precision");
         mq_ctx* ctx = p->ctx;
         hipStream_t st = ctx->stream;
         StagePlan& P = p->st;
+        if (library_only && (precision == MQ_PREC_F64 || !p->grouped)) {   // nothing to tune
+            if (timed) *timed = 0;
+            return;
+        }
+        if (library_only) {   // only GEMMs whose algorithm is not cached need operands and timing
+            bool any = false;
+            for (size_t i = 0; i < P.lib.size() && !any; ++i)
+                any = P.lt_prepare(precision == MQ_PREC_F32 ? 0 : 1, i, st) &&
+                      !P.ltg[precision == MQ_PREC_F32 ? 0 : 1][i].chosen;
+            if (!any) {
+                if (timed) *timed = 0;
+                return;
+            }
+        }
         if (!p->io.p) p->io.alloc(2 * std::max<uint64_t>(p->vec_size, 1) * sizeof(double));
         double* in = p->io.as<double>();
         double* out = in + std::max<uint64_t>(p->vec_size, 1);
@@ -2653,6 +2704,7 @@ int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
         };
         struct StageRef { int k; GemmStage* g; const std::vector<GemmSeg>* segs; };
         for (StageRef sr : {StageRef{1, &P.gT, &P.t_segs}, StageRef{3, &P.gF, &P.f_segs}}) {
+            if (library_only) break;
             GemmStage& G = *sr.g;
             std::vector<uint64_t> cf = G.class_flops(*sr.segs);
             std::vector<int> key = {sr.k, precision, int(p->grouped)};

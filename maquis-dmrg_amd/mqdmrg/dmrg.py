@@ -283,12 +283,18 @@ def split_factors(mats, keys, direction, device, Mmax, cutoff):
 class TwoSiteDMRG:
     def __init__(self, eng, model, L, m, cutoff=1e-10, init_m=16, seed=1, jd_tol=1e-8, jd_maxiter=8,
                  mpo=None, basis=None, target=None, verbose=False, hook=None, precision=E.PREC_F64, max_outer=1,
-                 init_bonds="sectors", **model_kw):
+                 init_bonds="sectors", tune_library=False, **model_kw):
         """precision != PREC_F64: sigma in that precision inside Jacobi-Davidson, with the FP64
         residual refinement of SigmaPlan.jd_refined (at most max_outer FP64 steps per site).  With
         max_outer = 1 a site costs jd_maxiter reduced-precision products and one FP64 product (the
         Rayleigh quotient and residual), and the sweep follows the FP64 sweep's trajectory; more
         outer steps restart the iteration and suit solves to a tight tolerance.
+
+        tune_library: in FP32 / BF16x3, autotune each site's S-stage library GEMMs that the
+        engine's shape and shape-class caches do not cover yet (SigmaPlan.autotune with
+        library_only); the timings are counted in tune_timings and their time in "plan".  Off by
+        default: at L=30, m=2048 the shapes of successive sites rarely share a class, and the
+        first sweep spent 24.6 s timing candidates to save 0.7 s of products.
 
         init_bonds: "sectors" caps every charge sector of the initial MPS at init_m (the reference's
         mps initialisation, mps_sectors.h); "total" draws bond indices of total dimension ~init_m
@@ -298,6 +304,7 @@ class TwoSiteDMRG:
         self.precision, self.max_outer = precision, max_outer
         self.sigma_calls = {"reduced": 0, "fp64": 0}
         self.split_methods = {}
+        self.tune_library, self.tune_timings = tune_library, 0
         self.hook = hook
         if mpo is None:
             basis, mpo, target = models.chain(model, L, **model_kw)
@@ -583,6 +590,8 @@ class TwoSiteDMRG:
         self.timings["merge"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         plan = E.SigmaPlan(self.eng, theta.desc, wdesc, self.Lb[i], self.Rb[i + 2])
+        if self.precision != E.PREC_F64 and self.tune_library:
+            self.tune_timings += plan.autotune(self.precision, library_only=True)
         self.timings["plan"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         if self.hook is not None:

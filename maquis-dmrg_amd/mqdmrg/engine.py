@@ -14,6 +14,10 @@ from .symmetry import Index
 
 
 class Engine:
+    # open engines in creation order (weak references): torch's current stream is the newest open
+    # engine's
+    _open = []
+
     def __init__(self, device=0):
         self.device = device
         self.ctx = C.c_void_p()
@@ -22,6 +26,7 @@ class Engine:
         self.torch_device = torch.device("cuda:%d" % device)
         # every torch op of the caller is ordered with the engine's launches on one stream
         torch.cuda.set_stream(self.stream)
+        Engine._open.append(weakref.ref(self))
         # boundaries and plans made on this context: freed before the context itself
         self._children = weakref.WeakSet()
 
@@ -38,6 +43,12 @@ class Engine:
                 obj.close()
             lib().mq_ctx_destroy(self.ctx)
             self.ctx = C.c_void_p()
+            # torch must not keep issuing work to the destroyed stream: hand the current stream
+            # back to the newest engine still open on the device, else to the default stream
+            Engine._open = [r for r in Engine._open if r() is not None and r() is not self]
+            if torch.cuda.current_stream(self.torch_device) == self.stream:
+                rest = [r() for r in Engine._open if r() is not None and r().ctx and r().device == self.device]
+                torch.cuda.set_stream(rest[-1].stream if rest else torch.cuda.default_stream(self.torch_device))
 
     def __del__(self):
         try:
@@ -266,6 +277,7 @@ def _schedule_result(kind, h):
 # ---------------------------------------------------------------------------- sigma
 SIGMA_AUTO, SIGMA_SCHEDULE, SIGMA_GROUPED = 0, 1, 2
 SIGMA_ADJOINT = 0x100   # OR-ed into the algorithm: the plan also supports SigmaPlan.adjoint
+AUTOTUNE_LIBRARY_ONLY = 0x10   # OR-ed into the precision of mq_plan_autotune
 # arithmetic of the sigma contraction (mq_precision)
 PREC_F64, PREC_F32, PREC_BF16X3 = 0, 1, 2
 PRECISIONS = {"f64": PREC_F64, "f32": PREC_F32, "bf16x3": PREC_BF16X3}
@@ -333,12 +345,15 @@ class SigmaPlan:
         check(lib().mq_sigma_adjoint(self.h, C.c_void_p(psi.data_ptr()), g.ctypes.data_as(C.c_void_p)))
         return g[:self.n_terms]
 
-    def autotune(self, precision=PREC_F64):
+    def autotune(self, precision=PREC_F64, library_only=False):
         """Time the tile configurations of the T and final stages and keep the fastest per row-size
-        class (cached per engine by the stage's block-size histogram).  Returns the number of stage
-        timings made (0 when the decision came from the cache)."""
+        class (cached per engine by the stage's block-size histogram); in FP32 / BF16x3 also the
+        S-stage library GEMMs (cached by shape and by shape class, whose solution later GEMMs of the
+        class adopt).  library_only: only the library GEMMs that no cache covers.  Returns the
+        number of timings made (0 when every decision came from a cache)."""
         n = C.c_int()
-        check(lib().mq_plan_autotune(self.h, int(precision), C.byref(n)))
+        flag = AUTOTUNE_LIBRARY_ONLY if library_only else 0
+        check(lib().mq_plan_autotune(self.h, int(precision) | flag, C.byref(n)))
         return n.value
 
     def set_graphs(self, enable):

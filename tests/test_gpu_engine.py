@@ -150,6 +150,40 @@ def test_autotuned_plan_matches_oracle(eng):
             plan.close()
 
 
+@pytest.mark.parametrize("prec", [E.PREC_F32, E.PREC_BF16X3])
+def test_library_solutions_adopted_by_shape_class(prec):
+    """A library GEMM solution tuned on one plan is adopted, through the shape-class cache, by the
+    GEMMs of a plan of nearby shapes (the same chain at a slightly larger bond dimension) when the
+    library supports them: that plan's library-only autotune times fewer candidates than the first
+    plan's, and its sigma still matches the oracle to the precision's tolerance."""
+    from mqdmrg.workloads import SyntheticSite
+    e = E.Engine(0)
+    try:
+        counts = []
+        for m, seed in ((128, 3), (136, 4)):
+            ws = SyntheticSite("qc", 12, 5, m, seed=seed)
+            Lc, Rc = ws.random_cohorts(ws.left_index), ws.random_cohorts(ws.right_index)
+            hl = O.HostBoundary(ws.left_index, [c if c is not None else np.zeros(0) for c in Lc])
+            hr = O.HostBoundary(ws.right_index, [c if c is not None else np.zeros(0) for c in Rc])
+            psi_h = ws.random_psi()
+            ref = O.SigmaOracle(ws.site_desc, ws.mpo_desc, hl, hr).apply(psi_h)
+            Lb = E.DeviceBoundary.upload(e, ws.left_index, Lc)
+            Rb = E.DeviceBoundary.upload(e, ws.right_index, Rc)
+            plan = E.SigmaPlan(e, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+            print("case m", m, "precision", prec, flush=True)
+            counts.append(plan.autotune(prec, library_only=True))
+            out = torch.full_like(dev(psi_h), np.nan)
+            plan.apply(dev(psi_h), out, precision=prec)
+            e.sync()
+            err = np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref)
+            assert err <= MIXED_RTOL[prec], (m, err)
+            assert plan.autotune(prec, library_only=True) == 0   # every GEMM now cached
+            plan.close()
+        assert counts[0] > 0 and counts[1] < counts[0], counts
+    finally:
+        e.close()
+
+
 def test_sigma_plan_from_caller_schedule(eng):
     """Drop-in seam: the caller hands its own ScheduleNew (here the oracle's) to the engine."""
     p = SiteProblem("hubbard", L=6, site=2, Mmax=6, seed=7)
