@@ -348,6 +348,19 @@ def test_slab_copy_matches_blockwise_copies():
     assert torch.equal(dst, ref)
 
 
+def test_identity_mpo_is_the_identity():
+    """models.identity_mpo: bond dimension 1 and the identity operator on every site (the MPO of
+    TwoSiteDMRG.norm2)."""
+    from mqdmrg import models
+    basis, _, _ = models.chain("hubbard", 3, U=4.0)
+    mpo = models.identity_mpo(basis.phys, 3)
+    assert len(mpo) == 3
+    for W in mpo:
+        D = W.dense(basis.phys)
+        assert D.shape[:2] == (1, 1)
+        assert np.array_equal(D[0, 0], np.eye(basis.phys.total()))
+
+
 def test_gram_buckets_cover_every_size_once():
     sizes = [3, 700, 64, 65, 300, 290, 31, 1, 1200, 256]
     groups = DM._gram_buckets(sizes)
