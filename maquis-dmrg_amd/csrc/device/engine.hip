@@ -161,6 +161,14 @@ struct mq_ctx {
     rocblas_handle blas = nullptr;   // plain dense GEMMs of the grouped S stage, on `stream`
     hipblasLtHandle_t lt = nullptr;  // reduced-precision S stage GEMMs
     DevBuf lt_ws;
+    // one hipBLASLt workspace per side stream: the instances of a batch run their library GEMMs
+    // concurrently, and a shared workspace would be a race for algorithms that use it
+    std::vector<std::unique_ptr<DevBuf>> lt_ws_side;
+    void* lt_workspace(hipStream_t s) const {
+        for (size_t k = 0; k < side.size() && k < lt_ws_side.size(); ++k)
+            if (side[k] == s) return lt_ws_side[k]->p;
+        return lt_ws.p;
+    }
     // hipBLASLt algorithms timed by mq_plan_autotune, keyed by (precision, M, N, K), and the
     // library solution index of each winner keyed by its shape class (precision and half-octave
     // buckets of M, N, K): a later GEMM of another shape in the same class adopts that solution
@@ -558,7 +566,7 @@ struct StagePlan {
         for (int c = 0; c < n; ++c) {
             auto run = [&]() {
                 return hipblasLtMatmul(lt_ctx->lt, L.op, &one, A, L.a, B, L.b, &zero, C, L.c, C, L.c, &res[c].algo,
-                                       lt_ctx->lt_ws.p, LT_WORKSPACE, st);
+                                       lt_ctx->lt_workspace(st), LT_WORKSPACE, st);
             };
             if (run() != HIPBLAS_STATUS_SUCCESS) continue;   // warm-up; a failing candidate is skipped
             HIPCHK(hipEventRecord(e0, st));
@@ -582,7 +590,7 @@ struct StagePlan {
         const LtGemm& L = ltg[mi][i];
         const float one = 1.f, zero = 0.f;
         LTCHK(hipblasLtMatmul(lt_ctx->lt, L.op, &one, A, L.a, B, L.b, &zero, C, L.c, C, L.c, &L.algo,
-                              lt_ctx->lt_ws.p, LT_WORKSPACE, st));
+                              lt_ctx->lt_workspace(st), LT_WORKSPACE, st));
     }
     uint64_t t_end = 0;     // T_D occupies workspace elements [0, t_end)
     uint64_t bf_elems = 0;  // size of the bf16 buffer
@@ -2592,6 +2600,8 @@ int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const doub
             for (int k = 0; k < NSIDE; ++k) {
                 HIPCHK(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
                 HIPCHK(hipEventCreateWithFlags(&ctx->side_ev[k], hipEventDisableTiming));
+                ctx->lt_ws_side.push_back(std::make_unique<DevBuf>());
+                ctx->lt_ws_side.back()->alloc(LT_WORKSPACE);
             }
             HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
         }
