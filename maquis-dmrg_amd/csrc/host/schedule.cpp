@@ -876,6 +876,49 @@ std::vector<uint32_t> cohort_table(const BoundaryIndex& B, const Index& bra, con
     return t;
 }
 
+// The flows inverted: for every physical sector pair (so, si) and MPO index b of the old boundary,
+// the new MPO indices whose flows start there (CSR over (so * np + si) * aux + b).
+struct FlowEdges {
+    uint32_t aux = 0;
+    std::vector<uint32_t> off, to;
+};
+FlowEdges invert_flows(const std::vector<std::vector<Flow>>& flows, uint32_t np, uint32_t aux) {
+    FlowEdges E;
+    E.aux = aux;
+    E.off.assign(size_t(np) * np * aux + 1, 0);
+    for (const auto& fl : flows)
+        for (const Flow& f : fl) ++E.off[(size_t(f.s_out) * np + f.s_in) * aux + f.b + 1];
+    for (size_t i = 1; i < E.off.size(); ++i) E.off[i] += E.off[i - 1];
+    E.to.resize(E.off.back());
+    std::vector<uint32_t> fill(E.off.begin(), E.off.end() - 1);
+    for (uint32_t nb = 0; nb < flows.size(); ++nb)
+        for (const Flow& f : flows[nb]) E.to[fill[(size_t(f.s_out) * np + f.s_in) * aux + f.b]++] = nb;
+    return E;
+}
+
+// the stored blocks (MPO indices) of every cohort of B, ascending
+std::vector<std::vector<uint32_t>> present_blocks(const BoundaryIndex& B) {
+    std::vector<std::vector<uint32_t>> out(B.n_cohorts());
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < int(B.n_cohorts()); ++c)
+        for (uint32_t b = 0; b < B.aux_dim; ++b)
+            if (B.offset(uint32_t(c), b) > -1) out[c].push_back(b);
+    return out;
+}
+
+// lines of one (ket, bra) sector pair of the new boundary: ln[nb] = 1 for every new MPO index nb
+// that a flow (b, so, si) reaches from a stored block b of the old cohort of (so, si); coh[so * np
+// + si] is that cohort (>= n_cohorts when absent)
+void mark_lines(uint8_t* ln, const uint32_t* coh, uint32_t n_cohorts, uint32_t np, const FlowEdges& E,
+                const std::vector<std::vector<uint32_t>>& present) {
+    for (uint32_t q = 0; q < np * np; ++q) {
+        if (coh[q] >= n_cohorts) continue;
+        const uint32_t* eo = &E.off[size_t(q) * E.aux];
+        for (uint32_t b : present[coh[q]])
+            for (uint32_t k = eo[b]; k < eo[b + 1]; ++k) ln[E.to[k]] = 1;
+    }
+}
+
 }  // namespace
 
 BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const BoundaryIndex& left, bool symmetric) {
@@ -888,6 +931,8 @@ BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const Boun
     auto flows = mpo_flows(mpo, phys_i, true);
     auto down = shift_table(right_i, phys_i, -1, left_i);   // rc - phys -> left sector
     auto coh = cohort_table(left, left_i, left_i);
+    const FlowEdges edges = invert_flows(flows, np, left.aux_dim);
+    const auto present = present_blocks(left);
     trace_phase("left index: flows");
     // lines[(rb_ket * nr + rb_bra) * w + b2]
     std::vector<uint8_t> lines(size_t(nr) * nr * w, 0);
@@ -898,26 +943,19 @@ BoundaryIndex left_step_index(const SiteLayout& site, const Mpo& mpo, const Boun
             Charge rc_bra = right_i.charges[rbb];
             if (std::abs(sy.particle_number(rc_bra) - sy.particle_number(rc_ket)) > 2) continue;
             uint8_t* ln = &lines[(size_t(rbk) * nr + rbb) * w];
-            // the left cohort's offsets row of every flow sector pair (so, si), or null
-            const int64_t* crow[MAX_PHYS_PAIRS];
+            // the left cohort of every flow sector pair (so, si)
+            uint32_t cq[MAX_PHYS_PAIRS];
             for (uint32_t so = 0; so < np; ++so)
                 for (uint32_t si = 0; si < np; ++si) {
                     uint32_t lbb = down[size_t(rbb) * np + so], lbk = down[size_t(rbk) * np + si];
-                    uint32_t c = (lbb == left_i.size() || lbk == left_i.size()) ? left.n_cohorts()
-                                                                                 : coh[size_t(lbb) * left_i.size() + lbk];
-                    crow[so * np + si] = c < left.n_cohorts() ? &left.offsets[size_t(c) * left.aux_dim] : nullptr;
+                    cq[so * np + si] = (lbb == left_i.size() || lbk == left_i.size()) ? left.n_cohorts()
+                                                                                       : coh[size_t(lbb) * left_i.size() + lbk];
                 }
-            for (uint32_t b2 = 0; b2 < w; ++b2) {
-                if (symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) continue;
-                if (sy.su2() && !triangle(sy.spin(rc_ket), mpo.right_spin[b2], sy.spin(rc_bra))) continue;
-                for (const Flow& f : flows[b2]) {
-                    const int64_t* row = crow[f.s_out * np + f.s_in];
-                    if (row && row[f.b] > -1) {
-                        ln[b2] = 1;
-                        break;
-                    }
-                }
-            }
+            mark_lines(ln, cq, left.n_cohorts(), np, edges, present);
+            for (uint32_t b2 = 0; b2 < w; ++b2)
+                if ((symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) ||
+                    (sy.su2() && !triangle(sy.spin(rc_ket), mpo.right_spin[b2], sy.spin(rc_bra))))
+                    ln[b2] = 0;
         }
     }
     trace_phase("left index: lines");
@@ -947,6 +985,8 @@ BoundaryIndex right_step_index(const SiteLayout& site, const Mpo& mpo, const Bou
     auto flows = mpo_flows(mpo, phys_i, false);
     auto up = shift_table(left_i, phys_i, +1, right_i);   // lc + phys -> right sector
     auto coh = cohort_table(right, right_i, right_i);     // (ket sector, bra sector) of the right boundary
+    const FlowEdges edges = invert_flows(flows, np, right.aux_dim);
+    const auto present = present_blocks(right);
     std::vector<uint8_t> lines(size_t(nl) * nl * w, 0);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int lbk = 0; lbk < int(nl); ++lbk) {
@@ -955,26 +995,19 @@ BoundaryIndex right_step_index(const SiteLayout& site, const Mpo& mpo, const Bou
             Charge lc_bra = left_i.charges[lbb];
             if (std::abs(sy.particle_number(lc_ket) - sy.particle_number(lc_bra)) > 2) continue;
             uint8_t* ln = &lines[(size_t(lbk) * nl + lbb) * w];
-            // the right cohort's offsets row of every flow sector pair (so, si), or null
-            const int64_t* crow[MAX_PHYS_PAIRS];
+            // the right cohort of every flow sector pair (so, si)
+            uint32_t cq[MAX_PHYS_PAIRS];
             for (uint32_t so = 0; so < np; ++so)
                 for (uint32_t si = 0; si < np; ++si) {
                     uint32_t rbb = up[size_t(lbb) * np + so], rbk = up[size_t(lbk) * np + si];
-                    uint32_t c = (rbb == right_i.size() || rbk == right_i.size()) ? right.n_cohorts()
-                                                                                   : coh[size_t(rbk) * right_i.size() + rbb];
-                    crow[so * np + si] = c < right.n_cohorts() ? &right.offsets[size_t(c) * right.aux_dim] : nullptr;
+                    cq[so * np + si] = (rbb == right_i.size() || rbk == right_i.size()) ? right.n_cohorts()
+                                                                                         : coh[size_t(rbk) * right_i.size() + rbb];
                 }
-            for (uint32_t b1 = 0; b1 < w; ++b1) {
-                if (symmetric && mpo.left_herm.skip(b1, lc_ket, lc_bra)) continue;
-                if (sy.su2() && !triangle(sy.spin(lc_ket), mpo.left_spin[b1], sy.spin(lc_bra))) continue;
-                for (const Flow& f : flows[b1]) {
-                    const int64_t* row = crow[f.s_out * np + f.s_in];
-                    if (row && row[f.b] > -1) {
-                        ln[b1] = 1;
-                        break;
-                    }
-                }
-            }
+            mark_lines(ln, cq, right.n_cohorts(), np, edges, present);
+            for (uint32_t b1 = 0; b1 < w; ++b1)
+                if ((symmetric && mpo.left_herm.skip(b1, lc_ket, lc_bra)) ||
+                    (sy.su2() && !triangle(sy.spin(lc_ket), mpo.left_spin[b1], sy.spin(lc_bra))))
+                    ln[b1] = 0;
         }
     }
     BoundaryIndex out(left_i, left_i, w);
