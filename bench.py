@@ -130,7 +130,8 @@ def headline_sweep(eng, args, report, prec_code, chain):
     out = {"model": "ab initio (synthetic integrals), TwoU1, L=30, N=30, m=2048, one sweep from a random "
                     "state of total bond dimension 2048, JD tol 1e-8, 8 iterations per site",
            "note": "timing workload: energies after one unconverged sweep depend on the trajectory and are "
-                   "not compared; |dE| is checked on the converged sweeps entry"}
+                   "not compared; |dE| is checked on the converged sweeps entry. This is the first sweep "
+                   "from the random start; the second sweep at this shape costs more (every bond at m)"}
     basis, mpo, target = chain
     for name, pc in (("f64", E.PREC_F64), (report, prec_code)):
         t0 = time.perf_counter()
