@@ -931,8 +931,8 @@ struct mq_plan {
     mutable uint64_t flops = 0;
     mutable std::function<uint64_t()> s_flops;   // grouped plans: the S-stage flops, evaluated on demand
     std::vector<uint64_t> out_offsets;
-    // grouped plans: the dense MPO matrices W_D (host copy) at ws_init_off of the workspace, and the
-    // FP32 workspace of the reduced-precision modes (allocated on first use)
+    // grouped plans: the dense MPO matrices W_D (ws_init_elems of them) at ws_init_off of the
+    // workspace, and the FP32 workspace of the reduced-precision modes (taken on first use)
     bool grouped = false;
     uint64_t ws_init_elems = 0;
     uint64_t ws_init_off = 0;
@@ -1764,8 +1764,8 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         }
     }
     trace_phase("lower: pairs and keys");
-    // ---- number each group's keys in ascending (b, s, ss) order: W_D then follows the MPO's block
-    // structure, and its all-zero tiles are skipped in the S GEMM
+    // ---- number each group's keys in ascending (b, s, ss) order, so that the rows and columns of
+    // W_D follow the MPO's block structure
     for (auto& g : G.groups) {
         auto number = [&](std::vector<int32_t>& pos, std::vector<uint32_t>& keys) {
             keys.clear();
