@@ -227,120 +227,159 @@ def qc_mpo(h, eri, ecore=0.0, tol=1e-14, sources=False):
         nz = np.argwhere(np.abs(table.mats[oid]) > 0)
         return int(any(parities[i] != parities[j] for i, j in nz)) if len(nz) else 0
     op_par = [op_parity(k) for k in range(len(table.mats))]
+    auto = operator_automaton(site_terms, L, op_par, lambda par, spin2: idF if par % 2 else idI,
+                              sources=key_src if sources else None)
+    ops = [Op.from_dense(M, phys, name="qc%d" % k) for k, M in enumerate(table.mats)]
+    mpo = auto.tensors(ops, keep_zero=sources)
+    basis = models.SiteBasis(SYM_TWOU1, phys, {})
+    if sources:
+        return basis, mpo, [len(b) for b in auto.bonds], auto.switch_src
+    return basis, mpo, [len(b) for b in auto.bonds]
 
-    # ---- automaton
-    bonds = [dict() for _ in range(L + 1)]      # state key -> index, per bond
+
+class operator_automaton:
+    """Bond states and transitions of an MPO built from operator strings (site_terms: {((site, op),
+    ...) ascending sites: coefficient}).  A term's bond state is its left operator string ("P")
+    while that is the shorter side and its right operator string ("S") afterwards; the
+    coefficient is applied once, at the switch; strings are shared between terms, which gives a
+    bond dimension O(L^2) for ab initio Hamiltonians.  Between operators a string passes a site
+    with pass_op(parity of the operators on its left, bond spin).  action[op] is the change of
+    the doubled bond spin across op (0 for abelian groups); the bond spin of a P state is the sum
+    of its actions, of an S state minus the sum.  sources: optional {key: list} accumulated per
+    switch coefficient (switch_src)."""
+
     START, DONE = ("start",), ("done",)
 
-    def state(k, key):
-        d = bonds[k]
+    def __init__(self, site_terms, L, op_par, pass_op, action=None, sources=None, tol=1e-14):
+        self.L = L
+        self.action = action
+        self.bonds = [dict() for _ in range(L + 1)]
+        START, DONE = self.START, self.DONE
+        for k in range(L + 1):
+            if k < L:
+                self._state(k, START)
+            if k > 0:
+                self._state(k, DONE)
+        self.switch = [dict() for _ in range(L)]     # (a, b, op) -> summed coefficient
+        self.fixed = [dict() for _ in range(L)]      # (a, b) -> op, transitions between shared states
+        self.switch_src = [dict() for _ in range(L)]
+        p_exit, s_entry = {}, {}                     # P key -> last bond alive; S key -> first bond alive
+        mid = L // 2
+
+        def key_parity(key):
+            return sum(op_par[o] for _, o in key) % 2
+
+        def key_spin(key):
+            return sum(action[o] for _, o in key) if action is not None else 0
+
+        def pass_of(state):
+            par = key_parity(state[1])
+            spin = key_spin(state[1]) * (1 if state[0] == "P" else -1)
+            return pass_op(par, spin)
+
+        cur = {"src": []}
+
+        def trans(j, prev, nxt, oid, coef):
+            a, b = self._state(j, prev), self._state(j + 1, nxt)
+            if prev[0] in ("start", "P") and nxt[0] in ("S", "done"):
+                k3 = (a, b, oid)
+                self.switch[j][k3] = self.switch[j].get(k3, 0.0) + coef
+                if sources is not None:
+                    self.switch_src[j].setdefault(k3, []).extend(cur["src"])
+            else:
+                self.fixed[j][(a, b)] = oid
+
+        for key, coef in site_terms.items():
+            if abs(coef) < tol and sources is None:
+                continue
+            cur["src"] = sources.get(key, []) if sources is not None else []
+            n = len(key)
+            prev = START
+            for t in range(n):
+                s, oid = key[t]
+                tl, tr = t + 1, n - t - 1
+                if t == n - 1:
+                    nxt = DONE
+                elif tl < tr or (tl == tr and s + 1 <= mid):
+                    nxt = ("P", key[:tl])
+                else:
+                    nxt = ("S", key[tl:])
+                trans(s, prev, nxt, oid, coef)
+                prev = nxt
+                if t == n - 1:
+                    break
+                s_next = key[t + 1][0]
+                if nxt[0] == "P":
+                    if tl == tr and s_next > mid:
+                        # tie gap straddling the middle: switch at pass-through site mid
+                        p_exit[nxt] = max(p_exit.get(nxt, 0), mid)
+                        snew = ("S", key[tl:])
+                        trans(mid, nxt, snew, pass_of(nxt), coef)
+                        s_entry[snew] = min(s_entry.get(snew, L + 1), mid + 1)
+                        prev = snew
+                    else:
+                        p_exit[nxt] = max(p_exit.get(nxt, 0), s_next)
+                else:
+                    s_entry[nxt] = min(s_entry.get(nxt, L + 1), s + 1)
+        # pass-through of shared states over sites without operators
+        for key, ex in p_exit.items():
+            first = key[1][-1][0] + 1
+            oid = pass_of(key)
+            for k in range(first, ex):
+                self.fixed[k][(self._state(k, key), self._state(k + 1, key))] = oid
+        for key, en in s_entry.items():
+            last = key[1][0][0]
+            oid = pass_of(key)
+            for k in range(en, last):
+                self.fixed[k][(self._state(k, key), self._state(k + 1, key))] = oid
+        ident = pass_op(0, 0)
+        for j in range(L):
+            if j < L - 1:
+                self.fixed[j][(self.bonds[j][START], self.bonds[j + 1][START])] = ident
+            if j > 0:
+                self.fixed[j][(self.bonds[j][DONE], self.bonds[j + 1][DONE])] = ident
+        # doubled bond spin of every state
+        self.spins = []
+        for d in self.bonds:
+            sp_ = [0] * len(d)
+            for key, i in d.items():
+                if key[0] in ("P", "S"):
+                    sp_[i] = key_spin(key[1]) * (1 if key[0] == "P" else -1)
+            self.spins.append(sp_)
+
+    def _state(self, k, key):
+        d = self.bonds[k]
         i = d.get(key)
         if i is None:
             i = d[key] = len(d)
         return i
-    for k in range(L + 1):
-        if k < L:
-            state(k, START)
-        if k > 0:
-            state(k, DONE)
-    switch = [dict() for _ in range(L)]     # (a, b, op) -> summed coefficient
-    fixed = [dict() for _ in range(L)]      # (a, b) -> op, transitions between shared states
-    p_exit, s_entry = {}, {}                # P key -> last bond alive; S key -> first bond alive
-    mid = L // 2
 
-    def key_parity(key):
-        return sum(op_par[o] for _, o in key) % 2
-
-    switch_src = [dict() for _ in range(L)]   # (a, b, op) -> integral sources of its coefficient
-    cur_src = []
-
-    def trans(j, prev, nxt, oid, coef):
-        a, b = state(j, prev), state(j + 1, nxt)
-        if prev[0] in ("start", "P") and nxt[0] in ("S", "done"):
-            k3 = (a, b, oid)
-            switch[j][k3] = switch[j].get(k3, 0.0) + coef
-            if sources:
-                switch_src[j].setdefault(k3, []).extend(cur_src)
-        else:
-            fixed[j][(a, b)] = oid
-
-    for key, coef in site_terms.items():
-        if abs(coef) < tol and not sources:
-            continue
-        cur_src = key_src.get(key, [])
-        n = len(key)
-        prev = START
-        for t in range(n):
-            s, oid = key[t]
-            tl, tr = t + 1, n - t - 1
-            if t == n - 1:
-                nxt = DONE
-            elif tl < tr or (tl == tr and s + 1 <= mid):
-                nxt = ("P", key[:tl])
-            else:
-                nxt = ("S", key[tl:])
-            trans(s, prev, nxt, oid, coef)
-            prev = nxt
-            if t == n - 1:
-                break
-            s_next = key[t + 1][0]
-            if nxt[0] == "P":
-                if tl == tr and s_next > mid:
-                    # tie gap straddling the middle: switch at pass-through site mid
-                    p_exit[nxt] = max(p_exit.get(nxt, 0), mid)
-                    snew = ("S", key[tl:])
-                    trans(mid, nxt, snew, idF if key_parity(key[tl:]) else idI, coef)
-                    s_entry[snew] = min(s_entry.get(snew, L + 1), mid + 1)
-                    prev = snew
-                else:
-                    p_exit[nxt] = max(p_exit.get(nxt, 0), s_next)
-            else:
-                s_entry[nxt] = min(s_entry.get(nxt, L + 1), s + 1)
-    # pass-through of shared states over sites without operators
-    for key, ex in p_exit.items():
-        first = key[1][-1][0] + 1
-        oid = idF if key_parity(key[1]) else idI
-        for k in range(first, ex):
-            fixed[k][(state(k, key), state(k + 1, key))] = oid
-    for key, en in s_entry.items():
-        last = key[1][0][0]
-        oid = idF if key_parity(key[1]) else idI
-        for k in range(en, last):
-            fixed[k][(state(k, key), state(k + 1, key))] = oid
-    for j in range(L):
-        if j < L - 1:
-            fixed[j][(bonds[j][START], bonds[j + 1][START])] = idI
-        if j > 0:
-            fixed[j][(bonds[j][DONE], bonds[j + 1][DONE])] = idI
-
-    # ---- sparse MPO tensors: sum over distinct local operators of coefficient matrices
-    import scipy.sparse as sp
-    ops = [Op.from_dense(M, phys, name="qc%d" % k) for k, M in enumerate(table.mats)]
-    mpo = []
-    for j in range(L):
-        rows, cols = len(bonds[j]), len(bonds[j + 1])
-        per_op = {}
-        for (a, b), oid in fixed[j].items():
-            per_op.setdefault(oid, []).append((a, b, 1.0))
-        for (a, b, oid), v in switch[j].items():
-            per_op.setdefault(oid, []).append((a, b, v))
-        used, coeffs, op_ids = [], [], []
-        for oid, ents in sorted(per_op.items()):
-            r, c_, v = zip(*ents)
-            C = sp.coo_matrix((v, (r, c_)), shape=(rows, cols)).tocsr()
-            C.sum_duplicates()
-            used.append(ops[oid])
-            coeffs.append(C)
-            op_ids.append(oid)
-        T = models.SparseMPOTensor(rows, cols, used, coeffs)
-        if sources:
-            T.op_ids = op_ids
-            T.keep_zero = True
-        mpo.append(T)
-    basis = models.SiteBasis(SYM_TWOU1, phys, {})
-    if sources:
-        return basis, mpo, [len(b) for b in bonds], switch_src
-    return basis, mpo, [len(b) for b in bonds]
+    def tensors(self, ops, keep_zero=False):
+        """Sparse MPO tensors: one coefficient matrix per distinct local operator of each site."""
+        import scipy.sparse as sp
+        mpo = []
+        for j in range(self.L):
+            rows, cols = len(self.bonds[j]), len(self.bonds[j + 1])
+            per_op = {}
+            for (a, b), oid in self.fixed[j].items():
+                per_op.setdefault(oid, []).append((a, b, 1.0))
+            for (a, b, oid), v in self.switch[j].items():
+                per_op.setdefault(oid, []).append((a, b, v))
+            used, coeffs, op_ids = [], [], []
+            for oid, ents in sorted(per_op.items()):
+                r, c_, v = zip(*ents)
+                C = sp.coo_matrix((v, (r, c_)), shape=(rows, cols)).tocsr()
+                C.sum_duplicates()
+                used.append(ops[oid])
+                coeffs.append(C)
+                op_ids.append(oid)
+            T = models.SparseMPOTensor(rows, cols, used, coeffs)
+            T.left_spin, T.right_spin = list(self.spins[j]), list(self.spins[j + 1])
+            if keep_zero:
+                T.op_ids = op_ids
+                T.keep_zero = True
+            mpo.append(T)
+        return mpo
 
 
 def mpo_gradient_to_integrals(mpo, switch_src, term_grads, L):
