@@ -59,19 +59,39 @@ class Engine:
     # vectors ------------------------------------------------------------------
     def dot(self, x, y):
         r = C.c_double()
-        check(lib().mq_vec_dot(self.ctx, x.numel(), x.data_ptr(), y.data_ptr(), C.byref(r)))
+        n = x.numel()
+        check(lib().mq_vec_dot(self.ctx, n, dev_ptr(self, x, n), dev_ptr(self, y, n), C.byref(r)))
         return r.value
 
     def nrm2(self, x):
         r = C.c_double()
-        check(lib().mq_vec_nrm2(self.ctx, x.numel(), x.data_ptr(), C.byref(r)))
+        check(lib().mq_vec_nrm2(self.ctx, x.numel(), dev_ptr(self, x, x.numel()), C.byref(r)))
         return r.value
 
     def axpy(self, a, x, y):
-        check(lib().mq_vec_axpy(self.ctx, x.numel(), float(a), x.data_ptr(), y.data_ptr()))
+        n = x.numel()
+        check(lib().mq_vec_axpy(self.ctx, n, float(a), dev_ptr(self, x, n), dev_ptr(self, y, n)))
 
     def scal(self, a, x):
-        check(lib().mq_vec_scal(self.ctx, x.numel(), float(a), x.data_ptr()))
+        check(lib().mq_vec_scal(self.ctx, x.numel(), float(a), dev_ptr(self, x, x.numel())))
+
+
+def dev_ptr(engine, t, n):
+    """Device address of a tensor the engine reads or writes n FP64 elements of.  The C-ABI takes
+    raw pointers and cannot tell a host pointer, another device or a narrower dtype from a valid
+    buffer, so every wrapper checks here, before any C call: an FP64, contiguous tensor on the
+    engine's device with at least n elements."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor, got %s" % type(t).__name__)
+    if t.dtype != torch.float64:
+        raise ValueError("engine buffers are float64, got %s" % t.dtype)
+    if not t.is_cuda or t.device != engine.torch_device:
+        raise ValueError("tensor on %s, the engine runs on %s" % (t.device, engine.torch_device))
+    if not t.is_contiguous():
+        raise ValueError("engine buffers must be contiguous")
+    if t.numel() < n:
+        raise ValueError("buffer of %d elements, %d needed" % (t.numel(), n))
+    return t.data_ptr()
 
 
 # ---------------------------------------------------------------------------- descriptors
@@ -165,8 +185,11 @@ class MpoDesc:
 
 class SiteDesc:
     def __init__(self, sym, left, phys, right):
+        from .symmetry import SiteLayout
         self.keep = []
         self.desc = capi.site_desc(sym, left, phys, right, self.keep)
+        lay = SiteLayout(left, phys, right)
+        self.rp_total, self.lp_total = lay.rp_total, lay.lp_total
 
 
 # ---------------------------------------------------------------------------- boundaries
@@ -334,15 +357,14 @@ class SigmaPlan:
         return self
 
     def apply(self, psi, sigma, precision=PREC_F64):
-        assert psi.numel() >= self.n and sigma.numel() >= self.n
-        check(lib().mq_sigma_apply_ex(self.h, psi.data_ptr(), sigma.data_ptr(), int(precision)))
+        e = self.engine
+        check(lib().mq_sigma_apply_ex(self.h, dev_ptr(e, psi, self.n), dev_ptr(e, sigma, self.n), int(precision)))
 
     def adjoint(self, psi):
         """d(psi . H_eff psi) / d term_scale[t] for every term of the plan's MPO descriptor (numpy,
         FP64): for a normalized eigenvector, dE/d scale_t."""
-        assert psi.dtype == torch.float64 and psi.is_cuda and psi.numel() >= self.n
         g = np.zeros(max(1, self.n_terms), np.float64)
-        check(lib().mq_sigma_adjoint(self.h, C.c_void_p(psi.data_ptr()), g.ctypes.data_as(C.c_void_p)))
+        check(lib().mq_sigma_adjoint(self.h, C.c_void_p(dev_ptr(self.engine, psi, self.n)), g.ctypes.data_as(C.c_void_p)))
         return g[:self.n_terms]
 
     def autotune(self, precision=PREC_F64, library_only=False):
@@ -366,15 +388,17 @@ class SigmaPlan:
         ms = (C.c_double * 4)()
         fl = (C.c_uint64 * 3)()
         gb = C.c_uint64()
-        check(lib().mq_plan_profile(self.h, psi.data_ptr(), sigma.data_ptr(), int(reps), int(precision), ms, fl,
-                                    C.byref(gb)))
+        e = self.engine
+        check(lib().mq_plan_profile(self.h, dev_ptr(e, psi, self.n), dev_ptr(e, sigma, self.n), int(reps),
+                                    int(precision), ms, fl, C.byref(gb)))
         return list(ms), list(fl), gb.value
 
     def jd(self, init, out, tol=1e-8, max_iter=8):
         e = C.c_double()
         it = C.c_int()
-        check(lib().mq_jd_solve(self.h, init.data_ptr(), out.data_ptr(), float(tol), int(max_iter), 0, C.byref(e),
-                                C.byref(it)))
+        g = self.engine
+        check(lib().mq_jd_solve(self.h, dev_ptr(g, init, self.n), dev_ptr(g, out, self.n), float(tol), int(max_iter), 0,
+                                C.byref(e), C.byref(it)))
         return e.value, it.value
 
     def jd_refined(self, init, out, tol=1e-8, max_iter=8, precision=PREC_F32, max_outer=4):
@@ -383,8 +407,9 @@ class SigmaPlan:
         e = C.c_double()
         cnt = (C.c_int * 3)()
         r = C.c_double()
-        check(lib().mq_jd_solve_refined(self.h, init.data_ptr(), out.data_ptr(), float(tol), int(max_iter),
-                                        int(precision), int(max_outer), C.byref(e), cnt, C.byref(r)))
+        g = self.engine
+        check(lib().mq_jd_solve_refined(self.h, dev_ptr(g, init, self.n), dev_ptr(g, out, self.n), float(tol),
+                                        int(max_iter), int(precision), int(max_outer), C.byref(e), cnt, C.byref(r)))
         return e.value, {"reduced": cnt[0], "fp64": cnt[1], "outer": cnt[2]}, r.value
 
     def close(self):
@@ -404,11 +429,9 @@ def apply_batched(engine, plans, psis, sigmas, precision=PREC_F64):
     into one HIP graph from the second call with the same plans on)."""
     n = len(plans)
     assert n == len(psis) == len(sigmas) and n > 0
-    for p, x, y in zip(plans, psis, sigmas):
-        assert x.numel() >= p.n and y.numel() >= p.n
     hp = (C.c_void_p * n)(*[p.h for p in plans])
-    xp = (C.c_void_p * n)(*[x.data_ptr() for x in psis])
-    yp = (C.c_void_p * n)(*[y.data_ptr() for y in sigmas])
+    xp = (C.c_void_p * n)(*[dev_ptr(engine, x, p.n) for p, x in zip(plans, psis)])
+    yp = (C.c_void_p * n)(*[dev_ptr(engine, y, p.n) for p, y in zip(plans, sigmas)])
     check(lib().mq_sigma_apply_batched(engine.ctx, n, hp, xp, yp, int(precision)))
 
 
@@ -416,22 +439,26 @@ def left_step(engine, site, mpo_desc, bra_lp, ket_rp, left, symmetric=True, algo
     """overlap_mpo_left_step; algorithm SIGMA_AUTO (grouped for abelian groups), SIGMA_SCHEDULE or
     SIGMA_GROUPED."""
     h = C.c_void_p()
-    check(lib().mq_boundary_left_step_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_lp.data_ptr(),
-                                         ket_rp.data_ptr(), left.h, int(symmetric), int(algorithm), C.byref(h)))
+    check(lib().mq_boundary_left_step_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc),
+                                         dev_ptr(engine, bra_lp, site.lp_total), dev_ptr(engine, ket_rp, site.rp_total),
+                                         left.h, int(symmetric), int(algorithm), C.byref(h)))
     return DeviceBoundary(engine, h)
 
 
 def right_step(engine, site, mpo_desc, bra_rp, ket_rp, right, symmetric=True, algorithm=SIGMA_AUTO):
     """overlap_mpo_right_step; algorithm as for left_step."""
     h = C.c_void_p()
-    check(lib().mq_boundary_right_step_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc), bra_rp.data_ptr(),
-                                          ket_rp.data_ptr(), right.h, int(symmetric), int(algorithm), C.byref(h)))
+    check(lib().mq_boundary_right_step_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc),
+                                          dev_ptr(engine, bra_rp, site.rp_total), dev_ptr(engine, ket_rp, site.rp_total),
+                                          right.h, int(symmetric), int(algorithm), C.byref(h)))
     return DeviceBoundary(engine, h)
 
 
 def reshape_right_to_left(engine, site, src, dst):
-    check(lib().mq_reshape_right_to_left(engine.ctx, C.byref(site.desc), src.data_ptr(), dst.data_ptr()))
+    check(lib().mq_reshape_right_to_left(engine.ctx, C.byref(site.desc), dev_ptr(engine, src, site.rp_total),
+                                         dev_ptr(engine, dst, site.lp_total)))
 
 
 def reshape_left_to_right(engine, site, src, dst):
-    check(lib().mq_reshape_left_to_right(engine.ctx, C.byref(site.desc), src.data_ptr(), dst.data_ptr()))
+    check(lib().mq_reshape_left_to_right(engine.ctx, C.byref(site.desc), dev_ptr(engine, src, site.lp_total),
+                                         dev_ptr(engine, dst, site.rp_total)))

@@ -385,3 +385,33 @@ def test_error_codes_not_exit():
     L = capi.lib()
     assert L.mq_sigma_apply(None, None, None) == 1
     assert b"bad argument" in L.mq_last_error()
+
+
+def test_wrappers_reject_bad_buffers_before_any_c_call(monkeypatch):
+    """A float32 buffer would be read as n doubles (out of bounds on the device) and a host
+    tensor would hand a host pointer to the kernels: the wrappers reject both before calling into
+    the library."""
+    class FakeEngine:
+        torch_device = torch.device("cuda:0")
+        ctx = None
+
+    class NoCalls:
+        def __getattr__(self, name):
+            def call(*args):
+                raise AssertionError("C-ABI %s called with an unchecked buffer" % name)
+            return call
+    monkeypatch.setattr(E, "lib", lambda: NoCalls())
+    eng = FakeEngine()
+    plan = E.SigmaPlan.__new__(E.SigmaPlan)
+    plan.engine, plan.n, plan.h = eng, 8, None
+    good_shape = torch.zeros(8, dtype=torch.float64)
+    with pytest.raises(ValueError, match="float64"):
+        plan.apply(torch.zeros(8, dtype=torch.float32), good_shape)
+    with pytest.raises(ValueError, match="cuda:0"):
+        plan.apply(good_shape, good_shape)
+    with pytest.raises(TypeError):
+        E.dev_ptr(eng, np.zeros(8), 8)
+    with pytest.raises(ValueError):
+        E.Engine.axpy(eng, 1.0, good_shape, good_shape)
+    with pytest.raises(ValueError):
+        E.apply_batched(eng, [plan], [torch.zeros(4, dtype=torch.float64)], [good_shape])
