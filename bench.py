@@ -13,7 +13,7 @@ with HIP events on the engine's stream; the CPU baseline is the oracle (the CPU 
 the reference's CPU branch) on a bounded sample of the same sigma (a subset of its MPS blocks)
 on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload qc-L30-m2048]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload qc_su2-L30-m2048]
 """
 import argparse
 import json
@@ -30,7 +30,10 @@ PEAK_HBM_GBS = 8000.0
 MIXED_RTOL = 1e-4
 
 WORKLOADS = {
-    # name: (model, L, m)
+    # name: (model, L, m); qc_su2: spin-adapted (SU2U1) ab initio chain, N = L, singlet, m multiplets
+    "qc_su2-L30-m2048": ("qc_su2", 30, 2048),
+    "qc_su2-L24-m1000": ("qc_su2", 24, 1000),
+    "qc_su2-L16-m256": ("qc_su2", 16, 256),
     "qc-L30-m2048": ("qc", 30, 2048),
     "qc-L24-m1000": ("qc", 24, 1000),
     "qc-L16-m256": ("qc", 16, 256),
@@ -44,7 +47,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="qc-L30-m2048", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="qc_su2-L30-m2048", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="auto", choices=["auto", "f64", "f32", "bf16x3"],
                     help="precision reported as value (auto: fastest reduced precision within the sigma bar)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU-baseline sample")
@@ -57,8 +60,9 @@ def parse():
                     help="also time N independent ab initio instances (L uniform in [12, 24], m=256) "
                          "applied one by one and as one batch")
     ap.add_argument("--no-headline-sweep", action="store_true",
-                    help="skip the wall-time of one full two-site sweep at L=30, m=2048 in FP64 and in the "
+                    help="skip the wall-time of full two-site sweeps at the workload's shape in FP64 and in the "
                          "reported precision")
+    ap.add_argument("--headline-sweeps", type=int, default=2, help="sweeps of the headline sweep comparison")
     ap.add_argument("--adjoint", action="store_true",
                     help="also run the differentiation workload: two-site DMRG of the ab initio chain "
                          "(--adjoint-L orbitals, N = L, bond dimension --adjoint-m), the sigma-adjoint sweep "
@@ -78,100 +82,116 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ws, Lcoh, Rcoh, psi, budget_s, gpu_sigma):
-    """Time the oracle's CPU sigma (OpenMP over MPS blocks, all host cores) on a sample of the MPS
-    blocks of the same sigma, sized to about budget_s of CPU work; check it against the GPU's sigma
-    restricted to the same blocks."""
+def cpu_baseline(ws, Lcoh, Rcoh, psi, budget_s, threads):
+    """Time the oracle's CPU sigma (OpenMP over MPS blocks, `threads` host threads) on a sample of
+    the MPS blocks of the same sigma: `threads` blocks of about the median size (so every thread
+    works; smaller blocks if one application of those exceeds budget_s / 3), applied repeatedly for
+    about budget_s; the GPU's sigma restricted to the same blocks is the parity check."""
     import numpy as np
     from oracle import oracle as O
     blas = O.use_numpy_blas()
     Lb = O.HostBoundary(ws.left_index, [c if c is not None else np.zeros(0) for c in Lcoh])
     Rb = O.HostBoundary(ws.right_index, [c if c is not None else np.zeros(0) for c in Rcoh])
-    nblk = ws.site_desc.desc.left_i.n
-    # grow the sample (every stride-th MPS block) until one application takes about budget_s / 4
-    stride = max(1, nblk // 4)
+    lay = ws.layout
+    nblk = len(lay.left.sizes)
+    by_size = sorted(range(nblk), key=lambda b: lay.left.sizes[b] * lay.rp_cols[b])
+    count = min(threads, nblk)
+    start = max(0, min(nblk - count, nblk // 2 - count // 2))
     while True:
-        blocks = list(range(0, nblk, stride))
+        blocks = sorted(by_size[start:start + count])
         so = O.SigmaOracle(ws.site_desc, ws.mpo_desc, Lb, Rb, only_blocks=blocks)
         t0 = time.perf_counter()
         out = so.apply(psi)
         t1 = time.perf_counter() - t0
-        if t1 >= budget_s / 4 or stride == 1:
+        if t1 <= budget_s / 3 or start == 0:
             break
-        stride = max(1, stride // 4)
+        start = max(0, start - max(1, count // 2))
     n, t = 1, t1
     while t < budget_s and n < 20:
         t0 = time.perf_counter()
         so.apply(psi)
         t += time.perf_counter() - t0
         n += 1
-    # sigma of a block subset = the contributions of those ket blocks; the GPU sigma of psi with
-    # the other blocks zeroed is the same quantity
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     res = {"value": so.flops * n / t / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-           "sample": "%d applications of sigma restricted to %d of %d MPS blocks (every %d-th; %.3e of the "
-                     "reference-model flops per application), %.1f s; dgemm: %s"
-                     % (n, len(blocks), nblk, stride, so.flops, t,
+           "sample": "%d applications of sigma restricted to %d of %d MPS blocks (ranks %d-%d by size; %.3e of the "
+                     "reference-model flops per application), %.1f s, OpenMP over the %d blocks; dgemm: %s"
+                     % (n, len(blocks), nblk, start, start + len(blocks) - 1, so.flops, t, threads,
                         "numpy OpenBLAS, 1 thread per call" if blas else "oracle blocked loop")}
     return res, blocks, out
 
 
-def headline_sweep(eng, args, report, prec_code, chain):
-    """Wall time of one full two-site sweep (both directions) of the ab initio chain at L=30, N=30,
-    m=2048, from the same random state of total bond dimension m, in FP64 and in the reported
-    precision (refined Jacobi-Davidson, one FP64 product per site), with its breakdown.  chain is
-    the (basis, mpo, target) of models.chain("qc", 30) (synthetic integrals, seed 0, N = 30).
-    Both runs are capped at 8 Jacobi-Davidson iterations per site, so the energy after this first
-    sweep from a random state is far from converged and depends on the trajectory: it is reported,
-    not compared; the |dE| criterion applies to the converged comparison of the sweeps entry."""
+def headline_sweep(eng, args, report, prec_code, chain, model, L, m):
+    """Wall time of n_sweeps full two-site sweeps (both directions) of the ab initio chain (L
+    orbitals, N = L, singlet for SU2U1) at bond dimension m, from the same random state of total
+    bond dimension m, in FP64 and in the reported precision (refined Jacobi-Davidson: reduced-
+    precision products, one FP64 product per site), with the per-sweep energies of both runs and
+    their difference.  chain is the (basis, mpo, target) of the timed workload."""
     import torch
     from mqdmrg.dmrg import TwoSiteDMRG
     from mqdmrg import engine as E
-    out = {"model": "ab initio (synthetic integrals), TwoU1, L=30, N=30, m=2048, one sweep from a random "
-                    "state of total bond dimension 2048, JD tol 1e-8, 8 iterations per site",
-           "note": "timing workload: energies after one unconverged sweep depend on the trajectory and are "
-                   "not compared; |dE| is checked on the converged sweeps entry. This is the first sweep "
-                   "from the random start; the second sweep at this shape costs more (every bond at m)"}
+    sym = "SU2U1, singlet" if model == "qc_su2" else "TwoU1"
+    n = args.headline_sweeps
+    out = {"model": "ab initio (synthetic integrals), %s, L=%d, N=%d, m=%d, %d sweep(s) from a random state of "
+                    "total bond dimension %d, JD tol 1e-8, 8 iterations per site" % (sym, L, L, m, n, m)}
     basis, mpo, target = chain
     for name, pc in (("f64", E.PREC_F64), (report, prec_code)):
         t0 = time.perf_counter()
-        d = TwoSiteDMRG(eng, "qc", 30, m=2048, cutoff=1e-10, init_m=2048, seed=3, jd_tol=1e-8, jd_maxiter=8,
+        d = TwoSiteDMRG(eng, model, L, m=m, cutoff=1e-10, init_m=m, seed=3, jd_tol=1e-8, jd_maxiter=8,
                         mpo=mpo, basis=basis, target=target, precision=pc, init_bonds="total")
         d.init_boundaries()
         t_setup = time.perf_counter() - t0
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        e = d.sweep()
-        torch.cuda.synchronize()
-        t_sweep = time.perf_counter() - t0
-        out[name] = {"sweep_seconds": round(t_sweep, 2), "setup_seconds": round(t_setup, 2), "energy_Eh": e,
+        secs = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            d.sweep()
+            torch.cuda.synchronize()
+            secs.append(round(time.perf_counter() - t0, 2))
+            log("headline sweep %s %d: %.1f s E=%.10f" % (name, len(secs), secs[-1], d.energies[-1]))
+        out[name] = {"sweep_seconds": secs, "setup_seconds": round(t_setup, 2), "energies_Eh": list(d.energies),
                      "breakdown_seconds": {k: round(v, 2) for k, v in d.timings.items() if k != "sweep"},
                      "sigma_calls": d.sigma_calls}
-        log("headline sweep %s: %.1f s %s" % (name, t_sweep, out[name]["breakdown_seconds"]))
+        log("headline sweeps %s: %s s %s" % (name, secs, out[name]["breakdown_seconds"]))
         del d
         torch.cuda.empty_cache()
-    out["speedup_vs_f64"] = out["f64"]["sweep_seconds"] / out[report]["sweep_seconds"] if report != "f64" else 1.0
+    if report != "f64":
+        f, r = out["f64"], out[report]
+        out["speedup_vs_f64"] = sum(f["sweep_seconds"]) / sum(r["sweep_seconds"])
+        out["abs_delta_e_per_sweep_Eh"] = [abs(a - b) for a, b in zip(f["energies_Eh"], r["energies_Eh"])]
     return out
 
 
-def adjoint_workload(eng, args):
+def adjoint_workload(eng, args, su2_model):
     """dE/dh_pq and dE/d(pq|rs) of the DMRG ground state from the sigma adjoint (one-site adjoint at
     every site, Hellmann-Feynman), with size-independent identities and finite differences of the
-    re-optimised energy (warm start from the converged state, step 1e-5)."""
+    re-optimised energy (warm start from the converged state, step 1e-5).  su2_model: the
+    spin-adapted Hamiltonian (singlet) instead of the TwoU1 one (n_up = n_down)."""
     import numpy as np
     import torch
-    from mqdmrg import qc
+    from mqdmrg import qc, su2
     from mqdmrg.dmrg import TwoSiteDMRG, SiteTensor
     L, m = args.adjoint_L, args.adjoint_m
     N = L
     h, eri = qc.synthetic_integrals(L, seed=0)
     kw = dict(m=m, cutoff=1e-12, init_m=16, seed=3, jd_tol=1e-10, jd_maxiter=12)
+    model = "qc_su2" if su2_model else "qc"
+    target = (N, 0) if su2_model else (N // 2, N - N // 2)
 
     def make(hh, ee):
+        if su2_model:
+            basis, mpo, _, elements, src = su2.qc_su2_mpo(hh, ee, N, sources=True)
+            return basis, mpo, (elements, src)
         basis, mpo, _, src = qc.qc_mpo(hh, ee, sources=True)
         return basis, mpo, src
+
+    def to_tensors(mpo, src, grads):
+        if su2_model:
+            elements, esrc = src
+            g = su2.mpo_gradient_to_elements(mpo, esrc, grads, len(elements))
+            return su2.element_gradient_to_tensors(elements, g, L)
+        return qc.mpo_gradient_to_integrals(mpo, src, grads, L)
     basis, mpo, src = make(h, eri)
-    d = TwoSiteDMRG(eng, "qc", L, mpo=mpo, basis=basis, target=(N // 2, N - N // 2), **kw)
+    d = TwoSiteDMRG(eng, model, L, mpo=mpo, basis=basis, target=target, **kw)
     t0 = time.perf_counter()
     e = d.run(args.adjoint_sweeps)
     t_dmrg = time.perf_counter() - t0
@@ -182,13 +202,13 @@ def adjoint_workload(eng, args):
     grads, energies = d.mpo_gradients()
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
-    dh, deri, _ = qc.mpo_gradient_to_integrals(mpo, src, grads, L)
+    dh, deri, _ = to_tensors(mpo, src, grads)
     t_map = time.perf_counter() - t0 - t_adj
     euler = float(np.sum(h * dh) + np.sum(eri * deri))
     contr = np.einsum("pqrr->pq", deri)
     res = {
-        "model": "ab initio (synthetic integrals), TwoU1, L=%d, N=%d, m=%d, %d sweeps, FP64" % (L, N, m,
-                                                                                              args.adjoint_sweeps),
+        "model": "ab initio (synthetic integrals), %s, L=%d, N=%d, m=%d, %d sweeps, FP64"
+                 % ("SU2U1 singlet" if su2_model else "TwoU1", L, N, m, args.adjoint_sweeps),
         "energy_Eh": e, "dmrg_seconds": round(t_dmrg, 2),
         "adjoint_sweep_seconds": round(t_adj, 3), "integral_map_seconds": round(t_map, 3),
         "site_energy_spread_Eh": float(max(energies) - min(energies)),
@@ -203,17 +223,28 @@ def adjoint_workload(eng, args):
     # finite differences of the re-optimised energy, warm-started from the converged state
     fd = []
     picks = [("h", (0, 0)), ("h", (L // 2, L // 2 + 1)), ("eri", (1, 1, 2, 2)), ("eri", (0, 1, 1, 0))][:args.adjoint_fd]
+    def perm_class(kind, idx):
+        # the spin-adapted generator reads one element per permutation class: perturb the class
+        if not su2_model:
+            return [idx]
+        if kind == "h":
+            return sorted({idx, idx[::-1]})
+        p, q, r, s_ = idx
+        return sorted({(p, q, r, s_), (q, p, r, s_), (p, q, s_, r), (q, p, s_, r), (r, s_, p, q), (s_, r, p, q),
+                       (r, s_, q, p), (s_, r, q, p)})
     for kind, idx in picks:
+        cls = perm_class(kind, idx)
         vals = []
         for sgn in (+1, -1):
             hh, ee = h.copy(), eri.copy()
-            (hh if kind == "h" else ee)[idx] += sgn * 1e-5
+            for x in cls:
+                (hh if kind == "h" else ee)[x] += sgn * 1e-5
             b2, mpo2, _ = make(hh, ee)
-            d2 = TwoSiteDMRG(eng, "qc", L, mpo=mpo2, basis=b2, target=(N // 2, N - N // 2), **kw)
+            d2 = TwoSiteDMRG(eng, model, L, mpo=mpo2, basis=b2, target=target, **kw)
             d2.sites = [SiteTensor(eng, s.sym, s.left, s.phys, s.right, s.data.clone()) for s in state]
             vals.append(d2.run(2))
         num = (vals[0] - vals[1]) / 2e-5
-        ana = float((dh if kind == "h" else deri)[idx])
+        ana = float(sum((dh if kind == "h" else deri)[x] for x in cls))
         fd.append({"integral": kind, "index": list(idx), "adjoint": ana, "finite_difference": num,
                    "rel_err": abs(ana - num) / max(abs(num), 1e-300)})
         log("  d E / d %s%s: adjoint %.10f  FD %.10f" % (kind, idx, ana, num))
@@ -307,9 +338,10 @@ def main():
         sL, sm, nsw = args.sweep_L, args.sweep_m, args.sweep_n
         kw = dict(m=sm, cutoff=1e-10, init_m=16, seed=3, jd_tol=1e-8, jd_maxiter=8, integral_seed=0, ecore=0.0,
                   N=sL)
+        sweep_model = "qc_su2" if model == "qc_su2" else "qc"
         runs = {}
         for name, pc in (("f64", E.PREC_F64), (report, prec_code)):
-            d = TwoSiteDMRG(eng, "qc", sL, precision=pc, **kw)
+            d = TwoSiteDMRG(eng, sweep_model, sL, precision=pc, **kw)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             e = d.run(nsw)
@@ -319,8 +351,9 @@ def main():
                           "sigma_calls": d.sigma_calls}
             log("sweeps %s: E=%.10f  %s s  calls %s" % (name, e, [round(x, 2) for x in d.timings["sweep"]],
                                                        d.sigma_calls))
-        sweep = {"model": "ab initio (synthetic integrals), TwoU1, L=%d, N=%d, m=%d, %d sweeps, JD tol 1e-8, "
-                          "max_iter 8; reduced precision: one FP64 refinement step per site" % (sL, sL, sm, nsw),
+        sweep = {"model": "ab initio (synthetic integrals), %s, L=%d, N=%d, m=%d, %d sweeps, JD tol 1e-8, "
+                          "max_iter 8; reduced precision: one FP64 refinement step per site"
+                          % ("SU2U1 singlet" if sweep_model == "qc_su2" else "TwoU1", sL, sL, sm, nsw),
                  "runs": runs, "abs_delta_e_Eh": abs(runs[report]["energy_Eh"] - runs["f64"]["energy_Eh"]),
                  "speedup_vs_f64": runs["f64"]["seconds"] / runs[report]["seconds"]}
 
@@ -354,7 +387,9 @@ def main():
     # ---- CPU baseline on a bounded sample of the same sigma
     cpu = None
     if not args.no_cpu:
-        cpu, blocks, ref_cpu = cpu_baseline(ws, Lcoh, Rcoh, psi_h, args.cpu_seconds, None)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        cpu, blocks, ref_cpu = cpu_baseline(ws, Lcoh, Rcoh, psi_h, args.cpu_seconds, threads)
         # GPU sigma of psi restricted to the sampled ket blocks (others zeroed): same contributions
         masked = np.zeros_like(psi_h)
         lay = ws.layout
@@ -373,9 +408,10 @@ def main():
         rng = np.random.default_rng(args.seed)
         plans, psis, sigs, keep = [], [], [], []
         flops = 0
+        batch_model = "qc_su2" if model == "qc_su2" else "qc"
         for i in range(args.batch):
             bL = int(rng.integers(12, 25))
-            inst = SyntheticSite("qc", bL, bL // 2 - 1, 256, seed=100 + i)
+            inst = SyntheticSite(batch_model, bL, bL // 2 - 1, 256, seed=100 + i)
             lb = E.DeviceBoundary.upload(eng, inst.left_index, inst.random_cohorts(inst.left_index))
             rb = E.DeviceBoundary.upload(eng, inst.right_index, inst.random_cohorts(inst.right_index))
             pl = E.SigmaPlan(eng, inst.site_desc, inst.mpo_desc, lb, rb, algorithm=E.SIGMA_GROUPED)
@@ -407,14 +443,15 @@ def main():
                 res["%s_%s" % (name, mode)] = {"ms_per_batch_step": ms, "tflops": flops / (ms * 1e-3) / 1e12}
                 log("batch %s %s: %.3f ms per step of %d instances, %.2f TFLOP/s" % (name, mode, ms, args.batch,
                                                                                    res["%s_%s" % (name, mode)]["tflops"]))
-        batch = {"instances": args.batch, "model": "ab initio (synthetic integrals), L uniform in [12, 24], m=256, "
-                 "middle site", "flops_per_step": flops, "runs": res}
+        batch = {"instances": args.batch, "model": "ab initio (synthetic integrals, %s), L uniform in [12, 24], "
+                 "m=256, middle site" % ("SU2U1" if batch_model == "qc_su2" else "TwoU1"),
+                 "flops_per_step": flops, "runs": res}
 
-    adjoint = adjoint_workload(eng, args) if args.adjoint else None
+    adjoint = adjoint_workload(eng, args, model == "qc_su2") if args.adjoint else None
     hsweep = None
-    if not args.no_headline_sweep and args.workload == "qc-L30-m2048":
+    if not args.no_headline_sweep and model in ("qc", "qc_su2") and args.headline_sweeps > 0:
         plan.close()   # the sweep builds its own plans; release the timed plan's workspace first
-        hsweep = headline_sweep(eng, args, report, prec_code, ws.chain)
+        hsweep = headline_sweep(eng, args, report, prec_code, ws.chain, model, L, m)
 
     out = {
         "metric": "sweep wall-time & σ TFLOP/s at L=30 m=2048, mixed vs FP64 at |ΔE|≤1e-6 Eh",
@@ -430,9 +467,11 @@ def main():
         "dtype": {"f64": "f64", "f32": "f32", "bf16x3": "bf16x3 (fp32 accumulate)"}[report],
         "data": "synthetic",
         "config": {
-            "workload": "sigma (two-site H_eff psi) at the middle site of a %s chain, L=%d, bond dimension m=%d "
-                        "(%s symmetry), grouped MFMA path" % (model, L, m, "TwoU1" if model in ("qc", "hubbard")
-                                                                  else "U1"),
+            "workload": ("sigma (two-site H_eff psi) at the middle site of the ab initio chain, SU2U1, L=%d, N=%d, "
+                         "singlet, m=%d multiplets, grouped MFMA path" % (L, L, m)) if model == "qc_su2" else
+                        ("sigma (two-site H_eff psi) at the middle site of a %s chain, L=%d, bond dimension m=%d "
+                         "(%s symmetry), grouped MFMA path" % (model, L, m, "TwoU1" if model in ("qc", "hubbard")
+                                                               else "U1")),
             "model": model, "L": L, "m": m, "site": site,
             "bond_sectors": len(ws.bonds[site + 1].sizes),
             "mpo_bond_dim": int(ws.mpo[site].col_dim),

@@ -244,9 +244,9 @@ int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sche
                                        const uint32_t* ket_cols, const mq_boundary* left,
                                        const mq_boundary* right, mq_plan** out);
 /* Sigma algorithms.  SCHEDULE executes the reference's ScheduleNew tables (T GEMMs, alpha-list
- * gather, L GEMMs).  GROUPED (abelian groups) groups sector pairs by charge transfer and performs
- * the MPO-term sum as one dense MFMA GEMM per group; same result, no per-sector alpha lists.
- * AUTO picks GROUPED for abelian groups and SCHEDULE for SU2U1. */
+ * gather, L GEMMs).  GROUPED groups sector pairs by charge transfer (and, for SU2U1, by the spin
+ * of the pair's left sector, which fixes every 9j coupling of the group) and performs the MPO-term
+ * sum as one dense MFMA GEMM per group; same result, no per-sector alpha lists.  AUTO = GROUPED. */
 enum mq_sigma_algorithm { MQ_SIGMA_AUTO = 0, MQ_SIGMA_SCHEDULE = 1, MQ_SIGMA_GROUPED = 2 };
 /* OR-ed into the algorithm of mq_sigma_plan_create_ex: keep what mq_sigma_adjoint needs (the map
  * from MPO terms to W_D entries and the adjoint task tables).  GROUPED plans only. */
@@ -332,8 +332,9 @@ int mq_boundary_right_step(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_d
 /* As mq_boundary_left_step / mq_boundary_right_step with the algorithm chosen (mq_sigma_algorithm).
  * GROUPED (abelian groups): the new boundary's index comes from the index-only builder; the step
  * is T (psi R, or L psi), S_D = T_D W_D (one library GEMM per charge-transfer group) and the
- * contraction with the bra, as in the grouped sigma, without the per-sector alpha lists.  The
- * plain entry points use AUTO (GROUPED for U1 / TwoU1, SCHEDULE for SU2U1). */
+ * contraction with the bra, as in the grouped sigma, without the per-sector alpha lists; for SU2U1
+ * the pairs are grouped by charge transfer and left-sector spin and each W_D carries the 9j
+ * couplings.  The plain entry points use AUTO (= GROUPED). */
 int mq_boundary_left_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_desc* mpo,
                              const double* bra_lp, const double* ket_rp, const mq_boundary* left,
                              int symmetric, int algorithm, mq_boundary** out);
@@ -348,6 +349,12 @@ int mq_vec_nrm2(mq_ctx* ctx, uint64_t n, const double* x, double* result);
 int mq_vec_axpy(mq_ctx* ctx, uint64_t n, double a, const double* x, double* y);
 int mq_vec_scal(mq_ctx* ctx, uint64_t n, double a, double* x);
 int mq_vec_copy(mq_ctx* ctx, uint64_t n, const double* x, double* y);
+/* Batched dot products result[j] = V_j . x, j < k, with V_j = V + j * ldv (k device vectors in one
+ * buffer, as the Jacobi-Davidson basis is stored); result is host memory.  Replaces the k separate
+ * ietl::dot calls of one Gram-Schmidt pass or one subspace-matrix column (ietl/jacobi.h:408-427).
+ * The partial sums live in a context buffer that grows with k.  Fixed reduction order. */
+int mq_vec_multi_dot(mq_ctx* ctx, uint64_t n, int k, const double* V, uint64_t ldv, const double* x,
+                     double* result);
 
 /* Jacobi-Davidson smallest eigenpair of H_eff (ietl/jacobi.h:386-516, solver/ietl_jacobi_davidson.h:35-76).
  * init and out are device vectors in the plan's layout (may alias).  gmres_iter == 0 uses the

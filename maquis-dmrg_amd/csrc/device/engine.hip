@@ -236,11 +236,28 @@ struct mq_ctx {
         HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), n / 8, 0.0));
     }
     DevBuf red_partial, red_result;
+    // batched reductions (mq_vec_multi_dot, Jacobi-Davidson): k * RED_BLOCKS partial sums, k results
+    DevBuf mdot_partial, mdot_result;
+    int mdot_cap = 0;
+    void mdot_reserve(int k) {
+        if (k <= mdot_cap) return;
+        mdot_partial.alloc(size_t(k) * RED_BLOCKS * sizeof(double));
+        mdot_result.alloc(size_t(k) * sizeof(double));
+        mdot_cap = k;
+    }
     DevBuf step_ws;   // grows; reused by boundary steps
     // batched applications: side streams (default priority) the instances of a batch fork onto,
     // and the captured graphs keyed by (plans, precision)
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> side_ev;
+    // one rocBLAS handle per side stream, bound to it once: a handle owns one device workspace,
+    // which library GEMMs of concurrent instances would otherwise share
+    std::vector<rocblas_handle> side_blas;
+    rocblas_handle blas_for(hipStream_t s) const {
+        for (size_t k = 0; k < side.size() && k < side_blas.size(); ++k)
+            if (side[k] == s) return side_blas[k];
+        return blas;
+    }
     hipEvent_t fork_ev = nullptr;
     std::map<std::pair<std::vector<const mq_plan*>, int>, BatchGraph> batches;
     // autotuned tile policies, keyed by (stage, precision, grouped, log2 flop histogram over size classes)
@@ -869,6 +886,7 @@ struct StagePlan {
     void run_lib(hipStream_t st, Bases b, int mode) const {
         if (lib.empty()) return;
         const rocblas_operation nn = rocblas_operation_none;
+        const rocblas_handle blas = lt_ctx ? lt_ctx->blas_for(st) : this->blas;
         if (mode == MODE_F64) {
             const double one = 1.0, zero = 0.0;
             const double* ws = b.p[BASE_WS];
@@ -1176,6 +1194,7 @@ int mq_ctx_destroy(mq_ctx* ctx) {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->stream);
         ctx->drop_batches();
+        for (auto h : ctx->side_blas) (void)rocblas_destroy_handle(h);
         for (auto sd : ctx->side) (void)hipStreamDestroy(sd);
         for (auto e : ctx->side_ev) (void)hipEventDestroy(e);
         if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
@@ -1597,6 +1616,7 @@ struct GroupedLowering {
     };
     struct Group {
         Charge delta;
+        int spin = 0;   // SU2: spin of the pairs' left sector (one value per group)
         std::vector<uint32_t> K, N;          // the group's keys, ascending
         std::vector<int32_t> kpos, npos;     // key -> position in K / N, -1 if absent
         uint64_t M = 0, t_off = 0, s_off = 0, w_off = 0, bf_off = 0;
@@ -1660,7 +1680,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     const Index &left_i = site.left_i, &right_i = site.right_i, &phys_i = site.phys_i;
     const int kind = step ? step->kind : Schedule::SIGMA;
     const bool sigma = kind == Schedule::SIGMA, rstep = kind == Schedule::RIGHT, lstep = kind == Schedule::LEFT;
-    MQ_REQUIRE(!site.symm.su2(), "the grouped sigma path handles abelian symmetries only");
+    const Symm& sy = site.symm;
     MQ_REQUIRE(!(adjoint && !sigma), "the adjoint is a sigma-plan option");
     static const BoundaryIndex none;
     const BoundaryIndex& LI = p->left ? p->left->index : none;
@@ -1688,7 +1708,11 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         MQ_REQUIRE(nk < (1ull << 31), "MPO bond dimension x physical dimension too large");
         G.nkeys = uint32_t(nk);
     }
-    std::map<uint64_t, uint32_t> group_of;   // delta key -> group (ordered: deterministic numbering)
+    // Pairs are grouped by the charge transfer D = rc_out - lc_in and, for SU2, by the spin of lc_in:
+    // the spins of all four sectors a coupling of sigma or a step involves then follow from the
+    // group and the physical sectors of the (k, n) keys, so every SU2 9j factor is a constant of
+    // the group's W_D.  (delta key, spin) -> group, ordered: deterministic numbering.
+    std::map<std::pair<uint64_t, int>, uint32_t> group_of;
     // stored block j of a (non-transposed) cohort -> MPO index b
     auto block_order = [](const BoundaryIndex& B, uint32_t ci) {
         std::vector<uint32_t> jb(B.n_blocks[ci], MQ_NONE);
@@ -1739,10 +1763,11 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
             }
             if (K.empty() || N.empty()) continue;
             Charge d = rc_out - lc_in;
-            auto ins = group_of.emplace(d.key(), uint32_t(G.groups.size()));
+            auto ins = group_of.emplace(std::make_pair(d.key(), sy.spin(lc_in)), uint32_t(G.groups.size()));
             if (ins.second) {
                 G.groups.emplace_back();
                 G.groups.back().delta = d;
+                G.groups.back().spin = sy.spin(lc_in);
                 G.groups.back().kpos.assign(G.nkeys, -1);
                 G.groups.back().npos.assign(G.nkeys, -1);
             }
@@ -1840,7 +1865,33 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     // right step, and of every K key of the left step), so they are assembled in parallel; each
     // entry still sums its contributions in MPO order.  The adjoint's entry list keeps MPO order.
     std::atomic<int> bad{0};
+    // SU2 coupling of an MPO entry inside group g (site_hamil_tasks.hpp:93-125, left_tasks.hpp /
+    // right_tasks.hpp): spins J (ket left), J' (ket right), I (bra left), I' (bra right) of the
+    // sectors it connects.  sigma and the right step pair a ket left sector with a bra right
+    // sector; the left step pairs a bra left sector with a ket right sector.
+    auto su2_factor = [&](CouplingCache& cc, const GroupedLowering::Group& g, uint32_t s_in, uint32_t s_out,
+                          int A, int K, int Ap, uint32_t e, const Op& op) {
+        const int din = sy.spin(phys_i.charges[s_in]), dout = sy.spin(phys_i.charges[s_out]);
+        int J, Jp, I, Ip;
+        if (lstep) {
+            I = g.spin;
+            Jp = g.spin + sy.spin(g.delta);
+            J = Jp - din;
+            Ip = I + dout;
+        } else {
+            J = g.spin;
+            Ip = g.spin + sy.spin(g.delta);
+            Jp = J + din;
+            I = Ip - dout;
+        }
+        if (J < 0 || Jp < 0 || I < 0 || Ip < 0) return 0.0;
+        const double* u = cc.get(J, std::abs(J - Jp), Jp, A, K, Ap, I, std::abs(I - Ip), Ip);
+        const unsigned rs = op.row_spin[e], cs = op.col_spin[e];
+        const int casenr = (rs == 2 && cs == 2) ? 3 : rs == 2 ? 1 : cs == 2 ? 2 : 0;   // op_iterate.hpp:56-60
+        return u[casenr];
+    };
     auto mpo_row = [&](uint32_t b1, std::vector<AdjEntry>* adj_out) {
+        thread_local CouplingCache ccache;
         for (const auto& cell : W.rows[b1]) {
             uint32_t b2 = cell.other;
             for (uint32_t t = cell.t0; t < cell.t1; ++t) {
@@ -1865,8 +1916,11 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
                             const int32_t ni = g.npos[nd];
                             if (ni < 0) continue;
                             const uint64_t w = g.w_off - ws_init_off + uint64_t(ni) * g.K.size() + kg_ent[q].second;
-                            wd[w] += op.coeff[e] * sc;
-                            if (adj_out) adj_out->push_back({t, w, op.coeff[e]});
+                            const double c = sy.su2() ? op.coeff[e] * su2_factor(ccache, g, s_in, s_out, W.left_spin[b1],
+                                                                                 op.spin, W.right_spin[b2], e, op)
+                                                      : op.coeff[e];
+                            wd[w] += c * sc;
+                            if (adj_out) adj_out->push_back({t, w, c});
                         }
                     }
                 }
@@ -2464,7 +2518,7 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         MQ_REQUIRE(ctx && site && mpo && left && right && out, "bad argument");
         trace_phase("(caller)");
         SiteLayout L(*site);
-        if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
+        if (algorithm == MQ_SIGMA_AUTO) algorithm = MQ_SIGMA_GROUPED;
         if (algorithm == (MQ_SIGMA_AUTO | MQ_SIGMA_ADJOINT)) algorithm = MQ_SIGMA_GROUPED | MQ_SIGMA_ADJOINT;
         if (algorithm == MQ_SIGMA_SCHEDULE) {
             int rc = mq_sigma_plan_create(ctx, site, mpo, left, right, out);
@@ -2473,8 +2527,8 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         }
         const bool adjoint = (algorithm & MQ_SIGMA_ADJOINT) != 0;
         algorithm &= ~int(MQ_SIGMA_ADJOINT);
-        MQ_REQUIRE(algorithm == MQ_SIGMA_GROUPED || (algorithm == MQ_SIGMA_AUTO && !L.symm.su2()),
-                   adjoint ? "the sigma adjoint needs a GROUPED plan (abelian symmetry)" : "unknown sigma algorithm");
+        MQ_REQUIRE(algorithm == MQ_SIGMA_GROUPED,
+                   adjoint ? "the sigma adjoint needs a GROUPED plan" : "unknown sigma algorithm");
         Mpo W(*mpo);
         auto p = std::make_unique<mq_plan>();
         p->ctx = ctx;
@@ -2602,6 +2656,10 @@ int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const doub
                 HIPCHK(hipEventCreateWithFlags(&ctx->side_ev[k], hipEventDisableTiming));
                 ctx->lt_ws_side.push_back(std::make_unique<DevBuf>());
                 ctx->lt_ws_side.back()->alloc(LT_WORKSPACE);
+                rocblas_handle h = nullptr;
+                BLASCHK(rocblas_create_handle(&h));
+                ctx->side_blas.push_back(h);
+                BLASCHK(rocblas_set_stream(h, ctx->side[k]));
             }
             HIPCHK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
         }
@@ -2621,11 +2679,9 @@ int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const doub
             for (int k = 0; k < NSIDE; ++k) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
             for (int i = 0; i < n; ++i) {
                 hipStream_t s = ctx->side[i % NSIDE];
-                BLASCHK(rocblas_set_stream(ctx->blas, s));
                 HIPCHK(launch_fill(s, outs[i], plans[i]->vec_size, 0.0));
-                plans[i]->st.run(s, bs[i], precision);
+                plans[i]->st.run(s, bs[i], precision);   // library GEMMs take the side stream's own handle
             }
-            BLASCHK(rocblas_set_stream(ctx->blas, st));
             for (int k = 0; k < NSIDE; ++k) {
                 HIPCHK(hipEventRecord(ctx->side_ev[k], ctx->side[k]));
                 HIPCHK(hipStreamWaitEvent(st, ctx->side_ev[k], 0));
@@ -2636,12 +2692,7 @@ int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const doub
         BatchGraph& bg = ctx->batches[std::make_pair(key, precision)];
         if (!bg.exec && !bg.seen) {
             bg.seen = true;   // first use runs eagerly (library code loads on first use)
-            try {
-                issue();
-            } catch (...) {
-                (void)rocblas_set_stream(ctx->blas, st);
-                throw;
-            }
+            issue();
         } else {
             if (!bg.exec) {
                 hipGraph_t g = nullptr;
@@ -2649,7 +2700,6 @@ int mq_sigma_apply_batched(mq_ctx* ctx, int n, mq_plan* const* plans, const doub
                 try {
                     issue();
                 } catch (...) {
-                    (void)rocblas_set_stream(ctx->blas, st);
                     (void)hipStreamEndCapture(st, &g);
                     if (g) (void)hipGraphDestroy(g);
                     throw;
@@ -3177,7 +3227,7 @@ int mq_boundary_left_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && bra_lp && ket_rp && left && out, "bad argument");
         SiteLayout L(*site);
-        if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
+        if (algorithm == MQ_SIGMA_AUTO) algorithm = MQ_SIGMA_GROUPED;
         int rc;
         if (algorithm == MQ_SIGMA_SCHEDULE) {
             rc = left_step_schedule(ctx, site, mpo, bra_lp, ket_rp, left, symmetric, out);
@@ -3204,7 +3254,7 @@ int mq_boundary_right_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mp
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && bra_rp && ket_rp && right && out, "bad argument");
         SiteLayout L(*site);
-        if (algorithm == MQ_SIGMA_AUTO) algorithm = L.symm.su2() ? MQ_SIGMA_SCHEDULE : MQ_SIGMA_GROUPED;
+        if (algorithm == MQ_SIGMA_AUTO) algorithm = MQ_SIGMA_GROUPED;
         int rc;
         if (algorithm == MQ_SIGMA_SCHEDULE) {
             rc = right_step_schedule(ctx, site, mpo, bra_rp, ket_rp, right, symmetric, out);
@@ -3227,6 +3277,18 @@ int mq_vec_dot(mq_ctx* ctx, uint64_t n, const double* x, const double* y, double
         MQ_REQUIRE(ctx && x && y && result, "bad argument");
         HIPCHK(launch_dot(ctx->stream, n, x, y, ctx->red_partial.as<double>(), ctx->red_result.as<double>()));
         HIPCHK(hipMemcpyAsync(result, ctx->red_result.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mq_vec_multi_dot(mq_ctx* ctx, uint64_t n, int k, const double* V, uint64_t ldv, const double* x, double* result) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && V && x && result && k >= 1, "bad argument");
+        MQ_REQUIRE(k == 1 || ldv >= n, "ldv < n");
+        ctx->mdot_reserve(k);
+        HIPCHK(launch_multi_dot(ctx->stream, n, k, V, ldv, x, ctx->mdot_partial.as<double>(),
+                                ctx->mdot_result.as<double>()));
+        HIPCHK(hipMemcpyAsync(result, ctx->mdot_result.p, size_t(k) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -3329,6 +3391,7 @@ struct JdSpace {
     uint64_t n, stride;
     int max_iter;
     double* base;
+    std::vector<double> hres;   // host copy of the last batched reduction
     JdSpace(mq_plan* p_, int max_iter_) : p(p_), ctx(p_->ctx), n(p_->vec_size), stride(align32(p_->vec_size)),
                                           max_iter(max_iter_) {
         int cap = 2 * max_iter + 5;
@@ -3337,6 +3400,26 @@ struct JdSpace {
             p->jd_cap = cap;
         }
         base = p->jd_buf.as<double>();
+        ctx->mdot_reserve(max_iter + 1);
+    }
+    // r[j] = W_j . x for j < k with W_j = first + j * stride (contiguous basis vectors), one batched
+    // reduction and one host round trip; the results also stay on the device (ctx->mdot_result)
+    const double* mdot(const double* first, int k, const double* x) {
+        HIPCHK(launch_multi_dot(ctx->stream, n, k, first, stride, x, ctx->mdot_partial.as<double>(),
+                                ctx->mdot_result.as<double>()));
+        hres.resize(k);
+        HIPCHK(hipMemcpyAsync(hres.data(), ctx->mdot_result.p, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return hres.data();
+    }
+    // y -= sum_{j<k} r_j W_j with the coefficients of the last mdot, still on the device
+    void subtract_last(const double* first, int k, double* y) {
+        HIPCHK(launch_multi_axpy(ctx->stream, n, k, -1.0, ctx->mdot_result.as<double>(), nullptr, first, stride, y));
+    }
+    // y = sum_{j<k} c_j W_j (host coefficients)
+    void combine(const double* first, int k, const double* c, double* y) {
+        HIPCHK(launch_fill(ctx->stream, y, n, 0.0));
+        HIPCHK(launch_multi_axpy(ctx->stream, n, k, 1.0, nullptr, c, first, stride, y));
     }
     double* V(int i) { return base + uint64_t(i) * stride; }
     double* VA(int i) { return base + uint64_t(max_iter + 1 + i) * stride; }
@@ -3374,11 +3457,23 @@ struct JdSpace {
         double theta = 0, rn = 0;
         while (true) {
             double* t = V(it);
-            double tau = std::sqrt(dot(t, t));
-            for (int i = 0; i < it; ++i) axpy(-dot(V(i), t), V(i), t);
-            if (std::sqrt(dot(t, t)) < kappa * tau)
-                for (int i = 0; i < it; ++i) axpy(-dot(V(i), t), V(i), t);
-            double nt = std::sqrt(dot(t, t));
+            // Gram-Schmidt with refinement (ietl/jacobi.h:408-415) in its classical form: the overlaps
+            // of a pass come from one batched reduction together with t.t (V(it) is t itself), and
+            // the projection uses them on the device.  A second pass runs under the reference's
+            // condition ||t|| < kappa tau.
+            const double* r = mdot(V(0), it + 1, t);
+            const double tau = std::sqrt(r[it]);
+            double nt2 = r[it];
+            if (it > 0) {
+                subtract_last(V(0), it, t);
+                r = mdot(V(0), it + 1, t);
+                nt2 = r[it];
+                if (std::sqrt(nt2) < kappa * tau) {
+                    subtract_last(V(0), it, t);
+                    nt2 = mdot(V(0), it + 1, t)[it];
+                }
+            }
+            double nt = std::sqrt(nt2);
             MQ_REQUIRE(nt > 0, "Jacobi-Davidson: zero search vector");
             scal(1.0 / nt, t);
             if (it == 0 && init_A) {
@@ -3388,27 +3483,27 @@ struct JdSpace {
                 sigma(t, VA(it), precision);
                 ++*n_sigma;
             }
-            for (int i = 0; i <= it; ++i) M[i + size_t(it) * max_iter] = dot(V(i), VA(it));
+            {
+                const double* mc = mdot(V(0), it + 1, VA(it));   // M(i, it) = V(i) . VA(it), i <= it
+                for (int i = 0; i <= it; ++i) M[i + size_t(it) * max_iter] = mc[i];
+            }
             // with an exact first product, the coupling of the first vector to the new one is taken
             // from it: V(it).(H V(0)) is exact where V(0).(H~ V(it)) carries the reduced-precision
             // error, and it is this coupling that sets the size of the update of V(0)
             if (init_A && it > 0) M[size_t(it) * max_iter] = dot(V(it), VA(0));
             theta = smallest_eigenpair(M, max_iter, it + 1, s);
-            copy(V(0), u());
-            scal(s[0], u());
-            copy(VA(0), uA());
-            scal(s[0], uA());
-            for (int j = 1; j <= it; ++j) {
-                axpy(s[j], V(j), u());
-                axpy(s[j], VA(j), uA());
-            }
+            combine(V(0), it + 1, s.data(), u());     // u = V s
+            combine(VA(0), it + 1, s.data(), uA());   // uA = VA s
             axpy(-theta, u(), uA());   // r = uA - theta u
             ++it;
-            rn = std::sqrt(dot(uA(), uA()));
+            // u and r are consecutive vectors: [u . r, r . r] in one reduction
+            const double* ur = mdot(u(), 2, uA());
+            const double dru = ur[0];
+            rn = std::sqrt(ur[1]);
             bool conv = (rn <= std::max(tol * std::fabs(theta), stop_abs) || rn < tol);
             if (conv || it >= max_iter) break;
             // t = -r + (r.u)/(u.u) u   (jcd_gmres_solver with max_iter 0, ietl/jacobi.h:141-157)
-            double dru = dot(uA(), u()), duu = dot(u(), u());
+            const double duu = dot(u(), u());
             double* tn = V(it);
             copy(uA(), tn);
             scal(-1.0, tn);

@@ -278,6 +278,79 @@ hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double*
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------- batched dot / axpy
+// partial[j * RED_BLOCKS + b] = sum over the b-th grid stride of V_j[i] x[i], j < k <= MDOT_MAXK: each
+// workgroup reads its part of x once for all k vectors (V_j = V + j * ldv).
+__global__ __launch_bounds__(256) void multi_dot_partial_kernel(uint64_t n, int k, const double* __restrict__ V,
+                                                                uint64_t ldv, const double* __restrict__ x,
+                                                                double* __restrict__ partial) {
+    double s[MDOT_MAXK];
+#pragma unroll
+    for (int j = 0; j < MDOT_MAXK; ++j) s[j] = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) {
+        const double xi = x[i];
+#pragma unroll
+        for (int j = 0; j < MDOT_MAXK; ++j)
+            if (j < k) s[j] += V[uint64_t(j) * ldv + i] * xi;
+    }
+#pragma unroll
+    for (int j = 0; j < MDOT_MAXK; ++j) {
+        if (j >= k) break;   // uniform across the workgroup
+        const double r = block_sum_256(s[j]);
+        if (threadIdx.x == 0) partial[size_t(j) * RED_BLOCKS + blockIdx.x] = r;
+    }
+}
+
+// out[j] = sum_b partial[j * RED_BLOCKS + b], one workgroup per vector (fixed order)
+__global__ __launch_bounds__(256) void multi_reduce_kernel(const double* __restrict__ partial, double* __restrict__ out) {
+    double s = 0.0;
+    const double* p = partial + size_t(blockIdx.x) * RED_BLOCKS;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += 256) s += p[i];
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+hipError_t launch_multi_dot(hipStream_t st, uint64_t n, int k, const double* V, uint64_t ldv, const double* x,
+                            double* partial, double* result) {
+    for (int j0 = 0; j0 < k; j0 += MDOT_MAXK) {
+        const int kk = k - j0 < MDOT_MAXK ? k - j0 : MDOT_MAXK;
+        hipLaunchKernelGGL(multi_dot_partial_kernel, dim3(RED_BLOCKS), dim3(256), 0, st, n, kk, V + uint64_t(j0) * ldv,
+                           ldv, x, partial + size_t(j0) * RED_BLOCKS);
+    }
+    hipLaunchKernelGGL(multi_reduce_kernel, dim3(k), dim3(256), 0, st, partial, result);
+    return hipGetLastError();
+}
+
+// y[i] += alpha * sum_{j<k} c_j V_j[i], with c from device memory (c_dev) or by value (ch)
+struct MdotCoef { double c[MDOT_MAXK]; };
+__global__ void multi_axpy_kernel(uint64_t n, int k, double alpha, const double* __restrict__ c_dev, MdotCoef ch,
+                                  const double* __restrict__ V, uint64_t ldv, double* __restrict__ y) {
+    double c[MDOT_MAXK];
+#pragma unroll
+    for (int j = 0; j < MDOT_MAXK; ++j) c[j] = j < k ? alpha * (c_dev ? c_dev[j] : ch.c[j]) : 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        double acc = y[i];
+#pragma unroll
+        for (int j = 0; j < MDOT_MAXK; ++j)
+            if (j < k) acc += c[j] * V[uint64_t(j) * ldv + i];
+        y[i] = acc;
+    }
+}
+
+hipError_t launch_multi_axpy(hipStream_t st, uint64_t n, int k, double alpha, const double* c_dev, const double* c_host,
+                             const double* V, uint64_t ldv, double* y) {
+    if (n == 0 || k <= 0) return hipSuccess;
+    for (int j0 = 0; j0 < k; j0 += MDOT_MAXK) {
+        const int kk = k - j0 < MDOT_MAXK ? k - j0 : MDOT_MAXK;
+        MdotCoef ch{};
+        if (c_host)
+            for (int j = 0; j < kk; ++j) ch.c[j] = c_host[j0 + j];
+        hipLaunchKernelGGL(multi_axpy_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, n, kk, alpha,
+                           c_dev ? c_dev + j0 : nullptr, ch, V + uint64_t(j0) * ldv, ldv, y);
+    }
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void block_sums_kernel(const uint64_t* __restrict__ blk_off,
                                                          const uint32_t* __restrict__ blk_len,
                                                          const double* __restrict__ data, double* __restrict__ out) {

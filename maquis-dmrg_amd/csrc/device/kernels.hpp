@@ -185,6 +185,15 @@ hipError_t launch_split_w_bf16(hipStream_t st, const double* W, uint64_t ldw, ui
 hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial,
                       double* result);
 hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y);
+// batched dot: result[j] = V_j . x for j < k (V_j = V + j * ldv; result on the device);
+// partial holds k * RED_BLOCKS doubles.  Fixed grid and order: bitwise reproducible.
+constexpr int MDOT_MAXK = 16;   // vectors per pass of the batched kernels
+hipError_t launch_multi_dot(hipStream_t st, uint64_t n, int k, const double* V, uint64_t ldv, const double* x,
+                            double* partial, double* result);
+// batched axpy: y += alpha * sum_{j<k} c_j V_j, coefficients from device memory (c_dev) or the host
+// (c_host, passed by value); exactly one of them is non-null
+hipError_t launch_multi_axpy(hipStream_t st, uint64_t n, int k, double alpha, const double* c_dev, const double* c_host,
+                             const double* V, uint64_t ldv, double* y);
 hipError_t launch_scal(hipStream_t st, uint64_t n, double a, double* x);
 hipError_t launch_permute(hipStream_t st, uint64_t n, const uint64_t* idx, const double* src, double* dst);
 // per-b sums of boundary blocks: out[b] = sum of block b over all cohorts

@@ -209,6 +209,25 @@ double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i);
 void su2_set_coupling(int a, int b, int c, int d, int e, int f, int g, int h, int i,
                       double init, double couplings[4]);
 
+// Memoised unit couplings keyed on all nine arguments (the reference caches per (J, I, I'),
+// Wigner9jCacheI in gsl_coupling.h:129-204; the values are the same).
+class CouplingCache {
+public:
+    const double* get(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
+        uint64_t key = 0;
+        int v[9] = {a, b, c, d, e, f, g, h, i};
+        for (int k = 0; k < 9; ++k) key = key * 131 + uint64_t(v[k] + 3);
+        auto it = map_.find(key);
+        if (it != map_.end()) return it->second.data();
+        std::array<double, 4> r;
+        su2_set_coupling(a, b, c, d, e, f, g, h, i, 1.0, r.data());
+        return map_.emplace(key, r).first->second.data();
+    }
+private:
+    std::unordered_map<uint64_t, std::array<double, 4>> map_;
+};
+
+
 // ---------------------------------------------------------------------------- Schedule
 // Flattened ScheduleNew: per MPS block the T tuples and cohorts; per cohort its units;
 // per unit its lines and (ti, col, alpha) tasks.

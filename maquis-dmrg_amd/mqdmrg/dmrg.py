@@ -16,7 +16,8 @@ import torch
 from . import engine as E
 from . import mps as M
 from . import models
-from .symmetry import Index, SiteLayout, fuse, sub, product_index
+from . import su2
+from .symmetry import Index, SiteLayout, fuse, sub, product_index, SYM_SU2U1
 
 
 def slab_copy(dst, src, slabs):
@@ -35,6 +36,22 @@ def slab_copy(dst, src, slabs):
     P = T[sid]
     i, j = loc % P[:, 6], loc // P[:, 6]
     dst[P[:, 0] + i * P[:, 1] + j * P[:, 2]] = src[P[:, 3] + i * P[:, 4] + j * P[:, 5]]
+
+
+def apply_segments(x, segs):
+    """out[o + t] = sum of c * x[i + t] over the segments (o, i, n, c), t < n (flat tensors on one
+    device); at most two segments meet in one output element, so the sum is order independent."""
+    out = torch.zeros_like(x)
+    if not segs:
+        return out
+    S = torch.tensor([(o, i, n) for o, i, n, _ in segs], dtype=torch.int64, device=x.device)
+    cf = torch.tensor([c for _, _, _, c in segs], dtype=x.dtype, device=x.device)
+    n = S[:, 2]
+    total = int(n.sum())
+    sid = torch.repeat_interleave(torch.arange(len(segs), device=x.device), n, output_size=total)
+    loc = torch.arange(total, device=x.device) - (torch.cumsum(n, 0) - n)[sid]
+    out.index_add_(0, S[sid, 0] + loc, x[S[sid, 1] + loc] * cf[sid])
+    return out
 
 
 class SiteTensor:
@@ -313,11 +330,13 @@ class TwoSiteDMRG:
         self.phys = basis.phys
         self.mpo_desc = [E.MpoDesc(W) for W in mpo]
         self.ts_mpo = {}
+        self.su2 = self.sym == SYM_SU2U1
         rng = np.random.default_rng(seed)
+        phys_ok = M.physical_filter(self.sym)
         if init_bonds == "total":
-            bonds = M.random_bond_dims([self.phys] * L, target, init_m)
+            bonds = M.random_bond_dims([self.phys] * L, target, init_m, physical=phys_ok)
         else:
-            bonds, _, _ = M.allowed_sectors([self.phys] * L, target, init_m)
+            bonds, _, _ = M.allowed_sectors([self.phys] * L, target, init_m, physical=phys_ok)
         self.sites = []
         for i in range(L):
             lay = SiteLayout(bonds[i], self.phys, bonds[i + 1])
@@ -405,9 +424,19 @@ class TwoSiteDMRG:
     # ------------------------------------------------------------------ two-site tensors
     def twosite_mpo(self, i):
         if i not in self.ts_mpo:
-            W12, phys12, pb = models.twosite_sparse(self.mpo[i], self.mpo[i + 1], self.phys, self.phys)
+            if self.su2:
+                W12, phys12, pb = su2.twosite_su2(self.mpo[i], self.mpo[i + 1], self.phys)
+            else:
+                W12, phys12, pb = models.twosite_sparse(self.mpo[i], self.mpo[i + 1], self.phys, self.phys)
             self.ts_mpo[i] = (W12, E.MpoDesc(W12), phys12, pb)
         return self.ts_mpo[i]
+
+    def _recouple(self, theta, reduce):
+        """SU2U1: the two-site tensor in the coupled two-site basis of the site problem
+        (TwoSiteTensor::make_mps -> ts_reduction::reduce_right, twositetensor.hpp:131-151) and
+        back before the split (operator<< -> unreduce_right, twositetensor.hpp:296-326)."""
+        segs = (su2.reduction_segments if reduce else su2.unreduction_segments)(theta.left, self.phys, theta.right)
+        theta.data = apply_segments(theta.data, segs)
 
     def _pairs(self, A1, A2, phys12, pb):
         """Enumerate (mid, s1, lc, s2, rc) slabs: rows of the (s1, lc) x mid left-paired A1 and
@@ -456,11 +485,15 @@ class TwoSiteDMRG:
         slabs = [(theta.layout.rp_offset[lb] + col12 * ls, 1, ls, base[mc] + r * ncols[mc] + c, ncols[mc], 1, ls, rs)
                  for (mc, lb, ls, r, rb, rs, c, col12) in self._pairs(A1, A2, phys12, pb) if mc in base]
         slab_copy(theta.data, flat, slabs)
+        if self.su2:
+            self._recouple(theta, True)
         return theta
 
     def split(self, i, theta, direction):
         """Truncated SVD split of the two-site tensor (split_mps_l2r / split_mps_r2l)."""
         A1, A2 = self.sites[i], self.sites[i + 1]
+        if self.su2:
+            self._recouple(theta, False)
         W12, wdesc, phys12, pb = self.twosite_mpo(i)
         # mid sectors reachable from both sides
         lidx = {}

@@ -68,6 +68,15 @@ class Engine:
         check(lib().mq_vec_nrm2(self.ctx, x.numel(), dev_ptr(self, x, x.numel()), C.byref(r)))
         return r.value
 
+    def multi_dot(self, V, k, ldv, x):
+        """[V_j . x for j < k], V_j = V[j * ldv : j * ldv + n] (n = x.numel()), one batched reduction."""
+        n = x.numel()
+        out = np.zeros(k, np.float64)
+        need = (k - 1) * ldv + n
+        check(lib().mq_vec_multi_dot(self.ctx, n, int(k), dev_ptr(self, V, need), int(ldv), dev_ptr(self, x, n),
+                                     out.ctypes.data_as(C.c_void_p)))
+        return out
+
     def axpy(self, a, x, y):
         n = x.numel()
         check(lib().mq_vec_axpy(self.ctx, n, float(a), dev_ptr(self, x, n), dev_ptr(self, y, n)))
@@ -436,8 +445,7 @@ def apply_batched(engine, plans, psis, sigmas, precision=PREC_F64):
 
 
 def left_step(engine, site, mpo_desc, bra_lp, ket_rp, left, symmetric=True, algorithm=SIGMA_AUTO):
-    """overlap_mpo_left_step; algorithm SIGMA_AUTO (grouped for abelian groups), SIGMA_SCHEDULE or
-    SIGMA_GROUPED."""
+    """overlap_mpo_left_step; algorithm SIGMA_AUTO (= grouped), SIGMA_SCHEDULE or SIGMA_GROUPED."""
     h = C.c_void_p()
     check(lib().mq_boundary_left_step_ex(engine.ctx, C.byref(site.desc), C.byref(mpo_desc.desc),
                                          dev_ptr(engine, bra_lp, site.lp_total), dev_ptr(engine, ket_rp, site.rp_total),

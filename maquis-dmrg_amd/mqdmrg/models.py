@@ -64,9 +64,6 @@ class Op:
                         res.add(ci, co, oi + r1 * dbi + r2, oo + c1 * dbo + c2, v1 * v2)
         return res
 
-    def scaled_sum(self, terms):
-        raise NotImplementedError
-
 
 def _merge(op):
     """Sum duplicate (row, col) entries and drop zeros."""
@@ -396,6 +393,16 @@ def chain(model, L, **kw):
         basis, mpo, _ = qc.qc_mpo(h, eri, ecore)
         N = kw.get("N", L)
         return basis, mpo, (N // 2, N - N // 2)
+    if model == "qc_su2":
+        from . import qc, su2
+        if "fcidump" in kw:
+            h, eri, ecore = qc.read_fcidump(kw["fcidump"])
+        else:
+            h, eri = qc.synthetic_integrals(L, seed=kw.get("integral_seed", 0))
+            ecore = kw.get("ecore", 0.0)
+        N = kw.get("N", L)
+        basis, mpo, _ = su2.qc_su2_mpo(h, eri, N, ecore)
+        return basis, mpo, (N, kw.get("two_s", 0))
     raise ValueError(model)
 
 

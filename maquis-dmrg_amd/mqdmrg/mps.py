@@ -7,32 +7,40 @@ import math
 
 import numpy as np
 
-from .symmetry import Index, SiteLayout, fuse, sub
+from .symmetry import Index, SiteLayout, fuse, sub, SYM_SU2U1
 
 
-def _fuse_index(a, b, sign=+1, cap=None):
+def _fuse_index(a, b, sign=+1, cap=None, physical=None):
     acc = {}
     for ca, sa in zip(a.charges, a.sizes):
         for cb, sb in zip(b.charges, b.sizes):
             c = fuse(ca, cb) if sign > 0 else sub(ca, cb)
+            if physical is not None and not physical(c):
+                continue
             acc[c] = acc.get(c, 0) + sa * sb
     if cap is not None:
         acc = {c: min(cap, s) for c, s in acc.items()}
     return acc
 
 
-def allowed_sectors(phys_list, right_end, Mmax, left_end=(0, 0)):
-    """Bond indices 0..L of a chain with per-site physical indices phys_list (mps_sectors.h:57-150)."""
+def physical_filter(sym):
+    """Charges a bond may carry: SU2U1 spins are non-negative (mps_sectors.h:43-52)."""
+    return (lambda c: c[1] >= 0) if sym == SYM_SU2U1 else None
+
+
+def allowed_sectors(phys_list, right_end, Mmax, left_end=(0, 0), physical=None):
+    """Bond indices 0..L of a chain with per-site physical indices phys_list (mps_sectors.h:57-150);
+    physical: optional predicate on charges (SU2U1: non-negative spin), applied at every step."""
     L = len(phys_list)
     left = [None] * (L + 1)
     right = [None] * (L + 1)
     left[0] = {tuple(left_end): 1}
     for i in range(L):
         # exact (arbitrary precision) state counts
-        left[i + 1] = _fuse_index(Index.from_dict(left[i]), phys_list[i], +1, cap=None)
+        left[i + 1] = _fuse_index(Index.from_dict(left[i]), phys_list[i], +1, cap=None, physical=physical)
     right[L] = {tuple(right_end): 1}
     for i in range(L - 1, -1, -1):
-        right[i] = _fuse_index(Index.from_dict(right[i + 1]), phys_list[i], -1, cap=None)
+        right[i] = _fuse_index(Index.from_dict(right[i + 1]), phys_list[i], -1, cap=None, physical=physical)
     out = []
     for i in range(L + 1):
         common = {c: min(Mmax, left[i][c], right[i][c]) for c in left[i] if c in right[i]}
@@ -40,12 +48,12 @@ def allowed_sectors(phys_list, right_end, Mmax, left_end=(0, 0)):
     return out, left, right
 
 
-def random_bond_dims(phys_list, right_end, m, left_end=(0, 0), sigma=2.0):
+def random_bond_dims(phys_list, right_end, m, left_end=(0, 0), sigma=2.0, physical=None):
     """Synthetic bond indices of total dimension ~m.  The sector sizes follow a Gaussian in the
     charge distance (in units of the smallest charge step) from the most populated sector, of
     width sigma, the way the Schmidt spectrum of a ground state concentrates around the mean
     charge; each size is capped by the number of states both chain ends can reach."""
-    _, left, right = allowed_sectors(phys_list, right_end, 1, left_end)
+    _, left, right = allowed_sectors(phys_list, right_end, 1, left_end, physical=physical)
     L = len(phys_list)
     steps = [abs(c[k]) for I in phys_list for c in I.charges for k in (0, 1) if c[k] != 0]
     unit = min(steps) if steps else 1

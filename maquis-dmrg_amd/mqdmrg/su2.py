@@ -331,21 +331,28 @@ def integral_elements(h, eri, ecore=0.0, tol=1e-14):
     return out
 
 
-def qc_su2_terms(elements, L, N):
+def qc_su2_terms(elements, L, N, sources=None):
     """Terms of qc_su2::create_terms (model.hpp:321-580): ({((pos, op), ...): coeff} with
     positions ascending, core energy).  Terms with identical operators and positions are summed
-    (chem_helper.h:69-100)."""
+    (chem_helper.h:69-100).  sources: optional dict filled with key -> [(element index, factor)],
+    coefficient = sum of factor * element value (every term is linear in one element)."""
     terms = {}
     ecore = 0.0
+    cur = {"m": 0, "v": 0.0}
 
     def add(lst):
-        for c, ops in lst:
+        # the builders run with a unit matrix element; the element scales the factor here
+        for f, ops in lst:
             key = tuple(sorted(ops))
-            terms[key] = terms.get(key, 0.0) + c
+            terms[key] = terms.get(key, 0.0) + f * cur["v"]
+            if sources is not None:
+                sources.setdefault(key, []).append((cur["m"], f))
     cr, de = _OPS["create"], _OPS["destroy"]
-    for v, i, j, k, l in elements:
+    for m_, (v, i, j, k, l) in enumerate(elements):
+        cur["m"], cur["v"] = m_, v
+        v = 1.0
         if i == j == k == l == -1:
-            ecore += v
+            ecore += cur["v"]
         elif i == j and k == -1 and l == -1:
             add([(v, ((i, "count"),))])
         elif k == -1 and l == -1:
@@ -383,25 +390,36 @@ def qc_su2_terms(elements, L, N):
     return terms, ecore
 
 
-def qc_su2_mpo(h, eri, N, ecore=0.0, tol=1e-14):
+def qc_su2_mpo(h, eri, N, ecore=0.0, tol=1e-14, sources=False):
     """MPO of the spin-adapted ab initio Hamiltonian in the N-electron sector.
-    Returns (site basis, MPO tensors with bond spins, bond dimensions)."""
+    Returns (site basis, MPO tensors with bond spins, bond dimensions).
+
+    sources=True: also returns the integral elements (integral_elements) and, per site j,
+    {(b1, b2, op_id): [(element index, factor), ...]}: the elements summed into each switch
+    coefficient; the tensors keep zero coefficients and carry op_ids, so that derivatives with
+    respect to every MPO coefficient map back to the elements (mpo_gradient_to_elements)."""
     from . import qc
     L = h.shape[0]
     site = su2_site()
     names = list(site.ops)
     oid = {n: k for k, n in enumerate(names)}
     ops = [site.ops[n] for n in names]
-    terms, ec = qc_su2_terms(integral_elements(h, eri, ecore, tol), L, N)
-    site_terms = {}
+    elements = integral_elements(h, eri, ecore, tol)
+    src = {} if sources else None
+    terms, ec = qc_su2_terms(elements, L, N, sources=src)
+    site_terms, key_src = {}, {}
     for key, c in terms.items():
-        if abs(c) < tol:
+        if abs(c) < tol and not sources:
             continue
         k2 = tuple((p, oid[n]) for p, n in key)
         site_terms[k2] = site_terms.get(k2, 0.0) + c
-    if abs(ec) > 0:
+        if sources:
+            key_src.setdefault(k2, []).extend(src[key])
+    if abs(ec) > 0 or sources:
         k2 = ((0, oid["ident"]),)
         site_terms[k2] = site_terms.get(k2, 0.0) + ec
+        if sources:
+            key_src.setdefault(k2, []).append((len(elements) - 1, 1.0))
     parity = [int(o.fermionic) for o in ops]
     action = [o.action for o in ops]
     i_id, i_full, i_fill = oid["ident"], oid["ident_full"], oid["fill"]
@@ -410,9 +428,47 @@ def qc_su2_mpo(h, eri, N, ecore=0.0, tol=1e-14):
         if par % 2:
             return i_fill
         return i_full if spin2 > 1 else i_id
-    auto = qc.operator_automaton(site_terms, L, parity, pass_op, action=action)
-    mpo = auto.tensors(ops)
+    auto = qc.operator_automaton(site_terms, L, parity, pass_op, action=action,
+                                 sources=key_src if sources else None)
+    mpo = auto.tensors(ops, keep_zero=sources)
+    if sources:
+        return site, mpo, [len(b) for b in auto.bonds], elements, auto.switch_src
     return site, mpo, [len(b) for b in auto.bonds]
+
+
+def mpo_gradient_to_elements(mpo, switch_src, term_grads, n_elements):
+    """dE/d(element value) for every integral element from the derivatives with respect to the MPO
+    term scales of every site (term_grads[j] in mpo[j].sorted_terms() order)."""
+    g = np.zeros(n_elements)
+    for j, (W, tg) in enumerate(zip(mpo, term_grads)):
+        b1s, b2s, ois, _ = W.sorted_terms()
+        assert len(tg) == len(b1s)
+        for t in range(len(b1s)):
+            for m, f in switch_src[j].get((int(b1s[t]), int(b2s[t]), W.op_ids[int(ois[t])]), ()):
+                g[m] += f * tg[t]
+    return g
+
+
+def element_gradient_to_tensors(elements, g, L):
+    """Per-entry derivatives dE/dh[p, q] and dE/d(pq|rs) (each tensor entry a parameter of its
+    own, as in qc.fci_energy_gradient_fd) from dE/d(element): an element stands for its whole
+    permutation class, whose entries share the derivative equally.  Returns (dh, deri, decore)."""
+    dh = np.zeros((L, L))
+    deri = np.zeros((L, L, L, L))
+    decore = 0.0
+    for (v, i, j, k, l), gm in zip(elements, g):
+        if i == j == k == l == -1:
+            decore += gm
+        elif k == -1:
+            cls = {(i, j), (j, i)}
+            for idx in cls:
+                dh[idx] += gm / len(cls)
+        else:
+            cls = {(i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k), (k, l, i, j), (l, k, i, j), (k, l, j, i),
+                   (l, k, j, i)}
+            for idx in cls:
+                deri[idx] += gm / len(cls)
+    return dh, deri, decore
 
 
 # ---------------------------------------------------------------------------- two-site MPO

@@ -6,10 +6,10 @@ depends only on these shapes, so this gives the benchmark workload of a site in 
 chain at bond dimension m without running sweeps first."""
 import numpy as np
 
-from . import models, mps
+from . import models, mps, su2
 from .capi import BIndex
 from .engine import MpoDesc, SiteDesc, step_index
-from .symmetry import SiteLayout
+from .symmetry import SiteLayout, SYM_SU2U1
 
 
 def trivial_bindex(index, aux_dim):
@@ -33,8 +33,8 @@ class SyntheticSite:
         self.chain = (basis, mpo, target)
         self.basis, self.mpo = basis, mpo
         phys = [basis.phys] * L
-        self.bonds = mps.random_bond_dims(phys, target, m)
         sym = basis.sym
+        self.bonds = mps.random_bond_dims(phys, target, m, physical=mps.physical_filter(sym))
         bi = trivial_bindex(self.bonds[0], mpo[0].row_dim)
         for k in range(site):
             sd = SiteDesc(sym, self.bonds[k], basis.phys, self.bonds[k + 1])
@@ -45,7 +45,10 @@ class SyntheticSite:
             sd = SiteDesc(sym, self.bonds[k], basis.phys, self.bonds[k + 1])
             bi = step_index("right", sd, MpoDesc(mpo[k]), bi)
         self.right_index = bi
-        W12, phys12, _ = models.twosite_sparse(mpo[site], mpo[site + 1], basis.phys, basis.phys)
+        if sym == SYM_SU2U1:
+            W12, phys12, _ = su2.twosite_su2(mpo[site], mpo[site + 1], basis.phys)
+        else:
+            W12, phys12, _ = models.twosite_sparse(mpo[site], mpo[site + 1], basis.phys, basis.phys)
         self.W12, self.phys12 = W12, phys12
         self.mpo_desc = MpoDesc(W12)
         self.layout = SiteLayout(self.bonds[site], phys12, self.bonds[site + 2])

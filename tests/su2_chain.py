@@ -47,7 +47,7 @@ class SU2BlockProblem:
         phys = basis.phys
         lb, rb = exact_bonds(phys, L, N, (N, two_s))
         self.Lb = O.trivial_boundary(lb[0], mpo[0].row_dim)
-        self.left_sites = []
+        self.left_sites, self.left_bounds = [], [self.Lb]
         for k in range(site):
             left, right = lb[k], lb[k + 1]
             lay = SiteLayout(left, phys, right)
@@ -60,8 +60,9 @@ class SU2BlockProblem:
             sd = SiteDesc(SYM_SU2U1, left, phys, right)
             self.left_sites.append((sd, v, rp))
             self.Lb = O.left_step(sd, MpoDesc(mpo[k]), v, rp, self.Lb, True)
+            self.left_bounds.append(self.Lb)
         self.Rb = O.trivial_boundary(rb[L], mpo[L - 1].col_dim)
-        self.right_sites = []
+        self.right_sites, self.right_bounds = [], [self.Rb]
         for k in range(L - 1, site + 1, -1):
             left, right = rb[k], rb[k + 1]
             lay = SiteLayout(left, phys, right)
@@ -73,6 +74,7 @@ class SU2BlockProblem:
             sd = SiteDesc(SYM_SU2U1, left, phys, right)
             self.right_sites.append((sd, v))
             self.Rb = O.right_step(sd, MpoDesc(mpo[k]), v, v, self.Rb, True)
+            self.right_bounds.append(self.Rb)
         self.W12, self.phys12, self.pb = su2.twosite_su2(mpo[site], mpo[site + 1], phys)
         self.mpo_desc = MpoDesc(self.W12)
         left, right = lb[site], rb[site + 2]

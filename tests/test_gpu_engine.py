@@ -478,3 +478,53 @@ def test_batched_apply_equals_individual(eng, precision):
         eng.sync()
         for i, (o, r) in enumerate(zip(outs, refs)):
             assert torch.equal(o, r), (rep, i)
+
+
+@pytest.mark.parametrize("model", ["qc_su2", "qc"])
+def test_batched_apply_at_batch_shapes_equals_individual(eng, model):
+    """16 independent ab initio instances at the batch workload's shapes (L uniform in [12, 24],
+    m=256): the FP64 batch (library GEMMs of the S stage on four side streams, one rocBLAS handle
+    and workspace per stream) equals the individual applications bit for bit, eagerly and on
+    graph replays."""
+    from mqdmrg.workloads import SyntheticSite
+    rng = np.random.default_rng(7)
+    plans, psis, refs, keep = [], [], [], []
+    for i in range(16):
+        L = int(rng.integers(12, 25))
+        print("instance", i, model, L, flush=True)
+        ws = SyntheticSite(model, L, L // 2 - 1, 256, seed=100 + i)
+        lb = E.DeviceBoundary.upload(eng, ws.left_index, ws.random_cohorts(ws.left_index))
+        rb = E.DeviceBoundary.upload(eng, ws.right_index, ws.random_cohorts(ws.right_index))
+        plan = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, lb, rb, algorithm=E.SIGMA_GROUPED)
+        psi = dev(ws.random_psi())
+        ref = torch.zeros_like(psi)
+        plan.set_graphs(False)
+        plan.apply(psi, ref)
+        plans.append(plan)
+        psis.append(psi)
+        refs.append(ref)
+        keep += [lb, rb]
+    eng.sync()
+    outs = [torch.zeros_like(x) for x in psis]
+    for rep in range(3):   # eager, capture, replay
+        for o in outs:
+            o.fill_(np.nan)
+        E.apply_batched(eng, plans, psis, outs)
+        eng.sync()
+        for i, (o, r) in enumerate(zip(outs, refs)):
+            assert torch.equal(o, r), (rep, i)
+    for p in plans:
+        p.close()
+
+
+def test_multi_dot_matches_numpy(eng):
+    """Batched dot products over k contiguous vectors (more than one pass of 16) equal numpy's to
+    rounding, and repeat bitwise."""
+    rng = np.random.default_rng(4)
+    n, k, ldv = 100003, 37, 100032
+    V = rng.standard_normal(k * ldv)
+    x = rng.standard_normal(n)
+    got = eng.multi_dot(dev(V), k, ldv, dev(x))
+    ref = np.array([V[j * ldv:j * ldv + n] @ x for j in range(k)])
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(got, eng.multi_dot(dev(V), k, ldv, dev(x)))
