@@ -37,6 +37,13 @@ thread_local std::string g_err;
 
 const bool g_trace = trace_enabled();
 
+// The block-sparse S stage (FP64 / FP32) is on by default; MQ_S_DENSE=1 makes plans created while
+// it is set run every group's S stage as one dense library GEMM instead.
+bool s_sparse_enabled() {
+    const char* e = std::getenv("MQ_S_DENSE");
+    return !(e && e[0] == '1');
+}
+
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
         hipError_t e_ = (x);                                                                       \
@@ -470,8 +477,15 @@ struct StagePlan {
         uint64_t a_off, b_off, c_off;
         uint32_t M, N, K, lda, ldb, ldc;
         uint64_t bfa_off, bfb_off;
+        bool sparse = false;   // FP64 / FP32: the group runs as block-sparse MFMA tasks (gS) instead
     };
     std::vector<LibGemm> lib;
+    // grouped path, FP64 and FP32: the S stage of groups whose W_D is sparse enough, as block-sparse
+    // MFMA tasks over the nonzero 16-row tiles of W_D (one task per 128-row panel of T_D and 128-column
+    // panel of W_D, its segments the runs of nonzero tiles of that column panel)
+    std::vector<GemmTask> s_tasks;
+    std::vector<GemmSeg> s_segs;
+    GemmStage gS;
     rocblas_handle blas = nullptr;
     // hipBLASLt form of the FP32 (index 0) and BF16x3 (index 1) library GEMMs, prepared on first
     // use outside graph capture; a GEMM without a hipBLASLt algorithm runs on rocBLAS
@@ -554,6 +568,7 @@ struct StagePlan {
     // (cached in the context by shape and by shape class); returns the number of timings.  A GEMM
     // whose algorithm came from either cache is not timed again.
     int lt_tune(int mi, size_t i, hipStream_t st, Bases b, hipEvent_t e0, hipEvent_t e1) const {
+        if (mi == 0 && lib[i].sparse) return 0;   // FP32 runs this group block-sparse
         if (!lt_prepare(mi, i, st)) return 0;
         LtGemm& L = ltg[mi][i];
         if (L.chosen) return 0;
@@ -622,6 +637,7 @@ struct StagePlan {
         gT.build(t_tasks);
         gF.build(f_tasks);
         gU.build(u_tasks);
+        gS.build(s_tasks);
         red.finish();
         std::vector<GatherRegion> big;
         for (const auto& r : regions) {
@@ -793,6 +809,7 @@ struct StagePlan {
         gemm(t_tasks, t_segs, split_t);
         gemm(f_tasks, f_segs, false);
         gemm(u_tasks, u_segs, false);
+        gemm(s_tasks, s_segs, false);
         if (split_t)
             for (const auto& t : t_tasks)
                 MQ_REQUIRE(t.rowunit != MAP_NONE && t.c_base == BASE_WS && !t.beta, "split T stage needs stacked tasks");
@@ -824,6 +841,7 @@ struct StagePlan {
         gT.upload_all(t_segs, st);
         gF.upload_all(f_segs, st);
         gU.upload_all(u_segs, st);
+        gS.upload_all(s_segs, st);
         upload(d_regions, regions, st);
         upload(d_small, small, st);
         upload(d_src_off, src_off, st);
@@ -867,6 +885,7 @@ struct StagePlan {
                 }
                 break;
             case 2:
+                if (mode != MODE_BF16X3) gS.run(st, mixed ? MODE_F32 : MODE_F64, mixed ? "fff" : "ddd", b, tb);
                 run_lib(st, b, mode);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
@@ -891,13 +910,15 @@ struct StagePlan {
             const double one = 1.0, zero = 0.0;
             const double* ws = b.p[BASE_WS];
             for (const auto& g : lib)
-                BLASCHK(rocblas_dgemm(blas, nn, nn, g.M, g.N, g.K, &one, ws + g.a_off, g.lda, ws + g.b_off, g.ldb, &zero,
+                if (!g.sparse)
+                    BLASCHK(rocblas_dgemm(blas, nn, nn, g.M, g.N, g.K, &one, ws + g.a_off, g.lda, ws + g.b_off, g.ldb, &zero,
                                       const_cast<double*>(ws) + g.c_off, g.ldc));
         } else if (mode == MODE_F32) {
             const float one = 1.f, zero = 0.f;
             float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
             for (size_t i = 0; i < lib.size(); ++i) {
                 const auto& g = lib[i];
+                if (g.sparse) continue;
                 if (lt_prepare(0, i, st))
                     lt_run(0, i, st, ws + g.a_off, ws + g.b_off, ws + g.c_off);
                 else
@@ -1673,6 +1694,130 @@ struct GroupedStep {
     uint32_t lo = 0, hi = 0xffffffffu;
 };
 
+// Pattern-class order of one group's keys (see lower_sigma_grouped): rows k and columns n of W_D
+// (Wg[n * K + k]) are classed by a hash of their nonzero index lists; classes are ordered by size
+// (largest first), then by their first key; the all-zero class goes last.  Permutes W_D, the group's
+// K / N lists, kpos / npos and its pairs' kk / nn position lists consistently.
+void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* Wg) {
+    const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
+    if (Kg < 2 && Ng < 2) return;
+    const uint64_t seed = 0x9E3779B97F4A7C15ull;
+    auto mix = [](uint64_t h, uint64_t v) {
+        h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        return h * 0xFF51AFD7ED558CCDull;
+    };
+    std::vector<uint64_t> hk(Kg, seed), hn(Ng, seed);
+    std::vector<uint32_t> ck(Kg, 0), cn(Ng, 0);   // nonzero counts (the all-zero class)
+    for (uint32_t n = 0; n < Ng; ++n) {
+        const double* col = Wg + uint64_t(n) * Kg;
+        for (uint32_t k = 0; k < Kg; ++k)
+            if (col[k] != 0.0) {
+                hk[k] = mix(hk[k], n + 1);
+                hn[n] = mix(hn[n], k + 1);
+                ++ck[k];
+                ++cn[n];
+            }
+    }
+    auto order = [](const std::vector<uint64_t>& h, const std::vector<uint32_t>& cnt) {
+        const uint32_t n = uint32_t(h.size());
+        std::unordered_map<uint64_t, std::pair<uint32_t, uint32_t>> cls;   // hash -> (size, first index)
+        for (uint32_t i = 0; i < n; ++i) {
+            auto it = cls.find(h[i]);
+            if (it == cls.end()) cls.emplace(h[i], std::make_pair(1u, i));
+            else ++it->second.first;
+        }
+        std::vector<uint32_t> perm(n);
+        for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+        std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
+            const bool za = cnt[a] == 0, zb = cnt[b] == 0;
+            if (za != zb) return zb;
+            const auto& ca = cls.at(h[a]);
+            const auto& cb = cls.at(h[b]);
+            if (ca.first != cb.first) return ca.first > cb.first;
+            return ca.second < cb.second;
+        });
+        return perm;   // new position -> old position
+    };
+    const std::vector<uint32_t> pk = order(hk, ck), pn = order(hn, cn);
+    std::vector<uint32_t> ik(Kg), in(Ng);
+    for (uint32_t i = 0; i < Kg; ++i) ik[pk[i]] = i;
+    for (uint32_t i = 0; i < Ng; ++i) in[pn[i]] = i;
+    std::vector<double> old(Wg, Wg + uint64_t(Kg) * Ng);
+    for (uint32_t n = 0; n < Ng; ++n) {
+        const double* src = old.data() + uint64_t(pn[n]) * Kg;
+        double* dst = Wg + uint64_t(n) * Kg;
+        for (uint32_t k = 0; k < Kg; ++k) dst[k] = src[pk[k]];
+    }
+    std::vector<uint32_t> K2(Kg), N2(Ng);
+    for (uint32_t i = 0; i < Kg; ++i) K2[i] = g.K[pk[i]], g.kpos[K2[i]] = int32_t(i);
+    for (uint32_t i = 0; i < Ng; ++i) N2[i] = g.N[pn[i]], g.npos[N2[i]] = int32_t(i);
+    g.K.swap(K2);
+    g.N.swap(N2);
+    for (uint32_t pi : g.pairs) {
+        for (auto& k : G.pairs[pi].kk) k = ik[k];
+        for (auto& n : G.pairs[pi].nn) n = in[n];
+    }
+}
+
+// Block-sparse S-stage tasks of one group (W_D at w_off, K x N, after reorder_group_keys): the
+// columns are cut into panels of S_TILE_N, the rows into tiles of S_TILE_K; each panel's segments
+// are the maximal runs of tiles holding a nonzero (a panel without any gets one tile, which
+// multiplies zeros), and one task per S_TILE_M-row panel of T_D and column panel shares them, in
+// row-panel-major order (the workgroups that run together read one T_D row panel).  Returns the
+// MFMA flops of the tasks, or 0 (no tasks added) when they would do more than `max_frac` of the
+// dense product's.
+constexpr uint32_t S_TILE_K = 16, S_TILE_N = 128, S_TILE_M = 128;
+uint64_t add_sparse_s_tasks(StagePlan& P, const GroupedLowering::Group& g, const double* Wg, double max_frac) {
+    const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
+    const uint32_t nkt = (Kg + S_TILE_K - 1) / S_TILE_K, npn = (Ng + S_TILE_N - 1) / S_TILE_N;
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> runs(npn);   // (k0, K) per panel
+    uint64_t visited = 0;
+    for (uint32_t j = 0; j < npn; ++j) {
+        const uint32_t n0 = j * S_TILE_N, n1 = std::min(Ng, n0 + S_TILE_N);
+        std::vector<char> nz(nkt, 0);
+        for (uint32_t n = n0; n < n1; ++n) {
+            const double* col = Wg + uint64_t(n) * Kg;
+            for (uint32_t k = 0; k < Kg; ++k)
+                if (col[k] != 0.0) nz[k / S_TILE_K] = 1;
+        }
+        bool any = false;
+        for (uint32_t t = 0; t < nkt;) {
+            if (!nz[t]) { ++t; continue; }
+            uint32_t t1 = t;
+            while (t1 < nkt && nz[t1]) ++t1;
+            const uint32_t k0 = t * S_TILE_K, k1 = std::min(Kg, t1 * S_TILE_K);
+            runs[j].push_back({k0, k1 - k0});
+            visited += uint64_t(k1 - k0) * (n1 - n0);
+            any = true;
+            t = t1;
+        }
+        if (!any) {
+            runs[j].push_back({0u, std::min(Kg, S_TILE_K)});
+            visited += uint64_t(std::min(Kg, S_TILE_K)) * (n1 - n0);
+        }
+    }
+    if (double(visited) > max_frac * double(Kg) * double(Ng)) return 0;
+    std::vector<std::pair<uint32_t, uint32_t>> seg_range(npn);
+    for (uint32_t j = 0; j < npn; ++j) {
+        const uint32_t n0 = j * S_TILE_N;
+        seg_range[j].first = uint32_t(P.s_segs.size());
+        for (const auto& r : runs[j])
+            P.s_segs.push_back(seg(BASE_WS, g.t_off + uint64_t(r.first) * g.M, uint32_t(g.M), false, BASE_WS,
+                                   g.w_off + uint64_t(n0) * Kg + r.first, Kg, false, r.second));
+        seg_range[j].second = uint32_t(P.s_segs.size());
+    }
+    const uint32_t M = uint32_t(g.M);
+    for (uint32_t m0 = 0; m0 < M; m0 += S_TILE_M) {
+        const uint32_t mr = std::min(S_TILE_M, M - m0);
+        for (uint32_t j = 0; j < npn; ++j) {
+            const uint32_t n0 = j * S_TILE_N, nw = std::min(Ng, n0 + S_TILE_N) - n0;
+            add_gemm_range(P.s_tasks, BASE_WS, g.s_off + uint64_t(n0) * g.M, mr, nw, M, seg_range[j].first,
+                           seg_range[j].second, m0);
+        }
+    }
+    return 2ull * g.M * visited;
+}
+
 // The dense W_D go to p->ctx's pinned staging buffer (wd_elems of them, at ws_init_off of the
 // workspace), which the caller uploads and synchronizes before the next lowering.
 void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint64_t& wd_elems,
@@ -1936,6 +2081,17 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     MQ_REQUIRE(bad != 1, "operator charge not in physical index");
     MQ_REQUIRE(bad != 2, "operator element outside its physical sector");
     trace_phase("lower: W_D");
+    // ---- key order for the block-sparse S stage: within each group, keys whose rows (K) or
+    // columns (N) of W_D have the same nonzero pattern are made adjacent, larger classes first, so
+    // that W_D's nonzeros form dense rectangles and few of its 16-row tiles are empty and visited.
+    // Only positions change: every later consumer reads them through kpos / npos and the pairs'
+    // kk / nn lists.  Adjoint plans keep the ascending order their term map was built in.
+    if (!adjoint) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int gi = 0; gi < int(G.groups.size()); ++gi)
+            reorder_group_keys(G, G.groups[gi], wd + (G.groups[gi].w_off - ws_init_off));
+        trace_phase("lower: key classes");
+    }
     StagePlan& P = p->st;
     if (!lstep) {
         // ---- T stage.  The psi blocks that meet one right cohort (every (lb_in, s, ss) with
@@ -2126,6 +2282,9 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         MQ_REQUIRE(3 * Kg < (1ull << 31), "group too large for one library GEMM");
         P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(Ng), uint32_t(Kg), uint32_t(g.M),
                          uint32_t(Kg), uint32_t(g.M), g.bf_off, 0});
+        // FP64 / FP32: block-sparse MFMA tasks where they visit at most half of the dense product
+        if (!adjoint && s_sparse_enabled() && g.M >= S_TILE_M)
+            P.lib.back().sparse = add_sparse_s_tasks(P, g, wd + (g.w_off - ws_init_off), 0.5) > 0;
     }
     for (auto& L : P.lib) {   // B' regions after the A' regions
         L.bfb_off = P.bf_elems;
@@ -2733,7 +2892,7 @@ int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
         if (library_only) {   // only GEMMs whose algorithm is not cached need operands and timing
             bool any = false;
             for (size_t i = 0; i < P.lib.size() && !any; ++i)
-                any = P.lt_prepare(precision == MQ_PREC_F32 ? 0 : 1, i, st) &&
+                any = !(precision == MQ_PREC_F32 && P.lib[i].sparse) && P.lt_prepare(precision == MQ_PREC_F32 ? 0 : 1, i, st) &&
                       !P.ltg[precision == MQ_PREC_F32 ? 0 : 1][i].chosen;
             if (!any) {
                 if (timed) *timed = 0;
@@ -2914,7 +3073,11 @@ int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, int 
             ft = gemm_flops(P.t_tasks, P.t_segs);
             ff = gemm_flops(P.f_tasks, P.f_segs);
             P.for_regions([&](const GatherRegion& r) { fg += 2ull * r.R * r.C * (r.t1 - r.t0); });
-            for (const auto& g : P.lib) fg += 2ull * g.M * g.N * g.K;
+            // S stage: the dense library GEMMs, and in FP64 / FP32 the block-sparse tasks in their place
+            const bool sparse_run = precision != MQ_PREC_BF16X3;
+            for (const auto& g : P.lib)
+                if (!(g.sparse && sparse_run)) fg += 2ull * g.M * g.N * g.K;
+            if (sparse_run) fg += gemm_flops(P.s_tasks, P.s_segs);
             flops_out[0] = ft;
             flops_out[1] = fg;
             flops_out[2] = ff;
@@ -2949,7 +3112,7 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
     return guarded([&] {
         MQ_REQUIRE(p && bytes, "bad argument");
         const StagePlan& P = p->st;
-        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gF.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_rows.bytes + P.d_regions.bytes +
+        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gF.device_bytes() + P.gS.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_rows.bytes + P.d_regions.bytes +
                  P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
     });
 }
@@ -3161,10 +3324,16 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
         HIPCHK(launch_fill(ctx->stream, nb->ptr(), std::max<uint64_t>(nb->total, 1), 0.0));
         // The step's intermediates (T_D, S_D) are several times the new boundary.  The ket sectors
         // are processed in chunks whose workspace fits a budget: 40% of the free device memory
-        // (at least 1 GiB); a chunk that does not fit is halved.
+        // (at least 1 GiB); a chunk that does not fit is halved.  MQ_STEP_BUDGET_ELEMS overrides the
+        // budget (FP64 elements, no floor): a budget of 1 runs every ket sector as its own chunk.
         size_t free_b = 0, total_b = 0;
         HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        const uint64_t budget = std::max<uint64_t>(uint64_t(1) << 27, uint64_t(0.4 * double(free_b + ctx->step_ws.bytes)) / 8);
+        uint64_t budget = std::max<uint64_t>(uint64_t(1) << 27, uint64_t(0.4 * double(free_b + ctx->step_ws.bytes)) / 8);
+        if (const char* env = std::getenv("MQ_STEP_BUDGET_ELEMS")) {
+            const unsigned long long v = std::strtoull(env, nullptr, 10);
+            MQ_REQUIRE(v > 0, "MQ_STEP_BUDGET_ELEMS must be a positive element count");
+            budget = uint64_t(v);
+        }
         const uint32_t n_ket = kind == Schedule::LEFT ? L.right_i.size() : L.left_i.size();
         std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, n_ket}};
         while (!todo.empty()) {

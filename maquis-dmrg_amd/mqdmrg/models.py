@@ -402,7 +402,8 @@ def chain(model, L, **kw):
             ecore = kw.get("ecore", 0.0)
         N = kw.get("N", L)
         basis, mpo, _ = su2.qc_su2_mpo(h, eri, N, ecore)
-        return basis, mpo, (N, kw.get("two_s", 0))
+        # lowest total spin the electron count allows (singlet for even N, doublet for odd N)
+        return basis, mpo, (N, kw.get("two_s", N % 2))
     raise ValueError(model)
 
 

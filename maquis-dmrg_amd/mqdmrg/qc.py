@@ -110,14 +110,17 @@ def fermion_site():
 
 
 # ---------------------------------------------------------------------------- terms
-def qc_terms(h, eri, tol=1e-14, with_source=False):
+def qc_terms(h, eri, tol=1e-14, with_source=False, h_tol=None):
     """Normal-ordered operator strings of H: list of (coeff, ((orb, spin, dagger), ...)).
     with_source: each entry also carries (kind, indices, factor) with coeff = factor * integral,
-    kind "h" (p, q) or "eri" (p, q, r, s)."""
+    kind "h" (p, q) or "eri" (p, q, r, s).  h_tol (default tol): threshold of the one-body entries;
+    a negative value keeps every h_pq, zeros included, so that the MPO's sources cover the whole
+    one-body density matrix dE/dh."""
     L = h.shape[0]
     terms = []
+    h_tol = tol if h_tol is None else h_tol
     for p, q in itertools.product(range(L), repeat=2):
-        if abs(h[p, q]) > tol:
+        if abs(h[p, q]) > h_tol:
             for s in range(2):
                 t = (h[p, q], ((p, s, 1), (q, s, 0)))
                 terms.append(t + (("h", (p, q), 1.0),) if with_source else t)
@@ -162,7 +165,7 @@ def _site_order(ops):
     return sign, ops
 
 
-def qc_mpo(h, eri, ecore=0.0, tol=1e-14, sources=False):
+def qc_mpo(h, eri, ecore=0.0, tol=1e-14, sources=False, h_tol=None):
     """MPO of the ab initio Hamiltonian in the TwoU1 site basis (see module docstring).
 
     sources=True: also return, per site j, {(b1, b2, op_id): [(kind, indices, factor), ...]}: the
@@ -192,7 +195,7 @@ def qc_mpo(h, eri, ecore=0.0, tol=1e-14, sources=False):
         return local_cache[k]
 
     key_src = {}
-    for item in qc_terms(h, eri, tol, with_source=sources):
+    for item in qc_terms(h, eri, tol, with_source=sources, h_tol=h_tol):
         coef, ops = item[0], item[1]
         sign, sops = _site_order(ops)
         loc = []

@@ -39,22 +39,10 @@ def _sector_states(phys, L, target):
     return np.array(sel)
 
 
-def test_mpo_integral_sources_give_fci_energy_gradient():
-    """Exact ground state of a 4-orbital ab initio chain (dense MPO): the derivatives with respect
-    to each MPO term scale (dense single-term chains), mapped through the integral sources of the
-    switch coefficients, equal central finite differences of the FCI energy with respect to every
-    h_pq and (pq|rs) entry (step 1e-5) to 1e-6 relative."""
-    L, nup, ndn = 4, 2, 2
-    h, eri = qc.synthetic_integrals(L, seed=3)
-    basis, mpo, _, src = qc.qc_mpo(h, eri, sources=True)
-    phys = basis.phys
-    H = models.mpo_dense_chain(mpo, phys)
-    sel = _sector_states(phys, L, (nup, ndn))
-    w, v = np.linalg.eigh(H[np.ix_(sel, sel)])
-    e0 = w[0]
-    assert abs(e0 - qc.fci_energy(h, eri, nup, ndn)) < 1e-10
-    psi = np.zeros(H.shape[0])
-    psi[sel] = v[:, 0]
+def _dense_term_gradients(mpo, phys, psi, e0):
+    """dE/d(scale) of every term of every site of a dense chain state psi (normalised, energy e0),
+    from dense left / right environments; checks the Euler identity per site."""
+    L = len(mpo)
     d = phys.total()
     Ws = [W.dense(phys) for W in mpo]
     # left[j]: (a, O, I) product of sites < j; right[j]: (b, O, I) product of sites > j
@@ -79,6 +67,26 @@ def test_mpo_integral_sources_give_fci_energy_gradient():
         # Euler: the Hamiltonian is linear in each site's coefficients
         assert abs(g @ scs - e0) < 1e-9
         grads.append(g)
+    return grads
+
+
+def test_mpo_integral_sources_give_fci_energy_gradient():
+    """Exact ground state of a 4-orbital ab initio chain (dense MPO): the derivatives with respect
+    to each MPO term scale (dense single-term chains), mapped through the integral sources of the
+    switch coefficients, equal central finite differences of the FCI energy with respect to every
+    h_pq and (pq|rs) entry (step 1e-5) to 1e-6 relative."""
+    L, nup, ndn = 4, 2, 2
+    h, eri = qc.synthetic_integrals(L, seed=3)
+    basis, mpo, _, src = qc.qc_mpo(h, eri, sources=True)
+    phys = basis.phys
+    H = models.mpo_dense_chain(mpo, phys)
+    sel = _sector_states(phys, L, (nup, ndn))
+    w, v = np.linalg.eigh(H[np.ix_(sel, sel)])
+    e0 = w[0]
+    assert abs(e0 - qc.fci_energy(h, eri, nup, ndn)) < 1e-10
+    psi = np.zeros(H.shape[0])
+    psi[sel] = v[:, 0]
+    grads = _dense_term_gradients(mpo, phys, psi, e0)
     dh, deri, _ = qc.mpo_gradient_to_integrals(mpo, src, grads, L)
     fd = qc.fci_energy_gradient_fd(h, eri, nup, ndn)
     fh = np.array([[fd[("h", (p, q))] for q in range(L)] for p in range(L)])
@@ -89,3 +97,27 @@ def test_mpo_integral_sources_give_fci_energy_gradient():
     assert np.linalg.norm(deri - fe) <= 1e-6 * np.linalg.norm(fe)
     # spin-summed 1-RDM trace = N
     assert abs(np.trace(dh) - (nup + ndn)) < 1e-8
+
+
+def test_hubbard_ladder_onebody_matrix_from_term_gradients():
+    """Hubbard ladder of 2 rungs (U=8, 2 up + 2 down electrons) as ab initio integrals with every
+    one-body entry a term (h_tol < 0, zeros included): the term derivatives of the exact ground
+    state, mapped to dE/dh_pq, equal the spin-summed one-body density matrix <c+_p c_q> of a direct
+    diagonalisation (the quantity the reference's one-body density matrix measurement reports)."""
+    from lattices import ladder_hopping, hubbard_integrals, exact_hubbard
+    hop = ladder_hopping(2)
+    h, eri = hubbard_integrals(hop, 8.0)
+    L, nup, ndn = 4, 2, 2
+    basis, mpo, _, src = qc.qc_mpo(h, eri, sources=True, h_tol=-1.0)
+    phys = basis.phys
+    H = models.mpo_dense_chain(mpo, phys)
+    sel = _sector_states(phys, L, (nup, ndn))
+    w, v = np.linalg.eigh(H[np.ix_(sel, sel)])
+    e_ex, gu, gd = exact_hubbard(hop, 8.0, nup, ndn)
+    assert abs(w[0] - e_ex) < 1e-10
+    assert w[1] - w[0] > 1e-6   # a unique ground state: its density matrix is well defined
+    psi = np.zeros(H.shape[0])
+    psi[sel] = v[:, 0]
+    grads = _dense_term_gradients(mpo, phys, psi, w[0])
+    dh, _, _ = qc.mpo_gradient_to_integrals(mpo, src, grads, L)
+    np.testing.assert_allclose(dh, gu + gd, atol=1e-10)

@@ -151,12 +151,14 @@ def test_autotuned_plan_matches_oracle(eng):
 
 
 @pytest.mark.parametrize("prec", [E.PREC_F32, E.PREC_BF16X3])
-def test_library_solutions_adopted_by_shape_class(prec):
+def test_library_solutions_adopted_by_shape_class(prec, monkeypatch):
     """A library GEMM solution tuned on one plan is adopted, through the shape-class cache, by the
     GEMMs of a plan of nearby shapes (the same chain at a slightly larger bond dimension) when the
     library supports them: that plan's library-only autotune times fewer candidates than the first
-    plan's, and its sigma still matches the oracle to the precision's tolerance."""
+    plan's, and its sigma still matches the oracle to the precision's tolerance.  The plans run
+    every S stage on the library (MQ_S_DENSE=1), so FP32 has library GEMMs to tune."""
     from mqdmrg.workloads import SyntheticSite
+    monkeypatch.setenv("MQ_S_DENSE", "1")
     e = E.Engine(0)
     try:
         counts = []
@@ -278,6 +280,40 @@ def test_grouped_steps_match_schedule_steps_at_synthetic_shapes(eng, model, L, m
         _cmp_boundaries(O.HostBoundary(a.index, a.download()), b)
 
 
+@pytest.mark.parametrize("model,site", [("qc", 3), ("hubbard", 3)])
+def test_grouped_steps_in_many_chunks_match_oracle(eng, model, site, monkeypatch):
+    """A workspace budget of one element (MQ_STEP_BUDGET_ELEMS=1) runs every ket sector of a grouped
+    step as its own chunk, reusing the context's step workspace across chunks without clearing it;
+    the new boundaries equal the one-chunk steps to 1e-14 and the oracle to 1e-12."""
+    print("case", model, site, flush=True)
+    p = SiteProblem(model, L=8, site=site, Mmax=8, seed=21)
+    k = site + 1
+    lay = p.layouts[k]
+    sd = E.SiteDesc(p.sym, lay.left, lay.phys, lay.right)
+    md = E.MpoDesc(p.mpo[k])
+    v = dev(p.vecs[k])
+    Rb = upload(eng, p.Rb)
+    one_r = E.right_step(eng, sd, md, v, v, Rb, True, algorithm=E.SIGMA_GROUPED)
+    kl = site
+    layl = p.layouts[kl]
+    sdl = E.SiteDesc(p.sym, layl.left, layl.phys, layl.right)
+    mdl = E.MpoDesc(p.mpo[kl])
+    bra_lp = mps.rp_to_lp(layl, p.vecs[kl])
+    Lb = upload(eng, p.Lb)
+    one_l = E.left_step(eng, sdl, mdl, dev(bra_lp), dev(p.vecs[kl]), Lb, True, algorithm=E.SIGMA_GROUPED)
+    assert len(lay.left) > 1 and len(layl.right) > 1   # several chunks
+    monkeypatch.setenv("MQ_STEP_BUDGET_ELEMS", "1")
+    many_r = E.right_step(eng, sd, md, v, v, Rb, True, algorithm=E.SIGMA_GROUPED)
+    many_l = E.left_step(eng, sdl, mdl, dev(bra_lp), dev(p.vecs[kl]), Lb, True, algorithm=E.SIGMA_GROUPED)
+    monkeypatch.delenv("MQ_STEP_BUDGET_ELEMS")
+    _cmp_boundaries(O.right_step(sd, md, p.vecs[k], p.vecs[k], p.Rb, True), many_r)
+    _cmp_boundaries(O.left_step(sdl, mdl, bra_lp, p.vecs[kl], p.Lb, True), many_l)
+    for a, b in ((one_r, many_r), (one_l, many_l)):
+        np.testing.assert_array_equal(a.index.offsets, b.index.offsets)
+        for x, y in zip(a.download(), b.download()):
+            assert np.abs(x - y).max(initial=0.0) <= 1e-14 * max(1.0, np.abs(x).max(initial=0.0))
+
+
 def test_reshape_roundtrip(eng):
     p = SiteProblem("hubbard", L=6, site=2, Mmax=6, seed=13)
     lay = p.layouts[2]
@@ -321,6 +357,61 @@ def test_jacobi_davidson_matches_oracle(eng, model):
     assert min(np.linalg.norm(v - v_ref), np.linalg.norm(v + v_ref)) <= 1e-8
 
 
+
+
+@pytest.mark.parametrize("model,L,m", [("qc_su2", 16, 256), ("qc", 14, 192)])
+def test_block_sparse_s_stage_matches_dense_and_oracle(eng, model, L, m, monkeypatch):
+    """Ab initio middle-of-chain shapes: the block-sparse S stage (keys in pattern-class order, MFMA
+    tasks over the nonzero 16-row tiles of W_D) does fewer flops than the dense library GEMMs of the
+    same plan built with MQ_S_DENSE=1, and its sigma equals the dense plan's to 1e-13 (FP64) and
+    2e-6 (FP32) and the oracle's on a block sample to 1e-12."""
+    from mqdmrg.workloads import SyntheticSite
+    print("case", model, L, m, flush=True)
+    ws = SyntheticSite(model, L, L // 2 - 1, m, seed=9)
+    Lc, Rc = ws.random_cohorts(ws.left_index), ws.random_cohorts(ws.right_index)
+    Lb = E.DeviceBoundary.upload(eng, ws.left_index, Lc)
+    Rb = E.DeviceBoundary.upload(eng, ws.right_index, Rc)
+    psi_h = ws.random_psi()
+    psi = dev(psi_h)
+    sparse = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    monkeypatch.setenv("MQ_S_DENSE", "1")
+    dense = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    monkeypatch.delenv("MQ_S_DENSE")
+    tmp = torch.zeros_like(psi)
+    _, fl_sparse, _ = sparse.profile(psi, tmp, reps=1)
+    _, fl_dense, _ = dense.profile(psi, tmp, reps=1)
+    print("  S-stage flops sparse %.3e dense %.3e" % (fl_sparse[1], fl_dense[1]), flush=True)
+    assert fl_sparse[1] < 0.5 * fl_dense[1]
+    outs = {}
+    for name, plan in (("sparse", sparse), ("dense", dense)):
+        for prec in (E.PREC_F64, E.PREC_F32):
+            out = torch.full_like(psi, np.nan)
+            plan.apply(psi, out, precision=prec)
+            eng.sync()
+            outs[name, prec] = out.cpu().numpy()
+    ref = outs["dense", E.PREC_F64]
+    nr = np.linalg.norm(ref)
+    e64 = np.linalg.norm(outs["sparse", E.PREC_F64] - ref) / nr
+    e32 = np.linalg.norm(outs["sparse", E.PREC_F32] - ref) / nr
+    print("  sparse vs dense: FP64 %.2e FP32 %.2e" % (e64, e32), flush=True)
+    assert e64 <= 1e-13 and e32 <= 2e-6
+    lay = ws.layout
+    nblk = len(lay.left.sizes)
+    blocks = sorted(np.random.default_rng(0).choice(nblk, size=min(6, nblk), replace=False).tolist())
+    hl = O.HostBoundary(ws.left_index, [c if c is not None else np.zeros(0) for c in Lc])
+    hr = O.HostBoundary(ws.right_index, [c if c is not None else np.zeros(0) for c in Rc])
+    # the oracle's sigma of the sampled ket blocks alone against the GPU sigma of psi restricted to them
+    oref = O.SigmaOracle(ws.site_desc, ws.mpo_desc, hl, hr, only_blocks=blocks).apply(psi_h)
+    masked = np.zeros_like(psi_h)
+    for b in blocks:
+        o, n = lay.rp_offset[b], lay.left.sizes[b] * lay.rp_cols[b]
+        masked[o:o + n] = psi_h[o:o + n]
+    out = torch.full_like(psi, np.nan)
+    sparse.apply(dev(masked), out)
+    eng.sync()
+    eo = np.linalg.norm(out.cpu().numpy() - oref) / np.linalg.norm(oref)
+    print("  sparse vs oracle, ket blocks", blocks, "%.2e" % eo, flush=True)
+    assert eo <= 1e-12
 
 
 @pytest.mark.parametrize("precision", [E.PREC_F32, E.PREC_BF16X3])
