@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--workload", default="qc_su2-L30-m2048", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="auto", choices=["auto", "f64", "f32", "bf16x3"],
                     help="precision reported as value (auto: fastest reduced precision within the sigma bar)")
+    ap.add_argument("--only", default=None, choices=["f64", "f32", "bf16x3"],
+                    help="time this precision alone (profiling runs; it is then also the reported one)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
@@ -286,7 +288,10 @@ def main():
     # ---- timed sigma applications per precision
     per = {}
     ref = None
-    for name, prec in (("f64", E.PREC_F64), ("f32", E.PREC_F32), ("bf16x3", E.PREC_BF16X3)):
+    precs = (("f64", E.PREC_F64), ("f32", E.PREC_F32), ("bf16x3", E.PREC_BF16X3))
+    if args.only:
+        precs = [x for x in precs if x[0] == args.only]
+    for name, prec in precs:
         tune = None
         if not args.no_autotune:
             # tile configurations of the T and final stages: stage times before and after tuning
@@ -312,9 +317,9 @@ def main():
         wall = time.perf_counter() - t0
         ms = ev0.elapsed_time(ev1) / args.steps
         out = sigma.clone()
-        if ref is None:
+        if name == "f64":
             ref = out
-        err = float(torch.linalg.norm(out - ref) / torch.linalg.norm(ref))
+        err = float(torch.linalg.norm(out - ref) / torch.linalg.norm(ref)) if ref is not None else None
         stage_ms, stage_flops, _ = plan.profile(psi, sigma, reps=max(2, min(args.steps, 5)), precision=prec)
         per[name] = {"ms_per_sigma": ms, "tflops": plan.flops / (ms * 1e-3) / 1e12, "sigma_rel_err_vs_f64": err,
                      "wall_seconds_timed": round(wall, 4),
@@ -323,7 +328,9 @@ def main():
         log("%s: %.3f ms/sigma, %.2f TFLOP/s, rel err %.2e, stages %s" % (name, ms, per[name]["tflops"], err,
                                                                          per[name]["stage_ms"]))
 
-    if args.precision == "auto":
+    if args.only:
+        report = args.only
+    elif args.precision == "auto":
         ok = [p for p in ("f32", "bf16x3") if per[p]["sigma_rel_err_vs_f64"] <= MIXED_RTOL]
         report = min(ok, key=lambda p: per[p]["ms_per_sigma"]) if ok else "f64"
     else:
@@ -357,16 +364,23 @@ def main():
                  "runs": runs, "abs_delta_e_Eh": abs(runs[report]["energy_Eh"] - runs["f64"]["energy_Eh"]),
                  "speedup_vs_f64": runs["f64"]["seconds"] / runs[report]["seconds"]}
 
-    # ---- roofline of the dominant kernel in the reported precision
+    # ---- roofline of the dominant kernel in the reported precision.  achieved = the stage's flops in
+    # the reference's model (solver/tasks.cpp:388-398, 777-796) per second: for the T and final stages
+    # these are the block products the kernel executes; for the S stage the model counts one alpha
+    # per nonzero MPO coupling, i.e. the plan's model flops minus those of the T and final stages
+    # (the executed MFMA flops, which include the zeros inside the visited W_D tiles, are reported
+    # beside it).
     pr = per[report]
     st = pr["stage_ms"]
     dom = max(["gemm_T", "gemm_S", "gemm_final"], key=lambda k: st[k])
-    fl = pr["stage_gemm_flops"][{"gemm_T": "T", "gemm_S": "S", "gemm_final": "final"}[dom]]
+    sf = pr["stage_gemm_flops"]
+    executed = sf[{"gemm_T": "T", "gemm_S": "S", "gemm_final": "final"}[dom]]
+    fl = max(0, plan.flops - sf["T"] - sf["final"]) if dom == "gemm_S" else executed
     achieved = fl / (st[dom] * 1e-3) / 1e12
     if dom == "gemm_S":
-        # the S stage is one dense library GEMM per charge-transfer group, in the reported precision
-        kernel = {"f64": "rocBLAS dgemm", "f32": "hipBLASLt FP32 GEMM",
-                  "bf16x3": "hipBLASLt bf16 GEMM over the split operands, K' = 3K per group"}[report] + " (S stage)"
+        kernel = {"f64": "mq::gemm_kernel, block-sparse S stage over the nonzero 16-row tiles of W_D (FP64 MFMA)",
+                  "f32": "mq::gemm_kernel, block-sparse S stage over the nonzero 16-row tiles of W_D (FP32 MFMA)",
+                  "bf16x3": "hipBLASLt bf16 GEMM over the split operands, K' = 3K per group (S stage)"}[report]
         peak_key = report
     else:
         # T and final stages: the hand-written MFMA GEMM, FP32 in the reduced precisions
@@ -374,7 +388,8 @@ def main():
         peak_key = "f64" if report == "f64" else "f32"
     roof = {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[peak_key],
             "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[peak_key], "traffic": None, "stage_ms": st,
-            "flops_per_launch_set": fl}
+            "flops_per_launch_set": fl, "flops_model": "reference (alpha per nonzero MPO coupling)",
+            "executed_mfma_flops": executed, "executed_tflops": executed / (st[dom] * 1e-3) / 1e12}
     # HBM bytes of the same launch set, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
     # this workload (profiles/), when they cover the reported stage and precision
     tf = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (args.workload.replace("-", "_"), report))
@@ -483,7 +498,7 @@ def main():
             "reported_precision": report,
         },
         "precisions": per,
-        "speedup_vs_f64": per["f64"]["ms_per_sigma"] / pr["ms_per_sigma"],
+        "speedup_vs_f64": per["f64"]["ms_per_sigma"] / pr["ms_per_sigma"] if "f64" in per else None,
         "sweeps": sweep,
         "headline_sweep": hsweep,
         "batch": batch,

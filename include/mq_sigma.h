@@ -358,9 +358,21 @@ int mq_vec_multi_dot(mq_ctx* ctx, uint64_t n, int k, const double* V, uint64_t l
 
 /* Jacobi-Davidson smallest eigenpair of H_eff (ietl/jacobi.h:386-516, solver/ietl_jacobi_davidson.h:35-76).
  * init and out are device vectors in the plan's layout (may alias).  gmres_iter == 0 uses the
- * plain correction t = -r + (r.u)/(u.u) u (ietl/jacobi.h:150-156). */
+ * plain correction t = -r + (r.u)/(u.u) u (ietl/jacobi.h:150-156); gmres_iter > 0 refines it with
+ * GMRES (at most gmres_iter Krylov vectors, ietl/gmres.h:78-140) on the projected correction
+ * equation (1 - uu^T)(H - theta)(1 - uu^T) t = -r to the tolerance 2^-(k+1) at step k
+ * (jcd_gmres_solver, ietl/jacobi.h:126-164; the reference's parameter ietl_jcd_gmres). */
 int mq_jd_solve(mq_plan* plan, const double* init, double* out, double tol, int max_iter,
                 int gmres_iter, double* energy, int* iterations);
+/* mq_jd_solve with ortho vectors: the search space is kept orthogonal to the n_ortho device vectors
+ * ortho[0..n_ortho) (plan layout), as the reference's solve(..., ortho_vecs, ...)
+ * (solver/solver.h:65-69, solver/ietl_jacobi_davidson.h:36-47) with DavidsonVS::project
+ * (solver/ietl_interface_dv.h:94-99): the vectors are Gram-Schmidt'ed among themselves (internal
+ * copies; the caller's are not modified), then projected out of the start vector, of every new
+ * search vector and of H u before the residual.  Finds the lowest eigenpair in their complement
+ * (excited states).  At most vec_size - 1 of them are used. */
+int mq_jd_solve_ex(mq_plan* plan, const double* init, double* out, double tol, int max_iter,
+                   int gmres_iter, int n_ortho, const double* const* ortho, double* energy, int* iterations);
 /* Jacobi-Davidson as iterative refinement over a reduced-precision sigma.  Each outer step runs
  * the Jacobi-Davidson iteration above with sigma in `precision` (the first search vector of an
  * outer step after the first carries its exact FP64 product), then recomputes the Ritz vector's

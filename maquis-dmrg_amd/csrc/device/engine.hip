@@ -1767,8 +1767,10 @@ void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* W
 // MFMA flops of the tasks, or 0 (no tasks added) when they would do more than `max_frac` of the
 // dense product's.
 constexpr uint32_t S_TILE_K = 16, S_TILE_N = 128, S_TILE_M = 128;
-uint64_t add_sparse_s_tasks(StagePlan& P, const GroupedLowering::Group& g, const double* Wg, double max_frac) {
+uint64_t add_sparse_s_tasks(StagePlan& P, const GroupedLowering::Group& g, const double* Wg, double max_frac,
+                            double* visited_frac = nullptr, uint64_t* nnz_out = nullptr) {
     const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
+    uint64_t nnz = 0;
     const uint32_t nkt = (Kg + S_TILE_K - 1) / S_TILE_K, npn = (Ng + S_TILE_N - 1) / S_TILE_N;
     std::vector<std::vector<std::pair<uint32_t, uint32_t>>> runs(npn);   // (k0, K) per panel
     uint64_t visited = 0;
@@ -1778,7 +1780,7 @@ uint64_t add_sparse_s_tasks(StagePlan& P, const GroupedLowering::Group& g, const
         for (uint32_t n = n0; n < n1; ++n) {
             const double* col = Wg + uint64_t(n) * Kg;
             for (uint32_t k = 0; k < Kg; ++k)
-                if (col[k] != 0.0) nz[k / S_TILE_K] = 1;
+                if (col[k] != 0.0) nz[k / S_TILE_K] = 1, ++nnz;
         }
         bool any = false;
         for (uint32_t t = 0; t < nkt;) {
@@ -1796,6 +1798,8 @@ uint64_t add_sparse_s_tasks(StagePlan& P, const GroupedLowering::Group& g, const
             visited += uint64_t(std::min(Kg, S_TILE_K)) * (n1 - n0);
         }
     }
+    if (visited_frac) *visited_frac = double(visited) / (double(Kg) * double(Ng));
+    if (nnz_out) *nnz_out = nnz;
     if (double(visited) > max_frac * double(Kg) * double(Ng)) return 0;
     std::vector<std::pair<uint32_t, uint32_t>> seg_range(npn);
     for (uint32_t j = 0; j < npn; ++j) {
@@ -2283,8 +2287,15 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(Ng), uint32_t(Kg), uint32_t(g.M),
                          uint32_t(Kg), uint32_t(g.M), g.bf_off, 0});
         // FP64 / FP32: block-sparse MFMA tasks where they visit at most half of the dense product
-        if (!adjoint && s_sparse_enabled() && g.M >= S_TILE_M)
-            P.lib.back().sparse = add_sparse_s_tasks(P, g, wd + (g.w_off - ws_init_off), 0.5) > 0;
+        if (!adjoint && s_sparse_enabled() && g.M >= S_TILE_M) {
+            double vf = 0;
+            uint64_t nnz = 0;
+            P.lib.back().sparse = add_sparse_s_tasks(P, g, wd + (g.w_off - ws_init_off), 0.5, &vf, &nnz) > 0;
+            if (g_trace)
+                std::fprintf(stderr, "[mq] S group M %llu K %zu N %zu: W_D density %.3f, visited tiles %.3f -> %s\n",
+                             (unsigned long long)g.M, g.K.size(), g.N.size(), double(nnz) / double(Kg * Ng), vf,
+                             P.lib.back().sparse ? "block-sparse" : "dense");
+        }
     }
     for (auto& L : P.lib) {   // B' regions after the A' regions
         L.bfb_off = P.bf_elems;
@@ -3559,11 +3570,15 @@ struct JdSpace {
     mq_ctx* ctx;
     uint64_t n, stride;
     int max_iter;
+    int gmres_iter = 0;   // Krylov vectors of the GMRES correction (0: the plain correction)
+    int n_ortho = 0;      // vectors the search space is kept orthogonal to
     double* base;
     std::vector<double> hres;   // host copy of the last batched reduction
-    JdSpace(mq_plan* p_, int max_iter_) : p(p_), ctx(p_->ctx), n(p_->vec_size), stride(align32(p_->vec_size)),
-                                          max_iter(max_iter_) {
-        int cap = 2 * max_iter + 5;
+    JdSpace(mq_plan* p_, int max_iter_, int gmres_iter_ = 0, int n_ortho_ = 0)
+        : p(p_), ctx(p_->ctx), n(p_->vec_size), stride(align32(p_->vec_size)), max_iter(max_iter_),
+          gmres_iter(gmres_iter_), n_ortho(n_ortho_) {
+        // V, VA, u, uA, x, xA; GMRES: its basis, w and the operator's scratch; the ortho vectors
+        int cap = 2 * max_iter + 5 + (gmres_iter > 0 ? gmres_iter + 3 : 0) + n_ortho;
         if (p->jd_cap < cap) {
             p->jd_buf.alloc(uint64_t(cap) * stride * sizeof(double));
             p->jd_cap = cap;
@@ -3596,6 +3611,90 @@ struct JdSpace {
     double* uA() { return base + uint64_t(2 * max_iter + 2) * stride; }
     double* x() { return base + uint64_t(2 * max_iter + 3) * stride; }
     double* xA() { return base + uint64_t(2 * max_iter + 4) * stride; }
+    double* Gv(int i) { return base + uint64_t(2 * max_iter + 5 + i) * stride; }   // GMRES basis, i <= gmres_iter
+    double* Gw() { return Gv(gmres_iter + 1); }
+    double* Gt() { return Gv(gmres_iter + 2); }
+    double* O(int j) { return base + uint64_t(2 * max_iter + 5 + (gmres_iter > 0 ? gmres_iter + 3 : 0) + j) * stride; }
+    std::vector<double> o_norm2;   // o_j . o_j
+
+    // The ortho vectors, Gram-Schmidt'ed in order as the caller's copies are in
+    // solver/ietl_jacobi_davidson.h:41-47
+    void set_ortho(const double* const* vecs) {
+        o_norm2.assign(n_ortho, 0.0);
+        for (int j = 0; j < n_ortho; ++j) {
+            copy(vecs[j], O(j));
+            for (int j0 = 0; j0 < j; ++j0) axpy(-dot(O(j0), O(j)) / o_norm2[j0], O(j0), O(j));
+            o_norm2[j] = dot(O(j), O(j));
+            MQ_REQUIRE(o_norm2[j] > 0, "Jacobi-Davidson: an ortho vector is zero or dependent on the others");
+        }
+    }
+    // t -= (o.t)/(o.o) o for each ortho vector in turn (DavidsonVS::project, solver/ietl_interface_dv.h:94-99)
+    void project(double* t) {
+        for (int j = 0; j < n_ortho; ++j) axpy(-dot(O(j), t) / o_norm2[j], O(j), t);
+    }
+    // y = (1 - u u^T)(H - theta)(1 - u u^T) x  (jcd_solver_operator, ietl/jacobi.h:266-284)
+    void jcd_op(const double* x, double* y, double theta, int precision, int* n_sigma) {
+        double* t2 = Gt();
+        copy(x, t2);
+        axpy(-dot(u(), x), u(), t2);
+        sigma(t2, y, precision);
+        ++*n_sigma;
+        axpy(-theta, t2, y);
+        axpy(-dot(u(), y), u(), y);
+    }
+    // GMRES on the correction equation from the initial guess t (ietl_gmres, ietl/gmres.h:78-140,
+    // with its bookkeeping: at most gmres_iter - 1 Arnoldi steps, and on convergence at step i the
+    // update uses the first i basis vectors).  b = -r; the result overwrites t.
+    void gmres(double* t, const double* r, double theta, double abs_tol, int precision, int* n_sigma) {
+        const int m = gmres_iter;
+        std::vector<double> H(size_t(m + 1) * (m + 1), 0.0), sv(m + 1, 0.0), cs(m + 1, 0.0), sn(m + 1, 0.0);
+        auto h = [&](int i, int j) -> double& { return H[size_t(i) + size_t(j) * (m + 1)]; };
+        jcd_op(t, Gv(0), theta, precision, n_sigma);
+        // v0 = b - A t = -r - A t
+        scal(-1.0, Gv(0));
+        axpy(-1.0, r, Gv(0));
+        sv[0] = std::sqrt(dot(Gv(0), Gv(0)));
+        if (std::fabs(sv[0]) < abs_tol) return;
+        scal(1.0 / sv[0], Gv(0));
+        auto rot = [](double& dx, double& dy, double c, double sn_) {
+            const double r0 = c * dx + sn_ * dy, r1 = -sn_ * dx + c * dy;
+            dx = r0, dy = r1;
+        };
+        int i = 0;
+        for (; i < m - 1; ++i) {
+            jcd_op(Gv(i), Gw(), theta, precision, n_sigma);
+            for (int k = 0; k <= i; ++k) {
+                h(k, i) = dot(Gw(), Gv(k));
+                axpy(-h(k, i), Gv(k), Gw());
+            }
+            h(i + 1, i) = std::sqrt(dot(Gw(), Gw()));
+            copy(Gw(), Gv(i + 1));
+            scal(1.0 / h(i + 1, i), Gv(i + 1));
+            for (int k = 0; k < i; ++k) rot(h(k, i), h(k + 1, i), cs[k], sn[k]);
+            const double dx = h(i, i), dy = h(i + 1, i);
+            if (dy == 0.0) {
+                cs[i] = 1, sn[i] = 0;
+            } else if (std::fabs(dy) > std::fabs(dx)) {
+                const double tq = dx / dy;
+                sn[i] = 1.0 / std::sqrt(1.0 + tq * tq);
+                cs[i] = tq * sn[i];
+            } else {
+                const double tq = dy / dx;
+                cs[i] = 1.0 / std::sqrt(1.0 + tq * tq);
+                sn[i] = tq * cs[i];
+            }
+            rot(h(i, i), h(i + 1, i), cs[i], sn[i]);
+            rot(sv[i], sv[i + 1], cs[i], sn[i]);
+            if (std::fabs(sv[i + 1]) < abs_tol) break;
+        }
+        // back substitution over the first i columns, t += sum_k y_k v_k
+        std::vector<double> y(sv.begin(), sv.begin() + i);
+        for (int a = i - 1; a >= 0; --a) {
+            y[a] /= h(a, a);
+            for (int b = a - 1; b >= 0; --b) y[b] -= h(b, a) * y[a];
+        }
+        for (int k = 0; k < i; ++k) axpy(y[k], Gv(k), t);
+    }
     double dot(const double* a, const double* b) {
         double r;
         int rc = mq_vec_dot(ctx, n, a, b, &r);
@@ -3622,6 +3721,7 @@ struct JdSpace {
         std::vector<double> M(size_t(max_iter) * max_iter, 0.0), s;
         const double kappa = 0.25;
         copy(init, V(0));
+        project(V(0));
         int it = 0;
         double theta = 0, rn = 0;
         while (true) {
@@ -3641,6 +3741,10 @@ struct JdSpace {
                     subtract_last(V(0), it, t);
                     nt2 = mdot(V(0), it + 1, t)[it];
                 }
+            }
+            if (n_ortho) {   // project out the ortho space (ietl/jacobi.h:417-418)
+                project(t);
+                nt2 = dot(t, t);
             }
             double nt = std::sqrt(nt2);
             MQ_REQUIRE(nt > 0, "Jacobi-Davidson: zero search vector");
@@ -3663,6 +3767,7 @@ struct JdSpace {
             theta = smallest_eigenpair(M, max_iter, it + 1, s);
             combine(V(0), it + 1, s.data(), u());     // u = V s
             combine(VA(0), it + 1, s.data(), uA());   // uA = VA s
+            project(uA());
             axpy(-theta, u(), uA());   // r = uA - theta u
             ++it;
             // u and r are consecutive vectors: [u . r, r . r] in one reduction
@@ -3671,12 +3776,14 @@ struct JdSpace {
             rn = std::sqrt(ur[1]);
             bool conv = (rn <= std::max(tol * std::fabs(theta), stop_abs) || rn < tol);
             if (conv || it >= max_iter) break;
-            // t = -r + (r.u)/(u.u) u   (jcd_gmres_solver with max_iter 0, ietl/jacobi.h:141-157)
+            // t = -r + (r.u)/(u.u) u, then GMRES on (1 - uu^T)(H - theta)(1 - uu^T) t = -r to the
+            // tolerance 2^-(it+1) when gmres_iter > 0 (jcd_gmres_solver, ietl/jacobi.h:141-157, 466-468)
             const double duu = dot(u(), u());
             double* tn = V(it);
             copy(uA(), tn);
             scal(-1.0, tn);
             axpy(dru / duu, u(), tn);
+            if (gmres_iter > 0) gmres(tn, uA(), theta, 1.0 / std::pow(2.0, double(it + 1)), precision, n_sigma);
         }
         if (iters) *iters = it;
         if (rnorm) *rnorm = rn;
@@ -3690,10 +3797,20 @@ extern "C" {
 
 int mq_jd_solve(mq_plan* p, const double* init, double* out, double tol, int max_iter, int gmres_iter,
                 double* energy, int* iterations) {
+    return mq_jd_solve_ex(p, init, out, tol, max_iter, gmres_iter, 0, nullptr, energy, iterations);
+}
+
+int mq_jd_solve_ex(mq_plan* p, const double* init, double* out, double tol, int max_iter, int gmres_iter, int n_ortho,
+                   const double* const* ortho, double* energy, int* iterations) {
     return guarded([&] {
         MQ_REQUIRE(p && init && out && energy && max_iter > 0, "bad argument");
-        MQ_REQUIRE(gmres_iter == 0, "GMRES correction is not implemented; use gmres_iter = 0");
-        JdSpace J(p, max_iter);
+        MQ_REQUIRE(gmres_iter >= 0 && gmres_iter <= 64, "gmres_iter must lie in [0, 64]");
+        MQ_REQUIRE(n_ortho >= 0 && (n_ortho == 0 || ortho), "bad ortho vectors");
+        for (int j = 0; j < n_ortho; ++j) MQ_REQUIRE(ortho[j], "null ortho vector");
+        // as the reference, no more ortho vectors than the space can hold (ietl_jacobi_davidson.h:41-42)
+        if (uint64_t(n_ortho) >= p->vec_size) n_ortho = int(p->vec_size) - 1;
+        JdSpace J(p, max_iter, gmres_iter, n_ortho);
+        if (n_ortho) J.set_ortho(ortho);
         int ns = 0;
         double theta = J.run(init, nullptr, tol, 0.0, MQ_PREC_F64, &ns, iterations, nullptr);
         J.copy(J.u(), out);

@@ -153,6 +153,8 @@ SIGNATURES = {
     "mq_vec_scal": (C.c_int, [P, C.c_uint64, C.c_double, P]),
     "mq_vec_copy": (C.c_int, [P, C.c_uint64, P, P]),
     "mq_jd_solve": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "mq_jd_solve_ex": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                               C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "mq_jd_solve_refined": (C.c_int, [P, P, P, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_int), C.POINTER(C.c_double)]),
 }

@@ -402,12 +402,15 @@ class SigmaPlan:
                                     int(precision), ms, fl, C.byref(gb)))
         return list(ms), list(fl), gb.value
 
-    def jd(self, init, out, tol=1e-8, max_iter=8):
+    def jd(self, init, out, tol=1e-8, max_iter=8, gmres_iter=0, ortho=()):
+        """Jacobi-Davidson lowest eigenpair (mq_jd_solve_ex): gmres_iter > 0 solves the correction
+        equation with GMRES; ortho: device vectors the solution is kept orthogonal to."""
         e = C.c_double()
         it = C.c_int()
         g = self.engine
-        check(lib().mq_jd_solve(self.h, dev_ptr(g, init, self.n), dev_ptr(g, out, self.n), float(tol), int(max_iter), 0,
-                                C.byref(e), C.byref(it)))
+        ptrs = (C.c_void_p * max(1, len(ortho)))(*[dev_ptr(g, o, self.n) for o in ortho])
+        check(lib().mq_jd_solve_ex(self.h, dev_ptr(g, init, self.n), dev_ptr(g, out, self.n), float(tol), int(max_iter),
+                                   int(gmres_iter), len(ortho), ptrs, C.byref(e), C.byref(it)))
         return e.value, it.value
 
     def jd_refined(self, init, out, tol=1e-8, max_iter=8, precision=PREC_F32, max_outer=4):

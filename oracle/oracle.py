@@ -46,6 +46,7 @@ def lib():
         L.mqo_boundary_free.argtypes = [P]
         L.mqo_subspace_min.argtypes = [P, C.c_int, P, P]
         L.mqo_jd.argtypes = [P, P, P, P, P, P, C.c_uint64, P, P, C.c_double, C.c_int, P, P]
+        L.mqo_jd_ex.argtypes = [P, P, P, P, P, P, C.c_uint64, P, P, C.c_double, C.c_int, C.c_int, C.c_int, P, P, P]
         L.mqo_set_dgemm.argtypes = [P]
         _lib = L
     return _lib
@@ -128,13 +129,18 @@ class SigmaOracle:
                                    self.rp, psi.ctypes.data, out.ctypes.data))
         return out
 
-    def jd(self, init, tol=1e-8, max_iter=8):
+    def jd(self, init, tol=1e-8, max_iter=8, gmres_iter=0, ortho=()):
+        """Jacobi-Davidson lowest eigenpair (ietl/jacobi.h:386-474); gmres_iter > 0: GMRES correction
+        (ietl/gmres.h), ortho: vectors the search space is kept orthogonal to (excited states)."""
         init = np.ascontiguousarray(init, np.float64)
         out = np.zeros_like(init)
         e = C.c_double()
         it = C.c_int()
-        _chk(lib().mqo_jd(self.h, C.byref(self.site.desc), C.byref(self.ld), self.lp, C.byref(self.rd), self.rp,
-                          init.size, init.ctypes.data, out.ctypes.data, tol, max_iter, C.byref(e), C.byref(it)))
+        keep = [np.ascontiguousarray(o, np.float64) for o in ortho]
+        ptrs = (C.c_void_p * max(1, len(keep)))(*[k.ctypes.data for k in keep])
+        _chk(lib().mqo_jd_ex(self.h, C.byref(self.site.desc), C.byref(self.ld), self.lp, C.byref(self.rd), self.rp,
+                             init.size, init.ctypes.data, out.ctypes.data, tol, max_iter, int(gmres_iter), len(keep),
+                             ptrs, C.byref(e), C.byref(it)))
         return e.value, out, it.value
 
     def __del__(self):

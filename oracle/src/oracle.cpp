@@ -24,6 +24,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <functional>
 #include <vector>
 
 #include "mq_sigma.h"
@@ -1174,8 +1175,8 @@ const double* mqo_boundary_data(void* h, uint32_t ci) { return ((Bnd*)h)->data[c
 
 void mqo_boundary_free(void* h) { delete (Bnd*)h; }
 
-// jacobi_davidson::calculate_eigenvalue (ietl/jacobi.h:386-474) with the plain correction
-// (jcd_gmres_solver, max_iter 0: ietl/jacobi.h:141-157) and basic_iteration (ietl/iteration.h:40-78).
+// jacobi_davidson::calculate_eigenvalue (ietl/jacobi.h:386-474) with the jcd_gmres_solver correction
+// (ietl/jacobi.h:126-164; plain when gmres_iter = 0) and basic_iteration (ietl/iteration.h:40-78).
 // The subspace eigenproblem is solved by cyclic Jacobi (dsyevx range 'I', il = iu = 1).
 static double jacobi_min(std::vector<double> A, int n, std::vector<double>& vec) {
     std::vector<double> V(n * n, 0.0);
@@ -1225,9 +1226,71 @@ int mqo_subspace_min(const double* A, int n, double* vec, double* val) {
     });
 }
 
-int mqo_jd(void* h, const mq_site_desc* sd, const mq_bindex_desc* ld, const double* const* ldata,
-           const mq_bindex_desc* rd, const double* const* rdata, uint64_t n, const double* init, double* out,
-           double tol, int maxiter, double* energy, int* iters) {
+// ietl_gmres (ietl/gmres.h:78-140) on A x = b from x0, A the projected correction operator
+// (1 - uu^T)(H - theta)(1 - uu^T) (jcd_solver_operator, ietl/jacobi.h:266-284); its bookkeeping kept:
+// at most max_iter - 1 Arnoldi steps, and on convergence at step i the update uses i basis vectors.
+static std::vector<double> gmres_restated(const std::function<void(const std::vector<double>&, std::vector<double>&)>& A,
+                                          const std::vector<double>& b, const std::vector<double>& x0, double abs_tol,
+                                          int max_iter) {
+    const size_t n = b.size();
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) {
+        double r = 0;
+        for (size_t i = 0; i < n; ++i) r += a[i] * c[i];
+        return r;
+    };
+    std::vector<double> sv(max_iter + 1, 0.0), cs(max_iter + 1, 0.0), sn(max_iter + 1, 0.0);
+    std::vector<std::vector<double>> v(max_iter + 1);
+    std::vector<double> w;
+    A(x0, v[0]);
+    for (size_t i = 0; i < n; ++i) v[0][i] = b[i] - v[0][i];
+    sv[0] = std::sqrt(dot(v[0], v[0]));
+    if (std::fabs(sv[0]) < abs_tol) return x0;
+    for (auto& x : v[0]) x /= sv[0];
+    std::vector<double> H(size_t(max_iter + 1) * (max_iter + 1), 0.0);
+    auto h = [&](int i, int j) -> double& { return H[size_t(i) * (max_iter + 1) + j]; };
+    auto rot = [](double& dx, double& dy, double c, double s_) {
+        double r0 = c * dx + s_ * dy, r1 = -s_ * dx + c * dy;
+        dx = r0, dy = r1;
+    };
+    int i = 0;
+    for (; i < max_iter - 1; ++i) {
+        A(v[i], w);
+        for (int k = 0; k <= i; ++k) {
+            h(k, i) = dot(w, v[k]);
+            axpy(n, -h(k, i), v[k].data(), w.data());
+        }
+        h(i + 1, i) = std::sqrt(dot(w, w));
+        v[i + 1] = w;
+        for (auto& x : v[i + 1]) x /= h(i + 1, i);
+        for (int k = 0; k < i; ++k) rot(h(k, i), h(k + 1, i), cs[k], sn[k]);
+        double dx = h(i, i), dy = h(i + 1, i);
+        if (dy == 0.0) {
+            cs[i] = 1, sn[i] = 0;
+        } else if (std::fabs(dy) > std::fabs(dx)) {
+            double t = dx / dy;
+            sn[i] = 1.0 / std::sqrt(1.0 + t * t), cs[i] = t * sn[i];
+        } else {
+            double t = dy / dx;
+            cs[i] = 1.0 / std::sqrt(1.0 + t * t), sn[i] = t * cs[i];
+        }
+        rot(h(i, i), h(i + 1, i), cs[i], sn[i]);
+        rot(sv[i], sv[i + 1], cs[i], sn[i]);
+        if (std::fabs(sv[i + 1]) < abs_tol) break;
+    }
+    std::vector<double> y(sv.begin(), sv.begin() + i);
+    for (int a = i - 1; a >= 0; --a) {
+        y[a] /= h(a, a);
+        for (int c = a - 1; c >= 0; --c) y[c] -= h(c, a) * y[a];
+    }
+    std::vector<double> x = x0;
+    for (int k = 0; k < i; ++k) axpy(n, y[k], v[k].data(), x.data());
+    return x;
+}
+
+int mqo_jd_ex(void* h, const mq_site_desc* sd, const mq_bindex_desc* ld, const double* const* ldata,
+              const mq_bindex_desc* rd, const double* const* rdata, uint64_t n, const double* init, double* out,
+              double tol, int maxiter, int gmres_iter, int n_ortho, const double* const* ortho, double* energy,
+              int* iters) {
     return guard([&] {
         auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
             double s = 0;
@@ -1239,9 +1302,21 @@ int mqo_jd(void* h, const mq_site_desc* sd, const mq_bindex_desc* ld, const doub
             int rc = mqo_sigma_apply(h, sd, ld, ldata, rd, rdata, x.data(), y.data());
             if (rc) throw std::runtime_error(last_error);
         };
+        // ortho vectors: Gram-Schmidt'ed copies (solver/ietl_jacobi_davidson.h:41-47), projected out by
+        // DavidsonVS::project (solver/ietl_interface_dv.h:94-99)
+        if (uint64_t(n_ortho) >= n) n_ortho = int(n) - 1;
+        std::vector<std::vector<double>> O(n_ortho);
+        for (int j = 0; j < n_ortho; ++j) {
+            O[j].assign(ortho[j], ortho[j] + n);
+            for (int j0 = 0; j0 < j; ++j0) axpy(n, -dot(O[j0], O[j]) / dot(O[j0], O[j0]), O[j0].data(), O[j].data());
+        }
+        auto project = [&](std::vector<double>& t) {
+            for (int j = 0; j < n_ortho; ++j) axpy(n, -dot(O[j], t) / dot(O[j], O[j]), O[j].data(), t.data());
+        };
         std::vector<std::vector<double>> V(maxiter + 1), VA(maxiter);
         std::vector<double> M(size_t(maxiter) * maxiter, 0.0), s;
         V[0].assign(init, init + n);
+        project(V[0]);
         int it = 0;
         double theta = 0;
         std::vector<double> u, uA;
@@ -1251,6 +1326,7 @@ int mqo_jd(void* h, const mq_site_desc* sd, const mq_bindex_desc* ld, const doub
             for (int i = 1; i <= it; ++i) axpy(n, -dot(V[i - 1], t), V[i - 1].data(), t.data());
             if (std::sqrt(dot(t, t)) < 0.25 * tau)
                 for (int i = 1; i <= it; ++i) axpy(n, -dot(V[i - 1], t), V[i - 1].data(), t.data());
+            project(t);
             double nt = std::sqrt(dot(t, t));
             for (auto& x : t) x /= nt;
             mult(t, VA[it]);
@@ -1267,6 +1343,7 @@ int mqo_jd(void* h, const mq_site_desc* sd, const mq_bindex_desc* ld, const doub
                 axpy(n, s[j], V[j].data(), u.data());
                 axpy(n, s[j], VA[j].data(), uA.data());
             }
+            project(uA);
             axpy(n, -theta, u.data(), uA.data());
             ++it;
             double rn = std::sqrt(dot(uA, uA));
@@ -1274,11 +1351,31 @@ int mqo_jd(void* h, const mq_site_desc* sd, const mq_bindex_desc* ld, const doub
             double dru = dot(uA, u), duu = dot(u, u);
             V[it].assign(n, 0.0);
             for (size_t i = 0; i < n; ++i) V[it][i] = -uA[i] + dru / duu * u[i];
+            if (gmres_iter > 0) {   // jcd_gmres_solver (ietl/jacobi.h:141-157), tolerance 2^-(it+1)
+                const std::vector<double> uu = u;
+                const double th = theta;
+                auto op = [&](const std::vector<double>& x, std::vector<double>& y) {
+                    std::vector<double> t2 = x;
+                    axpy(n, -dot(uu, x), uu.data(), t2.data());
+                    mult(t2, y);
+                    axpy(n, -th, t2.data(), y.data());
+                    axpy(n, -dot(uu, y), uu.data(), y.data());
+                };
+                std::vector<double> b(n);
+                for (size_t i = 0; i < n; ++i) b[i] = -uA[i];
+                V[it] = gmres_restated(op, b, V[it], 1.0 / std::pow(2.0, double(it + 1)), gmres_iter);
+            }
         }
         std::copy(u.begin(), u.end(), out);
         *energy = theta;
         *iters = it;
     });
+}
+
+int mqo_jd(void* h, const mq_site_desc* sd, const mq_bindex_desc* ld, const double* const* ldata,
+           const mq_bindex_desc* rd, const double* const* rdata, uint64_t n, const double* init, double* out,
+           double tol, int maxiter, double* energy, int* iters) {
+    return mqo_jd_ex(h, sd, ld, ldata, rd, rdata, n, init, out, tol, maxiter, 0, 0, nullptr, energy, iters);
 }
 
 }  // extern "C"

@@ -46,6 +46,10 @@ class SU2BlockProblem:
         self.basis, self.mpo, self.L, self.N, self.two_s, self.site = basis, mpo, L, N, two_s, site
         phys = basis.phys
         lb, rb = exact_bonds(phys, L, N, (N, two_s))
+        # keep the sectors reachable from both ends: a bond index and the boundary built on it then
+        # agree at every cut (a sector only one end reaches carries no weight of the target state)
+        both = lambda a, b: Index.from_dict({c: n for c, n in zip(a.charges, a.sizes) if b.has(c)})
+        lb, rb = [both(l, r) for l, r in zip(lb, rb)], [both(r, l) for l, r in zip(lb, rb)]
         self.Lb = O.trivial_boundary(lb[0], mpo[0].row_dim)
         self.left_sites, self.left_bounds = [], [self.Lb]
         for k in range(site):

@@ -415,3 +415,25 @@ def test_wrappers_reject_bad_buffers_before_any_c_call(monkeypatch):
         E.Engine.axpy(eng, 1.0, good_shape, good_shape)
     with pytest.raises(ValueError):
         E.apply_batched(eng, [plan], [torch.zeros(4, dtype=torch.float64)], [good_shape])
+
+
+def test_oracle_jd_gmres_and_ortho_vectors():
+    """The oracle's Jacobi-Davidson with the GMRES correction reaches the lowest eigenvalue of the
+    dense H_eff; with the ground state as ortho vector it reaches the second one, orthogonal to it."""
+    p = SiteProblem("heisenberg", L=8, site=3, Mmax=6, seed=21)
+    so = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    lay = p.layout
+    idx = np.concatenate([np.arange(lay.rp_offset[b], lay.rp_offset[b] + lay.left.sizes[b] * lay.rp_cols[b])
+                          for b in range(len(lay.left.sizes))])
+    H = np.zeros((len(idx), len(idx)))
+    for c, i in enumerate(idx):
+        e = np.zeros(lay.rp_total)
+        e[i] = 1.0
+        H[:, c] = so.apply(e)[idx]
+    w = np.linalg.eigvalsh(0.5 * (H + H.T))
+    e0, v0, it0 = so.jd(p.psi, tol=1e-11, max_iter=40)
+    e1, v1, it1 = so.jd(p.psi, tol=1e-11, max_iter=40, gmres_iter=4)
+    assert abs(e0 - w[0]) <= 1e-9 * abs(w[0]) and abs(e1 - w[0]) <= 1e-9 * abs(w[0]), (e0, e1, w[0])
+    e2, v2, it2 = so.jd(p.psi, tol=1e-11, max_iter=60, ortho=[v0])
+    assert abs(e2 - w[1]) <= 1e-8 * abs(w[1]), (e2, w[1])
+    assert abs(float(v2 @ v0)) <= 1e-10

@@ -357,14 +357,50 @@ def test_jacobi_davidson_matches_oracle(eng, model):
     assert min(np.linalg.norm(v - v_ref), np.linalg.norm(v + v_ref)) <= 1e-8
 
 
+@pytest.mark.parametrize("model", ["heisenberg", "qc"])
+def test_jacobi_davidson_gmres_and_ortho_vectors(eng, model):
+    """The GMRES correction (gmres_iter > 0) and ortho vectors follow the oracle's restatement of
+    ietl's GMRES-JD (same iteration count, energies to 1e-10 relative); on the Heisenberg block the
+    GMRES run reaches the lowest eigenvalue of the dense H_eff, and with the ground state as ortho
+    vector the second one, orthogonal to it (excited states, solver.h:65-69).  (On the ab initio block
+    with random boundaries neither correction converges in 30 steps; the trajectories still agree.)"""
+    p = SiteProblem(model, L=8, site=3, Mmax=6, seed=21)
+    so = O.SigmaOracle(p.site_desc, p.mpo_desc, p.Lb, p.Rb)
+    Lb, Rb = upload(eng, p.Lb), upload(eng, p.Rb)
+    plan = E.SigmaPlan(eng, p.site_desc, p.mpo_desc, Lb, Rb)
+    psi = dev(p.psi)
+    out0, out1, out2 = torch.zeros_like(psi), torch.zeros_like(psi), torch.zeros_like(psi)
+    e0, it0 = plan.jd(psi, out0, tol=1e-11, max_iter=30)
+    e1, it1 = plan.jd(psi, out1, tol=1e-11, max_iter=30, gmres_iter=4)
+    r1, _, rit1 = so.jd(p.psi, tol=1e-11, max_iter=30, gmres_iter=4)
+    print("plain: E %.12f in %d; gmres: E %.12f in %d (oracle %.12f in %d)" % (e0, it0, e1, it1, r1, rit1), flush=True)
+    assert it1 == rit1 and abs(e1 - r1) <= 1e-10 * abs(r1)
+    e2, it2 = plan.jd(psi, out2, tol=1e-10, max_iter=60, ortho=[out0])
+    r2, v2, rit2 = so.jd(p.psi, tol=1e-10, max_iter=60, ortho=[out0.cpu().numpy()])
+    ov = float(torch.dot(out2, out0))
+    print("with ortho: E %.12f in %d (oracle %.12f in %d), overlap %.2e" % (e2, it2, r2, rit2, ov), flush=True)
+    assert it2 == rit2 and abs(e2 - r2) <= 1e-10 * abs(r2)
+    assert abs(ov) <= 1e-9 * float(torch.dot(out0, out0))
+    if model == "heisenberg":
+        lay = p.layout
+        idx = np.concatenate([np.arange(lay.rp_offset[b], lay.rp_offset[b] + lay.left.sizes[b] * lay.rp_cols[b])
+                              for b in range(len(lay.left.sizes))])
+        H = np.zeros((len(idx), len(idx)))
+        for c, i in enumerate(idx):
+            x = np.zeros(lay.rp_total)
+            x[i] = 1.0
+            H[:, c] = so.apply(x)[idx]
+        w = np.linalg.eigvalsh(0.5 * (H + H.T))
+        assert abs(e1 - w[0]) <= 1e-9 * abs(w[0]) and abs(e2 - w[1]) <= 1e-8 * abs(w[1]), (e1, e2, w[:2])
 
 
-@pytest.mark.parametrize("model,L,m", [("qc_su2", 16, 256), ("qc", 14, 192)])
+@pytest.mark.parametrize("model,L,m", [("qc_su2", 24, 1000), ("qc", 20, 512)])
 def test_block_sparse_s_stage_matches_dense_and_oracle(eng, model, L, m, monkeypatch):
     """Ab initio middle-of-chain shapes: the block-sparse S stage (keys in pattern-class order, MFMA
-    tasks over the nonzero 16-row tiles of W_D) does fewer flops than the dense library GEMMs of the
-    same plan built with MQ_S_DENSE=1, and its sigma equals the dense plan's to 1e-13 (FP64) and
-    2e-6 (FP32) and the oracle's on a block sample to 1e-12."""
+    tasks over the nonzero 16-row tiles of W_D, for the groups where those visit at most half of
+    W_D) does fewer flops than the dense library GEMMs of the same plan built with MQ_S_DENSE=1,
+    and its sigma equals the dense plan's to 1e-13 (FP64) and 2e-6 (FP32) and the oracle's on a
+    block sample to 1e-12."""
     from mqdmrg.workloads import SyntheticSite
     print("case", model, L, m, flush=True)
     ws = SyntheticSite(model, L, L // 2 - 1, m, seed=9)
@@ -381,7 +417,7 @@ def test_block_sparse_s_stage_matches_dense_and_oracle(eng, model, L, m, monkeyp
     _, fl_sparse, _ = sparse.profile(psi, tmp, reps=1)
     _, fl_dense, _ = dense.profile(psi, tmp, reps=1)
     print("  S-stage flops sparse %.3e dense %.3e" % (fl_sparse[1], fl_dense[1]), flush=True)
-    assert fl_sparse[1] < 0.5 * fl_dense[1]
+    assert fl_sparse[1] < 0.9 * fl_dense[1]
     outs = {}
     for name, plan in (("sparse", sparse), ("dense", dense)):
         for prec in (E.PREC_F64, E.PREC_F32):
