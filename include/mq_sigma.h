@@ -355,6 +355,18 @@ int mq_vec_copy(mq_ctx* ctx, uint64_t n, const double* x, double* y);
  * The partial sums live in a context buffer that grows with k.  Fixed reduction order. */
 int mq_vec_multi_dot(mq_ctx* ctx, uint64_t n, int k, const double* V, uint64_t ldv, const double* x,
                      double* result);
+/* Allocation-free form of mq_vec_multi_dot.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  It calls no hipMalloc / hipFree and no allocating library routine, and
+ * does not synchronise: result is DEVICE memory (k doubles), written in stream order on the
+ * context's stream.  Its partial sums live in a static per-GPU scratch of MQ_NOALLOC_MAXK x 512
+ * doubles (256 KiB, held by every GPU the library runs on until the process ends); k above
+ * MQ_NOALLOC_MAXK is rejected with MQ_ERR_INVALID.  All calls on one GPU share that scratch, so calls
+ * that overlap in time on one GPU (other contexts' streams, other threads) race; calls ordered on
+ * one stream do not.  Same kernels and reduction order as mq_vec_multi_dot: identical results. */
+#define MQ_NOALLOC_MAXK 64
+int mq_vec_multi_dot_noalloc(mq_ctx* ctx, uint64_t n, int k, const double* V, uint64_t ldv, const double* x,
+                             double* result);
 
 /* Jacobi-Davidson smallest eigenpair of H_eff (ietl/jacobi.h:386-516, solver/ietl_jacobi_davidson.h:35-76).
  * init and out are device vectors in the plan's layout (may alias).  gmres_iter == 0 uses the

@@ -23,6 +23,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -252,6 +253,7 @@ struct mq_ctx {
         mdot_result.alloc(size_t(k) * sizeof(double));
         mdot_cap = k;
     }
+    double* noalloc_scratch = nullptr;   // this device's static scratch (mq_vec_multi_dot_noalloc)
     DevBuf step_ws;   // grows; reused by boundary steps
     // batched applications: side streams (default priority) the instances of a batch fork onto,
     // and the captured graphs keyed by (plans, precision)
@@ -1183,6 +1185,14 @@ extern "C" {
 
 const char* mq_last_error(void) { return g_err.c_str(); }
 
+// Scratch of the allocation-free batched dot (mq_vec_multi_dot_noalloc): the per-workgroup partial
+// sums of up to MQ_NOALLOC_MAXK vectors, one copy per GPU for the life of the process.  Host code
+// never uses the symbol itself: each device's address comes from hipGetSymbolAddress at context
+// creation (g_scratch_dev) and is what kernels receive.
+static __device__ __align__(256) double scratch[MQ_NOALLOC_MAXK * RED_BLOCKS];
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch_dev;   // device ordinal -> address of its scratch
+
 static std::vector<mq_ctx*> g_contexts;
 static void release_all_pools() {
     for (mq_ctx* c : g_contexts) {
@@ -1197,6 +1207,18 @@ int mq_ctx_create(int device, mq_ctx** out) {
         auto c = std::make_unique<mq_ctx>();
         c->device = device;
         HIPCHK(hipSetDevice(device));
+        {
+            // once per device, with that device current
+            std::lock_guard<std::mutex> lk(g_scratch_mu);
+            auto it = g_scratch_dev.find(device);
+            if (it == g_scratch_dev.end()) {
+                void* ptr = nullptr;
+                HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+                MQ_REQUIRE(ptr, "hipGetSymbolAddress returned no address for the static scratch");
+                it = g_scratch_dev.emplace(device, static_cast<double*>(ptr)).first;
+            }
+            c->noalloc_scratch = it->second;
+        }
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         BLASCHK(rocblas_create_handle(&c->blas));
         BLASCHK(rocblas_set_stream(c->blas, c->stream));
@@ -3470,6 +3492,18 @@ int mq_vec_multi_dot(mq_ctx* ctx, uint64_t n, int k, const double* V, uint64_t l
                                 ctx->mdot_result.as<double>()));
         HIPCHK(hipMemcpyAsync(result, ctx->mdot_result.p, size_t(k) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mq_vec_multi_dot_noalloc(mq_ctx* ctx, uint64_t n, int k, const double* V, uint64_t ldv, const double* x,
+                             double* result) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && V && x && result && k >= 1, "bad argument");
+        MQ_REQUIRE(k <= MQ_NOALLOC_MAXK, "mq_vec_multi_dot_noalloc: k exceeds the static scratch (MQ_NOALLOC_MAXK vectors)");
+        MQ_REQUIRE(k == 1 || ldv >= n, "ldv < n");
+        MQ_REQUIRE(ctx->noalloc_scratch, "no static scratch address for this device");
+        // Every partial sum a call reads is written by the same call: nothing needs clearing.
+        HIPCHK(launch_multi_dot(ctx->stream, n, k, V, ldv, x, ctx->noalloc_scratch, result));
     });
 }
 

@@ -149,6 +149,7 @@ SIGNATURES = {
     "mq_vec_dot": (C.c_int, [P, C.c_uint64, P, P, C.POINTER(C.c_double)]),
     "mq_vec_nrm2": (C.c_int, [P, C.c_uint64, P, C.POINTER(C.c_double)]),
     "mq_vec_multi_dot": (C.c_int, [P, C.c_uint64, C.c_int, P, C.c_uint64, P, P]),
+    "mq_vec_multi_dot_noalloc": (C.c_int, [P, C.c_uint64, C.c_int, P, C.c_uint64, P, P]),
     "mq_vec_axpy": (C.c_int, [P, C.c_uint64, C.c_double, P, P]),
     "mq_vec_scal": (C.c_int, [P, C.c_uint64, C.c_double, P]),
     "mq_vec_copy": (C.c_int, [P, C.c_uint64, P, P]),
