@@ -77,6 +77,9 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile configurations (no timing of candidates before the run)")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GB",
+                    help="only copy a device buffer of this many GB five times (a kernel of known HBM traffic: "
+                         "the reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
 
 
@@ -226,9 +229,11 @@ def adjoint_workload(eng, args, su2_model):
     fd = []
     picks = [("h", (0, 0)), ("h", (L // 2, L // 2 + 1)), ("eri", (1, 1, 2, 2)), ("eri", (0, 1, 1, 0))][:args.adjoint_fd]
     def perm_class(kind, idx):
-        # the spin-adapted generator reads one element per permutation class: perturb the class
+        # the spin-adapted generator reads one element per permutation class: perturb the class.
+        # TwoU1: the entry with its Hermitian partner (a single off-diagonal entry alone makes H
+        # non-Hermitian); the derivatives of the two are equal
         if not su2_model:
-            return [idx]
+            return sorted({idx, qc.hermitian_partner(kind, idx)})
         if kind == "h":
             return sorted({idx, idx[::-1]})
         p, q, r, s_ = idx
@@ -262,6 +267,15 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.calibrate_copy > 0:
+        n = int(args.calibrate_copy * 1e9) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 8 * n, "calibrate_copy_bytes_written": 8 * n, "copies": 5}))
+        return
     from mqdmrg import engine as E
     from mqdmrg.workloads import SyntheticSite
 
@@ -325,8 +339,8 @@ def main():
                      "wall_seconds_timed": round(wall, 4),
                      "stage_ms": dict(zip(["fill", "gemm_T", "gemm_S", "gemm_final"], [round(x, 4) for x in stage_ms])),
                      "stage_gemm_flops": dict(zip(["T", "S", "final"], stage_flops)), "autotune": tune}
-        log("%s: %.3f ms/sigma, %.2f TFLOP/s, rel err %.2e, stages %s" % (name, ms, per[name]["tflops"], err,
-                                                                         per[name]["stage_ms"]))
+        log("%s: %.3f ms/sigma, %.2f TFLOP/s, rel err %s, stages %s" % (
+            name, ms, per[name]["tflops"], "n/a" if err is None else "%.2e" % err, per[name]["stage_ms"]))
 
     if args.only:
         report = args.only

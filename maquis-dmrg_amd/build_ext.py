@@ -19,6 +19,7 @@ SOURCES = [
     "csrc/host/schedule.cpp",
     "csrc/device/kernels.hip",
     "csrc/device/gemm.hip",
+    "csrc/device/s_stage.hip",
     "csrc/device/engine.hip",
 ]
 

@@ -479,15 +479,21 @@ struct StagePlan {
         uint64_t a_off, b_off, c_off;
         uint32_t M, N, K, lda, ldb, ldc;
         uint64_t bfa_off, bfb_off;
-        bool sparse = false;   // FP64 / FP32: the group runs as block-sparse MFMA tasks (gS) instead
+        bool sparse = false;   // the group's S stage runs in the block-sparse kernel (bsr) instead
     };
     std::vector<LibGemm> lib;
-    // grouped path, FP64 and FP32: the S stage of groups whose W_D is sparse enough, as block-sparse
-    // MFMA tasks over the nonzero 16-row tiles of W_D (one task per 128-row panel of T_D and 128-column
-    // panel of W_D, its segments the runs of nonzero tiles of that column panel)
-    std::vector<GemmTask> s_tasks;
-    std::vector<GemmSeg> s_segs;
-    GemmStage gS;
+    // grouped path: the S stage as the block-sparse MFMA kernel (s_stage.hip) over the nonzero 16-key
+    // tiles of every group's W_D, in all three precisions; the library GEMMs above then serve only
+    // plans made with MQ_S_DENSE=1 and the adjoint
+    struct Bsr {
+        std::vector<BsrTask> tasks;
+        std::vector<BsrRun> runs;
+        std::vector<double> w;          // compact W tiles (host copy until upload)
+        uint64_t tiles = 0;
+        uint64_t flops = 0;             // MFMA flops of the visited tiles (unpadded)
+        DevBuf d_tasks, d_runs, d_w64, d_w32, d_whi, d_wlo;
+    } bsr;
+    bool bsr_on = false;
     rocblas_handle blas = nullptr;
     // hipBLASLt form of the FP32 (index 0) and BF16x3 (index 1) library GEMMs, prepared on first
     // use outside graph capture; a GEMM without a hipBLASLt algorithm runs on rocBLAS
@@ -570,7 +576,7 @@ struct StagePlan {
     // (cached in the context by shape and by shape class); returns the number of timings.  A GEMM
     // whose algorithm came from either cache is not timed again.
     int lt_tune(int mi, size_t i, hipStream_t st, Bases b, hipEvent_t e0, hipEvent_t e1) const {
-        if (mi == 0 && lib[i].sparse) return 0;   // FP32 runs this group block-sparse
+        if (lib[i].sparse) return 0;   // the group runs in the block-sparse kernel
         if (!lt_prepare(mi, i, st)) return 0;
         LtGemm& L = ltg[mi][i];
         if (L.chosen) return 0;
@@ -639,7 +645,6 @@ struct StagePlan {
         gT.build(t_tasks);
         gF.build(f_tasks);
         gU.build(u_tasks);
-        gS.build(s_tasks);
         red.finish();
         std::vector<GatherRegion> big;
         for (const auto& r : regions) {
@@ -811,7 +816,19 @@ struct StagePlan {
         gemm(t_tasks, t_segs, split_t);
         gemm(f_tasks, f_segs, false);
         gemm(u_tasks, u_segs, false);
-        gemm(s_tasks, s_segs, false);
+        for (const auto& t : bsr.tasks) {   // every T_D read (128 rows of each key column) and S_D write
+            MQ_REQUIRE(t.M <= S_BSR_TILE_M && t.N <= S_BSR_TILE_N && t.run0 <= t.run1 && t.run1 <= bsr.runs.size(),
+                       "block-sparse task shape");
+            uint32_t kmax = 0;
+            for (uint32_t r = t.run0; r < t.run1; ++r) {
+                const BsrRun& R = bsr.runs[r];
+                MQ_REQUIRE(R.k0 % S_BSR_TILE_K == 0 && R.nk > 0, "block-sparse run");
+                MQ_REQUIRE(uint64_t(R.tile) + (R.nk + S_BSR_TILE_K - 1) / S_BSR_TILE_K <= bsr.tiles, "block-sparse tile");
+                kmax = std::max(kmax, R.k0 + R.nk);
+            }
+            if (kmax) check_in(BASE_WS, t.a_off, uint64_t(kmax - 1) * t.ld + S_BSR_TILE_M, "block-sparse T_D rows");
+            check_in(BASE_WS, t.c_off, extent(t.M, t.N, t.ld), "block-sparse S_D tile");
+        }
         if (split_t)
             for (const auto& t : t_tasks)
                 MQ_REQUIRE(t.rowunit != MAP_NONE && t.c_base == BASE_WS && !t.beta, "split T stage needs stacked tasks");
@@ -843,7 +860,28 @@ struct StagePlan {
         gT.upload_all(t_segs, st);
         gF.upload_all(f_segs, st);
         gU.upload_all(u_segs, st);
-        gS.upload_all(s_segs, st);
+        if (bsr_on && !bsr.w.empty()) {   // once: the host copy of the tiles is released after it
+            upload(bsr.d_tasks, bsr.tasks, st);
+            upload(bsr.d_runs, bsr.runs, st);
+            // the W tiles in the three arithmetics: FP64, FP32, and the bf16 split of the FP32 value
+            std::vector<float> w32(bsr.w.size());
+            std::vector<uint16_t> whi(bsr.w.size()), wlo(bsr.w.size());
+            for (size_t i = 0; i < bsr.w.size(); ++i) {
+                const float x = float(bsr.w[i]);
+                w32[i] = x;
+                whi[i] = bf16_rne_host(x);
+                float h;
+                const uint32_t hb = uint32_t(whi[i]) << 16;
+                std::memcpy(&h, &hb, 4);
+                wlo[i] = bf16_rne_host(x - h);
+            }
+            upload(bsr.d_w64, bsr.w, st);
+            upload(bsr.d_w32, w32, st);
+            upload(bsr.d_whi, whi, st);
+            upload(bsr.d_wlo, wlo, st);
+            HIPCHK(hipStreamSynchronize(st));   // the host vectors are released below
+            std::vector<double>().swap(bsr.w);
+        }
         upload(d_regions, regions, st);
         upload(d_small, small, st);
         upload(d_src_off, src_off, st);
@@ -869,7 +907,7 @@ struct StagePlan {
                         HIPCHK(launch_fill(st, reinterpret_cast<double*>(reinterpret_cast<float*>(b.p[BASE_WS]) + s_begin),
                                            s_elems / 2, 0.0));
                 }
-                if (mode == MODE_BF16X3 && split_t)
+                if (mode == MODE_BF16X3 && split_t && !bsr_on)
                     HIPCHK(launch_zero_chunks(st, d_zero_bf.as<ZeroChunk>(), uint32_t(zero_bf.size()), bf, 2));
                 else
                     HIPCHK(launch_zero_chunks(st, d_zero_ws.as<ZeroChunk>(), uint32_t(zero_ws.size()), b.p[BASE_WS],
@@ -878,7 +916,7 @@ struct StagePlan {
             case 1:
                 // the T and final stages are bandwidth-bound block products: BF16x3 runs them in FP32
                 // and uses the split products in the S stage only
-                if (mode == MODE_BF16X3 && split_t) {
+                if (mode == MODE_BF16X3 && split_t && !bsr_on) {
                     Bases bs = b;   // C of the stacked T tasks: the A' copies in the bf16 buffer
                     bs.p[BASE_WS] = reinterpret_cast<double*>(bf);
                     gT.run(st, MODE_F32, "dds", bs, tb);
@@ -887,7 +925,12 @@ struct StagePlan {
                 }
                 break;
             case 2:
-                if (mode != MODE_BF16X3) gS.run(st, mixed ? MODE_F32 : MODE_F64, mixed ? "fff" : "ddd", b, tb);
+                if (bsr_on) {
+                    const void* b0 = mode == MODE_F64 ? bsr.d_w64.p : mode == MODE_F32 ? bsr.d_w32.p : bsr.d_whi.p;
+                    const void* b1 = mode == MODE_BF16X3 ? bsr.d_wlo.p : nullptr;
+                    HIPCHK(launch_s_bsr(st, mode, bsr.d_tasks.as<BsrTask>(), uint32_t(bsr.tasks.size()),
+                                        bsr.d_runs.as<BsrRun>(), b0, b1, b.p[BASE_WS]));
+                }
                 run_lib(st, b, mode);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
@@ -928,12 +971,13 @@ struct StagePlan {
                                           &zero, ws + g.c_off, g.ldc));
             }
         } else {
-            MQ_REQUIRE(bf, "bf16 split buffer not allocated");
             float* ws = reinterpret_cast<float*>(b.p[BASE_WS]);
             const float one = 1.f, zero = 0.f;
             const rocblas_datatype b16 = rocblas_datatype_bf16_r, f32 = rocblas_datatype_f32_r;
             for (size_t i = 0; i < lib.size(); ++i) {
                 const auto& g = lib[i];
+                if (g.sparse) continue;
+                MQ_REQUIRE(bf, "bf16 split buffer not allocated");
                 if (!split_t) HIPCHK(launch_split3_bf16(st, ws + g.a_off, bf + g.bfa_off, uint64_t(g.M) * g.K));
                 if (lt_prepare(1, i, st)) {
                     lt_run(1, i, st, bf + g.bfa_off, bf + g.bfb_off, ws + g.c_off);
@@ -1021,7 +1065,7 @@ struct mq_plan {
                                            ws_init_elems));
             }
             b.p[BASE_WS] = reinterpret_cast<double*>(ws32.as<float>());
-            if (mode == MODE_BF16X3 && st.bf_elems && !bf.p) {
+            if (mode == MODE_BF16X3 && st.bf_elems && !bf.p && !st.bsr_on) {
                 // bf16 operands of the one-GEMM BF16x3 form: A' regions (written per application) and
                 // B' = [W_lo; W_hi; W_hi] per group (constant, split here on the device from W_D)
                 ctx->take(mq_ctx::POOL_BF, bf, std::max<uint64_t>(st.bf_elems, 64) * sizeof(uint16_t));
@@ -1781,67 +1825,58 @@ void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* W
     }
 }
 
-// Block-sparse S-stage tasks of one group (W_D at w_off, K x N, after reorder_group_keys): the
-// columns are cut into panels of S_TILE_N, the rows into tiles of S_TILE_K; each panel's segments
-// are the maximal runs of tiles holding a nonzero (a panel without any gets one tile, which
-// multiplies zeros), and one task per S_TILE_M-row panel of T_D and column panel shares them, in
-// row-panel-major order (the workgroups that run together read one T_D row panel).  Returns the
-// MFMA flops of the tasks, or 0 (no tasks added) when they would do more than `max_frac` of the
-// dense product's.
-constexpr uint32_t S_TILE_K = 16, S_TILE_N = 128, S_TILE_M = 128;
-uint64_t add_sparse_s_tasks(StagePlan& P, const GroupedLowering::Group& g, const double* Wg, double max_frac,
-                            double* visited_frac = nullptr, uint64_t* nnz_out = nullptr) {
-    const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
-    uint64_t nnz = 0;
-    const uint32_t nkt = (Kg + S_TILE_K - 1) / S_TILE_K, npn = (Ng + S_TILE_N - 1) / S_TILE_N;
-    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> runs(npn);   // (k0, K) per panel
-    uint64_t visited = 0;
+// Block-sparse S stage of one group (W_D = Wg, K x N column-major, after reorder_group_keys): the
+// columns are cut into panels of 128, the keys into tiles of 16; each panel keeps the tiles holding
+// a nonzero, stored compactly ([n][k], zero padded) and grouped into runs of consecutive tiles; one
+// task per 128-row panel of T_D and column panel, row-panel major (the workgroups that run together
+// read one T_D row panel).  A panel without any nonzero gets no run: its tasks write zeros.
+// Returns the fraction of W_D the visited tiles cover; nnz_out: the nonzeros of W_D.
+double add_bsr_group(StagePlan::Bsr& B, const GroupedLowering::Group& g, const double* Wg, uint64_t* nnz_out) {
+    const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size()), M = uint32_t(g.M);
+    constexpr uint32_t TK = S_BSR_TILE_K, TN = S_BSR_TILE_N, TM = S_BSR_TILE_M;
+    const uint32_t nkt = (Kg + TK - 1) / TK, npn = (Ng + TN - 1) / TN;
+    uint64_t nnz = 0, visited = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> panel_runs(npn);
     for (uint32_t j = 0; j < npn; ++j) {
-        const uint32_t n0 = j * S_TILE_N, n1 = std::min(Ng, n0 + S_TILE_N);
+        const uint32_t n0 = j * TN, n1 = std::min(Ng, n0 + TN);
         std::vector<char> nz(nkt, 0);
         for (uint32_t n = n0; n < n1; ++n) {
             const double* col = Wg + uint64_t(n) * Kg;
             for (uint32_t k = 0; k < Kg; ++k)
-                if (col[k] != 0.0) nz[k / S_TILE_K] = 1, ++nnz;
+                if (col[k] != 0.0) nz[k / TK] = 1, ++nnz;
         }
-        bool any = false;
+        panel_runs[j].first = uint32_t(B.runs.size());
         for (uint32_t t = 0; t < nkt;) {
-            if (!nz[t]) { ++t; continue; }
+            if (!nz[t]) {
+                ++t;
+                continue;
+            }
             uint32_t t1 = t;
             while (t1 < nkt && nz[t1]) ++t1;
-            const uint32_t k0 = t * S_TILE_K, k1 = std::min(Kg, t1 * S_TILE_K);
-            runs[j].push_back({k0, k1 - k0});
+            const uint32_t k0 = t * TK, k1 = std::min(Kg, t1 * TK);
+            MQ_REQUIRE(B.tiles < (1ull << 32), "too many block-sparse tiles");
+            B.runs.push_back({k0, k1 - k0, uint32_t(B.tiles), 0u});
+            for (uint32_t tt = t; tt < t1; ++tt, ++B.tiles)
+                for (uint32_t n = n0; n < n0 + TN; ++n)
+                    for (uint32_t k = tt * TK; k < tt * TK + TK; ++k)
+                        B.w.push_back(n < n1 && k < Kg ? Wg[uint64_t(n) * Kg + k] : 0.0);
             visited += uint64_t(k1 - k0) * (n1 - n0);
-            any = true;
             t = t1;
         }
-        if (!any) {
-            runs[j].push_back({0u, std::min(Kg, S_TILE_K)});
-            visited += uint64_t(std::min(Kg, S_TILE_K)) * (n1 - n0);
-        }
+        panel_runs[j].second = uint32_t(B.runs.size());
     }
-    if (visited_frac) *visited_frac = double(visited) / (double(Kg) * double(Ng));
-    if (nnz_out) *nnz_out = nnz;
-    if (double(visited) > max_frac * double(Kg) * double(Ng)) return 0;
-    std::vector<std::pair<uint32_t, uint32_t>> seg_range(npn);
-    for (uint32_t j = 0; j < npn; ++j) {
-        const uint32_t n0 = j * S_TILE_N;
-        seg_range[j].first = uint32_t(P.s_segs.size());
-        for (const auto& r : runs[j])
-            P.s_segs.push_back(seg(BASE_WS, g.t_off + uint64_t(r.first) * g.M, uint32_t(g.M), false, BASE_WS,
-                                   g.w_off + uint64_t(n0) * Kg + r.first, Kg, false, r.second));
-        seg_range[j].second = uint32_t(P.s_segs.size());
-    }
-    const uint32_t M = uint32_t(g.M);
-    for (uint32_t m0 = 0; m0 < M; m0 += S_TILE_M) {
-        const uint32_t mr = std::min(S_TILE_M, M - m0);
+    MQ_REQUIRE(g.M < (1ull << 32), "group too large for the block-sparse S stage");
+    for (uint32_t m0 = 0; m0 < M; m0 += TM) {
+        const uint32_t mr = std::min(TM, M - m0);
         for (uint32_t j = 0; j < npn; ++j) {
-            const uint32_t n0 = j * S_TILE_N, nw = std::min(Ng, n0 + S_TILE_N) - n0;
-            add_gemm_range(P.s_tasks, BASE_WS, g.s_off + uint64_t(n0) * g.M, mr, nw, M, seg_range[j].first,
-                           seg_range[j].second, m0);
+            const uint32_t n0 = j * TN, nw = std::min(Ng, n0 + TN) - n0;
+            B.tasks.push_back({g.t_off + m0, g.s_off + uint64_t(n0) * g.M + m0, mr, nw, M, panel_runs[j].first,
+                               panel_runs[j].second, 0u});
         }
     }
-    return 2ull * g.M * visited;
+    B.flops += 2ull * g.M * visited;
+    if (nnz_out) *nnz_out = nnz;
+    return Kg && Ng ? double(visited) / (double(Kg) * double(Ng)) : 0.0;
 }
 
 // The dense W_D go to p->ctx's pinned staging buffer (wd_elems of them, at ws_init_off of the
@@ -2308,15 +2343,15 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         MQ_REQUIRE(3 * Kg < (1ull << 31), "group too large for one library GEMM");
         P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(Ng), uint32_t(Kg), uint32_t(g.M),
                          uint32_t(Kg), uint32_t(g.M), g.bf_off, 0});
-        // FP64 / FP32: block-sparse MFMA tasks where they visit at most half of the dense product
-        if (!adjoint && s_sparse_enabled() && g.M >= S_TILE_M) {
-            double vf = 0;
+        // the block-sparse kernel over the visited tiles of W_D (every group, every precision)
+        if (!adjoint && s_sparse_enabled()) {
             uint64_t nnz = 0;
-            P.lib.back().sparse = add_sparse_s_tasks(P, g, wd + (g.w_off - ws_init_off), 0.5, &vf, &nnz) > 0;
+            const double vf = add_bsr_group(P.bsr, g, wd + (g.w_off - ws_init_off), &nnz);
+            P.lib.back().sparse = true;
+            P.bsr_on = true;
             if (g_trace)
-                std::fprintf(stderr, "[mq] S group M %llu K %zu N %zu: W_D density %.3f, visited tiles %.3f -> %s\n",
-                             (unsigned long long)g.M, g.K.size(), g.N.size(), double(nnz) / double(Kg * Ng), vf,
-                             P.lib.back().sparse ? "block-sparse" : "dense");
+                std::fprintf(stderr, "[mq] S group M %llu K %zu N %zu: W_D density %.3f, visited tiles %.3f\n",
+                             (unsigned long long)g.M, g.K.size(), g.N.size(), double(nnz) / double(Kg * Ng), vf);
         }
     }
     for (auto& L : P.lib) {   // B' regions after the A' regions
@@ -2925,7 +2960,7 @@ int mq_plan_autotune(mq_plan* p, int precision, int* timed) {
         if (library_only) {   // only GEMMs whose algorithm is not cached need operands and timing
             bool any = false;
             for (size_t i = 0; i < P.lib.size() && !any; ++i)
-                any = !(precision == MQ_PREC_F32 && P.lib[i].sparse) && P.lt_prepare(precision == MQ_PREC_F32 ? 0 : 1, i, st) &&
+                any = !P.lib[i].sparse && P.lt_prepare(precision == MQ_PREC_F32 ? 0 : 1, i, st) &&
                       !P.ltg[precision == MQ_PREC_F32 ? 0 : 1][i].chosen;
             if (!any) {
                 if (timed) *timed = 0;
@@ -3106,11 +3141,10 @@ int mq_plan_profile(mq_plan* p, const double* psi, double* sigma, int reps, int 
             ft = gemm_flops(P.t_tasks, P.t_segs);
             ff = gemm_flops(P.f_tasks, P.f_segs);
             P.for_regions([&](const GatherRegion& r) { fg += 2ull * r.R * r.C * (r.t1 - r.t0); });
-            // S stage: the dense library GEMMs, and in FP64 / FP32 the block-sparse tasks in their place
-            const bool sparse_run = precision != MQ_PREC_BF16X3;
+            // S stage: the block-sparse kernel's visited tiles, and the dense library GEMMs of the rest
             for (const auto& g : P.lib)
-                if (!(g.sparse && sparse_run)) fg += 2ull * g.M * g.N * g.K;
-            if (sparse_run) fg += gemm_flops(P.s_tasks, P.s_segs);
+                if (!g.sparse) fg += 2ull * g.M * g.N * g.K;
+            fg += P.bsr.flops;
             flops_out[0] = ft;
             flops_out[1] = fg;
             flops_out[2] = ff;
@@ -3145,7 +3179,7 @@ int mq_plan_workspace_bytes(const mq_plan* p, uint64_t* bytes) {
     return guarded([&] {
         MQ_REQUIRE(p && bytes, "bad argument");
         const StagePlan& P = p->st;
-        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gF.device_bytes() + P.gS.device_bytes() + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_rows.bytes + P.d_regions.bytes +
+        *bytes = p->ws.bytes + p->ws32.bytes + p->part.bytes + P.gT.device_bytes() + P.gF.device_bytes() + P.bsr.d_tasks.bytes + P.bsr.d_runs.bytes + P.bsr.d_w64.bytes + P.bsr.d_w32.bytes + P.bsr.d_whi.bytes + P.bsr.d_wlo.bytes + P.red.device_bytes() + P.d_maps.bytes + P.d_units.bytes + P.d_rows.bytes + P.d_regions.bytes +
                  P.d_small.bytes + P.d_src_off.bytes + P.d_alpha.bytes;
     });
 }

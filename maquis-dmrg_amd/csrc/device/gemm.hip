@@ -79,14 +79,6 @@ __device__ inline TC* c_column(const GemmTask& tk, const Bases& bases, const uin
     return C + uint64_t(gn) * tk.ldc;
 }
 
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), each XCD with its
-// own L2.  Logical tiles are renumbered so that each XCD runs one contiguous range of them:
-// neighbouring tiles (same A row panel, same task) then share an L2.  A bijection on [0, n).
-__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
-    constexpr uint32_t X = 8;
-    const uint32_t q = n / X, r = n % X, x = b % X, i = b / X;
-    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
 
 // Task of a tile: a direct table where the launch has one (many small tasks), else the identity
 // (one tile per task) or a binary search over the tile prefixes.

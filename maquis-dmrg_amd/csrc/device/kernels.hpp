@@ -87,6 +87,14 @@ __device__ inline uint16_t bf16_rne(float x) {
     return uint16_t(u >> 16);
 }
 
+// host form of bf16_rne (the same bits)
+inline uint16_t bf16_rne_host(float x) {
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
 // Zeroing of scattered ranges: chunk c clears elements [off, off + len) of the base array
 // (len <= ZERO_CHUNK).
 struct ZeroChunk {
@@ -123,6 +131,15 @@ struct GatherRegion {
     uint32_t chunk_begin;
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), each XCD with its
+// own L2.  Logical tiles are renumbered so that each XCD runs one contiguous range of them:
+// neighbouring tiles (same A row panel, same task) then share an L2.  A bijection on [0, n).
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    constexpr uint32_t X = 8;
+    const uint32_t q = n / X, r = n % X, x = b % X, i = b / X;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 constexpr uint32_t MAP_NONE = 0xffffffffu;
 // Arithmetic of an application (mq_precision).  The hand-written GEMM runs MODE_F64 and MODE_F32;
 // MODE_BF16X3 applications run their T and final stages in MODE_F32 and the S stage as bf16
@@ -141,6 +158,25 @@ inline int gemm_pick_cfg(uint32_t M, uint32_t N) {
     return GEMM_CFG_64x64;
 }
 constexpr int GEMM_THREADS = 256;
+
+// Block-sparse S stage (s_stage.hip).  A task is one 128 x 128 tile of S_D = T_D W_D of one group:
+// rows [0, M) of the row panel starting at a_off / c_off (T_D and S_D column-major with leading
+// dimension ld, element offsets in the workspace), columns [0, N) of the column panel; its K steps
+// are the runs [run0, run1).  A run covers keys [k0, k0 + nk) (k0 a multiple of 16) whose 16-key
+// tiles of this panel's W_D are stored compactly from tile index `tile` on, each tile 128 x 16 as
+// [n][k] (zero padded).
+constexpr int S_BSR_TILE_M = 128, S_BSR_TILE_N = 128, S_BSR_TILE_K = 16;
+struct BsrTask {
+    uint64_t a_off, c_off;
+    uint32_t M, N, ld, run0, run1, pad;
+};
+struct BsrRun {
+    uint32_t k0, nk, tile, pad;
+};
+// mode MODE_F64: ws, W tiles (bt0) double; MODE_F32: float; MODE_BF16X3: ws float, bt0 / bt1 the
+// bf16 hi / lo tiles
+hipError_t launch_s_bsr(hipStream_t st, int mode, const BsrTask* tasks, uint32_t n_tasks, const BsrRun* runs,
+                        const void* bt0, const void* bt1, void* ws);
 
 // Side tables of a launch: remap entries, column units, row units, block scales.
 struct GemmTables {

@@ -1,0 +1,268 @@
+// Block-sparse S stage of the grouped sigma / boundary steps on gfx950 MFMA:
+//
+//   S_D[:, panel] = T_D[:, K runs] * W_D[K runs, panel]      per charge-transfer group D
+//
+// W_D's keys are in pattern-class order (engine.hip, reorder_group_keys), so its nonzeros sit in
+// dense rectangles; every 128-column panel of W_D keeps only its 16-key tiles that hold a nonzero,
+// stored compactly ([n][16 k] per tile, zero padded) and grouped into runs of consecutive tiles.
+// One workgroup computes one 128 x 128 tile of S_D: it walks the runs of its panel, staging the
+// T_D rows (column-major, rows contiguous) and the compact W tiles through LDS, double buffered,
+// and accumulates with MFMA.  Three arithmetics:
+//
+//   MODE_F64     v_mfma_f64_16x16x4f64 on FP64 T_D / W tiles (the plain path)
+//   MODE_F32     v_mfma_f32_16x16x4f32 on FP32 T_D / W tiles
+//   MODE_BF16X3  FP32 T_D split on the fly into bf16 hi + lo while it is staged, W tiles split once
+//                per plan; three v_mfma_f32_16x16x16bf16 products per block (hi*lo + lo*hi +
+//                hi*hi) with FP32 accumulation, the split-product scheme of the library path
+//
+// The reference's S stage is a gather over the nonzero MPO couplings alpha
+// (solver/numeric/gpu_dgemm_ddot.cu:282-359, solver/tasks.cpp:466-504); here the couplings of one
+// group form W_D and its empty tiles are never visited.  Each output element is written once, by
+// one workgroup, with a fixed summation order: the stage is deterministic.
+//
+// MFMA lane layouts (lane l, lk = l / 16):
+//   f64 16x16x4        A row l%16, k lk;  B col l%16, k lk;  D[r] row 4r + lk, col l%16
+//   f32 16x16x4        A row l%16, k lk;  B col l%16, k lk;  D[r] row 4lk + r, col l%16
+//   bf16 16x16x16      A row l%16, k 4lk..4lk+3;  B col l%16, k 4lk..4lk+3;  D[r] row 4lk + r
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "kernels.hpp"
+
+namespace mq {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = S_BSR_TILE_M, BN = S_BSR_TILE_N, BK = S_BSR_TILE_K, NT = 256;
+static_assert(BM == 128 && BN == 128 && BK == 16, "tile shape");
+
+// hi = bf16(x), lo = bf16(x - hi), round to nearest even (same rule as the T stage's split)
+__device__ inline void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
+    hi = bf16_rne(x);
+    lo = bf16_rne(x - __uint_as_float(uint32_t(hi) << 16));
+}
+
+template <int MODE> struct BsrTraits;
+template <> struct BsrTraits<MODE_F64> { typedef double T; typedef d4 Acc; };
+template <> struct BsrTraits<MODE_F32> { typedef float T; typedef f4 Acc; };
+template <> struct BsrTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; };
+
+// LDS of one stage buffer.  FP64 / FP32: A as [k][m] (T elements), B as [n][k].  BF16X3: A hi and
+// lo as [m][k] bf16 (the MFMA reads 4 consecutive k of one row), B hi and lo as [n][k] bf16.
+template <int MODE> struct BsrSmem;
+template <int MODE> struct BsrSmemPlain {
+    typedef typename BsrTraits<MODE>::T T;
+    T a[BK][BM + 2];
+    T b[BN][BK + 1];
+};
+template <> struct BsrSmem<MODE_F64> : BsrSmemPlain<MODE_F64> {};
+template <> struct BsrSmem<MODE_F32> : BsrSmemPlain<MODE_F32> {};
+template <> struct BsrSmem<MODE_BF16X3> {   // rows of 24 bf16 (48 bytes): 16-byte aligned runs of 8 keys
+    uint16_t ahi[BM][BK + 8], alo[BM][BK + 8];
+    uint16_t bhi[BN][BK + 8], blo[BN][BK + 8];
+};
+
+template <int MODE>
+__global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ tasks, const BsrRun* __restrict__ runs,
+                                                   const void* __restrict__ bt0, const void* __restrict__ bt1,
+                                                   void* __restrict__ ws) {
+    typedef typename BsrTraits<MODE>::T T;
+    typedef typename BsrTraits<MODE>::Acc Acc;
+    constexpr bool BF = MODE == MODE_BF16X3;
+    __shared__ __align__(16) BsrSmem<MODE> sm[2];
+
+    // tiles of one row panel of T_D run next to each other on one XCD (tasks are m-panel major)
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const BsrTask tk = tasks[tile];
+    const T* __restrict__ A = reinterpret_cast<const T*>(ws) + tk.a_off;
+    T* __restrict__ C = reinterpret_cast<T*>(ws) + tk.c_off;
+    const uint32_t M = tk.M, N = tk.N;
+    const uint64_t ld = tk.ld;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;   // 2 x 2 waves of 64 x 64
+    const int l16 = lane & 15, lk = lane >> 4;
+
+    Acc acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+
+    // ---- staging of one 16-key step (run r, key offset kk within the run)
+    // A: T_D rows m0 .. m0+127 of 16 key columns.  Thread t reads row t % 128 of the 8 keys
+    // (t / 128) * 8 .. + 7: every load instruction covers 64 consecutive rows of one column.  Rows
+    // beyond M read valid workspace memory (the plan checks 128 rows past every panel) and are
+    // zeroed; keys beyond the run's end read its first key and are zeroed.
+    // B: the compact tile (128 columns x 16 keys, [n][k], 16-byte aligned runs of 8 keys), thread t
+    // reads column t / 2, keys (t % 2) * 8 .. + 7 with vector loads.
+    const int a_m = tid & (BM - 1), a_k = (tid >> 7) * 8;
+    const int b_n = tid >> 1, b_k = (tid & 1) * 8;
+    const bool m_ok = uint32_t(a_m) < M;
+    T va[8];
+    T vb[8];
+    uint4 vbh, vbl;
+    auto gload = [&](const BsrRun& R, uint32_t kk) {
+        const T* src = A + a_m;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const bool kok = kk + a_k + e < R.nk;
+            va[e] = src[uint64_t(kok ? R.k0 + kk + a_k + e : R.k0) * ld];
+            if (!kok || !m_ok) va[e] = T(0);
+        }
+        const uint64_t toff = (uint64_t(R.tile) + kk / BK) * (BN * BK) + uint64_t(b_n) * BK + b_k;
+        if constexpr (BF) {
+            vbh = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(bt0) + toff);
+            vbl = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(bt1) + toff);
+        } else {
+            typedef typename std::conditional<MODE == MODE_F64, double2, float4>::type V;
+            constexpr int W = sizeof(V) / sizeof(T);
+            const V* bsrc = reinterpret_cast<const V*>(reinterpret_cast<const T*>(bt0) + toff);
+#pragma unroll
+            for (int v = 0; v < 8 / W; ++v) {
+                const V x = bsrc[v];
+                const T* px = reinterpret_cast<const T*>(&x);
+#pragma unroll
+                for (int w = 0; w < W; ++w) vb[v * W + w] = px[w];
+            }
+        }
+    };
+    auto lstore = [&](int buf) {
+        BsrSmem<MODE>& s = sm[buf];
+        if constexpr (BF) {
+            uint32_t h[4], l[4];   // pairs of bf16, the lower key in the low half
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint16_t h0, l0, h1, l1;
+                split_bf16(va[2 * e], h0, l0);
+                split_bf16(va[2 * e + 1], h1, l1);
+                h[e] = uint32_t(h0) | (uint32_t(h1) << 16);
+                l[e] = uint32_t(l0) | (uint32_t(l1) << 16);
+            }
+            *reinterpret_cast<uint4*>(&s.ahi[a_m][a_k]) = uint4{h[0], h[1], h[2], h[3]};
+            *reinterpret_cast<uint4*>(&s.alo[a_m][a_k]) = uint4{l[0], l[1], l[2], l[3]};
+            *reinterpret_cast<uint4*>(&s.bhi[b_n][b_k]) = vbh;
+            *reinterpret_cast<uint4*>(&s.blo[b_n][b_k]) = vbl;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s.a[a_k + e][a_m] = va[e];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s.b[b_n][b_k + e] = vb[e];
+        }
+    };
+    auto compute = [&](int buf) {
+        const BsrSmem<MODE>& s = sm[buf];
+        if constexpr (BF) {
+            s4 ah[4], al[4], bh[4], bl[4];   // 4 consecutive keys of one row / column: one 8-byte read each
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = wm + 16 * i + l16;
+                ah[i] = *reinterpret_cast<const s4*>(&s.ahi[r][4 * lk]);
+                al[i] = *reinterpret_cast<const s4*>(&s.alo[r][4 * lk]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = wn + 16 * j + l16;
+                bh[j] = *reinterpret_cast<const s4*>(&s.bhi[c][4 * lk]);
+                bl[j] = *reinterpret_cast<const s4*>(&s.blo[c][4 * lk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < BK / 4; ++ks) {
+                T a[4], b[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) a[i] = s.a[4 * ks + lk][wm + 16 * i + l16];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = s.b[wn + 16 * j + l16][4 * ks + lk];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if constexpr (MODE == MODE_F64)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+                        else
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+                    }
+            }
+        }
+    };
+
+    // ---- the steps of all runs of the panel: prologue, then compute step t while step t+1 loads
+    uint32_t r = tk.run0, kk = 0;
+    if (r < tk.run1) {
+        BsrRun R = runs[r];
+        gload(R, kk);
+        lstore(0);
+        __syncthreads();
+        int buf = 0;
+        while (true) {
+            // advance to the next step
+            kk += BK;
+            bool more = true;
+            if (kk >= R.nk) {
+                ++r;
+                kk = 0;
+                more = r < tk.run1;
+                if (more) R = runs[r];
+            }
+            if (more) gload(R, kk);
+            compute(buf);
+            if (!more) break;
+            lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+    // ---- epilogue: C = acc (rows < M, columns < N)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t col = wn + 16 * j + l16;
+            if (col >= N) continue;
+            T* cc = C + uint64_t(col) * ld;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t row = MODE == MODE_F64 ? wm + 16 * i + 4 * q + lk : wm + 16 * i + 4 * lk + q;
+                if (row < M) cc[row] = acc[i][j][q];
+            }
+        }
+}
+
+}  // namespace
+
+hipError_t launch_s_bsr(hipStream_t st, int mode, const BsrTask* tasks, uint32_t n_tasks, const BsrRun* runs,
+                        const void* bt0, const void* bt1, void* ws) {
+    if (n_tasks == 0) return hipSuccess;
+    const dim3 grid(n_tasks), block(NT);
+    switch (mode) {
+        case MODE_F64:
+            hipLaunchKernelGGL(s_bsr_kernel<MODE_F64>, grid, block, 0, st, tasks, runs, bt0, bt1, ws);
+            break;
+        case MODE_F32:
+            hipLaunchKernelGGL(s_bsr_kernel<MODE_F32>, grid, block, 0, st, tasks, runs, bt0, bt1, ws);
+            break;
+        case MODE_BF16X3:
+            hipLaunchKernelGGL(s_bsr_kernel<MODE_BF16X3>, grid, block, 0, st, tasks, runs, bt0, bt1, ws);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mq

@@ -440,20 +440,36 @@ def fci_energy(h, eri, n_up, n_down, ecore=0.0):
     return float(np.linalg.eigvalsh(H)[0]) + ecore
 
 
+def hermitian_partner(kind, idx):
+    """The entry whose operator is the Hermitian conjugate of this entry's (real orbitals):
+    h_pq c+_p c_q <-> h_qp, (pq|rs) c+_p c+_r c_s c_q <-> (qp|sr)."""
+    if kind == "h":
+        return (idx[1], idx[0])
+    p, q, r, s = idx
+    return (q, p, s, r)
+
+
 def fci_energy_gradient_fd(h, eri, n_up, n_down, step=1e-5, entries=None):
     """Central finite differences of the FCI energy with respect to single integral entries
-    (each h[p, q] and (pq|rs) entry an independent parameter).  entries: list of ("h", (p, q)) /
-    ("eri", (p, q, r, s)); default all.  Returns {entry: dE/dx}."""
+    (each h[p, q] and (pq|rs) entry an independent parameter).  A single off-diagonal entry alone
+    makes H non-Hermitian, which has no variational lowest eigenvalue; to first order in the step
+    its derivative is psi . (dH/dx) psi, the same for the entry and its Hermitian partner
+    (hermitian_partner), so the entry and its partner are stepped together and the difference
+    halved when they are distinct.  entries: list of ("h", (p, q)) / ("eri", (p, q, r, s));
+    default all.  Returns {entry: dE/dx}."""
     L = h.shape[0]
     if entries is None:
         entries = [("h", ij) for ij in itertools.product(range(L), repeat=2)] + \
                   [("eri", ijkl) for ijkl in itertools.product(range(L), repeat=4)]
     out = {}
     for kind, idx in entries:
+        idx = tuple(int(x) for x in idx)
+        cls = {idx, hermitian_partner(kind, idx)}
         vals = []
         for sgn in (+1, -1):
             hh, ee = h.copy(), eri.copy()
-            (hh if kind == "h" else ee)[idx] += sgn * step
+            for x in cls:
+                (hh if kind == "h" else ee)[x] += sgn * step
             vals.append(fci_energy(hh, ee, n_up, n_down))
-        out[(kind, idx)] = (vals[0] - vals[1]) / (2 * step)
+        out[(kind, idx)] = (vals[0] - vals[1]) / (2 * step) / len(cls)
     return out

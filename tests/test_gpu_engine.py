@@ -396,11 +396,11 @@ def test_jacobi_davidson_gmres_and_ortho_vectors(eng, model):
 
 @pytest.mark.parametrize("model,L,m", [("qc_su2", 24, 1000), ("qc", 20, 512)])
 def test_block_sparse_s_stage_matches_dense_and_oracle(eng, model, L, m, monkeypatch):
-    """Ab initio middle-of-chain shapes: the block-sparse S stage (keys in pattern-class order, MFMA
-    tasks over the nonzero 16-row tiles of W_D, for the groups where those visit at most half of
-    W_D) does fewer flops than the dense library GEMMs of the same plan built with MQ_S_DENSE=1,
-    and its sigma equals the dense plan's to 1e-13 (FP64) and 2e-6 (FP32) and the oracle's on a
-    block sample to 1e-12."""
+    """Ab initio middle-of-chain shapes: the block-sparse S stage (keys in pattern-class order, the
+    s_stage.hip kernel over the nonzero 16-key tiles of every group's W_D) does fewer flops than the
+    dense library GEMMs of the same plan built with MQ_S_DENSE=1; its sigma equals the dense plan's
+    to 1e-13 (FP64), 2e-6 (FP32) and 2e-5 (BF16x3, against the dense FP64 sigma), runs bitwise
+    reproducibly, and equals the oracle's on a block sample to 1e-12."""
     from mqdmrg.workloads import SyntheticSite
     print("case", model, L, m, flush=True)
     ws = SyntheticSite(model, L, L // 2 - 1, m, seed=9)
@@ -420,17 +420,24 @@ def test_block_sparse_s_stage_matches_dense_and_oracle(eng, model, L, m, monkeyp
     assert fl_sparse[1] < 0.9 * fl_dense[1]
     outs = {}
     for name, plan in (("sparse", sparse), ("dense", dense)):
-        for prec in (E.PREC_F64, E.PREC_F32):
+        for prec in (E.PREC_F64, E.PREC_F32, E.PREC_BF16X3):
             out = torch.full_like(psi, np.nan)
             plan.apply(psi, out, precision=prec)
             eng.sync()
             outs[name, prec] = out.cpu().numpy()
+            again = torch.full_like(psi, np.nan)
+            plan.apply(psi, again, precision=prec)
+            eng.sync()
+            assert torch.equal(out, again), (name, prec)
     ref = outs["dense", E.PREC_F64]
     nr = np.linalg.norm(ref)
     e64 = np.linalg.norm(outs["sparse", E.PREC_F64] - ref) / nr
     e32 = np.linalg.norm(outs["sparse", E.PREC_F32] - ref) / nr
-    print("  sparse vs dense: FP64 %.2e FP32 %.2e" % (e64, e32), flush=True)
-    assert e64 <= 1e-13 and e32 <= 2e-6
+    ebf = np.linalg.norm(outs["sparse", E.PREC_BF16X3] - ref) / nr
+    ebf_lib = np.linalg.norm(outs["dense", E.PREC_BF16X3] - ref) / nr
+    print("  sparse vs dense FP64: FP64 %.2e FP32 %.2e BF16x3 %.2e (library BF16x3 %.2e)" % (e64, e32, ebf, ebf_lib),
+          flush=True)
+    assert e64 <= 1e-13 and e32 <= 2e-6 and ebf <= 2e-5
     lay = ws.layout
     nblk = len(lay.left.sizes)
     blocks = sorted(np.random.default_rng(0).choice(nblk, size=min(6, nblk), replace=False).tolist())
