@@ -1828,8 +1828,8 @@ void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* W
 // Block-sparse S stage of one group (W_D = Wg, K x N column-major, after reorder_group_keys): the
 // columns are cut into panels of 128, the keys into tiles of 16; each panel keeps the tiles holding
 // a nonzero, stored compactly ([n][k], zero padded) and grouped into runs of consecutive tiles; one
-// task per 128-row panel of T_D and column panel, row-panel major (the workgroups that run together
-// read one T_D row panel).  A panel without any nonzero gets no run: its tasks write zeros.
+// task per 128-row panel of T_D and column panel, in blocks of 8 x 8 panels (below).  A panel
+// without any nonzero gets no run: its tasks write zeros.
 // Returns the fraction of W_D the visited tiles cover; nnz_out: the nonzeros of W_D.
 double add_bsr_group(StagePlan::Bsr& B, const GroupedLowering::Group& g, const double* Wg, uint64_t* nnz_out) {
     const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size()), M = uint32_t(g.M);
@@ -1866,14 +1866,20 @@ double add_bsr_group(StagePlan::Bsr& B, const GroupedLowering::Group& g, const d
         panel_runs[j].second = uint32_t(B.runs.size());
     }
     MQ_REQUIRE(g.M < (1ull << 32), "group too large for the block-sparse S stage");
-    for (uint32_t m0 = 0; m0 < M; m0 += TM) {
-        const uint32_t mr = std::min(TM, M - m0);
-        for (uint32_t j = 0; j < npn; ++j) {
-            const uint32_t n0 = j * TN, nw = std::min(Ng, n0 + TN) - n0;
-            B.tasks.push_back({g.t_off + m0, g.s_off + uint64_t(n0) * g.M + m0, mr, nw, M, panel_runs[j].first,
-                               panel_runs[j].second, 0u});
-        }
-    }
+    // tasks in blocks of 8 row panels x 8 column panels, row panel fastest inside a block: the ~64
+    // workgroups an XCD runs at once then share 8 T_D row panels and 8 W panels in its L2, instead
+    // of one row panel with every W panel of the group (or the reverse)
+    constexpr uint32_t SB = 8;
+    const uint32_t npm = (M + TM - 1) / TM;
+    for (uint32_t jb = 0; jb < npn; jb += SB)
+        for (uint32_t mb = 0; mb < npm; mb += SB)
+            for (uint32_t j = jb; j < std::min(npn, jb + SB); ++j)
+                for (uint32_t mp = mb; mp < std::min(npm, mb + SB); ++mp) {
+                    const uint32_t m0 = mp * TM, mr = std::min(TM, M - m0);
+                    const uint32_t n0 = j * TN, nw = std::min(Ng, n0 + TN) - n0;
+                    B.tasks.push_back({g.t_off + m0, g.s_off + uint64_t(n0) * g.M + m0, mr, nw, M,
+                                       panel_runs[j].first, panel_runs[j].second, 0u});
+                }
     B.flops += 2ull * g.M * visited;
     if (nnz_out) *nnz_out = nnz;
     return Kg && Ng ? double(visited) / (double(Kg) * double(Ng)) : 0.0;
@@ -2659,7 +2665,9 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
             }
         }
     }
-    P.ws_elems = ws;
+    // the block-sparse S stage reads 128 rows of every key column it visits (the rows of a short
+    // panel past M are zeroed after loading): room for those past the last region
+    P.ws_elems = ws + (P.bsr_on ? S_BSR_TILE_M : 0);
     if (!sigma) return;
     // T and L reference flops now; the S part (a scan of W_D per pair) when first asked for
     p->flops = 0;

@@ -62,9 +62,15 @@ template <int MODE> struct BsrSmemPlain {
 };
 template <> struct BsrSmem<MODE_F64> : BsrSmemPlain<MODE_F64> {};
 template <> struct BsrSmem<MODE_F32> : BsrSmemPlain<MODE_F32> {};
-template <> struct BsrSmem<MODE_BF16X3> {   // rows of 24 bf16 (48 bytes): 16-byte aligned runs of 8 keys
-    uint16_t ahi[BM][BK + 8], alo[BM][BK + 8];
-    uint16_t bhi[BN][BK + 8], blo[BN][BK + 8];
+// BF16X3 steps over two 16-key tiles (32 keys): its MFMAs are 16x faster per key than FP32's, and
+// one tile per step left too little work between two waits on global memory.  Rows of 40 bf16
+// (80 bytes): 16-byte aligned runs of 8 keys.
+template <int MODE> struct BsrStep { static constexpr int KS = BK; };
+template <> struct BsrStep<MODE_BF16X3> { static constexpr int KS = 2 * BK; };
+template <> struct BsrSmem<MODE_BF16X3> {
+    static constexpr int P = 2 * BK + 8;
+    uint16_t ahi[BM][P], alo[BM][P];
+    uint16_t bhi[BN][P], blo[BN][P];
 };
 
 template <int MODE>
@@ -76,7 +82,7 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
     constexpr bool BF = MODE == MODE_BF16X3;
     __shared__ __align__(16) BsrSmem<MODE> sm[2];
 
-    // tiles of one row panel of T_D run next to each other on one XCD (tasks are m-panel major)
+    // consecutive tasks (blocks of 8 x 8 panels of one group) run next to each other on one XCD
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const BsrTask tk = tasks[tile];
     const T* __restrict__ A = reinterpret_cast<const T*>(ws) + tk.a_off;
@@ -94,32 +100,42 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-    // ---- staging of one 16-key step (run r, key offset kk within the run)
-    // A: T_D rows m0 .. m0+127 of 16 key columns.  Thread t reads row t % 128 of the 8 keys
-    // (t / 128) * 8 .. + 7: every load instruction covers 64 consecutive rows of one column.  Rows
+    // ---- staging of one step of KS keys (run r, key offset kk within the run)
+    // A: T_D rows m0 .. m0+127 of KS key columns.  Thread t reads row t % 128 of the KS / 2 keys
+    // (t / 128) * KS / 2 .. : every load instruction covers 64 consecutive rows of one column.  Rows
     // beyond M read valid workspace memory (the plan checks 128 rows past every panel) and are
     // zeroed; keys beyond the run's end read its first key and are zeroed.
-    // B: the compact tile (128 columns x 16 keys, [n][k], 16-byte aligned runs of 8 keys), thread t
-    // reads column t / 2, keys (t % 2) * 8 .. + 7 with vector loads.
-    const int a_m = tid & (BM - 1), a_k = (tid >> 7) * 8;
-    const int b_n = tid >> 1, b_k = (tid & 1) * 8;
+    // B: the compact tile(s) (128 columns x 16 keys each, [n][k], 16-byte aligned runs of 8 keys),
+    // thread t reads column t / 2: 8 keys of the one tile (KS = 16), or the 16 keys of tile t % 2
+    // (KS = 32; a tile past the run's end is not read and counts as zero).
+    constexpr int KS = BsrStep<MODE>::KS, AK = KS / 2;
+    const int a_m = tid & (BM - 1), a_k = (tid >> 7) * AK;
+    const int b_n = tid >> 1, b_k = (tid & 1) * (KS / 2);
     const bool m_ok = uint32_t(a_m) < M;
-    T va[8];
+    T va[AK];
     T vb[8];
-    uint4 vbh, vbl;
+    uint4 vbh[KS / 16], vbl[KS / 16];
     auto gload = [&](const BsrRun& R, uint32_t kk) {
         const T* src = A + a_m;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < AK; ++e) {
             const bool kok = kk + a_k + e < R.nk;
             va[e] = src[uint64_t(kok ? R.k0 + kk + a_k + e : R.k0) * ld];
             if (!kok || !m_ok) va[e] = T(0);
         }
-        const uint64_t toff = (uint64_t(R.tile) + kk / BK) * (BN * BK) + uint64_t(b_n) * BK + b_k;
         if constexpr (BF) {
-            vbh = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(bt0) + toff);
-            vbl = *reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(bt1) + toff);
+            // KS = 32: tile kk / 16 + (t % 2), its 16 keys of column t / 2
+            const bool tok = kk + uint32_t(b_k) < R.nk;
+            const uint64_t toff = (uint64_t(R.tile) + (kk + b_k) / BK) * (BN * BK) + uint64_t(b_n) * BK;
+            const uint4* hi = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(bt0) + toff);
+            const uint4* lo = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(bt1) + toff);
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                vbh[v] = tok ? hi[v] : uint4{0, 0, 0, 0};
+                vbl[v] = tok ? lo[v] : uint4{0, 0, 0, 0};
+            }
         } else {
+            const uint64_t toff = (uint64_t(R.tile) + kk / BK) * (BN * BK) + uint64_t(b_n) * BK + b_k;
             typedef typename std::conditional<MODE == MODE_F64, double2, float4>::type V;
             constexpr int W = sizeof(V) / sizeof(T);
             const V* bsrc = reinterpret_cast<const V*>(reinterpret_cast<const T*>(bt0) + toff);
@@ -135,19 +151,25 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
     auto lstore = [&](int buf) {
         BsrSmem<MODE>& s = sm[buf];
         if constexpr (BF) {
-            uint32_t h[4], l[4];   // pairs of bf16, the lower key in the low half
+            uint32_t h[AK / 2], l[AK / 2];   // pairs of bf16, the lower key in the low half
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < AK / 2; ++e) {
                 uint16_t h0, l0, h1, l1;
                 split_bf16(va[2 * e], h0, l0);
                 split_bf16(va[2 * e + 1], h1, l1);
                 h[e] = uint32_t(h0) | (uint32_t(h1) << 16);
                 l[e] = uint32_t(l0) | (uint32_t(l1) << 16);
             }
-            *reinterpret_cast<uint4*>(&s.ahi[a_m][a_k]) = uint4{h[0], h[1], h[2], h[3]};
-            *reinterpret_cast<uint4*>(&s.alo[a_m][a_k]) = uint4{l[0], l[1], l[2], l[3]};
-            *reinterpret_cast<uint4*>(&s.bhi[b_n][b_k]) = vbh;
-            *reinterpret_cast<uint4*>(&s.blo[b_n][b_k]) = vbl;
+#pragma unroll
+            for (int v = 0; v < AK / 8; ++v) {
+                *reinterpret_cast<uint4*>(&s.ahi[a_m][a_k + 8 * v]) = uint4{h[4 * v], h[4 * v + 1], h[4 * v + 2], h[4 * v + 3]};
+                *reinterpret_cast<uint4*>(&s.alo[a_m][a_k + 8 * v]) = uint4{l[4 * v], l[4 * v + 1], l[4 * v + 2], l[4 * v + 3]};
+            }
+#pragma unroll
+            for (int v = 0; v < KS / 16; ++v) {
+                *reinterpret_cast<uint4*>(&s.bhi[b_n][b_k + 8 * v]) = vbh[v];
+                *reinterpret_cast<uint4*>(&s.blo[b_n][b_k + 8 * v]) = vbl[v];
+            }
         } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e) s.a[a_k + e][a_m] = va[e];
@@ -158,27 +180,30 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
     auto compute = [&](int buf) {
         const BsrSmem<MODE>& s = sm[buf];
         if constexpr (BF) {
-            s4 ah[4], al[4], bh[4], bl[4];   // 4 consecutive keys of one row / column: one 8-byte read each
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int r = wm + 16 * i + l16;
-                ah[i] = *reinterpret_cast<const s4*>(&s.ahi[r][4 * lk]);
-                al[i] = *reinterpret_cast<const s4*>(&s.alo[r][4 * lk]);
-            }
+            for (int ks = 0; ks < KS / 16; ++ks) {
+                s4 ah[4], al[4], bh[4], bl[4];   // 4 consecutive keys of one row / column: one 8-byte read each
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int c = wn + 16 * j + l16;
-                bh[j] = *reinterpret_cast<const s4*>(&s.bhi[c][4 * lk]);
-                bl[j] = *reinterpret_cast<const s4*>(&s.blo[c][4 * lk]);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < 4; ++i) {
+                    const int r = wm + 16 * i + l16;
+                    ah[i] = *reinterpret_cast<const s4*>(&s.ahi[r][16 * ks + 4 * lk]);
+                    al[i] = *reinterpret_cast<const s4*>(&s.alo[r][16 * ks + 4 * lk]);
+                }
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                    const int c = wn + 16 * j + l16;
+                    bh[j] = *reinterpret_cast<const s4*>(&s.bhi[c][16 * ks + 4 * lk]);
+                    bl[j] = *reinterpret_cast<const s4*>(&s.blo[c][16 * ks + 4 * lk]);
                 }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                    }
+            }
         } else {
 #pragma unroll
             for (int ks = 0; ks < BK / 4; ++ks) {
@@ -210,7 +235,7 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
         int buf = 0;
         while (true) {
             // advance to the next step
-            kk += BK;
+            kk += KS;
             bool more = true;
             if (kk >= R.nk) {
                 ++r;
