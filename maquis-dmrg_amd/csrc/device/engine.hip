@@ -863,23 +863,16 @@ struct StagePlan {
         if (bsr_on && !bsr.w.empty()) {   // once: the host copy of the tiles is released after it
             upload(bsr.d_tasks, bsr.tasks, st);
             upload(bsr.d_runs, bsr.runs, st);
-            // the W tiles in the three arithmetics: FP64, FP32, and the bf16 split of the FP32 value
-            std::vector<float> w32(bsr.w.size());
-            std::vector<uint16_t> whi(bsr.w.size()), wlo(bsr.w.size());
-            for (size_t i = 0; i < bsr.w.size(); ++i) {
-                const float x = float(bsr.w[i]);
-                w32[i] = x;
-                whi[i] = bf16_rne_host(x);
-                float h;
-                const uint32_t hb = uint32_t(whi[i]) << 16;
-                std::memcpy(&h, &hb, 4);
-                wlo[i] = bf16_rne_host(x - h);
-            }
+            // the W tiles in the three arithmetics: FP64 uploaded; FP32 and the bf16 split of the FP32
+            // value converted on the device
+            const uint64_t nw = bsr.w.size();
             upload(bsr.d_w64, bsr.w, st);
-            upload(bsr.d_w32, w32, st);
-            upload(bsr.d_whi, whi, st);
-            upload(bsr.d_wlo, wlo, st);
-            HIPCHK(hipStreamSynchronize(st));   // the host vectors are released below
+            bsr.d_w32.alloc(std::max<uint64_t>(nw, 1) * sizeof(float));
+            bsr.d_whi.alloc(std::max<uint64_t>(nw, 1) * sizeof(uint16_t));
+            bsr.d_wlo.alloc(std::max<uint64_t>(nw, 1) * sizeof(uint16_t));
+            HIPCHK(launch_split_tiles(st, bsr.d_w64.as<double>(), nw, bsr.d_w32.as<float>(), bsr.d_whi.as<uint16_t>(),
+                                      bsr.d_wlo.as<uint16_t>()));
+            HIPCHK(hipStreamSynchronize(st));   // the host copy is released below
             std::vector<double>().swap(bsr.w);
         }
         upload(d_regions, regions, st);
@@ -1825,64 +1818,117 @@ void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* W
     }
 }
 
-// Block-sparse S stage of one group (W_D = Wg, K x N column-major, after reorder_group_keys): the
+// Block-sparse S stage (W_D of each group, K x N column-major, after reorder_group_keys): the
 // columns are cut into panels of 128, the keys into tiles of 16; each panel keeps the tiles holding
 // a nonzero, stored compactly ([n][k], zero padded) and grouped into runs of consecutive tiles; one
 // task per 128-row panel of T_D and column panel, in blocks of 8 x 8 panels (below).  A panel
-// without any nonzero gets no run: its tasks write zeros.
-// Returns the fraction of W_D the visited tiles cover; nnz_out: the nonzeros of W_D.
-double add_bsr_group(StagePlan::Bsr& B, const GroupedLowering::Group& g, const double* Wg, uint64_t* nnz_out) {
-    const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size()), M = uint32_t(g.M);
+// without any nonzero gets no run: its tasks write zeros.  Groups are scanned and their tiles
+// copied in parallel (each group owns a contiguous tile range); tasks and runs are appended in group
+// order.  info receives per group (W_D nonzeros, visited fraction).
+struct BsrGroupScan {
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> panel_runs;   // (first key tile, tiles) per panel
+    uint64_t tiles = 0, nnz = 0, visited = 0;
+};
+void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>& groups, const double* wd,
+                    uint64_t wd_off, std::vector<std::pair<uint64_t, double>>* info) {
     constexpr uint32_t TK = S_BSR_TILE_K, TN = S_BSR_TILE_N, TM = S_BSR_TILE_M;
-    const uint32_t nkt = (Kg + TK - 1) / TK, npn = (Ng + TN - 1) / TN;
-    uint64_t nnz = 0, visited = 0;
-    std::vector<std::pair<uint32_t, uint32_t>> panel_runs(npn);
-    for (uint32_t j = 0; j < npn; ++j) {
-        const uint32_t n0 = j * TN, n1 = std::min(Ng, n0 + TN);
-        std::vector<char> nz(nkt, 0);
-        for (uint32_t n = n0; n < n1; ++n) {
-            const double* col = Wg + uint64_t(n) * Kg;
-            for (uint32_t k = 0; k < Kg; ++k)
-                if (col[k] != 0.0) nz[k / TK] = 1, ++nnz;
-        }
-        panel_runs[j].first = uint32_t(B.runs.size());
-        for (uint32_t t = 0; t < nkt;) {
-            if (!nz[t]) {
-                ++t;
-                continue;
+    const size_t ng = groups.size();
+    std::vector<BsrGroupScan> sc(ng);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int gi = 0; gi < int(ng); ++gi) {
+        const auto& g = groups[gi];
+        if (!g.M || g.K.empty() || g.N.empty()) continue;
+        const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
+        const double* Wg = wd + (g.w_off - wd_off);
+        const uint32_t nkt = (Kg + TK - 1) / TK, npn = (Ng + TN - 1) / TN;
+        BsrGroupScan& S = sc[gi];
+        S.panel_runs.resize(npn);
+        std::vector<char> nz(nkt);
+        for (uint32_t j = 0; j < npn; ++j) {
+            const uint32_t n0 = j * TN, n1 = std::min(Ng, n0 + TN);
+            std::fill(nz.begin(), nz.end(), 0);
+            for (uint32_t n = n0; n < n1; ++n) {
+                const double* col = Wg + uint64_t(n) * Kg;
+                for (uint32_t k = 0; k < Kg; ++k)
+                    if (col[k] != 0.0) nz[k / TK] = 1, ++S.nnz;
             }
-            uint32_t t1 = t;
-            while (t1 < nkt && nz[t1]) ++t1;
-            const uint32_t k0 = t * TK, k1 = std::min(Kg, t1 * TK);
-            MQ_REQUIRE(B.tiles < (1ull << 32), "too many block-sparse tiles");
-            B.runs.push_back({k0, k1 - k0, uint32_t(B.tiles), 0u});
-            for (uint32_t tt = t; tt < t1; ++tt, ++B.tiles)
-                for (uint32_t n = n0; n < n0 + TN; ++n)
-                    for (uint32_t k = tt * TK; k < tt * TK + TK; ++k)
-                        B.w.push_back(n < n1 && k < Kg ? Wg[uint64_t(n) * Kg + k] : 0.0);
-            visited += uint64_t(k1 - k0) * (n1 - n0);
-            t = t1;
-        }
-        panel_runs[j].second = uint32_t(B.runs.size());
-    }
-    MQ_REQUIRE(g.M < (1ull << 32), "group too large for the block-sparse S stage");
-    // tasks in blocks of 8 row panels x 8 column panels, row panel fastest inside a block: the ~64
-    // workgroups an XCD runs at once then share 8 T_D row panels and 8 W panels in its L2, instead
-    // of one row panel with every W panel of the group (or the reverse)
-    constexpr uint32_t SB = 8;
-    const uint32_t npm = (M + TM - 1) / TM;
-    for (uint32_t jb = 0; jb < npn; jb += SB)
-        for (uint32_t mb = 0; mb < npm; mb += SB)
-            for (uint32_t j = jb; j < std::min(npn, jb + SB); ++j)
-                for (uint32_t mp = mb; mp < std::min(npm, mb + SB); ++mp) {
-                    const uint32_t m0 = mp * TM, mr = std::min(TM, M - m0);
-                    const uint32_t n0 = j * TN, nw = std::min(Ng, n0 + TN) - n0;
-                    B.tasks.push_back({g.t_off + m0, g.s_off + uint64_t(n0) * g.M + m0, mr, nw, M,
-                                       panel_runs[j].first, panel_runs[j].second, 0u});
+            for (uint32_t t = 0; t < nkt;) {
+                if (!nz[t]) {
+                    ++t;
+                    continue;
                 }
-    B.flops += 2ull * g.M * visited;
-    if (nnz_out) *nnz_out = nnz;
-    return Kg && Ng ? double(visited) / (double(Kg) * double(Ng)) : 0.0;
+                uint32_t t1 = t;
+                while (t1 < nkt && nz[t1]) ++t1;
+                S.panel_runs[j].push_back({t, t1 - t});
+                S.tiles += t1 - t;
+                S.visited += uint64_t(std::min(Kg, t1 * TK) - t * TK) * (n1 - n0);
+                t = t1;
+            }
+        }
+    }
+    // tile ranges, then the tiles themselves in parallel
+    std::vector<uint64_t> tile0(ng + 1, B.tiles);
+    for (size_t gi = 0; gi < ng; ++gi) tile0[gi + 1] = tile0[gi] + sc[gi].tiles;
+    MQ_REQUIRE(tile0[ng] < (1ull << 32), "too many block-sparse tiles");
+    const uint64_t w0 = B.w.size();
+    B.w.resize(w0 + (tile0[ng] - B.tiles) * uint64_t(TN) * TK);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int gi = 0; gi < int(ng); ++gi) {
+        const auto& g = groups[gi];
+        const BsrGroupScan& S = sc[gi];
+        if (!S.tiles) continue;
+        const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
+        const double* Wg = wd + (g.w_off - wd_off);
+        double* out = B.w.data() + w0 + (tile0[gi] - B.tiles) * uint64_t(TN) * TK;
+        for (uint32_t j = 0; j < S.panel_runs.size(); ++j) {
+            const uint32_t n0 = j * TN, n1 = std::min(Ng, n0 + TN);
+            for (const auto& r : S.panel_runs[j])
+                for (uint32_t t = r.first; t < r.first + r.second; ++t, out += uint64_t(TN) * TK)
+                    for (uint32_t n = 0; n < TN; ++n)
+                        for (uint32_t k = 0; k < TK; ++k) {
+                            const uint32_t kk = t * TK + k;
+                            out[n * TK + k] = (n0 + n < n1 && kk < Kg) ? Wg[uint64_t(n0 + n) * Kg + kk] : 0.0;
+                        }
+        }
+    }
+    // runs and tasks in group order
+    for (size_t gi = 0; gi < ng; ++gi) {
+        const auto& g = groups[gi];
+        const BsrGroupScan& S = sc[gi];
+        if (info) info->push_back({S.nnz, g.K.empty() || g.N.empty() ? 0.0 :
+                                              double(S.visited) / (double(g.K.size()) * double(g.N.size()))});
+        if (!g.M || g.K.empty() || g.N.empty()) continue;
+        MQ_REQUIRE(g.M < (1ull << 32), "group too large for the block-sparse S stage");
+        const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size()), M = uint32_t(g.M);
+        const uint32_t npn = uint32_t(S.panel_runs.size());
+        uint64_t tile = tile0[gi];
+        std::vector<std::pair<uint32_t, uint32_t>> pr(npn);
+        for (uint32_t j = 0; j < npn; ++j) {
+            pr[j].first = uint32_t(B.runs.size());
+            for (const auto& r : S.panel_runs[j]) {
+                const uint32_t k0 = r.first * TK, k1 = std::min(Kg, (r.first + r.second) * TK);
+                B.runs.push_back({k0, k1 - k0, uint32_t(tile), 0u});
+                tile += r.second;
+            }
+            pr[j].second = uint32_t(B.runs.size());
+        }
+        // tasks in blocks of 8 row panels x 8 column panels, row panel fastest inside a block: the ~64
+        // workgroups an XCD runs at once then share 8 T_D row panels and 8 W panels in its L2, instead
+        // of one row panel with every W panel of the group (or the reverse)
+        constexpr uint32_t SB = 8;
+        const uint32_t npm = (M + TM - 1) / TM;
+        for (uint32_t jb = 0; jb < npn; jb += SB)
+            for (uint32_t mb = 0; mb < npm; mb += SB)
+                for (uint32_t j = jb; j < std::min(npn, jb + SB); ++j)
+                    for (uint32_t mp = mb; mp < std::min(npm, mb + SB); ++mp) {
+                        const uint32_t m0 = mp * TM, mr = std::min(TM, M - m0);
+                        const uint32_t n0 = j * TN, nw = std::min(Ng, n0 + TN) - n0;
+                        B.tasks.push_back({g.t_off + m0, g.s_off + uint64_t(n0) * g.M + m0, mr, nw, M, pr[j].first,
+                                           pr[j].second, 0u});
+                    }
+        B.flops += 2ull * g.M * S.visited;
+    }
+    B.tiles = tile0[ng];
 }
 
 // The dense W_D go to p->ctx's pinned staging buffer (wd_elems of them, at ws_init_off of the
@@ -2349,16 +2395,21 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         MQ_REQUIRE(3 * Kg < (1ull << 31), "group too large for one library GEMM");
         P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(Ng), uint32_t(Kg), uint32_t(g.M),
                          uint32_t(Kg), uint32_t(g.M), g.bf_off, 0});
-        // the block-sparse kernel over the visited tiles of W_D (every group, every precision)
-        if (!adjoint && s_sparse_enabled()) {
-            uint64_t nnz = 0;
-            const double vf = add_bsr_group(P.bsr, g, wd + (g.w_off - ws_init_off), &nnz);
-            P.lib.back().sparse = true;
-            P.bsr_on = true;
-            if (g_trace)
-                std::fprintf(stderr, "[mq] S group M %llu K %zu N %zu: W_D density %.3f, visited tiles %.3f\n",
-                             (unsigned long long)g.M, g.K.size(), g.N.size(), double(nnz) / double(Kg * Ng), vf);
+        // the block-sparse kernel runs every group (below)
+        P.lib.back().sparse = !adjoint && s_sparse_enabled();
+    }
+    if (!adjoint && s_sparse_enabled()) {   // the block-sparse kernel over the visited tiles of W_D
+        std::vector<std::pair<uint64_t, double>> info;
+        add_bsr_groups(P.bsr, G.groups, wd, ws_init_off, g_trace ? &info : nullptr);
+        P.bsr_on = true;
+        for (size_t gi = 0; gi < info.size(); ++gi) {
+            const auto& g = G.groups[gi];
+            const double dense = double(g.K.size()) * double(g.N.size());
+            std::fprintf(stderr, "[mq] S group M %llu K %zu N %zu: W_D density %.3f, visited tiles %.3f\n",
+                         (unsigned long long)g.M, g.K.size(), g.N.size(), dense ? double(info[gi].first) / dense : 0.0,
+                         info[gi].second);
         }
+        trace_phase("lower: S tiles");
     }
     for (auto& L : P.lib) {   // B' regions after the A' regions
         L.bfb_off = P.bf_elems;

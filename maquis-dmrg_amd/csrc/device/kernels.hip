@@ -211,6 +211,24 @@ hipError_t launch_split_w_bf16(hipStream_t st, const double* W, uint64_t ldw, ui
     return hipGetLastError();
 }
 
+// W tiles of the block-sparse S stage: FP32 value and its bf16 split hi + lo (same rounding as
+// the T stage's and the staging of T_D)
+__global__ void split_tiles_kernel(const double* __restrict__ w, uint64_t n, float* __restrict__ w32,
+                                   uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const float v = float(w[i]);
+        const uint16_t h = bf16_rne(v);
+        w32[i] = v;
+        hi[i] = h;
+        lo[i] = bf16_rne(v - __uint_as_float(uint32_t(h) << 16));
+    }
+}
+hipError_t launch_split_tiles(hipStream_t st, const double* w, uint64_t n, float* w32, uint16_t* hi, uint16_t* lo) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(split_tiles_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, w, n, w32, hi, lo);
+    return hipGetLastError();
+}
+
 __global__ void axpy_kernel(uint64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         y[i] += a * x[i];

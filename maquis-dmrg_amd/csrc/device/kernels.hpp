@@ -87,14 +87,6 @@ __device__ inline uint16_t bf16_rne(float x) {
     return uint16_t(u >> 16);
 }
 
-// host form of bf16_rne (the same bits)
-inline uint16_t bf16_rne_host(float x) {
-    uint32_t u;
-    __builtin_memcpy(&u, &x, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-
 // Zeroing of scattered ranges: chunk c clears elements [off, off + len) of the base array
 // (len <= ZERO_CHUNK).
 struct ZeroChunk {
@@ -218,6 +210,8 @@ hipError_t launch_to_float(hipStream_t st, const double* src, float* dst, uint64
 // B' = [W_lo; W_hi; W_hi] (3K x N, column-major, ld 3K) of W (K x N, column stride ldw), with
 // hi = bf16(float(w)), lo = bf16(float(w) - hi)
 hipError_t launch_split_w_bf16(hipStream_t st, const double* W, uint64_t ldw, uint64_t K, uint64_t N, uint16_t* dst);
+// n W-tile elements: FP32 value and bf16 split hi / lo
+hipError_t launch_split_tiles(hipStream_t st, const double* w, uint64_t n, float* w32, uint16_t* hi, uint16_t* lo);
 hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial,
                       double* result);
 hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y);
