@@ -297,10 +297,154 @@ def split_factors(mats, keys, direction, device, Mmax, cutoff):
     return fac, keeps, svals, tw, "svd"
 
 
+def _stored_cohorts(bnd):
+    """(ci, n_blocks, bra position, ket position) of the cohorts that hold blocks.  A step over an
+    MPO without Hermitian pairs (plain_mpo_desc) stores every block itself, with no transposed or
+    conjugate-scaled references, which the noise terms below rely on."""
+    idx = bnd.index
+    if idx.tr.any() or idx.transposes.any():
+        raise RuntimeError("noise term: boundary with transposed block references (expected a step over "
+                           "plain_mpo_desc)")
+    out = []
+    for ci in range(idx.n_cohorts):
+        nb = int(idx.n_blocks[ci])
+        if nb:
+            out.append((ci, nb, int(idx.bra_pos[ci]), int(idx.ket_pos[ci])))
+    return out
+
+
+def plain_mpo_desc(W):
+    """Descriptor of MPO tensor W without its Hermitian bond pairs: a boundary step over it computes
+    every block itself, with no transposed references to partner blocks (the reference's
+    complement_transpose, boundary.h:161-191, pairs blocks of bra == ket steps only)."""
+    import copy
+    W2 = copy.copy(W)
+    W2.left_herm, W2.right_herm = [None] * W.row_dim, [None] * W.col_dim
+    return E.MpoDesc(W2)
+
+
+def noise_density_left(eng, sym, left, phys, mids, mats, mpo_desc, Lb):
+    """The noise term of alpha_dm_direct (boundary_ops/move_boundary.hpp:38-92 with Cohort::lbtm,
+    solver/tasks.cpp:346-361) for a left-to-right split, without the factor alpha:
+
+        N_c = sum_b' X'_b' X'_b'^T,   X'_b' = sum_b L_b W_b,b' X    (rows (s1, l) of mid charge c)
+
+    X = mats[c] is the two-site tensor of mid charge c as the left-paired matrix of the left site
+    (rows (s1, l), columns (s2, r)), W the left site's MPO tensor (mpo_desc: plain_mpo_desc, bra and
+    ket differ) and L its left boundary: the S of a
+    left boundary step before the bra contraction.  It is computed as a grouped left step on the
+    device whose bra is the identity: the step's site has the mid charges with sizes
+    D_c = max(rows, columns), the bra's left-paired block is [I | 0] (rows x D_c) and the ket's
+    [X | 0], so that the new boundary's blocks are L'_b'(c, c') = [X'_b'(c, c') 0; 0 0] and N_c sums
+    L'[:rows] L'[:rows]^T over the blocks of the cohorts with bra charge c.  Returns {c: N_c}."""
+    f64, dev = torch.float64, eng.torch_device
+    rows = {c: mats[c].shape[0] for c in mids}
+    pad = Index(mids, [max(mats[c].shape[0], mats[c].shape[1], 1) for c in mids])
+    ket = SiteTensor(eng, sym, left, phys, pad)
+    lay = ket.layout
+    ket_lp = torch.zeros(max(lay.lp_total, 1), dtype=f64, device=dev)
+    bra_lp = torch.zeros_like(ket_lp)
+    for mb, c in enumerate(pad.charges):
+        n, D = lay.lp_rows[mb], pad.sizes[mb]
+        if n != rows[c]:
+            raise RuntimeError("noise term: left-paired rows of mid sector %s: %d, expected %d" % (c, n, rows[c]))
+        if n == 0:
+            continue
+        o = lay.lp_offset[mb]
+        ket_lp[o:o + n * D].view(D, n).t()[:, :mats[c].shape[1]] = mats[c]
+        bra_lp[o:o + n * D].view(D, n).t()[:, :n] = torch.eye(n, dtype=f64, device=dev)
+    E.reshape_left_to_right(eng, ket.desc, ket_lp, ket.data)
+    Ln = E.left_step(eng, ket.desc, mpo_desc, bra_lp, ket.data, Lb, symmetric=False)
+    out = {c: torch.zeros(rows[c], rows[c], dtype=f64, device=dev) for c in mids}
+    idx = Ln.index
+    for ci, nb, bp, kp in _stored_cohorts(Ln):
+        c = idx.bra.charges[bp]
+        n = rows.get(c, 0)
+        if n == 0:
+            continue
+        Db, Dk = idx.bra.sizes[bp], idx.ket.sizes[kp]
+        # block b is Db x Dk column-major: as (Dk, Db) row-major it is the transpose
+        blk = Ln.cohort(ci).view(nb * Dk, Db)[:, :n]
+        out[c] += blk.t() @ blk
+    Ln.close()
+    return out
+
+
+def noise_density_right(eng, sym, right, phys, mids, mats, mpo_desc, Rb):
+    """The noise term of alpha_dm_direct_right (boundary_ops/move_boundary.hpp:94-145 with
+    Cohort::rbtm, solver/tasks.cpp:363-378) for a right-to-left split, without the factor alpha:
+
+        N_c = sum_b X'_b^T X'_b,   X'_b = sum_b' X W_b,b' R_b'    (columns (s2, r) of mid charge c)
+
+    X = mats[c] as in noise_density_left (rows (s1, l), columns (s2, r): the right-paired matrix of
+    the right site with the left index (s1, l)), W the right site's MPO tensor (plain_mpo_desc) and
+    R its right boundary.  A grouped right step with the identity as bra: site left index of mid charges with
+    sizes D_c, bra block [I; 0] (D_c x columns), ket block [X; 0]; the new boundary's block b of
+    cohort (c', c) has the ket side (X' rows) as rows and the bra side (columns (s2, r)) as columns,
+    and N_c sums L'[:, :cols]^T L'[:, :cols] over the cohorts whose column charge is c."""
+    f64, dev = torch.float64, eng.torch_device
+    cols = {c: mats[c].shape[1] for c in mids}
+    pad = Index(mids, [max(mats[c].shape[0], mats[c].shape[1], 1) for c in mids])
+    ket = SiteTensor(eng, sym, pad, phys, right)
+    bra = SiteTensor(eng, sym, pad, phys, right)
+    lay = ket.layout
+    for mb, c in enumerate(pad.charges):
+        if lay.rp_cols[mb] != cols[c]:
+            raise RuntimeError("noise term: right-paired columns of mid sector %s: %d, expected %d"
+                               % (c, lay.rp_cols[mb], cols[c]))
+        if cols[c] == 0:
+            continue
+        ket.rp_block(mb)[:mats[c].shape[0], :] = mats[c]
+        bra.rp_block(mb)[:cols[c], :] = torch.eye(cols[c], dtype=f64, device=dev)
+    Rn = E.right_step(eng, ket.desc, mpo_desc, bra.data, ket.data, Rb, symmetric=False)
+    out = {c: torch.zeros(cols[c], cols[c], dtype=f64, device=dev) for c in mids}
+    idx = Rn.index
+    for ci, nb, bp, kp in _stored_cohorts(Rn):
+        c = idx.ket.charges[kp]
+        n = cols.get(c, 0)
+        if n == 0:
+            continue
+        Db, Dk = idx.bra.sizes[bp], idx.ket.sizes[kp]
+        # (Dk, Db) row-major per block: the block's transpose; its first n rows are columns < n
+        blk = Rn.cohort(ci).view(nb, Dk, Db)[:, :n, :].permute(1, 0, 2).reshape(n, nb * Db)
+        out[c] += blk @ blk.t()
+    Rn.close()
+    return out
+
+
+def dm_factors(mats, keys, direction, device, Mmax, cutoff, noise, alpha):
+    """heev_truncate (block_matrix_algorithms.h:298-354) of the density matrix with noise, as in
+    predict_split_l2r / r2l (boundary_ops/prediction.hpp:152-237): per sector X = M (left-to-right)
+    or M^T (right-to-left), rho = X X^T + alpha N, eigenvectors by descending eigenvalue, singular
+    values sqrt(max(lambda, 0)) into estimate_truncation, Q_k = the kept eigenvectors and
+    B_k = Q_k^T X.  Returns ({c: (Q_k, B_k)}, keeps, singular values, truncated weight)."""
+    svals, vecs = [], {}
+    for c in keys:
+        X = mats[c] if direction > 0 else mats[c].t()
+        if X.shape[0] == 0 or X.shape[1] == 0:
+            svals.append(np.zeros(0))
+            continue
+        rho = X @ X.t() + alpha * noise[c]
+        lam, P = torch.linalg.eigh(0.5 * (rho + rho.t()))
+        lam, P = lam.flip(0), P.flip(1)
+        vecs[c] = (X, P)
+        svals.append(torch.sqrt(torch.clamp(lam, min=0.0)).cpu().numpy())
+    if not any(len(v) for v in svals):
+        return {}, [0] * len(keys), svals, 0.0
+    keeps, tw = truncate(svals, Mmax, cutoff)
+    fac = {}
+    for c, k in zip(keys, keeps):
+        if k > 0:
+            X, P = vecs[c]
+            Q = P[:, :k]
+            fac[c] = (Q, Q.t() @ X)
+    return fac, keeps, svals, tw
+
+
 class TwoSiteDMRG:
     def __init__(self, eng, model, L, m, cutoff=1e-10, init_m=16, seed=1, jd_tol=1e-8, jd_maxiter=8,
                  mpo=None, basis=None, target=None, verbose=False, hook=None, precision=E.PREC_F64, max_outer=1,
-                 init_bonds="sectors", tune_library=False, **model_kw):
+                 init_bonds="sectors", tune_library=False, noise=0.0, **model_kw):
         """precision != PREC_F64: sigma in that precision inside Jacobi-Davidson, with the FP64
         residual refinement of SigmaPlan.jd_refined (at most max_outer FP64 steps per site).  With
         max_outer = 1 a site costs jd_maxiter reduced-precision products and one FP64 product (the
@@ -313,6 +457,12 @@ class TwoSiteDMRG:
         default: at L=30, m=2048 the shapes of successive sites rarely share a class, and the
         first sweep spent 24.6 s timing candidates to save 0.7 s of products.
 
+        noise: density-matrix noise alpha of the split (predict_split_l2r / r2l,
+        boundary_ops/prediction.hpp:152-237; the reference's alpha_initial / alpha_main /
+        alpha_final), a number for every sweep or a sequence with one value per sweep (0 after its
+        end).  With alpha > 0 the split diagonalises rho = X X^T + alpha N (noise_density_left /
+        _right, dm_factors) instead of taking the truncated SVD.
+
         init_bonds: "sectors" caps every charge sector of the initial MPS at init_m (the reference's
         mps initialisation, mps_sectors.h); "total" draws bond indices of total dimension ~init_m
         (mps.random_bond_dims), the shape a converged state at that bond dimension has."""
@@ -323,6 +473,8 @@ class TwoSiteDMRG:
         self.split_methods = {}
         self.tune_library, self.tune_timings = tune_library, 0
         self.hook = hook
+        self.noise = noise
+        self.plain_desc = {}
         if mpo is None:
             basis, mpo, target = models.chain(model, L, **model_kw)
         self.basis, self.mpo, self.target = basis, mpo, target
@@ -489,8 +641,16 @@ class TwoSiteDMRG:
             self._recouple(theta, True)
         return theta
 
-    def split(self, i, theta, direction):
-        """Truncated SVD split of the two-site tensor (split_mps_l2r / split_mps_r2l)."""
+    def alpha(self):
+        """Noise of the current sweep (self.energies holds one entry per finished sweep)."""
+        k = len(self.energies)
+        if np.isscalar(self.noise):
+            return float(self.noise)
+        return float(self.noise[k]) if k < len(self.noise) else 0.0
+
+    def split(self, i, theta, direction, alpha=0.0):
+        """Truncated SVD split of the two-site tensor (split_mps_l2r / split_mps_r2l), or with
+        alpha > 0 the density-matrix split with noise (predict_split_l2r / r2l)."""
         A1, A2 = self.sites[i], self.sites[i + 1]
         if self.su2:
             self._recouple(theta, False)
@@ -543,12 +703,30 @@ class TwoSiteDMRG:
                             slabs.append((mbase[mc] + (r0 + ss1 * ls) * ridx[mc] + c0 + ss2 * rs, ridx[mc], 1,
                                           o + (col12 + sub_i * rs) * ls, 1, ls, ls, rs))
         slab_copy(mflat, theta.data, slabs)
-        fac, keeps, svals, tw, method = split_factors(mats, mids, direction, self.eng.torch_device, self.m,
-                                                      self.cutoff)
+        if alpha > 0:
+            k = i if direction > 0 else i + 1
+            if k not in self.plain_desc:
+                self.plain_desc[k] = plain_mpo_desc(self.mpo[k])
+            if direction > 0:
+                N = noise_density_left(self.eng, self.sym, A1.left, self.phys, mids, mats, self.plain_desc[k],
+                                       self.Lb[i])
+            else:
+                N = noise_density_right(self.eng, self.sym, A2.right, self.phys, mids, mats, self.plain_desc[k],
+                                        self.Rb[i + 2])
+            fac, keeps, svals, tw = dm_factors(mats, mids, direction, self.eng.torch_device, self.m, self.cutoff,
+                                               N, alpha)
+            method = "dm"
+        else:
+            fac, keeps, svals, tw, method = split_factors(mats, mids, direction, self.eng.torch_device, self.m,
+                                                          self.cutoff)
         self.split_methods[method] = self.split_methods.get(method, 0) + 1
         kept = [(c, k) for c, k in zip(mids, keeps) if k > 0]
         newmid = Index([c for c, _ in kept], [k for _, k in kept])
-        norm = np.sqrt(sum(float(np.sum(s[:k] ** 2)) for s, k in zip(svals, keeps)))
+        if method == "dm":
+            # the kept eigenvalues include the noise: the norm is that of the projected tensor
+            norm = float(torch.sqrt(sum(torch.sum(fac[c][1] ** 2) for c, _ in kept)))
+        else:
+            norm = np.sqrt(sum(float(np.sum(s[:k] ** 2)) for s, k in zip(svals, keeps)))
         b1, b2 = {}, {}
         for c, k in kept:
             orth, other = fac[c][0], fac[c][1] / norm
@@ -643,7 +821,7 @@ class TwoSiteDMRG:
         self.timings["jd"] += time.perf_counter() - t0
         theta.data = out
         t0 = time.perf_counter()
-        tw, bd = self.split(i, theta, direction)
+        tw, bd = self.split(i, theta, direction, self.alpha())
         self.timings["split"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         if direction > 0:

@@ -198,6 +198,7 @@ class SiteDesc:
         self.keep = []
         self.desc = capi.site_desc(sym, left, phys, right, self.keep)
         lay = SiteLayout(left, phys, right)
+        self.layout = lay
         self.rp_total, self.lp_total = lay.rp_total, lay.lp_total
 
 
@@ -239,6 +240,17 @@ class DeviceBoundary:
             if n:
                 check(lib().mq_boundary_download(self.h, ci, a.ctypes.data))
             out.append(a)
+        return out
+
+    def cohort(self, ci):
+        """Device copy of cohort ci (its n_blocks blocks of ls x rs, column-major, back to back) as
+        a torch tensor, in stream order with the engine's launches."""
+        n = self.index.cohort_size(ci)
+        out = torch.empty(max(n, 1), dtype=torch.float64, device=self.engine.torch_device)[:n]
+        if n:
+            p = C.c_void_p()
+            check(lib().mq_boundary_cohort_ptr(self.h, ci, C.byref(p)))
+            check(lib().mq_vec_copy(self.engine.ctx, n, p, out.data_ptr()))
         return out
 
     def traces(self):
