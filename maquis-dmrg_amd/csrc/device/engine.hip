@@ -491,6 +491,7 @@ struct StagePlan {
         std::vector<double> w;          // compact W tiles (host copy until upload)
         uint64_t tiles = 0;
         uint64_t flops = 0;             // MFMA flops of the visited tiles (unpadded)
+        bool narrow = true;             // every task's T_D step window fits 32-bit byte offsets
         DevBuf d_tasks, d_runs, d_w64, d_w32, d_whi, d_wlo;
     } bsr;
     bool bsr_on = false;
@@ -817,6 +818,8 @@ struct StagePlan {
         gemm(f_tasks, f_segs, false);
         gemm(u_tasks, u_segs, false);
         for (const auto& t : bsr.tasks) {   // every T_D read (128 rows of each key column) and S_D write
+            MQ_REQUIRE(!bsr.narrow || (32ull * t.ld + S_BSR_TILE_M) * sizeof(double) < (1ull << 32),
+                       "block-sparse task too wide for 32-bit offsets");
             MQ_REQUIRE(t.M <= S_BSR_TILE_M && t.N <= S_BSR_TILE_N && t.run0 <= t.run1 && t.run1 <= bsr.runs.size(),
                        "block-sparse task shape");
             uint32_t kmax = 0;
@@ -922,7 +925,7 @@ struct StagePlan {
                     const void* b0 = mode == MODE_F64 ? bsr.d_w64.p : mode == MODE_F32 ? bsr.d_w32.p : bsr.d_whi.p;
                     const void* b1 = mode == MODE_BF16X3 ? bsr.d_wlo.p : nullptr;
                     HIPCHK(launch_s_bsr(st, mode, bsr.d_tasks.as<BsrTask>(), uint32_t(bsr.tasks.size()),
-                                        bsr.d_runs.as<BsrRun>(), b0, b1, b.p[BASE_WS]));
+                                        bsr.d_runs.as<BsrRun>(), b0, b1, b.p[BASE_WS], bsr.narrow));
                 }
                 run_lib(st, b, mode);
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
@@ -1917,6 +1920,7 @@ void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>
         // of one row panel with every W panel of the group (or the reverse)
         constexpr uint32_t SB = 8;
         const uint32_t npm = (M + TM - 1) / TM;
+        B.narrow = B.narrow && (32ull * M + TM) * sizeof(double) < (1ull << 32);
         for (uint32_t jb = 0; jb < npn; jb += SB)
             for (uint32_t mb = 0; mb < npm; mb += SB)
                 for (uint32_t j = jb; j < std::min(npn, jb + SB); ++j)

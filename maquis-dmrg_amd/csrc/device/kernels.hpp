@@ -167,8 +167,10 @@ struct BsrRun {
 };
 // mode MODE_F64: ws, W tiles (bt0) double; MODE_F32: float; MODE_BF16X3: ws float, bt0 / bt1 the
 // bf16 hi / lo tiles
+// narrow: every task's T_D step window, (32 * ld + S_BSR_TILE_M) elements of 8 bytes, fits in 2^32
+// bytes, so the kernel addresses it with 32-bit offsets from a uniform base
 hipError_t launch_s_bsr(hipStream_t st, int mode, const BsrTask* tasks, uint32_t n_tasks, const BsrRun* runs,
-                        const void* bt0, const void* bt1, void* ws);
+                        const void* bt0, const void* bt1, void* ws, bool narrow);
 
 // Side tables of a launch: remap entries, column units, row units, block scales.
 struct GemmTables {

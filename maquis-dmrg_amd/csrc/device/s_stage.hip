@@ -37,14 +37,18 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
 constexpr int BM = S_BSR_TILE_M, BN = S_BSR_TILE_N, BK = S_BSR_TILE_K, NT = 256;
 static_assert(BM == 128 && BN == 128 && BK == 16, "tile shape");
 
-// hi = bf16(x), lo = bf16(x - hi), round to nearest even (same rule as the T stage's split)
-__device__ inline void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
-    hi = bf16_rne(x);
-    lo = bf16_rne(x - __uint_as_float(uint32_t(hi) << 16));
+// Two elements: hi = bf16(x), lo = bf16(x - hi), round to nearest even (the rule of bf16_rne, the T
+// stage's split), each pair packed with the lower element in the low half.  The float -> bf16
+// conversions are gfx950's v_cvt_pk_bf16_f32 (two values per instruction).
+__device__ inline void split_bf16x2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+    hi = __builtin_bit_cast(uint32_t, bf2{(__bf16)x0, (__bf16)x1});
+    const float h0 = __uint_as_float(hi << 16), h1 = __uint_as_float(hi & 0xffff0000u);
+    lo = __builtin_bit_cast(uint32_t, bf2{(__bf16)(x0 - h0), (__bf16)(x1 - h1)});
 }
 
 template <int MODE> struct BsrTraits;
@@ -73,8 +77,8 @@ template <> struct BsrSmem<MODE_BF16X3> {
     uint16_t bhi[BN][P], blo[BN][P];
 };
 
-template <int MODE>
-__global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ tasks, const BsrRun* __restrict__ runs,
+template <int MODE, bool NARROW>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_F64 ? 1 : 2))) void s_bsr_kernel(const BsrTask* __restrict__ tasks, const BsrRun* __restrict__ runs,
                                                    const void* __restrict__ bt0, const void* __restrict__ bt1,
                                                    void* __restrict__ ws) {
     typedef typename BsrTraits<MODE>::T T;
@@ -103,25 +107,41 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
     // ---- staging of one step of KS keys (run r, key offset kk within the run)
     // A: T_D rows m0 .. m0+127 of KS key columns.  Thread t reads row t % 128 of the KS / 2 keys
     // (t / 128) * KS / 2 .. : every load instruction covers 64 consecutive rows of one column.  Rows
-    // beyond M read valid workspace memory (the plan checks 128 rows past every panel) and are
-    // zeroed; keys beyond the run's end read its first key and are zeroed.
+    // beyond M read valid workspace memory (the plan checks 128 rows past every panel); they only
+    // feed output rows that are not stored.  Keys beyond the run's end read its last key: their W
+    // entries are zero (the run's last tile is zero padded, and a tile past the run's end is not
+    // read), so they add nothing.  The addresses are a wave-uniform base (the step's first key
+    // column) plus 32-bit byte offsets when the plan allows it (NARROW: (KS * ld + BM) elements of
+    // every task fit in 2^32 bytes), else full 64-bit offsets.
     // B: the compact tile(s) (128 columns x 16 keys each, [n][k], 16-byte aligned runs of 8 keys),
     // thread t reads column t / 2: 8 keys of the one tile (KS = 16), or the 16 keys of tile t % 2
     // (KS = 32; a tile past the run's end is not read and counts as zero).
     constexpr int KS = BsrStep<MODE>::KS, AK = KS / 2;
     const int a_m = tid & (BM - 1), a_k = (tid >> 7) * AK;
     const int b_n = tid >> 1, b_k = (tid & 1) * (KS / 2);
-    const bool m_ok = uint32_t(a_m) < M;
+    const uint32_t ld_b = uint32_t(ld) * uint32_t(sizeof(T));             // narrow plans only
+    const uint32_t a_b = uint32_t(a_m) * uint32_t(sizeof(T)) + uint32_t(a_k) * ld_b;
     T va[AK];
     T vb[8];
     uint4 vbh[KS / 16], vbl[KS / 16];
     auto gload = [&](const BsrRun& R, uint32_t kk) {
-        const T* src = A + a_m;
+        const T* col = A + (uint64_t(R.k0) + kk) * ld;   // the step's first key column (uniform)
+        const uint32_t last = R.nk - 1 - kk;              // last key of the run, relative to the step
+        if constexpr (NARROW) {
+            const char* base = reinterpret_cast<const char*>(col);
+            if (kk + uint32_t(a_k + AK) <= R.nk) {        // uniform per wave: every key in the run
 #pragma unroll
-        for (int e = 0; e < AK; ++e) {
-            const bool kok = kk + a_k + e < R.nk;
-            va[e] = src[uint64_t(kok ? R.k0 + kk + a_k + e : R.k0) * ld];
-            if (!kok || !m_ok) va[e] = T(0);
+                for (int e = 0; e < AK; ++e) va[e] = *reinterpret_cast<const T*>(base + (a_b + uint32_t(e) * ld_b));
+            } else {
+#pragma unroll
+                for (int e = 0; e < AK; ++e) {
+                    const uint32_t key = min(uint32_t(a_k + e), last);
+                    va[e] = *reinterpret_cast<const T*>(base + (uint32_t(a_m) * uint32_t(sizeof(T)) + key * ld_b));
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < AK; ++e) va[e] = col[uint64_t(min(uint32_t(a_k + e), last)) * ld + a_m];
         }
         if constexpr (BF) {
             // KS = 32: tile kk / 16 + (t % 2), its 16 keys of column t / 2
@@ -153,13 +173,7 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
         if constexpr (BF) {
             uint32_t h[AK / 2], l[AK / 2];   // pairs of bf16, the lower key in the low half
 #pragma unroll
-            for (int e = 0; e < AK / 2; ++e) {
-                uint16_t h0, l0, h1, l1;
-                split_bf16(va[2 * e], h0, l0);
-                split_bf16(va[2 * e + 1], h1, l1);
-                h[e] = uint32_t(h0) | (uint32_t(h1) << 16);
-                l[e] = uint32_t(l0) | (uint32_t(l1) << 16);
-            }
+            for (int e = 0; e < AK / 2; ++e) split_bf16x2(va[2 * e], va[2 * e + 1], h[e], l[e]);
 #pragma unroll
             for (int v = 0; v < AK / 8; ++v) {
                 *reinterpret_cast<uint4*>(&s.ahi[a_m][a_k + 8 * v]) = uint4{h[4 * v], h[4 * v + 1], h[4 * v + 2], h[4 * v + 3]};
@@ -252,41 +266,58 @@ __global__ __launch_bounds__(NT) void s_bsr_kernel(const BsrTask* __restrict__ t
         }
     }
 
-    // ---- epilogue: C = acc (rows < M, columns < N)
+    // ---- epilogue: C = acc (rows < M, columns < N).  FP32 accumulators of a lane are 4 consecutive
+    // rows of one column: one 16-byte store when all four are in range (element-aligned vector
+    // stores are legal for global memory)
+    typedef float __attribute__((ext_vector_type(4), aligned(4))) f4_a4;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t col = wn + 16 * j + l16;
+        if (col >= N) continue;
+        T* cc = C + uint64_t(col) * ld;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t col = wn + 16 * j + l16;
-            if (col >= N) continue;
-            T* cc = C + uint64_t(col) * ld;
+        for (int i = 0; i < 4; ++i) {
+            if constexpr (MODE != MODE_F64) {
+                const uint32_t row0 = wm + 16 * i + 4 * lk;
+                if (row0 + 3 < M) {
+                    *reinterpret_cast<f4_a4*>(cc + row0) = f4_a4{acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+                    continue;
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const uint32_t row = MODE == MODE_F64 ? wm + 16 * i + 4 * q + lk : wm + 16 * i + 4 * lk + q;
                 if (row < M) cc[row] = acc[i][j][q];
             }
         }
+    }
 }
 
 }  // namespace
 
 hipError_t launch_s_bsr(hipStream_t st, int mode, const BsrTask* tasks, uint32_t n_tasks, const BsrRun* runs,
-                        const void* bt0, const void* bt1, void* ws) {
+                        const void* bt0, const void* bt1, void* ws, bool narrow) {
     if (n_tasks == 0) return hipSuccess;
     const dim3 grid(n_tasks), block(NT);
+#define MQ_S_BSR(M_)                                                                                  \
+    if (narrow)                                                                                       \
+        hipLaunchKernelGGL((s_bsr_kernel<M_, true>), grid, block, 0, st, tasks, runs, bt0, bt1, ws);  \
+    else                                                                                              \
+        hipLaunchKernelGGL((s_bsr_kernel<M_, false>), grid, block, 0, st, tasks, runs, bt0, bt1, ws)
     switch (mode) {
         case MODE_F64:
-            hipLaunchKernelGGL(s_bsr_kernel<MODE_F64>, grid, block, 0, st, tasks, runs, bt0, bt1, ws);
+            MQ_S_BSR(MODE_F64);
             break;
         case MODE_F32:
-            hipLaunchKernelGGL(s_bsr_kernel<MODE_F32>, grid, block, 0, st, tasks, runs, bt0, bt1, ws);
+            MQ_S_BSR(MODE_F32);
             break;
         case MODE_BF16X3:
-            hipLaunchKernelGGL(s_bsr_kernel<MODE_BF16X3>, grid, block, 0, st, tasks, runs, bt0, bt1, ws);
+            MQ_S_BSR(MODE_BF16X3);
             break;
         default:
             return hipErrorInvalidValue;
     }
+#undef MQ_S_BSR
     return hipGetLastError();
 }
 
