@@ -194,7 +194,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                 const int e = tid + GEMM_THREADS * i;
                 const uint32_t gm = m0 + e / BK;
                 const uint32_t ok = uint32_t(gm < M) & uint32_t(gk < sg.K);
-                oka |= ok << i;
+                oka |= uint32_t(gk < sg.K) << i;
                 va[i] = A[(boff + uint64_t(gm + tk.m_off) * sg.lda) & (0ull - uint64_t(ok))];
             }
             return;
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                 const int e = tid + GEMM_THREADS * i;
                 const uint32_t gk = k0 + e / BM;
                 const uint32_t ok = a_ok & uint32_t(gk < sg.K);
-                oka |= ok << i;
+                oka |= uint32_t(gk < sg.K) << i;
                 va[i] = A[(row + uint64_t(gk) * ldk) & (0ull - uint64_t(ok))];
             }
             return;
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
             const int k = r_trans_a ? e % BK : e / BM;
             const uint32_t gm = m0 + m, gk = k0 + k;
             const uint32_t ok = uint32_t(gm < M) & uint32_t(gk < sg.K);
-            oka |= ok << i;
+            oka |= uint32_t(gk < sg.K) << i;
             va[i] = A[(uint64_t(gm + tk.m_off) * sm + uint64_t(gk) * sk) & (0ull - uint64_t(ok))];
         }
     };
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                 const int e = tid + GEMM_THREADS * i;
                 const uint32_t gn = n0 + e / BK, gk = k0 + e % BK;
                 const uint32_t ok = uint32_t(gn < N) & uint32_t(gk < sg.K);
-                okb |= ok << i;
+                okb |= uint32_t(gk < sg.K) << i;
                 vb[i] = B[(uint64_t(row[i]) * sg.bldk + kin[i] + uint64_t(gn) * sg.ldb) & (0ull - uint64_t(ok))];
             }
         } else {
@@ -306,28 +306,52 @@ __global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(const GemmTask* __re
                 const int k = r_kfast_b ? e % BK : e / BN;
                 const uint32_t gn = n0 + n, gk = k0 + k;
                 const uint32_t ok = uint32_t(gn < N) & uint32_t(gk < sg.K);
-                okb |= ok << i;
+                okb |= uint32_t(gk < sg.K) << i;
                 vb[i] = B[(uint64_t(gk) * bsk + uint64_t(gn) * bsn) & (0ull - uint64_t(ok))];
             }
         }
     };
+    // Staged element i of thread tid is e = tid + 256 i.  m-fast A: m = e % BM = tid % BM for every i
+    // (BM divides 256), k = tid / BM + i (256 / BM); k-fast A: k = tid % BK, m = tid / BK + 16 i; B
+    // likewise with n and BN.  Both give LDS offsets base + i * constant, so the two orders are two
+    // uniform branches with immediate offsets.  Values are masked only where they must read as zero
+    // (the mask bits: k beyond the segment, B of an invalid column unit); out-of-range rows and
+    // columns carry the finite value at their clamped address into outputs that are not stored.
+    static_assert(GEMM_THREADS % BM == 0 && GEMM_THREADS % BN == 0 && GEMM_THREADS % BK == 0, "staging maps");
     auto lstore = [&](int buf) {
         char* base = smem + buf * SM::buf;
+        T* As = reinterpret_cast<T*>(base);
+        T* Bs = reinterpret_cast<T*>(base + sizeof(T) * BK * (BM + 1));
+        const T sa = T(r_alpha * a_scale);
+        const bool mask_a = oka != (1u << NA) - 1u, mask_b = okb != (1u << NB) - 1u;
+        T xa[NA], xb[NB];
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
-            const int e = tid + GEMM_THREADS * i;
-            const int m = r_trans_a ? e / BK : e % BM;
-            const int k = r_trans_a ? e % BK : e / BM;
-            const double v = double(va[i]) * (((oka >> i) & 1u) ? r_alpha * a_scale : 0.0);
-            reinterpret_cast<T(*)[BM + 1]>(base)[k][m] = T(v);
+            xa[i] = T(va[i]) * sa;
+            if (mask_a && !((oka >> i) & 1u)) xa[i] = T(0);
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
-            const int e = tid + GEMM_THREADS * i;
-            const int n = r_kfast_b ? e / BK : e % BN;
-            const int k = r_kfast_b ? e % BK : e / BN;
-            const double v = double(vb[i]) * (((okb >> i) & 1u) ? 1.0 : 0.0);
-            reinterpret_cast<T(*)[BN + 1]>(base + sizeof(T) * BK * (BM + 1))[k][n] = T(v);
+            xb[i] = T(vb[i]);
+            if (mask_b && !((okb >> i) & 1u)) xb[i] = T(0);
+        }
+        if (r_trans_a) {
+            T* p = As + (tid % BK) * (BM + 1) + tid / BK;
+#pragma unroll
+            for (int i = 0; i < NA; ++i) p[i * (GEMM_THREADS / BK)] = xa[i];
+        } else {
+            T* p = As + (tid / BM) * (BM + 1) + tid % BM;
+#pragma unroll
+            for (int i = 0; i < NA; ++i) p[i * (GEMM_THREADS / BM) * (BM + 1)] = xa[i];
+        }
+        if (r_kfast_b) {
+            T* p = Bs + (tid % BK) * (BN + 1) + tid / BK;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) p[i * (GEMM_THREADS / BK)] = xb[i];
+        } else {
+            T* p = Bs + (tid / BN) * (BN + 1) + tid % BN;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) p[i * (GEMM_THREADS / BN) * (BN + 1)] = xb[i];
         }
     };
     auto compute = [&](int buf) {
