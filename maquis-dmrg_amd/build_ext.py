@@ -20,6 +20,7 @@ SOURCES = [
     "csrc/device/kernels.hip",
     "csrc/device/gemm.hip",
     "csrc/device/s_stage.hip",
+    "csrc/device/t_stage.hip",
     "csrc/device/engine.hip",
 ]
 

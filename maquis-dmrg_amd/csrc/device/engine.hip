@@ -343,6 +343,7 @@ struct GemmStage {
     std::vector<uint32_t> tile_task[GEMM_N_CFG];   // launches with many multi-tile tasks: task of each tile
     DevBuf d_tasks[GEMM_N_CFG], d_tile_task[GEMM_N_CFG];
     DevBuf d_segs;
+    bool stacked128 = false;   // every 128x128 task suits the lean stacked T-stage kernel (t_stage.hip)
 
     // Flops per size class: the block-size histogram the autotuner's decisions are keyed on.
     std::vector<uint64_t> class_flops(const std::vector<GemmSeg>& segs) const {
@@ -387,6 +388,16 @@ struct GemmStage {
         }
     }
     void upload_all(const std::vector<GemmSeg>& segs, hipStream_t st) {
+        static const bool generic = std::getenv("MQ_T_GENERIC") != nullptr;
+        stacked128 = !generic && !cfg[GEMM_CFG_128x128].empty();
+        for (const auto& t : cfg[GEMM_CFG_128x128]) {
+            bool ok = t.rowunit != MAP_NONE && t.seg1 == t.seg0 + 1 && !t.beta && t.M > 0 && t.N > 0;
+            if (ok) {
+                const GemmSeg& g = segs[t.seg0];
+                ok = g.amap == MAP_NONE && g.bmap == MAP_NONE && g.units == MAP_NONE && !g.trans_a;
+            }
+            stacked128 = stacked128 && ok;
+        }
         upload(d_segs, segs, st);
         for (int c = 0; c < GEMM_N_CFG; ++c) {
             upload(d_tasks[c], cfg[c], st);
@@ -394,11 +405,17 @@ struct GemmStage {
         }
     }
     void run(hipStream_t st, int mode, const char* types, Bases b, GemmTables tb) const {
-        for (int c = 0; c < GEMM_N_CFG; ++c)
-            if (tiles[c])
+        for (int c = 0; c < GEMM_N_CFG; ++c) {
+            if (!tiles[c]) continue;
+            const uint32_t* tt = tile_task[c].empty() ? nullptr : d_tile_task[c].as<uint32_t>();
+            const bool f64 = mode == MODE_F64, ddf = !f64 && std::strcmp(types, "ddf") == 0;
+            if (c == GEMM_CFG_128x128 && stacked128 && (f64 || ddf))
+                HIPCHK(launch_t_stack(st, mode, ddf, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
+                                      d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
+            else
                 HIPCHK(launch_gemm(st, c, mode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
-                                   d_segs.as<GemmSeg>(), tiles[c], b, tb,
-                                   tile_task[c].empty() ? nullptr : d_tile_task[c].as<uint32_t>()));
+                                   d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
+        }
     }
     uint64_t device_bytes() const {
         uint64_t n = d_segs.bytes;

@@ -187,6 +187,11 @@ struct GemmTables {
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
                        const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tables,
                        const uint32_t* tile_task);
+// Stacked 128 x 128 T-stage tasks (row units, one segment, no A / B remaps, no beta) on the lean
+// kernel of t_stage.hip: mode MODE_F64 (C double) or MODE_F32 (C float if c_float, else double),
+// A and B double; same tables, tile numbering and results as launch_gemm's GEMM_CFG_128x128.
+hipError_t launch_t_stack(hipStream_t st, int mode, bool c_float, const GemmTask* tasks, uint32_t n_tasks,
+                          const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task);
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
 constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions
