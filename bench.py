@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--headline-sweeps", type=int, default=1, help="sweeps of the headline sweep comparison")
     ap.add_argument("--headline-which", default="both", choices=["both", "f64", "reduced"],
                     help="run the headline sweeps in FP64, in the reported precision, or both")
+    ap.add_argument("--headline-verbose", action="store_true",
+                    help="print every site pair of the headline sweeps (energy, iterations, bond dimension)")
     ap.add_argument("--adjoint", action="store_true",
                     help="also run the differentiation workload: two-site DMRG of the ab initio chain "
                          "(--adjoint-L orbitals, N = L, bond dimension --adjoint-m), the sigma-adjoint sweep "
@@ -76,6 +78,10 @@ def parse():
     ap.add_argument("--adjoint-m", type=int, default=256)
     ap.add_argument("--adjoint-sweeps", type=int, default=6)
     ap.add_argument("--adjoint-fd", type=int, default=3, help="integrals checked by finite differences")
+    ap.add_argument("--adjoint-cache", action="store_true",
+                    help="run the differentiation workload with the boundary cache: only the boundaries of the "
+                         "current site pair on the device, the others in pinned host memory, and during the "
+                         "adjoint sweep the left boundaries recomputed as the orthogonality center moves")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile configurations (no timing of candidates before the run)")
     ap.add_argument("--seed", type=int, default=7)
@@ -147,7 +153,8 @@ def headline_sweep(eng, args, report, prec_code, chain, model, L, m):
     for name, pc in runs:
         t0 = time.perf_counter()
         d = TwoSiteDMRG(eng, model, L, m=m, cutoff=1e-10, init_m=m, seed=3, jd_tol=1e-8, jd_maxiter=8,
-                        mpo=mpo, basis=basis, target=target, precision=pc, init_bonds="total")
+                        mpo=mpo, basis=basis, target=target, precision=pc, init_bonds="total",
+                        verbose=args.headline_verbose)
         d.init_boundaries()
         t_setup = time.perf_counter() - t0
         torch.cuda.synchronize()
@@ -183,7 +190,7 @@ def adjoint_workload(eng, args, su2_model):
     L, m = args.adjoint_L, args.adjoint_m
     N = L
     h, eri = qc.synthetic_integrals(L, seed=0)
-    kw = dict(m=m, cutoff=1e-12, init_m=16, seed=3, jd_tol=1e-10, jd_maxiter=12)
+    kw = dict(m=m, cutoff=1e-12, init_m=16, seed=3, jd_tol=1e-10, jd_maxiter=12, boundary_cache=args.adjoint_cache)
     model = "qc_su2" if su2_model else "qc"
     target = (N, 0) if su2_model else (N // 2, N - N // 2)
 
@@ -217,8 +224,10 @@ def adjoint_workload(eng, args, su2_model):
     euler = float(np.sum(h * dh) + np.sum(eri * deri))
     contr = np.einsum("pqrr->pq", deri)
     res = {
-        "model": "ab initio (synthetic integrals), %s, L=%d, N=%d, m=%d, %d sweeps, FP64"
-                 % ("SU2U1 singlet" if su2_model else "TwoU1", L, N, m, args.adjoint_sweeps),
+        "model": "ab initio (synthetic integrals), %s, L=%d, N=%d, m=%d, %d sweeps, FP64%s"
+                 % ("SU2U1 singlet" if su2_model else "TwoU1", L, N, m, args.adjoint_sweeps,
+                    ", boundary cache (left boundaries recomputed in the adjoint sweep, right boundaries "
+                    "fetched from host memory)" if args.adjoint_cache else ""),
         "energy_Eh": e, "dmrg_seconds": round(t_dmrg, 2),
         "adjoint_sweep_seconds": round(t_adj, 3), "integral_map_seconds": round(t_map, 3),
         "site_energy_spread_Eh": float(max(energies) - min(energies)),

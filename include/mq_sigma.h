@@ -194,6 +194,23 @@ int mq_boundary_index(const mq_boundary* b, mq_bindex_desc* out);
 int mq_boundary_download(const mq_boundary* b, uint32_t ci, double* host);
 /* Device pointer of cohort ci. */
 int mq_boundary_cohort_ptr(const mq_boundary* b, uint32_t ci, const double** dev);
+/* Residency of a boundary, in place of the reference's storage::gpu / storage::disk boundary
+ * cache (utils/storage.h:87-200 evict / prefetch / fetch, 271-539 disk spill; driven from
+ * optimize/ts_optimize.hpp:153-182,307-345).
+ * mq_boundary_evict: the device copy is freed after queued work on the context's stream has run;
+ * the first eviction copies the data to pinned host memory (a boundary never changes after it is
+ * made, so that copy stays valid).  With path != NULL the host copy is also written to that file
+ * (once) and freed.  A boundary that a live sigma plan reads cannot be evicted (MQ_ERR_INVALID).
+ * mq_boundary_prefetch: allocates the device copy and starts the host -> device copy (reading the
+ * spill file first if there is no host copy) on the context's copy stream; it returns at once,
+ * and work that reads the boundary afterwards waits for the copy on the device.  Every entry
+ * point that reads an evicted boundary (plan creation, steps, download, traces, cohort_ptr)
+ * prefetches it first.  mq_boundary_residency reports where the data is and the bytes held on the
+ * device and in pinned host memory.  Spill files are deleted with the boundary. */
+enum { MQ_RESIDENT_DEVICE = 0, MQ_RESIDENT_HOST = 1, MQ_RESIDENT_DISK = 2 };
+int mq_boundary_evict(mq_ctx* ctx, mq_boundary* b, const char* path);
+int mq_boundary_prefetch(mq_ctx* ctx, mq_boundary* b);
+int mq_boundary_residency(const mq_boundary* b, int* where, uint64_t* device_bytes, uint64_t* host_bytes);
 /* Boundary::traces() (mp_tensors/boundary.h:339-350): per bond index b, the sum of all elements
  * of every block b; out has aux_dim entries. */
 int mq_boundary_traces(const mq_boundary* b, double* out);

@@ -21,6 +21,7 @@ SOURCES = [
     "csrc/device/gemm.hip",
     "csrc/device/s_stage.hip",
     "csrc/device/t_stage.hip",
+    "csrc/device/f_stage.hip",
     "csrc/device/engine.hip",
 ]
 

@@ -166,6 +166,7 @@ constexpr size_t LT_WORKSPACE = size_t(64) << 20;
 struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;   // boundary prefetches (host -> device), overlapping `stream`
     rocblas_handle blas = nullptr;   // plain dense GEMMs of the grouped S stage, on `stream`
     hipblasLtHandle_t lt = nullptr;  // reduced-precision S stage GEMMs
     DevBuf lt_ws;
@@ -298,9 +299,25 @@ struct mq_boundary {
     std::vector<uint64_t> cohort_off;
     uint64_t total = 0;
     DevBuf data;
+    // Residency (the reference's storage::gpu / storage::disk, utils/storage.h:87-200,271-539).  A
+    // boundary's data never changes after it is made, so a host or disk copy, once written, stays
+    // valid: evicting again only frees the device copy.
+    double* host = nullptr;               // pinned host copy (total doubles), or null
+    std::string disk;                     // spill file holding the data, or empty
+    bool resident = true;                 // the device copy is allocated and (after `arrived`) valid
+    hipEvent_t arrived = nullptr;         // end of the last prefetch on ctx->copy_stream
+    mutable int plans = 0;                // live sigma plans that read the boundary (no eviction)
     // storage for exported descriptors
     mutable std::vector<mq_charge> bra_c, ket_c;
-    double* ptr() const { return data.as<double>(); }
+    double* ptr() const {
+        MQ_REQUIRE(resident, "boundary not resident on the device (mq_boundary_prefetch)");
+        return data.as<double>();
+    }
+    ~mq_boundary() {
+        if (arrived) (void)hipEventSynchronize(arrived), (void)hipEventDestroy(arrived);
+        if (host) (void)hipHostFree(host);
+        if (!disk.empty()) std::remove(disk.c_str());
+    }
     void layout() {
         cohort_off.resize(index.n_cohorts());
         total = 0;
@@ -344,6 +361,7 @@ struct GemmStage {
     DevBuf d_tasks[GEMM_N_CFG], d_tile_task[GEMM_N_CFG];
     DevBuf d_segs;
     bool stacked128 = false;   // every 128x128 task suits the lean stacked T-stage kernel (t_stage.hip)
+    bool units128 = false;     // every 128x128 task suits the lean column-unit final-stage kernel (f_stage.hip)
 
     // Flops per size class: the block-size histogram the autotuner's decisions are keyed on.
     std::vector<uint64_t> class_flops(const std::vector<GemmSeg>& segs) const {
@@ -389,7 +407,19 @@ struct GemmStage {
     }
     void upload_all(const std::vector<GemmSeg>& segs, hipStream_t st) {
         static const bool generic = std::getenv("MQ_T_GENERIC") != nullptr;
+        static const bool generic_f = std::getenv("MQ_F_GENERIC") != nullptr;
         stacked128 = !generic && !cfg[GEMM_CFG_128x128].empty();
+        units128 = !generic_f && !cfg[GEMM_CFG_128x128].empty();
+        for (const auto& t : cfg[GEMM_CFG_128x128]) {
+            bool ok = t.rowunit == MAP_NONE && t.colunit != MAP_NONE && !t.beta && t.m_off == 0 &&
+                      t.cmap == MAP_NONE && t.c_base != BASE_WS && t.M > 0 && t.N > 0 && t.seg1 > t.seg0;
+            for (uint32_t k = t.seg0; ok && k < t.seg1; ++k) {
+                const GemmSeg& g = segs[k];
+                ok = g.units != MAP_NONE && g.bmap == MAP_NONE && !g.trans_b && g.bbh > 0 &&
+                     (g.amap == MAP_NONE ? !g.trans_a : g.trans_a && g.ascale != MAP_NONE);
+            }
+            units128 = units128 && ok;
+        }
         for (const auto& t : cfg[GEMM_CFG_128x128]) {
             bool ok = t.rowunit != MAP_NONE && t.seg1 == t.seg0 + 1 && !t.beta && t.M > 0 && t.N > 0;
             if (ok) {
@@ -409,9 +439,13 @@ struct GemmStage {
             if (!tiles[c]) continue;
             const uint32_t* tt = tile_task[c].empty() ? nullptr : d_tile_task[c].as<uint32_t>();
             const bool f64 = mode == MODE_F64, ddf = !f64 && std::strcmp(types, "ddf") == 0;
+            const bool dfd = !f64 && std::strcmp(types, "dfd") == 0;
             if (c == GEMM_CFG_128x128 && stacked128 && (f64 || ddf))
                 HIPCHK(launch_t_stack(st, mode, ddf, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
                                       d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
+            else if (c == GEMM_CFG_128x128 && units128 && (f64 || dfd))
+                HIPCHK(launch_f_unit(st, mode, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
+                                     d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
             else
                 HIPCHK(launch_gemm(st, c, mode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
                                    d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
@@ -1061,7 +1095,14 @@ struct mq_plan {
             if (g) (void)hipGraphExecDestroy(g), g = nullptr;
         for (auto& b : seen) b = false;
     }
-    ~mq_plan() { drop_graphs(); }
+    ~mq_plan() {
+        drop_graphs();
+        if (left) --left->plans;
+        if (right) --right->plans;
+    }
+    // the boundaries the plan reads: made resident (device copies complete in `ctx->stream` order)
+    // and pinned there until the plan is destroyed
+    void bind(const mq_boundary* l, const mq_boundary* r);
 
     // the base pointers of one application in the given mode
     Bases bases(const double* psi, double* sigma, int mode) {
@@ -1277,6 +1318,7 @@ int mq_ctx_create(int device, mq_ctx** out) {
             c->noalloc_scratch = it->second;
         }
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
         BLASCHK(rocblas_create_handle(&c->blas));
         BLASCHK(rocblas_set_stream(c->blas, c->stream));
         LTCHK(hipblasLtCreate(&c->lt));
@@ -1303,6 +1345,8 @@ int mq_ctx_destroy(mq_ctx* ctx) {
         g_contexts.erase(std::remove(g_contexts.begin(), g_contexts.end(), ctx), g_contexts.end());
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
         if (ctx->lt) (void)hipblasLtDestroy(ctx->lt);
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        (void)hipStreamDestroy(ctx->copy_stream);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -1421,6 +1465,96 @@ int mq_reshape_left_to_right(mq_ctx* ctx, const mq_site_desc* site, const double
 }
 
 // ---------------------------------------------------------------------------- boundaries
+// Residency of a boundary: bring an evicted boundary back to the device (allocate, copy its host
+// copy, read from the spill file first if there is none, on ctx->copy_stream, record `arrived`).
+static void boundary_prefetch(mq_boundary* b) {
+    if (b->resident) return;
+    mq_ctx* ctx = b->ctx;
+    const size_t bytes = b->total * sizeof(double);
+    if (!b->host) {
+        MQ_REQUIRE(!b->disk.empty(), "evicted boundary without a host or disk copy");
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->host), bytes, hipHostMallocDefault));
+        FILE* f = std::fopen(b->disk.c_str(), "rb");
+        MQ_REQUIRE(f != nullptr, "cannot open the boundary spill file " + b->disk);
+        const size_t got = std::fread(b->host, 1, bytes, f);
+        std::fclose(f);
+        MQ_REQUIRE(got == bytes, "short read of the boundary spill file " + b->disk);
+    }
+    b->data.alloc(bytes);
+    if (!b->arrived) HIPCHK(hipEventCreateWithFlags(&b->arrived, hipEventDisableTiming));
+    HIPCHK(hipMemcpyAsync(b->data.p, b->host, bytes, hipMemcpyHostToDevice, ctx->copy_stream));
+    HIPCHK(hipEventRecord(b->arrived, ctx->copy_stream));
+    b->resident = true;
+}
+
+// Make the work queued next on `st` see the boundary's device data, prefetching it first if it was
+// evicted (the reference fetches a boundary the same way before use, storage::gpu::fetch).  Only
+// residency changes: the data itself is immutable.
+static void boundary_ready(const mq_boundary* bc, hipStream_t st) {
+    mq_boundary* b = const_cast<mq_boundary*>(bc);
+    boundary_prefetch(b);
+    if (b->arrived) HIPCHK(hipStreamWaitEvent(st, b->arrived, 0));
+}
+
+void mq_plan::bind(const mq_boundary* l, const mq_boundary* r) {
+    if (l) {
+        boundary_ready(l, ctx->stream);
+        left = l;
+        ++l->plans;
+    }
+    if (r) {
+        boundary_ready(r, ctx->stream);
+        right = r;
+        ++r->plans;
+    }
+}
+
+int mq_boundary_evict(mq_ctx* ctx, mq_boundary* b, const char* path) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && b && b->ctx == ctx, "bad argument");
+        MQ_REQUIRE(b->plans == 0, "boundary read by a live sigma plan: destroy the plan first");
+        const size_t bytes = b->total * sizeof(double);
+        if (b->resident) {
+            if (b->arrived) HIPCHK(hipEventSynchronize(b->arrived));
+            if (!b->host && b->disk.empty()) {
+                HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->host), bytes, hipHostMallocDefault));
+                HIPCHK(hipMemcpyAsync(b->host, b->data.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+            }
+            HIPCHK(hipStreamSynchronize(ctx->stream));   // queued work that reads the boundary, and the copy
+            b->data.release();
+            b->resident = false;
+        }
+        if (path) {
+            if (b->disk.empty()) {
+                MQ_REQUIRE(b->host != nullptr, "boundary without a host copy to spill");
+                FILE* f = std::fopen(path, "wb");
+                MQ_REQUIRE(f != nullptr, std::string("cannot create the boundary spill file ") + path);
+                const size_t put = std::fwrite(b->host, 1, bytes, f);
+                const int rc = std::fclose(f);
+                MQ_REQUIRE(put == bytes && rc == 0, std::string("write of the boundary spill file failed: ") + path);
+                b->disk = path;
+            }
+            if (b->host) (void)hipHostFree(b->host), b->host = nullptr;
+        }
+    });
+}
+
+int mq_boundary_prefetch(mq_ctx* ctx, mq_boundary* b) {
+    return guarded([&] {
+        MQ_REQUIRE(ctx && b && b->ctx == ctx, "bad argument");
+        boundary_prefetch(b);
+    });
+}
+
+int mq_boundary_residency(const mq_boundary* b, int* where, uint64_t* device_bytes, uint64_t* host_bytes) {
+    return guarded([&] {
+        MQ_REQUIRE(b && where, "bad argument");
+        *where = b->resident ? MQ_RESIDENT_DEVICE : b->host ? MQ_RESIDENT_HOST : MQ_RESIDENT_DISK;
+        if (device_bytes) *device_bytes = b->data.bytes;
+        if (host_bytes) *host_bytes = b->host ? b->total * sizeof(double) : 0;
+    });
+}
+
 int mq_boundary_create_trivial(mq_ctx* ctx, const mq_index_desc* idx, uint32_t aux_dim, mq_boundary** out) {
     return guarded([&] {
         MQ_REQUIRE(ctx && idx && out && aux_dim > 0, "bad argument");
@@ -1489,6 +1623,7 @@ int mq_boundary_index(const mq_boundary* b, mq_bindex_desc* out) {
 int mq_boundary_download(const mq_boundary* b, uint32_t ci, double* host) {
     return guarded([&] {
         MQ_REQUIRE(b && host && ci < b->index.n_cohorts(), "bad argument");
+        boundary_ready(b, b->ctx->stream);
         uint64_t n = b->index.cohort_size(ci);
         if (n)
             HIPCHK(hipMemcpyAsync(host, b->ptr() + b->cohort_off[ci], n * sizeof(double), hipMemcpyDeviceToHost,
@@ -1500,6 +1635,7 @@ int mq_boundary_download(const mq_boundary* b, uint32_t ci, double* host) {
 int mq_boundary_cohort_ptr(const mq_boundary* b, uint32_t ci, const double** dev) {
     return guarded([&] {
         MQ_REQUIRE(b && dev && ci < b->index.n_cohorts(), "bad argument");
+        boundary_ready(b, b->ctx->stream);
         *dev = b->ptr() + b->cohort_off[ci];
     });
 }
@@ -1507,6 +1643,7 @@ int mq_boundary_cohort_ptr(const mq_boundary* b, uint32_t ci, const double** dev
 int mq_boundary_traces(const mq_boundary* b, double* out) {
     return guarded([&] {
         MQ_REQUIRE(b && out, "bad argument");
+        boundary_ready(b, b->ctx->stream);
         const BoundaryIndex& I = b->index;
         std::vector<uint64_t> off;
         std::vector<uint32_t> len, bb;
@@ -2795,8 +2932,7 @@ int mq_sigma_plan_create_from_schedule(mq_ctx* ctx, const mq_schedule_desc* sche
         S.import_desc(*sched, nc, nu, nl, nt);
         auto p = std::make_unique<mq_plan>();
         p->ctx = ctx;
-        p->left = left;
-        p->right = right;
+        p->bind(left, right);
         std::vector<uint32_t> rows(ket_rows, ket_rows + n_ket_blocks), cols(ket_cols, ket_cols + n_ket_blocks);
         std::vector<uint64_t> off(n_ket_blocks);
         uint64_t tot = 0;
@@ -2839,8 +2975,7 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         Mpo W(*mpo);
         auto p = std::make_unique<mq_plan>();
         p->ctx = ctx;
-        p->left = left;
-        p->right = right;
+        p->bind(left, right);
         p->vec_size = L.rp_total;
         uint64_t wd_elems = 0, ws_init_off = 0;
         trace_phase("plan: layout, mpo");
@@ -3412,6 +3547,7 @@ static int right_step_schedule(mq_ctx* ctx, const mq_site_desc* site, const mq_m
                                const double* ket_rp, const mq_boundary* right, int symmetric, mq_boundary** out) {
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && bra_rp && ket_rp && right && out, "bad argument");
+        boundary_ready(right, ctx->stream);
         SiteLayout L(*site);
         Mpo W(*mpo);
         Schedule S = build_right_schedule(L, W, right->index, symmetric != 0);
@@ -3489,8 +3625,7 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
             if (range.first >= range.second) continue;
             mq_plan p;
             p.ctx = ctx;
-            p.left = kind == Schedule::LEFT ? bnd : nullptr;
-            p.right = kind == Schedule::RIGHT ? bnd : nullptr;
+            p.bind(kind == Schedule::LEFT ? bnd : nullptr, kind == Schedule::RIGHT ? bnd : nullptr);
             uint64_t wd_elems = 0, ws_init_off = 0;
             GroupedStep spec{kind, &nb->index, &nb->cohort_off};
             spec.lo = range.first;
@@ -3542,6 +3677,7 @@ int mq_boundary_left_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo
                              mq_boundary** out) {
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && bra_lp && ket_rp && left && out, "bad argument");
+        boundary_ready(left, ctx->stream);
         SiteLayout L(*site);
         if (algorithm == MQ_SIGMA_AUTO) algorithm = MQ_SIGMA_GROUPED;
         int rc;
@@ -3569,6 +3705,7 @@ int mq_boundary_right_step_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mp
                               mq_boundary** out) {
     return guarded([&] {
         MQ_REQUIRE(ctx && site && mpo && bra_rp && ket_rp && right && out, "bad argument");
+        boundary_ready(right, ctx->stream);
         SiteLayout L(*site);
         if (algorithm == MQ_SIGMA_AUTO) algorithm = MQ_SIGMA_GROUPED;
         int rc;

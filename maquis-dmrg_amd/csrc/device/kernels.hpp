@@ -192,6 +192,12 @@ hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, con
 // A and B double; same tables, tile numbering and results as launch_gemm's GEMM_CFG_128x128.
 hipError_t launch_t_stack(hipStream_t st, int mode, bool c_float, const GemmTask* tasks, uint32_t n_tasks,
                           const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task);
+// 128 x 128 final-stage tasks whose segments all read B through column units (no row units, no
+// B block map, no beta, m_off 0) on the lean kernel of f_stage.hip: MODE_F64 (all double) or
+// MODE_F32 (A double, B float, C double); same tables, tile numbering and results as launch_gemm's
+// GEMM_CFG_128x128.
+hipError_t launch_f_unit(hipStream_t st, int mode, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
+                         uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task);
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
 constexpr int RED_BLOCKS = 512;      // fixed grid of the deterministic reductions

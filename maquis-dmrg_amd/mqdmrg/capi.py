@@ -108,6 +108,9 @@ SIGNATURES = {
     "mq_boundary_download": (C.c_int, [P, C.c_uint32, P]),
     "mq_boundary_cohort_ptr": (C.c_int, [P, C.c_uint32, PP]),
     "mq_boundary_traces": (C.c_int, [P, P]),
+    "mq_boundary_evict": (C.c_int, [P, P, C.c_char_p]),
+    "mq_boundary_prefetch": (C.c_int, [P, P]),
+    "mq_boundary_residency": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "mq_su2_coupling": (C.c_int, [C.c_int] * 9 + [C.c_double, C.POINTER(C.c_double)]),
     "mq_wigner9j": (C.c_double, [C.c_int] * 9),
     "mq_schedule_build_sigma": (C.c_int, [C.POINTER(mq_site_desc), C.POINTER(mq_mpo_desc),

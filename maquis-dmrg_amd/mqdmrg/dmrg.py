@@ -246,7 +246,12 @@ def svd_factors(mats, keys, direction, device, threads=8):
     (default priority)."""
     def one(c):
         X = mats[c] if direction > 0 else mats[c].t()
-        U, S, Vh = torch.linalg.svd(X, full_matrices=False)
+        try:
+            U, S, Vh = torch.linalg.svd(X, full_matrices=False)
+        except torch.linalg.LinAlgError:
+            # the device solver's iteration can fail to converge on matrices with many repeated or
+            # tiny singular values; the host LAPACK divide-and-conquer SVD of the same matrix does not
+            U, S, Vh = (t.to(X.device) for t in torch.linalg.svd(X.cpu(), full_matrices=False))
         return c, (U, S[:, None] * Vh, S)
 
     if torch.device(device).type != "cuda" or len(keys) < 4:
@@ -444,7 +449,8 @@ def dm_factors(mats, keys, direction, device, Mmax, cutoff, noise, alpha):
 class TwoSiteDMRG:
     def __init__(self, eng, model, L, m, cutoff=1e-10, init_m=16, seed=1, jd_tol=1e-8, jd_maxiter=8,
                  mpo=None, basis=None, target=None, verbose=False, hook=None, precision=E.PREC_F64, max_outer=1,
-                 init_bonds="sectors", tune_library=False, noise=0.0, **model_kw):
+                 init_bonds="sectors", tune_library=False, noise=0.0, boundary_cache=False, spill_dir=None,
+                 **model_kw):
         """precision != PREC_F64: sigma in that precision inside Jacobi-Davidson, with the FP64
         residual refinement of SigmaPlan.jd_refined (at most max_outer FP64 steps per site).  With
         max_outer = 1 a site costs jd_maxiter reduced-precision products and one FP64 product (the
@@ -463,6 +469,12 @@ class TwoSiteDMRG:
         end).  With alpha > 0 the split diagonalises rho = X X^T + alpha N (noise_density_left /
         _right, dm_factors) instead of taking the truncated SVD.
 
+        boundary_cache: keep only the boundaries of the current and the next site pair on the
+        device (the reference's storage::gpu policy around each site, ts_optimize.hpp:153-182,
+        307-345): a boundary is evicted to pinned host memory (spill_dir: to a file there) when the
+        sweep leaves it and prefetched on the copy stream one pair before it is read again;
+        boundaries the other half-sweep recomputes before reading are freed.
+
         init_bonds: "sectors" caps every charge sector of the initial MPS at init_m (the reference's
         mps initialisation, mps_sectors.h); "total" draws bond indices of total dimension ~init_m
         (mps.random_bond_dims), the shape a converged state at that bond dimension has."""
@@ -475,6 +487,8 @@ class TwoSiteDMRG:
         self.hook = hook
         self.noise = noise
         self.plain_desc = {}
+        self.boundary_cache, self.spill_dir = boundary_cache, spill_dir
+        self._spills = 0
         if mpo is None:
             basis, mpo, target = models.chain(model, L, **model_kw)
         self.basis, self.mpo, self.target = basis, mpo, target
@@ -558,6 +572,8 @@ class TwoSiteDMRG:
         grads, energies = [], []
         for j in range(self.L):
             s = self.sites[j]
+            if self.boundary_cache and j + 2 <= self.L and self.Rb[j + 2] is not None:
+                self.Rb[j + 2].prefetch()   # the next site's right boundary arrives meanwhile
             nrm2 = float(torch.dot(s.data, s.data))
             plan = E.SigmaPlan(self.eng, s.desc, self.mpo_desc[j], self.Lb[j], self.Rb[j + 1],
                                algorithm=E.SIGMA_GROUPED, adjoint=True)
@@ -571,6 +587,11 @@ class TwoSiteDMRG:
                 self.move_center_right(j)
                 a = self.sites[j]
                 self.Lb[j + 1] = E.left_step(self.eng, a.desc, self.mpo_desc[j], a.lp_data(), a.data, self.Lb[j])
+                if self.boundary_cache:   # both stay valid for the state; off the device until read again
+                    if j >= 1:
+                        self._evict(self.Lb[j])
+                    if j + 1 < self.L:
+                        self._evict(self.Rb[j + 1])
         return grads, energies
 
     # ------------------------------------------------------------------ two-site tensors
@@ -760,6 +781,56 @@ class TwoSiteDMRG:
         self.sites[i + 1] = SiteTensor.from_rp(self.eng, self.sym, newmid, self.phys, A2.right, rp_blocks)
         return tw, sum(newmid.sizes)
 
+    # ------------------------------------------------------------------ boundary residency
+    def _evict(self, b):
+        if b is None:
+            return
+        path = None
+        if self.spill_dir is not None:
+            import os
+            self._spills += 1
+            path = os.path.join(self.spill_dir, "boundary_%d_%d.bin" % (id(self), self._spills))
+        b.evict(path)
+
+    def _drop(self, side, k):
+        b = side[k]
+        if b is not None:
+            b.close()
+        side[k] = None
+
+    def _cache_before(self, i, direction):
+        """Prefetch the boundary the next site pair reads (it arrives while this pair computes)."""
+        if not self.boundary_cache:
+            return
+        nxt = None
+        if direction > 0 and i + 3 <= self.L:
+            nxt = self.Rb[i + 3]
+        elif direction < 0 and i >= 1:
+            nxt = self.Lb[i - 1]
+        if nxt is not None:
+            nxt.prefetch()
+
+    def _cache_after(self, i, direction):
+        """After pair i: the boundary this pair read on the sweep's trailing side is evicted (the
+        other half-sweep reads it again), the one on the leading side is freed (the other
+        half-sweep recomputes it before reading it).  The trivial end boundaries stay."""
+        if not self.boundary_cache:
+            return
+        if direction > 0:
+            if i >= 1:
+                self._evict(self.Lb[i])
+            if i + 2 < self.L:
+                self._drop(self.Rb, i + 2)
+        else:
+            if i + 2 < self.L:
+                self._evict(self.Rb[i + 2])
+            if i >= 1:
+                self._drop(self.Lb, i)
+
+    def resident_boundaries(self):
+        """Number of boundaries whose data is on the device."""
+        return sum(1 for b in list(self.Lb) + list(self.Rb) if b is not None and b.residency()[0] == E.DeviceBoundary.DEVICE)
+
     # ------------------------------------------------------------------ sweeps
     def init_boundaries(self):
         self.right_normalize()
@@ -771,6 +842,8 @@ class TwoSiteDMRG:
         for k in range(L - 1, 1, -1):
             s = self.sites[k]
             self.Rb[k] = E.right_step(self.eng, s.desc, self.mpo_desc[k], s.data, s.data, self.Rb[k + 1])
+            if self.boundary_cache and k + 1 < L and k + 1 > 3:
+                self._evict(self.Rb[k + 1])   # read again from the left end of the first sweep
 
     def multi_expval(self, mpo=None):
         """<psi|W|psi> per column index of the last MPO tensor for an MPO W on the chain (default:
@@ -796,6 +869,7 @@ class TwoSiteDMRG:
 
     def optimize_pair(self, i, direction):
         t0 = time.perf_counter()
+        self._cache_before(i, direction)
         W12, wdesc, phys12, pb = self.twosite_mpo(i)
         theta = self.merge(i)
         self.timings["merge"] += time.perf_counter() - t0
@@ -830,6 +904,7 @@ class TwoSiteDMRG:
         else:
             s = self.sites[i + 1]
             self.Rb[i + 1] = E.right_step(self.eng, s.desc, self.mpo_desc[i + 1], s.data, s.data, self.Rb[i + 2])
+        self._cache_after(i, direction)
         self.timings["boundary"] += time.perf_counter() - t0
         if self.verbose:
             print("pair %d dir %+d E=%.12f iters=%d bond=%d tw=%.2e" % (i, direction, e, it, bd, tw), flush=True)

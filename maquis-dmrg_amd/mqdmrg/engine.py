@@ -253,6 +253,23 @@ class DeviceBoundary:
             check(lib().mq_vec_copy(self.engine.ctx, n, p, out.data_ptr()))
         return out
 
+    # residency (mq_boundary_evict / _prefetch / _residency) ---------------------------------
+    DEVICE, HOST, DISK = 0, 1, 2
+
+    def evict(self, path=None):
+        """Free the device copy (keeping a pinned host copy, or with path a spill file)."""
+        check(lib().mq_boundary_evict(self.engine.ctx, self.h, None if path is None else str(path).encode()))
+
+    def prefetch(self):
+        """Start bringing an evicted boundary back to the device (asynchronous copy stream)."""
+        check(lib().mq_boundary_prefetch(self.engine.ctx, self.h))
+
+    def residency(self):
+        """(where: DEVICE / HOST / DISK, device bytes, pinned host bytes)."""
+        w, d, hb = C.c_int(), C.c_uint64(), C.c_uint64()
+        check(lib().mq_boundary_residency(self.h, C.byref(w), C.byref(d), C.byref(hb)))
+        return w.value, d.value, hb.value
+
     def traces(self):
         a = np.zeros(self.index.aux_dim, dtype=np.float64)
         check(lib().mq_boundary_traces(self.h, a.ctypes.data))
