@@ -438,6 +438,8 @@ struct GemmStage {
         for (int c = 0; c < GEMM_N_CFG; ++c) {
             if (!tiles[c]) continue;
             const uint32_t* tt = tile_task[c].empty() ? nullptr : d_tile_task[c].as<uint32_t>();
+            // MODE_BF16X3 reaches the lean kernels (split products); the generic kernel runs it in FP32
+            const int gmode = mode == MODE_BF16X3 ? MODE_F32 : mode;
             const bool f64 = mode == MODE_F64, ddf = !f64 && std::strcmp(types, "ddf") == 0;
             const bool dfd = !f64 && std::strcmp(types, "dfd") == 0;
             if (c == GEMM_CFG_128x128 && stacked128 && (f64 || ddf))
@@ -447,7 +449,7 @@ struct GemmStage {
                 HIPCHK(launch_f_unit(st, mode, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
                                      d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
             else
-                HIPCHK(launch_gemm(st, c, mode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
+                HIPCHK(launch_gemm(st, c, gmode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
                                    d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
         }
     }
@@ -961,14 +963,13 @@ struct StagePlan {
                                               mixed ? 4 : 8));
                 break;
             case 1:
-                // the T and final stages are bandwidth-bound block products: BF16x3 runs them in FP32
-                // and uses the split products in the S stage only
+                // BF16x3: split products in the lean 128 x 128 T tasks (t_stage.hip), FP32 elsewhere
                 if (mode == MODE_BF16X3 && split_t && !bsr_on) {
                     Bases bs = b;   // C of the stacked T tasks: the A' copies in the bf16 buffer
                     bs.p[BASE_WS] = reinterpret_cast<double*>(bf);
                     gT.run(st, MODE_F32, "dds", bs, tb);
                 } else {
-                    gT.run(st, mixed ? MODE_F32 : MODE_F64, "ddf", b, tb);
+                    gT.run(st, mode, "ddf", b, tb);
                 }
                 break;
             case 2:
@@ -982,7 +983,7 @@ struct StagePlan {
                 if (!mixed) run_gather(st, b.p[BASE_WS], b.p[BASE_WS]);
                 break;
             default:
-                gF.run(st, mixed ? MODE_F32 : MODE_F64, "dfd", b, tb);
+                gF.run(st, mode, "dfd", b, tb);
                 red.run(st, b.p[BASE_BRA], b.p[BASE_OUT]);
         }
     }

@@ -22,7 +22,8 @@
 //     the B side.
 //
 // MODE_F64: FP64 in LDS, v_mfma_f64_16x16x4f64; MODE_F32: FP32 in LDS, v_mfma_f32_16x16x4f32 (TB
-// float: the reduced-precision S_D).  C is FP64.
+// float: the reduced-precision S_D); MODE_BF16X3: A and B split while staged into bf16 hi / lo and
+// three v_mfma_f32_16x16x16bf16 products per block and step (the t_stage.hip scheme).  C is FP64.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -37,6 +38,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef double d2_a8 __attribute__((ext_vector_type(2), aligned(8)));
 typedef float f4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
 constexpr int FM = 128, FN = 128, FK = 16, FT = GEMM_THREADS;
 
@@ -44,11 +47,24 @@ template <int MODE> struct FsTraits;
 template <> struct FsTraits<MODE_F64> { typedef double T; typedef d4 Acc; static constexpr int P = FK + 2; };
 template <> struct FsTraits<MODE_F32> { typedef float T; typedef f4 Acc; static constexpr int P = FK + 4; };
 
+template <> struct FsTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; static constexpr int P = FK + 8; };
+
 template <int MODE> struct FsSmem {
     typedef typename FsTraits<MODE>::T T;
     T a[FM][FsTraits<MODE>::P];
     T b[FN][FsTraits<MODE>::P];
 };
+template <> struct FsSmem<MODE_BF16X3> {
+    static constexpr int P = FsTraits<MODE_BF16X3>::P;
+    uint16_t ahi[FM][P], alo[FM][P], bhi[FN][P], blo[FN][P];
+};
+
+// hi = bf16(x), lo = bf16(x - hi) of two elements, each pair packed lower element first
+__device__ inline void fsplit2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+    hi = __builtin_bit_cast(uint32_t, bf2{(__bf16)x0, (__bf16)x1});
+    lo = __builtin_bit_cast(uint32_t, bf2{(__bf16)(x0 - __uint_as_float(hi << 16)),
+                                          (__bf16)(x1 - __uint_as_float(hi & 0xffff0000u))});
+}
 
 __device__ inline uint32_t f_task_of(const GemmTask* tasks, uint32_t n_tasks, const uint32_t* tile_task, uint32_t tile) {
     if (tile_task) return tile_task[tile];
@@ -190,7 +206,18 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
             if (part && kb + e >= sg.K) xa[e] = T(0);
             xb[e] = u_ok ? T(vb[e]) : T(0);
         }
-        if constexpr (MODE == MODE_F32) {
+        if constexpr (MODE == MODE_BF16X3) {
+            uint32_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                fsplit2(xa[2 * e], xa[2 * e + 1], ah[e], al[e]);
+                fsplit2(xb[2 * e], xb[2 * e + 1], bh[e], bl[e]);
+            }
+            *reinterpret_cast<uint4*>(&s.ahi[r_i][r_k]) = uint4{ah[0], ah[1], ah[2], ah[3]};
+            *reinterpret_cast<uint4*>(&s.alo[r_i][r_k]) = uint4{al[0], al[1], al[2], al[3]};
+            *reinterpret_cast<uint4*>(&s.bhi[r_i][r_k]) = uint4{bh[0], bh[1], bh[2], bh[3]};
+            *reinterpret_cast<uint4*>(&s.blo[r_i][r_k]) = uint4{bl[0], bl[1], bl[2], bl[3]};
+        } else if constexpr (MODE == MODE_F32) {
 #pragma unroll
             for (int v = 0; v < 2; ++v) {
                 *reinterpret_cast<float4*>(&s.a[r_i][r_k + 4 * v]) = float4{xa[4 * v], xa[4 * v + 1], xa[4 * v + 2], xa[4 * v + 3]};
@@ -206,37 +233,56 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
     };
     auto compute = [&](int buf) {
         const FsSmem<MODE>& s = sm[buf];
-        T a[4][4], b[4][4];
+        if constexpr (MODE == MODE_BF16X3) {
+            s4 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const T* pa = &s.a[wm + 16 * i + l16][4 * lk];
-            const T* pb = &s.b[wn + 16 * i + l16][4 * lk];
-            if constexpr (MODE == MODE_F32) {
-                const float4 x = *reinterpret_cast<const float4*>(pa);
-                const float4 y = *reinterpret_cast<const float4*>(pb);
-                a[i][0] = x.x, a[i][1] = x.y, a[i][2] = x.z, a[i][3] = x.w;
-                b[i][0] = y.x, b[i][1] = y.y, b[i][2] = y.z, b[i][3] = y.w;
-            } else {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const double2 x = *reinterpret_cast<const double2*>(pa + 2 * h);
-                    const double2 y = *reinterpret_cast<const double2*>(pb + 2 * h);
-                    a[i][2 * h] = x.x, a[i][2 * h + 1] = x.y;
-                    b[i][2 * h] = y.x, b[i][2 * h + 1] = y.y;
-                }
+            for (int i = 0; i < 4; ++i) {
+                ah[i] = *reinterpret_cast<const s4*>(&s.ahi[wm + 16 * i + l16][4 * lk]);
+                al[i] = *reinterpret_cast<const s4*>(&s.alo[wm + 16 * i + l16][4 * lk]);
+                bh[i] = *reinterpret_cast<const s4*>(&s.bhi[wn + 16 * i + l16][4 * lk]);
+                bl[i] = *reinterpret_cast<const s4*>(&s.blo[wn + 16 * i + l16][4 * lk]);
             }
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    if constexpr (MODE == MODE_F64)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
-                    else
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
                 }
+        } else {
+            T a[4][4], b[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const T* pa = &s.a[wm + 16 * i + l16][4 * lk];
+                const T* pb = &s.b[wn + 16 * i + l16][4 * lk];
+                if constexpr (MODE == MODE_F32) {
+                    const float4 x = *reinterpret_cast<const float4*>(pa);
+                    const float4 y = *reinterpret_cast<const float4*>(pb);
+                    a[i][0] = x.x, a[i][1] = x.y, a[i][2] = x.z, a[i][3] = x.w;
+                    b[i][0] = y.x, b[i][1] = y.y, b[i][2] = y.z, b[i][3] = y.w;
+                } else {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const double2 x = *reinterpret_cast<const double2*>(pa + 2 * h);
+                        const double2 y = *reinterpret_cast<const double2*>(pb + 2 * h);
+                        a[i][2 * h] = x.x, a[i][2 * h + 1] = x.y;
+                        b[i][2 * h] = y.x, b[i][2 * h + 1] = y.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if constexpr (MODE == MODE_F64)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
+                        else
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
+                    }
+        }
     };
 
     // ---- the K steps of all segments: compute step t while step t + 1 loads (one barrier per step)
@@ -267,7 +313,7 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
         double* __restrict__ Cn = C + uint64_t(gn) * tk.ldc;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            if constexpr (MODE == MODE_F32) {
+            if constexpr (MODE != MODE_F64) {
                 const uint32_t gm0 = m0 + wm + 16 * i + 4 * lk;
                 if (gm0 + 3 < M) {
                     d2_a8* q = reinterpret_cast<d2_a8*>(Cn + gm0);
@@ -295,6 +341,8 @@ hipError_t launch_f_unit(hipStream_t st, int mode, const GemmTask* tasks, uint32
         hipLaunchKernelGGL((f_unit_kernel<MODE_F64, double>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else if (mode == MODE_F32)
         hipLaunchKernelGGL((f_unit_kernel<MODE_F32, float>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
+    else if (mode == MODE_BF16X3)
+        hipLaunchKernelGGL((f_unit_kernel<MODE_BF16X3, float>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -133,9 +133,10 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t n) {
 }
 
 constexpr uint32_t MAP_NONE = 0xffffffffu;
-// Arithmetic of an application (mq_precision).  The hand-written GEMM runs MODE_F64 and MODE_F32;
-// MODE_BF16X3 applications run their T and final stages in MODE_F32 and the S stage as bf16
-// library GEMMs.
+// Arithmetic of an application (mq_precision).  MODE_BF16X3: split products (hi*lo + lo*hi + hi*hi
+// of bf16 hi / lo parts, FP32 accumulation) in the S-stage kernel and the lean 128 x 128 T and
+// final kernels; the generic GEMM kernel runs MODE_F64 and MODE_F32 only and takes BF16X3's other
+// tasks in MODE_F32.
 enum GemmMode { MODE_F64 = 0, MODE_F32 = 1, MODE_BF16X3 = 2 };
 
 // Tile configurations of the GEMM (gemm.hip); a task runs with the one that fits its (M, N).

@@ -19,7 +19,11 @@
 //     columns beyond the tile's range read clamped, finite data into outputs that are not stored.
 //
 // MODE_F64: FP64 operands in LDS, v_mfma_f64_16x16x4f64; MODE_F32: FP32 in LDS (converted while
-// staged, A scaled by its unit and segment factors), v_mfma_f32_16x16x4f32.  C is FP64 or FP32.
+// staged, A scaled by its unit and segment factors), v_mfma_f32_16x16x4f32; MODE_BF16X3: each FP32
+// value split while staged into hi = bf16(x), lo = bf16(x - hi) (gfx950's v_cvt_pk_bf16_f32, round to
+// nearest even) and three v_mfma_f32_16x16x16bf16 products per block and 16-key step (hi*lo + lo*hi +
+// hi*hi, FP32 accumulation: the S stage's split-product scheme; lane group lk supplies keys
+// 4 lk .. 4 lk + 3 as one 8-byte LDS read).  C is FP64 or FP32.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -33,6 +37,8 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef double d2_a8 __attribute__((ext_vector_type(2), aligned(8)));   // 8-byte aligned global runs
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
 constexpr int TM = 128, TN = 128, TK = 16, TT = GEMM_THREADS;
 
@@ -40,11 +46,25 @@ template <int MODE> struct TsTraits;
 template <> struct TsTraits<MODE_F64> { typedef double T; typedef d4 Acc; static constexpr int P = TK + 2; };
 template <> struct TsTraits<MODE_F32> { typedef float T; typedef f4 Acc; static constexpr int P = TK + 4; };
 
+template <> struct TsTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; static constexpr int P = TK + 8; };
+
 template <int MODE> struct TsSmem {
     typedef typename TsTraits<MODE>::T T;
     T a[TM][TsTraits<MODE>::P];
     T b[TN][TsTraits<MODE>::P];
 };
+// BF16X3: hi and lo bf16 of A and B, rows of 16 keys + 8 pad (48 bytes, 16-byte aligned runs)
+template <> struct TsSmem<MODE_BF16X3> {
+    static constexpr int P = TsTraits<MODE_BF16X3>::P;
+    uint16_t ahi[TM][P], alo[TM][P], bhi[TN][P], blo[TN][P];
+};
+
+// hi = bf16(x), lo = bf16(x - hi) of two elements, each pair packed lower element first
+__device__ inline void split2(float x0, float x1, uint32_t& hi, uint32_t& lo) {
+    hi = __builtin_bit_cast(uint32_t, bf2{(__bf16)x0, (__bf16)x1});
+    lo = __builtin_bit_cast(uint32_t, bf2{(__bf16)(x0 - __uint_as_float(hi << 16)),
+                                          (__bf16)(x1 - __uint_as_float(hi & 0xffff0000u))});
+}
 
 __device__ inline uint32_t task_of(const GemmTask* tasks, uint32_t n_tasks, const uint32_t* tile_task, uint32_t tile) {
     if (tile_task) return tile_task[tile];
@@ -58,7 +78,7 @@ __device__ inline uint32_t task_of(const GemmTask* tasks, uint32_t n_tasks, cons
 }
 
 template <int MODE, typename TC>
-__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_F64 ? 1 : 3))) void t_stack_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_F64 ? 1 : MODE == MODE_F32 ? 3 : 2))) void t_stack_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                      const GemmSeg* __restrict__ segs, Bases bases, GemmTables tables,
                                                      const uint32_t* __restrict__ tile_task) {
     typedef typename TsTraits<MODE>::T T;
@@ -145,7 +165,18 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(MODE == MODE
             if (part && k0 + uint32_t(a_k + e) >= K) xa[e] = T(0);
             xb[e] = T(vb[e]);
         }
-        if constexpr (MODE == MODE_F32) {
+        if constexpr (MODE == MODE_BF16X3) {
+            uint32_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                split2(xa[2 * e], xa[2 * e + 1], ah[e], al[e]);
+                split2(xb[2 * e], xb[2 * e + 1], bh[e], bl[e]);
+            }
+            *reinterpret_cast<uint4*>(&s.ahi[a_m][a_k]) = uint4{ah[0], ah[1], ah[2], ah[3]};
+            *reinterpret_cast<uint4*>(&s.alo[a_m][a_k]) = uint4{al[0], al[1], al[2], al[3]};
+            *reinterpret_cast<uint4*>(&s.bhi[b_n][b_k]) = uint4{bh[0], bh[1], bh[2], bh[3]};
+            *reinterpret_cast<uint4*>(&s.blo[b_n][b_k]) = uint4{bl[0], bl[1], bl[2], bl[3]};
+        } else if constexpr (MODE == MODE_F32) {
 #pragma unroll
             for (int v = 0; v < 2; ++v) {
                 *reinterpret_cast<float4*>(&s.a[a_m][a_k + 4 * v]) = float4{xa[4 * v], xa[4 * v + 1], xa[4 * v + 2], xa[4 * v + 3]};
@@ -161,37 +192,56 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(MODE == MODE
     };
     auto compute = [&](int buf) {
         const TsSmem<MODE>& s = sm[buf];
-        T a[4][4], b[4][4];   // [block][k-step]: keys 4 lk + t
+        if constexpr (MODE == MODE_BF16X3) {
+            s4 ah[4], al[4], bh[4], bl[4];   // keys 4 lk .. 4 lk + 3 of one row / column
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const T* pa = &s.a[wm + 16 * i + l16][4 * lk];
-            const T* pb = &s.b[wn + 16 * i + l16][4 * lk];
-            if constexpr (MODE == MODE_F32) {
-                const float4 x = *reinterpret_cast<const float4*>(pa);
-                const float4 y = *reinterpret_cast<const float4*>(pb);
-                a[i][0] = x.x, a[i][1] = x.y, a[i][2] = x.z, a[i][3] = x.w;
-                b[i][0] = y.x, b[i][1] = y.y, b[i][2] = y.z, b[i][3] = y.w;
-            } else {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const double2 x = *reinterpret_cast<const double2*>(pa + 2 * h);
-                    const double2 y = *reinterpret_cast<const double2*>(pb + 2 * h);
-                    a[i][2 * h] = x.x, a[i][2 * h + 1] = x.y;
-                    b[i][2 * h] = y.x, b[i][2 * h + 1] = y.y;
-                }
+            for (int i = 0; i < 4; ++i) {
+                ah[i] = *reinterpret_cast<const s4*>(&s.ahi[wm + 16 * i + l16][4 * lk]);
+                al[i] = *reinterpret_cast<const s4*>(&s.alo[wm + 16 * i + l16][4 * lk]);
+                bh[i] = *reinterpret_cast<const s4*>(&s.bhi[wn + 16 * i + l16][4 * lk]);
+                bl[i] = *reinterpret_cast<const s4*>(&s.blo[wn + 16 * i + l16][4 * lk]);
             }
-        }
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    if constexpr (MODE == MODE_F64)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
-                    else
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
                 }
+        } else {
+            T a[4][4], b[4][4];   // [block][k-step]: keys 4 lk + t
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const T* pa = &s.a[wm + 16 * i + l16][4 * lk];
+                const T* pb = &s.b[wn + 16 * i + l16][4 * lk];
+                if constexpr (MODE == MODE_F32) {
+                    const float4 x = *reinterpret_cast<const float4*>(pa);
+                    const float4 y = *reinterpret_cast<const float4*>(pb);
+                    a[i][0] = x.x, a[i][1] = x.y, a[i][2] = x.z, a[i][3] = x.w;
+                    b[i][0] = y.x, b[i][1] = y.y, b[i][2] = y.z, b[i][3] = y.w;
+                } else {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const double2 x = *reinterpret_cast<const double2*>(pa + 2 * h);
+                        const double2 y = *reinterpret_cast<const double2*>(pb + 2 * h);
+                        a[i][2 * h] = x.x, a[i][2 * h + 1] = x.y;
+                        b[i][2 * h] = y.x, b[i][2 * h + 1] = y.y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if constexpr (MODE == MODE_F64)
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
+                        else
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
+                    }
+        }
     };
 
     // ---- the K steps: compute step k0 while step k0 + 16 loads (one barrier per step)
@@ -235,7 +285,7 @@ __global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(MODE == MODE
                 coff = uint64_t(gn) * R.ldc;
             }
             TC* __restrict__ Cc = C + R.c_src + coff;
-            if constexpr (MODE == MODE_F32 && std::is_same<TC, float>::value) {
+            if constexpr (MODE != MODE_F64 && std::is_same<TC, float>::value) {
                 const uint32_t li0 = mb + 4 * lk - R.row0;
                 if (R.c_si == 1 && mb + 4 * lk + 3 < M && li0 + 3 < R.nrows) {
                     typedef float __attribute__((ext_vector_type(4), aligned(4))) f4_a4;
@@ -263,6 +313,8 @@ hipError_t launch_t_stack(hipStream_t st, int mode, bool c_float, const GemmTask
         hipLaunchKernelGGL((t_stack_kernel<MODE_F64, double>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else if (mode == MODE_F32 && c_float)
         hipLaunchKernelGGL((t_stack_kernel<MODE_F32, float>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
+    else if (mode == MODE_BF16X3 && c_float)
+        hipLaunchKernelGGL((t_stack_kernel<MODE_BF16X3, float>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else if (mode == MODE_F32)
         hipLaunchKernelGGL((t_stack_kernel<MODE_F32, double>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else
