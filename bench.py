@@ -68,7 +68,8 @@ def parse():
     ap.add_argument("--headline-which", default="both", choices=["both", "f64", "reduced"],
                     help="run the headline sweeps in FP64, in the reported precision, or both")
     ap.add_argument("--headline-verbose", action="store_true",
-                    help="print every site pair of the headline sweeps (energy, iterations, bond dimension)")
+                    help="print every site pair of the headline (and differentiation) sweeps: energy, "
+                         "iterations, bond dimension")
     ap.add_argument("--adjoint", action="store_true",
                     help="also run the differentiation workload: two-site DMRG of the ab initio chain "
                          "(--adjoint-L orbitals, N = L, bond dimension --adjoint-m), the sigma-adjoint sweep "
@@ -190,7 +191,8 @@ def adjoint_workload(eng, args, su2_model):
     L, m = args.adjoint_L, args.adjoint_m
     N = L
     h, eri = qc.synthetic_integrals(L, seed=0)
-    kw = dict(m=m, cutoff=1e-12, init_m=16, seed=3, jd_tol=1e-10, jd_maxiter=12, boundary_cache=args.adjoint_cache)
+    kw = dict(m=m, cutoff=1e-12, init_m=16, seed=3, jd_tol=1e-10, jd_maxiter=12, boundary_cache=args.adjoint_cache,
+              verbose=args.headline_verbose)
     model = "qc_su2" if su2_model else "qc"
     target = (N, 0) if su2_model else (N // 2, N - N // 2)
 
