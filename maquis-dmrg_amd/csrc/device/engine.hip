@@ -545,6 +545,10 @@ struct StagePlan {
         uint64_t tiles = 0;
         uint64_t flops = 0;             // MFMA flops of the visited tiles (unpadded)
         bool narrow = true;             // every task's T_D step window fits 32-bit byte offsets
+        // per tile: its group, first key and first column: the dense W_D can be rebuilt from the
+        // tiles (the flop model reads W_D; the device keeps only the tiles)
+        struct TileAt { uint32_t group, k0, n0; };
+        std::vector<TileAt> tile_at;
         DevBuf d_tasks, d_runs, d_w64, d_w32, d_whi, d_wlo;
     } bsr;
     bool bsr_on = false;
@@ -1115,7 +1119,7 @@ struct mq_plan {
             if (!ws32.p) {
                 // the FP32 workspace; W_D converted on the device from the FP64 copy
                 ctx->take(mq_ctx::POOL_WS32, ws32, std::max<uint64_t>(st.ws_elems, 64) * sizeof(float));
-                if (ws_init_elems)
+                if (ws_init_elems && !st.bsr_on)
                     HIPCHK(launch_to_float(ctx->stream, ws.as<double>() + ws_init_off, ws32.as<float>() + ws_init_off,
                                            ws_init_elems));
             }
@@ -2066,6 +2070,8 @@ void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>
             for (const auto& r : S.panel_runs[j]) {
                 const uint32_t k0 = r.first * TK, k1 = std::min(Kg, (r.first + r.second) * TK);
                 B.runs.push_back({k0, k1 - k0, uint32_t(tile), 0u});
+                for (uint32_t t = 0; t < r.second; ++t)
+                    B.tile_at.push_back({uint32_t(gi), (r.first + t) * TK, j * TN});
                 tile += r.second;
             }
             pr[j].second = uint32_t(B.runs.size());
@@ -2904,9 +2910,28 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     }
     auto lowered = std::make_shared<const GroupedLowering>(std::move(G));
     p->s_flops = [lowered, p, lsz = left_i.sizes, rsz = right_i.sizes] {
-        // W_D from the plan's workspace (no application writes that region)
         std::vector<double> wd(p->ws_init_elems);
-        if (!wd.empty()) {
+        const StagePlan::Bsr& B = p->st.bsr;
+        if (!wd.empty() && p->st.bsr_on) {
+            // the block-sparse plan keeps only W_D's nonzero tiles on the device: rebuild W_D from them
+            constexpr uint32_t TK = S_BSR_TILE_K, TN = S_BSR_TILE_N;
+            std::vector<double> tiles(B.tiles * TN * TK);
+            if (!tiles.empty()) {
+                HIPCHK(hipMemcpyAsync(tiles.data(), B.d_w64.p, tiles.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      p->ctx->stream));
+                HIPCHK(hipStreamSynchronize(p->ctx->stream));
+            }
+            for (uint64_t t = 0; t < B.tile_at.size(); ++t) {
+                const auto& at = B.tile_at[t];
+                const auto& g = lowered->groups[at.group];
+                const uint64_t Kg = g.K.size(), Ng = g.N.size();
+                double* Wg = wd.data() + (g.w_off - p->ws_init_off);
+                for (uint32_t n = 0; n < TN && at.n0 + n < Ng; ++n)
+                    for (uint32_t k = 0; k < TK && at.k0 + k < Kg; ++k)
+                        Wg[uint64_t(at.n0 + n) * Kg + at.k0 + k] = tiles[t * TN * TK + n * TK + k];
+            }
+        } else if (!wd.empty()) {
+            // W_D from the plan's workspace (no application writes that region)
             HIPCHK(hipMemcpyAsync(wd.data(), p->ws.as<double>() + p->ws_init_off, wd.size() * sizeof(double),
                                   hipMemcpyDeviceToHost, p->ctx->stream));
             HIPCHK(hipStreamSynchronize(p->ctx->stream));
@@ -2995,7 +3020,8 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         P.validate();
         if (P.part_elems) ctx->take(mq_ctx::POOL_PART, p->part, P.part_elems * sizeof(double));
         ctx->take(mq_ctx::POOL_WS, p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
-        if (wd_elems)
+        // the dense W_D serve the library S stage and the adjoint; the block-sparse S stage reads tiles
+        if (wd_elems && !P.bsr_on)
             HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ctx->wd_host, wd_elems * sizeof(double),
                                   hipMemcpyHostToDevice, ctx->stream));
         trace_phase("plan: workspaces");
@@ -3652,7 +3678,7 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
             const size_t need = std::max<uint64_t>(P.ws_elems, 32) * sizeof(double);
             if (need > ctx->step_ws.bytes) ctx->step_ws.alloc_zeroed(need, ctx->stream);
             double* ws = ctx->step_ws.as<double>();
-            if (wd_elems)
+            if (wd_elems && !P.bsr_on)
                 HIPCHK(hipMemcpyAsync(ws + ws_init_off, ctx->wd_host, wd_elems * sizeof(double), hipMemcpyHostToDevice,
                                       ctx->stream));
             Bases bs{};

@@ -95,7 +95,7 @@ __device__ inline void load8(const float* p, float* v) {
 }
 
 template <int MODE, typename TB>
-__global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_F64 ? 1 : 2)))
+__global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(2)))
 void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const GemmSeg* __restrict__ segs,
                    Bases bases, GemmTables tables, const uint32_t* __restrict__ tile_task) {
     typedef typename FsTraits<MODE>::T T;

@@ -78,7 +78,7 @@ template <> struct BsrSmem<MODE_BF16X3> {
 };
 
 template <int MODE, bool NARROW>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_F64 ? 1 : 2))) void s_bsr_kernel(const BsrTask* __restrict__ tasks, const BsrRun* __restrict__ runs,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void s_bsr_kernel(const BsrTask* __restrict__ tasks, const BsrRun* __restrict__ runs,
                                                    const void* __restrict__ bt0, const void* __restrict__ bt1,
                                                    void* __restrict__ ws) {
     typedef typename BsrTraits<MODE>::T T;

@@ -78,7 +78,7 @@ __device__ inline uint32_t task_of(const GemmTask* tasks, uint32_t n_tasks, cons
 }
 
 template <int MODE, typename TC>
-__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_F64 ? 1 : MODE == MODE_F32 ? 3 : 2))) void t_stack_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
+__global__ __launch_bounds__(TT) __attribute__((amdgpu_waves_per_eu(MODE == MODE_F32 ? 3 : MODE == MODE_F64 ? 1 : 2))) void t_stack_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks,
                                                      const GemmSeg* __restrict__ segs, Bases bases, GemmTables tables,
                                                      const uint32_t* __restrict__ tile_task) {
     typedef typename TsTraits<MODE>::T T;
