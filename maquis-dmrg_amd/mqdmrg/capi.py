@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "libmqdmrg.so")
+# MQ_LIB_PATH: another build of the same library (the sanitizer test's host-instrumented build)
+LIB_PATH = os.environ.get("MQ_LIB_PATH") or os.path.join(os.path.dirname(HERE), "libmqdmrg.so")
 
 MQ_OK = 0
 MQ_NONE = 0xFFFFFFFF

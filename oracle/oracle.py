@@ -10,7 +10,8 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "_build", "libmqoracle.so")
+# MQ_ORACLE_LIB: another build of the oracle (the sanitizer test's instrumented build)
+LIB = os.environ.get("MQ_ORACLE_LIB") or os.path.join(HERE, "_build", "libmqoracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "maquis-dmrg_amd"))
 
 from mqdmrg import capi  # noqa: E402  (descriptor structs only)
