@@ -360,8 +360,9 @@ struct GemmStage {
     std::vector<uint32_t> tile_task[GEMM_N_CFG];   // launches with many multi-tile tasks: task of each tile
     DevBuf d_tasks[GEMM_N_CFG], d_tile_task[GEMM_N_CFG];
     DevBuf d_segs;
-    bool stacked128 = false;   // every 128x128 task suits the lean stacked T-stage kernel (t_stage.hip)
-    bool units128 = false;     // every 128x128 task suits the lean column-unit final-stage kernel (f_stage.hip)
+    // per tile configuration (128x128 and 64x64): every task of it suits the lean stacked T-stage
+    // kernel (t_stage.hip) / the lean column-unit final-stage kernel (f_stage.hip)
+    bool stacked[GEMM_N_CFG] = {}, units[GEMM_N_CFG] = {};
 
     // Flops per size class: the block-size histogram the autotuner's decisions are keyed on.
     std::vector<uint64_t> class_flops(const std::vector<GemmSeg>& segs) const {
@@ -408,25 +409,28 @@ struct GemmStage {
     void upload_all(const std::vector<GemmSeg>& segs, hipStream_t st) {
         static const bool generic = std::getenv("MQ_T_GENERIC") != nullptr;
         static const bool generic_f = std::getenv("MQ_F_GENERIC") != nullptr;
-        stacked128 = !generic && !cfg[GEMM_CFG_128x128].empty();
-        units128 = !generic_f && !cfg[GEMM_CFG_128x128].empty();
-        for (const auto& t : cfg[GEMM_CFG_128x128]) {
-            bool ok = t.rowunit == MAP_NONE && t.colunit != MAP_NONE && !t.beta && t.m_off == 0 &&
-                      t.cmap == MAP_NONE && t.c_base != BASE_WS && t.M > 0 && t.N > 0 && t.seg1 > t.seg0;
-            for (uint32_t k = t.seg0; ok && k < t.seg1; ++k) {
-                const GemmSeg& g = segs[k];
-                ok = g.units != MAP_NONE && g.bmap == MAP_NONE && !g.trans_b && g.bbh > 0 &&
-                     (g.amap == MAP_NONE ? !g.trans_a : g.trans_a && g.ascale != MAP_NONE);
+        for (int c = 0; c < GEMM_N_CFG; ++c) {
+            const bool lean = c == GEMM_CFG_128x128 || c == GEMM_CFG_64x64;
+            units[c] = lean && !generic_f && !cfg[c].empty();
+            stacked[c] = lean && !generic && !cfg[c].empty();
+            for (const auto& t : cfg[c]) {
+                bool ok = t.rowunit == MAP_NONE && t.colunit != MAP_NONE && !t.beta && t.m_off == 0 &&
+                          t.cmap == MAP_NONE && t.c_base != BASE_WS && t.M > 0 && t.N > 0 && t.seg1 > t.seg0;
+                for (uint32_t k = t.seg0; ok && k < t.seg1; ++k) {
+                    const GemmSeg& g = segs[k];
+                    ok = g.units != MAP_NONE && g.bmap == MAP_NONE && !g.trans_b && g.bbh > 0 &&
+                         (g.amap == MAP_NONE ? !g.trans_a : g.trans_a && g.ascale != MAP_NONE);
+                }
+                units[c] = units[c] && ok;
             }
-            units128 = units128 && ok;
-        }
-        for (const auto& t : cfg[GEMM_CFG_128x128]) {
-            bool ok = t.rowunit != MAP_NONE && t.seg1 == t.seg0 + 1 && !t.beta && t.M > 0 && t.N > 0;
-            if (ok) {
-                const GemmSeg& g = segs[t.seg0];
-                ok = g.amap == MAP_NONE && g.bmap == MAP_NONE && g.units == MAP_NONE && !g.trans_a;
+            for (const auto& t : cfg[c]) {
+                bool ok = t.rowunit != MAP_NONE && t.seg1 == t.seg0 + 1 && !t.beta && t.M > 0 && t.N > 0;
+                if (ok) {
+                    const GemmSeg& g = segs[t.seg0];
+                    ok = g.amap == MAP_NONE && g.bmap == MAP_NONE && g.units == MAP_NONE && !g.trans_a;
+                }
+                stacked[c] = stacked[c] && ok;
             }
-            stacked128 = stacked128 && ok;
         }
         upload(d_segs, segs, st);
         for (int c = 0; c < GEMM_N_CFG; ++c) {
@@ -442,11 +446,11 @@ struct GemmStage {
             const int gmode = mode == MODE_BF16X3 ? MODE_F32 : mode;
             const bool f64 = mode == MODE_F64, ddf = !f64 && std::strcmp(types, "ddf") == 0;
             const bool dfd = !f64 && std::strcmp(types, "dfd") == 0;
-            if (c == GEMM_CFG_128x128 && stacked128 && (f64 || ddf))
-                HIPCHK(launch_t_stack(st, mode, ddf, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
+            if (stacked[c] && (f64 || ddf))
+                HIPCHK(launch_t_stack(st, c, mode, ddf, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
                                       d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
-            else if (c == GEMM_CFG_128x128 && units128 && (f64 || dfd))
-                HIPCHK(launch_f_unit(st, mode, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
+            else if (units[c] && (f64 || dfd))
+                HIPCHK(launch_f_unit(st, c, mode, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),
                                      d_segs.as<GemmSeg>(), tiles[c], b, tb, tt));
             else
                 HIPCHK(launch_gemm(st, c, gmode, types, d_tasks[c].as<GemmTask>(), uint32_t(cfg[c].size()),

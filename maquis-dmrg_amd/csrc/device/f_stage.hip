@@ -7,14 +7,15 @@
 // Hermitian partner's blocks transposed and scaled (A(m, k) = alpha * scale_j * L[a_off +
 // maps[amap + j] + k % bbh + m lda], j = k / bbh), and B read through the task's column units (the
 // S_D slab of the pair of each unit, column maps[U.map + k / bbh] of it, rows k % bbh of the unit's
-// column).  These are the 128 x 128 tasks of the generic grouped GEMM whose segments all use column
-// units (GemmTask / GemmSeg / GemmUnit, no B block map, no beta); this kernel is their lean form:
+// column).  These are the 128 x 128 and 64 x 64 tasks of the generic grouped GEMM whose segments
+// all use column units (GemmTask / GemmSeg / GemmUnit, no B block map, no beta); this kernel is
+// their lean form:
 //
-//   * thread t stages row t % 128 of A and column t % 128 of B for 8 consecutive keys of the
-//     16-key step (keys 8 (t / 128) ..): within a wave the keys, and so the block j = k / bbh and
-//     the offset in it, are uniform, so block lookups are scalar loads and the 8 keys of one block
-//     are one contiguous run (two 16-byte loads of FP32 S_D, four of FP64; a transposed A block's
-//     run likewise);
+//   * with BM x BM tiles (BM = 128 or 64) thread t stages row t % BM of A and column t % BM of B
+//     for KPT = BM / 16 consecutive keys of the 16-key step (keys KPT (t / BM) ..): within a wave
+//     the keys, and so the block j = k / bbh and the offset in it, are uniform, so block lookups
+//     are scalar loads and the KPT keys of one block are one contiguous run (16-byte loads of FP32
+//     S_D or FP64; a transposed A block's run likewise);
 //   * operands are staged k-contiguous ([m][k], [n][k]) and read with one 16-byte LDS read per
 //     4 keys, k-step t using key 4 lk + t (the t_stage.hip scheme);
 //   * a column unit is resolved once per segment; a B row (maps[U.map + j]) is reloaded only when
@@ -41,7 +42,7 @@ typedef float f4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
-constexpr int FM = 128, FN = 128, FK = 16, FT = GEMM_THREADS;
+constexpr int FK = 16, FT = GEMM_THREADS;
 
 template <int MODE> struct FsTraits;
 template <> struct FsTraits<MODE_F64> { typedef double T; typedef d4 Acc; static constexpr int P = FK + 2; };
@@ -49,14 +50,14 @@ template <> struct FsTraits<MODE_F32> { typedef float T; typedef f4 Acc; static 
 
 template <> struct FsTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; static constexpr int P = FK + 8; };
 
-template <int MODE> struct FsSmem {
+template <int MODE, int BM> struct FsSmem {
     typedef typename FsTraits<MODE>::T T;
-    T a[FM][FsTraits<MODE>::P];
-    T b[FN][FsTraits<MODE>::P];
+    T a[BM][FsTraits<MODE>::P];
+    T b[BM][FsTraits<MODE>::P];
 };
-template <> struct FsSmem<MODE_BF16X3> {
+template <int BM> struct FsSmem<MODE_BF16X3, BM> {
     static constexpr int P = FsTraits<MODE_BF16X3>::P;
-    uint16_t ahi[FM][P], alo[FM][P], bhi[FN][P], blo[FN][P];
+    uint16_t ahi[BM][P], alo[BM][P], bhi[BM][P], blo[BM][P];
 };
 
 // hi = bf16(x), lo = bf16(x - hi) of two elements, each pair packed lower element first
@@ -77,30 +78,33 @@ __device__ inline uint32_t f_task_of(const GemmTask* tasks, uint32_t n_tasks, co
     return lo;
 }
 
-// 8 consecutive elements from p (8-byte aligned doubles or 4-byte aligned floats)
-__device__ inline void load8(const double* p, double* v) {
+// N consecutive elements from p (8-byte aligned doubles or 4-byte aligned floats; N = 4 or 8)
+template <int N> __device__ inline void loadn(const double* p, double* v) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < N / 2; ++q) {
         const d2_a8 x = reinterpret_cast<const d2_a8*>(p)[q];
         v[2 * q] = x[0];
         v[2 * q + 1] = x[1];
     }
 }
-__device__ inline void load8(const float* p, float* v) {
+template <int N> __device__ inline void loadn(const float* p, float* v) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < N / 4; ++q) {
         const f4_a4 x = reinterpret_cast<const f4_a4*>(p)[q];
         v[4 * q] = x[0], v[4 * q + 1] = x[1], v[4 * q + 2] = x[2], v[4 * q + 3] = x[3];
     }
 }
 
-template <int MODE, typename TB>
-__global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(2)))
+// BM x BM tiles, 2 x 2 waves of WB x WB 16 x 16 blocks
+template <int MODE, typename TB, int BM>
+__global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(BM == 64 ? 4 : 2)))
 void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const GemmSeg* __restrict__ segs,
                    Bases bases, GemmTables tables, const uint32_t* __restrict__ tile_task) {
     typedef typename FsTraits<MODE>::T T;
     typedef typename FsTraits<MODE>::Acc Acc;
-    __shared__ __align__(16) FsSmem<MODE> sm[2];
+    constexpr int FM = BM, FN = BM, KPT = BM / 16, WB = BM / 32;
+    static_assert(BM == 128 || BM == 64, "tile");
+    __shared__ __align__(16) FsSmem<MODE, BM> sm[2];
     const uint32_t* __restrict__ maps = tables.maps;
     const GemmUnit* __restrict__ units = tables.units;
     const double* __restrict__ scales = tables.scales;
@@ -112,18 +116,18 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
     const uint32_t m0 = (local / tk.tiles_n) * FM, n0 = (local % tk.tiles_n) * FN;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int wm = (wave & 1) * (BM / 2), wn = (wave >> 1) * (BM / 2);
     const int l16 = lane & 15, lk = lane >> 4;
-    const int r_i = tid & 127, r_k = (tid >> 7) * 8;   // staged row (of A) / column (of B) and first key
+    const int r_i = tid % BM, r_k = (tid / BM) * KPT;   // staged row (of A) / column (of B) and first key
     const uint32_t am = min(m0 + uint32_t(r_i), M - 1);
     const uint32_t bn = min(n0 + uint32_t(r_i), N - 1);
     const uint32_t unit = maps[tk.colunit + bn];
 
-    Acc acc[4][4];
+    Acc acc[WB][WB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+        for (int j = 0; j < WB; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
     // per-segment state
     GemmSeg sg;
@@ -154,29 +158,29 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
         return false;
     };
 
-    double va[8];
-    TB vb[8];
+    double va[KPT];
+    TB vb[KPT];
     bool part = false;
     auto gload = [&]() {
         const uint32_t K = sg.K, bbh = sg.bbh;
         const uint32_t kb = k0 + uint32_t(r_k);   // first key of this thread (uniform per wave)
-        part = kb + 8 > K;
+        part = kb + KPT > K;
         const uint32_t j0 = min(kb, K - 1) / bbh, kin0 = min(kb, K - 1) - j0 * bbh;
-        const bool run = !part && kin0 + 8 <= bbh;   // the 8 keys lie in one block (uniform)
+        const bool run = !part && kin0 + KPT <= bbh;   // the KPT keys lie in one block (uniform)
         // ---- A
         if (sg.amap == MAP_NONE) {
             const double* p = A + am;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) va[e] = sg.alpha * p[uint64_t(min(kb + e, K - 1)) * sg.lda];
+            for (int e = 0; e < KPT; ++e) va[e] = sg.alpha * p[uint64_t(min(kb + e, K - 1)) * sg.lda];
         } else if (run) {
             const double sc = sg.alpha * scales[sg.ascale + j0];
             const double* p = A + maps[sg.amap + j0] + kin0 + uint64_t(am) * sg.lda;
-            load8(p, va);
+            loadn<KPT>(p, va);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) va[e] *= sc;
+            for (int e = 0; e < KPT; ++e) va[e] *= sc;
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
+            for (int e = 0; e < KPT; ++e) {
                 const uint32_t k = min(kb + e, K - 1), j = k / bbh;
                 va[e] = sg.alpha * scales[sg.ascale + j] * A[maps[sg.amap + j] + (k - j * bbh) + uint64_t(am) * sg.lda];
             }
@@ -187,73 +191,80 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
                 blk_c = j0;
                 brow = Bcol + uint64_t(maps[u_map + j0]) * u_ld;
             }
-            load8(brow + kin0, vb);
+            loadn<KPT>(brow + kin0, vb);
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
+            for (int e = 0; e < KPT; ++e) {
                 const uint32_t k = min(kb + e, K - 1), j = k / bbh;
                 vb[e] = Bcol[uint64_t(maps[u_map + j]) * u_ld + (k - j * bbh)];
             }
         }
     };
     auto lstore = [&](int buf) {
-        FsSmem<MODE>& s = sm[buf];
+        FsSmem<MODE, BM>& s = sm[buf];
         const uint32_t kb = k0 + uint32_t(r_k);
-        T xa[8], xb[8];
+        T xa[KPT], xb[KPT];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
+        for (int e = 0; e < KPT; ++e) {
             xa[e] = T(va[e]);
             if (part && kb + e >= sg.K) xa[e] = T(0);
             xb[e] = u_ok ? T(vb[e]) : T(0);
         }
         if constexpr (MODE == MODE_BF16X3) {
-            uint32_t ah[4], al[4], bh[4], bl[4];
+            uint32_t ah[KPT / 2], al[KPT / 2], bh[KPT / 2], bl[KPT / 2];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < KPT / 2; ++e) {
                 fsplit2(xa[2 * e], xa[2 * e + 1], ah[e], al[e]);
                 fsplit2(xb[2 * e], xb[2 * e + 1], bh[e], bl[e]);
             }
-            *reinterpret_cast<uint4*>(&s.ahi[r_i][r_k]) = uint4{ah[0], ah[1], ah[2], ah[3]};
-            *reinterpret_cast<uint4*>(&s.alo[r_i][r_k]) = uint4{al[0], al[1], al[2], al[3]};
-            *reinterpret_cast<uint4*>(&s.bhi[r_i][r_k]) = uint4{bh[0], bh[1], bh[2], bh[3]};
-            *reinterpret_cast<uint4*>(&s.blo[r_i][r_k]) = uint4{bl[0], bl[1], bl[2], bl[3]};
+            if constexpr (KPT == 8) {
+                *reinterpret_cast<uint4*>(&s.ahi[r_i][r_k]) = uint4{ah[0], ah[1], ah[2], ah[3]};
+                *reinterpret_cast<uint4*>(&s.alo[r_i][r_k]) = uint4{al[0], al[1], al[2], al[3]};
+                *reinterpret_cast<uint4*>(&s.bhi[r_i][r_k]) = uint4{bh[0], bh[1], bh[2], bh[3]};
+                *reinterpret_cast<uint4*>(&s.blo[r_i][r_k]) = uint4{bl[0], bl[1], bl[2], bl[3]};
+            } else {
+                *reinterpret_cast<uint2*>(&s.ahi[r_i][r_k]) = uint2{ah[0], ah[1]};
+                *reinterpret_cast<uint2*>(&s.alo[r_i][r_k]) = uint2{al[0], al[1]};
+                *reinterpret_cast<uint2*>(&s.bhi[r_i][r_k]) = uint2{bh[0], bh[1]};
+                *reinterpret_cast<uint2*>(&s.blo[r_i][r_k]) = uint2{bl[0], bl[1]};
+            }
         } else if constexpr (MODE == MODE_F32) {
 #pragma unroll
-            for (int v = 0; v < 2; ++v) {
+            for (int v = 0; v < KPT / 4; ++v) {
                 *reinterpret_cast<float4*>(&s.a[r_i][r_k + 4 * v]) = float4{xa[4 * v], xa[4 * v + 1], xa[4 * v + 2], xa[4 * v + 3]};
                 *reinterpret_cast<float4*>(&s.b[r_i][r_k + 4 * v]) = float4{xb[4 * v], xb[4 * v + 1], xb[4 * v + 2], xb[4 * v + 3]};
             }
         } else {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < KPT / 2; ++v) {
                 *reinterpret_cast<double2*>(&s.a[r_i][r_k + 2 * v]) = double2{xa[2 * v], xa[2 * v + 1]};
                 *reinterpret_cast<double2*>(&s.b[r_i][r_k + 2 * v]) = double2{xb[2 * v], xb[2 * v + 1]};
             }
         }
     };
     auto compute = [&](int buf) {
-        const FsSmem<MODE>& s = sm[buf];
+        const FsSmem<MODE, BM>& s = sm[buf];
         if constexpr (MODE == MODE_BF16X3) {
-            s4 ah[4], al[4], bh[4], bl[4];
+            s4 ah[WB], al[WB], bh[WB], bl[WB];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < WB; ++i) {
                 ah[i] = *reinterpret_cast<const s4*>(&s.ahi[wm + 16 * i + l16][4 * lk]);
                 al[i] = *reinterpret_cast<const s4*>(&s.alo[wm + 16 * i + l16][4 * lk]);
                 bh[i] = *reinterpret_cast<const s4*>(&s.bhi[wn + 16 * i + l16][4 * lk]);
                 bl[i] = *reinterpret_cast<const s4*>(&s.blo[wn + 16 * i + l16][4 * lk]);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < WB; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < WB; ++j) {
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
                 }
         } else {
-            T a[4][4], b[4][4];
+            T a[WB][4], b[WB][4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < WB; ++i) {
                 const T* pa = &s.a[wm + 16 * i + l16][4 * lk];
                 const T* pb = &s.b[wn + 16 * i + l16][4 * lk];
                 if constexpr (MODE == MODE_F32) {
@@ -274,9 +285,9 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < WB; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
+                    for (int j = 0; j < WB; ++j) {
                         if constexpr (MODE == MODE_F64)
                             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][t], b[j][t], acc[i][j], 0, 0, 0);
                         else
@@ -307,12 +318,12 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
     // ---- epilogue: C (FP64) rows < M, columns < N; plain column-major block at c_off + m_off
     double* __restrict__ C = bases.p[tk.c_base] + tk.c_off + tk.m_off;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < WB; ++j) {
         const uint32_t gn = n0 + wn + 16 * j + l16;
         if (gn >= N) continue;
         double* __restrict__ Cn = C + uint64_t(gn) * tk.ldc;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < WB; ++i) {
             if constexpr (MODE != MODE_F64) {
                 const uint32_t gm0 = m0 + wm + 16 * i + 4 * lk;
                 if (gm0 + 3 < M) {
@@ -331,21 +342,31 @@ void f_unit_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const G
     }
 }
 
-}  // namespace
-
-hipError_t launch_f_unit(hipStream_t st, int mode, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
+template <int BM>
+hipError_t launch_f_tile(hipStream_t st, int mode, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
                          uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task) {
-    if (n_tiles == 0) return hipSuccess;
     const dim3 grid(n_tiles), block(FT);
     if (mode == MODE_F64)
-        hipLaunchKernelGGL((f_unit_kernel<MODE_F64, double>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
+        hipLaunchKernelGGL((f_unit_kernel<MODE_F64, double, BM>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else if (mode == MODE_F32)
-        hipLaunchKernelGGL((f_unit_kernel<MODE_F32, float>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
+        hipLaunchKernelGGL((f_unit_kernel<MODE_F32, float, BM>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else if (mode == MODE_BF16X3)
-        hipLaunchKernelGGL((f_unit_kernel<MODE_BF16X3, float>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
+        hipLaunchKernelGGL((f_unit_kernel<MODE_BF16X3, float, BM>), grid, block, 0, st, tasks, n_tasks, segs, bases, tb, tile_task);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_f_unit(hipStream_t st, int cfg, int mode, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
+                         uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task) {
+    if (n_tiles == 0) return hipSuccess;
+    if (cfg == GEMM_CFG_128x128)
+        return launch_f_tile<128>(st, mode, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
+    if (cfg == GEMM_CFG_64x64)
+        return launch_f_tile<64>(st, mode, tasks, n_tasks, segs, n_tiles, bases, tb, tile_task);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace mq

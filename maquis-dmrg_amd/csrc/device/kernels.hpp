@@ -188,16 +188,17 @@ struct GemmTables {
 hipError_t launch_gemm(hipStream_t st, int cfg, int mode, const char* types, const GemmTask* tasks, uint32_t n_tasks,
                        const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tables,
                        const uint32_t* tile_task);
-// Stacked 128 x 128 T-stage tasks (row units, one segment, no A / B remaps, no beta) on the lean
-// kernel of t_stage.hip: mode MODE_F64 (C double) or MODE_F32 (C float if c_float, else double),
-// A and B double; same tables, tile numbering and results as launch_gemm's GEMM_CFG_128x128.
-hipError_t launch_t_stack(hipStream_t st, int mode, bool c_float, const GemmTask* tasks, uint32_t n_tasks,
+// Stacked T-stage tasks (row units, one segment, no A / B remaps, no beta) of tile configuration
+// cfg (GEMM_CFG_128x128 or GEMM_CFG_64x64) on the lean kernel of t_stage.hip: mode MODE_F64
+// (C double), MODE_F32 (C float if c_float, else double) or MODE_BF16X3 (C float), A and B double;
+// same tables, tiles per task and results as launch_gemm with that cfg.
+hipError_t launch_t_stack(hipStream_t st, int cfg, int mode, bool c_float, const GemmTask* tasks, uint32_t n_tasks,
                           const GemmSeg* segs, uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task);
-// 128 x 128 final-stage tasks whose segments all read B through column units (no row units, no
-// B block map, no beta, m_off 0) on the lean kernel of f_stage.hip: MODE_F64 (all double) or
-// MODE_F32 (A double, B float, C double); same tables, tile numbering and results as launch_gemm's
-// GEMM_CFG_128x128.
-hipError_t launch_f_unit(hipStream_t st, int mode, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
+// Final-stage tasks of tile configuration cfg (GEMM_CFG_128x128 or GEMM_CFG_64x64) whose segments
+// all read B through column units (no row units, no B block map, no beta, m_off 0) on the lean
+// kernel of f_stage.hip: MODE_F64 (all double), MODE_F32 or MODE_BF16X3 (A double, B float,
+// C double); same tables, tiles per task and results as launch_gemm with that cfg.
+hipError_t launch_f_unit(hipStream_t st, int cfg, int mode, const GemmTask* tasks, uint32_t n_tasks, const GemmSeg* segs,
                          uint32_t n_tiles, Bases bases, GemmTables tb, const uint32_t* tile_task);
 constexpr int GATHER_THREADS = 256;
 constexpr int GATHER_CHUNK = 1024;   // elements per workgroup
