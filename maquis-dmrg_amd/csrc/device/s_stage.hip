@@ -12,8 +12,9 @@
 //   MODE_F64     v_mfma_f64_16x16x4f64 on FP64 T_D / W tiles (the plain path)
 //   MODE_F32     v_mfma_f32_16x16x4f32 on FP32 T_D / W tiles
 //   MODE_BF16X3  FP32 T_D split on the fly into bf16 hi + lo while it is staged, W tiles split once
-//                per plan; three v_mfma_f32_16x16x16bf16 products per block (hi*lo + lo*hi +
-//                hi*hi) with FP32 accumulation, the split-product scheme of the library path
+//                per plan; three v_mfma_f32_16x16x32_bf16 products per block and 32-key step
+//                (hi*lo + lo*hi + hi*hi) with FP32 accumulation, the split-product scheme of the
+//                library path (gfx950's K = 32 bf16 MFMA: twice the rate per key of the K = 16 form)
 //
 // The reference's S stage is a gather over the nonzero MPO couplings alpha
 // (solver/numeric/gpu_dgemm_ddot.cu:282-359, solver/tasks.cpp:466-504); here the couplings of one
@@ -23,7 +24,10 @@
 // MFMA lane layouts (lane l, lk = l / 16):
 //   f64 16x16x4        A row l%16, k lk;  B col l%16, k lk;  D[r] row 4r + lk, col l%16
 //   f32 16x16x4        A row l%16, k lk;  B col l%16, k lk;  D[r] row 4lk + r, col l%16
-//   bf16 16x16x16      A row l%16, k 4lk..4lk+3;  B col l%16, k 4lk..4lk+3;  D[r] row 4lk + r
+//   bf16 16x16x32      A row l%16, 8 k per lane;  B col l%16, the same 8 k;  D[r] row 4lk + r
+// (the bf16 kernel gives lane group lk the keys 8 lk .. 8 lk + 7 of its row and of its column; the
+// sum over k does not depend on which of the 32 k positions each key takes, as long as A and B
+// agree, and they use one layout)
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -36,8 +40,8 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-typedef short s4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int BM = S_BSR_TILE_M, BN = S_BSR_TILE_N, BK = S_BSR_TILE_K, NT = 256;
 static_assert(BM == 128 && BN == 128 && BK == 16, "tile shape");
@@ -57,7 +61,7 @@ template <> struct BsrTraits<MODE_F32> { typedef float T; typedef f4 Acc; };
 template <> struct BsrTraits<MODE_BF16X3> { typedef float T; typedef f4 Acc; };
 
 // LDS of one stage buffer.  FP64 / FP32: A as [k][m] (T elements), B as [n][k].  BF16X3: A hi and
-// lo as [m][k] bf16 (the MFMA reads 4 consecutive k of one row), B hi and lo as [n][k] bf16.
+// lo as [m][k] bf16 (the MFMA reads 8 consecutive k of one row), B hi and lo as [n][k] bf16.
 template <int MODE> struct BsrSmem;
 template <int MODE> struct BsrSmemPlain {
     typedef typename BsrTraits<MODE>::T T;
@@ -194,30 +198,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void s_
     auto compute = [&](int buf) {
         const BsrSmem<MODE>& s = sm[buf];
         if constexpr (BF) {
+            static_assert(KS == 32, "one K = 32 MFMA step");
+            bf8 ah[4], al[4], bh[4], bl[4];   // 8 consecutive keys of one row / column: one 16-byte read each
 #pragma unroll
-            for (int ks = 0; ks < KS / 16; ++ks) {
-                s4 ah[4], al[4], bh[4], bl[4];   // 4 consecutive keys of one row / column: one 8-byte read each
+            for (int i = 0; i < 4; ++i) {
+                const int r = wm + 16 * i + l16;
+                ah[i] = *reinterpret_cast<const bf8*>(&s.ahi[r][8 * lk]);
+                al[i] = *reinterpret_cast<const bf8*>(&s.alo[r][8 * lk]);
+            }
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int r = wm + 16 * i + l16;
-                    ah[i] = *reinterpret_cast<const s4*>(&s.ahi[r][16 * ks + 4 * lk]);
-                    al[i] = *reinterpret_cast<const s4*>(&s.alo[r][16 * ks + 4 * lk]);
-                }
+            for (int j = 0; j < 4; ++j) {
+                const int c = wn + 16 * j + l16;
+                bh[j] = *reinterpret_cast<const bf8*>(&s.bhi[c][8 * lk]);
+                bl[j] = *reinterpret_cast<const bf8*>(&s.blo[c][8 * lk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int c = wn + 16 * j + l16;
-                    bh[j] = *reinterpret_cast<const s4*>(&s.bhi[c][16 * ks + 4 * lk]);
-                    bl[j] = *reinterpret_cast<const s4*>(&s.blo[c][16 * ks + 4 * lk]);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
                 }
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                    }
-            }
         } else {
 #pragma unroll
             for (int ks = 0; ks < BK / 4; ++ks) {
