@@ -833,6 +833,11 @@ class TwoSiteDMRG:
 
     # ------------------------------------------------------------------ sweeps
     def init_boundaries(self):
+        """Optimizer set-up before the first sweep: the two-site MPO cache (make_ts_cache_mpo in
+        the ts_optimize constructor, ts_optimize.hpp:59), right normalization and the right
+        boundaries."""
+        for i in range(self.L - 1):
+            self.twosite_mpo(i)
         self.right_normalize()
         L = self.L
         self.Lb = [None] * (L + 1)

@@ -513,40 +513,56 @@ def op_kron_su2(o1, o2, k, phys, phys12, pb):
 
 def twosite_su2(W1, W2, phys):
     """make_twosite_mpo for SU2 (ts_ops.hpp:38-168): W12(b1,b3) = sum_{b2, o1, o2, k} 6j-factor
-    (x) op_kron_k(o1, o2), one operator per (o1, o2, k) with its coefficient matrix."""
+    (x) op_kron_k(o1, o2), one operator per (o1, o2, k) with its coefficient matrix.
+
+    The 6j factor of a term depends on the spins j (b1), jm (b2) and jp (b3) of the three bond
+    states; per middle spin jm the product over b2 is one sparse product C1 P_jm C2 (P_jm: the
+    projector on the b2 of spin jm), whose entry (b1, b3) is then scaled by the factor of
+    (spin of b1, jm, spin of b3) and summed over jm in ascending order."""
     phys12, pb = product_index(phys, phys)
     ls = np.asarray(W1.left_spin)
     ms = np.asarray(W1.right_spin)
     rs = np.asarray(W2.right_spin)
     spins_l, spins_m, spins_r = sorted(set(ls.tolist())), sorted(set(ms.tolist())), sorted(set(rs.tolist()))
-    proj = lambda arr, v: sp.diags((arr == v).astype(np.float64))
-    Pl = {v: proj(ls, v) for v in spins_l}
-    Pm = {v: proj(ms, v) for v in spins_m}
-    Pr = {v: proj(rs, v) for v in spins_r}
-    C1s = [{(j, jm): (Pl[j] @ C @ Pm[jm]).tocsr() for j in spins_l for jm in spins_m} for C in W1.coeffs]
-    C2s = [{(jm, jp): (Pm[jm] @ C @ Pr[jp]).tocsr() for jm in spins_m for jp in spins_r} for C in W2.coeffs]
+    li = np.searchsorted(np.asarray(spins_l), ls)   # position of each b1's spin in spins_l
+    ri = np.searchsorted(np.asarray(spins_r), rs)
+    Pm = {v: sp.diags((ms == v).astype(np.float64)) for v in spins_m}
+    C1s = [{jm: (C @ Pm[jm]).tocsr() for jm in spins_m} for C in W1.coeffs]
+    C2s = [{jm: (Pm[jm] @ C).tocsr() for jm in spins_m} for C in W2.coeffs]
+    ftab = {}
+
+    def factors(k1, k2, k, jm):
+        key = (k1, k2, k, jm)
+        F = ftab.get(key)
+        if F is None:
+            F = np.zeros((len(spins_l), len(spins_r)))
+            for a, j in enumerate(spins_l):
+                for c, jp in enumerate(spins_r):
+                    f = math.sqrt((jm + 1.0) * (k + 1.0)) * sixj(j, jp, k, k2, k1, jm)
+                    F[a, c] = -f if ((j + jp + k1 + k2) // 2) % 2 else f
+            ftab[key] = F
+        return F
+
     ops, coeffs = [], []
     for o1, c1 in zip(W1.ops, C1s):
         for o2, c2 in zip(W2.ops, C2s):
             k1, k2 = o1.spin, o2.spin
+            prods = []
+            for jm in spins_m:
+                A, Bm = c1[jm], c2[jm]
+                if A.nnz == 0 or Bm.nnz == 0:
+                    continue
+                X = (A @ Bm).tocoo()
+                if X.nnz:
+                    prods.append((jm, X))
+            if not prods:
+                continue
             for k in range(abs(k1 - k2), k1 + k2 + 1, 2):
                 M = None
-                for (j, jm), A in c1.items():
-                    if A.nnz == 0:
-                        continue
-                    for jp in spins_r:
-                        Bm = c2[(jm, jp)]
-                        if Bm.nnz == 0:
-                            continue
-                        f = math.sqrt((jm + 1.0) * (k + 1.0)) * sixj(j, jp, k, k2, k1, jm)
-                        if ((j + jp + k1 + k2) // 2) % 2:
-                            f = -f
-                        if f == 0.0:
-                            continue
-                        P = (A @ Bm) * f
-                        M = P if M is None else M + P
-                if M is None:
-                    continue
+                for jm, X in prods:
+                    f = factors(k1, k2, k, jm)[li[X.row], ri[X.col]]
+                    P = sp.csr_matrix((X.data * f, (X.row, X.col)), shape=X.shape)
+                    M = P if M is None else M + P
                 M = M.tocsr()
                 M.eliminate_zeros()
                 if M.nnz == 0:
