@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -221,14 +222,24 @@ struct mq_ctx {
     void pool_clear() {
         for (auto& b : pool) b.release();
     }
-    // a workspace of at least n bytes whose first n bytes are cleared to zero: the slot's buffer
-    // if it is large enough, else a new allocation (with headroom; exactly n bytes after the
-    // pools are released when that fails)
-    void take(PoolKind kind, DevBuf& dst, size_t n) {
+    // a workspace of at least n bytes: the slot's buffer if it is large enough, else a new
+    // allocation (with headroom; exactly n bytes after the pools are released when that fails).
+    // The first n bytes are cleared to zero, except that a reused buffer is taken as it is when
+    // zero_reused is false: it holds the previous plans' finite values, and callers that pass
+    // false read no workspace element before writing it in the same application other than
+    // through masks (the grouped sigma plans: T_D slabs no T task writes are cleared per
+    // application by zero chunks; tile rows and keys past the data feed only outputs that are not
+    // stored or are zeroed).  MQ_POISON_WS=1 fills such reused buffers with NaN instead (tests).
+    void take(PoolKind kind, DevBuf& dst, size_t n, bool zero_reused = true) {
         n = (n + 7) / 8 * 8;
         dst.release();
         if (pool[kind].p && pool[kind].bytes >= n) {
             dst.swap(pool[kind]);
+            if (!zero_reused) {
+                if (std::getenv("MQ_POISON_WS"))
+                    HIPCHK(launch_fill(stream, static_cast<double*>(dst.p), n / 8, std::numeric_limits<double>::quiet_NaN()));
+                return;
+            }
         } else {
             pool[kind].release();
             const size_t want = (n + n / 4 + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
@@ -3022,8 +3033,8 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         P.sizes[BASE_RIGHT] = right->total;
         P.sizes[BASE_BRA] = P.part_elems;
         P.validate();
-        if (P.part_elems) ctx->take(mq_ctx::POOL_PART, p->part, P.part_elems * sizeof(double));
-        ctx->take(mq_ctx::POOL_WS, p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double));
+        if (P.part_elems) ctx->take(mq_ctx::POOL_PART, p->part, P.part_elems * sizeof(double), adjoint);
+        ctx->take(mq_ctx::POOL_WS, p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double), adjoint);
         // the dense W_D serve the library S stage and the adjoint; the block-sparse S stage reads tiles
         if (wd_elems && !P.bsr_on)
             HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ctx->wd_host, wd_elems * sizeof(double),

@@ -457,6 +457,42 @@ def test_block_sparse_s_stage_matches_dense_and_oracle(eng, model, L, m, monkeyp
     assert eo <= 1e-12
 
 
+@pytest.mark.parametrize("model,L,m", [("qc_su2", 16, 256), ("qc", 12, 128)])
+def test_reused_workspace_needs_no_zero_fill(eng, model, L, m, monkeypatch):
+    """A grouped sigma plan takes a reused workspace without clearing it.  With MQ_POISON_WS=1 a
+    reused workspace (and split-K partial buffer) is filled with NaN instead: a plan built on it
+    must give sigma bitwise equal to the same plan built before, in every precision, so no
+    kernel reads a workspace element it has not written in the same application."""
+    from mqdmrg.workloads import SyntheticSite
+    print("case", model, L, m, flush=True)
+    ws = SyntheticSite(model, L, L // 2 - 1, m, seed=4)
+    Lb = E.DeviceBoundary.upload(eng, ws.left_index, ws.random_cohorts(ws.left_index))
+    Rb = E.DeviceBoundary.upload(eng, ws.right_index, ws.random_cohorts(ws.right_index))
+    psi = dev(ws.random_psi())
+    precs = (E.PREC_F64, E.PREC_F32, E.PREC_BF16X3)
+
+    def run(plan):
+        outs = {}
+        for prec in precs:
+            out = torch.full_like(psi, np.nan)
+            plan.apply(psi, out, precision=prec)
+            eng.sync()
+            outs[prec] = out
+        return outs
+
+    first = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    ref = run(first)
+    first.close()   # its workspace returns to the pool and is taken by the next plan
+    monkeypatch.setenv("MQ_POISON_WS", "1")
+    poisoned = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
+    monkeypatch.delenv("MQ_POISON_WS")
+    got = run(poisoned)
+    poisoned.close()
+    for prec in precs:
+        assert torch.isfinite(got[prec]).all(), prec
+        assert torch.equal(got[prec], ref[prec]), prec
+
+
 @pytest.mark.parametrize("precision", [E.PREC_F32, E.PREC_BF16X3])
 @pytest.mark.parametrize("model,L,site,M,seed", [("hubbard", 8, 3, 10, 5), ("qc", 8, 3, 10, 7),
                                                   ("heisenberg", 10, 0, 8, 2)])
