@@ -413,8 +413,9 @@ def main():
             {"f64": "FP64", "f32": "FP32", "bf16x3": "bf16 split-product, FP32 accumulate"}[report])
         peak_key = report
     else:
-        # T and final stages: the hand-written MFMA GEMM, FP32 in the reduced precisions
-        kernel = "mq::gemm_kernel (%s stage)" % ("T" if dom == "gemm_T" else "final")
+        # T and final stages: the lean MFMA kernels (and the generic grouped GEMM for other tile
+        # shapes), FP32 accumulation in the reduced precisions
+        kernel = ("mq::t_stack_kernel (T stage)" if dom == "gemm_T" else "mq::f_unit_kernel (final stage)")
         peak_key = "f64" if report == "f64" else "f32"
     roof = {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[peak_key],
             "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[peak_key], "traffic": None, "stage_ms": st,

@@ -225,11 +225,11 @@ struct mq_ctx {
     // a workspace of at least n bytes: the slot's buffer if it is large enough, else a new
     // allocation (with headroom; exactly n bytes after the pools are released when that fails).
     // The first n bytes are cleared to zero, except that a reused buffer is taken as it is when
-    // zero_reused is false: it holds the previous plans' finite values, and callers that pass
-    // false read no workspace element before writing it in the same application other than
-    // through masks (the grouped sigma plans: T_D slabs no T task writes are cleared per
-    // application by zero chunks; tile rows and keys past the data feed only outputs that are not
-    // stored or are zeroed).  MQ_POISON_WS=1 fills such reused buffers with NaN instead (tests).
+    // zero_reused is false: callers that pass false read no workspace element before writing it
+    // in the same application other than through masks (grouped sigma plans and steps with the
+    // block-sparse S stage: T_D slabs no T task writes are cleared per application by zero
+    // chunks; tile rows and keys past the data feed only outputs that are not stored or are
+    // zeroed).  MQ_POISON_WS=1 fills such reused buffers with NaN instead (tests).
     void take(PoolKind kind, DevBuf& dst, size_t n, bool zero_reused = true) {
         n = (n + 7) / 8 * 8;
         dst.release();
@@ -556,7 +556,11 @@ struct StagePlan {
     struct Bsr {
         std::vector<BsrTask> tasks;
         std::vector<BsrRun> runs;
-        std::vector<double> w;          // compact W tiles (host copy until upload)
+        // the nonzeros of the compact W tiles: positions in the tile array (tiles x 128 columns x 16
+        // keys) and values; the device builds the tiles from them (host copy until upload)
+        std::vector<uint64_t> ent_pos;
+        std::vector<double> ent_val;
+        bool uploaded = false;
         uint64_t tiles = 0;
         uint64_t flops = 0;             // MFMA flops of the visited tiles (unpadded)
         bool narrow = true;             // every task's T_D step window fits 32-bit byte offsets
@@ -935,20 +939,31 @@ struct StagePlan {
         gT.upload_all(t_segs, st);
         gF.upload_all(f_segs, st);
         gU.upload_all(u_segs, st);
-        if (bsr_on && !bsr.w.empty()) {   // once: the host copy of the tiles is released after it
+        if (bsr_on && !bsr.uploaded) {   // once: the host copy of the nonzeros is released after it
             upload(bsr.d_tasks, bsr.tasks, st);
             upload(bsr.d_runs, bsr.runs, st);
-            // the W tiles in the three arithmetics: FP64 uploaded; FP32 and the bf16 split of the FP32
-            // value converted on the device
-            const uint64_t nw = bsr.w.size();
-            upload(bsr.d_w64, bsr.w, st);
+            // the W tiles in the three arithmetics: FP64 zero-filled and the nonzeros scattered into
+            // it; FP32 and the bf16 split of the FP32 value converted on the device
+            const uint64_t nw = bsr.tiles * uint64_t(S_BSR_TILE_N) * S_BSR_TILE_K;
+            bsr.d_w64.alloc(std::max<uint64_t>(nw, 1) * sizeof(double));
+            HIPCHK(launch_fill(st, bsr.d_w64.as<double>(), nw, 0.0));
+            {
+                DevBuf d_pos, d_val;
+                upload(d_pos, bsr.ent_pos, st);
+                upload(d_val, bsr.ent_val, st);
+                HIPCHK(launch_scatter(st, bsr.ent_pos.size(), d_pos.as<uint64_t>(), d_val.as<double>(),
+                                      bsr.d_w64.as<double>()));
+                HIPCHK(hipStreamSynchronize(st));   // before the entry buffers are freed
+            }
             bsr.d_w32.alloc(std::max<uint64_t>(nw, 1) * sizeof(float));
             bsr.d_whi.alloc(std::max<uint64_t>(nw, 1) * sizeof(uint16_t));
             bsr.d_wlo.alloc(std::max<uint64_t>(nw, 1) * sizeof(uint16_t));
             HIPCHK(launch_split_tiles(st, bsr.d_w64.as<double>(), nw, bsr.d_w32.as<float>(), bsr.d_whi.as<uint16_t>(),
                                       bsr.d_wlo.as<uint16_t>()));
             HIPCHK(hipStreamSynchronize(st));   // the host copy is released below
-            std::vector<double>().swap(bsr.w);
+            std::vector<uint64_t>().swap(bsr.ent_pos);
+            std::vector<double>().swap(bsr.ent_val);
+            bsr.uploaded = true;
         }
         upload(d_regions, regions, st);
         upload(d_small, small, st);
@@ -1930,11 +1945,20 @@ struct GroupedStep {
     uint32_t lo = 0, hi = 0xffffffffu;
 };
 
+// One nonzero of a group's W_D: column n (N key position), row k (K key position), value.
+struct WEntry {
+    uint32_t n, k;
+    double v;
+};
+
 // Pattern-class order of one group's keys (see lower_sigma_grouped): rows k and columns n of W_D
-// (Wg[n * K + k]) are classed by a hash of their nonzero index lists; classes are ordered by size
-// (largest first), then by their first key; the all-zero class goes last.  Permutes W_D, the group's
-// K / N lists, kpos / npos and its pairs' kk / nn position lists consistently.
-void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* Wg) {
+// are classed by a hash of their nonzero index lists; classes are ordered by size (largest first),
+// then by their first key; the all-zero class goes last.  W holds the group's nonzeros in an order
+// in which the columns n of every row k, and the rows k of every column n, ascend (as in a
+// column-major scan of the dense matrix, which fixes the hashes); their positions, the group's
+// K / N lists, kpos / npos and its pairs' kk / nn position lists are renumbered consistently (W is
+// left in the old order of its elements).
+void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, std::vector<WEntry>& W) {
     const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
     if (Kg < 2 && Ng < 2) return;
     const uint64_t seed = 0x9E3779B97F4A7C15ull;
@@ -1944,15 +1968,11 @@ void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* W
     };
     std::vector<uint64_t> hk(Kg, seed), hn(Ng, seed);
     std::vector<uint32_t> ck(Kg, 0), cn(Ng, 0);   // nonzero counts (the all-zero class)
-    for (uint32_t n = 0; n < Ng; ++n) {
-        const double* col = Wg + uint64_t(n) * Kg;
-        for (uint32_t k = 0; k < Kg; ++k)
-            if (col[k] != 0.0) {
-                hk[k] = mix(hk[k], n + 1);
-                hn[n] = mix(hn[n], k + 1);
-                ++ck[k];
-                ++cn[n];
-            }
+    for (const WEntry& e : W) {
+        hk[e.k] = mix(hk[e.k], e.n + 1);
+        hn[e.n] = mix(hn[e.n], e.k + 1);
+        ++ck[e.k];
+        ++cn[e.n];
     }
     auto order = [](const std::vector<uint64_t>& h, const std::vector<uint32_t>& cnt) {
         const uint32_t n = uint32_t(h.size());
@@ -1978,11 +1998,9 @@ void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* W
     std::vector<uint32_t> ik(Kg), in(Ng);
     for (uint32_t i = 0; i < Kg; ++i) ik[pk[i]] = i;
     for (uint32_t i = 0; i < Ng; ++i) in[pn[i]] = i;
-    std::vector<double> old(Wg, Wg + uint64_t(Kg) * Ng);
-    for (uint32_t n = 0; n < Ng; ++n) {
-        const double* src = old.data() + uint64_t(pn[n]) * Kg;
-        double* dst = Wg + uint64_t(n) * Kg;
-        for (uint32_t k = 0; k < Kg; ++k) dst[k] = src[pk[k]];
+    for (WEntry& e : W) {
+        e.n = in[e.n];
+        e.k = ik[e.k];
     }
     std::vector<uint32_t> K2(Kg), N2(Ng);
     for (uint32_t i = 0; i < Kg; ++i) K2[i] = g.K[pk[i]], g.kpos[K2[i]] = int32_t(i);
@@ -1995,7 +2013,7 @@ void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, double* W
     }
 }
 
-// Block-sparse S stage (W_D of each group, K x N column-major, after reorder_group_keys): the
+// Block-sparse S stage (the nonzeros of each group's W_D, after reorder_group_keys): the
 // columns are cut into panels of 128, the keys into tiles of 16; each panel keeps the tiles holding
 // a nonzero, stored compactly ([n][k], zero padded) and grouped into runs of consecutive tiles; one
 // task per 128-row panel of T_D and column panel, in blocks of 8 x 8 panels (below).  A panel
@@ -2006,36 +2024,37 @@ struct BsrGroupScan {
     std::vector<std::vector<std::pair<uint32_t, uint32_t>>> panel_runs;   // (first key tile, tiles) per panel
     uint64_t tiles = 0, nnz = 0, visited = 0;
 };
-void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>& groups, const double* wd,
-                    uint64_t wd_off, std::vector<std::pair<uint64_t, double>>* info) {
+void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>& groups,
+                    const std::vector<std::vector<WEntry>>& wg, std::vector<std::pair<uint64_t, double>>* info) {
     constexpr uint32_t TK = S_BSR_TILE_K, TN = S_BSR_TILE_N, TM = S_BSR_TILE_M;
     const size_t ng = groups.size();
     std::vector<BsrGroupScan> sc(ng);
+    // per group and (panel, key tile): the tile's number within the group (in panel, then key
+    // order), ~0u for a tile without a nonzero
+    std::vector<std::vector<uint32_t>> at(ng);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int gi = 0; gi < int(ng); ++gi) {
         const auto& g = groups[gi];
         if (!g.M || g.K.empty() || g.N.empty()) continue;
         const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
-        const double* Wg = wd + (g.w_off - wd_off);
         const uint32_t nkt = (Kg + TK - 1) / TK, npn = (Ng + TN - 1) / TN;
         BsrGroupScan& S = sc[gi];
         S.panel_runs.resize(npn);
-        std::vector<char> nz(nkt);
+        std::vector<uint32_t>& A = at[gi];
+        A.assign(uint64_t(npn) * nkt, ~0u);
+        for (const WEntry& e : wg[gi]) A[uint64_t(e.n / TN) * nkt + e.k / TK] = 0;
+        S.nnz = wg[gi].size();
+        uint32_t local = 0;
         for (uint32_t j = 0; j < npn; ++j) {
             const uint32_t n0 = j * TN, n1 = std::min(Ng, n0 + TN);
-            std::fill(nz.begin(), nz.end(), 0);
-            for (uint32_t n = n0; n < n1; ++n) {
-                const double* col = Wg + uint64_t(n) * Kg;
-                for (uint32_t k = 0; k < Kg; ++k)
-                    if (col[k] != 0.0) nz[k / TK] = 1, ++S.nnz;
-            }
+            uint32_t* z = A.data() + uint64_t(j) * nkt;
             for (uint32_t t = 0; t < nkt;) {
-                if (!nz[t]) {
+                if (z[t] == ~0u) {
                     ++t;
                     continue;
                 }
                 uint32_t t1 = t;
-                while (t1 < nkt && nz[t1]) ++t1;
+                while (t1 < nkt && z[t1] != ~0u) z[t1++] = local++;
                 S.panel_runs[j].push_back({t, t1 - t});
                 S.tiles += t1 - t;
                 S.visited += uint64_t(std::min(Kg, t1 * TK) - t * TK) * (n1 - n0);
@@ -2043,29 +2062,27 @@ void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>
             }
         }
     }
-    // tile ranges, then the tiles themselves in parallel
+    // tile ranges, then each nonzero's place in the compact tiles ([n][k] per tile, zero padded):
+    // the device fills the tiles from these (position, value) pairs
     std::vector<uint64_t> tile0(ng + 1, B.tiles);
     for (size_t gi = 0; gi < ng; ++gi) tile0[gi + 1] = tile0[gi] + sc[gi].tiles;
     MQ_REQUIRE(tile0[ng] < (1ull << 32), "too many block-sparse tiles");
-    const uint64_t w0 = B.w.size();
-    B.w.resize(w0 + (tile0[ng] - B.tiles) * uint64_t(TN) * TK);
+    std::vector<uint64_t> e0(ng + 1, B.ent_pos.size());
+    for (size_t gi = 0; gi < ng; ++gi) e0[gi + 1] = e0[gi] + (sc[gi].tiles ? wg[gi].size() : 0);
+    B.ent_pos.resize(e0[ng]);
+    B.ent_val.resize(e0[ng]);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int gi = 0; gi < int(ng); ++gi) {
-        const auto& g = groups[gi];
-        const BsrGroupScan& S = sc[gi];
-        if (!S.tiles) continue;
-        const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
-        const double* Wg = wd + (g.w_off - wd_off);
-        double* out = B.w.data() + w0 + (tile0[gi] - B.tiles) * uint64_t(TN) * TK;
-        for (uint32_t j = 0; j < S.panel_runs.size(); ++j) {
-            const uint32_t n0 = j * TN, n1 = std::min(Ng, n0 + TN);
-            for (const auto& r : S.panel_runs[j])
-                for (uint32_t t = r.first; t < r.first + r.second; ++t, out += uint64_t(TN) * TK)
-                    for (uint32_t n = 0; n < TN; ++n)
-                        for (uint32_t k = 0; k < TK; ++k) {
-                            const uint32_t kk = t * TK + k;
-                            out[n * TK + k] = (n0 + n < n1 && kk < Kg) ? Wg[uint64_t(n0 + n) * Kg + kk] : 0.0;
-                        }
+        if (!sc[gi].tiles) continue;
+        const uint32_t nkt = (uint32_t(groups[gi].K.size()) + TK - 1) / TK;
+        const std::vector<uint32_t>& A = at[gi];
+        uint64_t q = e0[gi];
+        for (const WEntry& e : wg[gi]) {
+            const uint32_t j = e.n / TN, t = e.k / TK;
+            const uint64_t tile = tile0[gi] + A[uint64_t(j) * nkt + t];
+            B.ent_pos[q] = tile * (uint64_t(TN) * TK) + uint64_t(e.n - j * TN) * TK + (e.k - t * TK);
+            B.ent_val[q] = e.v;
+            ++q;
         }
     }
     // runs and tasks in group order
@@ -2276,7 +2293,13 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         ws += align32(uint64_t(g.K.size()) * g.N.size());
     }
     wd_elems = ws - ws_init_off;
-    double* const wd = p->ctx->wd_stage(wd_elems);
+    // W_D is assembled as its nonzeros per group (wg) for every plan but the adjoint's; the dense
+    // matrices (the pinned staging, then the workspace) serve only the adjoint and the library S
+    // stage (MQ_S_DENSE=1), while the block-sparse S stage builds its tiles from the nonzeros
+    const bool sparse_w = !adjoint;
+    const bool bsr = !adjoint && s_sparse_enabled();
+    double* const wd = bsr ? nullptr : p->ctx->wd_stage(wd_elems);
+    std::vector<std::vector<WEntry>> wg(sparse_w ? G.groups.size() : 0);
     trace_phase("lower: key order, layout");
     if (g_trace) {
         uint64_t tk = 0, tn = 0, tm = 0;
@@ -2287,7 +2310,10 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     }
     struct AdjEntry { uint32_t t; uint64_t w; double c; };
     std::vector<AdjEntry> adj;
-    // ---- dense W_D: for every MPO entry, its (k, n) in each group holding both keys.  The groups of
+    // one nonzero of W_D of group g at (column n, row k), from one MPO row
+    struct RawW { uint32_t g, n, k; double v; };
+    std::vector<std::vector<RawW>> raw(sparse_w ? W.row_dim : 0);
+    // ---- W_D: for every MPO entry, its (k, n) in each group holding both keys.  The groups of
     // a K key are listed per key (CSR over the dense key numbering)
     std::vector<uint32_t> kg_off(size_t(G.nkeys) + 1, 0);
     for (const auto& g : G.groups)
@@ -2307,8 +2333,8 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     // right_tasks.hpp): spins J (ket left), J' (ket right), I (bra left), I' (bra right) of the
     // sectors it connects.  sigma and the right step pair a ket left sector with a bra right
     // sector; the left step pairs a bra left sector with a ket right sector.
-    auto su2_factor = [&](CouplingCache& cc, const GroupedLowering::Group& g, uint32_t s_in, uint32_t s_out,
-                          int A, int K, int Ap, uint32_t e, const Op& op) {
+    auto su2_unit = [&](CouplingCache& cc, const GroupedLowering::Group& g, uint32_t s_in, uint32_t s_out,
+                        int A, int K, int Ap) -> const double* {
         const int din = sy.spin(phys_i.charges[s_in]), dout = sy.spin(phys_i.charges[s_out]);
         int J, Jp, I, Ip;
         if (lstep) {
@@ -2322,20 +2348,49 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
             Jp = J + din;
             I = Ip - dout;
         }
-        if (J < 0 || Jp < 0 || I < 0 || Ip < 0) return 0.0;
-        const double* u = cc.get(J, std::abs(J - Jp), Jp, A, K, Ap, I, std::abs(I - Ip), Ip);
+        if (J < 0 || Jp < 0 || I < 0 || Ip < 0) return nullptr;
+        return cc.get(J, std::abs(J - Jp), Jp, A, K, Ap, I, std::abs(I - Ip), Ip);
+    };
+    // the coupling of entry e of op (unit couplings u, null: none): its (row spin, col spin) case
+    auto su2_case = [](const double* u, uint32_t e, const Op& op) {
+        if (!u) return 0.0;
         const unsigned rs = op.row_spin[e], cs = op.col_spin[e];
         const int casenr = (rs == 2 && cs == 2) ? 3 : rs == 2 ? 1 : cs == 2 ? 2 : 0;   // op_iterate.hpp:56-60
         return u[casenr];
     };
-    auto mpo_row = [&](uint32_t b1, std::vector<AdjEntry>* adj_out) {
+    // raw_out: the row's W_D nonzeros.  All contributions to one entry come from one MPO cell
+    // (b1, b2) (its K and N keys carry b1 and b2), which touches at most nst x nst key pairs: they
+    // are summed per cell in a small accumulator, in the order the cell makes them (as the dense
+    // accumulation adds them), and the cell's nonzero sums are emitted in (k, n) order.
+    auto mpo_row = [&](uint32_t b1, std::vector<AdjEntry>* adj_out, std::vector<RawW>* raw_out) {
         thread_local CouplingCache ccache;
+        thread_local std::vector<double> cacc;   // per (k key, n key, group of the k key) of a cell
+        thread_local std::vector<char> cset;
+        // SU2: the unit couplings of a group depend on the operator block, not on its entries: kept
+        // per group for the block being expanded (ustamp: the block it was computed for)
+        thread_local std::vector<const double*> ucache;
+        thread_local std::vector<uint64_t> ustamp;
+        thread_local uint64_t stamp = 0;
+        if (ustamp.size() < G.groups.size()) {
+            ucache.assign(G.groups.size(), nullptr);
+            ustamp.assign(G.groups.size(), 0);
+        }
+        const uint32_t nst = G.nst;
         for (const auto& cell : W.rows[b1]) {
             uint32_t b2 = cell.other;
+            const uint32_t bk = lstep ? b1 : b2, bn = lstep ? b2 : b1;
+            uint32_t qmax = 0;
+            if (raw_out) {
+                for (uint32_t kl = 0; kl < nst; ++kl)
+                    qmax = std::max(qmax, kg_off[bk * nst + kl + 1] - kg_off[bk * nst + kl]);
+                cacc.assign(size_t(nst) * nst * qmax, 0.0);
+                cset.assign(size_t(nst) * nst * qmax, 0);
+            }
             for (uint32_t t = cell.t0; t < cell.t1; ++t) {
                 const Op& op = W.ops[W.terms[t].op];
                 double sc = W.terms[t].scale;
                 for (const OpBlock& wb : op.blocks) {
+                    ++stamp;
                     uint32_t s_in = phys_i.position(wb.in), s_out = phys_i.position(wb.out);
                     if (s_in >= phys_i.size() || s_out >= phys_i.size()) {
                         bad = 1;
@@ -2350,29 +2405,80 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
                         const uint32_t kd = G.key(lstep ? b1 : b2, s_in, op.row[e]);
                         const uint32_t nd = G.key(lstep ? b2 : b1, s_out, op.col[e]);
                         for (uint32_t q = kg_off[kd]; q < kg_off[kd + 1]; ++q) {
-                            const auto& g = G.groups[kg_ent[q].first];
+                            const uint32_t gi = kg_ent[q].first;
+                            const auto& g = G.groups[gi];
                             const int32_t ni = g.npos[nd];
                             if (ni < 0) continue;
+                            double c = op.coeff[e];
+                            if (sy.su2()) {
+                                if (ustamp[gi] != stamp) {
+                                    ucache[gi] = su2_unit(ccache, g, s_in, s_out, W.left_spin[b1], op.spin, W.right_spin[b2]);
+                                    ustamp[gi] = stamp;
+                                }
+                                c = op.coeff[e] * su2_case(ucache[gi], e, op);
+                            }
+                            if (raw_out) {
+                                const size_t slot = (size_t(kd - bk * nst) * nst + (nd - bn * nst)) * qmax + (q - kg_off[kd]);
+                                cacc[slot] += c * sc;
+                                cset[slot] = 1;
+                                continue;
+                            }
                             const uint64_t w = g.w_off - ws_init_off + uint64_t(ni) * g.K.size() + kg_ent[q].second;
-                            const double c = sy.su2() ? op.coeff[e] * su2_factor(ccache, g, s_in, s_out, W.left_spin[b1],
-                                                                                 op.spin, W.right_spin[b2], e, op)
-                                                      : op.coeff[e];
                             wd[w] += c * sc;
                             if (adj_out) adj_out->push_back({t, w, c});
                         }
                     }
                 }
             }
+            if (raw_out)
+                for (uint32_t kl = 0; kl < nst; ++kl) {
+                    const uint32_t kd = bk * nst + kl;
+                    for (uint32_t nl = 0; nl < nst; ++nl) {
+                        const uint32_t nd = bn * nst + nl;
+                        for (uint32_t q = kg_off[kd]; q < kg_off[kd + 1]; ++q) {
+                            const size_t slot = (size_t(kl) * nst + nl) * qmax + (q - kg_off[kd]);
+                            if (!cset[slot] || cacc[slot] == 0.0) continue;
+                            raw_out->push_back({kg_ent[q].first, uint32_t(G.groups[kg_ent[q].first].npos[nd]),
+                                                kg_ent[q].second, cacc[slot]});
+                        }
+                    }
+                }
         }
     };
     if (adjoint) {
-        for (uint32_t b1 = 0; b1 < W.row_dim; ++b1) mpo_row(b1, &adj);
+        for (uint32_t b1 = 0; b1 < W.row_dim; ++b1) mpo_row(b1, &adj, nullptr);
     } else {
 #pragma omp parallel for schedule(dynamic, 4)
-        for (int b1 = 0; b1 < int(W.row_dim); ++b1) mpo_row(uint32_t(b1), nullptr);
+        for (int b1 = 0; b1 < int(W.row_dim); ++b1) mpo_row(uint32_t(b1), nullptr, &raw[b1]);
     }
     MQ_REQUIRE(bad != 1, "operator charge not in physical index");
     MQ_REQUIRE(bad != 2, "operator element outside its physical sector");
+    trace_phase("lower: W_D rows");
+    if (sparse_w) {
+        // A group's entries are the rows' entries in row order.  Key positions grow with b1 and
+        // b2 (keys are b-major; a row's cells ascend in b2; a cell emits in (k, n) order), so for
+        // every k its columns n, and for every n its rows k, come in ascending order, the order
+        // reorder_group_keys hashes them in.
+        const size_t ng = G.groups.size(), nb = raw.size();
+        std::vector<uint64_t> off(nb * ng, 0);   // per (b1, g): entries, then first slot
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int b1 = 0; b1 < int(nb); ++b1)
+            for (const RawW& e : raw[b1]) ++off[size_t(b1) * ng + e.g];
+        for (size_t g = 0; g < ng; ++g) {
+            uint64_t n = 0;
+            for (size_t b1 = 0; b1 < nb; ++b1) {
+                const uint64_t c = off[b1 * ng + g];
+                off[b1 * ng + g] = n;
+                n += c;
+            }
+            wg[g].resize(n);
+        }
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int b1 = 0; b1 < int(nb); ++b1) {
+            for (const RawW& e : raw[b1]) wg[e.g][off[size_t(b1) * ng + e.g]++] = {e.n, e.k, e.v};
+            std::vector<RawW>().swap(raw[b1]);
+        }
+    }
     trace_phase("lower: W_D");
     // ---- key order for the block-sparse S stage: within each group, keys whose rows (K) or
     // columns (N) of W_D have the same nonzero pattern are made adjacent, larger classes first, so
@@ -2381,8 +2487,15 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     // kk / nn lists.  Adjoint plans keep the ascending order their term map was built in.
     if (!adjoint) {
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int gi = 0; gi < int(G.groups.size()); ++gi)
-            reorder_group_keys(G, G.groups[gi], wd + (G.groups[gi].w_off - ws_init_off));
+        for (int gi = 0; gi < int(G.groups.size()); ++gi) {
+            auto& g = G.groups[gi];
+            reorder_group_keys(G, g, wg[gi]);
+            if (!bsr) {   // the library S stage: the dense W_D in the new order
+                double* Wg = wd + (g.w_off - ws_init_off);
+                const uint64_t Kg = g.K.size();
+                for (const WEntry& e : wg[gi]) Wg[uint64_t(e.n) * Kg + e.k] = e.v;
+            }
+        }
         trace_phase("lower: key classes");
     }
     StagePlan& P = p->st;
@@ -2576,11 +2689,12 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
         P.lib.push_back({g.t_off, g.w_off, g.s_off, uint32_t(g.M), uint32_t(Ng), uint32_t(Kg), uint32_t(g.M),
                          uint32_t(Kg), uint32_t(g.M), g.bf_off, 0});
         // the block-sparse kernel runs every group (below)
-        P.lib.back().sparse = !adjoint && s_sparse_enabled();
+        P.lib.back().sparse = bsr;
     }
-    if (!adjoint && s_sparse_enabled()) {   // the block-sparse kernel over the visited tiles of W_D
+    if (bsr) {   // the block-sparse kernel over the visited tiles of W_D
         std::vector<std::pair<uint64_t, double>> info;
-        add_bsr_groups(P.bsr, G.groups, wd, ws_init_off, g_trace ? &info : nullptr);
+        add_bsr_groups(P.bsr, G.groups, wg, g_trace ? &info : nullptr);
+        std::vector<std::vector<WEntry>>().swap(wg);
         P.bsr_on = true;
         for (size_t gi = 0; gi < info.size(); ++gi) {
             const auto& g = G.groups[gi];
@@ -3033,8 +3147,11 @@ int mq_sigma_plan_create_ex(mq_ctx* ctx, const mq_site_desc* site, const mq_mpo_
         P.sizes[BASE_RIGHT] = right->total;
         P.sizes[BASE_BRA] = P.part_elems;
         P.validate();
-        if (P.part_elems) ctx->take(mq_ctx::POOL_PART, p->part, P.part_elems * sizeof(double), adjoint);
-        ctx->take(mq_ctx::POOL_WS, p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double), adjoint);
+        // a reused workspace is cleared for library S stages (the adjoint, MQ_S_DENSE=1): library
+        // GEMMs called with beta = 0 were seen to propagate NaN from stale memory in their C operand
+        const bool zero_reused = adjoint || !P.bsr_on;
+        if (P.part_elems) ctx->take(mq_ctx::POOL_PART, p->part, P.part_elems * sizeof(double), zero_reused);
+        ctx->take(mq_ctx::POOL_WS, p->ws, std::max<uint64_t>(P.ws_elems, 32) * sizeof(double), zero_reused);
         // the dense W_D serve the library S stage and the adjoint; the block-sparse S stage reads tiles
         if (wd_elems && !P.bsr_on)
             HIPCHK(hipMemcpyAsync(p->ws.as<double>() + ws_init_off, ctx->wd_host, wd_elems * sizeof(double),
@@ -3661,6 +3778,7 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
         }
         const uint32_t n_ket = kind == Schedule::LEFT ? L.right_i.size() : L.left_i.size();
         std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, n_ket}};
+        DevBuf sws;   // the chunks' workspace
         while (!todo.empty()) {
             const auto range = todo.back();
             todo.pop_back();
@@ -3688,11 +3806,17 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
             P.sizes[kind == Schedule::LEFT ? BASE_LEFT : BASE_RIGHT] = bnd->total;
             P.validate();
             P.upload_all(ctx->stream);
-            // workspace: the context's step buffer (cleared whenever it grows), W_D copied in
+            // workspace: the context's pooled workspace (the site's sigma plan has returned it), taken
+            // as it is with the block-sparse S stage (the grouped stages clear what they read
+            // unwritten; cleared for library GEMMs) and grown between chunks when a chunk needs
+            // more; W_D copied in
             trace_phase("step: tables");
             const size_t need = std::max<uint64_t>(P.ws_elems, 32) * sizeof(double);
-            if (need > ctx->step_ws.bytes) ctx->step_ws.alloc_zeroed(need, ctx->stream);
-            double* ws = ctx->step_ws.as<double>();
+            if (need > sws.bytes) {
+                ctx->pool_put(mq_ctx::POOL_WS, sws);
+                ctx->take(mq_ctx::POOL_WS, sws, need, !P.bsr_on);
+            }
+            double* ws = sws.as<double>();
             if (wd_elems && !P.bsr_on)
                 HIPCHK(hipMemcpyAsync(ws + ws_init_off, ctx->wd_host, wd_elems * sizeof(double), hipMemcpyHostToDevice,
                                       ctx->stream));
@@ -3709,7 +3833,7 @@ static int step_grouped(mq_ctx* ctx, int kind, const mq_site_desc* site, const m
             HIPCHK(hipStreamSynchronize(ctx->stream));   // the chunk's tables and the W_D staging are reused next
             trace_phase("step: run");
         }
-        ctx->step_ws.release();   // steps run once per site: keep no workspace between them
+        ctx->pool_put(mq_ctx::POOL_WS, sws);   // back to the pool for the next site's sigma plan
         *out = nb.release();
     });
 }

@@ -229,6 +229,19 @@ hipError_t launch_split_tiles(hipStream_t st, const double* w, uint64_t n, float
     return hipGetLastError();
 }
 
+// out[pos[i]] = val[i] (positions distinct): the nonzeros of the block-sparse S stage's W tiles
+// placed into their zero-filled tiles
+__global__ void scatter_kernel(uint64_t n, const uint64_t* __restrict__ pos, const double* __restrict__ val,
+                               double* __restrict__ out) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        out[pos[i]] = val[i];
+}
+hipError_t launch_scatter(hipStream_t st, uint64_t n, const uint64_t* pos, const double* val, double* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, n, pos, val, out);
+    return hipGetLastError();
+}
+
 __global__ void axpy_kernel(uint64_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
         y[i] += a * x[i];

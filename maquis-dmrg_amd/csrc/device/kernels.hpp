@@ -227,6 +227,8 @@ hipError_t launch_to_float(hipStream_t st, const double* src, float* dst, uint64
 hipError_t launch_split_w_bf16(hipStream_t st, const double* W, uint64_t ldw, uint64_t K, uint64_t N, uint16_t* dst);
 // n W-tile elements: FP32 value and bf16 split hi / lo
 hipError_t launch_split_tiles(hipStream_t st, const double* w, uint64_t n, float* w32, uint16_t* hi, uint16_t* lo);
+// out[pos[i]] = val[i], i < n (distinct positions)
+hipError_t launch_scatter(hipStream_t st, uint64_t n, const uint64_t* pos, const double* val, double* out);
 hipError_t launch_dot(hipStream_t st, uint64_t n, const double* x, const double* y, double* partial,
                       double* result);
 hipError_t launch_axpy(hipStream_t st, uint64_t n, double a, const double* x, double* y);

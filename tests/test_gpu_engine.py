@@ -459,10 +459,11 @@ def test_block_sparse_s_stage_matches_dense_and_oracle(eng, model, L, m, monkeyp
 
 @pytest.mark.parametrize("model,L,m", [("qc_su2", 16, 256), ("qc", 12, 128)])
 def test_reused_workspace_needs_no_zero_fill(eng, model, L, m, monkeypatch):
-    """A grouped sigma plan takes a reused workspace without clearing it.  With MQ_POISON_WS=1 a
-    reused workspace (and split-K partial buffer) is filled with NaN instead: a plan built on it
-    must give sigma bitwise equal to the same plan built before, in every precision, so no
-    kernel reads a workspace element it has not written in the same application."""
+    """Grouped sigma plans and grouped boundary steps take a reused workspace without clearing it.
+    With MQ_POISON_WS=1 a reused workspace (and split-K partial buffer) is filled with NaN instead:
+    a plan built on it must give sigma bitwise equal to the same plan built before, in every
+    precision, and the left and right steps of the site bitwise equal boundaries, so no kernel
+    reads a workspace element it has not written in the same application."""
     from mqdmrg.workloads import SyntheticSite
     print("case", model, L, m, flush=True)
     ws = SyntheticSite(model, L, L // 2 - 1, m, seed=4)
@@ -480,17 +481,40 @@ def test_reused_workspace_needs_no_zero_fill(eng, model, L, m, monkeypatch):
             outs[prec] = out
         return outs
 
+    def steps():
+        sym, phys = ws.basis.sym, ws.basis.phys
+        res = []
+        for k, side in ((ws.site, "left"), (ws.site + 1, "right")):
+            sd = E.SiteDesc(sym, ws.bonds[k], phys, ws.bonds[k + 1])
+            md = E.MpoDesc(ws.mpo[k])
+            lay = SiteLayout(ws.bonds[k], phys, ws.bonds[k + 1])
+            dv = dev(np.random.default_rng(k).standard_normal(max(lay.rp_total, 1)))
+            if side == "left":
+                lp = torch.zeros(max(lay.lp_total, 1), dtype=torch.float64, device="cuda:0")
+                E.reshape_right_to_left(eng, sd, dv, lp)
+                b = E.left_step(eng, sd, md, lp, dv, Lb, True, algorithm=E.SIGMA_GROUPED)
+            else:
+                b = E.right_step(eng, sd, md, dv, dv, Rb, True, algorithm=E.SIGMA_GROUPED)
+            res.append([np.asarray(c) for c in b.download()])
+        return res
+
     first = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
     ref = run(first)
     first.close()   # its workspace returns to the pool and is taken by the next plan
+    ref_steps = steps()
     monkeypatch.setenv("MQ_POISON_WS", "1")
     poisoned = E.SigmaPlan(eng, ws.site_desc, ws.mpo_desc, Lb, Rb, algorithm=E.SIGMA_GROUPED)
-    monkeypatch.delenv("MQ_POISON_WS")
     got = run(poisoned)
     poisoned.close()
+    got_steps = steps()
+    monkeypatch.delenv("MQ_POISON_WS")
     for prec in precs:
         assert torch.isfinite(got[prec]).all(), prec
         assert torch.equal(got[prec], ref[prec]), prec
+    for a, b in zip(ref_steps, got_steps):
+        assert len(a) == len(b)
+        for x, y in zip(a, b):
+            assert np.all(np.isfinite(y)) and np.array_equal(x, y)
 
 
 @pytest.mark.parametrize("precision", [E.PREC_F32, E.PREC_BF16X3])
