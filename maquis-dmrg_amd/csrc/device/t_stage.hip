@@ -285,6 +285,18 @@ void t_stack_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const 
     // the unit's column map; the 4 FP32 accumulator rows of a lane are consecutive (one 16-byte
     // store when all are inside the unit)
     TC* __restrict__ C = reinterpret_cast<TC*>(bases.p[tk.c_base]) + tk.c_off;
+    // the lane's column of block j: its column block in the unit's column map and the column in it
+    // (the same for every row block)
+    uint32_t cblk[WB], cin[WB];
+    {
+        const uint32_t cbw = max(tk.cbw, 1u);
+#pragma unroll
+        for (int j = 0; j < WB; ++j) {
+            const uint32_t gn = n0 + wn + 16 * j + l16;
+            cblk[j] = gn / cbw;
+            cin[j] = gn - cblk[j] * cbw;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < WB; ++i) {
         const uint32_t mb = m0 + wm + 16 * i;
@@ -296,8 +308,7 @@ void t_stack_kernel(const GemmTask* __restrict__ tasks, uint32_t n_tasks, const 
             if (gn >= N) continue;
             uint64_t coff;
             if (R.cmap != MAP_NONE) {
-                const uint32_t blk = gn / tk.cbw;
-                coff = uint64_t(maps[R.cmap + blk]) * R.cldk + uint64_t(gn - blk * tk.cbw) * R.ldc;
+                coff = uint64_t(maps[R.cmap + cblk[j]]) * R.cldk + uint64_t(cin[j]) * R.ldc;
             } else {
                 coff = uint64_t(gn) * R.ldc;
             }
