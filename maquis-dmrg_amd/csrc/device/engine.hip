@@ -199,7 +199,7 @@ struct mq_ctx {
     // grows, and is reused by the next lowering once the previous upload has been synchronized
     double* wd_host = nullptr;
     size_t wd_cap = 0;
-    double* wd_stage(size_t n) {
+    double* wd_stage(size_t n, bool zero = true) {
         if (n > wd_cap) {
             if (wd_host) (void)hipHostFree(wd_host);
             wd_host = nullptr;
@@ -208,6 +208,7 @@ struct mq_ctx {
             HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&wd_host), cap * sizeof(double), hipHostMallocDefault));
             wd_cap = cap;
         }
+        if (!zero) return wd_host;
         const int64_t chunk = int64_t(1) << 18;
 #pragma omp parallel for schedule(static)
         for (int64_t o = 0; o < int64_t(n); o += chunk)
@@ -557,9 +558,12 @@ struct StagePlan {
         std::vector<BsrTask> tasks;
         std::vector<BsrRun> runs;
         // the nonzeros of the compact W tiles: positions in the tile array (tiles x 128 columns x 16
-        // keys) and values; the device builds the tiles from them (host copy until upload)
-        std::vector<uint64_t> ent_pos;
-        std::vector<double> ent_val;
+        // keys) and values, n_ent of each, in the context's pinned staging buffer (valid until the
+        // upload, which the lowering's caller runs before the next lowering); the device builds the
+        // tiles from them
+        uint64_t* ent_pos = nullptr;
+        double* ent_val = nullptr;
+        uint64_t n_ent = 0;
         bool uploaded = false;
         uint64_t tiles = 0;
         uint64_t flops = 0;             // MFMA flops of the visited tiles (unpadded)
@@ -947,13 +951,14 @@ struct StagePlan {
             const uint64_t nw = bsr.tiles * uint64_t(S_BSR_TILE_N) * S_BSR_TILE_K;
             bsr.d_w64.alloc(std::max<uint64_t>(nw, 1) * sizeof(double));
             HIPCHK(launch_fill(st, bsr.d_w64.as<double>(), nw, 0.0));
-            {
+            if (bsr.n_ent) {
                 DevBuf d_pos, d_val;
-                upload(d_pos, bsr.ent_pos, st);
-                upload(d_val, bsr.ent_val, st);
-                HIPCHK(launch_scatter(st, bsr.ent_pos.size(), d_pos.as<uint64_t>(), d_val.as<double>(),
-                                      bsr.d_w64.as<double>()));
-                HIPCHK(hipStreamSynchronize(st));   // before the entry buffers are freed
+                d_pos.alloc(bsr.n_ent * sizeof(uint64_t));
+                d_val.alloc(bsr.n_ent * sizeof(double));
+                HIPCHK(hipMemcpyAsync(d_pos.p, bsr.ent_pos, bsr.n_ent * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(d_val.p, bsr.ent_val, bsr.n_ent * sizeof(double), hipMemcpyHostToDevice, st));
+                HIPCHK(launch_scatter(st, bsr.n_ent, d_pos.as<uint64_t>(), d_val.as<double>(), bsr.d_w64.as<double>()));
+                HIPCHK(hipStreamSynchronize(st));   // before the entry buffers are freed and the staging reused
             }
             bsr.d_w32.alloc(std::max<uint64_t>(nw, 1) * sizeof(float));
             bsr.d_whi.alloc(std::max<uint64_t>(nw, 1) * sizeof(uint16_t));
@@ -961,8 +966,9 @@ struct StagePlan {
             HIPCHK(launch_split_tiles(st, bsr.d_w64.as<double>(), nw, bsr.d_w32.as<float>(), bsr.d_whi.as<uint16_t>(),
                                       bsr.d_wlo.as<uint16_t>()));
             HIPCHK(hipStreamSynchronize(st));   // the host copy is released below
-            std::vector<uint64_t>().swap(bsr.ent_pos);
-            std::vector<double>().swap(bsr.ent_val);
+            bsr.ent_pos = nullptr;
+            bsr.ent_val = nullptr;
+            bsr.n_ent = 0;
             bsr.uploaded = true;
         }
         upload(d_regions, regions, st);
@@ -1085,7 +1091,8 @@ struct StagePlan {
     }
 };
 
-static_assert(MQ_PREC_F64 == MODE_F64 && MQ_PREC_F32 == MODE_F32 && MQ_PREC_BF16X3 == MODE_BF16X3,
+static_assert(int(MQ_PREC_F64) == int(MODE_F64) && int(MQ_PREC_F32) == int(MODE_F32) &&
+                  int(MQ_PREC_BF16X3) == int(MODE_BF16X3),
               "precision enum and GEMM modes agree");
 
 struct mq_plan {
@@ -1950,6 +1957,16 @@ struct WEntry {
     uint32_t n, k;
     double v;
 };
+// Allocator whose resize() leaves trivial elements uninitialised (the nonzero lists are filled
+// completely right after they are sized; clearing hundreds of MB first would cost as much)
+template <class T> struct UninitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = UninitAlloc<U>; };
+    UninitAlloc() = default;
+    template <class U> UninitAlloc(const UninitAlloc<U>&) {}
+    template <class U> void construct(U* q) { ::new (static_cast<void*>(q)) U; }
+    template <class U, class... A> void construct(U* q, A&&... a) { ::new (static_cast<void*>(q)) U(std::forward<A>(a)...); }
+};
+typedef std::vector<WEntry, UninitAlloc<WEntry>> WList;
 
 // Pattern-class order of one group's keys (see lower_sigma_grouped): rows k and columns n of W_D
 // are classed by a hash of their nonzero index lists; classes are ordered by size (largest first),
@@ -1958,7 +1975,7 @@ struct WEntry {
 // column-major scan of the dense matrix, which fixes the hashes); their positions, the group's
 // K / N lists, kpos / npos and its pairs' kk / nn position lists are renumbered consistently (W is
 // left in the old order of its elements).
-void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, std::vector<WEntry>& W) {
+void reorder_group_keys(GroupedLowering& G, GroupedLowering::Group& g, WList& W) {
     const uint32_t Kg = uint32_t(g.K.size()), Ng = uint32_t(g.N.size());
     if (Kg < 2 && Ng < 2) return;
     const uint64_t seed = 0x9E3779B97F4A7C15ull;
@@ -2025,7 +2042,7 @@ struct BsrGroupScan {
     uint64_t tiles = 0, nnz = 0, visited = 0;
 };
 void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>& groups,
-                    const std::vector<std::vector<WEntry>>& wg, std::vector<std::pair<uint64_t, double>>* info) {
+                    const std::vector<WList>& wg, std::vector<std::pair<uint64_t, double>>* info) {
     constexpr uint32_t TK = S_BSR_TILE_K, TN = S_BSR_TILE_N, TM = S_BSR_TILE_M;
     const size_t ng = groups.size();
     std::vector<BsrGroupScan> sc(ng);
@@ -2067,10 +2084,10 @@ void add_bsr_groups(StagePlan::Bsr& B, const std::vector<GroupedLowering::Group>
     std::vector<uint64_t> tile0(ng + 1, B.tiles);
     for (size_t gi = 0; gi < ng; ++gi) tile0[gi + 1] = tile0[gi] + sc[gi].tiles;
     MQ_REQUIRE(tile0[ng] < (1ull << 32), "too many block-sparse tiles");
-    std::vector<uint64_t> e0(ng + 1, B.ent_pos.size());
+    MQ_REQUIRE(B.n_ent == 0 && B.ent_pos && B.ent_val, "block-sparse entries: one lowering per staging");
+    std::vector<uint64_t> e0(ng + 1, 0);
     for (size_t gi = 0; gi < ng; ++gi) e0[gi + 1] = e0[gi] + (sc[gi].tiles ? wg[gi].size() : 0);
-    B.ent_pos.resize(e0[ng]);
-    B.ent_val.resize(e0[ng]);
+    B.n_ent = e0[ng];
 #pragma omp parallel for schedule(dynamic, 1)
     for (int gi = 0; gi < int(ng); ++gi) {
         if (!sc[gi].tiles) continue;
@@ -2299,7 +2316,7 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     const bool sparse_w = !adjoint;
     const bool bsr = !adjoint && s_sparse_enabled();
     double* const wd = bsr ? nullptr : p->ctx->wd_stage(wd_elems);
-    std::vector<std::vector<WEntry>> wg(sparse_w ? G.groups.size() : 0);
+    std::vector<WList> wg(sparse_w ? G.groups.size() : 0);
     trace_phase("lower: key order, layout");
     if (g_trace) {
         uint64_t tk = 0, tn = 0, tm = 0;
@@ -2693,8 +2710,13 @@ void lower_sigma_grouped(mq_plan* p, const SiteLayout& site, const Mpo& W, uint6
     }
     if (bsr) {   // the block-sparse kernel over the visited tiles of W_D
         std::vector<std::pair<uint64_t, double>> info;
+        uint64_t cap = 1;
+        for (const auto& w : wg) cap += w.size();
+        double* stage = p->ctx->wd_stage(2 * cap, false);   // positions, then values
+        P.bsr.ent_pos = reinterpret_cast<uint64_t*>(stage);
+        P.bsr.ent_val = stage + cap;
         add_bsr_groups(P.bsr, G.groups, wg, g_trace ? &info : nullptr);
-        std::vector<std::vector<WEntry>>().swap(wg);
+        std::vector<WList>().swap(wg);
         P.bsr_on = true;
         for (size_t gi = 0; gi < info.size(); ++gi) {
             const auto& g = G.groups[gi];
