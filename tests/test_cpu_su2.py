@@ -96,3 +96,39 @@ def test_su2_terms_cover_every_integral_class():
               "destroy_fill_count"):
         assert n in names, n
     assert max(len(k) for k in terms) == 4
+
+
+@pytest.mark.parametrize("site", [0, 2, 3])
+def test_twosite_su2_mpo_equals_the_direct_formula(site):
+    """The two-site SU2 MPO (make_twosite_mpo, ts_ops.hpp:38-168) built with one sparse product per
+    middle bond spin equals the direct sum over the three bond states: for every operator pair
+    (o1, o2) and coupled rank k, W12(b1, b3) = sum_b2 f(j(b1), j(b2), j(b3)) C1(b1, b2) C2(b2, b3)
+    with f = sqrt((j_m + 1)(k + 1)) {j j' k; k2 k1 j_m} (-1)^((j + j' + k1 + k2) / 2), evaluated here
+    entry by entry on dense matrices; every nonzero (o1, o2, k) with a nonempty coupled operator is
+    present, and nothing else."""
+    from mqdmrg import models
+    basis, mpo, _ = models.chain("qc_su2", 6)
+    W1, W2 = mpo[site], mpo[site + 1]
+    W12, phys12, pb = su2.twosite_su2(W1, W2, basis.phys)
+    ls, ms, rs = W1.left_spin, W1.right_spin, W2.right_spin
+    got = {op.name: C.toarray() for op, C in zip(W12.ops, W12.coeffs)}
+    want = {}
+    for o1, C1 in zip(W1.ops, W1.coeffs):
+        A = C1.toarray()
+        for o2, C2 in zip(W2.ops, W2.coeffs):
+            B = C2.toarray()
+            k1, k2 = o1.spin, o2.spin
+            for k in range(abs(k1 - k2), k1 + k2 + 1, 2):
+                M = np.zeros((A.shape[0], B.shape[1]))
+                for b1, b2 in zip(*np.nonzero(A)):
+                    for b3 in np.nonzero(B[b2])[0]:
+                        j, jm, jp = ls[b1], ms[b2], rs[b3]
+                        f = np.sqrt((jm + 1.0) * (k + 1.0)) * su2.sixj(j, jp, k, k2, k1, jm)
+                        if ((j + jp + k1 + k2) // 2) % 2:
+                            f = -f
+                        M[b1, b3] += f * A[b1, b2] * B[b2, b3]
+                if np.any(M != 0) and su2.op_kron_su2(o1, o2, k, basis.phys, phys12, pb).blocks:
+                    want["%s*%s/%d" % (o1.name, o2.name, k)] = M
+    assert set(got) == set(want)
+    for name, M in want.items():
+        np.testing.assert_allclose(got[name], M, rtol=0, atol=1e-13 * max(1.0, np.abs(M).max()), err_msg=name)
