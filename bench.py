@@ -278,7 +278,7 @@ def adjoint_workload(eng, args, su2_model):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
+        print("bench.py runs on one GPU only (the engine has no multi-GPU path); got --gpus %d" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
