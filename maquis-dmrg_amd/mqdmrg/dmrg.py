@@ -872,6 +872,31 @@ class TwoSiteDMRG:
         """<psi|psi>: expval of the identity MPO."""
         return self.expval(models.identity_mpo(self.phys, self.L))
 
+    def rdm1(self):
+        """Spin-summed one-particle density matrix gamma_pq = <psi|E_pq|psi> / <psi|psi>, E_pq =
+        sum_sigma a+_{p sigma} a_{q sigma}, of an ab initio chain ('qc': TwoU1, 'qc_su2': SU2U1),
+        measured as expectation values of one-body MPOs (the route of the reference's 1-RDM
+        measurement, tagged_nrankrdm.h, through expval, mps_mpo_ops.h:52-100): the chain's MPO
+        builder with h = (e_pq + e_qp) / 2 and no two-body part gives <(E_pq + E_qp) / 2> =
+        gamma_pq for a real state.  L (L + 1) / 2 MPOs, each evaluated by left boundary steps over
+        the chain on the device."""
+        from . import qc, su2 as su2m
+        L = self.L
+        zero = np.zeros((L, L, L, L))
+        nrm = self.norm2()
+        gamma = np.zeros((L, L))
+        for p in range(L):
+            for q in range(p, L):
+                h = np.zeros((L, L))
+                h[p, q] += 0.5
+                h[q, p] += 0.5
+                if self.su2:
+                    _, mpo, _ = su2m.qc_su2_mpo(h, zero, self.target[0])
+                else:
+                    _, mpo, _ = qc.qc_mpo(h, zero)
+                gamma[p, q] = gamma[q, p] = self.expval(mpo) / nrm
+        return gamma
+
     def optimize_pair(self, i, direction):
         t0 = time.perf_counter()
         self._cache_before(i, direction)

@@ -147,3 +147,23 @@ def test_rdm_from_adjoint_matches_fci_finite_differences(eng):
     print("dE/dh rel err", eh, "dE/deri rel err", ee, flush=True)
     assert eh <= 1e-6 and ee <= 1e-6
     assert abs(np.trace(ht.grad.numpy()) - (nup + ndn)) < 1e-8
+
+
+def test_rdm1_measurement_equals_adjoint_gradient(eng):
+    """The spin-summed 1-RDM measured as expectation values of one-body MPOs (TwoSiteDMRG.rdm1,
+    the measurement route) equals dE/dh from the sigma adjoint of the same converged state
+    (Hellmann-Feynman; the adjoint's dE/dh is pinned to FCI finite differences above) to 1e-8, and
+    its trace is the electron number."""
+    L, nup, ndn = 4, 2, 2
+    h, eri = qc.synthetic_integrals(L, seed=3)
+    basis, mpo, _, src = qc.qc_mpo(h, eri, sources=True)
+    d = TwoSiteDMRG(eng, "qc", L, m=64, cutoff=1e-14, init_m=8, seed=2, jd_tol=1e-12, jd_maxiter=20,
+                    mpo=mpo, basis=basis, target=(nup, ndn))
+    d.run(4)
+    gamma = d.rdm1()
+    grads, _ = d.mpo_gradients()
+    dh, _, _ = qc.mpo_gradient_to_integrals(mpo, src, grads, L)
+    dh = 0.5 * (dh + dh.T)
+    print("trace", np.trace(gamma), "max |gamma - dE/dh|", np.abs(gamma - dh).max(), flush=True)
+    assert abs(np.trace(gamma) - (nup + ndn)) < 1e-9
+    assert np.abs(gamma - dh).max() < 1e-8
