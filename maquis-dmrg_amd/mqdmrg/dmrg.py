@@ -8,6 +8,7 @@ block_matrix_algorithms.h:135-183), and the boundary is moved (optimize.h:134-15
 left/right steps).  Site tensors live on the device in the right-paired padded layout; the small
 dense algebra of the split (batched Gram matrices and eigh in place of per-sector SVDs) uses torch
 on the engine's stream."""
+import itertools
 import time
 
 import numpy as np
@@ -896,6 +897,30 @@ class TwoSiteDMRG:
                     _, mpo, _ = qc.qc_mpo(h, zero)
                 gamma[p, q] = gamma[q, p] = self.expval(mpo) / nrm
         return gamma
+
+    def rdm2_classes(self):
+        """SU2U1 ab initio chain: for every permutation class of (pq|rs) (one element of
+        su2.integral_elements, keyed by its aligned indices), <psi|V|psi> / <psi|psi> with V the
+        two-body operator that element multiplies in the spin-adapted Hamiltonian: the sum of the
+        spin-summed 2-RDM entries of the class as the Hamiltonian weighs them, i.e. dE/d(element).
+        Each is the expectation value of the MPO built from that element alone (unit value, no
+        one-body part), L^4 / 8 MPOs in all."""
+        from . import su2 as su2m
+        assert self.su2, "rdm2_classes: SU2U1 chains"
+        L = self.L
+        zero_h = np.zeros((L, L))
+        nrm = self.norm2()
+        out = {}
+        for i, j, k, l in itertools.product(range(L), repeat=4):
+            if i < j or k < l or (i * L + j) < (k * L + l):
+                continue
+            eri = np.zeros((L, L, L, L))
+            for a, b, c, d in ((i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                               (k, l, i, j), (l, k, i, j), (k, l, j, i), (l, k, j, i)):
+                eri[a, b, c, d] = 1.0
+            _, mpo, _ = su2m.qc_su2_mpo(zero_h, eri, self.target[0])
+            out[su2m._align(i, j, k, l)] = self.expval(mpo) / nrm
+        return out
 
     def optimize_pair(self, i, direction):
         t0 = time.perf_counter()
